@@ -1,0 +1,186 @@
+"""Benchmark: GDoF/s of HONEE's Newtonian residual (IFunction) + Jacobian (IJacobian) apply on one MI355X.
+
+Workload (the implicit SUPG Taylor-Green configuration): 32^3 periodic hexahedra on [0, 2pi]^3, p=3, Q=4 Gauss points,
+primitive FP64 state, SUPG with the Taylor-Green YAML Tau coefficients, velocity perturbed by seeded uniform noise of
+amplitude 1e-3 U0 (seed 1600).  DoF = 5 x 884,736 nodes = 4,423,680.
+
+A step is one IFunction apply followed by one IJacobian apply (FP64), each through CeedOperatorApply of the composite
+operators HONEE builds, on vectors resident in HBM.  value = DoF / (seconds per step): whole residual+Jacobian pairs
+per second, expressed in DoF.  The per-kernel figures (HIP events on the backend stream) give each operator's
+GDoF/s and the HBM roofline fraction of the dominant kernel; the FP32-Jacobian variant is timed alongside.
+The CPU baseline times this repository's FP64 restatement of the reference's /cpu/self path (the oracle) on a bounded
+sample of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "honee-4aiad_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
+
+
+def algorithmic_bytes(nelem, P, Q, nnodes, njac=14, jac_bytes=8, vec_bytes=8):
+    """Compulsory HBM traffic of one apply, each stream counted once ('Measurement')."""
+    Q3, P3 = Q**3, P**3
+    lvec = nnodes * 5 * vec_bytes
+    qdata = nelem * Q3 * 10 * 8
+    offsets = nelem * P3 * 4
+    coords = nelem * 8 * 3 * 8
+    ifunc = 2 * lvec + qdata + coords + offsets + lvec + nelem * Q3 * njac * jac_bytes
+    ijac = lvec + qdata + nelem * Q3 * njac * jac_bytes + offsets + lvec
+    return ifunc, ijac
+
+
+def cpu_baseline(mesh_shape_full, degree, seconds=10.0):
+    """Time the oracle's IFunction + IJacobian on a 1/64 sample (8^3 elements) with all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from pyhonee import mesh as M
+    from pyhonee import problems as PR
+
+    shape = tuple(max(1, s // 4) for s in mesh_shape_full)
+    mesh = M.box_mesh(shape, degree)
+    tab = M.tabulate(degree, 0)
+    om = O.Mesh(mesh, tab)
+    ctx = PR.tgv_context_dict(state_var=1, stabilization=2, dt=2e-3)
+    c = O.newt_from_dict(ctx)
+    qd = om.setup_qdata()
+    q = PR.taylor_green_ic(mesh, 1).ravel()
+    qdot = np.zeros_like(q)
+    dq = np.random.default_rng(0).uniform(-1e-3, 1e-3, q.shape)
+    c.ijacobian_time_shift = 1.0 / 2e-3
+    n, t0 = 0, time.perf_counter()
+    while True:
+        _, jac = O.ifunction(om, c, qd, q, qdot)
+        O.ijacobian(om, c, qd, jac, dq)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = (time.perf_counter() - t0) / n
+    dof = 5 * mesh.nnodes
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": dof / dt / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "sample": f"TGV {shape[0]}x{shape[1]}x{shape[2]} p={degree} (1/{int(np.prod(mesh_shape_full) // np.prod(shape))} of the "
+                      f"workload), {n} residual+Jacobian pairs in {n * dt:.1f} s, oracle (FP64 C restatement of /cpu/self)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--shape", type=int, nargs=3, default=(32, 32, 32))
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from pyhonee import mesh as M
+    from pyhonee import problems as PR
+    from pyhonee.driver import CEED_MEM_DEVICE, NewtonianProblem
+
+    shape, degree = tuple(args.shape), args.degree
+    print(f"# building mesh {shape} p={degree}", file=sys.stderr, flush=True)
+    mesh = M.box_mesh(shape, degree)
+    tab = M.tabulate(degree, 0)
+    dof = 5 * mesh.nnodes
+    ctx = PR.tgv_context(state_var=1, stabilization=2, dt=2e-3)
+    rng = np.random.default_rng(1600)
+    Y = PR.nodal_field(mesh, lambda x: PR.taylor_green_prim(x))
+    Y[:, 1:4] += rng.uniform(-1e-3, 1e-3, (mesh.nnodes, 3))  # seeded velocity noise, amplitude 1e-3 U0
+    dev = torch.device("cuda:0")
+    q = torch.tensor(Y.ravel(), dtype=torch.float64, device=dev)
+    qdot = torch.tensor(rng.uniform(-1e-3, 1e-3, Y.size), dtype=torch.float64, device=dev)
+    dq = torch.tensor(rng.uniform(-1e-3, 1e-3, Y.size), dtype=torch.float64, device=dev)
+    g = torch.zeros_like(q)
+    v = torch.zeros_like(q)
+    shift = 1.0 / 2e-3
+
+    results = {}
+    for label, resource in [("fp64", "/gpu/hip/mi355x")] + ([] if args.no_fp32 else [("fp32", "/gpu/hip/mi355x:jacobian_precision=fp32")]):
+        print(f"# setting up {label} operators", file=sys.stderr, flush=True)
+        prob = NewtonianProblem(mesh, tab, ctx, 1, resource)
+        for _ in range(args.warmup):
+            prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)
+            prob.ijacobian(dq, v, shift, mem=CEED_MEM_DEVICE)
+        prob.synchronize()
+        torch.cuda.synchronize()
+        prob.set_timing(True)
+        prob.reset_times()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)
+            prob.ijacobian(dq, v, shift, mem=CEED_MEM_DEVICE)
+        prob.synchronize()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        times = prob.kernel_times()
+        prob.set_timing(False)
+        results[label] = dict(elapsed=elapsed, times=times)
+        print(f"# {label}: {elapsed / args.steps * 1e3:.3f} ms/step, kernels {times}", file=sys.stderr, flush=True)
+        if label == "fp64":
+            # timing-free pass for the whole-step rate (events add host work between launches)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)
+                prob.ijacobian(dq, v, shift, mem=CEED_MEM_DEVICE)
+            prob.synchronize()
+            torch.cuda.synchronize()
+            results[label]["elapsed_untimed"] = time.perf_counter() - t0
+        prob.close()
+
+    r64 = results["fp64"]
+    ms_per_step = r64["elapsed_untimed"] / args.steps * 1e3
+    kt = r64["times"]
+    t_if = kt["ifunction_newtonian"][1] / kt["ifunction_newtonian"][0] * 1e-3
+    t_ij = kt["ijacobian_newtonian"][1] / kt["ijacobian_newtonian"][0] * 1e-3
+    b_if, b_ij = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes)
+    line = {
+        "metric": METRIC,
+        "value": dof / (ms_per_step * 1e-3) / 1e9,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Taylor-Green state, seed 1600)",
+        "config": {"workload": f"TGV {shape[0]}^3 periodic hex, p={degree} Q={tab.Q}, primitive FP64, SUPG; step = IFunction + IJacobian apply",
+                   "dof": dof, "elements": mesh.nelem},
+        "ifunction_gdofs": dof / t_if / 1e9,
+        "ijacobian_gdofs": dof / t_ij / 1e9,
+        "roofline": {"bound": "hbm", "kernel": "ifunction_newtonian", "achieved": b_if / t_if / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_if, "avg_launch_us": t_if * 1e6},
+    }
+    if "fp32" in results:
+        k32 = results["fp32"]["times"]
+        t32 = k32["ijacobian_newtonian_fp32"][1] / k32["ijacobian_newtonian_fp32"][0] * 1e-3
+        _, b32 = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes, jac_bytes=4)
+        line["ijacobian_fp32_gdofs"] = dof / t32 / 1e9
+        line["ijacobian_fp32_roofline_frac"] = b32 / t32 / 1e9 / HBM_PEAK_GBS
+    line["ijacobian_roofline_frac"] = b_ij / t_ij / 1e9 / HBM_PEAK_GBS
+    if not args.no_cpu_baseline:
+        print("# timing CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(shape, degree)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

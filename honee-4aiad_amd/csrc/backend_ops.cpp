@@ -1,0 +1,268 @@
+// Operator binding for the MI355X backend.
+//
+// A non-composite CeedOperator is bound, on its first application, to one fused kernel chosen by the identity of its
+// QFunction (the "<file>:<name>" location HONEE registers, e.g. newtonian.h:IFunction_Newtonian_Prim).  Binding checks
+// that the operator's fields have exactly the layout HONEE's SetupLibceed gives them (src/setuplibceed.c:209-269,
+// src/qdata.c:154-169) and records the restriction/basis/vector objects; device pointers and the QFunction context are
+// read again at every application, because HONEE swaps vector arrays and context values between calls.
+// QFunctions without a fused kernel are rejected with CEED_ERROR_UNSUPPORTED: there is no host execution path.
+#include <string.h>
+
+#include <string>
+
+#include "ceed_impl.hpp"
+
+struct OpPlan {
+  QfKind              kind = QfKind::Unknown;
+  int                 var  = 0;
+  int                 P = 0, Q = 0, Px = 0, nelem = 0;
+  int                 njac       = 0;
+  CeedElemRestriction r_q        = nullptr, r_x = nullptr, r_qd = nullptr, r_jd = nullptr;
+  CeedBasis           b_q        = nullptr, b_x = nullptr;
+  CeedVector          v_qdot     = nullptr, v_qdata = nullptr, v_x = nullptr, v_jac = nullptr;
+  QDataLayout         qd{}, jd{};
+  std::string         kernel_name;
+};
+
+void op_plan_destroy(OpPlan *plan) { delete plan; }
+
+static QDataLayout strided_layout(CeedElemRestriction r) {
+  return QDataLayout{r->strides[0], r->strides[1], r->strides[2]};
+}
+
+static bool is_active(CeedVector v) { return v == CEED_VECTOR_ACTIVE; }
+static bool is_strided(CeedElemRestriction r) { return r && r != CEED_ELEMRESTRICTION_NONE && r->strided; }
+static bool is_offsets(CeedElemRestriction r) { return r && r != CEED_ELEMRESTRICTION_NONE && !r->strided; }
+static bool is_tensor3(CeedBasis b) { return b && b != CEED_BASIS_NONE && b->tensor && b->dim == 3; }
+
+#define BIND_CHECK(cond, ...)                                                       \
+  do {                                                                              \
+    if (!(cond)) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, __VA_ARGS__);      \
+  } while (0)
+
+static int check_field(CeedOperator op, size_t idx, bool input, const char *name, CeedEvalMode emode, CeedInt size) {
+  const auto &fields = input ? op->qf->inputs : op->qf->outputs;
+  BIND_CHECK(idx < fields.size(), "QFunction %s: missing %s field %zu (%s)", op->qf->name.c_str(), input ? "input" : "output", idx, name);
+  const QFField &f = fields[idx];
+  BIND_CHECK(f.emode == emode, "QFunction %s: field '%s' has eval mode %d, expected %d", op->qf->name.c_str(), f.name.c_str(), (int)f.emode,
+             (int)emode);
+  BIND_CHECK(size < 0 || f.size == size, "QFunction %s: field '%s' has size %d, expected %d", op->qf->name.c_str(), f.name.c_str(), f.size, size);
+  return 0;
+}
+
+static int bind_solution_basis(CeedOperator op, OpPlan *p, CeedOperatorField_private *f) {
+  BIND_CHECK(is_offsets(f->rstr) && is_tensor3(f->basis), "field '%s' needs an offset restriction and a 3D tensor basis", f->name.c_str());
+  BIND_CHECK(f->rstr->num_comp == 5 && f->basis->num_comp == 5, "field '%s' must have 5 components", f->name.c_str());
+  p->r_q   = f->rstr;
+  p->b_q   = f->basis;
+  p->P     = f->basis->P_1d;
+  p->Q     = f->basis->Q_1d;
+  p->nelem = f->rstr->num_elem;
+  BIND_CHECK(volume_pq_supported(p->P, p->Q), "no fused kernel for P_1d=%d, Q_1d=%d", p->P, p->Q);
+  BIND_CHECK(f->rstr->elem_size == p->P * p->P * p->P, "restriction element size does not match basis");
+  MI_CALL(basis_device_tables(f->basis));
+  return 0;
+}
+
+static int bind_coords(CeedOperator op, OpPlan *p, CeedOperatorField_private *f) {
+  BIND_CHECK(is_offsets(f->rstr) && is_tensor3(f->basis) && f->basis->P_1d == 2 && f->rstr->num_comp == 3,
+             "coordinate field '%s' must be a 3-component trilinear (P_1d = 2) field", f->name.c_str());
+  BIND_CHECK(!is_active(f->vec) && f->vec != CEED_VECTOR_NONE, "coordinate field must be passive");
+  BIND_CHECK(f->rstr->num_elem == p->nelem && f->basis->Q_1d == p->Q, "coordinate field does not match the solution field");
+  p->r_x = f->rstr;
+  p->b_x = f->basis;
+  p->v_x = f->vec;
+  MI_CALL(basis_device_tables(f->basis));
+  return 0;
+}
+
+static int plan_bind(CeedOperator op) {
+  CeedQFunction qf = op->qf;
+  auto          p  = new OpPlan();
+  p->kind          = qf->kind;
+  p->var           = qf->state_var;
+  op->plan         = p;
+  auto fail        = [&](int ierr) {
+    delete p;
+    op->plan = nullptr;
+    return ierr;
+  };
+  int ierr = 0;
+  switch (qf->kind) {
+    case QfKind::Setup: {  // setupgeo.h:42-56 wired by QDataGet (src/qdata.c:154-169)
+      if ((ierr = check_field(op, 0, true, "dx", CEED_EVAL_GRAD, 9))) return fail(ierr);
+      if ((ierr = check_field(op, 1, true, "weight", CEED_EVAL_WEIGHT, 1))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "qdata", CEED_EVAL_NONE, 10))) return fail(ierr);
+      auto *fx = op->in_fields[0], *fq = op->out_fields[0];
+      if (!(is_offsets(fx->rstr) && is_tensor3(fx->basis) && fx->basis->P_1d == 2 && fx->rstr->num_comp == 3) || !is_active(fx->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "Setup: 'dx' must be the active trilinear coordinate field"));
+      if (!is_strided(fq->rstr) || !is_active(fq->vec) || fq->rstr->num_comp != 10)
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "Setup: qdata output must be an active strided 10-component field"));
+      p->r_x   = fx->rstr;
+      p->b_x   = fx->basis;
+      p->Q     = fx->basis->Q_1d;
+      p->Px    = 2;
+      p->nelem = fx->rstr->num_elem;
+      p->r_qd  = fq->rstr;
+      p->qd    = strided_layout(fq->rstr);
+      if ((ierr = basis_device_tables(fx->basis))) return fail(ierr);
+      p->kernel_name = "setup_qdata";
+      return 0;
+    }
+    case QfKind::IFunction: {  // newtonian.h:205-257 wired by src/setuplibceed.c:209-243
+      const char *in_names[5] = {"q", "Grad_q", "q dot", "qdata", "x"};
+      CeedEvalMode in_modes[5] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_INTERP, CEED_EVAL_NONE, CEED_EVAL_INTERP};
+      CeedInt      in_sizes[5] = {5, 15, 5, 10, 3};
+      if (qf->inputs.size() != 5)
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: only div_diff_flux_projection_method none is supported", qf->name.c_str()));
+      for (int i = 0; i < 5; i++)
+        if ((ierr = check_field(op, i, true, in_names[i], in_modes[i], in_sizes[i]))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "v", CEED_EVAL_INTERP, 5))) return fail(ierr);
+      if ((ierr = check_field(op, 1, false, "Grad_v", CEED_EVAL_GRAD, 15))) return fail(ierr);
+      auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+      if ((ierr = bind_solution_basis(op, p, fi[0]))) return fail(ierr);
+      if (!(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: q and Grad_q must share the active restriction/basis", qf->name.c_str()));
+      if (!(fi[2]->rstr == p->r_q && fi[2]->basis == p->b_q && !is_active(fi[2]->vec)))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: 'q dot' must be a passive field on the solution space", qf->name.c_str()));
+      p->v_qdot = fi[2]->vec;
+      if (!is_strided(fi[3]->rstr) || is_active(fi[3]->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: qdata must be a passive strided field", qf->name.c_str()));
+      p->r_qd    = fi[3]->rstr;
+      p->v_qdata = fi[3]->vec;
+      p->qd      = strided_layout(p->r_qd);
+      if ((ierr = bind_coords(op, p, fi[4]))) return fail(ierr);
+      if (!(is_active(fo[0]->vec) && is_active(fo[1]->vec) && fo[0]->rstr == p->r_q && fo[1]->rstr == p->r_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: v and Grad_v must be active on the solution space", qf->name.c_str()));
+      if (qf->outputs.size() == 3) {
+        if ((ierr = check_field(op, 2, false, "jac_data", CEED_EVAL_NONE, -1))) return fail(ierr);
+        if (!is_strided(fo[2]->rstr) || is_active(fo[2]->vec))
+          return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: jac_data must be a passive strided field", qf->name.c_str()));
+        p->njac  = qf->outputs[2].size;
+        p->r_jd  = fo[2]->rstr;
+        p->v_jac = fo[2]->vec;
+        p->jd    = strided_layout(p->r_jd);
+      }
+      p->kernel_name = "ifunction_newtonian";
+      return 0;
+    }
+    case QfKind::IJacobian: {  // newtonian.h:272-311 wired by src/setuplibceed.c:245-269
+      const char  *in_names[4] = {"dq", "Grad_dq", "qdata", "jac_data"};
+      CeedEvalMode in_modes[4] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_NONE, CEED_EVAL_NONE};
+      CeedInt      in_sizes[4] = {5, 15, 10, -1};
+      if (qf->inputs.size() != 4) return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: unexpected input fields", qf->name.c_str()));
+      for (int i = 0; i < 4; i++)
+        if ((ierr = check_field(op, i, true, in_names[i], in_modes[i], in_sizes[i]))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "v", CEED_EVAL_INTERP, 5))) return fail(ierr);
+      if ((ierr = check_field(op, 1, false, "Grad_v", CEED_EVAL_GRAD, 15))) return fail(ierr);
+      auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+      if ((ierr = bind_solution_basis(op, p, fi[0]))) return fail(ierr);
+      if (!(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: dq and Grad_dq must share the active restriction/basis", qf->name.c_str()));
+      if (!is_strided(fi[2]->rstr) || is_active(fi[2]->vec) || !is_strided(fi[3]->rstr) || is_active(fi[3]->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: qdata and jac_data must be passive strided fields", qf->name.c_str()));
+      p->r_qd    = fi[2]->rstr;
+      p->v_qdata = fi[2]->vec;
+      p->qd      = strided_layout(p->r_qd);
+      p->r_jd    = fi[3]->rstr;
+      p->v_jac   = fi[3]->vec;
+      p->jd      = strided_layout(p->r_jd);
+      p->njac    = qf->inputs[3].size;
+      if (!(is_active(fo[0]->vec) && is_active(fo[1]->vec) && fo[0]->rstr == p->r_q && fo[1]->rstr == p->r_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: v and Grad_v must be active on the solution space", qf->name.c_str()));
+      p->kernel_name = op->ceed->jac_fp32 ? "ijacobian_newtonian_fp32" : "ijacobian_newtonian";
+      return 0;
+    }
+    default:
+      break;
+  }
+  return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED,
+                     "QFunction '%s' (%s) has no fused MI355X kernel; operators of this type cannot run on /gpu/hip/mi355x", qf->name.c_str(),
+                     qf->source.c_str()));
+}
+
+static int newtonian_context(CeedOperator op, CeedMiNewtonianContext *ctx) {
+  CeedQFunctionContext c = op->qf->ctx;
+  if (!c || !c->data) return MI_ERR(op->ceed, CEED_ERROR_INCOMPLETE, "%s: missing QFunction context", op->qf->name.c_str());
+  if (c->size != sizeof(CeedMiNewtonianContext))
+    return MI_ERR(op->ceed, CEED_ERROR_INCOMPATIBLE, "%s: context size %zu does not match struct NewtonianIdealGasContext_ (%zu)",
+                  op->qf->name.c_str(), c->size, sizeof(CeedMiNewtonianContext));
+  memcpy(ctx, c->data, sizeof(*ctx));
+  if (ctx->divFdiff_method != 0)
+    return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: divergence of diffusive flux projection is not supported", op->qf->name.c_str());
+  return 0;
+}
+
+int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
+  if (!op->plan) MI_CALL(plan_bind(op));
+  OpPlan      *p    = op->plan;
+  Ceed         ceed = op->ceed;
+  hipStream_t  s    = ceed->stream;
+  hipEvent_t   ev;
+  switch (p->kind) {
+    case QfKind::Setup: {
+      SetupArgs a{};
+      a.nelem = p->nelem;
+      MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
+      a.xcomp_stride = p->r_x->comp_stride;
+      MI_CALL(vec_device_read(in, &a.x));
+      a.Bx  = p->b_x->d_interp_1d;
+      a.Dx  = p->b_x->d_grad_1d;
+      a.w1d = p->b_x->d_weight_1d;
+      if (out->length < (CeedSize)p->nelem * p->Q * p->Q * p->Q * 10) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "qdata vector too short");
+      MI_CALL(vec_device_write(out, &a.qdata, true));
+      a.qd = p->qd;
+      MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+      MI_HIP(ceed, launch_setup_qdata(2, p->Q, a, s));
+      MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      return 0;
+    }
+    case QfKind::IFunction:
+    case QfKind::IJacobian: {
+      CeedMiNewtonianContext ctx;
+      MI_CALL(newtonian_context(op, &ctx));
+      if (in->length != p->r_q->l_size || out->length != p->r_q->l_size)
+        return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: active vectors must have the restriction's L-vector size", op->qf->name.c_str());
+      VolumeArgs a{};
+      a.nelem = p->nelem;
+      MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+      a.comp_stride = p->r_q->comp_stride;
+      MI_CALL(vec_device_read(in, &a.u));
+      MI_CALL(vec_device_write(out, &a.v, true));
+      MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+      a.qd = p->qd;
+      a.jd = p->jd;
+      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo};
+      if (p->kind == QfKind::IFunction) {
+        const int need = 14 + (ctx.idl_enable ? 1 : 0);
+        if (p->v_jac && p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
+        MI_CALL(vec_device_read(p->v_qdot, &a.u2));
+        MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
+        MI_CALL(vec_device_read(p->v_x, &a.x));
+        a.Bx = p->b_x->d_interp_1d;
+        if (p->v_jac) {
+          if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, true));
+          else MI_CALL(vec_device_write(p->v_jac, &a.jac, true));
+        }
+        MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+        MI_HIP(ceed, launch_ifunction(p->P, p->Q, p->var, a, ctx, s));
+        MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      } else {
+        const int need = 14 + (ctx.idl_enable ? 1 : 0);
+        if (p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
+        if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, false));
+        else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
+        MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+        MI_HIP(ceed, launch_ijacobian(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, s));
+        MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      }
+      return 0;
+    }
+    default:
+      return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "operator kind not executable");
+  }
+}
+
+int op_plan_diagonal(CeedOperator op, CeedVector diag) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly of '%s' is not provided", op->qf->name.c_str());
+}

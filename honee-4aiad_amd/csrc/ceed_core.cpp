@@ -1,0 +1,1519 @@
+// libCEED-compatible public C API on the MI355X backend: object lifetimes, errors, vectors, restrictions, bases,
+// QFunctions, contexts and operators.  Operator application is dispatched to the fused HIP kernels by
+// backend_ops.cpp; there is no host execution path for operators.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "ceed_impl.hpp"
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Sentinels
+// ---------------------------------------------------------------------------------------------------------------------
+static CeedVector_private          s_vec_active, s_vec_none;
+static CeedBasis_private           s_basis_none;
+static CeedElemRestriction_private s_rstr_none;
+static CeedQFunction_private       s_qf_none;
+static CeedRequest                 s_req_immediate = nullptr, s_req_ordered = nullptr;
+
+extern "C" {
+const CeedVector          CEED_VECTOR_ACTIVE        = &s_vec_active;
+const CeedVector          CEED_VECTOR_NONE          = &s_vec_none;
+const CeedBasis           CEED_BASIS_NONE           = &s_basis_none;
+const CeedElemRestriction CEED_ELEMRESTRICTION_NONE = &s_rstr_none;
+const CeedQFunction       CEED_QFUNCTION_NONE       = &s_qf_none;
+const CeedInt             CEED_STRIDES_BACKEND[3]   = {0, 0, 0};
+CeedRequest *const        CEED_REQUEST_IMMEDIATE    = &s_req_immediate;
+CeedRequest *const        CEED_REQUEST_ORDERED      = &s_req_ordered;
+}
+
+static bool is_active(CeedVector v) { return v == CEED_VECTOR_ACTIVE; }
+static bool is_none_vec(CeedVector v) { return v == CEED_VECTOR_NONE; }
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------------------------------------------------
+int ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int ecode, const char *fmt, ...) {
+  va_list args;
+  va_start(args, fmt);
+  int ret = ecode;
+  if (ceed && ceed->handler) {
+    ret = ceed->handler(ceed, file, line, func, ecode, fmt, &args);
+  } else if (ceed) {
+    ret = CeedErrorStore(ceed, file, line, func, ecode, fmt, &args);
+  } else {
+    vfprintf(stderr, fmt, args);
+    fprintf(stderr, "\n");
+  }
+  va_end(args);
+  return ret;
+}
+
+extern "C" int CeedErrorImpl(Ceed ceed, const char *filename, int lineno, const char *func, int ecode, const char *format, ...) {
+  va_list args;
+  va_start(args, format);
+  int ret = ecode;
+  if (ceed && ceed->handler) ret = ceed->handler(ceed, filename, lineno, func, ecode, format, &args);
+  else if (ceed) ret = CeedErrorStore(ceed, filename, lineno, func, ecode, format, &args);
+  va_end(args);
+  return ret;
+}
+
+extern "C" int CeedErrorReturn(Ceed ceed, const char *filename, int lineno, const char *func, int ecode, const char *format, va_list *args) {
+  return ecode;
+}
+
+extern "C" int CeedErrorStore(Ceed ceed, const char *filename, int lineno, const char *func, int ecode, const char *format, va_list *args) {
+  char buf[1024];
+  int  n = snprintf(buf, sizeof(buf), "%s:%d in %s(): ", filename, lineno, func);
+  if (n < 0) n = 0;
+  vsnprintf(buf + n, sizeof(buf) - n, format, *args);
+  if (ceed) ceed->err_msg = buf;
+  return ecode;
+}
+
+extern "C" int CeedErrorAbort(Ceed ceed, const char *filename, int lineno, const char *func, int ecode, const char *format, va_list *args) {
+  fprintf(stderr, "%s:%d in %s(): ", filename, lineno, func);
+  vfprintf(stderr, format, *args);
+  fprintf(stderr, "\n");
+  abort();
+  return ecode;
+}
+
+extern "C" int CeedSetErrorHandler(Ceed ceed, CeedErrorHandler handler) {
+  ceed->handler = handler;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedGetErrorMessage(Ceed ceed, const char **err_msg) {
+  *err_msg = ceed->err_msg.c_str();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedResetErrorMessage(Ceed ceed, const char **err_msg) {
+  ceed->err_msg.clear();
+  if (err_msg) *err_msg = ceed->err_msg.c_str();
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Ceed
+// ---------------------------------------------------------------------------------------------------------------------
+extern "C" int CeedInit(const char *resource, Ceed *ceed) {
+  if (!resource || !ceed) return CEED_ERROR_MAJOR;
+  std::string res(resource);
+  std::string base = res.substr(0, res.find(':'));
+  if (base != "/gpu/hip/mi355x" && base != "/gpu/hip" && base != "/gpu/hip/gen" && base != "/gpu/hip/shared") {
+    fprintf(stderr, "CeedInit: resource '%s' is not provided by the MI355X library (use /gpu/hip/mi355x)\n", resource);
+    return CEED_ERROR_UNSUPPORTED;
+  }
+  Ceed c      = new Ceed_private();
+  c->resource = "/gpu/hip/mi355x";
+  int ndev    = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    delete c;
+    fprintf(stderr, "CeedInit: no HIP device available for /gpu/hip/mi355x\n");
+    return CEED_ERROR_BACKEND;
+  }
+  hipGetDevice(&c->device);
+  // options: ":key=value" pairs
+  size_t pos = res.find(':');
+  while (pos != std::string::npos) {
+    size_t      next = res.find_first_of(":,", pos + 1);
+    std::string kv   = res.substr(pos + 1, next == std::string::npos ? std::string::npos : next - pos - 1);
+    size_t      eq   = kv.find('=');
+    std::string k = kv.substr(0, eq), v = eq == std::string::npos ? "" : kv.substr(eq + 1);
+    if (k == "device_id") {
+      c->device = atoi(v.c_str());
+    } else if (k == "jacobian_precision") {
+      c->jac_fp32 = (v == "fp32" || v == "float");
+    } else if (k == "scatter") {
+      c->colored = (v == "colored");
+    } else if (!k.empty()) {
+      fprintf(stderr, "CeedInit: unknown resource option '%s'\n", k.c_str());
+      delete c;
+      return CEED_ERROR_UNSUPPORTED;
+    }
+    pos = next;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) {
+    delete c;
+    return CEED_ERROR_BACKEND;
+  }
+  *ceed = c;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedReferenceCopy(Ceed ceed, Ceed *ceed_copy) {
+  ceed->refcount++;
+  if (*ceed_copy) CeedDestroy(ceed_copy);
+  *ceed_copy = ceed;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedGetResource(Ceed ceed, const char **resource) {
+  *resource = ceed->resource.c_str();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedGetPreferredMemType(Ceed ceed, CeedMemType *type) {
+  *type = CEED_MEM_DEVICE;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedSetStream(Ceed ceed, void *handle) {
+  ceed->stream = handle ? *(hipStream_t *)handle : nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedGetVersion(int *major, int *minor, int *patch, bool *release) {
+  if (major) *major = CEED_VERSION_MAJOR;
+  if (minor) *minor = CEED_VERSION_MINOR;
+  if (patch) *patch = CEED_VERSION_PATCH;
+  if (release) *release = CEED_VERSION_RELEASE;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedGetGitVersion(const char **git_version) {
+  *git_version = "honee-mi355x";
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedDestroy(Ceed *ceed) {
+  if (!*ceed) return CEED_ERROR_SUCCESS;
+  if (--(*ceed)->refcount > 0) {
+    *ceed = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  Ceed c = *ceed;
+  for (auto &pe : c->pending_events) {
+    hipEventDestroy(pe.second.first);
+    hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+  if (c->d_scratch) hipFree(c->d_scratch);
+  if (c->h_scratch) hipHostFree(c->h_scratch);
+  delete c;
+  *ceed = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedFree(void *p) {
+  void **pp = (void **)p;
+  free(*pp);
+  *pp = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Kernel timing (HIP events on the backend stream)
+// ---------------------------------------------------------------------------------------------------------------------
+static hipEvent_t event_get(Ceed ceed) {
+  if (!ceed->event_pool.empty()) {
+    hipEvent_t e = ceed->event_pool.back();
+    ceed->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+int timing_begin(Ceed ceed, const char *name, hipEvent_t *start) {
+  *start = nullptr;
+  if (!ceed->timing) return 0;
+  *start = event_get(ceed);
+  MI_HIP(ceed, hipEventRecord(*start, ceed->stream));
+  return 0;
+}
+
+int timing_end(Ceed ceed, const char *name, hipEvent_t start) {
+  if (!ceed->timing || !start) return 0;
+  hipEvent_t stop = event_get(ceed);
+  MI_HIP(ceed, hipEventRecord(stop, ceed->stream));
+  ceed->pending_events.push_back({name, {start, stop}});
+  return 0;
+}
+
+static int timing_collect(Ceed ceed) {
+  if (ceed->pending_events.empty()) return 0;
+  MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
+  for (auto &pe : ceed->pending_events) {
+    float ms = 0;
+    MI_HIP(ceed, hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    auto &acc = ceed->kernel_times[pe.first];
+    acc.first += 1;
+    acc.second += ms;
+    ceed->event_pool.push_back(pe.second.first);
+    ceed->event_pool.push_back(pe.second.second);
+  }
+  ceed->pending_events.clear();
+  return 0;
+}
+
+extern "C" int CeedMiGetStream(Ceed ceed, void **hip_stream) {
+  *hip_stream = (void *)ceed->stream;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiSetKernelTiming(Ceed ceed, bool enable) {
+  ceed->timing = enable;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiResetKernelTimes(Ceed ceed) {
+  MI_CALL(timing_collect(ceed));
+  ceed->kernel_times.clear();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiGetKernelTimes(Ceed ceed, int max_kernels, char *names, size_t names_len, int *launches, double *total_ms, int *num_kernels) {
+  MI_CALL(timing_collect(ceed));
+  int         n = 0;
+  std::string joined;
+  for (auto &kv : ceed->kernel_times) {
+    if (n >= max_kernels) break;
+    if (n) joined += ",";
+    joined += kv.first;
+    launches[n] = kv.second.first;
+    total_ms[n] = kv.second.second;
+    n++;
+  }
+  if (names && names_len) {
+    strncpy(names, joined.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  *num_kernels = n;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiSynchronize(Ceed ceed) {
+  MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Vectors
+// ---------------------------------------------------------------------------------------------------------------------
+extern "C" int CeedVectorCreate(Ceed ceed, CeedSize len, CeedVector *vec) {
+  if (len < 0) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "vector length must be non-negative");
+  CeedVector v = new CeedVector_private();
+  v->ceed      = ceed;
+  ceed->refcount++;
+  v->length = len;
+  *vec      = v;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorReferenceCopy(CeedVector vec, CeedVector *vec_copy) {
+  if (vec != CEED_VECTOR_ACTIVE && vec != CEED_VECTOR_NONE) vec->refcount++;
+  if (*vec_copy) MI_CALL(CeedVectorDestroy(vec_copy));
+  *vec_copy = vec;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int vec_alloc_device(CeedVector v) {
+  if (!v->d_array) {
+    MI_HIP(v->ceed, hipMalloc((void **)&v->d_array, sizeof(double) * std::max<CeedSize>(v->length, 1)));
+    v->d_owned = true;
+  }
+  return 0;
+}
+
+static int vec_alloc_host(CeedVector v) {
+  if (!v->h_array) {
+    v->h_array = (double *)malloc(sizeof(double) * std::max<CeedSize>(v->length, 1));
+    if (!v->h_array) return MI_ERR(v->ceed, CEED_ERROR_MAJOR, "host allocation failed");
+    v->h_owned = true;
+  }
+  return 0;
+}
+
+static int vec_sync_device(CeedVector v) {
+  if (v->d_valid) return 0;
+  MI_CALL(vec_alloc_device(v));
+  if (v->h_valid && v->length) {
+    MI_HIP(v->ceed, hipMemcpyAsync(v->d_array, v->h_array, sizeof(double) * v->length, hipMemcpyHostToDevice, v->ceed->stream));
+    MI_HIP(v->ceed, hipStreamSynchronize(v->ceed->stream));
+  }
+  v->d_valid = true;
+  return 0;
+}
+
+static int vec_sync_host(CeedVector v) {
+  if (v->h_valid) return 0;
+  MI_CALL(vec_alloc_host(v));
+  if (v->d_valid && v->length) {
+    MI_HIP(v->ceed, hipMemcpyAsync(v->h_array, v->d_array, sizeof(double) * v->length, hipMemcpyDeviceToHost, v->ceed->stream));
+    MI_HIP(v->ceed, hipStreamSynchronize(v->ceed->stream));
+  }
+  v->h_valid = true;
+  return 0;
+}
+
+static bool vec_has_data(CeedVector v) { return v->h_valid || v->d_valid; }
+
+int vec_device_read(CeedVector v, const double **d) {
+  if (!vec_has_data(v)) return MI_ERR(v->ceed, CEED_ERROR_INCOMPLETE, "CeedVector has no data set (length %lld)", (long long)v->length);
+  MI_CALL(vec_sync_device(v));
+  *d = v->d_array;
+  return 0;
+}
+
+int vec_device_write(CeedVector v, double **d, bool keep_contents) {
+  if (keep_contents) {
+    if (vec_has_data(v)) MI_CALL(vec_sync_device(v));
+    else MI_CALL(vec_alloc_device(v));
+  } else {
+    MI_CALL(vec_alloc_device(v));
+  }
+  v->d_valid = true;
+  v->h_valid = false;
+  *d         = v->d_array;
+  return 0;
+}
+
+int vec_device_shadow32(CeedVector v, float **d, bool create) {
+  if (!v->d_shadow32) {
+    if (!create) return MI_ERR(v->ceed, CEED_ERROR_INCOMPLETE, "vector has no FP32 shadow");
+    MI_HIP(v->ceed, hipMalloc((void **)&v->d_shadow32, sizeof(float) * std::max<CeedSize>(v->length, 1)));
+    v->shadow_len = v->length;
+  }
+  *d = v->d_shadow32;
+  return 0;
+}
+
+extern "C" int CeedMiVectorGetFp32ShadowRead(CeedVector vec, const float **d_array, CeedSize *length) {
+  float *d;
+  MI_CALL(vec_device_shadow32(vec, &d, false));
+  *d_array = d;
+  if (length) *length = vec->shadow_len;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorSetArray(CeedVector v, CeedMemType mem_type, CeedCopyMode copy_mode, CeedScalar *array) {
+  if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "cannot set array while it is accessed");
+  if (mem_type == CEED_MEM_HOST) {
+    switch (copy_mode) {
+      case CEED_COPY_VALUES:
+        MI_CALL(vec_alloc_host(v));
+        if (array) memcpy(v->h_array, array, sizeof(double) * v->length);
+        break;
+      case CEED_OWN_POINTER:
+      case CEED_USE_POINTER:
+        if (v->h_owned) free(v->h_array);
+        v->h_array = array;
+        v->h_owned = copy_mode == CEED_OWN_POINTER;
+        break;
+    }
+    v->h_valid = true;
+    v->d_valid = false;
+  } else {
+    switch (copy_mode) {
+      case CEED_COPY_VALUES:
+        MI_CALL(vec_alloc_device(v));
+        if (array) MI_HIP(v->ceed, hipMemcpyAsync(v->d_array, array, sizeof(double) * v->length, hipMemcpyDeviceToDevice, v->ceed->stream));
+        break;
+      case CEED_OWN_POINTER:
+      case CEED_USE_POINTER:
+        if (v->d_owned) MI_HIP(v->ceed, hipFree(v->d_array));
+        v->d_array = array;
+        v->d_owned = copy_mode == CEED_OWN_POINTER;
+        break;
+    }
+    v->d_valid = true;
+    v->h_valid = false;
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorSetValue(CeedVector v, CeedScalar value) {
+  double *d;
+  MI_CALL(vec_device_write(v, &d, false));
+  MI_HIP(v->ceed, launch_fill(d, value, v->length, v->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorSyncArray(CeedVector v, CeedMemType mem_type) {
+  if (!vec_has_data(v)) return CEED_ERROR_SUCCESS;
+  return mem_type == CEED_MEM_HOST ? vec_sync_host(v) : vec_sync_device(v);
+}
+
+extern "C" int CeedVectorTakeArray(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
+  if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "cannot take array while it is accessed");
+  if (mem_type == CEED_MEM_HOST) {
+    if (vec_has_data(v)) MI_CALL(vec_sync_host(v));
+    *array     = v->h_array;
+    v->h_array = nullptr;
+    v->h_owned = false;
+    v->h_valid = false;
+  } else {
+    if (vec_has_data(v)) MI_CALL(vec_sync_device(v));
+    *array     = v->d_array;
+    v->d_array = nullptr;
+    v->d_owned = false;
+    v->d_valid = false;
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorGetArray(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
+  if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "vector already accessed");
+  if (mem_type == CEED_MEM_HOST) {
+    if (vec_has_data(v)) MI_CALL(vec_sync_host(v));
+    else MI_CALL(vec_alloc_host(v));
+    v->h_valid = true;
+    v->d_valid = false;
+    *array     = v->h_array;
+  } else {
+    MI_CALL(vec_device_write(v, array, true));
+  }
+  v->writing = true;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorGetArrayWrite(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
+  if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "vector already accessed");
+  if (mem_type == CEED_MEM_HOST) {
+    MI_CALL(vec_alloc_host(v));
+    v->h_valid = true;
+    v->d_valid = false;
+    *array     = v->h_array;
+  } else {
+    MI_CALL(vec_device_write(v, array, false));
+  }
+  v->writing = true;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorGetArrayRead(CeedVector v, CeedMemType mem_type, const CeedScalar **array) {
+  if (v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "vector is being written");
+  if (!vec_has_data(v)) return MI_ERR(v->ceed, CEED_ERROR_INCOMPLETE, "CeedVector has no data");
+  if (mem_type == CEED_MEM_HOST) {
+    MI_CALL(vec_sync_host(v));
+    *array = v->h_array;
+  } else {
+    MI_CALL(vec_sync_device(v));
+    *array = v->d_array;
+  }
+  v->num_readers++;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorRestoreArray(CeedVector v, CeedScalar **array) {
+  v->writing = false;
+  *array     = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorRestoreArrayRead(CeedVector v, const CeedScalar **array) {
+  if (v->num_readers > 0) v->num_readers--;
+  *array = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorCopy(CeedVector vec, CeedVector vec_copy) {
+  if (vec->length != vec_copy->length) return MI_ERR(vec->ceed, CEED_ERROR_DIMENSION, "vector lengths differ");
+  const double *s;
+  double       *d;
+  MI_CALL(vec_device_read(vec, &s));
+  MI_CALL(vec_device_write(vec_copy, &d, false));
+  MI_HIP(vec->ceed, hipMemcpyAsync(d, s, sizeof(double) * vec->length, hipMemcpyDeviceToDevice, vec->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorNorm(CeedVector v, CeedNormType type, CeedScalar *norm) {
+  Ceed          ceed = v->ceed;
+  const double *d;
+  MI_CALL(vec_device_read(v, &d));
+  const int npart = 1024;
+  if (ceed->scratch_len < npart) {
+    if (ceed->d_scratch) hipFree(ceed->d_scratch);
+    if (ceed->h_scratch) hipHostFree(ceed->h_scratch);
+    MI_HIP(ceed, hipMalloc((void **)&ceed->d_scratch, sizeof(double) * npart));
+    MI_HIP(ceed, hipHostMalloc((void **)&ceed->h_scratch, sizeof(double) * npart));
+    ceed->scratch_len = npart;
+  }
+  MI_HIP(ceed, launch_norm(d, v->length, (int)type, ceed->d_scratch, npart, ceed->stream));
+  MI_HIP(ceed, hipMemcpyAsync(ceed->h_scratch, ceed->d_scratch, sizeof(double) * npart, hipMemcpyDeviceToHost, ceed->stream));
+  MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
+  double acc = 0;
+  for (int i = 0; i < npart; i++) {
+    if (type == CEED_NORM_MAX) acc = std::max(acc, ceed->h_scratch[i]);
+    else acc += ceed->h_scratch[i];
+  }
+  *norm = type == CEED_NORM_2 ? sqrt(acc) : acc;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorScale(CeedVector x, CeedScalar alpha) {
+  double *d;
+  MI_CALL(vec_device_write(x, &d, true));
+  MI_HIP(x->ceed, launch_scale(d, alpha, x->length, x->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorAXPY(CeedVector y, CeedScalar alpha, CeedVector x) { return CeedVectorAXPBY(y, alpha, 1.0, x); }
+
+extern "C" int CeedVectorAXPBY(CeedVector y, CeedScalar alpha, CeedScalar beta, CeedVector x) {
+  if (x->length != y->length) return MI_ERR(y->ceed, CEED_ERROR_DIMENSION, "vector lengths differ");
+  const double *dx;
+  double       *dy;
+  MI_CALL(vec_device_read(x, &dx));
+  MI_CALL(vec_device_write(y, &dy, true));
+  MI_HIP(y->ceed, launch_axpby(dy, alpha, beta, dx, y->length, y->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorPointwiseMult(CeedVector w, CeedVector x, CeedVector y) {
+  const double *dx, *dy;
+  double       *dw;
+  MI_CALL(vec_device_read(x, &dx));
+  MI_CALL(vec_device_read(y, &dy));
+  MI_CALL(vec_device_write(w, &dw, w == x || w == y));
+  MI_HIP(w->ceed, launch_pointwise_mult(dw, dx, dy, w->length, w->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorReciprocal(CeedVector v) {
+  double *d;
+  MI_CALL(vec_device_write(v, &d, true));
+  MI_HIP(v->ceed, launch_reciprocal(d, v->length, v->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorView(CeedVector v, const char *fp_fmt, FILE *stream) {
+  const double *h;
+  MI_CALL(CeedVectorGetArrayRead(v, CEED_MEM_HOST, &h));
+  fprintf(stream, "CeedVector length %lld\n", (long long)v->length);
+  for (CeedSize i = 0; i < v->length; i++) {
+    fprintf(stream, fp_fmt ? fp_fmt : "  %12.8f\n", h[i]);
+  }
+  return CeedVectorRestoreArrayRead(v, &h);
+}
+
+extern "C" int CeedVectorGetCeed(CeedVector vec, Ceed *ceed) {
+  *ceed = nullptr;
+  return CeedReferenceCopy(vec->ceed, ceed);
+}
+extern "C" Ceed CeedVectorReturnCeed(CeedVector vec) { return vec->ceed; }
+extern "C" int  CeedVectorGetLength(CeedVector vec, CeedSize *length) {
+  *length = vec->length;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedVectorDestroy(CeedVector *vec) {
+  CeedVector v = *vec;
+  if (!v || v == CEED_VECTOR_ACTIVE || v == CEED_VECTOR_NONE) {
+    *vec = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (--v->refcount > 0) {
+    *vec = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (v->h_owned) free(v->h_array);
+  if (v->d_owned) hipFree(v->d_array);
+  if (v->d_shadow32) hipFree(v->d_shadow32);
+  Ceed c = v->ceed;
+  delete v;
+  CeedDestroy(&c);
+  *vec = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Element restrictions
+// ---------------------------------------------------------------------------------------------------------------------
+extern "C" int CeedElemRestrictionCreate(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedInt comp_stride, CeedSize l_size,
+                                         CeedMemType mem_type, CeedCopyMode copy_mode, const CeedInt *offsets, CeedElemRestriction *rstr) {
+  if (mem_type != CEED_MEM_HOST) return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "restriction offsets must be given in host memory");
+  if (num_elem < 0 || elem_size < 1 || num_comp < 1) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "invalid restriction dimensions");
+  CeedElemRestriction r = new CeedElemRestriction_private();
+  r->ceed               = ceed;
+  ceed->refcount++;
+  r->num_elem    = num_elem;
+  r->elem_size   = elem_size;
+  r->num_comp    = num_comp;
+  r->comp_stride = comp_stride;
+  r->l_size      = l_size;
+  r->h_offsets.assign(offsets, offsets + (size_t)num_elem * elem_size);
+  for (CeedInt o : r->h_offsets) {
+    if (o < 0 || (CeedSize)o + (CeedSize)(num_comp - 1) * comp_stride >= l_size) {
+      delete r;
+      CeedDestroy(&ceed);
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "restriction offset %d out of range for L-vector size %lld", o, (long long)l_size);
+    }
+  }
+  if (copy_mode == CEED_OWN_POINTER) free((void *)offsets);
+  *rstr = r;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionCreateStrided(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedSize l_size,
+                                                const CeedInt strides[3], CeedElemRestriction *rstr) {
+  CeedElemRestriction r = new CeedElemRestriction_private();
+  r->ceed               = ceed;
+  ceed->refcount++;
+  r->num_elem        = num_elem;
+  r->elem_size       = elem_size;
+  r->num_comp        = num_comp;
+  r->l_size          = l_size;
+  r->strided         = true;
+  r->backend_strides = strides == CEED_STRIDES_BACKEND;
+  if (r->backend_strides) {  // [comp][elem][point]: each component's points of consecutive elements are contiguous
+    r->strides[0] = 1;
+    r->strides[1] = num_elem * elem_size;
+    r->strides[2] = elem_size;
+  } else {
+    for (int i = 0; i < 3; i++) r->strides[i] = strides[i];
+  }
+  *rstr = r;
+  return CEED_ERROR_SUCCESS;
+}
+
+int rstr_device_offsets(CeedElemRestriction r, const int **d) {
+  if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no offsets");
+  if (!r->d_offsets) {
+    MI_HIP(r->ceed, hipMalloc((void **)&r->d_offsets, sizeof(int) * std::max<size_t>(r->h_offsets.size(), 1)));
+    MI_HIP(r->ceed, hipMemcpy(r->d_offsets, r->h_offsets.data(), sizeof(int) * r->h_offsets.size(), hipMemcpyHostToDevice));
+  }
+  *d = r->d_offsets;
+  return 0;
+}
+
+extern "C" int CeedElemRestrictionReferenceCopy(CeedElemRestriction rstr, CeedElemRestriction *rstr_copy) {
+  if (rstr != CEED_ELEMRESTRICTION_NONE) rstr->refcount++;
+  if (*rstr_copy) MI_CALL(CeedElemRestrictionDestroy(rstr_copy));
+  *rstr_copy = rstr;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionCreateVector(CeedElemRestriction rstr, CeedVector *l_vec, CeedVector *e_vec) {
+  if (l_vec) MI_CALL(CeedVectorCreate(rstr->ceed, rstr->l_size, l_vec));
+  if (e_vec) MI_CALL(CeedVectorCreate(rstr->ceed, (CeedSize)rstr->num_elem * rstr->elem_size * rstr->num_comp, e_vec));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionApply(CeedElemRestriction r, CeedTransposeMode t_mode, CeedVector u, CeedVector ru, CeedRequest *request) {
+  if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_UNSUPPORTED, "CeedElemRestrictionApply on strided restrictions is not provided");
+  const int    *off;
+  const double *din;
+  double       *dout;
+  MI_CALL(rstr_device_offsets(r, &off));
+  MI_CALL(vec_device_read(u, &din));
+  if (t_mode == CEED_NOTRANSPOSE) {
+    MI_CALL(vec_device_write(ru, &dout, false));
+    MI_HIP(r->ceed, launch_restriction_notranspose(off, r->num_elem, r->elem_size, r->num_comp, r->comp_stride, din, dout, r->ceed->stream));
+  } else {
+    MI_CALL(vec_device_write(ru, &dout, true));
+    MI_HIP(r->ceed, launch_restriction_transpose(off, r->num_elem, r->elem_size, r->num_comp, r->comp_stride, din, dout, r->ceed->stream));
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionGetCeed(CeedElemRestriction rstr, Ceed *ceed) {
+  *ceed = nullptr;
+  return CeedReferenceCopy(rstr->ceed, ceed);
+}
+extern "C" Ceed CeedElemRestrictionReturnCeed(CeedElemRestriction rstr) { return rstr->ceed; }
+extern "C" int  CeedElemRestrictionGetNumElements(CeedElemRestriction rstr, CeedInt *num_elem) {
+  *num_elem = rstr->num_elem;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedElemRestrictionGetElementSize(CeedElemRestriction rstr, CeedInt *elem_size) {
+  *elem_size = rstr->elem_size;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedElemRestrictionGetLVectorSize(CeedElemRestriction rstr, CeedSize *l_size) {
+  *l_size = rstr->l_size;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedElemRestrictionGetEVectorSize(CeedElemRestriction rstr, CeedSize *e_size) {
+  *e_size = (CeedSize)rstr->num_elem * rstr->elem_size * rstr->num_comp;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedElemRestrictionGetNumComponents(CeedElemRestriction rstr, CeedInt *num_comp) {
+  *num_comp = rstr->num_comp;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionGetMultiplicity(CeedElemRestriction r, CeedVector mult) {
+  if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_UNSUPPORTED, "multiplicity of strided restriction");
+  const int *off;
+  double    *d;
+  MI_CALL(rstr_device_offsets(r, &off));
+  MI_CALL(vec_device_write(mult, &d, false));
+  MI_HIP(r->ceed, launch_fill(d, 0.0, mult->length, r->ceed->stream));
+  MI_HIP(r->ceed, launch_multiplicity(off, (long long)r->h_offsets.size(), r->num_comp, r->comp_stride, d, r->ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
+  CeedElemRestriction r = *rstr;
+  if (!r || r == CEED_ELEMRESTRICTION_NONE) {
+    *rstr = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (--r->refcount > 0) {
+    *rstr = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (r->d_offsets) hipFree(r->d_offsets);
+  Ceed c = r->ceed;
+  delete r;
+  CeedDestroy(&c);
+  *rstr = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Bases
+// ---------------------------------------------------------------------------------------------------------------------
+// Pseudo-inverse of a tall Q x P matrix (Q >= P): (A^T A)^{-1} A^T, returned P x Q row-major.
+static int pinv_tall(const double *A, int Q, int P, std::vector<double> &Ap) {
+  std::vector<double> M(P * P, 0.0), Minv(P * P, 0.0);
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++)
+      for (int q = 0; q < Q; q++) M[i * P + j] += A[q * P + i] * A[q * P + j];
+  for (int i = 0; i < P; i++) Minv[i * P + i] = 1.0;
+  for (int c = 0; c < P; c++) {  // Gauss-Jordan with partial pivoting
+    int piv = c;
+    for (int r = c + 1; r < P; r++)
+      if (fabs(M[r * P + c]) > fabs(M[piv * P + c])) piv = r;
+    if (fabs(M[piv * P + c]) < 1e-300) return 1;
+    for (int k = 0; k < P; k++) {
+      std::swap(M[c * P + k], M[piv * P + k]);
+      std::swap(Minv[c * P + k], Minv[piv * P + k]);
+    }
+    const double d = M[c * P + c];
+    for (int k = 0; k < P; k++) {
+      M[c * P + k] /= d;
+      Minv[c * P + k] /= d;
+    }
+    for (int r = 0; r < P; r++) {
+      if (r == c) continue;
+      const double f = M[r * P + c];
+      for (int k = 0; k < P; k++) {
+        M[r * P + k] -= f * M[c * P + k];
+        Minv[r * P + k] -= f * Minv[c * P + k];
+      }
+    }
+  }
+  Ap.assign(P * Q, 0.0);
+  for (int i = 0; i < P; i++)
+    for (int q = 0; q < Q; q++)
+      for (int k = 0; k < P; k++) Ap[i * Q + q] += Minv[i * P + k] * A[q * P + k];
+  return 0;
+}
+
+extern "C" int CeedBasisCreateTensorH1(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P_1d, CeedInt Q_1d, const CeedScalar *interp_1d,
+                                       const CeedScalar *grad_1d, const CeedScalar *q_ref_1d, const CeedScalar *q_weight_1d, CeedBasis *basis) {
+  if (dim < 1 || dim > 3 || P_1d < 1 || Q_1d < 1) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "invalid tensor basis dimensions");
+  CeedBasis b = new CeedBasis_private();
+  b->ceed     = ceed;
+  ceed->refcount++;
+  b->tensor    = true;
+  b->dim       = dim;
+  b->num_comp  = num_comp;
+  b->P_1d      = P_1d;
+  b->Q_1d      = Q_1d;
+  b->num_nodes = (CeedInt)pow(P_1d, dim);
+  b->num_qpts  = (CeedInt)pow(Q_1d, dim);
+  b->interp_1d.assign(interp_1d, interp_1d + P_1d * Q_1d);
+  b->grad_1d.assign(grad_1d, grad_1d + P_1d * Q_1d);
+  if (q_ref_1d) b->q_ref_1d.assign(q_ref_1d, q_ref_1d + Q_1d);
+  if (q_weight_1d) b->q_weight_1d.assign(q_weight_1d, q_weight_1d + Q_1d);
+  if (Q_1d >= P_1d) {  // collocated derivative Dc = G B^+ (Dc B = G on the element space)
+    std::vector<double> Bp;
+    if (!pinv_tall(b->interp_1d.data(), Q_1d, P_1d, Bp)) {
+      b->collo_grad_1d.assign(Q_1d * Q_1d, 0.0);
+      for (int i = 0; i < Q_1d; i++)
+        for (int j = 0; j < Q_1d; j++)
+          for (int k = 0; k < P_1d; k++) b->collo_grad_1d[i * Q_1d + j] += b->grad_1d[i * P_1d + k] * Bp[k * Q_1d + j];
+    }
+  }
+  *basis = b;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedBasisCreateH1(Ceed ceed, CeedElemTopology topo, CeedInt num_comp, CeedInt num_nodes, CeedInt num_qpts, const CeedScalar *interp,
+                                 const CeedScalar *grad, const CeedScalar *q_ref, const CeedScalar *q_weights, CeedBasis *basis) {
+  CeedBasis b = new CeedBasis_private();
+  b->ceed     = ceed;
+  ceed->refcount++;
+  b->tensor    = false;
+  b->topo      = topo;
+  b->dim       = CEED_TOPOLOGY_DIM(topo);
+  b->num_comp  = num_comp;
+  b->num_nodes = num_nodes;
+  b->num_qpts  = num_qpts;
+  b->interp.assign(interp, interp + num_nodes * num_qpts);
+  b->grad.assign(grad, grad + num_nodes * num_qpts * b->dim);
+  if (q_ref) b->q_ref.assign(q_ref, q_ref + num_qpts * b->dim);
+  if (q_weights) b->q_weight.assign(q_weights, q_weights + num_qpts);
+  *basis = b;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedBasisCreateProjection(CeedBasis from, CeedBasis to, CeedBasis *basis_project) {
+  if (!from->tensor || !to->tensor || from->dim != to->dim || from->Q_1d != to->Q_1d)
+    return MI_ERR(from->ceed, CEED_ERROR_UNSUPPORTED, "projection needs two tensor bases with the same quadrature");
+  std::vector<double> Bp;  // (P_to x Q)
+  if (pinv_tall(to->interp_1d.data(), to->Q_1d, to->P_1d, Bp)) return MI_ERR(from->ceed, CEED_ERROR_BACKEND, "singular target basis");
+  const int           Q = to->Q_1d, Pt = to->P_1d, Pf = from->P_1d;
+  std::vector<double> I(Pt * Pf, 0.0), G(Pt * Pf, 0.0), qref(Pt, 0.0), w(Pt, 0.0);
+  for (int i = 0; i < Pt; i++)
+    for (int j = 0; j < Pf; j++)
+      for (int q = 0; q < Q; q++) {
+        I[i * Pf + j] += Bp[i * Q + q] * from->interp_1d[q * Pf + j];
+        G[i * Pf + j] += Bp[i * Q + q] * from->grad_1d[q * Pf + j];
+      }
+  MI_CALL(CeedBasisCreateTensorH1(from->ceed, from->dim, from->num_comp, Pf, Pt, I.data(), G.data(), qref.data(), w.data(), basis_project));
+  (*basis_project)->is_projection = true;
+  return CEED_ERROR_SUCCESS;
+}
+
+int basis_device_tables(CeedBasis b) {
+  if (b->d_interp_1d) return 0;
+  if (!b->tensor) return MI_ERR(b->ceed, CEED_ERROR_UNSUPPORTED, "non-tensor bases are not supported by the fused kernels");
+  if (b->collo_grad_1d.empty()) return MI_ERR(b->ceed, CEED_ERROR_UNSUPPORTED, "basis needs Q_1d >= P_1d");
+  const int P = b->P_1d, Q = b->Q_1d;
+  MI_HIP(b->ceed, hipMalloc((void **)&b->d_interp_1d, sizeof(double) * P * Q));
+  MI_HIP(b->ceed, hipMalloc((void **)&b->d_grad_1d, sizeof(double) * P * Q));
+  MI_HIP(b->ceed, hipMalloc((void **)&b->d_collo, sizeof(double) * Q * Q));
+  MI_HIP(b->ceed, hipMalloc((void **)&b->d_weight_1d, sizeof(double) * Q));
+  MI_HIP(b->ceed, hipMemcpy(b->d_interp_1d, b->interp_1d.data(), sizeof(double) * P * Q, hipMemcpyHostToDevice));
+  MI_HIP(b->ceed, hipMemcpy(b->d_grad_1d, b->grad_1d.data(), sizeof(double) * P * Q, hipMemcpyHostToDevice));
+  MI_HIP(b->ceed, hipMemcpy(b->d_collo, b->collo_grad_1d.data(), sizeof(double) * Q * Q, hipMemcpyHostToDevice));
+  std::vector<double> w = b->q_weight_1d;
+  w.resize(Q, 0.0);
+  MI_HIP(b->ceed, hipMemcpy(b->d_weight_1d, w.data(), sizeof(double) * Q, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int CeedBasisReferenceCopy(CeedBasis basis, CeedBasis *basis_copy) {
+  if (basis != CEED_BASIS_NONE) basis->refcount++;
+  if (*basis_copy) MI_CALL(CeedBasisDestroy(basis_copy));
+  *basis_copy = basis;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetCeed(CeedBasis basis, Ceed *ceed) {
+  *ceed = nullptr;
+  return CeedReferenceCopy(basis->ceed, ceed);
+}
+extern "C" Ceed CeedBasisReturnCeed(CeedBasis basis) { return basis->ceed; }
+extern "C" int  CeedBasisGetDimension(CeedBasis basis, CeedInt *dim) {
+  *dim = basis->dim;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetNumComponents(CeedBasis basis, CeedInt *num_comp) {
+  *num_comp = basis->num_comp;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetNumNodes(CeedBasis basis, CeedInt *P) {
+  *P = basis->num_nodes;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetNumNodes1D(CeedBasis basis, CeedInt *P_1d) {
+  *P_1d = basis->P_1d;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetNumQuadraturePoints(CeedBasis basis, CeedInt *Q) {
+  *Q = basis->num_qpts;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisGetNumQuadraturePoints1D(CeedBasis basis, CeedInt *Q_1d) {
+  *Q_1d = basis->Q_1d;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedBasisDestroy(CeedBasis *basis) {
+  CeedBasis b = *basis;
+  if (!b || b == CEED_BASIS_NONE) {
+    *basis = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (--b->refcount > 0) {
+    *basis = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (b->d_interp_1d) hipFree(b->d_interp_1d);
+  if (b->d_grad_1d) hipFree(b->d_grad_1d);
+  if (b->d_collo) hipFree(b->d_collo);
+  if (b->d_weight_1d) hipFree(b->d_weight_1d);
+  Ceed c = b->ceed;
+  delete b;
+  CeedDestroy(&c);
+  *basis = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// QFunctions
+// ---------------------------------------------------------------------------------------------------------------------
+// Identify a QFunction from its "<file>:<name>" source location; the fused kernels implement these by name.
+static void qf_identify(CeedQFunction qf) {
+  const std::string &n = qf->name;
+  auto               var_of = [](const std::string &s) {
+    if (s.size() >= 7 && s.compare(s.size() - 7, 7, "Conserv") == 0) return 0;
+    if (s.size() >= 4 && s.compare(s.size() - 4, 4, "Prim") == 0) return 1;
+    if (s.size() >= 7 && s.compare(s.size() - 7, 7, "Entropy") == 0) return 2;
+    return -1;
+  };
+  if (n == "Setup") {
+    qf->kind = QfKind::Setup;
+  } else if (n.rfind("IFunction_Newtonian_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::IFunction;
+    qf->state_var = var_of(n);
+  } else if (n.rfind("IJacobian_Newtonian_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::IJacobian;
+    qf->state_var = var_of(n);
+  }
+}
+
+extern "C" int CeedQFunctionCreateInterior(Ceed ceed, CeedInt vec_length, CeedQFunctionUser f, const char *source, CeedQFunction *qf) {
+  if (!source) return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "QFunction source location required");
+  CeedQFunction q = new CeedQFunction_private();
+  q->ceed         = ceed;
+  ceed->refcount++;
+  q->vec_length  = vec_length;
+  q->user        = f;
+  q->source      = source;
+  size_t colon   = q->source.rfind(':');
+  q->name        = colon == std::string::npos ? q->source : q->source.substr(colon + 1);
+  qf_identify(q);
+  *qf = q;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedQFunctionCreateIdentity(Ceed ceed, CeedInt size, CeedEvalMode in_mode, CeedEvalMode out_mode, CeedQFunction *qf) {
+  CeedQFunction q = new CeedQFunction_private();
+  q->ceed         = ceed;
+  ceed->refcount++;
+  q->name = "Identity";
+  q->kind = QfKind::Identity;
+  q->inputs.push_back({"input", size, in_mode});
+  q->outputs.push_back({"output", size, out_mode});
+  *qf = q;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedQFunctionReferenceCopy(CeedQFunction qf, CeedQFunction *qf_copy) {
+  if (qf != CEED_QFUNCTION_NONE) qf->refcount++;
+  if (*qf_copy) MI_CALL(CeedQFunctionDestroy(qf_copy));
+  *qf_copy = qf;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedQFunctionAddInput(CeedQFunction qf, const char *field_name, CeedInt size, CeedEvalMode eval_mode) {
+  qf->inputs.push_back({field_name, size, eval_mode});
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionAddOutput(CeedQFunction qf, const char *field_name, CeedInt size, CeedEvalMode eval_mode) {
+  if (eval_mode == CEED_EVAL_WEIGHT) return MI_ERR(qf->ceed, CEED_ERROR_INCOMPATIBLE, "CEED_EVAL_WEIGHT cannot be an output");
+  qf->outputs.push_back({field_name, size, eval_mode});
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionSetContext(CeedQFunction qf, CeedQFunctionContext ctx) {
+  if (ctx) ctx->refcount++;
+  if (qf->ctx) CeedQFunctionContextDestroy(&qf->ctx);
+  qf->ctx = ctx;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionGetContext(CeedQFunction qf, CeedQFunctionContext *ctx) {
+  *ctx = qf->ctx;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionSetUserFlopsEstimate(CeedQFunction qf, CeedSize flops) {
+  qf->user_flops = flops;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionDestroy(CeedQFunction *qf) {
+  CeedQFunction q = *qf;
+  if (!q || q == CEED_QFUNCTION_NONE) {
+    *qf = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (--q->refcount > 0) {
+    *qf = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (q->ctx) CeedQFunctionContextDestroy(&q->ctx);
+  Ceed c = q->ceed;
+  delete q;
+  CeedDestroy(&c);
+  *qf = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// QFunction contexts
+// ---------------------------------------------------------------------------------------------------------------------
+extern "C" int CeedQFunctionContextCreate(Ceed ceed, CeedQFunctionContext *ctx) {
+  CeedQFunctionContext c = new CeedQFunctionContext_private();
+  c->ceed                = ceed;
+  ceed->refcount++;
+  *ctx = c;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextReference(CeedQFunctionContext ctx) {
+  ctx->refcount++;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextReferenceCopy(CeedQFunctionContext ctx, CeedQFunctionContext *ctx_copy) {
+  ctx->refcount++;
+  if (*ctx_copy) MI_CALL(CeedQFunctionContextDestroy(ctx_copy));
+  *ctx_copy = ctx;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextSetData(CeedQFunctionContext ctx, CeedMemType mem_type, CeedCopyMode copy_mode, size_t size, void *data) {
+  if (mem_type != CEED_MEM_HOST) return MI_ERR(ctx->ceed, CEED_ERROR_UNSUPPORTED, "context data must be host memory");
+  if (ctx->owned) free(ctx->data);
+  ctx->size = size;
+  if (copy_mode == CEED_COPY_VALUES) {
+    ctx->data = malloc(size);
+    memcpy(ctx->data, data, size);
+    ctx->owned = true;
+  } else {
+    ctx->data  = data;
+    ctx->owned = copy_mode == CEED_OWN_POINTER;
+  }
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextSetDataDestroy(CeedQFunctionContext ctx, CeedMemType f_mem_type, CeedQFunctionContextDataDestroyUser f) {
+  ctx->destroy = f;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextGetData(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
+  if (mem_type != CEED_MEM_HOST) return MI_ERR(ctx->ceed, CEED_ERROR_UNSUPPORTED, "context data lives in host memory");
+  *(void **)data = ctx->data;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextGetDataRead(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
+  return CeedQFunctionContextGetData(ctx, mem_type, data);
+}
+extern "C" int CeedQFunctionContextRestoreData(CeedQFunctionContext ctx, void *data) {
+  *(void **)data = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextRestoreDataRead(CeedQFunctionContext ctx, void *data) {
+  *(void **)data = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextGetContextSize(CeedQFunctionContext ctx, size_t *ctx_size) {
+  *ctx_size = ctx->size;
+  return CEED_ERROR_SUCCESS;
+}
+static int ctx_register(CeedQFunctionContext ctx, const char *name, size_t offset, size_t n, const char *desc, CeedContextFieldType type) {
+  for (auto &f : ctx->fields)
+    if (f.name == name) return MI_ERR(ctx->ceed, CEED_ERROR_MINOR, "context field '%s' already registered", name);
+  ctx->fields.push_back({name, desc ? desc : "", offset, n, type});
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedQFunctionContextRegisterDouble(CeedQFunctionContext ctx, const char *field_name, size_t field_offset, size_t num_values,
+                                                  const char *field_description) {
+  return ctx_register(ctx, field_name, field_offset, num_values, field_description, CEED_CONTEXT_FIELD_DOUBLE);
+}
+extern "C" int CeedQFunctionContextRegisterInt32(CeedQFunctionContext ctx, const char *field_name, size_t field_offset, size_t num_values,
+                                                 const char *field_description) {
+  return ctx_register(ctx, field_name, field_offset, num_values, field_description, CEED_CONTEXT_FIELD_INT32);
+}
+extern "C" int CeedQFunctionContextRegisterBoolean(CeedQFunctionContext ctx, const char *field_name, size_t field_offset, size_t num_values,
+                                                   const char *field_description) {
+  return ctx_register(ctx, field_name, field_offset, num_values, field_description, CEED_CONTEXT_FIELD_BOOL);
+}
+extern "C" int CeedQFunctionContextDestroy(CeedQFunctionContext *ctx) {
+  CeedQFunctionContext c = *ctx;
+  if (!c) return CEED_ERROR_SUCCESS;
+  if (--c->refcount > 0) {
+    *ctx = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (c->destroy && c->data) {
+    c->destroy(c->data);
+  } else if (c->owned) {
+    free(c->data);
+  }
+  Ceed cd = c->ceed;
+  delete c;
+  CeedDestroy(&cd);
+  *ctx = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Operators
+// ---------------------------------------------------------------------------------------------------------------------
+extern "C" int CeedOperatorCreate(Ceed ceed, CeedQFunction qf, CeedQFunction dqf, CeedQFunction dqfT, CeedOperator *op) {
+  if (!qf || qf == CEED_QFUNCTION_NONE) return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "operator needs a QFunction");
+  CeedOperator o = new CeedOperator_private();
+  o->ceed        = ceed;
+  ceed->refcount++;
+  o->qf = qf;
+  qf->refcount++;
+  for (auto &f : qf->inputs) {
+    auto *fld = new CeedOperatorField_private();
+    fld->name = f.name;
+    o->in_fields.push_back(fld);
+  }
+  for (auto &f : qf->outputs) {
+    auto *fld = new CeedOperatorField_private();
+    fld->name = f.name;
+    o->out_fields.push_back(fld);
+  }
+  *op = o;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorCreateComposite(Ceed ceed, CeedOperator *op) {
+  CeedOperator o = new CeedOperator_private();
+  o->ceed        = ceed;
+  ceed->refcount++;
+  o->composite = true;
+  *op          = o;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorCompositeAddSub(CeedOperator composite_op, CeedOperator sub_op) {
+  if (!composite_op->composite) return MI_ERR(composite_op->ceed, CEED_ERROR_MINOR, "not a composite operator");
+  sub_op->refcount++;
+  composite_op->subs.push_back(sub_op);
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorCompositeGetNumSub(CeedOperator op, CeedInt *num_sub) {
+  *num_sub = (CeedInt)op->subs.size();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorCompositeGetSubList(CeedOperator op, CeedOperator **sub_ops) {
+  if (!op->composite) return MI_ERR(op->ceed, CEED_ERROR_MINOR, "not a composite operator");
+  *sub_ops = op->subs.data();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorIsComposite(CeedOperator op, bool *is_composite) {
+  *is_composite = op->composite;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorReferenceCopy(CeedOperator op, CeedOperator *op_copy) {
+  op->refcount++;
+  if (*op_copy) MI_CALL(CeedOperatorDestroy(op_copy));
+  *op_copy = op;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorSetField(CeedOperator op, const char *field_name, CeedElemRestriction rstr, CeedBasis basis, CeedVector vec) {
+  if (op->composite) return MI_ERR(op->ceed, CEED_ERROR_INCOMPATIBLE, "cannot set fields on a composite operator");
+  CeedOperatorField_private *f = nullptr;
+  for (auto *fld : op->in_fields)
+    if (fld->name == field_name) f = fld;
+  for (auto *fld : op->out_fields)
+    if (fld->name == field_name) f = fld;
+  if (!f) return MI_ERR(op->ceed, CEED_ERROR_INCOMPLETE, "QFunction has no field named '%s'", field_name);
+  CeedElemRestrictionReferenceCopy(rstr, &f->rstr);
+  CeedBasisReferenceCopy(basis, &f->basis);
+  CeedVectorReferenceCopy(vec, &f->vec);
+  if (op->plan) {
+    op_plan_destroy(op->plan);
+    op->plan = nullptr;
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorGetFields(CeedOperator op, CeedInt *num_input_fields, CeedOperatorField **input_fields, CeedInt *num_output_fields,
+                                     CeedOperatorField **output_fields) {
+  if (op->composite) return MI_ERR(op->ceed, CEED_ERROR_MINOR, "composite operators have no fields");
+  if (num_input_fields) *num_input_fields = (CeedInt)op->in_fields.size();
+  if (input_fields) *input_fields = op->in_fields.data();
+  if (num_output_fields) *num_output_fields = (CeedInt)op->out_fields.size();
+  if (output_fields) *output_fields = op->out_fields.data();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorGetFieldByName(CeedOperator op, const char *field_name, CeedOperatorField *op_field) {
+  for (auto *f : op->in_fields)
+    if (f->name == field_name) {
+      *op_field = f;
+      return CEED_ERROR_SUCCESS;
+    }
+  for (auto *f : op->out_fields)
+    if (f->name == field_name) {
+      *op_field = f;
+      return CEED_ERROR_SUCCESS;
+    }
+  return MI_ERR(op->ceed, CEED_ERROR_MINOR, "operator has no field '%s'", field_name);
+}
+
+extern "C" int CeedOperatorFieldGetName(CeedOperatorField op_field, const char **field_name) {
+  *field_name = op_field->name.c_str();
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorFieldGetElemRestriction(CeedOperatorField op_field, CeedElemRestriction *rstr) {
+  *rstr = nullptr;
+  return CeedElemRestrictionReferenceCopy(op_field->rstr, rstr);
+}
+extern "C" int CeedOperatorFieldGetBasis(CeedOperatorField op_field, CeedBasis *basis) {
+  *basis = nullptr;
+  return CeedBasisReferenceCopy(op_field->basis, basis);
+}
+extern "C" int CeedOperatorFieldGetVector(CeedOperatorField op_field, CeedVector *vec) {
+  *vec = nullptr;
+  return CeedVectorReferenceCopy(op_field->vec, vec);
+}
+extern "C" int CeedOperatorFieldGetData(CeedOperatorField op_field, const char **field_name, CeedElemRestriction *rstr, CeedBasis *basis,
+                                        CeedVector *vec) {
+  if (field_name) *field_name = op_field->name.c_str();
+  if (rstr) MI_CALL(CeedOperatorFieldGetElemRestriction(op_field, rstr));
+  if (basis) MI_CALL(CeedOperatorFieldGetBasis(op_field, basis));
+  if (vec) MI_CALL(CeedOperatorFieldGetVector(op_field, vec));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorSetName(CeedOperator op, const char *name) {
+  op->name = name ? name : "";
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorGetName(CeedOperator op, const char **name) {
+  *name = op->name.c_str();
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorGetContext(CeedOperator op, CeedQFunctionContext *ctx) {
+  *ctx = nullptr;
+  if (op->composite) return MI_ERR(op->ceed, CEED_ERROR_INCOMPATIBLE, "composite operators have no single context");
+  if (op->qf->ctx) MI_CALL(CeedQFunctionContextReferenceCopy(op->qf->ctx, ctx));
+  return CEED_ERROR_SUCCESS;
+}
+
+static void collect_label_targets(CeedOperator op, const char *field_name, CeedContextFieldLabel_private *label) {
+  if (op->composite) {
+    for (CeedOperator s : op->subs) collect_label_targets(s, field_name, label);
+    return;
+  }
+  CeedQFunctionContext ctx = op->qf->ctx;
+  if (!ctx) return;
+  for (auto &f : ctx->fields) {
+    if (f.name == field_name) {
+      for (auto &t : label->targets)
+        if (t.first == ctx && t.second == f.offset) return;  // shared context already listed
+      label->targets.push_back({ctx, f.offset});
+      label->type       = f.type;
+      label->num_values = f.num_values;
+    }
+  }
+}
+
+extern "C" int CeedOperatorGetContextFieldLabel(CeedOperator op, const char *field_name, CeedContextFieldLabel *field_label) {
+  for (auto &l : op->labels)
+    if (l->name == field_name) {
+      *field_label = l.get();
+      return CEED_ERROR_SUCCESS;
+    }
+  auto label  = std::make_unique<CeedContextFieldLabel_private>();
+  label->name = field_name;
+  collect_label_targets(op, field_name, label.get());
+  if (label->targets.empty()) {
+    *field_label = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  *field_label = label.get();
+  op->labels.push_back(std::move(label));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorSetContextDouble(CeedOperator op, CeedContextFieldLabel field_label, double *values) {
+  if (!field_label) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "invalid context field label");
+  if (field_label->type != CEED_CONTEXT_FIELD_DOUBLE) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "context field is not double");
+  for (auto &t : field_label->targets) memcpy((char *)t.first->data + t.second, values, sizeof(double) * field_label->num_values);
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorGetContextDoubleRead(CeedOperator op, CeedContextFieldLabel field_label, size_t *num_values, const double **values) {
+  if (!field_label || field_label->targets.empty()) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "invalid context field label");
+  auto &t     = field_label->targets[0];
+  *values     = (const double *)((char *)t.first->data + t.second);
+  *num_values = field_label->num_values;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorRestoreContextDoubleRead(CeedOperator op, CeedContextFieldLabel field_label, const double **values) {
+  *values = nullptr;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int active_lengths(CeedOperator op, CeedSize *in, CeedSize *out) {
+  if (op->composite) {
+    *in = *out = -1;
+    for (CeedOperator s : op->subs) {
+      CeedSize i, o;
+      MI_CALL(active_lengths(s, &i, &o));
+      if (i >= 0) *in = i;
+      if (o >= 0) *out = o;
+    }
+    return 0;
+  }
+  *in = *out = -1;
+  for (auto *f : op->in_fields)
+    if (is_active(f->vec)) *in = f->rstr->strided ? f->rstr->l_size : f->rstr->l_size;
+  for (auto *f : op->out_fields)
+    if (is_active(f->vec)) *out = f->rstr->l_size;
+  return 0;
+}
+
+extern "C" int CeedOperatorGetActiveVectorLengths(CeedOperator op, CeedSize *input_size, CeedSize *output_size) {
+  CeedSize i, o;
+  MI_CALL(active_lengths(op, &i, &o));
+  if (input_size) *input_size = i;
+  if (output_size) *output_size = o;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int apply_add(CeedOperator op, CeedVector in, CeedVector out) {
+  if (op->composite) {
+    for (CeedOperator s : op->subs) MI_CALL(apply_add(s, in, out));
+    return 0;
+  }
+  return op_plan_apply(op, in, out);
+}
+
+extern "C" int CeedOperatorApplyAdd(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
+  return apply_add(op, in, out);
+}
+
+extern "C" int CeedOperatorApply(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
+  if (out && out != CEED_VECTOR_NONE) {
+    double *d;
+    MI_CALL(vec_device_write(out, &d, false));
+    MI_HIP(op->ceed, launch_fill(d, 0.0, out->length, op->ceed->stream));
+  }
+  return apply_add(op, in, out);
+}
+
+extern "C" int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  if (op->composite) {
+    for (CeedOperator s : op->subs) MI_CALL(CeedOperatorLinearAssembleAddDiagonal(s, assembled, request));
+    return CEED_ERROR_SUCCESS;
+  }
+  return op_plan_diagonal(op, assembled);
+}
+
+extern "C" int CeedOperatorLinearAssembleDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  double *d;
+  MI_CALL(vec_device_write(assembled, &d, false));
+  MI_HIP(op->ceed, launch_fill(d, 0.0, assembled->length, op->ceed->stream));
+  return CeedOperatorLinearAssembleAddDiagonal(op, assembled, request);
+}
+
+extern "C" int CeedOperatorLinearAssemblePointBlockDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly is not provided by the MI355X backend");
+}
+extern "C" int CeedOperatorLinearAssemblePointBlockDiagonalSymbolic(CeedOperator op, CeedSize *num_entries, CeedInt **rows, CeedInt **cols) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly is not provided by the MI355X backend");
+}
+extern "C" int CeedOperatorLinearAssembleSymbolic(CeedOperator op, CeedSize *num_entries, CeedInt **rows, CeedInt **cols) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "full matrix assembly is not provided by the MI355X backend");
+}
+extern "C" int CeedOperatorLinearAssemble(CeedOperator op, CeedVector values) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "full matrix assembly is not provided by the MI355X backend");
+}
+extern "C" int CeedOperatorLinearAssembleGetNumEntries(CeedOperator op, CeedSize *num_entries) {
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "full matrix assembly is not provided by the MI355X backend");
+}
+
+// Tensor-basis work per element plus the QFunction's user estimate (the backend does not count its own QFunctions).
+static CeedSize op_flops(CeedOperator op) {
+  if (op->composite) {
+    CeedSize f = 0;
+    for (CeedOperator s : op->subs) f += op_flops(s);
+    return f;
+  }
+  CeedSize f = 0, nelem = 0, Q3 = 0;
+  for (auto *fields : {&op->in_fields, &op->out_fields}) {
+    for (size_t i = 0; i < fields->size(); i++) {
+      auto *fld = (*fields)[i];
+      if (!fld->rstr || fld->rstr == CEED_ELEMRESTRICTION_NONE) continue;
+      nelem = fld->rstr->num_elem;
+      if (!fld->basis || fld->basis == CEED_BASIS_NONE || !fld->basis->tensor) continue;
+      const CeedSize P = fld->basis->P_1d, Q = fld->basis->Q_1d, nc = fld->rstr->num_comp;
+      Q3               = Q * Q * Q;
+      CeedSize interp  = 2 * nc * (P * P * Q * P + P * Q * Q * P + Q * Q * Q * P);
+      CeedSize grad    = 2 * nc * 3 * Q * Q * Q * Q;
+      f += nelem * (interp + grad);
+    }
+  }
+  return f + nelem * Q3 * op->qf->user_flops;
+}
+
+extern "C" int CeedOperatorGetFlopsEstimate(CeedOperator op, CeedSize *flops) {
+  *flops = op_flops(op);
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorSetQFunctionAssemblyDataUpdateNeeded(CeedOperator op, bool needs_data_update) { return CEED_ERROR_SUCCESS; }
+
+static void op_view(CeedOperator op, FILE *stream, int tabs, bool terse) {
+  std::string pad(2 * tabs, ' ');
+  if (op->composite) {
+    fprintf(stream, "%sComposite CeedOperator%s%s (%zu sub-operators)\n", pad.c_str(), op->name.empty() ? "" : " - ", op->name.c_str(), op->subs.size());
+    for (CeedOperator s : op->subs) op_view(s, stream, tabs + 1, terse);
+    return;
+  }
+  fprintf(stream, "%sCeedOperator%s%s, QFunction %s\n", pad.c_str(), op->name.empty() ? "" : " - ", op->name.c_str(), op->qf->name.c_str());
+  if (terse) return;
+  for (auto *f : op->in_fields) fprintf(stream, "%s  input  \"%s\"%s\n", pad.c_str(), f->name.c_str(), is_active(f->vec) ? " (active)" : "");
+  for (auto *f : op->out_fields) fprintf(stream, "%s  output \"%s\"%s\n", pad.c_str(), f->name.c_str(), is_active(f->vec) ? " (active)" : "");
+}
+
+extern "C" int CeedOperatorView(CeedOperator op, FILE *stream) {
+  op_view(op, stream, op->num_tabs, false);
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorViewTerse(CeedOperator op, FILE *stream) {
+  op_view(op, stream, op->num_tabs, true);
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorGetNumViewTabs(CeedOperator op, CeedInt *num_tabs) {
+  *num_tabs = op->num_tabs;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorSetNumViewTabs(CeedOperator op, CeedInt num_tabs) {
+  op->num_tabs = num_tabs;
+  return CEED_ERROR_SUCCESS;
+}
+extern "C" int CeedOperatorGetCeed(CeedOperator op, Ceed *ceed) {
+  *ceed = nullptr;
+  return CeedReferenceCopy(op->ceed, ceed);
+}
+extern "C" Ceed CeedOperatorReturnCeed(CeedOperator op) { return op->ceed; }
+
+extern "C" int CeedOperatorDestroy(CeedOperator *op) {
+  CeedOperator o = *op;
+  if (!o) return CEED_ERROR_SUCCESS;
+  if (--o->refcount > 0) {
+    *op = nullptr;
+    return CEED_ERROR_SUCCESS;
+  }
+  if (o->plan) op_plan_destroy(o->plan);
+  for (auto *fields : {&o->in_fields, &o->out_fields}) {
+    for (auto *f : *fields) {
+      CeedElemRestrictionDestroy(&f->rstr);
+      CeedBasisDestroy(&f->basis);
+      CeedVectorDestroy(&f->vec);
+      delete f;
+    }
+  }
+  for (CeedOperator s : o->subs) CeedOperatorDestroy(&s);
+  if (o->qf) CeedQFunctionDestroy(&o->qf);
+  Ceed c = o->ceed;
+  delete o;
+  CeedDestroy(&c);
+  *op = nullptr;
+  return CEED_ERROR_SUCCESS;
+}

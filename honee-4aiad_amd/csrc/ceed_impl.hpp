@@ -1,0 +1,168 @@
+// Internal object model of the MI355X libCEED-compatible library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ceed.h"
+#include "honee_mi355x.h"
+#include "kernels.hpp"
+
+struct Ceed_private {
+  int              refcount = 1;
+  std::string      resource;
+  int              device   = 0;
+  hipStream_t      stream   = nullptr;  // null stream unless CeedSetStream
+  bool             jac_fp32 = false;
+  bool             colored  = false;
+  CeedErrorHandler handler  = nullptr;
+  std::string      err_msg;
+  // kernel timing
+  bool                                     timing = false;
+  std::map<std::string, std::pair<int, double>> kernel_times;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events;
+  std::vector<hipEvent_t>                  event_pool;
+  double                                  *d_scratch = nullptr;  // reduction scratch (norms)
+  double                                  *h_scratch = nullptr;
+  int                                      scratch_len = 0;
+};
+
+struct CeedVector_private {
+  Ceed     ceed;
+  int      refcount = 1;
+  CeedSize length   = 0;
+  double  *h_array  = nullptr;
+  double  *d_array  = nullptr;
+  bool     h_owned  = false, d_owned = false;
+  bool     h_valid  = false, d_valid = false;
+  int      num_readers = 0;
+  bool     writing     = false;
+  float   *d_shadow32  = nullptr;  // FP32 mirror (backend-internal; jac_data in FP32 Jacobian mode)
+  CeedSize shadow_len  = 0;
+};
+
+struct CeedElemRestriction_private {
+  Ceed                 ceed;
+  int                  refcount = 1;
+  CeedInt              num_elem = 0, elem_size = 0, num_comp = 0, comp_stride = 1;
+  CeedSize             l_size   = 0;
+  bool                 strided  = false;
+  bool                 backend_strides = false;
+  CeedInt              strides[3] = {0, 0, 0};
+  std::vector<CeedInt> h_offsets;
+  int                 *d_offsets = nullptr;
+};
+
+struct CeedBasis_private {
+  Ceed                ceed;
+  int                 refcount = 1;
+  bool                tensor   = true;
+  CeedInt             dim = 0, num_comp = 0, P_1d = 0, Q_1d = 0, num_nodes = 0, num_qpts = 0;
+  CeedElemTopology    topo = CEED_TOPOLOGY_HEX;
+  std::vector<double> interp_1d, grad_1d, q_ref_1d, q_weight_1d, collo_grad_1d;
+  std::vector<double> interp, grad, q_ref, q_weight;  // non-tensor bases
+  double             *d_interp_1d = nullptr, *d_grad_1d = nullptr, *d_collo = nullptr, *d_weight_1d = nullptr;
+  bool                is_projection = false;
+};
+
+struct QFField {
+  std::string  name;
+  CeedInt      size;
+  CeedEvalMode emode;
+};
+
+enum class QfKind {
+  Unknown,
+  Identity,
+  Setup,
+  IFunction,
+  IJacobian,
+};
+
+struct CeedQFunction_private {
+  Ceed                 ceed;
+  int                  refcount = 1;
+  CeedInt              vec_length = 1;
+  CeedQFunctionUser    user       = nullptr;
+  std::string          source, name;
+  std::vector<QFField> inputs, outputs;
+  CeedQFunctionContext ctx = nullptr;
+  CeedSize             user_flops = 0;
+  QfKind               kind       = QfKind::Unknown;
+  int                  state_var  = 0;
+};
+
+struct CtxField {
+  std::string          name, description;
+  size_t               offset, num_values;
+  CeedContextFieldType type;
+};
+
+struct CeedContextFieldLabel_private {
+  std::string                       name;
+  CeedContextFieldType              type;
+  size_t                            num_values;
+  std::vector<std::pair<CeedQFunctionContext, size_t>> targets;  // (context, byte offset)
+};
+
+struct CeedQFunctionContext_private {
+  Ceed                                  ceed;
+  int                                   refcount = 1;
+  size_t                                size     = 0;
+  void                                 *data     = nullptr;
+  bool                                  owned    = false;
+  CeedQFunctionContextDataDestroyUser   destroy  = nullptr;
+  std::vector<CtxField>                 fields;
+  std::vector<std::unique_ptr<CeedContextFieldLabel_private>> labels;
+};
+
+struct CeedOperatorField_private {
+  std::string         name;
+  CeedElemRestriction rstr  = nullptr;
+  CeedBasis           basis = nullptr;
+  CeedVector          vec   = nullptr;
+};
+
+struct OpPlan;  // backend binding of a non-composite operator
+
+struct CeedOperator_private {
+  Ceed                                     ceed;
+  int                                      refcount = 1;
+  bool                                     composite = false;
+  CeedQFunction                            qf = nullptr;
+  std::vector<CeedOperatorField_private *> in_fields, out_fields;
+  std::vector<CeedOperator>                subs;
+  std::string                              name;
+  CeedInt                                  num_tabs = 0;
+  std::vector<std::unique_ptr<CeedContextFieldLabel_private>> labels;
+  OpPlan                                  *plan = nullptr;
+};
+
+// ---- helpers shared by the translation units ----
+int  ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int ecode, const char *fmt, ...);
+#define MI_ERR(ceed, code, ...) ceed_set_error((ceed), __FILE__, __LINE__, __func__, (code), __VA_ARGS__)
+#define MI_CALL(expr)                \
+  do {                               \
+    int ierr_ = (expr);              \
+    if (ierr_) return ierr_;         \
+  } while (0)
+#define MI_HIP(ceed, expr)                                                                                              \
+  do {                                                                                                                 \
+    hipError_t herr_ = (expr);                                                                                         \
+    if (herr_ != hipSuccess) return MI_ERR((ceed), CEED_ERROR_BACKEND, "HIP error %s: %s", hipGetErrorName(herr_), #expr); \
+  } while (0)
+
+int vec_device_read(CeedVector v, const double **d);
+int vec_device_write(CeedVector v, double **d, bool keep_contents);
+int vec_device_shadow32(CeedVector v, float **d, bool create);
+int rstr_device_offsets(CeedElemRestriction r, const int **d);
+int basis_device_tables(CeedBasis b);
+int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out);
+int op_plan_diagonal(CeedOperator op, CeedVector diag);
+void op_plan_destroy(OpPlan *plan);
+int timing_begin(Ceed ceed, const char *name, hipEvent_t *start);
+int timing_end(Ceed ceed, const char *name, hipEvent_t start);

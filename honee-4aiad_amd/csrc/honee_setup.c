@@ -1,0 +1,289 @@
+// PETSc-free restatement of HONEE's operator wiring for the Newtonian ideal-gas problem, written against the public
+// libCEED API only (it links to libceed.so like HONEE does).  It builds exactly the fields, composites and context
+// registrations of SetupLibceed (src/setuplibceed.c:97-318), QDataGet (src/qdata.c:133-190) and NS_NEWTONIAN_IG's
+// context setup (problems/newtonian.c:723-757), with the mesh, basis tabulation and offsets supplied by the caller in
+// place of DMPlex/PetscFE.  Vectors are handed to the operators with CeedVectorSetArray/TakeArray around every
+// apply, as VecReadPetscToCeed/VecReadCeedToPetsc do (include/petsc-ceed-utils.h:127-160).
+#include "honee_setup.h"
+
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+// QFunction bodies stay on the host side of HONEE; the MI355X backend identifies them by location string and runs its
+// fused kernels instead, so the host bodies here only report that they are not executed.
+static int HoneeMiQFunctionNotOnHost(void *ctx, const CeedInt Q, const CeedScalar *const *in, CeedScalar *const *out) {
+  return CEED_ERROR_UNSUPPORTED;
+}
+
+static const char *const kIFunctionLoc[3] = {"qfunctions/newtonian.h:IFunction_Newtonian_Conserv", "qfunctions/newtonian.h:IFunction_Newtonian_Prim",
+                                             "qfunctions/newtonian.h:IFunction_Newtonian_Entropy"};
+static const char *const kIJacobianLoc[3] = {"qfunctions/newtonian.h:IJacobian_Newtonian_Conserv", "qfunctions/newtonian.h:IJacobian_Newtonian_Prim",
+                                             "qfunctions/newtonian.h:IJacobian_Newtonian_Entropy"};
+static const char *const kSetupLoc        = "qfunctions/setupgeo.h:Setup";
+
+struct HoneeMiProblem_ {
+  Ceed                  ceed;
+  CeedElemRestriction   elem_restr_q, elem_restr_x, elem_restr_qd, elem_restr_jd;
+  CeedBasis             basis_q, basis_x;
+  CeedVector            q_ceed, q_dot_ceed, g_ceed, x_coord, q_data, jac_data;
+  CeedOperator          op_ifunction, op_ijacobian;
+  CeedContextFieldLabel solution_time_label, timestep_size_label, ijacobian_shift_label;
+  CeedQFunctionContext  newt_qfctx;
+  CeedMiNewtonianContext *ctx;
+  int                   nelem, Q3, nnodes, num_comps_jac_data;
+  char                  err[512];
+};
+
+#define CHK(p, expr)                                                                                         \
+  do {                                                                                                       \
+    int ierr_ = (expr);                                                                                      \
+    if (ierr_) {                                                                                             \
+      const char *msg_ = "";                                                                                 \
+      if ((p)->ceed) CeedGetErrorMessage((p)->ceed, &msg_);                                                  \
+      snprintf((p)->err, sizeof((p)->err), "%s failed (%d): %s", #expr, ierr_, msg_);                       \
+      return ierr_;                                                                                          \
+    }                                                                                                        \
+  } while (0)
+
+static char g_last_error[512];
+
+const char *HoneeMiLastError(void) { return g_last_error; }
+
+static int FreeContext(void *data) {
+  free(data);
+  return 0;
+}
+
+static int Setup(HoneeMiProblem p, const char *resource, const HoneeMiMesh *m, const CeedMiNewtonianContext *ctx_in, int state_var) {
+  const int P = m->P, Q = m->Q, P3 = P * P * P;
+  p->nelem  = m->nelem;
+  p->Q3     = Q * Q * Q;
+  p->nnodes = m->nnodes;
+  CHK(p, CeedInit(resource, &p->ceed));
+  CHK(p, CeedSetErrorHandler(p->ceed, CeedErrorStore));
+
+  // Bases: solution (5 comps) and trilinear coordinates (3 comps), DMPlexCeedBasisCreate (src/dm-utils.c:844-850)
+  CHK(p, CeedBasisCreateTensorH1(p->ceed, 3, 5, P, Q, m->B, m->D, m->qref, m->qweight, &p->basis_q));
+  CHK(p, CeedBasisCreateTensorH1(p->ceed, 3, 3, 2, Q, m->Bx, m->Dx, m->qref, m->qweight, &p->basis_x));
+
+  // Restrictions with L-vector offsets of component 0 (src/dm-utils.c:108-112)
+  {
+    CeedInt *off = malloc(sizeof(CeedInt) * (size_t)m->nelem * P3);
+    for (size_t i = 0; i < (size_t)m->nelem * P3; i++) off[i] = 5 * m->offsets[i];
+    CHK(p, CeedElemRestrictionCreate(p->ceed, m->nelem, P3, 5, 1, 5 * (CeedSize)m->nnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off, &p->elem_restr_q));
+    for (size_t i = 0; i < (size_t)m->nelem * 8; i++) off[i] = 3 * m->xoffsets[i];
+    CHK(p, CeedElemRestrictionCreate(p->ceed, m->nelem, 8, 3, 1, 3 * (CeedSize)m->nxnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off, &p->elem_restr_x));
+    free(off);
+  }
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_x, &p->x_coord, NULL));
+  CHK(p, CeedVectorSetArray(p->x_coord, CEED_MEM_HOST, CEED_COPY_VALUES, (CeedScalar *)m->xcoord));
+
+  // QDataGet: Setup QFunction applied to the coordinates (src/qdata.c:133-190)
+  {
+    CeedQFunction qf_setup;
+    CeedOperator  op_setup;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kSetupLoc, &qf_setup));
+    CHK(p, CeedQFunctionSetUserFlopsEstimate(qf_setup, 0));
+    CHK(p, CeedQFunctionAddInput(qf_setup, "dx", 9, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf_setup, "weight", 1, CEED_EVAL_WEIGHT));
+    CHK(p, CeedQFunctionAddOutput(qf_setup, "surface qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedElemRestrictionCreateStrided(p->ceed, m->nelem, p->Q3, 10, (CeedSize)m->nelem * p->Q3 * 10, CEED_STRIDES_BACKEND, &p->elem_restr_qd));
+    CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_qd, &p->q_data, NULL));
+    CHK(p, CeedOperatorCreate(p->ceed, qf_setup, NULL, NULL, &op_setup));
+    CHK(p, CeedOperatorSetField(op_setup, "dx", p->elem_restr_x, p->basis_x, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_setup, "weight", CEED_ELEMRESTRICTION_NONE, p->basis_x, CEED_VECTOR_NONE));
+    CHK(p, CeedOperatorSetField(op_setup, "surface qdata", p->elem_restr_qd, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorApply(op_setup, p->x_coord, p->q_data, CEED_REQUEST_IMMEDIATE));
+    CHK(p, CeedOperatorDestroy(&op_setup));
+    CHK(p, CeedQFunctionDestroy(&qf_setup));
+  }
+
+  // Newtonian QFunction context with the registered fields of problems/newtonian.c:744-753
+  p->ctx  = malloc(sizeof(*p->ctx));
+  *p->ctx = *ctx_in;
+  CHK(p, CeedQFunctionContextCreate(p->ceed, &p->newt_qfctx));
+  CHK(p, CeedQFunctionContextSetData(p->newt_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, sizeof(*p->ctx), p->ctx));
+  CHK(p, CeedQFunctionContextSetDataDestroy(p->newt_qfctx, CEED_MEM_HOST, FreeContext));
+  CHK(p, CeedQFunctionContextRegisterDouble(p->newt_qfctx, "timestep size", offsetof(CeedMiNewtonianContext, dt), 1, "Size of timestep, delta t"));
+  CHK(p, CeedQFunctionContextRegisterDouble(p->newt_qfctx, "ijacobian time shift", offsetof(CeedMiNewtonianContext, ijacobian_time_shift), 1,
+                                            "Shift for mass matrix in IJacobian"));
+  CHK(p, CeedQFunctionContextRegisterDouble(p->newt_qfctx, "solution time", offsetof(CeedMiNewtonianContext, time), 1, "Current solution time"));
+
+  p->num_comps_jac_data = 14 + (ctx_in->idl_enable ? 1 : 0);  // problems/newtonian.c:756-757
+  CHK(p, CeedElemRestrictionCreateStrided(p->ceed, m->nelem, p->Q3, p->num_comps_jac_data, (CeedSize)m->nelem * p->Q3 * p->num_comps_jac_data,
+                                          CEED_STRIDES_BACKEND, &p->elem_restr_jd));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_jd, &p->jac_data, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_q, &p->q_ceed, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_q, &p->q_dot_ceed, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_q, &p->g_ceed, NULL));
+
+  CeedOperator op_ifunction_vol, op_ijacobian_vol;
+  {  // IFunction volume operator (src/setuplibceed.c:209-243)
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kIFunctionLoc[state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+    CHK(p, CeedQFunctionSetUserFlopsEstimate(qf, 0));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "q dot", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddOutput(qf, "jac_data", p->num_comps_jac_data, CEED_EVAL_NONE));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op_ifunction_vol));
+    CHK(p, CeedOperatorSetName(op_ifunction_vol, "IFunction Domain Operator"));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "Grad_q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "q dot", p->elem_restr_q, p->basis_q, p->q_dot_ceed));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "Grad_v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ifunction_vol, "jac_data", p->elem_restr_jd, CEED_BASIS_NONE, p->jac_data));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  {  // IJacobian volume operator (src/setuplibceed.c:245-269)
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kIJacobianLoc[state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+    CHK(p, CeedQFunctionSetUserFlopsEstimate(qf, 0));
+    CHK(p, CeedQFunctionAddInput(qf, "dq", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_dq", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "jac_data", p->num_comps_jac_data, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op_ijacobian_vol));
+    CHK(p, CeedOperatorSetName(op_ijacobian_vol, "IJacobian Domain Operator"));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "dq", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "Grad_dq", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "jac_data", p->elem_restr_jd, CEED_BASIS_NONE, p->jac_data));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_ijacobian_vol, "Grad_v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  // Composites (src/setuplibceed.c:291-306)
+  CHK(p, CeedOperatorCreateComposite(p->ceed, &p->op_ifunction));
+  CHK(p, CeedOperatorSetName(p->op_ifunction, "IFunction Composite Operator"));
+  CHK(p, CeedOperatorCompositeAddSub(p->op_ifunction, op_ifunction_vol));
+  CHK(p, CeedOperatorCreateComposite(p->ceed, &p->op_ijacobian));
+  CHK(p, CeedOperatorSetName(p->op_ijacobian, "IJacobian Composite Operator"));
+  CHK(p, CeedOperatorCompositeAddSub(p->op_ijacobian, op_ijacobian_vol));
+  CHK(p, CeedOperatorGetContextFieldLabel(p->op_ifunction, "solution time", &p->solution_time_label));
+  CHK(p, CeedOperatorGetContextFieldLabel(p->op_ifunction, "timestep size", &p->timestep_size_label));
+  CHK(p, CeedOperatorGetContextFieldLabel(p->op_ijacobian, "ijacobian time shift", &p->ijacobian_shift_label));
+  CHK(p, CeedOperatorDestroy(&op_ifunction_vol));
+  CHK(p, CeedOperatorDestroy(&op_ijacobian_vol));
+  return 0;
+}
+
+int HoneeMiProblemCreate(const char *resource, const HoneeMiMesh *mesh, const CeedMiNewtonianContext *ctx, int state_var, HoneeMiProblem *problem) {
+  HoneeMiProblem p = calloc(1, sizeof(*p));
+  if (state_var < 0 || state_var > 2) {
+    snprintf(g_last_error, sizeof(g_last_error), "invalid state variable %d", state_var);
+    free(p);
+    return CEED_ERROR_MAJOR;
+  }
+  int ierr = Setup(p, resource, mesh, ctx, state_var);
+  if (ierr) {
+    snprintf(g_last_error, sizeof(g_last_error), "%s", p->err);
+    HoneeMiProblemDestroy(&p);
+    return ierr;
+  }
+  *problem = p;
+  return 0;
+}
+
+Ceed HoneeMiProblemGetCeed(HoneeMiProblem p) { return p->ceed; }
+
+CeedOperator HoneeMiProblemGetOperator(HoneeMiProblem p, int which) { return which == 0 ? p->op_ifunction : p->op_ijacobian; }
+
+// IFunction_NS's operator section (src/setupts.c:90-111): update context, hand over arrays, apply, take arrays back.
+int HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g) {
+  CeedScalar *arr;
+  if (p->solution_time_label) CHK(p, CeedOperatorSetContextDouble(p->op_ifunction, p->solution_time_label, &time));
+  if (p->timestep_size_label) CHK(p, CeedOperatorSetContextDouble(p->op_ifunction, p->timestep_size_label, &dt));
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q_dot));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, g));
+  CHK(p, CeedOperatorApply(p->op_ifunction, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+// MatMult_Ceed's operator section (src/mat-ceed.c:1643-1694) with FormIJacobian_NS's shift (src/setupts.c:141)
+int HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const double *dq, double *v) {
+  CeedScalar *arr;
+  if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)dq));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, v));
+  CHK(p, CeedOperatorApply(p->op_ijacobian, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+// Copy backend-strided quadrature data to host as [elem][comp][qpt] (for checking against the CPU oracle)
+static int CopyStrided(HoneeMiProblem p, CeedVector v, CeedElemRestriction r, int ncomp, double *out) {
+  const CeedScalar *a;
+  CHK(p, CeedVectorGetArrayRead(v, CEED_MEM_HOST, &a));
+  const long long Q3 = p->Q3;
+  for (long long e = 0; e < p->nelem; e++)
+    for (int c = 0; c < ncomp; c++)
+      for (long long i = 0; i < Q3; i++) out[(e * ncomp + c) * Q3 + i] = a[c * (long long)p->nelem * Q3 + e * Q3 + i];
+  CHK(p, CeedVectorRestoreArrayRead(v, &a));
+  return 0;
+}
+
+int HoneeMiGetQData(HoneeMiProblem p, double *out) { return CopyStrided(p, p->q_data, p->elem_restr_qd, 10, out); }
+int HoneeMiGetJacData(HoneeMiProblem p, double *out) { return CopyStrided(p, p->jac_data, p->elem_restr_jd, p->num_comps_jac_data, out); }
+
+int HoneeMiGetJacData32(HoneeMiProblem p, float *out) {
+  const float *d;
+  CeedSize     n;
+  CHK(p, CeedMiVectorGetFp32ShadowRead(p->jac_data, &d, &n));
+  float *h = malloc(sizeof(float) * n);
+  if (hipMemcpy(h, d, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+    free(h);
+    return CEED_ERROR_BACKEND;
+  }
+  const long long Q3 = p->Q3;
+  for (long long e = 0; e < p->nelem; e++)
+    for (int c = 0; c < p->num_comps_jac_data; c++)
+      for (long long i = 0; i < Q3; i++) out[(e * p->num_comps_jac_data + c) * Q3 + i] = h[c * (long long)p->nelem * Q3 + e * Q3 + i];
+  free(h);
+  return 0;
+}
+
+const char *HoneeMiProblemError(HoneeMiProblem p) { return p->err; }
+
+int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
+  HoneeMiProblem p = *problem;
+  if (!p) return 0;
+  CeedOperatorDestroy(&p->op_ifunction);
+  CeedOperatorDestroy(&p->op_ijacobian);
+  CeedVectorDestroy(&p->q_ceed);
+  CeedVectorDestroy(&p->q_dot_ceed);
+  CeedVectorDestroy(&p->g_ceed);
+  CeedVectorDestroy(&p->x_coord);
+  CeedVectorDestroy(&p->q_data);
+  CeedVectorDestroy(&p->jac_data);
+  CeedElemRestrictionDestroy(&p->elem_restr_q);
+  CeedElemRestrictionDestroy(&p->elem_restr_x);
+  CeedElemRestrictionDestroy(&p->elem_restr_qd);
+  CeedElemRestrictionDestroy(&p->elem_restr_jd);
+  CeedBasisDestroy(&p->basis_q);
+  CeedBasisDestroy(&p->basis_x);
+  if (p->newt_qfctx) CeedQFunctionContextDestroy(&p->newt_qfctx);
+  else free(p->ctx);
+  CeedDestroy(&p->ceed);
+  free(p);
+  *problem = NULL;
+  return 0;
+}

@@ -1,0 +1,63 @@
+// Launch interfaces of the fused MI355X kernels (host side of the backend calls these).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "honee_mi355x.h"
+
+// Strided quadrature-data layout: value (elem e, component c, point i) at i*sn + c*sc + e*se.
+struct QDataLayout {
+  long long sn, sc, se;
+};
+
+// Device copies of the 1D tables of a tensor basis: interpolation B [Q][P] and collocated derivative Dc [Q][Q].
+struct BasisTables {
+  const double *B;
+  const double *Dc;
+};
+
+struct SetupArgs {
+  int           nelem;
+  const int    *xoffsets;  // [nelem][Px^3], L index of component 0
+  int           xcomp_stride;
+  const double *x;
+  const double *Bx, *Dx;  // [Q][Px]
+  const double *w1d;      // [Q]
+  double       *qdata;
+  QDataLayout   qd;
+};
+
+struct VolumeArgs {
+  int           nelem;
+  const int    *offsets;  // [nelem][P^3], L index of component 0
+  int           comp_stride;
+  const double *u;        // active input L-vector (q or dq)
+  const double *u2;       // second L-vector input (q_dot), if any
+  double       *v;        // output L-vector (summed into)
+  const double *qdata;
+  QDataLayout   qd;
+  double       *jac;      // FP64 jac_data written by the IFunction
+  const double *jacr;     // FP64 jac_data read by the IJacobian
+  float        *jac32;    // FP32 jac_data mirror (written by IFunction / read by IJacobian in FP32 mode)
+  QDataLayout   jd;
+  BasisTables   bt;
+  const int    *xoffsets;  // coordinate restriction [nelem][8]
+  const double *x;
+  const double *Bx;        // [Q][2]
+};
+
+hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream);
+bool       volume_pq_supported(int P, int Q);
+hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+
+// Vector kernels (kernels_vector.hip)
+hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);
+hipError_t launch_axpby(double *y, double alpha, double beta, const double *x, long long n, hipStream_t s);
+hipError_t launch_scale(double *x, double alpha, long long n, hipStream_t s);
+hipError_t launch_pointwise_mult(double *w, const double *x, const double *y, long long n, hipStream_t s);
+hipError_t launch_reciprocal(double *x, long long n, hipStream_t s);
+hipError_t launch_norm(const double *x, long long n, int type, double *d_partial, int npartial, hipStream_t s);
+hipError_t launch_multiplicity(const int *offsets, long long n_offsets, int ncomp, int comp_stride, double *mult, hipStream_t s);
+hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *u, double *e, hipStream_t s);
+hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u, hipStream_t s);

@@ -1,0 +1,124 @@
+// Vector and restriction kernels of the MI355X backend (CeedVector arithmetic, norms, multiplicity, restriction apply).
+// All are bandwidth-bound streaming kernels with grid-stride loops.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int grid_for(long long n) {
+  long long g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+__global__ void k_fill(double *x, double v, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) x[i] = v;
+}
+__global__ void k_axpby(double *y, double a, double b, const double *x, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_scale(double *x, double a, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) x[i] *= a;
+}
+__global__ void k_pmult(double *w, const double *x, const double *y, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) w[i] = x[i] * y[i];
+}
+__global__ void k_recip(double *x, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    if (fabs(x[i]) > 1e-300) x[i] = 1.0 / x[i];
+}
+
+// Per-block partial: 0 -> sum|x|, 1 -> sum x^2, 2 -> max|x|; npartial blocks, results in partial[block]
+__global__ void k_norm(const double *x, long long n, int type, double *partial) {
+  __shared__ double red[kBlock];
+  double            acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = x[i];
+    if (type == 0) acc += fabs(v);
+    else if (type == 1) acc += v * v;
+    else acc = fmax(acc, fabs(v));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = (type == 2) ? fmax(red[threadIdx.x], red[threadIdx.x + s]) : red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_multiplicity(const int *off, long long n, int ncomp, int cs, double *mult) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < ncomp; c++) atomicAdd(&mult[off[i] + c * cs], 1.0);
+}
+
+// E-vector layout [elem][comp][node] (libCEED's offset-restriction E-vector ordering)
+__global__ void k_rstr_n(const int *off, int nelem, int esize, int ncomp, int cs, const double *u, double *e) {
+  const long long n = (long long)nelem * esize;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long el = i / esize, node = i % esize;
+    for (int c = 0; c < ncomp; c++) e[(el * ncomp + c) * esize + node] = u[off[i] + c * cs];
+  }
+}
+__global__ void k_rstr_t(const int *off, int nelem, int esize, int ncomp, int cs, const double *e, double *u) {
+  const long long n = (long long)nelem * esize;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long el = i / esize, node = i % esize;
+    for (int c = 0; c < ncomp; c++) atomicAdd(&u[off[i] + c * cs], e[(el * ncomp + c) * esize + node]);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_fill(double *x, double value, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, x, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_axpby(double *y, double alpha, double beta, const double *x, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, y, alpha, beta, x, n);
+  return hipGetLastError();
+}
+hipError_t launch_scale(double *x, double alpha, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBlock), 0, s, x, alpha, n);
+  return hipGetLastError();
+}
+hipError_t launch_pointwise_mult(double *w, const double *x, const double *y, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pmult, dim3(grid_for(n)), dim3(kBlock), 0, s, w, x, y, n);
+  return hipGetLastError();
+}
+hipError_t launch_reciprocal(double *x, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n);
+  return hipGetLastError();
+}
+hipError_t launch_norm(const double *x, long long n, int type, double *d_partial, int npartial, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm, dim3(npartial), dim3(kBlock), 0, s, x, n, type, d_partial);
+  return hipGetLastError();
+}
+hipError_t launch_multiplicity(const int *offsets, long long n_offsets, int ncomp, int comp_stride, double *mult, hipStream_t s) {
+  if (n_offsets <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_multiplicity, dim3(grid_for(n_offsets)), dim3(kBlock), 0, s, offsets, n_offsets, ncomp, comp_stride, mult);
+  return hipGetLastError();
+}
+hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *u, double *e,
+                                          hipStream_t s) {
+  const long long n = (long long)nelem * elemsize;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rstr_n, dim3(grid_for(n)), dim3(kBlock), 0, s, offsets, nelem, elemsize, ncomp, comp_stride, u, e);
+  return hipGetLastError();
+}
+hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u,
+                                        hipStream_t s) {
+  const long long n = (long long)nelem * elemsize;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rstr_t, dim3(grid_for(n)), dim3(kBlock), 0, s, offsets, nelem, elemsize, ncomp, comp_stride, e, u);
+  return hipGetLastError();
+}

@@ -1,0 +1,219 @@
+"""ctypes binding of libhonee_setup.so / libceed.so (the MI355X backend).
+
+Loading fails loudly when the native libraries are missing: there is no Python or CPU fallback for the operators.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import LIB_DIR
+from .mesh import BoxMesh, Tabulation
+
+_dp, _ip, _fp = ct.POINTER(ct.c_double), ct.POINTER(ct.c_int), ct.POINTER(ct.c_float)
+
+CEED_MEM_HOST, CEED_MEM_DEVICE = 0, 1
+VAR_CONS, VAR_PRIM, VAR_ENTROPY = 0, 1, 2
+STAB_NONE, STAB_SU, STAB_SUPG = 0, 1, 2
+
+
+class Gas(ct.Structure):
+    _fields_ = [("lambda_", ct.c_double), ("mu", ct.c_double), ("k", ct.c_double), ("cv", ct.c_double), ("cp", ct.c_double)]
+
+
+class TauCoeffs(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in ("Ctau_t", "Ctau_v", "Ctau_C", "Ctau_M", "Ctau_E")]
+
+
+class NewtonianContext(ct.Structure):
+    """Binary layout of struct NewtonianIdealGasContext_ (qfunctions/newtonian_types.h:29-47)."""
+
+    _fields_ = [
+        ("gas", Gas),
+        ("g", ct.c_double * 3),
+        ("time", ct.c_double),
+        ("ijacobian_time_shift", ct.c_double),
+        ("is_implicit", ct.c_bool),
+        ("state_var", ct.c_int),
+        ("stabilization", ct.c_int),
+        ("idl_enable", ct.c_bool),
+        ("idl_pressure", ct.c_double),
+        ("idl_amplitude", ct.c_double),
+        ("idl_start", ct.c_double),
+        ("idl_length", ct.c_double),
+        ("tau_coeffs", TauCoeffs),
+        ("dt", ct.c_double),
+        ("divFdiff_method", ct.c_int),
+    ]
+
+
+class HoneeMiMesh(ct.Structure):
+    _fields_ = [
+        ("nelem", ct.c_int),
+        ("P", ct.c_int),
+        ("Q", ct.c_int),
+        ("nnodes", ct.c_int),
+        ("nxnodes", ct.c_int),
+        ("offsets", _ip),
+        ("xoffsets", _ip),
+        ("xcoord", _dp),
+        ("B", _dp),
+        ("D", _dp),
+        ("Bx", _dp),
+        ("Dx", _dp),
+        ("qref", _dp),
+        ("qweight", _dp),
+    ]
+
+
+def context_from_dict(d: dict) -> NewtonianContext:
+    c = NewtonianContext()
+    g = d["gas"]
+    c.gas = Gas(g["lambda"], g["mu"], g["k"], g["cv"], g["cp"])
+    for i in range(3):
+        c.g[i] = d.get("g", (0.0, 0.0, 0.0))[i]
+    c.time = d.get("time", 0.0)
+    c.ijacobian_time_shift = d.get("ijacobian_time_shift", 0.0)
+    c.is_implicit = bool(d.get("is_implicit", True))
+    c.state_var = int(d.get("state_var", 0))
+    c.stabilization = int(d.get("stabilization", 0))
+    c.idl_enable = bool(d.get("idl_enable", False))
+    c.idl_pressure = d.get("idl_pressure", 0.0)
+    c.idl_amplitude = d.get("idl_amplitude", 0.0)
+    c.idl_start = d.get("idl_start", 0.0)
+    c.idl_length = d.get("idl_length", 0.0)
+    t = d["tau"]
+    c.tau_coeffs = TauCoeffs(t["Ctau_t"], t["Ctau_v"], t["Ctau_C"], t["Ctau_M"], t["Ctau_E"])
+    c.dt = d.get("dt", 1.0)
+    c.divFdiff_method = 0
+    return c
+
+
+_libs: dict = {}
+
+
+def load_libs():
+    """Load libceed.so and libhonee_setup.so from the package's lib/ directory (raises if they are not built)."""
+    if "setup" not in _libs:
+        ceed_path = os.path.join(LIB_DIR, "libceed.so")
+        setup_path = os.path.join(LIB_DIR, "libhonee_setup.so")
+        for p in (ceed_path, setup_path):
+            if not os.path.exists(p):
+                raise RuntimeError(f"MI355X backend library {p} is not built; run __graft_entry__.build()")
+        _libs["ceed"] = ct.CDLL(ceed_path, mode=ct.RTLD_GLOBAL)
+        lib = ct.CDLL(setup_path)
+        lib.HoneeMiLastError.restype = ct.c_char_p
+        lib.HoneeMiProblemError.restype = ct.c_char_p
+        lib.HoneeMiProblemError.argtypes = [ct.c_void_p]
+        lib.HoneeMiProblemGetCeed.restype = ct.c_void_p
+        lib.HoneeMiProblemGetCeed.argtypes = [ct.c_void_p]
+        lib.HoneeMiProblemGetOperator.restype = ct.c_void_p
+        lib.HoneeMiProblemGetOperator.argtypes = [ct.c_void_p, ct.c_int]
+        lib.HoneeMiApplyIFunction.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiApplyIJacobian.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiGetQData.argtypes = [ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiGetJacData.argtypes = [ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiGetJacData32.argtypes = [ct.c_void_p, ct.c_void_p]
+        c = _libs["ceed"]
+        c.CeedMiSetKernelTiming.argtypes = [ct.c_void_p, ct.c_bool]
+        c.CeedMiResetKernelTimes.argtypes = [ct.c_void_p]
+        c.CeedMiGetKernelTimes.argtypes = [ct.c_void_p, ct.c_int, ct.c_char_p, ct.c_size_t, _ip, _dp, _ip]
+        c.CeedMiSynchronize.argtypes = [ct.c_void_p]
+        _libs["setup"] = lib
+    return _libs["ceed"], _libs["setup"]
+
+
+def _ptr(a):
+    """Pointer of a numpy array (host) or a torch tensor (device)."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ct.c_void_p)
+    return ct.c_void_p(a.data_ptr())
+
+
+class NewtonianProblem:
+    """Newtonian IFunction/IJacobian operators on the MI355X backend for a box mesh."""
+
+    def __init__(self, mesh: BoxMesh, tab: Tabulation, ctx: NewtonianContext, state_var: int, resource: str = "/gpu/hip/mi355x"):
+        self.ceedlib, self.lib = load_libs()
+        self.mesh, self.tab = mesh, tab
+        self.ctx = ctx
+        self._keep = dict(
+            offsets=np.ascontiguousarray(mesh.offsets, dtype=np.int32),
+            xoffsets=np.ascontiguousarray(mesh.xoffsets, dtype=np.int32),
+            xcoord=np.ascontiguousarray(mesh.xcoord, dtype=np.float64),
+            B=np.ascontiguousarray(tab.B), D=np.ascontiguousarray(tab.D),
+            Bx=np.ascontiguousarray(tab.Bx), Dx=np.ascontiguousarray(tab.Dx),
+            qref=np.ascontiguousarray(tab.qref), qweight=np.ascontiguousarray(tab.qweight),
+        )
+        k = self._keep
+        d = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+        self.m = HoneeMiMesh(mesh.nelem, tab.P, tab.Q, mesh.nnodes, k["xcoord"].shape[0], k["offsets"].ctypes.data_as(_ip),
+                             k["xoffsets"].ctypes.data_as(_ip), d(k["xcoord"]), d(k["B"]), d(k["D"]), d(k["Bx"]), d(k["Dx"]),
+                             d(k["qref"]), d(k["qweight"]))
+        self.handle = ct.c_void_p()
+        ierr = self.lib.HoneeMiProblemCreate(resource.encode(), ct.byref(self.m), ct.byref(ctx), int(state_var), ct.byref(self.handle))
+        if ierr:
+            raise RuntimeError(f"HoneeMiProblemCreate failed ({ierr}): {self.lib.HoneeMiLastError().decode()}")
+        self.njac = 14 + (1 if ctx.idl_enable else 0)
+        self.nelem, self.Q3, self.nnodes = mesh.nelem, tab.Q**3, mesh.nnodes
+
+    def _check(self, ierr, what):
+        if ierr:
+            raise RuntimeError(f"{what} failed ({ierr}): {self.lib.HoneeMiProblemError(self.handle).decode()}")
+
+    @property
+    def ceed(self):
+        return self.lib.HoneeMiProblemGetCeed(self.handle)
+
+    def ifunction(self, q, qdot, g, time=0.0, dt=None, mem=CEED_MEM_HOST):
+        dt = self.ctx.dt if dt is None else dt
+        self._check(self.lib.HoneeMiApplyIFunction(self.handle, mem, time, dt, _ptr(q), _ptr(qdot), _ptr(g)), "IFunction")
+
+    def ijacobian(self, dq, v, shift, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiApplyIJacobian(self.handle, mem, shift, _ptr(dq), _ptr(v)), "IJacobian")
+
+    def qdata(self):
+        out = np.zeros((self.nelem, 10, self.Q3))
+        self._check(self.lib.HoneeMiGetQData(self.handle, _ptr(out)), "GetQData")
+        return out
+
+    def jac_data(self):
+        out = np.zeros((self.nelem, self.njac, self.Q3))
+        self._check(self.lib.HoneeMiGetJacData(self.handle, _ptr(out)), "GetJacData")
+        return out
+
+    def jac_data32(self):
+        out = np.zeros((self.nelem, self.njac, self.Q3), dtype=np.float32)
+        self._check(self.lib.HoneeMiGetJacData32(self.handle, _ptr(out)), "GetJacData32")
+        return out
+
+    def set_timing(self, enable: bool):
+        self.ceedlib.CeedMiSetKernelTiming(self.ceed, enable)
+
+    def reset_times(self):
+        self.ceedlib.CeedMiResetKernelTimes(self.ceed)
+
+    def kernel_times(self) -> dict:
+        names = ct.create_string_buffer(4096)
+        launches = (ct.c_int * 32)()
+        total = (ct.c_double * 32)()
+        n = ct.c_int(0)
+        self.ceedlib.CeedMiGetKernelTimes(self.ceed, 32, names, 4096, launches, total, ct.byref(n))
+        keys = names.value.decode().split(",") if n.value else []
+        return {k: (launches[i], total[i]) for i, k in enumerate(keys)}
+
+    def synchronize(self):
+        self.ceedlib.CeedMiSynchronize(self.ceed)
+
+    def close(self):
+        if self.handle:
+            self.lib.HoneeMiProblemDestroy(ct.byref(self.handle))
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,44 @@
+"""HONEE binary Vec files (src/honee-file.c:37-39, 136-217).
+
+Layout, all big-endian: an optional HONEE header ``int32 token`` (0xceedf32 -> int32 step, 0xceedf64 -> int64 step,
+0xceedf00 legacy) followed by ``int step`` and ``float64 time``; then PETSc's ``VecView`` binary record
+``int32 VEC_FILE_CLASSID (1211214)``, ``int32 n`` and ``n`` float64 values.  A file may also be a bare Vec record.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+VEC_FILE_CLASSID = 1211214
+TOKEN_LEGACY, TOKEN_32, TOKEN_64 = 0xCEEDF00, 0xCEEDF32, 0xCEEDF64
+
+
+def read_vec(path: str) -> tuple[np.ndarray, int, float]:
+    """Return (values, step, time); step and time are 0 for bare Vec files."""
+    buf = open(path, "rb").read()
+    tok = int(np.frombuffer(buf[:4], ">i4")[0])
+    off, step, time = 0, 0, 0.0
+    if tok in (TOKEN_32, TOKEN_64, TOKEN_LEGACY):
+        isz = 8 if tok == TOKEN_64 else 4
+        step = int(np.frombuffer(buf[4:4 + isz], ">i8" if isz == 8 else ">i4")[0])
+        time = float(np.frombuffer(buf[4 + isz:12 + isz], ">f8")[0])
+        off = 12 + isz
+    cid, n = (int(v) for v in np.frombuffer(buf[off:off + 8], ">i4"))
+    if cid != VEC_FILE_CLASSID:
+        raise ValueError(f"{path}: not a PETSc Vec record (class id {cid})")
+    vals = np.frombuffer(buf[off + 8:off + 8 + 8 * n], ">f8").astype(np.float64)
+    if vals.size != n:
+        raise ValueError(f"{path}: truncated Vec ({vals.size} of {n} values)")
+    return vals, step, time
+
+
+def write_vec(path: str, values: np.ndarray, step: int = 0, time: float = 0.0, index_bits: int = 32) -> None:
+    """Write a HONEE-formatted file (HoneeWriteBinaryVec)."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    with open(path, "wb") as f:
+        if index_bits == 64:
+            f.write(np.array([TOKEN_64], ">i4").tobytes() + np.array([step], ">i8").tobytes())
+        else:
+            f.write(np.array([TOKEN_32, step], ">i4").tobytes())
+        f.write(np.array([time], ">f8").tobytes())
+        f.write(np.array([VEC_FILE_CLASSID, v.size], ">i4").tobytes())
+        f.write(v.astype(">f8").tobytes())

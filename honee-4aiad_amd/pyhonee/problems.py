@@ -1,0 +1,98 @@
+"""Problem definitions (contexts and initial conditions) for the configurations this package runs.
+
+Mirrors HONEE's problem setup: NS_NEWTONIAN_IG defaults (problems/newtonian.c:588-613), the Taylor-Green YAML
+(examples/taylor_green_vortex.yaml) and ICsTaylorGreen (qfunctions/taylorgreen.h:23-51).  Host-side setup only.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .driver import STAB_SUPG, VAR_CONS, VAR_ENTROPY, VAR_PRIM, context_from_dict
+from .mesh import BoxMesh
+
+# examples/taylor_green_vortex.yaml: Re=1600, M=0.1, Pr=0.71
+TGV_GAS = {"lambda": -2.0 / 3.0, "mu": 6.25e-4, "k": 0.8836267606, "cv": 717.0, "cp": 1003.8}
+TGV_REFERENCE_Y = (71.42857143, 1.0, 0.0, 0.0, 0.2490535963)  # pressure, velocity, temperature
+
+
+def default_tau(degree: int) -> dict:
+    """Default Tau coefficients of NS_NEWTONIAN_IG (problems/newtonian.c:601-608)."""
+    cv_func = (36.0, 60.0, 128.0)
+    return {"Ctau_t": 1.0, "Ctau_v": cv_func[min(3, degree) - 1], "Ctau_C": 0.25 / degree, "Ctau_M": 0.25 / degree, "Ctau_E": 0.125}
+
+
+TGV_TAU = {"Ctau_t": 1.0, "Ctau_v": 60.0, "Ctau_C": 0.5, "Ctau_M": 0.5, "Ctau_E": 0.5}
+
+
+def tgv_context_dict(state_var=VAR_CONS, stabilization=STAB_SUPG, implicit=True, dt=2e-3, tau=None) -> dict:
+    return {"gas": dict(TGV_GAS), "tau": dict(tau or TGV_TAU), "state_var": state_var, "stabilization": stabilization,
+            "is_implicit": implicit, "dt": dt}
+
+
+def tgv_context(**kw):
+    return context_from_dict(tgv_context_dict(**kw))
+
+
+# ---- state conversions needed to express an initial condition in the chosen variables (newtonian_state.h) ----
+def _rgas(gas):
+    return gas["cp"] - gas["cv"]
+
+
+def cons_from_prim(gas, Y):
+    rho = Y[..., 0] / (_rgas(gas) * Y[..., 4])
+    u = Y[..., 1:4]
+    U = np.empty_like(Y)
+    U[..., 0] = rho
+    U[..., 1:4] = rho[..., None] * u
+    U[..., 4] = rho * (gas["cv"] * Y[..., 4] + 0.5 * np.sum(u * u, axis=-1))
+    return U
+
+
+def entropy_from_prim(gas, Y):
+    gamma = gas["cp"] / gas["cv"]
+    rho = Y[..., 0] / (_rgas(gas) * Y[..., 4])
+    entropy = np.log(Y[..., 0]) - gamma * np.log(rho)
+    rdp = rho / Y[..., 0]
+    ek = 0.5 * np.sum(Y[..., 1:4] ** 2, axis=-1)
+    V = np.empty_like(Y)
+    V[..., 0] = (gamma - entropy) / (gamma - 1) - rdp * ek
+    V[..., 1:4] = rdp[..., None] * Y[..., 1:4]
+    V[..., 4] = -rdp
+    return V
+
+
+def q_from_prim(gas, Y, state_var):
+    if state_var == VAR_PRIM:
+        return np.array(Y, dtype=np.float64)
+    if state_var == VAR_CONS:
+        return cons_from_prim(gas, Y)
+    return entropy_from_prim(gas, Y)
+
+
+def taylor_green_prim(xyz, ref_Y=TGV_REFERENCE_Y, gas=TGV_GAS, background=(0.0, 0.0, 0.0), lx=2 * np.pi):
+    """Primitive TGV state at points xyz[..., 3] (qfunctions/taylorgreen.h:35-45); all directions scale by 2 pi / lx."""
+    R = _rgas(gas)
+    V0 = float(np.linalg.norm(ref_Y[1:4]))
+    rho0 = ref_Y[0] / (ref_Y[4] * R)
+    x = np.asarray(xyz, dtype=np.float64) * (2 * np.pi / lx)
+    Y = np.empty(x.shape[:-1] + (5,))
+    Y[..., 0] = ref_Y[0] + (rho0 * V0**2 / 16) * (np.cos(2 * x[..., 0]) + np.cos(2 * x[..., 1])) * np.cos(2 * x[..., 2] + 2)
+    Y[..., 1] = background[0] + V0 * np.sin(x[..., 0]) * np.cos(x[..., 1]) * np.cos(x[..., 2])
+    Y[..., 2] = background[1] - V0 * np.cos(x[..., 0]) * np.sin(x[..., 1]) * np.cos(x[..., 2])
+    Y[..., 3] = background[2]
+    Y[..., 4] = ref_Y[4]
+    return Y
+
+
+def nodal_field(mesh: BoxMesh, per_node_fn) -> np.ndarray:
+    """Evaluate a pointwise function at every element's nodes and average over node multiplicity, as HONEE's
+    ICs_FixMultiplicity does (src/misc.c:16-48).  Returns an L-vector [nnodes][5]."""
+    xyz = mesh.element_node_coords()
+    vals = per_node_fn(xyz.reshape(-1, 3)).reshape(mesh.nelem, -1, 5)
+    q = np.zeros((mesh.nnodes, 5))
+    np.add.at(q, mesh.offsets.ravel(), vals.reshape(-1, 5))
+    return q / mesh.multiplicity()[:, None]
+
+
+def taylor_green_ic(mesh: BoxMesh, state_var=VAR_CONS, gas=TGV_GAS, ref_Y=TGV_REFERENCE_Y, background=(0.0, 0.0, 0.0), lx=2 * np.pi):
+    return nodal_field(mesh, lambda x: q_from_prim(gas, taylor_green_prim(x, ref_Y, gas, background, lx), state_var))

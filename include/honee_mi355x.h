@@ -1,0 +1,81 @@
+// MI355X backend extensions and the QFunction context layouts the backend interprets.
+//
+// The fused kernels read HONEE's QFunction context structs by value, so their binary layout is part of the interface:
+// each struct below has exactly the member order and types of the reference struct it mirrors (cited), and the
+// backend checks the registered context size against sizeof() before binding an operator.
+#pragma once
+
+#include "ceed.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// struct NewtonianIdealGasContext_ (qfunctions/newtonian_types.h:20-47, stabilization_types.h:32-38)
+typedef struct {
+  double lambda, mu, k, cv, cp;  // NewtonianIGProperties
+} CeedMiGas;
+
+typedef struct {
+  double Ctau_t, Ctau_v, Ctau_C, Ctau_M, Ctau_E;  // TauDiagCoefficients
+} CeedMiTauCoeffs;
+
+typedef struct {
+  CeedMiGas       gas;
+  double          g[3];
+  double          time;
+  double          ijacobian_time_shift;
+  bool            is_implicit;
+  int             state_var;      // StateVariable: 0 conservative, 1 primitive, 2 entropy
+  int             stabilization;  // StabilizationType: 0 none, 1 SU, 2 SUPG
+  bool            idl_enable;
+  double          idl_pressure;
+  double          idl_amplitude;
+  double          idl_start;
+  double          idl_length;
+  CeedMiTauCoeffs tau_coeffs;
+  double          dt;
+  int             divFdiff_method;  // DivDiffFluxProjectionMethod: only NONE (0) is supported by the fused kernels
+} CeedMiNewtonianContext;
+
+// State (qfunctions/newtonian_state.h:13-22): conservative then primitive
+typedef struct {
+  double U[5];
+  double Y[5];
+} CeedMiState;
+
+// struct FreestreamContext_ (qfunctions/bc_freestream_type.h:9-12)
+typedef struct {
+  CeedMiNewtonianContext newt_ctx;
+  CeedMiState            S_infty;
+} CeedMiFreestreamContext;
+
+// struct TaylorGreenContext_ (qfunctions/taylorgreen.h:13-20)
+typedef struct {
+  double                 reference[5];  // StatePrimitive: pressure, velocity[3], temperature
+  CeedMiNewtonianContext newt_ctx;
+  double                 lx, ly, lz;
+  double                 u[3];
+} CeedMiTaylorGreenContext;
+
+// ---- Backend extensions (no counterpart in libCEED; used by this repository's driver, tests and benchmark) ----
+
+// HIP stream the backend launches on (a hipStream_t).
+int CeedMiGetStream(Ceed ceed, void **hip_stream);
+
+// Kernel timing: when enabled, every fused kernel launch of an operator apply is bracketed by HIP events on the
+// backend stream.  CeedMiGetKernelTimes returns, per kernel name (comma-separated in names), the number of launches
+// and the summed event time in milliseconds since the last reset.
+int CeedMiSetKernelTiming(Ceed ceed, bool enable);
+int CeedMiResetKernelTimes(Ceed ceed);
+int CeedMiGetKernelTimes(Ceed ceed, int max_kernels, char *names, size_t names_len, int *launches, double *total_ms, int *num_kernels);
+
+// Device pointer of a vector's FP32 shadow array (the jac_data mirror written in jacobian_precision=fp32 mode).
+int CeedMiVectorGetFp32ShadowRead(CeedVector vec, const float **d_array, CeedSize *length);
+
+// Synchronize the backend stream.
+int CeedMiSynchronize(Ceed ceed);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,620 @@
+// CPU ORACLE -- test infrastructure only (see ns_pointwise.h).
+//
+// libCEED /cpu/self-style operator application for the HONEE Newtonian volume operators.  Per element: gather the
+// L-vector through the offsets, apply the tensor-product basis (interp with B, gradient with D in one direction and B
+// in the others, contracting x first as libCEED's CeedTensorContractApply does), run the QFunction restatement at each
+// quadrature point, apply the basis transpose, and scatter-add.  Element outputs are computed in parallel (OpenMP) into
+// a private E-vector and assembled serially in element order, so results do not depend on the thread count.
+#include "honee_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+static inline int ipow3(int n) { return n * n * n; }
+
+// ----------------------------------------------------------------------------------------------------------------------
+// Tensor contractions.  u is [nz][ny][nx] with x fastest.  A is [nrow][ncol] row-major.
+// ----------------------------------------------------------------------------------------------------------------------
+// out = A applied along direction dir (A: [nout][nin]); ext = input extents {nx, ny, nz}
+static void contract(const double *A, int nout, int nin, int dir, const int ext[3], const double *in, double *out) {
+  int stride = 1, pre = 1;
+  for (int d = 0; d < dir; d++) stride *= ext[d];
+  for (int d = dir + 1; d < 3; d++) pre *= ext[d];
+  for (int a = 0; a < pre; a++)
+    for (int o = 0; o < nout; o++)
+      for (int s = 0; s < stride; s++) {
+        double acc = 0;
+        for (int i = 0; i < nin; i++) acc += A[o * nin + i] * in[(a * nin + i) * stride + s];
+        out[(a * nout + o) * stride + s] = acc;
+      }
+}
+
+// out = A^T applied along direction dir (A: [nin][nout], i.e. contract over A's row index)
+static void contract_t(const double *A, int nin, int nout, int dir, const int ext[3], const double *in, double *out) {
+  int stride = 1, pre = 1;
+  for (int d = 0; d < dir; d++) stride *= ext[d];
+  for (int d = dir + 1; d < 3; d++) pre *= ext[d];
+  for (int a = 0; a < pre; a++)
+    for (int o = 0; o < nout; o++)
+      for (int s = 0; s < stride; s++) {
+        double acc = 0;
+        for (int i = 0; i < nin; i++) acc += A[i * nout + o] * in[(a * nin + i) * stride + s];
+        out[(a * nout + o) * stride + s] = acc;
+      }
+}
+
+// v[Q^3] = (Az (x) Ay (x) Ax) u[P^3], matrices [Q][P]
+static void tensor3(int P, int Q, const double *Ax, const double *Ay, const double *Az, const double *u, double *v) {
+  double t1[512], t2[512];
+  int    e0[3] = {P, P, P}, e1[3] = {Q, P, P}, e2[3] = {Q, Q, P};
+  contract(Ax, Q, P, 0, e0, u, t1);
+  contract(Ay, Q, P, 1, e1, t1, t2);
+  contract(Az, Q, P, 2, e2, t2, v);
+}
+
+// u[P^3] += (Az (x) Ay (x) Ax)^T v[Q^3]
+static void tensor3_t_add(int P, int Q, const double *Ax, const double *Ay, const double *Az, const double *v, double *u) {
+  double t1[512], t2[512], t3[512];
+  int    e0[3] = {Q, Q, Q}, e1[3] = {P, Q, Q}, e2[3] = {P, P, Q};
+  contract_t(Ax, Q, P, 0, e0, v, t1);
+  contract_t(Ay, Q, P, 1, e1, t1, t2);
+  contract_t(Az, Q, P, 2, e2, t2, t3);
+  for (int i = 0; i < ipow3(P); i++) u[i] += t3[i];
+}
+
+// Interp + grad of ncomp components of an element (u_e: [ncomp][P^3]) -> uq [ncomp][Q^3], duq [3][ncomp][Q^3]
+static void interp_grad(int P, int Q, const double *B, const double *D, int ncomp, const double *u_e, double *uq, double *duq) {
+  const int Q3 = ipow3(Q), P3 = ipow3(P);
+  for (int c = 0; c < ncomp; c++) {
+    if (uq) tensor3(P, Q, B, B, B, &u_e[c * P3], &uq[c * Q3]);
+    if (duq) {
+      tensor3(P, Q, D, B, B, &u_e[c * P3], &duq[(0 * ncomp + c) * Q3]);
+      tensor3(P, Q, B, D, B, &u_e[c * P3], &duq[(1 * ncomp + c) * Q3]);
+      tensor3(P, Q, B, B, D, &u_e[c * P3], &duq[(2 * ncomp + c) * Q3]);
+    }
+  }
+}
+
+// Transpose: r_e[ncomp][P^3] = B^T v + sum_d G_d^T gv[d]
+static void interp_grad_t(int P, int Q, const double *B, const double *D, int ncomp, const double *vq, const double *gvq, double *r_e) {
+  const int Q3 = ipow3(Q), P3 = ipow3(P);
+  memset(r_e, 0, sizeof(double) * ncomp * P3);
+  for (int c = 0; c < ncomp; c++) {
+    if (vq) tensor3_t_add(P, Q, B, B, B, &vq[c * Q3], &r_e[c * P3]);
+    if (gvq) {
+      tensor3_t_add(P, Q, D, B, B, &gvq[(0 * ncomp + c) * Q3], &r_e[c * P3]);
+      tensor3_t_add(P, Q, B, D, B, &gvq[(1 * ncomp + c) * Q3], &r_e[c * P3]);
+      tensor3_t_add(P, Q, B, B, D, &gvq[(2 * ncomp + c) * Q3], &r_e[c * P3]);
+    }
+  }
+}
+
+// Restriction gather for an L-vector with 5 interleaved components (comp_stride = 1, src/dm-utils.c:108-112)
+static void gather5(const OMesh *m, int e, const double *L, double *u_e) {
+  const int P3 = ipow3(m->P);
+  for (int n = 0; n < P3; n++) {
+    const int node = m->offsets[e * P3 + n];
+    for (int c = 0; c < 5; c++) u_e[c * P3 + n] = L[node * 5 + c];
+  }
+}
+
+// Serial scatter-add of E-vector [nelem][5][P^3] in element order
+static void scatter5(const OMesh *m, const double *E, double *L) {
+  const int P3 = ipow3(m->P);
+  memset(L, 0, sizeof(double) * 5 * m->nnodes);
+  for (int e = 0; e < m->nelem; e++)
+    for (int n = 0; n < P3; n++) {
+      const int node = m->offsets[e * P3 + n];
+      for (int c = 0; c < 5; c++) L[node * 5 + c] += E[(e * 5 + c) * P3 + n];
+    }
+}
+
+// Coordinates of the element's quadrature points: x[3][Q^3] (coordinate basis, INTERP), and dxdX[3 dirs][3 comps][Q^3] (GRAD)
+static void elem_coords(const OMesh *m, int e, double *xq, double *dxq) {
+  const int Px3 = ipow3(m->Px);
+  double    xe[3 * 64];
+  for (int n = 0; n < Px3; n++) {
+    const int node = m->xoffsets[e * Px3 + n];
+    for (int c = 0; c < 3; c++) xe[c * Px3 + n] = m->xcoord[node * 3 + c];
+  }
+  interp_grad(m->Px, m->Q, m->Bx, m->Dx, 3, xe, xq, dxq);
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// Setup (qfunctions/setupgeo.h:42-56, setupgeo_helpers.h:26-32, utils.h:201-216)
+// ----------------------------------------------------------------------------------------------------------------------
+static void mat_inv3(const double A[3][3], double Ai[3][3], double *detJ) {
+  Ai[0][0]       = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  Ai[0][1]       = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+  Ai[0][2]       = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+  Ai[1][0]       = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+  Ai[1][1]       = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+  Ai[1][2]       = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+  Ai[2][0]       = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  Ai[2][1]       = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+  Ai[2][2]       = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  const double d = A[0][0] * Ai[0][0] + A[1][0] * Ai[0][1] + A[2][0] * Ai[0][2];
+  const double s = 1 / d;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Ai[i][j] *= s;
+  *detJ = d;
+}
+
+int oracle_setup_qdata(const OMesh *m, double *qdata) {
+  const int Q = m->Q, Q3 = ipow3(Q);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double dxq[9 * 512];
+    elem_coords(m, e, NULL, dxq);
+    for (int i = 0; i < Q3; i++) {
+      const int    qx = i % Q, qy = (i / Q) % Q, qz = i / (Q * Q);
+      const double w  = m->w1d[qx] * m->w1d[qy] * m->w1d[qz];
+      double       J[3][3], Ji[3][3], detJ;
+      for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) J[c][d] = dxq[(d * 3 + c) * Q3 + i];  // dx_c / dX_d (GradUnpack3D)
+      mat_inv3(J, Ji, &detJ);
+      double *qd        = &qdata[(size_t)e * 10 * Q3];
+      qd[0 * Q3 + i]    = w * detJ;
+      for (int k = 0; k < 9; k++) qd[(1 + k) * Q3 + i] = (&Ji[0][0])[k];
+    }
+  }
+  return 0;
+}
+
+double oracle_volume(const OMesh *m, const double *qdata) {
+  const int Q3  = ipow3(m->Q);
+  double    vol = 0;
+  for (int e = 0; e < m->nelem; e++)
+    for (int i = 0; i < Q3; i++) vol += qdata[(size_t)e * 10 * Q3 + i];
+  return vol;
+}
+
+static inline void unpack_qdata(const double *qd, int Q3, int i, double *wdetJ, double dXdx[3][3]) {
+  *wdetJ = qd[i];
+  for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = qd[(1 + k) * Q3 + i];
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// IFunction_Newtonian (qfunctions/newtonian.h:205-257) + IFunction_Newtonian_Integrand (newtonian_integrands.h:87-115)
+// ----------------------------------------------------------------------------------------------------------------------
+static void qf_ifunction(const ONewt *c, const double q[5], const double gref[3][5], const double qdot[5], const double wdetJ,
+                         const double dXdx[3][3], const double x[3], double v[5], double gv[3][5], double *jd) {
+  const OGas   g   = c->gas;
+  const int    var = c->state_var;
+  const OState s   = o_state_from_q(g, q, var);
+  const OState sd  = o_state_from_q_fwd(g, &s, qdot, var);
+  OState       gs[3];
+  o_phys_gradient(g, &s, var, gref, dXdx, gs);
+  const double divF[5] = {0};
+
+  double v_i[5] = {0}, gv_i[5][3] = {{0}}, sr[6], Tau[3], sigma = 0;
+  {
+    double kmst[6], st[3][3], Fe[3], Finv[3][5], Ft[5][3];
+    o_km_strain_rate(gs, sr);
+    o_newtonian_stress(g, sr, kmst);
+    o_km_unpack(kmst, st);
+    o_viscous_energy_flux(g, s.Y, gs, st, Fe);
+    o_flux_inviscid(&s, Finv);
+    o_flux_total(Finv, st, Fe, Ft);
+    for (int j = 0; j < 5; j++)
+      for (int k = 0; k < 3; k++) gv_i[j][k] += -1 * Ft[j][k];
+    const double bf[5] = {0, s.U[0] * c->g[0], s.U[0] * c->g[1], s.U[0] * c->g[2], o_dot3(&s.U[1], c->g)};
+    double       Udot[5];
+    memcpy(Udot, sd.U, sizeof(Udot));
+    for (int j = 0; j < 5; j++) v_i[j] += Udot[j] - bf[j];
+    double stab[5][3];
+    o_tau_diag_prim(c->tau, g, &s, dXdx, c->dt, Tau);
+    o_stabilization(c->stabilization, g, &s, Tau, gs, Udot, bf, divF, stab);
+    for (int j = 0; j < 5; j++)
+      for (int k = 0; k < 3; k++) gv_i[j][k] += 1 * stab[j][k];
+  }
+  if (c->idl_enable) {  // InternalDampingLayer_IFunction_Integrand (newtonian_integrands.h:40-48)
+    sigma             = o_linear_ramp(c->idl_amplitude, c->idl_length, c->idl_start, x[0]);
+    double damp[5]    = {s.Y[0] - c->idl_pressure, 0, 0, 0, 0}, res[5] = {0};
+    o_idl_residual(g, &s, sigma, damp, res);
+    for (int j = 0; j < 5; j++) v_i[j] += 1 * res[j];
+  }
+  for (int j = 0; j < 5; j++) v[j] = wdetJ * v_i[j];
+  for (int j = 0; j < 5; j++)
+    for (int k = 0; k < 3; k++) gv[k][j] = wdetJ * (gv_i[j][0] * dXdx[k][0] + gv_i[j][1] * dXdx[k][1] + gv_i[j][2] * dXdx[k][2]);
+  for (int j = 0; j < 5; j++) jd[j] = q[j];
+  for (int j = 0; j < 6; j++) jd[5 + j] = sr[j];
+  for (int j = 0; j < 3; j++) jd[11 + j] = Tau[j];
+  if (c->idl_enable) jd[14] = sigma;
+}
+
+static inline int njac(const ONewt *c) { return 14 + (c->idl_enable ? 1 : 0); }
+
+int oracle_ifunction(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v, double *jac) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q), nj = njac(c);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], duq[15 * 512], udq[5 * 512], vq[5 * 512], gvq[15 * 512], xq[3 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, duq);
+    gather5(m, e, qdot, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, udq, NULL);
+    if (c->idl_enable) elem_coords(m, e, xq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5], qdi[5], gref[3][5], wdetJ, dXdx[3][3], x[3] = {0}, vi[5], gvi[3][5], jd[15];
+      for (int k = 0; k < 5; k++) {
+        qi[k]  = uq[k * Q3 + i];
+        qdi[k] = udq[k * Q3 + i];
+        for (int d = 0; d < 3; d++) gref[d][k] = duq[(d * 5 + k) * Q3 + i];
+      }
+      if (c->idl_enable)
+        for (int d = 0; d < 3; d++) x[d] = xq[d * Q3 + i];
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      qf_ifunction(c, qi, gref, qdi, wdetJ, dXdx, x, vi, gvi, jd);
+      for (int k = 0; k < 5; k++) {
+        vq[k * Q3 + i] = vi[k];
+        for (int d = 0; d < 3; d++) gvq[(d * 5 + k) * Q3 + i] = gvi[d][k];
+      }
+      for (int k = 0; k < nj; k++) jac[((size_t)e * nj + k) * Q3 + i] = jd[k];
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 5, vq, gvq, &E[(size_t)e * 5 * P3]);
+  }
+  scatter5(m, E, v);
+  free(E);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// IJacobian_Newtonian (newtonian.h:272-311) + IJacobian_Newtonian_Integrand (newtonian_integrands.h:134-166)
+// ----------------------------------------------------------------------------------------------------------------------
+static void qf_ijacobian(const ONewt *c, const double dq[5], const double gref[3][5], const double wdetJ, const double dXdx[3][3],
+                         const double *jd, double v[5], double gv[3][5]) {
+  const OGas   g   = c->gas;
+  const int    var = c->state_var;
+  double       qi[5], sr[6], Tau[3];
+  for (int j = 0; j < 5; j++) qi[j] = jd[j];
+  for (int j = 0; j < 6; j++) sr[j] = jd[5 + j];
+  for (int j = 0; j < 3; j++) Tau[j] = jd[11 + j];
+  const OState s  = o_state_from_q(g, qi, var);
+  const OState ds = o_state_from_q_fwd(g, &s, dq, var);
+  OState       gds[3];
+  o_phys_gradient(g, &s, var, gref, dXdx, gds);
+
+  double v_i[5] = {0}, gv_i[5][3] = {{0}};
+  {
+    double dsr[6], dkmst[6], kmst[6], st[3][3], dst[3][3], dFe[3], dFinv[3][5], dFt[5][3];
+    o_km_strain_rate(gds, dsr);
+    o_newtonian_stress(g, dsr, dkmst);
+    o_newtonian_stress(g, sr, kmst);
+    o_km_unpack(dkmst, dst);
+    o_km_unpack(kmst, st);
+    o_viscous_energy_flux_fwd(g, s.Y, ds.Y, gds, st, dst, dFe);
+    o_flux_inviscid_fwd(&s, &ds, dFinv);
+    o_flux_total(dFinv, dst, dFe, dFt);
+    for (int j = 0; j < 5; j++)
+      for (int k = 0; k < 3; k++) gv_i[j][k] += -1 * dFt[j][k];
+    const double dbf[5] = {0, ds.U[0] * c->g[0], ds.U[0] * c->g[1], ds.U[0] * c->g[2], o_dot3(&ds.U[1], c->g)};
+    double       dUdot[5];
+    for (int j = 0; j < 5; j++) {
+      dUdot[j] = c->ijacobian_time_shift * ds.U[j];
+      v_i[j]   = dUdot[j] - dbf[j];
+    }
+    double       dstab[5][3];
+    const double zero[5] = {0};
+    o_stabilization(c->stabilization, g, &s, Tau, gds, dUdot, dbf, zero, dstab);
+    for (int j = 0; j < 5; j++)
+      for (int k = 0; k < 3; k++) gv_i[j][k] += 1 * dstab[j][k];
+  }
+  if (c->idl_enable) {  // InternalDampingLayer_IJacobian_Integrand (newtonian_integrands.h:61-65)
+    const double sigma   = jd[14];
+    double       damp[5] = {ds.Y[0], 0, 0, 0, 0}, res[5] = {0};
+    o_idl_residual(g, &s, sigma, damp, res);
+    for (int j = 0; j < 5; j++) v_i[j] += 1 * res[j];
+  }
+  for (int j = 0; j < 5; j++) v[j] = wdetJ * v_i[j];
+  for (int j = 0; j < 5; j++)
+    for (int k = 0; k < 3; k++) gv[k][j] = wdetJ * (gv_i[j][0] * dXdx[k][0] + gv_i[j][1] * dXdx[k][1] + gv_i[j][2] * dXdx[k][2]);
+}
+
+// Element action of the IJacobian on an element vector ue [5][P^3] -> re [5][P^3]
+static void elem_ijacobian(const OMesh *m, const ONewt *c, const double *qd, const double *jde, const double *ue, double *re) {
+  const int Q3 = ipow3(m->Q), nj = njac(c);
+  double    uq[5 * 512], duq[15 * 512], vq[5 * 512], gvq[15 * 512];
+  interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, duq);
+  for (int i = 0; i < Q3; i++) {
+    double dqi[5], gref[3][5], wdetJ, dXdx[3][3], vi[5], gvi[3][5], jd[15];
+    for (int k = 0; k < 5; k++) {
+      dqi[k] = uq[k * Q3 + i];
+      for (int d = 0; d < 3; d++) gref[d][k] = duq[(d * 5 + k) * Q3 + i];
+    }
+    for (int k = 0; k < nj; k++) jd[k] = jde[k * Q3 + i];
+    unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+    qf_ijacobian(c, dqi, gref, wdetJ, dXdx, jd, vi, gvi);
+    for (int k = 0; k < 5; k++) {
+      vq[k * Q3 + i] = vi[k];
+      for (int d = 0; d < 3; d++) gvq[(d * 5 + k) * Q3 + i] = gvi[d][k];
+    }
+  }
+  interp_grad_t(m->P, m->Q, m->B, m->D, 5, vq, gvq, re);
+}
+
+int oracle_ijacobian(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, const double *dq, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q), nj = njac(c);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512];
+    gather5(m, e, dq, ue);
+    elem_ijacobian(m, c, &qdata[(size_t)e * 10 * Q3], &jac[(size_t)e * nj * Q3], ue, &E[(size_t)e * 5 * P3]);
+  }
+  scatter5(m, E, v);
+  free(E);
+  return 0;
+}
+
+// Diagonal of the assembled operator (CeedOperatorLinearAssembleDiagonal semantics): sum over elements of the element
+// matrix diagonal, obtained here by applying the element operator to each local unit vector.
+int oracle_ijacobian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, double *diag) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q), nj = njac(c);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], re[5 * 512];
+    for (int k = 0; k < 5 * P3; k++) {
+      memset(ue, 0, sizeof(double) * 5 * P3);
+      ue[k] = 1.0;
+      elem_ijacobian(m, c, &qdata[(size_t)e * 10 * Q3], &jac[(size_t)e * nj * Q3], ue, re);
+      E[(size_t)e * 5 * P3 + k] = re[k];
+    }
+  }
+  scatter5(m, E, diag);
+  free(E);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// RHSFunction_Newtonian (newtonian.h:133-202), conservative variables
+// ----------------------------------------------------------------------------------------------------------------------
+static void qf_rhs(const ONewt *c, const double U[5], const double gref[3][5], const double wdetJ, const double dXdx[3][3], const double x[3],
+                   double v[5], double gv[3][5]) {
+  const OGas   g = c->gas;
+  const OState s = o_state_from_q(g, U, OVAR_CONS);
+  OState       gs[3];
+  o_phys_gradient(g, &s, OVAR_CONS, gref, dXdx, gs);
+  double sr[6], kmst[6], st[3][3], Fe[3], Finv[3][5], F[5][3];
+  o_km_strain_rate(gs, sr);
+  o_newtonian_stress(g, sr, kmst);
+  o_km_unpack(kmst, st);
+  o_viscous_energy_flux(g, s.Y, gs, st, Fe);
+  o_flux_inviscid(&s, Finv);
+  o_flux_total(Finv, st, Fe, F);
+  for (int j = 0; j < 5; j++)
+    for (int k = 0; k < 3; k++) gv[k][j] = wdetJ * (dXdx[k][0] * F[j][0] + dXdx[k][1] * F[j][1] + dXdx[k][2] * F[j][2]);
+  const double bf[5] = {0, s.U[0] * c->g[0], s.U[0] * c->g[1], s.U[0] * c->g[2], o_dot3(&s.U[1], c->g)};
+  for (int j = 0; j < 5; j++) v[j] = wdetJ * bf[j];
+  if (c->idl_enable) {
+    const double sigma   = o_linear_ramp(c->idl_amplitude, c->idl_length, c->idl_start, x[0]);
+    double       damp[5] = {s.Y[0] - c->idl_pressure, 0, 0, 0, 0}, res[5] = {0};
+    o_idl_residual(g, &s, sigma, damp, res);
+    for (int j = 0; j < 5; j++) v[j] -= wdetJ * res[j];
+  }
+  const double divF[5] = {0}, Udot[5] = {0};
+  double       Tau[3], stab[5][3];
+  o_tau_diag_prim(c->tau, g, &s, dXdx, c->dt, Tau);
+  o_stabilization(c->stabilization, g, &s, Tau, gs, Udot, bf, divF, stab);
+  for (int j = 0; j < 5; j++)
+    for (int k = 0; k < 3; k++) gv[k][j] -= wdetJ * (stab[j][0] * dXdx[k][0] + stab[j][1] * dXdx[k][1] + stab[j][2] * dXdx[k][2]);
+}
+
+int oracle_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], duq[15 * 512], vq[5 * 512], gvq[15 * 512], xq[3 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, duq);
+    if (c->idl_enable) elem_coords(m, e, xq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5], gref[3][5], wdetJ, dXdx[3][3], x[3] = {0}, vi[5], gvi[3][5];
+      for (int k = 0; k < 5; k++) {
+        qi[k] = uq[k * Q3 + i];
+        for (int d = 0; d < 3; d++) gref[d][k] = duq[(d * 5 + k) * Q3 + i];
+      }
+      if (c->idl_enable)
+        for (int d = 0; d < 3; d++) x[d] = xq[d * Q3 + i];
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      qf_rhs(c, qi, gref, wdetJ, dXdx, x, vi, gvi);
+      for (int k = 0; k < 5; k++) {
+        vq[k * Q3 + i] = vi[k];
+        for (int d = 0; d < 3; d++) gvq[(d * 5 + k) * Q3 + i] = gvi[d][k];
+      }
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 5, vq, gvq, &E[(size_t)e * 5 * P3]);
+  }
+  scatter5(m, E, v);
+  free(E);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// MassFunction_Newtonian (newtonian.h:44-82): stabilized consistent mass applied to q_dot, state q (conservative)
+// ----------------------------------------------------------------------------------------------------------------------
+static void qf_mass(const ONewt *c, const double qdot[5], const double q[5], const double wdetJ, const double dXdx[3][3], double v[5],
+                    double gv[3][5]) {
+  const OGas   g  = c->gas;
+  const OState s  = o_state_from_q(g, q, OVAR_CONS);
+  const OState sd = o_state_from_q(g, qdot, OVAR_CONS);
+  for (int f = 0; f < 5; f++) v[f] = wdetJ * qdot[f];
+  OState gs[3];
+  memset(gs, 0, sizeof(gs));
+  double Tau[3], stab[5][3], bf[5] = {0}, divF[5] = {0}, Udot[5];
+  memcpy(Udot, sd.U, sizeof(Udot));
+  o_tau_diag_prim(c->tau, g, &s, dXdx, c->dt, Tau);
+  o_stabilization(c->stabilization, g, &s, Tau, gs, Udot, bf, divF, stab);
+  for (int j = 0; j < 5; j++)
+    for (int k = 0; k < 3; k++) gv[k][j] = wdetJ * (stab[j][0] * dXdx[k][0] + stab[j][1] * dXdx[k][1] + stab[j][2] * dXdx[k][2]);
+}
+
+static void elem_mass(const OMesh *m, const ONewt *c, const double *qd, const double *qq /* [5][Q3] state at qpts */, const double *ue,
+                      double *re) {
+  const int Q3 = ipow3(m->Q);
+  double    uq[5 * 512], vq[5 * 512], gvq[15 * 512];
+  interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, NULL);
+  for (int i = 0; i < Q3; i++) {
+    double qdi[5], qi[5], wdetJ, dXdx[3][3], vi[5], gvi[3][5];
+    for (int k = 0; k < 5; k++) {
+      qdi[k] = uq[k * Q3 + i];
+      qi[k]  = qq[k * Q3 + i];
+    }
+    unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+    qf_mass(c, qdi, qi, wdetJ, dXdx, vi, gvi);
+    for (int k = 0; k < 5; k++) {
+      vq[k * Q3 + i] = vi[k];
+      for (int d = 0; d < 3; d++) gvq[(d * 5 + k) * Q3 + i] = gvi[d][k];
+    }
+  }
+  interp_grad_t(m->P, m->Q, m->B, m->D, 5, vq, gvq, re);
+}
+
+int oracle_mass_newtonian(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], qq[5 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, qq, NULL);
+    gather5(m, e, qdot, ue);
+    elem_mass(m, c, &qdata[(size_t)e * 10 * Q3], qq, ue, &E[(size_t)e * 5 * P3]);
+  }
+  scatter5(m, E, v);
+  free(E);
+  return 0;
+}
+
+int oracle_mass_newtonian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *diag) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], qq[5 * 512], re[5 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, qq, NULL);
+    for (int k = 0; k < 5 * P3; k++) {
+      memset(ue, 0, sizeof(double) * 5 * P3);
+      ue[k] = 1.0;
+      elem_mass(m, c, &qdata[(size_t)e * 10 * Q3], qq, ue, re);
+      E[(size_t)e * 5 * P3 + k] = re[k];
+    }
+  }
+  scatter5(m, E, diag);
+  free(E);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// MonitorTotalKineticEnergy (qfunctions/monitor_totalkineticenergy.h:8-39); sums over all quadrature points
+// ----------------------------------------------------------------------------------------------------------------------
+int oracle_monitor_ke(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double sums[5]) {
+  const int Q3 = ipow3(m->Q);
+  const OGas g  = c->gas;
+  double   *vals = malloc(sizeof(double) * (size_t)m->nelem * 5 * Q3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], duq[15 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, duq);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5], gref[3][5], wdetJ, dXdx[3][3];
+      for (int k = 0; k < 5; k++) {
+        qi[k] = uq[k * Q3 + i];
+        for (int d = 0; d < 3; d++) gref[d][k] = duq[(d * 5 + k) * Q3 + i];
+      }
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      const OState s = o_state_from_q(g, qi, c->state_var);
+      OState       gs[3];
+      o_phys_gradient(g, &s, c->state_var, gref, dXdx, gs);
+      double *out = &vals[(size_t)e * 5 * Q3];
+      out[0 * Q3 + i] = wdetJ * 0.5 * s.U[0] * o_dot3(&s.Y[1], &s.Y[1]);
+      double kmsr[6], sr[3][3];
+      o_km_strain_rate(gs, kmsr);
+      double div_u = kmsr[0] + kmsr[1] + kmsr[2];
+      o_km_unpack(kmsr, sr);
+      out[1 * Q3 + i] = wdetJ * -2 * g.mu * o_dotn(&sr[0][0], &sr[0][0], 9);
+      out[2 * Q3 + i] = wdetJ * -g.lambda * g.mu * o_sq(div_u);
+      out[3 * Q3 + i] = wdetJ * s.Y[0] * div_u;
+      double gu[3][3], w[3];
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) gu[a][b] = gs[b].Y[1 + a];
+      w[0]            = gu[2][1] - gu[1][2];
+      w[1]            = gu[0][2] - gu[2][0];
+      w[2]            = gu[1][0] - gu[0][1];
+      out[4 * Q3 + i] = wdetJ * g.mu * o_dot3(w, w);
+    }
+  }
+  for (int k = 0; k < 5; k++) sums[k] = 0;
+  for (int e = 0; e < m->nelem; e++)
+    for (int k = 0; k < 5; k++)
+      for (int i = 0; i < Q3; i++) sums[k] += vals[((size_t)e * 5 + k) * Q3 + i];
+  free(vals);
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// ICsTaylorGreen (qfunctions/taylorgreen.h:23-51) evaluated at n points xyz[n][3] -> q[n][5]
+// ----------------------------------------------------------------------------------------------------------------------
+void oracle_ic_taylorgreen(const ONewt *c, const double ref_Y[5], const double bg_u[3], const double L[3], int n, const double *xyz,
+                           double *q) {
+  const OGas   g    = c->gas;
+  const double R    = o_rgas(g);
+  const double V0   = sqrt(ref_Y[1] * ref_Y[1] + ref_Y[2] * ref_Y[2] + ref_Y[3] * ref_Y[3]);
+  const double rho0 = ref_Y[0] / (ref_Y[4] * R);
+  for (int i = 0; i < n; i++) {
+    double x[3];
+    const double scale = 2 * M_PI / L[0];  // all three directions use lx (taylorgreen.h:40)
+    for (int d = 0; d < 3; d++) x[d] = xyz[i * 3 + d] * scale;
+    double Y[5];
+    Y[0]     = ref_Y[0] + (rho0 * o_sq(V0) / 16) * (cos(2 * x[0]) + cos(2 * x[1])) * (cos(2 * x[2] + 2));
+    Y[1]     = bg_u[0] + V0 * sin(x[0]) * cos(x[1]) * cos(x[2]);
+    Y[2]     = bg_u[1] - V0 * cos(x[0]) * sin(x[1]) * cos(x[2]);
+    Y[3]     = bg_u[2];
+    Y[4]     = ref_Y[4];
+    OState s = o_state_from_q(g, Y, OVAR_PRIM);
+    o_state_to_q(g, &s, &q[i * 5], c->state_var);
+  }
+}
+
+// ----------------------------------------------------------------------------------------------------------------------
+// Pointwise entry points for unit tests
+// ----------------------------------------------------------------------------------------------------------------------
+void oracle_state_from_q(const OGas *g, const double *q, int var, double *U, double *Y) {
+  OState s = o_state_from_q(*g, q, var);
+  memcpy(U, s.U, sizeof(s.U));
+  memcpy(Y, s.Y, sizeof(s.Y));
+}
+void oracle_state_from_q_fwd(const OGas *g, const double *q, const double *dq, int var, double *dU, double *dY) {
+  OState s  = o_state_from_q(*g, q, var);
+  OState ds = o_state_from_q_fwd(*g, &s, dq, var);
+  memcpy(dU, ds.U, sizeof(ds.U));
+  memcpy(dY, ds.Y, sizeof(ds.Y));
+}
+void oracle_state_to_q(const OGas *g, const double *U, const double *Y, int var, double *q) {
+  OState s;
+  memcpy(s.U, U, sizeof(s.U));
+  memcpy(s.Y, Y, sizeof(s.Y));
+  o_state_to_q(*g, &s, q, var);
+}
+void oracle_state_to_q_fwd(const OGas *g, const double *U, const double *Y, const double *dU, const double *dY, int var, double *dq) {
+  OState s, ds;
+  memcpy(s.U, U, sizeof(s.U));
+  memcpy(s.Y, Y, sizeof(s.Y));
+  memcpy(ds.U, dU, sizeof(ds.U));
+  memcpy(ds.Y, dY, sizeof(ds.Y));
+  o_state_to_q_fwd(*g, &s, &ds, dq, var);
+}
+void oracle_tau_diag(const OGas *g, const OTauCoef *tc, const double *q, int var, const double *dXdx, double dt, double *Tau) {
+  OState s = o_state_from_q(*g, q, var);
+  double J[3][3];
+  memcpy(J, dXdx, sizeof(J));
+  o_tau_diag_prim(*tc, *g, &s, J, dt, Tau);
+}

@@ -1,0 +1,100 @@
+// CPU ORACLE -- test infrastructure only (see ns_pointwise.h).
+//
+// Public C interface of the oracle shared library liboracle.so.  The oracle
+// restates libCEED's /cpu/self operator semantics (restriction gather, tensor
+// basis apply, QFunction per quadrature point, basis transpose, scatter-add)
+// for the HONEE Newtonian operators, in plain FP64 C.
+//
+// Layouts:
+//   L-vector       double[nnodes][5]       node-major, components interleaved
+//   offsets        int[nelem][P^3]         node index of each element node (lexicographic, x fastest)
+//   qdata          double[nelem][10][Q^3]  wdetJ, dXdx[3][3] row-major (setupgeo.h:42-56)
+//   jac_data       double[nelem][njd][Q^3] q[5], strain rate (KM)[6], Tau_d[3] (+ sigma) (newtonian.h:251-254)
+//   B, D           double[Q][P]            1D interpolation / derivative on [-1, 1]
+#pragma once
+
+#include "ns_pointwise.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int           nelem, P, Q, Px;
+  int           nnodes, nxnodes;
+  const int    *offsets;   // [nelem][P^3]
+  const int    *xoffsets;  // [nelem][Px^3]
+  const double *xcoord;    // [nxnodes][3]
+  const double *B, *D;     // [Q][P]
+  const double *Bx, *Dx;   // [Q][Px]
+  const double *w1d;       // [Q]
+} OMesh;
+
+// Flat view of struct NewtonianIdealGasContext_ (qfunctions/newtonian_types.h:29-47)
+typedef struct {
+  OGas     gas;
+  double   g[3];
+  double   time, ijacobian_time_shift;
+  double   idl_pressure, idl_amplitude, idl_start, idl_length;
+  OTauCoef tau;
+  double   dt;
+  int      is_implicit, state_var, stabilization, idl_enable;
+} ONewt;
+
+// Freestream context: Newtonian context plus the exterior state (bc_freestream_type.h:9-12)
+typedef struct {
+  ONewt  newt;
+  double S_inf_U[5], S_inf_Y[5];
+  int    riemann;  // 0 HLL, 1 HLLC
+} OFreestream;
+
+// Boundary faces of a hex mesh: each face is described by its volume element and local face id
+// (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z).
+typedef struct {
+  int        nfaces;
+  const int *face_elem;
+  const int *face_id;
+} OFaces;
+
+int    oracle_setup_qdata(const OMesh *m, double *qdata);
+int    oracle_ifunction(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v, double *jac);
+int    oracle_ijacobian(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, const double *dq, double *v);
+int    oracle_ijacobian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, double *diag);
+int    oracle_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *v);
+int    oracle_mass_newtonian(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v);
+int    oracle_mass_newtonian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *diag);
+int    oracle_monitor_ke(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double sums[5]);
+void   oracle_ic_taylorgreen(const ONewt *c, const double ref_Y[5], const double bg_u[3], const double L[3], int n, const double *xyz, double *q);
+double oracle_volume(const OMesh *m, const double *qdata);
+
+// Explicit time stepping of the RHS with the SUPG-stabilized consistent mass (problems/newtonian.c:212-265) solved by
+// left-Jacobi-preconditioned GMRES; used to reproduce the reference's Taylor-Green kinetic-energy fixture.
+int oracle_rk4_stabilized(const OMesh *m, ONewt *c, const double *qdata, double *q, double dt, int nsteps, double ksp_rtol, int ksp_restart,
+                          int record_every, double *ke_rows /* [(nsteps/record_every+2)][7] */, int *nrows);
+
+// Faces
+int oracle_setup_qdata_boundary(const OMesh *m, const OFaces *f, double *qdata_sur);
+int oracle_freestream(const OMesh *m, const OFaces *f, const OFreestream *c, const double *qdata_sur, const double *q, double *v, double *jac_sur);
+int oracle_freestream_jacobian(const OMesh *m, const OFaces *f, const OFreestream *c, const double *qdata_sur, const double *jac_sur,
+                               const double *dq, double *v);
+int oracle_boundary_integral(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *q, double *v,
+                             double *jac_sur);
+int oracle_boundary_integral_jacobian(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *jac_sur,
+                                      const double *dq, double *v);
+
+// Pointwise entry points used by the unit tests
+void oracle_state_from_q(const OGas *g, const double *q, int var, double *U, double *Y);
+void oracle_state_from_q_fwd(const OGas *g, const double *q, const double *dq, int var, double *dU, double *dY);
+void oracle_state_to_q(const OGas *g, const double *U, const double *Y, int var, double *q);
+void oracle_state_to_q_fwd(const OGas *g, const double *U, const double *Y, const double *dU, const double *dY, int var, double *dq);
+void oracle_riemann_flux(const OGas *g, int hllc, const double *qL, const double *qR, int var, const double *n, double *F);
+void oracle_riemann_flux_fwd(const OGas *g, int hllc, const double *qL, const double *dqL, const double *qR, const double *dqR, int var,
+                             const double *n, double *dF);
+void oracle_hll_speeds(const OGas *g, const double *qL, const double *qR, int var, const double *n, double *sL, double *sR);
+void oracle_hll_speeds_fwd(const OGas *g, const double *qL, const double *dqL, const double *qR, const double *dqR, int var, const double *n,
+                           double *sL, double *dsL, double *sR, double *dsR);
+void oracle_tau_diag(const OGas *g, const OTauCoef *tc, const double *q, int var, const double *dXdx, double dt, double *Tau);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,233 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- test infrastructure only.
+
+Only the tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` load this module; the product
+path never does.  See honee_oracle.h for layouts.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+
+
+class OGas(ct.Structure):
+    _fields_ = [("lambda_", ct.c_double), ("mu", ct.c_double), ("k", ct.c_double), ("cv", ct.c_double), ("cp", ct.c_double)]
+
+
+class OTauCoef(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in ("Ctau_t", "Ctau_v", "Ctau_C", "Ctau_M", "Ctau_E")]
+
+
+class ONewt(ct.Structure):
+    _fields_ = [
+        ("gas", OGas),
+        ("g", ct.c_double * 3),
+        ("time", ct.c_double),
+        ("ijacobian_time_shift", ct.c_double),
+        ("idl_pressure", ct.c_double),
+        ("idl_amplitude", ct.c_double),
+        ("idl_start", ct.c_double),
+        ("idl_length", ct.c_double),
+        ("tau", OTauCoef),
+        ("dt", ct.c_double),
+        ("is_implicit", ct.c_int),
+        ("state_var", ct.c_int),
+        ("stabilization", ct.c_int),
+        ("idl_enable", ct.c_int),
+    ]
+
+
+class OMesh(ct.Structure):
+    _fields_ = [
+        ("nelem", ct.c_int),
+        ("P", ct.c_int),
+        ("Q", ct.c_int),
+        ("Px", ct.c_int),
+        ("nnodes", ct.c_int),
+        ("nxnodes", ct.c_int),
+        ("offsets", _ip),
+        ("xoffsets", _ip),
+        ("xcoord", _dp),
+        ("B", _dp),
+        ("D", _dp),
+        ("Bx", _dp),
+        ("Dx", _dp),
+        ("w1d", _dp),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ct.CDLL(LIB_PATH)
+        _lib.oracle_volume.restype = ct.c_double
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Mesh:
+    """Keeps the numpy arrays alive and exposes the OMesh struct."""
+
+    def __init__(self, mesh, tab):
+        self.mesh, self.tab = mesh, tab
+        self._arrs = dict(
+            offsets=np.ascontiguousarray(mesh.offsets, dtype=np.int32),
+            xoffsets=np.ascontiguousarray(mesh.xoffsets, dtype=np.int32),
+            xcoord=np.ascontiguousarray(mesh.xcoord, dtype=np.float64),
+            B=np.ascontiguousarray(tab.B),
+            D=np.ascontiguousarray(tab.D),
+            Bx=np.ascontiguousarray(tab.Bx),
+            Dx=np.ascontiguousarray(tab.Dx),
+            w1d=np.ascontiguousarray(tab.qweight),
+        )
+        a = self._arrs
+        self.s = OMesh(
+            mesh.nelem, tab.P, tab.Q, 2, mesh.nnodes, a["xcoord"].shape[0],
+            _i(a["offsets"]), _i(a["xoffsets"]), _d(a["xcoord"]), _d(a["B"]), _d(a["D"]), _d(a["Bx"]), _d(a["Dx"]), _d(a["w1d"]),
+        )
+        self.nelem, self.Q3, self.nnodes = mesh.nelem, tab.Q**3, mesh.nnodes
+
+    def setup_qdata(self):
+        qd = np.zeros((self.nelem, 10, self.Q3))
+        lib().oracle_setup_qdata(ct.byref(self.s), _d(qd))
+        return qd
+
+
+def newt_from_dict(d: dict) -> ONewt:
+    c = ONewt()
+    g = d["gas"]
+    c.gas = OGas(g["lambda"], g["mu"], g["k"], g["cv"], g["cp"])
+    for i in range(3):
+        c.g[i] = d.get("g", (0, 0, 0))[i]
+    c.time = d.get("time", 0.0)
+    c.ijacobian_time_shift = d.get("ijacobian_time_shift", 0.0)
+    c.idl_pressure = d.get("idl_pressure", 0.0)
+    c.idl_amplitude = d.get("idl_amplitude", 0.0)
+    c.idl_start = d.get("idl_start", 0.0)
+    c.idl_length = d.get("idl_length", 0.0)
+    t = d["tau"]
+    c.tau = OTauCoef(t["Ctau_t"], t["Ctau_v"], t["Ctau_C"], t["Ctau_M"], t["Ctau_E"])
+    c.dt = d.get("dt", 1.0)
+    c.is_implicit = int(d.get("is_implicit", 1))
+    c.state_var = int(d.get("state_var", 0))
+    c.stabilization = int(d.get("stabilization", 0))
+    c.idl_enable = int(d.get("idl_enable", 0))
+    return c
+
+
+def njac(c: ONewt) -> int:
+    return 14 + (1 if c.idl_enable else 0)
+
+
+def ifunction(m: Mesh, c: ONewt, qdata, q, qdot):
+    v = np.zeros(5 * m.nnodes)
+    jac = np.zeros((m.nelem, njac(c), m.Q3))
+    lib().oracle_ifunction(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(qdot), _d(v), _d(jac))
+    return v, jac
+
+
+def ijacobian(m: Mesh, c: ONewt, qdata, jac, dq):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_ijacobian(ct.byref(m.s), ct.byref(c), _d(qdata), _d(jac), _d(dq), _d(v))
+    return v
+
+
+def ijacobian_diagonal(m: Mesh, c: ONewt, qdata, jac):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_ijacobian_diagonal(ct.byref(m.s), ct.byref(c), _d(qdata), _d(jac), _d(v))
+    return v
+
+
+def rhs(m: Mesh, c: ONewt, qdata, q):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_rhs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(v))
+    return v
+
+
+def mass_newtonian(m: Mesh, c: ONewt, qdata, q, qdot):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_mass_newtonian(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(qdot), _d(v))
+    return v
+
+
+def monitor_ke(m: Mesh, c: ONewt, qdata, q):
+    s = np.zeros(5)
+    lib().oracle_monitor_ke(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(s))
+    return s
+
+
+def volume(m: Mesh, qdata) -> float:
+    return lib().oracle_volume(ct.byref(m.s), _d(qdata))
+
+
+def ic_taylorgreen(c: ONewt, ref_Y, bg_u, L, xyz):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    q = np.zeros((xyz.shape[0], 5))
+    lib().oracle_ic_taylorgreen(ct.byref(c), _d(np.asarray(ref_Y, float)), _d(np.asarray(bg_u, float)), _d(np.asarray(L, float)),
+                                xyz.shape[0], _d(xyz), _d(q))
+    return q
+
+
+def rk4_stabilized(m: Mesh, c: ONewt, qdata, q, dt, nsteps, rtol=1e-5, restart=30, record_every=1):
+    q = np.array(q, dtype=np.float64)
+    rows = np.zeros((nsteps + 2, 7))
+    nrows = ct.c_int(0)
+    its = lib().oracle_rk4_stabilized(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(dt), nsteps, ct.c_double(rtol), restart,
+                                      record_every, _d(rows), ct.byref(nrows))
+    return q, rows[: nrows.value], its
+
+
+def state_from_q(gas: OGas, q, var):
+    U, Y = np.zeros(5), np.zeros(5)
+    lib().oracle_state_from_q(ct.byref(gas), _d(np.asarray(q, float)), var, _d(U), _d(Y))
+    return U, Y
+
+
+def state_from_q_fwd(gas: OGas, q, dq, var):
+    dU, dY = np.zeros(5), np.zeros(5)
+    lib().oracle_state_from_q_fwd(ct.byref(gas), _d(np.asarray(q, float)), _d(np.asarray(dq, float)), var, _d(dU), _d(dY))
+    return dU, dY
+
+
+def state_to_q(gas: OGas, U, Y, var):
+    q = np.zeros(5)
+    lib().oracle_state_to_q(ct.byref(gas), _d(np.asarray(U, float)), _d(np.asarray(Y, float)), var, _d(q))
+    return q
+
+
+def state_to_q_fwd(gas: OGas, U, Y, dU, dY, var):
+    dq = np.zeros(5)
+    lib().oracle_state_to_q_fwd(ct.byref(gas), _d(np.asarray(U, float)), _d(np.asarray(Y, float)), _d(np.asarray(dU, float)),
+                                _d(np.asarray(dY, float)), var, _d(dq))
+    return dq
+
+
+def tau_diag(gas: OGas, tc: OTauCoef, q, var, dXdx, dt):
+    T = np.zeros(3)
+    lib().oracle_tau_diag(ct.byref(gas), ct.byref(tc), _d(np.asarray(q, float)), var, _d(np.ascontiguousarray(dXdx, float)),
+                          ct.c_double(dt), _d(T))
+    return T
