@@ -618,3 +618,22 @@ void oracle_tau_diag(const OGas *g, const OTauCoef *tc, const double *q, int var
   memcpy(J, dXdx, sizeof(J));
   o_tau_diag_prim(*tc, *g, &s, J, dt, Tau);
 }
+
+// Pointwise IFunction / IJacobian QFunction bodies (one quadrature point), for the oracle's own consistency tests.
+void oracle_qf_ifunction(const ONewt *c, const double *q, const double *gref, const double *qdot, double wdetJ, const double *dXdx,
+                         const double *x, double *v, double *gv, double *jd) {
+  double G[3][5], J[3][3], GV[3][5];
+  memcpy(G, gref, sizeof(G));
+  memcpy(J, dXdx, sizeof(J));
+  qf_ifunction(c, q, G, qdot, wdetJ, J, x, v, GV, jd);
+  memcpy(gv, GV, sizeof(GV));
+}
+
+void oracle_qf_ijacobian(const ONewt *c, const double *dq, const double *gref, double wdetJ, const double *dXdx, const double *jd, double *v,
+                         double *gv) {
+  double G[3][5], J[3][3], GV[3][5];
+  memcpy(G, gref, sizeof(G));
+  memcpy(J, dXdx, sizeof(J));
+  qf_ijacobian(c, dq, G, wdetJ, J, jd, v, GV);
+  memcpy(gv, GV, sizeof(GV));
+}

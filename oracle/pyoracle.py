@@ -231,3 +231,17 @@ def tau_diag(gas: OGas, tc: OTauCoef, q, var, dXdx, dt):
     lib().oracle_tau_diag(ct.byref(gas), ct.byref(tc), _d(np.asarray(q, float)), var, _d(np.ascontiguousarray(dXdx, float)),
                           ct.c_double(dt), _d(T))
     return T
+
+
+def qf_ifunction(c: ONewt, q, gref, qdot, wdetJ, dXdx, x=(0.0, 0.0, 0.0)):
+    v, gv, jd = np.zeros(5), np.zeros(15), np.zeros(15)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    lib().oracle_qf_ifunction(ct.byref(c), _d(a(q)), _d(a(gref)), _d(a(qdot)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(x)), _d(v), _d(gv), _d(jd))
+    return v, gv.reshape(3, 5), jd
+
+
+def qf_ijacobian(c: ONewt, dq, gref, wdetJ, dXdx, jd):
+    v, gv = np.zeros(5), np.zeros(15)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    lib().oracle_qf_ijacobian(ct.byref(c), _d(a(dq)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(jd)), _d(v), _d(gv))
+    return v, gv.reshape(3, 5)
