@@ -173,6 +173,85 @@ static int plan_bind(CeedOperator op) {
       p->kernel_name = op->ceed->jac_fp32 ? "ijacobian_newtonian_fp32" : "ijacobian_newtonian";
       return 0;
     }
+    case QfKind::RHSFunction: {  // newtonian.h:133-202 wired by src/setuplibceed.c:176-203
+      const char  *in_names[4] = {"q", "Grad_q", "qdata", "x"};
+      CeedEvalMode in_modes[4] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_NONE, CEED_EVAL_INTERP};
+      CeedInt      in_sizes[4] = {5, 15, 10, 3};
+      if (qf->inputs.size() != 4)
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: only div_diff_flux_projection_method none is supported", qf->name.c_str()));
+      for (int i = 0; i < 4; i++)
+        if ((ierr = check_field(op, i, true, in_names[i], in_modes[i], in_sizes[i]))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "v", CEED_EVAL_INTERP, 5))) return fail(ierr);
+      if ((ierr = check_field(op, 1, false, "Grad_v", CEED_EVAL_GRAD, 15))) return fail(ierr);
+      auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+      if ((ierr = bind_solution_basis(op, p, fi[0]))) return fail(ierr);
+      if (!(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: q and Grad_q must share the active restriction/basis", qf->name.c_str()));
+      if (!is_strided(fi[2]->rstr) || is_active(fi[2]->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: qdata must be a passive strided field", qf->name.c_str()));
+      p->r_qd    = fi[2]->rstr;
+      p->v_qdata = fi[2]->vec;
+      p->qd      = strided_layout(p->r_qd);
+      if ((ierr = bind_coords(op, p, fi[3]))) return fail(ierr);
+      if (!(is_active(fo[0]->vec) && is_active(fo[1]->vec) && fo[0]->rstr == p->r_q && fo[1]->rstr == p->r_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: v and Grad_v must be active on the solution space", qf->name.c_str()));
+      p->kernel_name = "rhs_newtonian";
+      return 0;
+    }
+    case QfKind::MassNewtonian:  // newtonian.h:44-82 wired by problems/newtonian.c:240-256
+    case QfKind::Mass5: {        // mass.h Mass_N wired by src/setuplibceed.c:36-45
+      const bool stab = qf->kind == QfKind::MassNewtonian;
+      if (stab) {
+        if ((ierr = check_field(op, 0, true, "q_dot", CEED_EVAL_INTERP, 5))) return fail(ierr);
+        if ((ierr = check_field(op, 1, true, "q", CEED_EVAL_INTERP, 5))) return fail(ierr);
+        if ((ierr = check_field(op, 2, true, "qdata", CEED_EVAL_NONE, -1))) return fail(ierr);
+        if ((ierr = check_field(op, 1, false, "Grad_v", CEED_EVAL_GRAD, 15))) return fail(ierr);
+      } else {
+        if ((ierr = check_field(op, 0, true, "u", CEED_EVAL_INTERP, 5))) return fail(ierr);
+        if ((ierr = check_field(op, 1, true, "qdata", CEED_EVAL_NONE, -1))) return fail(ierr);
+      }
+      if ((ierr = check_field(op, 0, false, "v", CEED_EVAL_INTERP, 5))) return fail(ierr);
+      auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+      if ((ierr = bind_solution_basis(op, p, fi[0]))) return fail(ierr);
+      if (!is_active(fi[0]->vec)) return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: first input must be active", qf->name.c_str()));
+      if (stab) {
+        if (!(fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q && !is_active(fi[1]->vec)))
+          return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: 'q' must be a passive field on the solution space", qf->name.c_str()));
+        p->v_qdot = fi[1]->vec;  // the state the stabilization is evaluated at
+      }
+      auto *fqd = fi[stab ? 2 : 1];
+      if (!is_strided(fqd->rstr) || is_active(fqd->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: qdata must be a passive strided field", qf->name.c_str()));
+      p->r_qd    = fqd->rstr;
+      p->v_qdata = fqd->vec;
+      p->qd      = strided_layout(p->r_qd);
+      for (auto *f : op->out_fields)
+        if (!(is_active(f->vec) && f->rstr == p->r_q))
+          return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: outputs must be active on the solution space", qf->name.c_str()));
+      p->kernel_name = stab ? "mass_newtonian" : "mass";
+      return 0;
+    }
+    case QfKind::MonitorKE: {  // monitor_totalkineticenergy.h wired by src/monitor_totalkineticenergy.c:95-104
+      if ((ierr = check_field(op, 0, true, "q", CEED_EVAL_INTERP, 5))) return fail(ierr);
+      if ((ierr = check_field(op, 1, true, "Grad_q", CEED_EVAL_GRAD, 15))) return fail(ierr);
+      if ((ierr = check_field(op, 2, true, "q_data", CEED_EVAL_NONE, 10))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "v", CEED_EVAL_NONE, 5))) return fail(ierr);
+      auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+      if ((ierr = bind_solution_basis(op, p, fi[0]))) return fail(ierr);
+      if (!(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: q and Grad_q must be active", qf->name.c_str()));
+      if (!is_strided(fi[2]->rstr) || is_active(fi[2]->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: q_data must be a passive strided field", qf->name.c_str()));
+      p->r_qd    = fi[2]->rstr;
+      p->v_qdata = fi[2]->vec;
+      p->qd      = strided_layout(p->r_qd);
+      if (!is_strided(fo[0]->rstr) || !is_active(fo[0]->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: output must be an active strided field", qf->name.c_str()));
+      p->r_jd        = fo[0]->rstr;
+      p->jd          = strided_layout(p->r_jd);
+      p->kernel_name = "monitor_total_ke";
+      return 0;
+    }
     default:
       break;
   }
@@ -232,7 +311,8 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
       a.qd = p->qd;
       a.jd = p->jd;
-      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo};
+      const int PQ = p->P * p->Q;
+      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
       if (p->kind == QfKind::IFunction) {
         const int need = 14 + (ctx.idl_enable ? 1 : 0);
         if (p->v_jac && p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
@@ -258,11 +338,93 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       }
       return 0;
     }
+    case QfKind::RHSFunction:
+    case QfKind::MassNewtonian:
+    case QfKind::Mass5:
+    case QfKind::MonitorKE: {
+      CeedMiNewtonianContext ctx{};
+      if (p->kind != QfKind::Mass5) MI_CALL(newtonian_context(op, &ctx));
+      if (in->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: input must have the L-vector size", op->qf->name.c_str());
+      VolumeArgs a{};
+      a.nelem = p->nelem;
+      MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+      a.comp_stride = p->r_q->comp_stride;
+      MI_CALL(vec_device_read(in, &a.u));
+      MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+      a.qd         = p->qd;
+      const int PQ = p->P * p->Q;
+      a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+      if (p->kind == QfKind::MonitorKE) {
+        if (out->length < p->r_jd->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "monitor output too short");
+        MI_CALL(vec_device_write(out, &a.jac, true));
+        a.jd = p->jd;
+      } else {
+        if (out->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: output must have the L-vector size", op->qf->name.c_str());
+        MI_CALL(vec_device_write(out, &a.v, true));
+      }
+      if (p->kind == QfKind::RHSFunction) {
+        MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
+        MI_CALL(vec_device_read(p->v_x, &a.x));
+        a.Bx = p->b_x->d_interp_1d;
+      }
+      if (p->kind == QfKind::MassNewtonian) MI_CALL(vec_device_read(p->v_qdot, &a.u2));
+      MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+      if (p->kind == QfKind::RHSFunction) MI_HIP(ceed, launch_rhs(p->P, p->Q, a, ctx, s));
+      else if (p->kind == QfKind::MonitorKE) MI_HIP(ceed, launch_monitor_ke(p->P, p->Q, p->var, a, ctx, s));
+      else MI_HIP(ceed, launch_mass(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, s));
+      MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      return 0;
+    }
     default:
       return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "operator kind not executable");
   }
 }
 
+// Point-Jacobi diagonal (MatGetDiagonal_Ceed -> CeedOperatorLinearAssembleDiagonal, src/mat-ceed.c:1603-1630)
 int op_plan_diagonal(CeedOperator op, CeedVector diag) {
-  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly of '%s' is not provided", op->qf->name.c_str());
+  if (!op->plan) MI_CALL(plan_bind(op));
+  OpPlan *p    = op->plan;
+  Ceed    ceed = op->ceed;
+  if (p->kind == QfKind::MassNewtonian || p->kind == QfKind::Mass5) {  // mass matrices for the explicit path
+    CeedMiNewtonianContext ctx{};
+    if (p->kind == QfKind::MassNewtonian) MI_CALL(newtonian_context(op, &ctx));
+    if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+    VolumeArgs a{};
+    a.nelem = p->nelem;
+    MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+    a.comp_stride = p->r_q->comp_stride;
+    MI_CALL(vec_device_write(diag, &a.v, true));
+    MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+    if (p->kind == QfKind::MassNewtonian) MI_CALL(vec_device_read(p->v_qdot, &a.u2));
+    a.qd         = p->qd;
+    const int PQ = p->P * p->Q;
+    a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+    hipEvent_t ev;
+    MI_CALL(timing_begin(ceed, "mass_diagonal", &ev));
+    MI_HIP(ceed, launch_mass_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
+    MI_CALL(timing_end(ceed, "mass_diagonal", ev));
+    return 0;
+  }
+  if (p->kind != QfKind::IJacobian)
+    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly of '%s' is not provided", op->qf->name.c_str());
+  CeedMiNewtonianContext ctx;
+  MI_CALL(newtonian_context(op, &ctx));
+  if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+  VolumeArgs a{};
+  a.nelem = p->nelem;
+  MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+  a.comp_stride = p->r_q->comp_stride;
+  MI_CALL(vec_device_write(diag, &a.v, true));
+  MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+  a.qd         = p->qd;
+  a.jd         = p->jd;
+  const int PQ = p->P * p->Q;
+  a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+  if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, false));
+  else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
+  hipEvent_t ev;
+  MI_CALL(timing_begin(ceed, "ijacobian_diagonal", &ev));
+  MI_HIP(ceed, launch_ijacobian_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
+  MI_CALL(timing_end(ceed, "ijacobian_diagonal", ev));
+  return 0;
 }

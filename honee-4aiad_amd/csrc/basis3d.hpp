@@ -128,6 +128,37 @@ struct Elem {
     __syncthreads();
     return r;
   }
+
+  // Node value sum_q Fz[qz][c] Fy[qy][b] Fx[qx][a] w(q) for node t = (a, b, c), with arbitrary 1D matrices [Q][P]
+  // (used by diagonal assembly with products of B and G).  Every thread passes its quadrature-point value w.
+  __device__ double contract_t(double w, const double *Fx, const double *Fy, const double *Fz) {
+    __syncthreads();
+    s0[t] = w;
+    __syncthreads();
+    if (t < D::PQQ) {
+      const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
+      double    acc = 0;
+#pragma unroll
+      for (int z = 0; z < Q; z++) acc += Fz[z * P + c] * s0[(z * Q + iy) * Q + ix];
+      s1[t] = acc;
+    }
+    __syncthreads();
+    if (t < D::PPQ) {
+      const int ix = t % Q, b = (t / Q) % P, c = t / (Q * P);
+      double    acc = 0;
+#pragma unroll
+      for (int y = 0; y < Q; y++) acc += Fy[y * P + b] * s1[(c * Q + y) * Q + ix];
+      s2[t] = acc;
+    }
+    __syncthreads();
+    double r = 0;
+    if (t < D::P3) {
+      const int a = t % P, cb = t / P;
+#pragma unroll
+      for (int x = 0; x < Q; x++) r += Fx[x * P + a] * s2[cb * Q + x];
+    }
+    return r;
+  }
 };
 
 }  // namespace basis3

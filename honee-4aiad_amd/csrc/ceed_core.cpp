@@ -894,6 +894,14 @@ int basis_device_tables(CeedBasis b) {
   std::vector<double> w = b->q_weight_1d;
   w.resize(Q, 0.0);
   MI_HIP(b->ceed, hipMemcpy(b->d_weight_1d, w.data(), sizeof(double) * Q, hipMemcpyHostToDevice));
+  std::vector<double> prod(3 * Q * P);
+  for (int i = 0; i < Q * P; i++) {
+    prod[i]             = b->interp_1d[i] * b->interp_1d[i];
+    prod[Q * P + i]     = b->grad_1d[i] * b->interp_1d[i];
+    prod[2 * Q * P + i] = b->grad_1d[i] * b->grad_1d[i];
+  }
+  MI_HIP(b->ceed, hipMalloc((void **)&b->d_prod, sizeof(double) * 3 * Q * P));
+  MI_HIP(b->ceed, hipMemcpy(b->d_prod, prod.data(), sizeof(double) * 3 * Q * P, hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -946,6 +954,7 @@ extern "C" int CeedBasisDestroy(CeedBasis *basis) {
   if (b->d_grad_1d) hipFree(b->d_grad_1d);
   if (b->d_collo) hipFree(b->d_collo);
   if (b->d_weight_1d) hipFree(b->d_weight_1d);
+  if (b->d_prod) hipFree(b->d_prod);
   Ceed c = b->ceed;
   delete b;
   CeedDestroy(&c);
@@ -972,6 +981,15 @@ static void qf_identify(CeedQFunction qf) {
     qf->state_var = var_of(n);
   } else if (n.rfind("IJacobian_Newtonian_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::IJacobian;
+    qf->state_var = var_of(n);
+  } else if (n == "RHSFunction_Newtonian") {
+    qf->kind = QfKind::RHSFunction;
+  } else if (n == "MassFunction_Newtonian_Conserv") {
+    qf->kind = QfKind::MassNewtonian;
+  } else if (n == "Mass_5") {
+    qf->kind = QfKind::Mass5;
+  } else if (n.rfind("MonitorTotalKineticEnergy_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::MonitorKE;
     qf->state_var = var_of(n);
   }
 }

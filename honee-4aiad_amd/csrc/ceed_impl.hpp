@@ -66,6 +66,7 @@ struct CeedBasis_private {
   std::vector<double> interp_1d, grad_1d, q_ref_1d, q_weight_1d, collo_grad_1d;
   std::vector<double> interp, grad, q_ref, q_weight;  // non-tensor bases
   double             *d_interp_1d = nullptr, *d_grad_1d = nullptr, *d_collo = nullptr, *d_weight_1d = nullptr;
+  double             *d_prod = nullptr;  // [3][Q][P]: B.B, G.B, G.G
   bool                is_projection = false;
 };
 
@@ -81,6 +82,10 @@ enum class QfKind {
   Setup,
   IFunction,
   IJacobian,
+  RHSFunction,
+  MassNewtonian,
+  Mass5,
+  MonitorKE,
 };
 
 struct CeedQFunction_private {

@@ -229,6 +229,16 @@ int HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const do
   return 0;
 }
 
+// Point-Jacobi diagonal of the shifted IJacobian (MatGetDiagonal_Ceed, src/mat-ceed.c:1603-1630)
+int HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, double *diag) {
+  CeedScalar *arr;
+  if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, diag));
+  CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_ijacobian, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
 // Copy backend-strided quadrature data to host as [elem][comp][qpt] (for checking against the CPU oracle)
 static int CopyStrided(HoneeMiProblem p, CeedVector v, CeedElemRestriction r, int ncomp, double *out) {
   const CeedScalar *a;

@@ -31,6 +31,7 @@ Ceed         HoneeMiProblemGetCeed(HoneeMiProblem p);
 CeedOperator HoneeMiProblemGetOperator(HoneeMiProblem p, int which);  // 0: IFunction composite, 1: IJacobian composite
 int          HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g);
 int          HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const double *dq, double *v);
+int          HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, double *diag);
 int          HoneeMiGetQData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData32(HoneeMiProblem p, float *out);

@@ -14,6 +14,7 @@ struct QDataLayout {
 struct BasisTables {
   const double *B;
   const double *Dc;
+  const double *BB, *GB, *GG;  // elementwise products B.B, G.B, G.G of the 1D tables (diagonal assembly)
 };
 
 struct SetupArgs {
@@ -50,6 +51,11 @@ hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t str
 bool       volume_pq_supported(int P, int Q);
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_ijacobian_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_rhs(int P, int Q, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_mass(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+hipError_t launch_monitor_ke(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 
 // Vector kernels (kernels_vector.hip)
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);

@@ -1,15 +1,20 @@
 // Fused element kernels for HONEE's Newtonian volume operators on MI355X (gfx950).
 //
-// One kernel launch applies a whole volume sub-operator: per element it gathers the L-vector through the element
-// restriction, interpolates values and reference gradients to the quadrature points in LDS, evaluates the QFunction
-// in registers, applies the basis transpose and scatter-adds into the output L-vector.  Quadrature data (qdata,
-// jac_data) is streamed in the backend's strided layout.  Each element owns Q^3 threads (one per quadrature point) and
-// a block holds EPB elements; blocks loop over element batches with a grid stride.
+// k_volume<P, Q, Op> applies a whole volume sub-operator in one launch: per element it gathers the L-vector through
+// the element restriction, interpolates values and reference gradients to the quadrature points in LDS, evaluates the
+// operator's quadrature-point body (newtonian_ops.hpp) in registers, applies the basis transpose and scatter-adds into
+// the output L-vector.  Quadrature data (qdata, jac_data) is streamed in the backend's strided layout.  Each element
+// owns Q^3 threads (one per quadrature point) and a block holds EPB elements; blocks loop over element batches.
+//
+// k_diagonal<P, Q, Op> assembles the diagonal of a linear operator (CeedOperatorLinearAssembleDiagonal): at each
+// quadrature point the 4x4 block coupling {value, d/dX0, d/dX1, d/dX2} of a component to itself is obtained by applying
+// the linear quadrature-point action to unit inputs, then contracted against products of the 1D tables
+// (B.B, G.B, G.G) with sum factorization.
 #include <hip/hip_runtime.h>
 
 #include "basis3d.hpp"
 #include "kernels.hpp"
-#include "ns_physics.hpp"
+#include "newtonian_ops.hpp"
 
 namespace {
 
@@ -21,17 +26,6 @@ constexpr int elems_per_block() {
   return (Q * Q * Q >= 64) ? 1 : 64 / (Q * Q * Q);
 }
 
-// Strided access to backend quadrature data
-struct StridedQ {
-  const double *p;
-  float        *p32w;
-  double       *pw;
-  long long     sn, sc, se;
-};
-
-__device__ __forceinline__ long long qidx(const QDataLayout &l, int e, int c, int i) { return (long long)i * l.sn + (long long)c * l.sc + (long long)e * l.se; }
-
-// Load block-shared basis tables into LDS
 template <int P, int Q>
 __device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc) {
   for (int i = threadIdx.x; i < Q * P; i += blockDim.x) sB[i] = bt.B[i];
@@ -39,12 +33,12 @@ __device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, d
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Setup: qdata = {w detJ, dXdx} from the coordinate field (qfunctions/setupgeo.h:42-56).  The coordinate basis
-// (Px nodes) is applied with B and its (non-collocated) derivative directly: coordinates are trilinear.
+// Setup: qdata = {w detJ, dXdx} from the coordinate field (qfunctions/setupgeo.h:42-56).  The trilinear coordinate
+// basis is applied directly with its 1D interpolation and derivative tables.
 // ------------------------------------------------------------------------------------------------------------------
 template <int PX, int Q>
 __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
-  constexpr int Q3 = Q * Q * Q;
+  constexpr int   Q3  = Q * Q * Q;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)a.nelem * Q3) return;
   const int e = (int)(gid / Q3), i = (int)(gid % Q3);
@@ -82,173 +76,25 @@ __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
   const double d = J[0][0] * Ai[0][0] + J[1][0] * Ai[0][1] + J[2][0] * Ai[0][2];
   const double s = 1 / d;
   const double w = a.w1d[qx] * a.w1d[qy] * a.w1d[qz];
-  a.qdata[qidx(a.qd, e, 0, i)] = w * d;
+  a.qdata[nsop::qidx(a.qd, e, 0, i)] = w * d;
 #pragma unroll
-  for (int k = 0; k < 9; k++) a.qdata[qidx(a.qd, e, 1 + k, i)] = (&Ai[0][0])[k] * s;
+  for (int k = 0; k < 9; k++) a.qdata[nsop::qidx(a.qd, e, 1 + k, i)] = (&Ai[0][0])[k] * s;
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// IFunction_Newtonian (qfunctions/newtonian.h:205-257) with IFunction_Newtonian_Integrand
-// (newtonian_integrands.h:87-115) and the internal damping layer (newtonian_integrands.h:40-48).
+// Generic fused volume kernel
 // ------------------------------------------------------------------------------------------------------------------
-template <int P, int Q, int VAR>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_ifunction(VolumeArgs a, const CeedMiNewtonianContext ctx) {
-  using Dm           = Dims<P, Q>;
-  constexpr int EPB  = elems_per_block<Q>();
-  constexpr int Q3   = Dm::Q3;
-  __shared__ double sB[Q * P], sDc[Q * Q];
-  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
-  load_tables<P, Q>(a.bt, sB, sDc);
-  const int      le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>     el(sB, sDc, s0[le], s1[le], s2[le], t);
-  const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
-  const long long nbatch  = ((long long)a.nelem + EPB - 1) / EPB;
-
-  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
-    const long long e_  = bt * EPB + le;
-    const bool      ok  = e_ < a.nelem;
-    const int       e   = ok ? (int)e_ : 0;
-    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
-
-    double q[5], gref[3][5], qdot[5];
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-      const double u = (t < Dm::P3) ? a.u[off + c * a.comp_stride] : 0.0;
-      q[c]           = el.interp(u);
-      double gg[3];
-      el.grad(q[c], gg);
-      gref[0][c] = gg[0];
-      gref[1][c] = gg[1];
-      gref[2][c] = gg[2];
-    }
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-      const double u = (t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0;
-      qdot[c]        = el.interp(u);
-    }
-    double x0 = 0;
-    if (ctx.idl_enable) {  // x coordinate at the quadrature point (trilinear coordinate field)
-      constexpr int PX = 2;
-      const int     qx = t % Q, qy = (t / Q) % Q, qz = t / (Q * Q);
-#pragma unroll
-      for (int n = 0; n < 8; n++) {
-        const int xo = a.xoffsets[(long long)e * 8 + n];
-        x0 += a.Bx[qx * PX + (n & 1)] * a.Bx[qy * PX + ((n >> 1) & 1)] * a.Bx[qz * PX + (n >> 2)] * a.x[xo];
-      }
-    }
-
-    // ---- QFunction at quadrature point t
-    double wdetJ, dXdx[3][3];
-    wdetJ = ok ? a.qdata[qidx(a.qd, e, 0, t)] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = ok ? a.qdata[qidx(a.qd, e, 1 + k, t)] : 0.0;
-    if (!ok) {
-      wdetJ = 0;
-      for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (k % 4 == 0) ? 1.0 : 0.0;
-      for (int c = 0; c < 5; c++) {  // a harmless physical state for padding threads
-        q[c] = (VAR == ns::VAR_ENTROPY) ? (c == 4 ? -1.0 : 0.0) : 1.0;
-        qdot[c] = 0;
-        for (int d = 0; d < 3; d++) gref[d][c] = 0;
-      }
-    }
-    const ns::St<double> s  = ns::state_from_q<VAR>(g, q);
-    const ns::St<double> sd = ns::state_from_q_fwd<VAR>(g, s, qdot);
-    ns::St<double>       gs[3];
-    ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);
-
-    double sr[6], st[3][3];
-    ns::km_strain_rate(gs, sr);
-    ns::stress_from_strain(g, sr, st);
-    double Fe[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gs[i].Y[4];
-    double gv[5][3];  // grad_v_i[var][dir]
-#pragma unroll
-    for (int j = 0; j < 3; j++) {  // -FluxTotal (newtonian_state.h:477-484)
-      double F[5];
-      ns::flux_inviscid_dir(s, j, F);
-      gv[0][j] = -F[0];
-#pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(F[1 + k] - st[k][j]);
-      gv[4][j] = -(F[4] + Fe[j]);
-    }
-    const double bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2],
-                          s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
-    double       vi[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) vi[j] = sd.U[j] - bf[j];
-    double Tau[3];
-    ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
-    if (ctx.stabilization != ns::STAB_NONE) {  // Stabilization (stabilization.h:41-58); divFdiff = 0
-      double r[5];
-      ns::flux_inviscid_strong(s, gs, r);
-      if (ctx.stabilization == ns::STAB_SUPG)
-#pragma unroll
-        for (int j = 0; j < 5; j++) r[j] += sd.U[j] - bf[j];
-      ns::add_stab_matrix(g, s, Tau, r, gv);
-    }
-    double sigma = 0;
-    if (ctx.idl_enable) {  // InternalDampingLayer_IFunction_Integrand
-      sigma = ns::linear_ramp<double>(ctx.idl_amplitude, ctx.idl_length, ctx.idl_start, x0);
-      ns::St<double> ds;
-      ds.Y[0] = (s.Y[0] - ctx.idl_pressure) * sigma;
-      for (int i = 1; i < 5; i++) ds.Y[i] = 0 * sigma;
-      ns::cons_from_prim_fwd(g, s, ds.Y, ds.U);
-#pragma unroll
-      for (int j = 0; j < 5; j++) vi[j] += ds.U[j];
-    }
-    // jac_data: q, strain rate (KM), Tau_d (+ sigma)  (newtonian.h:251-254)
-    if (ok) {
-      if (a.jac32) {
-        float *J = a.jac32;
-#pragma unroll
-        for (int j = 0; j < 5; j++) J[qidx(a.jd, e, j, t)] = (float)q[j];
-#pragma unroll
-        for (int j = 0; j < 6; j++) J[qidx(a.jd, e, 5 + j, t)] = (float)sr[j];
-#pragma unroll
-        for (int j = 0; j < 3; j++) J[qidx(a.jd, e, 11 + j, t)] = (float)Tau[j];
-        if (ctx.idl_enable) J[qidx(a.jd, e, 14, t)] = (float)sigma;
-      } else if (a.jac) {
-        double *J = a.jac;
-#pragma unroll
-        for (int j = 0; j < 5; j++) J[qidx(a.jd, e, j, t)] = q[j];
-#pragma unroll
-        for (int j = 0; j < 6; j++) J[qidx(a.jd, e, 5 + j, t)] = sr[j];
-#pragma unroll
-        for (int j = 0; j < 3; j++) J[qidx(a.jd, e, 11 + j, t)] = Tau[j];
-        if (ctx.idl_enable) J[qidx(a.jd, e, 14, t)] = sigma;
-      }
-    }
-    // ---- transpose and scatter-add, one component at a time
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-      double gw[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) gw[k] = wdetJ * (gv[c][0] * dXdx[k][0] + gv[c][1] * dXdx[k][1] + gv[c][2] * dXdx[k][2]);
-      const double r = el.template transpose<true, true>(wdetJ * vi[c], gw);
-      if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
-    }
-  }
-}
-
-// ------------------------------------------------------------------------------------------------------------------
-// IJacobian_Newtonian (newtonian.h:272-311) with IJacobian_Newtonian_Integrand (newtonian_integrands.h:134-166),
-// computed in T (double or float).  Stored jac_data supplies q, the strain rate and Tau_d; the IDL term uses the
-// stored sigma (newtonian_integrands.h:61-65).
-// ------------------------------------------------------------------------------------------------------------------
-template <int P, int Q, int VAR, typename T>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_ijacobian(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+template <int P, int Q, class Op>
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm          = Dims<P, Q>;
   constexpr int EPB = elems_per_block<Q>();
   constexpr int Q3  = Dm::Q3;
   __shared__ double sB[Q * P], sDc[Q * Q];
   __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
   load_tables<P, Q>(a.bt, sB, sDc);
-  const int        le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>       el(sB, sDc, s0[le], s1[le], s2[le], t);
-  const ns::Gas<T> g      = ns::Gas<T>::from(ctx.gas);
-  const long long  nbatch = ((long long)a.nelem + EPB - 1) / EPB;
-  const T          shift  = (T)ctx.ijacobian_time_shift;
+  const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
+  const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
 
   for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
     const long long e_  = bt * EPB + le;
@@ -256,104 +102,94 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_ijacobian(
     const int       e   = ok ? (int)e_ : 0;
     const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
 
-    T dq[5], gref[3][5];
+    double u[5], gu[3][5], u2[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) {
-      const double u = (t < Dm::P3) ? a.u[off + c * a.comp_stride] : 0.0;
-      const double v = el.interp(u);
-      double       gg[3];
-      el.grad(v, gg);
-      dq[c]      = (T)v;
-      gref[0][c] = (T)gg[0];
-      gref[1][c] = (T)gg[1];
-      gref[2][c] = (T)gg[2];
-    }
-    T wdetJ = 0, dXdx[3][3], qi[5], sr[6], Tau[3], sigma = 0;
-    if (ok) {
-      wdetJ = (T)a.qdata[qidx(a.qd, e, 0, t)];
-#pragma unroll
-      for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)a.qdata[qidx(a.qd, e, 1 + k, t)];
-      if (a.jac32) {
-#pragma unroll
-        for (int j = 0; j < 5; j++) qi[j] = (T)a.jac32[qidx(a.jd, e, j, t)];
-#pragma unroll
-        for (int j = 0; j < 6; j++) sr[j] = (T)a.jac32[qidx(a.jd, e, 5 + j, t)];
-#pragma unroll
-        for (int j = 0; j < 3; j++) Tau[j] = (T)a.jac32[qidx(a.jd, e, 11 + j, t)];
-        if (ctx.idl_enable) sigma = (T)a.jac32[qidx(a.jd, e, 14, t)];
-      } else {
-#pragma unroll
-        for (int j = 0; j < 5; j++) qi[j] = (T)a.jacr[qidx(a.jd, e, j, t)];
-#pragma unroll
-        for (int j = 0; j < 6; j++) sr[j] = (T)a.jacr[qidx(a.jd, e, 5 + j, t)];
-#pragma unroll
-        for (int j = 0; j < 3; j++) Tau[j] = (T)a.jacr[qidx(a.jd, e, 11 + j, t)];
-        if (ctx.idl_enable) sigma = (T)a.jacr[qidx(a.jd, e, 14, t)];
+      const double un = (t < Dm::P3) ? a.u[off + c * a.comp_stride] : 0.0;
+      u[c]            = el.interp(un);
+      if constexpr (Op::kGrad) {
+        double gg[3];
+        el.grad(u[c], gg);
+        gu[0][c] = gg[0];
+        gu[1][c] = gg[1];
+        gu[2][c] = gg[2];
       }
-    } else {
-      for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (k % 4 == 0) ? T(1) : T(0);
-      for (int j = 0; j < 5; j++) {
-        qi[j] = (VAR == ns::VAR_ENTROPY) ? (j == 4 ? T(-1) : T(0)) : T(1);
-        dq[j] = 0;
-        for (int d = 0; d < 3; d++) gref[d][j] = 0;
-      }
-      for (int j = 0; j < 6; j++) sr[j] = 0;
-      for (int j = 0; j < 3; j++) Tau[j] = 0;
     }
-    const ns::St<T> s  = ns::state_from_q<VAR>(g, qi);
-    const ns::St<T> ds = ns::state_from_q_fwd<VAR>(g, s, dq);
-    ns::St<T>       gds[3];
-    ns::phys_gradient<VAR>(g, s, gref, dXdx, gds);
+    if constexpr (Op::kIn2) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        const double un = (t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0;
+        u2[c]           = el.interp(un);
+      }
+    }
+    double v[5], gw[3][5];
+    Op::template qf<Q>(a, ctx, e, t, ok, u, gu, u2, v, gw);
+    if constexpr (Op::kScatter) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        const double g3[3] = {gw[0][c], gw[1][c], gw[2][c]};
+        const double r     = el.template transpose<Op::kOutV, Op::kOutG>(v[c], g3);
+        if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
+      }
+    }
+  }
+}
 
-    T dsr[6], st[3][3], dst[3][3];
-    ns::km_strain_rate(gds, dsr);
-    ns::stress_from_strain(g, dsr, dst);
-    ns::stress_from_strain(g, sr, st);
-    T dFe[3];
+// ------------------------------------------------------------------------------------------------------------------
+// Diagonal of a linear operator whose quadrature-point action is Op::action (IJacobian)
+// ------------------------------------------------------------------------------------------------------------------
+template <int P, int Q, class Op, typename T>
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+  using Dm          = Dims<P, Q>;
+  constexpr int EPB = elems_per_block<Q>();
+  constexpr int Q3  = Dm::Q3;
+  __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
+  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    sBB[i] = a.bt.BB[i];
+    sGB[i] = a.bt.GB[i];
+    sGG[i] = a.bt.GG[i];
+  }
+  __shared__ double sB[Q * P], sDc[Q * Q];
+  load_tables<P, Q>(a.bt, sB, sDc);
+  const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
+  const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
+
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long e_  = bt * EPB + le;
+    const bool      ok  = e_ < a.nelem;
+    const int       e   = ok ? (int)e_ : 0;
+    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
+    double          u2[5];
+    if constexpr (Op::kIn2) {
 #pragma unroll
-    for (int i = 0; i < 3; i++)  // ViscousEnergyFlux_fwd (newtonian_state.h:567-573)
-      dFe[i] = -s.Y[1] * dst[0][i] - ds.Y[1] * st[0][i] - s.Y[2] * dst[1][i] - ds.Y[2] * st[1][i] - s.Y[3] * dst[2][i] - ds.Y[3] * st[2][i] -
-               g.k * gds[i].Y[4];
-    T gv[5][3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      T dF[5];
-      ns::flux_inviscid_fwd_dir(s, ds, j, dF);
-      gv[0][j] = -dF[0];
-#pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(dF[1 + k] - dst[k][j]);
-      gv[4][j] = -(dF[4] + dFe[j]);
+      for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
     }
-    const T gx = (T)ctx.g[0], gy = (T)ctx.g[1], gz = (T)ctx.g[2];
-    const T dbf[5] = {0, ds.U[0] * gx, ds.U[0] * gy, ds.U[0] * gz, ds.U[1] * gx + ds.U[2] * gy + ds.U[3] * gz};
-    T       vi[5], dUdot[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-      dUdot[j] = shift * ds.U[j];
-      vi[j]    = dUdot[j] - dbf[j];
-    }
-    if (ctx.stabilization != ns::STAB_NONE) {
-      T r[5];
-      ns::flux_inviscid_strong(s, gds, r);
-      if (ctx.stabilization == ns::STAB_SUPG)
-#pragma unroll
-        for (int j = 0; j < 5; j++) r[j] += dUdot[j] - dbf[j];
-      ns::add_stab_matrix(g, s, Tau, r, gv);
-    }
-    if (ctx.idl_enable) {
-      ns::St<T> dd;
-      dd.Y[0] = ds.Y[0] * sigma;
-      for (int i = 1; i < 5; i++) dd.Y[i] = 0 * sigma;
-      ns::cons_from_prim_fwd(g, s, dd.Y, dd.U);
-#pragma unroll
-      for (int j = 0; j < 5; j++) vi[j] += dd.U[j];
-    }
-#pragma unroll
+    typename Op::Point p;
+    Op::load_point(a, ctx, e, t, ok, u2, p);
     for (int c = 0; c < 5; c++) {
-      double gw[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) gw[k] = (double)(wdetJ * (gv[c][0] * dXdx[k][0] + gv[c][1] * dXdx[k][1] + gv[c][2] * dXdx[k][2]));
-      const double r = el.template transpose<true, true>((double)(wdetJ * vi[c]), gw);
+      double M[4][4] = {{0}};
+      for (int s = 0; s < Op::kInSlots; s++) {
+        T dq[5] = {0, 0, 0, 0, 0}, gref[3][5] = {{0}};
+        if (s == 0) dq[c] = 1;
+        else gref[s - 1][c] = 1;
+        double v[5], gw[3][5];
+        Op::action(ctx, p, dq, gref, v, gw);
+        M[s][0] = ok ? v[c] : 0.0;
+        for (int d = 0; d < 3; d++) M[s][1 + d] = ok ? gw[d][c] : 0.0;
+      }
+      // sum over input slot a and test slot b of M[a][b] phi_a phi_b, grouped by 1D factor triples
+      double r = el.contract_t(M[0][0], sBB, sBB, sBB);
+      r += el.contract_t(M[0][1] + M[1][0], sGB, sBB, sBB);
+      r += el.contract_t(M[0][2] + M[2][0], sBB, sGB, sBB);
+      r += el.contract_t(M[0][3] + M[3][0], sBB, sBB, sGB);
+      r += el.contract_t(M[1][1], sGG, sBB, sBB);
+      r += el.contract_t(M[2][2], sBB, sGG, sBB);
+      r += el.contract_t(M[3][3], sBB, sBB, sGG);
+      r += el.contract_t(M[1][2] + M[2][1], sGB, sGB, sBB);
+      r += el.contract_t(M[1][3] + M[3][1], sGB, sBB, sGB);
+      r += el.contract_t(M[2][3] + M[3][2], sBB, sGB, sGB);
       if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
     }
   }
@@ -371,21 +207,21 @@ int grid_for(K kernel, int block, long long nbatch) {
   return (int)g;
 }
 
-template <int P, int Q, int VAR>
-hipError_t launch_ifunction_pq(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
+template <int P, int Q, class Op>
+hipError_t launch_op(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
   constexpr int EPB   = elems_per_block<Q>();
   const int     block = EPB * Q * Q * Q;
-  auto          k     = k_ifunction<P, Q, VAR>;
+  auto          k     = k_volume<P, Q, Op>;
   const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
   return hipGetLastError();
 }
 
-template <int P, int Q, int VAR, typename T>
-hipError_t launch_ijacobian_pq(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
+template <int P, int Q, class Op, typename T>
+hipError_t launch_diag(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
   constexpr int EPB   = elems_per_block<Q>();
   const int     block = EPB * Q * Q * Q;
-  auto          k     = k_ijacobian<P, Q, VAR, T>;
+  auto          k     = k_diagonal<P, Q, Op, T>;
   const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
   return hipGetLastError();
@@ -394,9 +230,9 @@ hipError_t launch_ijacobian_pq(const VolumeArgs &a, const CeedMiNewtonianContext
 template <int P, int Q>
 hipError_t launch_ifunction_var(int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
   switch (var) {
-    case 0: return launch_ifunction_pq<P, Q, 0>(a, ctx, s);
-    case 1: return launch_ifunction_pq<P, Q, 1>(a, ctx, s);
-    default: return launch_ifunction_pq<P, Q, 2>(a, ctx, s);
+    case 0: return launch_op<P, Q, nsop::IFunction<0>>(a, ctx, s);
+    case 1: return launch_op<P, Q, nsop::IFunction<1>>(a, ctx, s);
+    default: return launch_op<P, Q, nsop::IFunction<2>>(a, ctx, s);
   }
 }
 
@@ -404,15 +240,31 @@ template <int P, int Q>
 hipError_t launch_ijacobian_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
   if (fp32) {
     switch (var) {
-      case 0: return launch_ijacobian_pq<P, Q, 0, float>(a, ctx, s);
-      case 1: return launch_ijacobian_pq<P, Q, 1, float>(a, ctx, s);
-      default: return launch_ijacobian_pq<P, Q, 2, float>(a, ctx, s);
+      case 0: return launch_op<P, Q, nsop::IJacobian<0, float>>(a, ctx, s);
+      case 1: return launch_op<P, Q, nsop::IJacobian<1, float>>(a, ctx, s);
+      default: return launch_op<P, Q, nsop::IJacobian<2, float>>(a, ctx, s);
     }
   }
   switch (var) {
-    case 0: return launch_ijacobian_pq<P, Q, 0, double>(a, ctx, s);
-    case 1: return launch_ijacobian_pq<P, Q, 1, double>(a, ctx, s);
-    default: return launch_ijacobian_pq<P, Q, 2, double>(a, ctx, s);
+    case 0: return launch_op<P, Q, nsop::IJacobian<0, double>>(a, ctx, s);
+    case 1: return launch_op<P, Q, nsop::IJacobian<1, double>>(a, ctx, s);
+    default: return launch_op<P, Q, nsop::IJacobian<2, double>>(a, ctx, s);
+  }
+}
+
+template <int P, int Q>
+hipError_t launch_ijacobian_diag_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+  if (fp32) {
+    switch (var) {
+      case 0: return launch_diag<P, Q, nsop::IJacobian<0, float>, float>(a, ctx, s);
+      case 1: return launch_diag<P, Q, nsop::IJacobian<1, float>, float>(a, ctx, s);
+      default: return launch_diag<P, Q, nsop::IJacobian<2, float>, float>(a, ctx, s);
+    }
+  }
+  switch (var) {
+    case 0: return launch_diag<P, Q, nsop::IJacobian<0, double>, double>(a, ctx, s);
+    case 1: return launch_diag<P, Q, nsop::IJacobian<1, double>, double>(a, ctx, s);
+    default: return launch_diag<P, Q, nsop::IJacobian<2, double>, double>(a, ctx, s);
   }
 }
 
@@ -420,6 +272,11 @@ hipError_t launch_ijacobian_var(int var, bool fp32, const VolumeArgs &a, const C
 
 #define HONEE_PQ_DISPATCH(P_, Q_, CALL) \
   if (P == P_ && Q == Q_) return CALL;
+#define HONEE_PQ_ALL(CALL_T)          \
+  HONEE_PQ_DISPATCH(2, 2, CALL_T(2, 2)) \
+  HONEE_PQ_DISPATCH(3, 3, CALL_T(3, 3)) \
+  HONEE_PQ_DISPATCH(4, 4, CALL_T(4, 4)) \
+  HONEE_PQ_DISPATCH(5, 5, CALL_T(5, 5))
 
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream) {
   const long long n     = (long long)a.nelem * Q * Q * Q;
@@ -441,17 +298,53 @@ bool volume_pq_supported(int P, int Q) {
 }
 
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-  HONEE_PQ_DISPATCH(2, 2, (launch_ifunction_var<2, 2>(var, a, ctx, s)));
-  HONEE_PQ_DISPATCH(3, 3, (launch_ifunction_var<3, 3>(var, a, ctx, s)));
-  HONEE_PQ_DISPATCH(4, 4, (launch_ifunction_var<4, 4>(var, a, ctx, s)));
-  HONEE_PQ_DISPATCH(5, 5, (launch_ifunction_var<5, 5>(var, a, ctx, s)));
+#define CALL_IF(P_, Q_) (launch_ifunction_var<P_, Q_>(var, a, ctx, s))
+  HONEE_PQ_ALL(CALL_IF)
+#undef CALL_IF
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-  HONEE_PQ_DISPATCH(2, 2, (launch_ijacobian_var<2, 2>(var, fp32, a, ctx, s)));
-  HONEE_PQ_DISPATCH(3, 3, (launch_ijacobian_var<3, 3>(var, fp32, a, ctx, s)));
-  HONEE_PQ_DISPATCH(4, 4, (launch_ijacobian_var<4, 4>(var, fp32, a, ctx, s)));
-  HONEE_PQ_DISPATCH(5, 5, (launch_ijacobian_var<5, 5>(var, fp32, a, ctx, s)));
+#define CALL_IJ(P_, Q_) (launch_ijacobian_var<P_, Q_>(var, fp32, a, ctx, s))
+  HONEE_PQ_ALL(CALL_IJ)
+#undef CALL_IJ
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ijacobian_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_ID(P_, Q_) (launch_ijacobian_diag_var<P_, Q_>(var, fp32, a, ctx, s))
+  HONEE_PQ_ALL(CALL_ID)
+#undef CALL_ID
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs(int P, int Q, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_RHS(P_, Q_) (launch_op<P_, Q_, nsop::RHSFunction>(a, ctx, s))
+  HONEE_PQ_ALL(CALL_RHS)
+#undef CALL_RHS
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mass(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_MS(P_, Q_) (stabilized ? launch_op<P_, Q_, nsop::MassNewtonian>(a, ctx, s) : launch_op<P_, Q_, nsop::Mass5>(a, ctx, s))
+  HONEE_PQ_ALL(CALL_MS)
+#undef CALL_MS
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_MD(P_, Q_) \
+  (stabilized ? launch_diag<P_, Q_, nsop::MassNewtonian, double>(a, ctx, s) : launch_diag<P_, Q_, nsop::Mass5, double>(a, ctx, s))
+  HONEE_PQ_ALL(CALL_MD)
+#undef CALL_MD
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_monitor_ke(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_KE(P_, Q_)                                                                                       \
+  (var == 0 ? launch_op<P_, Q_, nsop::MonitorKE<0>>(a, ctx, s)                                               \
+            : (var == 1 ? launch_op<P_, Q_, nsop::MonitorKE<1>>(a, ctx, s) : launch_op<P_, Q_, nsop::MonitorKE<2>>(a, ctx, s)))
+  HONEE_PQ_ALL(CALL_KE)
+#undef CALL_KE
   return hipErrorInvalidValue;
 }

@@ -1,0 +1,469 @@
+// Quadrature-point bodies of the fused Newtonian volume operators.
+//
+// Each Op describes one HONEE QFunction for the generic fused element kernel (kernels_volume.hip): which element
+// fields it interpolates (active input with/without gradient, a second passive L-vector), and a device qf() that maps
+// the interpolated values at one quadrature point to the test-function weights v (for B^T) and gw (for the reference
+// gradient transpose), reading/writing its strided quadrature data directly.  gw[d][c] already includes
+// wdetJ * dXdx, i.e. gw[d][c] = wdetJ * sum_j gv_i[c][j] dXdx[d][j] as the reference QFunctions store Grad_v.
+#pragma once
+
+#include "kernels.hpp"
+#include "ns_physics.hpp"
+
+namespace nsop {
+
+__device__ __forceinline__ long long qidx(const QDataLayout &l, int e, int c, int i) {
+  return (long long)i * l.sn + (long long)c * l.sc + (long long)e * l.se;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_qdata(const VolumeArgs &a, int e, int t, bool ok, T &wdetJ, T dXdx[3][3]) {
+  if (ok) {
+    wdetJ = (T)a.qdata[qidx(a.qd, e, 0, t)];
+#pragma unroll
+    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)a.qdata[qidx(a.qd, e, 1 + k, t)];
+  } else {  // padding thread of a partial batch: harmless geometry, zero weight
+    wdetJ = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (k % 4 == 0) ? T(1) : T(0);
+  }
+}
+
+// A physical state for padding threads so the pointwise physics stays finite
+template <int VAR, typename T>
+__device__ __forceinline__ void padding_state(T q[5]) {
+#pragma unroll
+  for (int c = 0; c < 5; c++) q[c] = (VAR == ns::VAR_ENTROPY) ? (c == 4 ? T(-1) : T(0)) : T(1);
+}
+
+// gw[d][c] = wdetJ * (gv[c][0] dXdx[d][0] + gv[c][1] dXdx[d][1] + gv[c][2] dXdx[d][2])   (newtonian.h:245-249)
+template <typename T>
+__device__ __forceinline__ void map_grad_out(T wdetJ, const T dXdx[3][3], const T gv[5][3], double gw[3][5]) {
+#pragma unroll
+  for (int c = 0; c < 5; c++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) gw[d][c] = (double)(wdetJ * (gv[c][0] * dXdx[d][0] + gv[c][1] * dXdx[d][1] + gv[c][2] * dXdx[d][2]));
+}
+
+// x coordinate (first component) of the quadrature point from the trilinear coordinate field
+template <int Q>
+__device__ __forceinline__ double coord_x(const VolumeArgs &a, int e, int t) {
+  const int qx = t % Q, qy = (t / Q) % Q, qz = t / (Q * Q);
+  double    x0 = 0;
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const int xo = a.xoffsets[(long long)e * 8 + n];
+    x0 += a.Bx[qx * 2 + (n & 1)] * a.Bx[qy * 2 + ((n >> 1) & 1)] * a.Bx[qz * 2 + (n >> 2)] * a.x[xo];
+  }
+  return x0;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// IFunction_Newtonian (newtonian.h:205-257) with IFunction_Newtonian_Integrand (newtonian_integrands.h:87-115) and the
+// internal damping layer (newtonian_integrands.h:40-48).  Writes jac_data: q, strain rate (KM), Tau_d (+ sigma).
+// ---------------------------------------------------------------------------------------------------------------------
+template <int VAR>
+struct IFunction {
+  static constexpr bool kGrad = true, kIn2 = true, kOutV = true, kOutG = true, kScatter = true;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+                            double qdot[5], double v[5], double gw[3][5]) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    double                wdetJ, dXdx[3][3];
+    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    if (!ok) {
+      padding_state<VAR>(q);
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        qdot[c] = 0;
+        gref[0][c] = gref[1][c] = gref[2][c] = 0;
+      }
+    }
+    const ns::St<double> s  = ns::state_from_q<VAR>(g, q);
+    const ns::St<double> sd = ns::state_from_q_fwd<VAR>(g, s, qdot);
+    ns::St<double>       gs[3];
+    ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);
+
+    double sr[6], st[3][3], Fe[3];
+    ns::km_strain_rate(gs, sr);
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gs[i].Y[4];
+    double gv[5][3];  // grad_v_i[var][dir] = -F_total (+ stab)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double F[5];
+      ns::flux_inviscid_dir(s, j, F);
+      gv[0][j] = -F[0];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(F[1 + k] - st[k][j]);
+      gv[4][j] = -(F[4] + Fe[j]);
+    }
+    const double bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2], s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
+    double       vi[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) vi[j] = sd.U[j] - bf[j];
+    double Tau[3];
+    ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
+    if (ctx.stabilization != ns::STAB_NONE) {  // Stabilization (stabilization.h:41-58) with divFdiff = 0
+      double r[5];
+      ns::flux_inviscid_strong(s, gs, r);
+      if (ctx.stabilization == ns::STAB_SUPG)
+#pragma unroll
+        for (int j = 0; j < 5; j++) r[j] += sd.U[j] - bf[j];
+      ns::add_stab_matrix(g, s, Tau, r, gv);
+    }
+    double sigma = 0;
+    if (ctx.idl_enable) {
+      sigma = ns::linear_ramp<double>(ctx.idl_amplitude, ctx.idl_length, ctx.idl_start, ok ? coord_x<Q>(a, e, t) : 0.0);
+      ns::St<double> ds;
+      ds.Y[0] = (s.Y[0] - ctx.idl_pressure) * sigma;
+#pragma unroll
+      for (int i = 1; i < 5; i++) ds.Y[i] = 0 * sigma;
+      ns::cons_from_prim_fwd(g, s, ds.Y, ds.U);
+#pragma unroll
+      for (int j = 0; j < 5; j++) vi[j] += ds.U[j];
+    }
+    if (ok) {  // jac_data (newtonian.h:251-254)
+      if (a.jac32) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) a.jac32[qidx(a.jd, e, j, t)] = (float)q[j];
+#pragma unroll
+        for (int j = 0; j < 6; j++) a.jac32[qidx(a.jd, e, 5 + j, t)] = (float)sr[j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) a.jac32[qidx(a.jd, e, 11 + j, t)] = (float)Tau[j];
+        if (ctx.idl_enable) a.jac32[qidx(a.jd, e, 14, t)] = (float)sigma;
+      } else if (a.jac) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) a.jac[qidx(a.jd, e, j, t)] = q[j];
+#pragma unroll
+        for (int j = 0; j < 6; j++) a.jac[qidx(a.jd, e, 5 + j, t)] = sr[j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) a.jac[qidx(a.jd, e, 11 + j, t)] = Tau[j];
+        if (ctx.idl_enable) a.jac[qidx(a.jd, e, 14, t)] = sigma;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[j] = wdetJ * vi[j];
+    map_grad_out(wdetJ, dXdx, gv, gw);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------------
+// IJacobian_Newtonian (newtonian.h:272-311) with IJacobian_Newtonian_Integrand (newtonian_integrands.h:134-166), in T.
+// ---------------------------------------------------------------------------------------------------------------------
+template <int VAR, typename T>
+struct IJacobian {
+  static constexpr bool kGrad = true, kIn2 = false, kOutV = true, kOutG = true, kScatter = true;
+
+  // State-dependent part at one quadrature point (shared by apply and diagonal assembly)
+  struct Point {
+    ns::St<T> s;
+    T         wdetJ, dXdx[3][3], Tau[3], st[3][3], sigma;
+  };
+
+  static constexpr int kInSlots = 4;  // linear inputs: value and three reference derivatives
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *, Point &p) {
+    const ns::Gas<T> g = ns::Gas<T>::from(ctx.gas);
+    load_qdata(a, e, t, ok, p.wdetJ, p.dXdx);
+    T qi[5], sr[6];
+    p.sigma = 0;
+    if (ok) {
+      if (a.jac32) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) qi[j] = (T)a.jac32[qidx(a.jd, e, j, t)];
+#pragma unroll
+        for (int j = 0; j < 6; j++) sr[j] = (T)a.jac32[qidx(a.jd, e, 5 + j, t)];
+#pragma unroll
+        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jac32[qidx(a.jd, e, 11 + j, t)];
+        if (ctx.idl_enable) p.sigma = (T)a.jac32[qidx(a.jd, e, 14, t)];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 5; j++) qi[j] = (T)a.jacr[qidx(a.jd, e, j, t)];
+#pragma unroll
+        for (int j = 0; j < 6; j++) sr[j] = (T)a.jacr[qidx(a.jd, e, 5 + j, t)];
+#pragma unroll
+        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jacr[qidx(a.jd, e, 11 + j, t)];
+        if (ctx.idl_enable) p.sigma = (T)a.jacr[qidx(a.jd, e, 14, t)];
+      }
+    } else {
+      padding_state<VAR>(qi);
+#pragma unroll
+      for (int j = 0; j < 6; j++) sr[j] = 0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) p.Tau[j] = 0;
+    }
+    p.s = ns::state_from_q<VAR>(g, qi);
+    ns::stress_from_strain(g, sr, p.st);
+  }
+
+  // Linear action at a quadrature point: (dq, gref) -> (v, gw)
+  __device__ static void action(const CeedMiNewtonianContext &ctx, const Point &p, const T dq[5], const T gref[3][5], double v[5],
+                                double gw[3][5]) {
+    const ns::Gas<T> g     = ns::Gas<T>::from(ctx.gas);
+    const ns::St<T> &s     = p.s;
+    const ns::St<T>  ds    = ns::state_from_q_fwd<VAR>(g, s, dq);
+    const T          shift = (T)ctx.ijacobian_time_shift;
+    ns::St<T>        gds[3];
+    ns::phys_gradient<VAR>(g, s, gref, p.dXdx, gds);
+    T dsr[6], dst[3][3], dFe[3];
+    ns::km_strain_rate(gds, dsr);
+    ns::stress_from_strain(g, dsr, dst);
+#pragma unroll
+    for (int i = 0; i < 3; i++)  // ViscousEnergyFlux_fwd (newtonian_state.h:567-573)
+      dFe[i] = -s.Y[1] * dst[0][i] - ds.Y[1] * p.st[0][i] - s.Y[2] * dst[1][i] - ds.Y[2] * p.st[1][i] - s.Y[3] * dst[2][i] -
+               ds.Y[3] * p.st[2][i] - g.k * gds[i].Y[4];
+    T gv[5][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T dF[5];
+      ns::flux_inviscid_fwd_dir(s, ds, j, dF);
+      gv[0][j] = -dF[0];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(dF[1 + k] - dst[k][j]);
+      gv[4][j] = -(dF[4] + dFe[j]);
+    }
+    const T gx = (T)ctx.g[0], gy = (T)ctx.g[1], gz = (T)ctx.g[2];
+    const T dbf[5] = {0, ds.U[0] * gx, ds.U[0] * gy, ds.U[0] * gz, ds.U[1] * gx + ds.U[2] * gy + ds.U[3] * gz};
+    T       vi[5], dUdot[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      dUdot[j] = shift * ds.U[j];
+      vi[j]    = dUdot[j] - dbf[j];
+    }
+    if (ctx.stabilization != ns::STAB_NONE) {
+      T r[5];
+      ns::flux_inviscid_strong(s, gds, r);
+      if (ctx.stabilization == ns::STAB_SUPG)
+#pragma unroll
+        for (int j = 0; j < 5; j++) r[j] += dUdot[j] - dbf[j];
+      ns::add_stab_matrix(g, s, p.Tau, r, gv);
+    }
+    if (ctx.idl_enable) {  // InternalDampingLayer_IJacobian_Integrand (newtonian_integrands.h:61-65)
+      ns::St<T> dd;
+      dd.Y[0] = ds.Y[0] * p.sigma;
+#pragma unroll
+      for (int i = 1; i < 5; i++) dd.Y[i] = 0 * p.sigma;
+      ns::cons_from_prim_fwd(g, s, dd.Y, dd.U);
+#pragma unroll
+      for (int j = 0; j < 5; j++) vi[j] += dd.U[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[j] = (double)(p.wdetJ * vi[j]);
+    map_grad_out(p.wdetJ, p.dXdx, gv, gw);
+  }
+
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double u[5], double gu[3][5],
+                            double *, double v[5], double gw[3][5]) {
+    Point p;
+    load_point(a, ctx, e, t, ok, nullptr, p);
+    T dq[5], gref[3][5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      dq[c] = ok ? (T)u[c] : T(0);
+#pragma unroll
+      for (int d = 0; d < 3; d++) gref[d][c] = ok ? (T)gu[d][c] : T(0);
+    }
+    action(ctx, p, dq, gref, v, gw);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------------
+// RHSFunction_Newtonian (newtonian.h:133-202): explicit right-hand side in conservative variables
+// ---------------------------------------------------------------------------------------------------------------------
+struct RHSFunction {
+  static constexpr bool kGrad = true, kIn2 = false, kOutV = true, kOutG = true, kScatter = true;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+                            double *, double v[5], double gw[3][5]) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    double                wdetJ, dXdx[3][3];
+    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    if (!ok) {
+      padding_state<ns::VAR_CONS>(q);
+#pragma unroll
+      for (int c = 0; c < 5; c++) gref[0][c] = gref[1][c] = gref[2][c] = 0;
+    }
+    const ns::St<double> s = ns::state_from_q<ns::VAR_CONS>(g, q);
+    ns::St<double>       gs[3];
+    ns::phys_gradient<ns::VAR_CONS>(g, s, gref, dXdx, gs);
+    double sr[6], st[3][3], Fe[3];
+    ns::km_strain_rate(gs, sr);
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gs[i].Y[4];
+    double F[5][3];  // FluxTotal
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double Fi[5];
+      ns::flux_inviscid_dir(s, j, Fi);
+      F[0][j] = Fi[0];
+#pragma unroll
+      for (int k = 0; k < 3; k++) F[k + 1][j] = Fi[1 + k] - st[k][j];
+      F[4][j] = Fi[4] + Fe[j];
+    }
+    const double bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2], s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[j] = wdetJ * bf[j];
+    if (ctx.idl_enable) {
+      const double   sigma = ns::linear_ramp<double>(ctx.idl_amplitude, ctx.idl_length, ctx.idl_start, ok ? coord_x<Q>(a, e, t) : 0.0);
+      ns::St<double> ds;
+      ds.Y[0] = (s.Y[0] - ctx.idl_pressure) * sigma;
+#pragma unroll
+      for (int i = 1; i < 5; i++) ds.Y[i] = 0 * sigma;
+      ns::cons_from_prim_fwd(g, s, ds.Y, ds.U);
+#pragma unroll
+      for (int j = 0; j < 5; j++) v[j] -= wdetJ * ds.U[j];
+    }
+    double stab[5][3] = {{0}};
+    if (ctx.stabilization != ns::STAB_NONE) {  // U_dot = 0, divFdiff = 0
+      double Tau[3], r[5];
+      ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
+      ns::flux_inviscid_strong(s, gs, r);
+      if (ctx.stabilization == ns::STAB_SUPG)
+#pragma unroll
+        for (int j = 0; j < 5; j++) r[j] += 0 - bf[j];
+      ns::add_stab_matrix(g, s, Tau, r, stab);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        gw[d][c] = wdetJ * (dXdx[d][0] * F[c][0] + dXdx[d][1] * F[c][1] + dXdx[d][2] * F[c][2]) -
+                   wdetJ * (stab[c][0] * dXdx[d][0] + stab[c][1] * dXdx[d][1] + stab[c][2] * dXdx[d][2]);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------------
+// MassFunction_Newtonian_Conserv (newtonian.h:44-82): SUPG-stabilized consistent mass acting on q_dot (active), with
+// the state q passed as the second field.
+// ---------------------------------------------------------------------------------------------------------------------
+struct MassNewtonian {
+  static constexpr bool kGrad = false, kIn2 = true, kOutV = true, kOutG = true, kScatter = true;
+  static constexpr int  kInSlots = 1;  // linear in the value of q_dot only
+
+  struct Point {
+    ns::St<double> s;
+    double         wdetJ, dXdx[3][3], Tau[3];
+  };
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *q2, Point &p) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    load_qdata(a, e, t, ok, p.wdetJ, p.dXdx);
+    double q[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) q[c] = q2[c];
+    if (!ok) padding_state<ns::VAR_CONS>(q);
+    p.s = ns::state_from_q<ns::VAR_CONS>(g, q);
+    ns::tau_diag_prim<double>(ctx.tau_coeffs, g, p.s, p.dXdx, ctx.dt, p.Tau);
+  }
+  __device__ static void action(const CeedMiNewtonianContext &ctx, const Point &p, const double qdot[5], const double (*)[5], double v[5],
+                                double gw[3][5]) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+#pragma unroll
+    for (int f = 0; f < 5; f++) v[f] = p.wdetJ * qdot[f];
+    double stab[5][3] = {{0}};
+    if (ctx.stabilization != ns::STAB_NONE) {
+      double r[5];
+#pragma unroll
+      for (int j = 0; j < 5; j++) r[j] = (ctx.stabilization == ns::STAB_SUPG) ? qdot[j] : 0.0;
+      ns::add_stab_matrix(g, p.s, p.Tau, r, stab);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) gw[d][c] = p.wdetJ * (stab[c][0] * p.dXdx[d][0] + stab[c][1] * p.dXdx[d][1] + stab[c][2] * p.dXdx[d][2]);
+  }
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double qdot[5], double (*)[5],
+                            double q[5], double v[5], double gw[3][5]) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    double                wdetJ, dXdx[3][3];
+    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    if (!ok) {
+      padding_state<ns::VAR_CONS>(q);
+#pragma unroll
+      for (int c = 0; c < 5; c++) qdot[c] = 0;
+    }
+    const ns::St<double> s = ns::state_from_q<ns::VAR_CONS>(g, q);
+#pragma unroll
+    for (int f = 0; f < 5; f++) v[f] = wdetJ * qdot[f];
+    double stab[5][3] = {{0}};
+    if (ctx.stabilization != ns::STAB_NONE) {  // zero gradient, zero body force: r = U_dot for SUPG, 0 for SU
+      double Tau[3], r[5];
+      ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
+#pragma unroll
+      for (int j = 0; j < 5; j++) r[j] = (ctx.stabilization == ns::STAB_SUPG) ? qdot[j] : 0.0;
+      ns::add_stab_matrix(g, s, Tau, r, stab);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) gw[d][c] = wdetJ * (stab[c][0] * dXdx[d][0] + stab[c][1] * dXdx[d][1] + stab[c][2] * dXdx[d][2]);
+  }
+};
+
+// Mass_5 (qfunctions/mass.h): v = wdetJ u
+struct Mass5 {
+  static constexpr bool kGrad = false, kIn2 = false, kOutV = true, kOutG = false, kScatter = true;
+  static constexpr int  kInSlots = 1;
+  struct Point {
+    double wdetJ;
+  };
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, const double *, Point &p) {
+    p.wdetJ = ok ? a.qdata[qidx(a.qd, e, 0, t)] : 0.0;
+  }
+  __device__ static void action(const CeedMiNewtonianContext &, const Point &p, const double u[5], const double (*)[5], double v[5],
+                                double gw[3][5]) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      v[c]     = p.wdetJ * u[c];
+      gw[0][c] = gw[1][c] = gw[2][c] = 0;
+    }
+  }
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, double u[5], double (*)[5], double *,
+                            double v[5], double gw[3][5]) {
+    const double wdetJ = ok ? a.qdata[qidx(a.qd, e, 0, t)] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = wdetJ * u[c];
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------------
+// MonitorTotalKineticEnergy (qfunctions/monitor_totalkineticenergy.h:8-39): five integrands per quadrature point,
+// written to a strided field (no basis transpose).
+// ---------------------------------------------------------------------------------------------------------------------
+template <int VAR>
+struct MonitorKE {
+  static constexpr bool kGrad = true, kIn2 = false, kOutV = false, kOutG = false, kScatter = false;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+                            double *, double *, double (*)[5]) {
+    if (!ok) return;
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    double                wdetJ, dXdx[3][3];
+    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    const ns::St<double> s = ns::state_from_q<VAR>(g, q);
+    ns::St<double>       gs[3];
+    ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);
+    double kmsr[6];
+    ns::km_strain_rate(gs, kmsr);
+    const double div_u = kmsr[0] + kmsr[1] + kmsr[2];
+    const double w     = 1 / sqrt(2.);
+    const double S[9]  = {kmsr[0], w * kmsr[5], w * kmsr[4], w * kmsr[5], kmsr[1], w * kmsr[3], w * kmsr[4], w * kmsr[3], kmsr[2]};
+    double       SS    = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) SS += S[i] * S[i];
+    // vorticity = curl u with grad_u[i][j] = d u_i / d x_j = gs[j].Y[1 + i]
+    const double w0 = gs[1].Y[3] - gs[2].Y[2], w1 = gs[2].Y[1] - gs[0].Y[3], w2 = gs[0].Y[2] - gs[1].Y[1];
+    double      *out = a.jac;  // strided output field
+    out[qidx(a.jd, e, 0, t)] = wdetJ * 0.5 * s.U[0] * (s.Y[1] * s.Y[1] + s.Y[2] * s.Y[2] + s.Y[3] * s.Y[3]);
+    out[qidx(a.jd, e, 1, t)] = wdetJ * -2 * g.mu * SS;
+    out[qidx(a.jd, e, 2, t)] = wdetJ * -g.lambda * g.mu * (div_u * div_u);
+    out[qidx(a.jd, e, 3, t)] = wdetJ * s.Y[0] * div_u;
+    out[qidx(a.jd, e, 4, t)] = wdetJ * g.mu * (w0 * w0 + w1 * w1 + w2 * w2);
+  }
+};
+
+}  // namespace nsop

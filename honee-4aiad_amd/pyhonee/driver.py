@@ -113,6 +113,7 @@ def load_libs():
         lib.HoneeMiProblemGetOperator.argtypes = [ct.c_void_p, ct.c_int]
         lib.HoneeMiApplyIFunction.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiApplyIJacobian.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiIJacobianDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
         lib.HoneeMiGetQData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData32.argtypes = [ct.c_void_p, ct.c_void_p]
@@ -173,6 +174,9 @@ class NewtonianProblem:
 
     def ijacobian(self, dq, v, shift, mem=CEED_MEM_HOST):
         self._check(self.lib.HoneeMiApplyIJacobian(self.handle, mem, shift, _ptr(dq), _ptr(v)), "IJacobian")
+
+    def ijacobian_diagonal(self, diag, shift, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiIJacobianDiagonal(self.handle, mem, shift, _ptr(diag)), "IJacobian diagonal")
 
     def qdata(self):
         out = np.zeros((self.nelem, 10, self.Q3))

@@ -116,6 +116,23 @@ def test_ijacobian_fp32_variant():
     assert e2 < 1e-5, (e2, emax)
 
 
+@pytest.mark.parametrize("state_var,degree", [(0, 2), (1, 3), (2, 3), (1, 1)])
+def test_ijacobian_diagonal(state_var, degree):
+    """Point-Jacobi diagonal (CeedOperatorLinearAssembleDiagonal) against the oracle's element-column assembly."""
+    _say(f"ijacobian diagonal var={state_var} degree={degree}")
+    case = make_case(shape=(2, 2, 2), degree=degree, state_var=state_var, stabilization=2, seed=80 + state_var, idl=(degree == 1))
+    om, oc, qd = oracle_setup(case)
+    _, jac_ref = O.ifunction(om, oc, qd, case["q"], case["qdot"])
+    shift = 120.0
+    oc.ijacobian_time_shift = shift
+    d_ref = O.ijacobian_diagonal(om, oc, qd, jac_ref)
+    prob, _ = _run_ifunction(case)
+    d = np.zeros_like(d_ref)
+    prob.ijacobian_diagonal(d, shift)
+    e2, emax = rel_err(d, d_ref)
+    assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+
+
 def test_repeated_apply_is_stable():
     """Two applies with updated context values reuse the bound kernel and read the new values."""
     _say("repeated apply")
