@@ -84,16 +84,25 @@ __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
 // ------------------------------------------------------------------------------------------------------------------
 // Generic fused volume kernel
 // ------------------------------------------------------------------------------------------------------------------
+// The interpolated inputs of every quadrature point and the quadrature-point outputs are staged in LDS (sX), one slot
+// per (field component, point): the basis stages then work one component at a time with few live registers, and the
+// register peak of the kernel is that of the quadrature-point body alone.  Each thread reads and writes only its own
+// point's slots, so the hand-over between stages needs no barrier.
 template <int P, int Q, class Op>
 __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
-  using Dm          = Dims<P, Q>;
-  constexpr int EPB = elems_per_block<Q>();
-  constexpr int Q3  = Dm::Q3;
+  using Dm           = Dims<P, Q>;
+  constexpr int EPB  = elems_per_block<Q>();
+  constexpr int Q3   = Dm::Q3;
+  constexpr int NIN  = 5 + (Op::kGrad ? 15 : 0) + (Op::kIn2 ? 5 : 0);
+  constexpr int NOUT = Op::kScatter ? 20 : 1;
+  constexpr int NST  = NIN > NOUT ? NIN : NOUT;
   __shared__ double sB[Q * P], sDc[Q * Q];
   __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
+  __shared__ double sX[EPB][NST][Q3];
   load_tables<P, Q>(a.bt, sB, sDc);
   const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
   Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
+  double(*X)[Q3]         = sX[le];
   const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
 
   for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
@@ -102,33 +111,49 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
     const int       e   = ok ? (int)e_ : 0;
     const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
 
-    double u[5], gu[3][5], u2[5];
-#pragma unroll
+#pragma unroll 1
     for (int c = 0; c < 5; c++) {
       const double un = (t < Dm::P3) ? a.u[off + c * a.comp_stride] : 0.0;
-      u[c]            = el.interp(un);
+      const double uq = el.interp(un);
+      X[c][t]         = uq;
       if constexpr (Op::kGrad) {
         double gg[3];
-        el.grad(u[c], gg);
-        gu[0][c] = gg[0];
-        gu[1][c] = gg[1];
-        gu[2][c] = gg[2];
+        el.grad(uq, gg);
+        X[5 + c][t]  = gg[0];
+        X[10 + c][t] = gg[1];
+        X[15 + c][t] = gg[2];
       }
     }
     if constexpr (Op::kIn2) {
+      constexpr int base = Op::kGrad ? 20 : 5;
+#pragma unroll 1
+      for (int c = 0; c < 5; c++) X[base + c][t] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
+    }
+    {
+      double u[5], gu[3][5], u2[5], v[5], gw[3][5];
 #pragma unroll
       for (int c = 0; c < 5; c++) {
-        const double un = (t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0;
-        u2[c]           = el.interp(un);
+        u[c] = X[c][t];
+        if constexpr (Op::kGrad)
+#pragma unroll
+          for (int d = 0; d < 3; d++) gu[d][c] = X[5 + 5 * d + c][t];
+        if constexpr (Op::kIn2) u2[c] = X[(Op::kGrad ? 20 : 5) + c][t];
+      }
+      Op::template qf<Q>(a, ctx, e, t, ok, u, gu, u2, v, gw);
+      if constexpr (Op::kScatter) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+          X[c][t] = v[c];
+#pragma unroll
+          for (int d = 0; d < 3; d++) X[5 + 5 * d + c][t] = gw[d][c];
+        }
       }
     }
-    double v[5], gw[3][5];
-    Op::template qf<Q>(a, ctx, e, t, ok, u, gu, u2, v, gw);
     if constexpr (Op::kScatter) {
-#pragma unroll
+#pragma unroll 1
       for (int c = 0; c < 5; c++) {
-        const double g3[3] = {gw[0][c], gw[1][c], gw[2][c]};
-        const double r     = el.template transpose<Op::kOutV, Op::kOutG>(v[c], g3);
+        const double g3[3] = {X[5 + c][t], X[10 + c][t], X[15 + c][t]};
+        const double r     = el.template transpose<Op::kOutV, Op::kOutG>(X[c][t], g3);
         if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
       }
     }

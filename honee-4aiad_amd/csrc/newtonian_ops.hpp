@@ -81,23 +81,19 @@ struct IFunction {
     }
     const ns::St<double> s  = ns::state_from_q<VAR>(g, q);
     const ns::St<double> sd = ns::state_from_q_fwd<VAR>(g, s, qdot);
-    ns::St<double>       gs[3];
-    ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);
-
-    double sr[6], st[3][3], Fe[3];
-    ns::km_strain_rate(gs, sr);
-    ns::stress_from_strain(g, sr, st);
-#pragma unroll
-    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gs[i].Y[4];
-    double gv[5][3];  // grad_v_i[var][dir] = -F_total (+ stab)
+    // Gradient states one direction at a time: the strong convective residual (FluxInviscidStrong,
+    // newtonian_state.h:466-475) is accumulated in the same direction order, and only the primitive gradients
+    // (velocity, temperature) are kept for the viscous fluxes.
+    double r[5] = {0, 0, 0, 0, 0}, gY[3][4];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      double F[5];
-      ns::flux_inviscid_dir(s, j, F);
-      gv[0][j] = -F[0];
+      const ns::St<double> gsj = ns::phys_gradient_dir<VAR>(g, s, gref, dXdx, j);
+      double               dF[5];
+      ns::flux_inviscid_fwd_dir(s, gsj, j, dF);
 #pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(F[1 + k] - st[k][j]);
-      gv[4][j] = -(F[4] + Fe[j]);
+      for (int k = 0; k < 5; k++) r[k] += dF[k];
+#pragma unroll
+      for (int k = 0; k < 4; k++) gY[j][k] = gsj.Y[1 + k];
     }
     const double bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2], s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
     double       vi[5];
@@ -105,13 +101,29 @@ struct IFunction {
     for (int j = 0; j < 5; j++) vi[j] = sd.U[j] - bf[j];
     double Tau[3];
     ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
+    double stab[5][3] = {{0}};
     if (ctx.stabilization != ns::STAB_NONE) {  // Stabilization (stabilization.h:41-58) with divFdiff = 0
-      double r[5];
-      ns::flux_inviscid_strong(s, gs, r);
       if (ctx.stabilization == ns::STAB_SUPG)
 #pragma unroll
         for (int j = 0; j < 5; j++) r[j] += sd.U[j] - bf[j];
-      ns::add_stab_matrix(g, s, Tau, r, gv);
+      ns::add_stab_matrix(g, s, Tau, r, stab);
+    }
+    // KMStrainRate_State / NewtonianStress / ViscousEnergyFlux from the kept primitive gradients
+    const double w   = 1 / sqrt(2.);
+    double       sr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
+    double       st[3][3], Fe[3];
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gY[i][3];
+    double gv[5][3];  // grad_v_i[var][dir] = -F_total + stab
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double F[5];
+      ns::flux_inviscid_dir(s, j, F);
+      gv[0][j] = -F[0] + stab[0][j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(F[1 + k] - st[k][j]) + stab[k + 1][j];
+      gv[4][j] = -(F[4] + Fe[j]) + stab[4][j];
     }
     double sigma = 0;
     if (ctx.idl_enable) {
@@ -204,24 +216,16 @@ struct IJacobian {
     const ns::St<T> &s     = p.s;
     const ns::St<T>  ds    = ns::state_from_q_fwd<VAR>(g, s, dq);
     const T          shift = (T)ctx.ijacobian_time_shift;
-    ns::St<T>        gds[3];
-    ns::phys_gradient<VAR>(g, s, gref, p.dXdx, gds);
-    T dsr[6], dst[3][3], dFe[3];
-    ns::km_strain_rate(gds, dsr);
-    ns::stress_from_strain(g, dsr, dst);
+    T                r[5]  = {0, 0, 0, 0, 0}, gY[3][4];
 #pragma unroll
-    for (int i = 0; i < 3; i++)  // ViscousEnergyFlux_fwd (newtonian_state.h:567-573)
-      dFe[i] = -s.Y[1] * dst[0][i] - ds.Y[1] * p.st[0][i] - s.Y[2] * dst[1][i] - ds.Y[2] * p.st[1][i] - s.Y[3] * dst[2][i] -
-               ds.Y[3] * p.st[2][i] - g.k * gds[i].Y[4];
-    T gv[5][3];
+    for (int j = 0; j < 3; j++) {  // gradient states one direction at a time (see IFunction)
+      const ns::St<T> gdsj = ns::phys_gradient_dir<VAR>(g, s, gref, p.dXdx, j);
+      T               dFj[5];
+      ns::flux_inviscid_fwd_dir(s, gdsj, j, dFj);
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      T dF[5];
-      ns::flux_inviscid_fwd_dir(s, ds, j, dF);
-      gv[0][j] = -dF[0];
+      for (int k = 0; k < 5; k++) r[k] += dFj[k];
 #pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(dF[1 + k] - dst[k][j]);
-      gv[4][j] = -(dF[4] + dFe[j]);
+      for (int k = 0; k < 4; k++) gY[j][k] = gdsj.Y[1 + k];
     }
     const T gx = (T)ctx.g[0], gy = (T)ctx.g[1], gz = (T)ctx.g[2];
     const T dbf[5] = {0, ds.U[0] * gx, ds.U[0] * gy, ds.U[0] * gz, ds.U[1] * gx + ds.U[2] * gy + ds.U[3] * gz};
@@ -231,13 +235,30 @@ struct IJacobian {
       dUdot[j] = shift * ds.U[j];
       vi[j]    = dUdot[j] - dbf[j];
     }
+    T stab[5][3] = {{0}};
     if (ctx.stabilization != ns::STAB_NONE) {
-      T r[5];
-      ns::flux_inviscid_strong(s, gds, r);
       if (ctx.stabilization == ns::STAB_SUPG)
 #pragma unroll
         for (int j = 0; j < 5; j++) r[j] += dUdot[j] - dbf[j];
-      ns::add_stab_matrix(g, s, p.Tau, r, gv);
+      ns::add_stab_matrix(g, s, p.Tau, r, stab);
+    }
+    const T w      = T(1) / (T)sqrt(2.);
+    T       dsr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
+    T       dst[3][3], dFe[3];
+    ns::stress_from_strain(g, dsr, dst);
+#pragma unroll
+    for (int i = 0; i < 3; i++)  // ViscousEnergyFlux_fwd (newtonian_state.h:567-573)
+      dFe[i] = -s.Y[1] * dst[0][i] - ds.Y[1] * p.st[0][i] - s.Y[2] * dst[1][i] - ds.Y[2] * p.st[1][i] - s.Y[3] * dst[2][i] -
+               ds.Y[3] * p.st[2][i] - g.k * gY[i][3];
+    T gv[5][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T dF[5];
+      ns::flux_inviscid_fwd_dir(s, ds, j, dF);
+      gv[0][j] = -dF[0] + stab[0][j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(dF[1 + k] - dst[k][j]) + stab[k + 1][j];
+      gv[4][j] = -(dF[4] + dFe[j]) + stab[4][j];
     }
     if (ctx.idl_enable) {  // InternalDampingLayer_IJacobian_Integrand (newtonian_integrands.h:61-65)
       ns::St<T> dd;

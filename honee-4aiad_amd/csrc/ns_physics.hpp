@@ -292,6 +292,15 @@ __device__ __forceinline__ T linear_ramp(T amplitude, T length, T start, T x) {
   return 0;
 }
 
+// Physical gradient state in direction j from reference gradients gref[d][c] (one column of
+// StatePhysicalGradientFromReference, newtonian_state.h:581-593)
+template <int VAR, typename T>
+__device__ __forceinline__ St<T> phys_gradient_dir(const Gas<T> &g, const St<T> &s, const T gref[3][5], const T dXdx[3][3], int j) {
+  T dq[5];
+  for (int c = 0; c < 5; c++) dq[c] = gref[0][c] * dXdx[0][j] + gref[1][c] * dXdx[1][j] + gref[2][c] * dXdx[2][j];
+  return state_from_q_fwd<VAR>(g, s, dq);
+}
+
 // Physical gradient states from reference gradients gref[d][c] (StatePhysicalGradientFromReference,
 // newtonian_state.h:581-593)
 template <int VAR, typename T>

@@ -97,8 +97,9 @@ _libs: dict = {}
 def load_libs():
     """Load libceed.so and libhonee_setup.so from the package's lib/ directory (raises if they are not built)."""
     if "setup" not in _libs:
-        ceed_path = os.path.join(LIB_DIR, "libceed.so")
-        setup_path = os.path.join(LIB_DIR, "libhonee_setup.so")
+        lib_dir = os.environ.get("HONEE_MI355X_LIB_DIR", LIB_DIR)
+        ceed_path = os.path.join(lib_dir, "libceed.so")
+        setup_path = os.path.join(lib_dir, "libhonee_setup.so")
         for p in (ceed_path, setup_path):
             if not os.path.exists(p):
                 raise RuntimeError(f"MI355X backend library {p} is not built; run __graft_entry__.build()")
