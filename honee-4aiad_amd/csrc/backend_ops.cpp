@@ -272,6 +272,27 @@ static int newtonian_context(CeedOperator op, CeedMiNewtonianContext *ctx) {
   return 0;
 }
 
+// Output binding of a scattering volume kernel.  Scatter mode 0 adds into the L-vector with FP64 atomics; mode 1
+// (deterministic two-pass) writes the element-node contributions to the Ceed's E-vector workspace, and
+// scatter_finish sums them per node in a fixed order with launch_node_gather.
+static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
+  a.ev = nullptr;
+  if (ceed->scatter != 1) return 0;
+  MI_CALL(rstr_transpose_map(p->r_q));
+  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->P * p->P * p->P * 5, &a.ev));
+  return 0;
+}
+
+static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a) {
+  if (!a.ev) return 0;
+  CeedElemRestriction r = p->r_q;
+  hipEvent_t          ev;
+  MI_CALL(timing_begin(ceed, "scatter_node_gather", &ev));
+  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, r->d_t_index, r->d_t_offset, a.ev, a.v, false, ceed->stream));
+  MI_CALL(timing_end(ceed, "scatter_node_gather", ev));
+  return 0;
+}
+
 int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
   if (!op->plan) MI_CALL(plan_bind(op));
   OpPlan      *p    = op->plan;
@@ -308,6 +329,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       a.comp_stride = p->r_q->comp_stride;
       MI_CALL(vec_device_read(in, &a.u));
       MI_CALL(vec_device_write(out, &a.v, true));
+      MI_CALL(scatter_prepare(p, ceed, a));
       MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
       a.qd = p->qd;
       a.jd = p->jd;
@@ -327,6 +349,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
         MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
         MI_HIP(ceed, launch_ifunction(p->P, p->Q, p->var, a, ctx, s));
         MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+        MI_CALL(scatter_finish(p, ceed, a));
       } else {
         const int need = 14 + (ctx.idl_enable ? 1 : 0);
         if (p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
@@ -335,6 +358,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
         MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
         MI_HIP(ceed, launch_ijacobian(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, s));
         MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+        MI_CALL(scatter_finish(p, ceed, a));
       }
       return 0;
     }
@@ -361,6 +385,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       } else {
         if (out->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: output must have the L-vector size", op->qf->name.c_str());
         MI_CALL(vec_device_write(out, &a.v, true));
+        MI_CALL(scatter_prepare(p, ceed, a));
       }
       if (p->kind == QfKind::RHSFunction) {
         MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
@@ -373,6 +398,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       else if (p->kind == QfKind::MonitorKE) MI_HIP(ceed, launch_monitor_ke(p->P, p->Q, p->var, a, ctx, s));
       else MI_HIP(ceed, launch_mass(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, s));
       MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      MI_CALL(scatter_finish(p, ceed, a));
       return 0;
     }
     default:

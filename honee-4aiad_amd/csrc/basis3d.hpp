@@ -161,4 +161,193 @@ struct Elem {
   }
 };
 
+
+// Multi-component basis stage: the element's LDS stage X[slot][Q^3] holds one slot per (field component, point);
+// every pass treats NC components together, so each 1D coefficient row is read from LDS once per pass and the NC
+// independent contractions overlap their LDS latency.  Barriers are block-wide; a pass reads other threads' slots only
+// after a barrier that follows their writes.
+template <int P, int Q>
+struct Stage {
+  using D = Dims<P, Q>;
+  static constexpr int Q3 = D::Q3;
+  const double *B;   // [Q][P]
+  const double *Dc;  // [Q][Q]
+  double (*X)[Q3];
+  int t, qx, qy, qz;
+
+  __device__ Stage(const double *B_, const double *Dc_, double (*X_)[Q3], int t_) : B(B_), Dc(Dc_), X(X_), t(t_) {
+    qx = t % Q;
+    qy = (t / Q) % Q;
+    qz = t / (Q * Q);
+  }
+
+  // Interpolate NC components from node values u[c] (threads t < P^3) to this thread's quadrature point, leaving the
+  // results in slots dst..dst+NC-1.  Slots s0..s0+NC-1 and s1..s1+NC-1 are scratch; dst may equal s1 but not s0.
+  // Ends without a barrier: only this thread's own dst slots are written by the last pass.
+  template <int NC>
+  __device__ void interp(const double (&u)[NC], int s0, int s1, int dst) {
+    if (t < D::P3) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[s0 + c][t] = u[c];
+    }
+    __syncthreads();
+    if (t < D::PPQ) {  // x: s1[k][j][ix] = sum_a B[ix][a] u[k][j][a]
+      const int ix = t % Q, kj = t / Q;
+      double    b[P];
+#pragma unroll
+      for (int i = 0; i < P; i++) b[i] = B[ix * P + i];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < P; i++) acc += b[i] * X[s0 + c][kj * P + i];
+        X[s1 + c][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (t < D::PQQ) {  // y: s0[k][iy][ix] = sum_b B[iy][b] s1[k][b][ix]
+      const int ix = t % Q, iy = (t / Q) % Q, k = t / (Q * Q);
+      double    b[P];
+#pragma unroll
+      for (int i = 0; i < P; i++) b[i] = B[iy * P + i];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < P; i++) acc += b[i] * X[s1 + c][(k * P + i) * Q + ix];
+        X[s0 + c][t] = acc;
+      }
+    }
+    __syncthreads();
+    {  // z: value at (qx, qy, qz)
+      double b[P];
+#pragma unroll
+      for (int i = 0; i < P; i++) b[i] = B[qz * P + i];
+      double r[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < P; i++) acc += b[i] * X[s0 + c][(i * Q + qy) * Q + qx];
+        r[c] = acc;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[dst + c][t] = r[c];
+    }
+  }
+
+  // Collocated reference gradient of the NC quadrature-point fields in slots src.. (written before the last barrier
+  // or by this thread): slot gdst + NC*d + c receives d/dX_d of component c.  Starts with a barrier.
+  template <int NC>
+  __device__ void grad(int src, int gdst) {
+    __syncthreads();
+    double dx[Q], dy[Q], dz[Q];
+#pragma unroll
+    for (int m = 0; m < Q; m++) {
+      dx[m] = Dc[qx * Q + m];
+      dy[m] = Dc[qy * Q + m];
+      dz[m] = Dc[qz * Q + m];
+    }
+    double g[3][NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      double gx = 0, gy = 0, gz = 0;
+#pragma unroll
+      for (int m = 0; m < Q; m++) {
+        gx += dx[m] * X[src + c][(qz * Q + qy) * Q + m];
+        gy += dy[m] * X[src + c][(qz * Q + m) * Q + qx];
+        gz += dz[m] * X[src + c][(m * Q + qy) * Q + qx];
+      }
+      g[0][c] = gx;
+      g[1][c] = gy;
+      g[2][c] = gz;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[gdst + NC * d + c][t] = g[d][c];
+  }
+
+  // Transpose of interp (+ collocated gradient) for NC components: the quadrature-point values are in slots v0..
+  // (HAS_V) and the reference-gradient test weights in slots g0 + NC*d + c (HAS_G), all written before the call.
+  // Returns in r[c] (threads t < P^3) sum_q B3(q,n) v_c(q) + sum_q sum_d G3_d(q,n) g_{d,c}(q) for node n = t.
+  // Uses slots v0.. and g0.. as scratch (needs g0 + 2*NC <= slot count when !HAS_G).  Starts and ends with a barrier.
+  template <int NC, bool HAS_V, bool HAS_G>
+  __device__ void transpose(int v0, int g0, double (&r)[NC]) {
+    __syncthreads();
+    double w[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) w[c] = HAS_V ? X[v0 + c][t] : 0.0;
+    if (HAS_G) {
+      double dx[Q], dy[Q], dz[Q];
+#pragma unroll
+      for (int m = 0; m < Q; m++) {
+        dx[m] = Dc[m * Q + qx];
+        dy[m] = Dc[m * Q + qy];
+        dz[m] = Dc[m * Q + qz];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = w[c];
+#pragma unroll
+        for (int m = 0; m < Q; m++) {
+          acc += dx[m] * X[g0 + c][(qz * Q + qy) * Q + m];
+          acc += dy[m] * X[g0 + NC + c][(qz * Q + m) * Q + qx];
+          acc += dz[m] * X[g0 + 2 * NC + c][(m * Q + qy) * Q + qx];
+        }
+        w[c] = acc;
+      }
+      __syncthreads();
+    }
+    const int sa = v0, sb = g0, sc = g0 + NC;
+#pragma unroll
+    for (int c = 0; c < NC; c++) X[sa + c][t] = w[c];
+    __syncthreads();
+    if (t < D::PQQ) {  // z^T: sb[c][iy][ix] = sum_qz B[qz][c] w[qz][iy][ix]
+      const int ix = t % Q, iy = (t / Q) % Q, cz = t / (Q * Q);
+      double    b[Q];
+#pragma unroll
+      for (int z = 0; z < Q; z++) b[z] = B[z * P + cz];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int z = 0; z < Q; z++) acc += b[z] * X[sa + c][(z * Q + iy) * Q + ix];
+        X[sb + c][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (t < D::PPQ) {  // y^T: sc[c][b][ix] = sum_qy B[qy][b] sb[c][qy][ix]
+      const int ix = t % Q, bb = (t / Q) % P, cz = t / (Q * P);
+      double    b[Q];
+#pragma unroll
+      for (int y = 0; y < Q; y++) b[y] = B[y * P + bb];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int y = 0; y < Q; y++) acc += b[y] * X[sb + c][(cz * Q + y) * Q + ix];
+        X[sc + c][t] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++) r[c] = 0;
+    if (t < D::P3) {  // x^T: r[c][b][a] = sum_qx B[qx][a] sc[c][b][qx]
+      const int a = t % P, cb = t / P;
+      double    b[Q];
+#pragma unroll
+      for (int x = 0; x < Q; x++) b[x] = B[x * P + a];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int x = 0; x < Q; x++) acc += b[x] * X[sc + c][cb * Q + x];
+        r[c] = acc;
+      }
+    }
+    __syncthreads();
+  }
+};
+
 }  // namespace basis3

@@ -133,7 +133,13 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
     } else if (k == "jacobian_precision") {
       c->jac_fp32 = (v == "fp32" || v == "float");
     } else if (k == "scatter") {
-      c->colored = (v == "colored");
+      if (v == "atomic") c->scatter = 0;
+      else if (v == "twopass") c->scatter = 1;
+      else {
+        fprintf(stderr, "CeedInit: scatter must be atomic or twopass, got '%s'\n", v.c_str());
+        delete c;
+        return CEED_ERROR_UNSUPPORTED;
+      }
     } else if (!k.empty()) {
       fprintf(stderr, "CeedInit: unknown resource option '%s'\n", k.c_str());
       delete c;
@@ -198,6 +204,7 @@ extern "C" int CeedDestroy(Ceed *ceed) {
   for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
   if (c->d_scratch) hipFree(c->d_scratch);
   if (c->h_scratch) hipHostFree(c->h_scratch);
+  if (c->d_work) hipFree(c->d_work);
   delete c;
   *ceed = nullptr;
   return CEED_ERROR_SUCCESS;
@@ -687,6 +694,52 @@ int rstr_device_offsets(CeedElemRestriction r, const int **d) {
   return 0;
 }
 
+// Transpose of the offsets: L-vector offsets in ascending order, each with the ascending list of element-node
+// indices e * elem_size + j that gather from it.  Built once per restriction on the host.
+int rstr_transpose_map(CeedElemRestriction r) {
+  if (r->d_t_rowptr) return 0;
+  if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no transpose map");
+  const size_t                         n = r->h_offsets.size();
+  std::vector<std::pair<CeedInt, int>> pairs(n);
+  for (size_t i = 0; i < n; i++) pairs[i] = {r->h_offsets[i], (int)i};
+  std::sort(pairs.begin(), pairs.end());
+  std::vector<int> rowptr, index(n), toff;
+  rowptr.reserve(n / 2 + 2);
+  for (size_t i = 0; i < n; i++) {
+    if (i == 0 || pairs[i].first != pairs[i - 1].first) {
+      rowptr.push_back((int)i);
+      toff.push_back(pairs[i].first);
+    }
+    index[i] = pairs[i].second;
+  }
+  rowptr.push_back((int)n);
+  r->t_nrows      = (int)toff.size();
+  r->t_covers_all = (CeedSize)r->t_nrows * r->num_comp == r->l_size;
+  const size_t nr = std::max<size_t>(toff.size(), 1);
+  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_rowptr, sizeof(int) * (nr + 1)));
+  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_index, sizeof(int) * std::max<size_t>(n, 1)));
+  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_offset, sizeof(int) * nr));
+  MI_HIP(r->ceed, hipMemcpy(r->d_t_rowptr, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
+  if (n) MI_HIP(r->ceed, hipMemcpy(r->d_t_index, index.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (!toff.empty()) MI_HIP(r->ceed, hipMemcpy(r->d_t_offset, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Device workspace of at least len doubles, owned by the Ceed and reused by every operator apply on its stream.
+int ceed_workspace(Ceed ceed, size_t len, double **d) {
+  if (ceed->work_len < len) {
+    if (ceed->d_work) {
+      MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
+      MI_HIP(ceed, hipFree(ceed->d_work));
+      ceed->d_work = nullptr;
+    }
+    MI_HIP(ceed, hipMalloc((void **)&ceed->d_work, sizeof(double) * len));
+    ceed->work_len = len;
+  }
+  *d = ceed->d_work;
+  return 0;
+}
+
 extern "C" int CeedElemRestrictionReferenceCopy(CeedElemRestriction rstr, CeedElemRestriction *rstr_copy) {
   if (rstr != CEED_ELEMRESTRICTION_NONE) rstr->refcount++;
   if (*rstr_copy) MI_CALL(CeedElemRestrictionDestroy(rstr_copy));
@@ -765,6 +818,9 @@ extern "C" int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
     return CEED_ERROR_SUCCESS;
   }
   if (r->d_offsets) hipFree(r->d_offsets);
+  if (r->d_t_rowptr) hipFree(r->d_t_rowptr);
+  if (r->d_t_index) hipFree(r->d_t_index);
+  if (r->d_t_offset) hipFree(r->d_t_offset);
   Ceed c = r->ceed;
   delete r;
   CeedDestroy(&c);

@@ -18,7 +18,9 @@ struct Ceed_private {
   int              device   = 0;
   hipStream_t      stream   = nullptr;  // null stream unless CeedSetStream
   bool             jac_fp32 = false;
-  bool             colored  = false;
+  int              scatter  = 0;  // ScatterMode: 0 atomic, 1 two-pass (deterministic)
+  double          *d_work   = nullptr;  // E-vector workspace of the two-pass scatter
+  size_t           work_len = 0;
   CeedErrorHandler handler  = nullptr;
   std::string      err_msg;
   // kernel timing
@@ -55,6 +57,11 @@ struct CeedElemRestriction_private {
   CeedInt              strides[3] = {0, 0, 0};
   std::vector<CeedInt> h_offsets;
   int                 *d_offsets = nullptr;
+  // transpose map for the two-pass scatter: row k gathers the E-vector entries t_index[t_rowptr[k]..t_rowptr[k+1])
+  // (ascending element-node index) into L-vector offset t_offset[k]
+  int                  t_nrows     = 0;
+  bool                 t_covers_all = false;  // the rows cover every L-vector entry
+  int                 *d_t_rowptr = nullptr, *d_t_index = nullptr, *d_t_offset = nullptr;
 };
 
 struct CeedBasis_private {
@@ -162,6 +169,8 @@ int  ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int
   } while (0)
 
 int vec_device_read(CeedVector v, const double **d);
+int rstr_transpose_map(CeedElemRestriction r);
+int ceed_workspace(Ceed ceed, size_t len, double **d);
 int vec_device_write(CeedVector v, double **d, bool keep_contents);
 int vec_device_shadow32(CeedVector v, float **d, bool create);
 int rstr_device_offsets(CeedElemRestriction r, const int **d);

@@ -72,7 +72,30 @@ __global__ void k_rstr_t(const int *off, int nelem, int esize, int ncomp, int cs
   }
 }
 
+// Deterministic second scatter pass: one thread per (L-vector node, component); the node's element-node
+// contributions are summed in ascending element-node order.
+__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
+                              const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
+  const long long n = (long long)nrows * ncomp;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(i / ncomp), c = (int)(i % ncomp);
+    double    acc = 0;
+    for (int j = rowptr[row]; j < rowptr[row + 1]; j++) acc += ev[(long long)index[j] * ncomp + c];
+    double *dst = &v[toff[row] + (long long)c * cs];
+    *dst        = overwrite ? acc : *dst + acc;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
+                              const double *ev, double *v, bool overwrite, hipStream_t s) {
+  const long long n = (long long)nrows * ncomp;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_node_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev, v,
+                     overwrite);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -26,6 +26,25 @@ constexpr int elems_per_block() {
   return (Q * Q * Q >= 64) ? 1 : 64 / (Q * Q * Q);
 }
 
+// Element-batch schedule of the persistent blocks.  The workgroups of a dispatch are distributed round-robin over
+// the 8 XCDs of an MI355X, each with its own L2 (block b runs on XCD b % 8).  Each XCD gets one contiguous range of
+// element batches, which its blocks sweep together, so the nodes that neighbouring elements share are fetched into
+// one L2 once instead of into several.
+constexpr int kNumXcd = 8;
+
+struct BatchRange {
+  long long first, end, stride;
+};
+
+__device__ __forceinline__ BatchRange batch_range(long long nbatch) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G % kNumXcd != 0) return {b, nbatch, G};
+  const long long per = (nbatch + kNumXcd - 1) / kNumXcd;
+  const int       x = b % kNumXcd, j = b / kNumXcd;
+  const long long lo = x * per, hi = (x + 1) * per < nbatch ? (x + 1) * per : nbatch;
+  return {lo + j, hi, G / kNumXcd};
+}
+
 template <int P, int Q>
 __device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc) {
   for (int i = threadIdx.x; i < Q * P; i += blockDim.x) sB[i] = bt.B[i];
@@ -84,10 +103,14 @@ __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
 // ------------------------------------------------------------------------------------------------------------------
 // Generic fused volume kernel
 // ------------------------------------------------------------------------------------------------------------------
-// The interpolated inputs of every quadrature point and the quadrature-point outputs are staged in LDS (sX), one slot
-// per (field component, point): the basis stages then work one component at a time with few live registers, and the
-// register peak of the kernel is that of the quadrature-point body alone.  Each thread reads and writes only its own
-// point's slots, so the hand-over between stages needs no barrier.
+// The interpolated inputs of every quadrature point and the quadrature-point outputs are staged in LDS (X), one slot
+// per (field component, point).  The basis stages treat the five components of a field together (basis3::Stage), so
+// a 1D coefficient row is read once per pass and five independent contractions are in flight; the quadrature-point
+// body then runs with only its own registers live.
+//
+// Slot map of the stage: u -> 0..4, reference gradient of u -> 5 + 5 d + c (kGrad), u2 -> 20..24 (kGrad) or 5..9;
+// outputs: v -> 0..4, gradient test weights -> 5 + 5 d + c.  Interpolation scratch is chosen from the slots that are
+// not live at that point.
 template <int P, int Q, class Op>
 __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm           = Dims<P, Q>;
@@ -95,40 +118,37 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
   constexpr int Q3   = Dm::Q3;
   constexpr int NIN  = 5 + (Op::kGrad ? 15 : 0) + (Op::kIn2 ? 5 : 0);
   constexpr int NOUT = Op::kScatter ? 20 : 1;
-  constexpr int NST  = NIN > NOUT ? NIN : NOUT;
+  constexpr int NST0 = NIN > NOUT ? NIN : NOUT;
+  constexpr int NST  = NST0 > 15 ? NST0 : 15;            // interpolation scratch needs slots up to 14
+  constexpr int U2   = Op::kGrad ? 20 : 5;               // slot of u2
+  constexpr int SU2  = Op::kGrad ? 0 : 10;               // scratch of the u2 interpolation
+  constexpr int SU   = (Op::kIn2 && !Op::kGrad) ? 10 : 5;  // scratch of the u interpolation
+  static_assert(!Op::kIn2 || Op::kGrad || NST >= 20, "stage too small for the u2 scratch");
   __shared__ double sB[Q * P], sDc[Q * Q];
-  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
   __shared__ double sX[EPB][NST][Q3];
+  static_assert(Dm::P3 <= Q3, "P <= Q is required");
   load_tables<P, Q>(a.bt, sB, sDc);
   const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
+  basis3::Stage<P, Q> st(sB, sDc, sX[le], t);
   double(*X)[Q3]         = sX[le];
-  const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
+  const BatchRange br    = batch_range(((long long)a.nelem + EPB - 1) / EPB);
 
-  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+  for (long long bt = br.first; bt < br.end; bt += br.stride) {
     const long long e_  = bt * EPB + le;
     const bool      ok  = e_ < a.nelem;
     const int       e   = ok ? (int)e_ : 0;
-    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
-
-#pragma unroll 1
-    for (int c = 0; c < 5; c++) {
-      const double un = (t < Dm::P3) ? a.u[off + c * a.comp_stride] : 0.0;
-      const double uq = el.interp(un);
-      X[c][t]         = uq;
-      if constexpr (Op::kGrad) {
-        double gg[3];
-        el.grad(uq, gg);
-        X[5 + c][t]  = gg[0];
-        X[10 + c][t] = gg[1];
-        X[15 + c][t] = gg[2];
-      }
-    }
+    const bool      nd  = t < Dm::P3;
+    const int       off = nd ? a.offsets[(long long)e * Dm::P3 + t] : 0;
+    double          un[5], u2n[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) un[c] = nd ? a.u[off + c * a.comp_stride] : 0.0;
     if constexpr (Op::kIn2) {
-      constexpr int base = Op::kGrad ? 20 : 5;
-#pragma unroll 1
-      for (int c = 0; c < 5; c++) X[base + c][t] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
+#pragma unroll
+      for (int c = 0; c < 5; c++) u2n[c] = nd ? a.u2[off + c * a.comp_stride] : 0.0;
+      st.template interp<5>(u2n, SU2, SU2 + 5, U2);
     }
+    st.template interp<5>(un, SU, SU + 5, 0);
+    if constexpr (Op::kGrad) st.template grad<5>(0, 5);
     {
       double u[5], gu[3][5], u2[5], v[5], gw[3][5];
 #pragma unroll
@@ -137,25 +157,35 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
         if constexpr (Op::kGrad)
 #pragma unroll
           for (int d = 0; d < 3; d++) gu[d][c] = X[5 + 5 * d + c][t];
-        if constexpr (Op::kIn2) u2[c] = X[(Op::kGrad ? 20 : 5) + c][t];
+        if constexpr (Op::kIn2) u2[c] = X[U2 + c][t];
       }
       Op::template qf<Q>(a, ctx, e, t, ok, u, gu, u2, v, gw);
       if constexpr (Op::kScatter) {
+        __syncthreads();  // other threads' gradient passes read slots 0..4
 #pragma unroll
         for (int c = 0; c < 5; c++) {
-          X[c][t] = v[c];
+          if (Op::kOutV) X[c][t] = v[c];
+          if (Op::kOutG)
 #pragma unroll
-          for (int d = 0; d < 3; d++) X[5 + 5 * d + c][t] = gw[d][c];
+            for (int d = 0; d < 3; d++) X[5 + 5 * d + c][t] = gw[d][c];
         }
       }
     }
     if constexpr (Op::kScatter) {
-#pragma unroll 1
-      for (int c = 0; c < 5; c++) {
-        const double g3[3] = {X[5 + c][t], X[10 + c][t], X[15 + c][t]};
-        const double r     = el.template transpose<Op::kOutV, Op::kOutG>(X[c][t], g3);
-        if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
+      double r[5];
+      st.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
+      if (ok && nd) {
+        if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
+          double *dst = a.ev + ((long long)e * Dm::P3 + t) * 5;
+#pragma unroll
+          for (int c = 0; c < 5; c++) dst[c] = r[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; c++) atomicAdd(&a.v[off + c * a.comp_stride], r[c]);
+        }
       }
+    } else {
+      __syncthreads();  // the next batch's first writes go to slots that the gradient pass read
     }
   }
 }
@@ -226,8 +256,9 @@ int grid_for(K kernel, int block, long long nbatch) {
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
-  long long g = (long long)ncu * (per_cu > 0 ? per_cu : 1) * 8;
-  if (g > nbatch) g = nbatch;
+  // one persistent block per resident slot: each XCD's blocks then sweep a contiguous window of its batch range
+  long long g = (long long)ncu * (per_cu > 0 ? per_cu : 1);
+  if (g > nbatch) g = nbatch >= kNumXcd ? nbatch / kNumXcd * kNumXcd : nbatch;
   if (g < 1) g = 1;
   return (int)g;
 }

@@ -145,3 +145,30 @@ def test_repeated_apply_is_stable():
     v_ref, _ = O.ifunction(om, oc, qd, case["q"], case["qdot"])
     assert rel_err(g2, v_ref)[0] < 1e-12
     assert rel_err(g1, g2)[0] > 1e-10  # dt enters Tau, so the residual changed
+
+
+@pytest.mark.parametrize("degree,periodic", [(3, (True, True, True)), (2, (False, True, False)), (1, (True, False, True))])
+def test_twopass_scatter(degree, periodic):
+    """Deterministic two-pass scatter (resource option scatter=twopass): oracle parity for the IFunction and the
+    IJacobian, and bitwise-identical results on repeated applies."""
+    _say(f"two-pass scatter degree={degree} periodic={periodic}")
+    case = make_case(shape=(3, 2, 2), degree=degree, periodic=periodic, state_var=1, stabilization=2, seed=90 + degree)
+    om, oc, qd = oracle_setup(case)
+    v_ref, jac_ref = O.ifunction(om, oc, qd, case["q"], case["qdot"])
+    prob, g = _run_ifunction(case, resource="/gpu/hip/mi355x:scatter=twopass")
+    e2, emax = rel_err(g, v_ref)
+    assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+    g2 = np.zeros_like(g)
+    prob.ifunction(case["q"], case["qdot"], g2, time=0.0)
+    assert np.array_equal(g, g2)
+    shift = 250.0
+    oc.ijacobian_time_shift = shift
+    dq = case["rng"].uniform(-1e-3, 1e-3, case["q"].shape)
+    vj_ref = O.ijacobian(om, oc, qd, jac_ref, dq)
+    vj = np.zeros_like(dq)
+    prob.ijacobian(dq, vj, shift)
+    e2, emax = rel_err(vj, vj_ref)
+    assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+    vj2 = np.zeros_like(dq)
+    prob.ijacobian(dq, vj2, shift)
+    assert np.array_equal(vj, vj2)
