@@ -276,24 +276,46 @@ static int newtonian_context(CeedOperator op, CeedMiNewtonianContext *ctx) {
 // (deterministic two-pass) writes the element-node contributions to the Ceed's E-vector workspace, and
 // scatter_finish sums them per node in a fixed order with launch_node_gather.
 static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
-  a.ev = nullptr;
+  a.ev   = nullptr;
+  a.perm = nullptr;
   if (ceed->scatter != 1) return 0;
   MI_CALL(rstr_transpose_map(p->r_q));
   MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->P * p->P * p->P * 5, &a.ev));
+  a.perm = p->r_q->d_t_perm;
   return 0;
 }
 
-static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a) {
+static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a, bool overwrite) {
   if (!a.ev) return 0;
   CeedElemRestriction r = p->r_q;
   hipEvent_t          ev;
   MI_CALL(timing_begin(ceed, "scatter_node_gather", &ev));
-  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, r->d_t_index, r->d_t_offset, a.ev, a.v, false, ceed->stream));
+  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, r->d_t_offset, a.ev, a.v, overwrite, ceed->stream));
   MI_CALL(timing_end(ceed, "scatter_node_gather", ev));
   return 0;
 }
 
-int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
+// True when op_plan_apply(op, in, out, true) writes every entry of out, so CeedOperatorApply can skip zeroing it:
+// a scattering volume operator in two-pass mode whose restriction rows cover the whole L-vector.
+bool op_plan_overwrites(CeedOperator op, CeedVector out) {
+  if (op->composite || op->ceed->scatter != 1) return false;
+  if (!op->plan && plan_bind(op)) return false;
+  const OpPlan *p = op->plan;
+  switch (p->kind) {
+    case QfKind::IFunction:
+    case QfKind::IJacobian:
+    case QfKind::RHSFunction:
+    case QfKind::MassNewtonian:
+    case QfKind::Mass5:
+      break;
+    default:
+      return false;
+  }
+  if (rstr_transpose_map(p->r_q)) return false;
+  return p->r_q->t_covers_all && out->length == p->r_q->l_size;
+}
+
+int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite) {
   if (!op->plan) MI_CALL(plan_bind(op));
   OpPlan      *p    = op->plan;
   Ceed         ceed = op->ceed;
@@ -349,7 +371,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
         MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
         MI_HIP(ceed, launch_ifunction(p->P, p->Q, p->var, a, ctx, s));
         MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
-        MI_CALL(scatter_finish(p, ceed, a));
+        MI_CALL(scatter_finish(p, ceed, a, overwrite));
       } else {
         const int need = 14 + (ctx.idl_enable ? 1 : 0);
         if (p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
@@ -358,7 +380,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
         MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
         MI_HIP(ceed, launch_ijacobian(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, s));
         MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
-        MI_CALL(scatter_finish(p, ceed, a));
+        MI_CALL(scatter_finish(p, ceed, a, overwrite));
       }
       return 0;
     }
@@ -398,7 +420,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out) {
       else if (p->kind == QfKind::MonitorKE) MI_HIP(ceed, launch_monitor_ke(p->P, p->Q, p->var, a, ctx, s));
       else MI_HIP(ceed, launch_mass(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, s));
       MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
-      MI_CALL(scatter_finish(p, ceed, a));
+      MI_CALL(scatter_finish(p, ceed, a, overwrite));
       return 0;
     }
     default:

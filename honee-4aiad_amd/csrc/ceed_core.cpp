@@ -694,8 +694,9 @@ int rstr_device_offsets(CeedElemRestriction r, const int **d) {
   return 0;
 }
 
-// Transpose of the offsets: L-vector offsets in ascending order, each with the ascending list of element-node
-// indices e * elem_size + j that gather from it.  Built once per restriction on the host.
+// Transpose of the offsets for the two-pass scatter: L-vector offsets in ascending order (rows), each with the
+// ascending list of element-node indices e * elem_size + j that gather from it; t_perm maps an element node to its
+// slot in that node-sorted order.  Built once per restriction on the host.
 int rstr_transpose_map(CeedElemRestriction r) {
   if (r->d_t_rowptr) return 0;
   if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no transpose map");
@@ -703,24 +704,24 @@ int rstr_transpose_map(CeedElemRestriction r) {
   std::vector<std::pair<CeedInt, int>> pairs(n);
   for (size_t i = 0; i < n; i++) pairs[i] = {r->h_offsets[i], (int)i};
   std::sort(pairs.begin(), pairs.end());
-  std::vector<int> rowptr, index(n), toff;
+  std::vector<int> rowptr, perm(n), toff;
   rowptr.reserve(n / 2 + 2);
   for (size_t i = 0; i < n; i++) {
     if (i == 0 || pairs[i].first != pairs[i - 1].first) {
       rowptr.push_back((int)i);
       toff.push_back(pairs[i].first);
     }
-    index[i] = pairs[i].second;
+    perm[pairs[i].second] = (int)i;
   }
   rowptr.push_back((int)n);
   r->t_nrows      = (int)toff.size();
   r->t_covers_all = (CeedSize)r->t_nrows * r->num_comp == r->l_size;
   const size_t nr = std::max<size_t>(toff.size(), 1);
   MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_rowptr, sizeof(int) * (nr + 1)));
-  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_index, sizeof(int) * std::max<size_t>(n, 1)));
+  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_perm, sizeof(int) * std::max<size_t>(n, 1)));
   MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_offset, sizeof(int) * nr));
   MI_HIP(r->ceed, hipMemcpy(r->d_t_rowptr, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
-  if (n) MI_HIP(r->ceed, hipMemcpy(r->d_t_index, index.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (n) MI_HIP(r->ceed, hipMemcpy(r->d_t_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   if (!toff.empty()) MI_HIP(r->ceed, hipMemcpy(r->d_t_offset, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
   return 0;
 }
@@ -819,7 +820,7 @@ extern "C" int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
   }
   if (r->d_offsets) hipFree(r->d_offsets);
   if (r->d_t_rowptr) hipFree(r->d_t_rowptr);
-  if (r->d_t_index) hipFree(r->d_t_index);
+  if (r->d_t_perm) hipFree(r->d_t_perm);
   if (r->d_t_offset) hipFree(r->d_t_offset);
   Ceed c = r->ceed;
   delete r;
@@ -1449,25 +1450,37 @@ extern "C" int CeedOperatorGetActiveVectorLengths(CeedOperator op, CeedSize *inp
   return CEED_ERROR_SUCCESS;
 }
 
-static int apply_add(CeedOperator op, CeedVector in, CeedVector out) {
+// Apply the (sub-)operators in order, adding into out; when overwrite is set, the first leaf operator writes out
+// instead of adding to it (it covers every entry, see op_plan_overwrites).
+static int apply_add(CeedOperator op, CeedVector in, CeedVector out, bool overwrite) {
   if (op->composite) {
-    for (CeedOperator s : op->subs) MI_CALL(apply_add(s, in, out));
+    for (size_t i = 0; i < op->subs.size(); i++) MI_CALL(apply_add(op->subs[i], in, out, overwrite && i == 0));
     return 0;
   }
-  return op_plan_apply(op, in, out);
+  return op_plan_apply(op, in, out, overwrite);
+}
+
+static CeedOperator first_leaf(CeedOperator op) {
+  while (op && op->composite) op = op->subs.empty() ? nullptr : op->subs[0];
+  return op;
 }
 
 extern "C" int CeedOperatorApplyAdd(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
-  return apply_add(op, in, out);
+  return apply_add(op, in, out, false);
 }
 
 extern "C" int CeedOperatorApply(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
+  bool overwrite = false;
   if (out && out != CEED_VECTOR_NONE) {
-    double *d;
-    MI_CALL(vec_device_write(out, &d, false));
-    MI_HIP(op->ceed, launch_fill(d, 0.0, out->length, op->ceed->stream));
+    CeedOperator leaf = first_leaf(op);
+    overwrite         = leaf && op_plan_overwrites(leaf, out);
+    if (!overwrite) {
+      double *d;
+      MI_CALL(vec_device_write(out, &d, false));
+      MI_HIP(op->ceed, launch_fill(d, 0.0, out->length, op->ceed->stream));
+    }
   }
-  return apply_add(op, in, out);
+  return apply_add(op, in, out, overwrite);
 }
 
 extern "C" int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {

@@ -57,11 +57,11 @@ struct CeedElemRestriction_private {
   CeedInt              strides[3] = {0, 0, 0};
   std::vector<CeedInt> h_offsets;
   int                 *d_offsets = nullptr;
-  // transpose map for the two-pass scatter: row k gathers the E-vector entries t_index[t_rowptr[k]..t_rowptr[k+1])
-  // (ascending element-node index) into L-vector offset t_offset[k]
-  int                  t_nrows     = 0;
+  // node-sorted E-vector map of the two-pass scatter: element node i = e * elem_size + j writes its contribution to
+  // slot t_perm[i]; the slots of L-vector row k (offset t_offset[k]) are t_rowptr[k]..t_rowptr[k+1]-1, ordered by i
+  int                  t_nrows      = 0;
   bool                 t_covers_all = false;  // the rows cover every L-vector entry
-  int                 *d_t_rowptr = nullptr, *d_t_index = nullptr, *d_t_offset = nullptr;
+  int                 *d_t_rowptr = nullptr, *d_t_perm = nullptr, *d_t_offset = nullptr;
 };
 
 struct CeedBasis_private {
@@ -175,7 +175,8 @@ int vec_device_write(CeedVector v, double **d, bool keep_contents);
 int vec_device_shadow32(CeedVector v, float **d, bool create);
 int rstr_device_offsets(CeedElemRestriction r, const int **d);
 int basis_device_tables(CeedBasis b);
-int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out);
+int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite);
+bool op_plan_overwrites(CeedOperator op, CeedVector out);
 int op_plan_diagonal(CeedOperator op, CeedVector diag);
 void op_plan_destroy(OpPlan *plan);
 int timing_begin(Ceed ceed, const char *name, hipEvent_t *start);

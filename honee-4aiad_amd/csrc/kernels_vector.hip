@@ -72,15 +72,15 @@ __global__ void k_rstr_t(const int *off, int nelem, int esize, int ncomp, int cs
   }
 }
 
-// Deterministic second scatter pass: one thread per (L-vector node, component); the node's element-node
-// contributions are summed in ascending element-node order.
-__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
-                              const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
+// Deterministic second scatter pass: one thread per (L-vector node, component).  The node-sorted E-vector holds a
+// node's contributions in consecutive slots, so a wave reads one contiguous stretch of it.
+__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ toff,
+                              const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
   const long long n = (long long)nrows * ncomp;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const int row = (int)(i / ncomp), c = (int)(i % ncomp);
     double    acc = 0;
-    for (int j = rowptr[row]; j < rowptr[row + 1]; j++) acc += ev[(long long)index[j] * ncomp + c];
+    for (int j = rowptr[row]; j < rowptr[row + 1]; j++) acc += ev[(long long)j * ncomp + c];
     double *dst = &v[toff[row] + (long long)c * cs];
     *dst        = overwrite ? acc : *dst + acc;
   }
@@ -88,12 +88,11 @@ __global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restric
 
 }  // namespace
 
-hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
-                              const double *ev, double *v, bool overwrite, hipStream_t s) {
+hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_offset, const double *ev, double *v,
+                              bool overwrite, hipStream_t s) {
   const long long n = (long long)nrows * ncomp;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_node_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev, v,
-                     overwrite);
+  hipLaunchKernelGGL(k_node_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_offset, ev, v, overwrite);
   return hipGetLastError();
 }
 

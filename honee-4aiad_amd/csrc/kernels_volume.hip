@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
       st.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {
         if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
-          double *dst = a.ev + ((long long)e * Dm::P3 + t) * 5;
+          double *dst = a.ev + (long long)a.perm[(long long)e * Dm::P3 + t] * 5;
 #pragma unroll
           for (int c = 0; c < 5; c++) dst[c] = r[c];
         } else {
