@@ -26,6 +26,32 @@ struct OpPlan {
 
 void op_plan_destroy(OpPlan *plan) { delete plan; }
 
+static int plan_bind_fields(CeedOperator op);
+
+// The fused kernels address strided quadrature data in the backend layout [elem][comp][qpt], i.e. strides
+// {1, Q^3, ncomp Q^3} with one point per Q^3 slot; a restriction with other strides is not bound.
+static int check_point_layout(CeedOperator op, CeedElemRestriction r, int Q) {
+  if (!r) return 0;
+  const CeedInt Q3 = Q * Q * Q;
+  if (!(r->elem_size == Q3 && r->strides[0] == 1 && r->strides[1] == Q3 && r->strides[2] == (CeedInt)r->num_comp * Q3))
+    return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "strided quadrature data must use CEED_STRIDES_BACKEND (or strides {1, Q^3, ncomp Q^3}) "
+                                                    "with Q^3 = %d points per element",
+                  Q3);
+  return 0;
+}
+
+static int plan_bind(CeedOperator op) {
+  MI_CALL(plan_bind_fields(op));
+  OpPlan *p    = op->plan;
+  int     ierr = check_point_layout(op, p->r_qd, p->Q);
+  if (!ierr) ierr = check_point_layout(op, p->r_jd, p->Q);
+  if (ierr) {
+    delete p;
+    op->plan = nullptr;
+  }
+  return ierr;
+}
+
 static QDataLayout strided_layout(CeedElemRestriction r) {
   return QDataLayout{r->strides[0], r->strides[1], r->strides[2]};
 }
@@ -76,7 +102,7 @@ static int bind_coords(CeedOperator op, OpPlan *p, CeedOperatorField_private *f)
   return 0;
 }
 
-static int plan_bind(CeedOperator op) {
+static int plan_bind_fields(CeedOperator op) {
   CeedQFunction qf = op->qf;
   auto          p  = new OpPlan();
   p->kind          = qf->kind;
@@ -275,13 +301,17 @@ static int newtonian_context(CeedOperator op, CeedMiNewtonianContext *ctx) {
 // Output binding of a scattering volume kernel.  Scatter mode 0 adds into the L-vector with FP64 atomics; mode 1
 // (deterministic two-pass) writes the element-node contributions to the Ceed's E-vector workspace, and
 // scatter_finish sums them per node in a fixed order with launch_node_gather.
+// E-vector layout of the two-pass scatter: node-sorted (the fused kernel writes each element node to its slot in
+// node order; the gather streams) or element-ordered (coalesced fused-kernel writes; the gather reads by index).
+constexpr bool kSortedEvec = true;
+
 static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
   a.ev   = nullptr;
   a.perm = nullptr;
   if (ceed->scatter != 1) return 0;
   MI_CALL(rstr_transpose_map(p->r_q));
   MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->P * p->P * p->P * 5, &a.ev));
-  a.perm = p->r_q->d_t_perm;
+  a.perm = kSortedEvec ? p->r_q->d_t_perm : nullptr;
   return 0;
 }
 
@@ -290,7 +320,8 @@ static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a, bool overwr
   CeedElemRestriction r = p->r_q;
   hipEvent_t          ev;
   MI_CALL(timing_begin(ceed, "scatter_node_gather", &ev));
-  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, r->d_t_offset, a.ev, a.v, overwrite, ceed->stream));
+  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, a.perm ? nullptr : r->d_t_index, r->d_t_offset, a.ev, a.v,
+                                  overwrite, ceed->stream));
   MI_CALL(timing_end(ceed, "scatter_node_gather", ev));
   return 0;
 }

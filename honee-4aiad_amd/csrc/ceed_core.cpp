@@ -673,10 +673,10 @@ extern "C" int CeedElemRestrictionCreateStrided(Ceed ceed, CeedInt num_elem, Cee
   r->l_size          = l_size;
   r->strided         = true;
   r->backend_strides = strides == CEED_STRIDES_BACKEND;
-  if (r->backend_strides) {  // [comp][elem][point]: each component's points of consecutive elements are contiguous
+  if (r->backend_strides) {  // [elem][comp][point]: a component's points of one element are contiguous
     r->strides[0] = 1;
-    r->strides[1] = num_elem * elem_size;
-    r->strides[2] = elem_size;
+    r->strides[1] = elem_size;
+    r->strides[2] = elem_size * num_comp;
   } else {
     for (int i = 0; i < 3; i++) r->strides[i] = strides[i];
   }
@@ -704,7 +704,7 @@ int rstr_transpose_map(CeedElemRestriction r) {
   std::vector<std::pair<CeedInt, int>> pairs(n);
   for (size_t i = 0; i < n; i++) pairs[i] = {r->h_offsets[i], (int)i};
   std::sort(pairs.begin(), pairs.end());
-  std::vector<int> rowptr, perm(n), toff;
+  std::vector<int> rowptr, perm(n), index(n), toff;
   rowptr.reserve(n / 2 + 2);
   for (size_t i = 0; i < n; i++) {
     if (i == 0 || pairs[i].first != pairs[i - 1].first) {
@@ -712,6 +712,7 @@ int rstr_transpose_map(CeedElemRestriction r) {
       toff.push_back(pairs[i].first);
     }
     perm[pairs[i].second] = (int)i;
+    index[i]              = pairs[i].second;
   }
   rowptr.push_back((int)n);
   r->t_nrows      = (int)toff.size();
@@ -721,7 +722,9 @@ int rstr_transpose_map(CeedElemRestriction r) {
   MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_perm, sizeof(int) * std::max<size_t>(n, 1)));
   MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_offset, sizeof(int) * nr));
   MI_HIP(r->ceed, hipMemcpy(r->d_t_rowptr, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
+  MI_HIP(r->ceed, hipMalloc((void **)&r->d_t_index, sizeof(int) * std::max<size_t>(n, 1)));
   if (n) MI_HIP(r->ceed, hipMemcpy(r->d_t_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (n) MI_HIP(r->ceed, hipMemcpy(r->d_t_index, index.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   if (!toff.empty()) MI_HIP(r->ceed, hipMemcpy(r->d_t_offset, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
   return 0;
 }
@@ -821,6 +824,7 @@ extern "C" int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
   if (r->d_offsets) hipFree(r->d_offsets);
   if (r->d_t_rowptr) hipFree(r->d_t_rowptr);
   if (r->d_t_perm) hipFree(r->d_t_perm);
+  if (r->d_t_index) hipFree(r->d_t_index);
   if (r->d_t_offset) hipFree(r->d_t_offset);
   Ceed c = r->ceed;
   delete r;

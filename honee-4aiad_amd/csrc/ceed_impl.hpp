@@ -62,6 +62,7 @@ struct CeedElemRestriction_private {
   int                  t_nrows      = 0;
   bool                 t_covers_all = false;  // the rows cover every L-vector entry
   int                 *d_t_rowptr = nullptr, *d_t_perm = nullptr, *d_t_offset = nullptr;
+  int                 *d_t_index = nullptr;  // element node of each slot (inverse of t_perm)
 };
 
 struct CeedBasis_private {

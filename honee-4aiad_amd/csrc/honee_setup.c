@@ -239,14 +239,14 @@ int HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, doubl
   return 0;
 }
 
-// Copy backend-strided quadrature data to host as [elem][comp][qpt] (for checking against the CPU oracle)
+// Copy backend-strided quadrature data ([elem][comp][qpt], CEED_STRIDES_BACKEND of this backend) to host
 static int CopyStrided(HoneeMiProblem p, CeedVector v, CeedElemRestriction r, int ncomp, double *out) {
   const CeedScalar *a;
   CHK(p, CeedVectorGetArrayRead(v, CEED_MEM_HOST, &a));
   const long long Q3 = p->Q3;
   for (long long e = 0; e < p->nelem; e++)
     for (int c = 0; c < ncomp; c++)
-      for (long long i = 0; i < Q3; i++) out[(e * ncomp + c) * Q3 + i] = a[c * (long long)p->nelem * Q3 + e * Q3 + i];
+      for (long long i = 0; i < Q3; i++) out[(e * ncomp + c) * Q3 + i] = a[(e * ncomp + c) * Q3 + i];
   CHK(p, CeedVectorRestoreArrayRead(v, &a));
   return 0;
 }
@@ -266,7 +266,7 @@ int HoneeMiGetJacData32(HoneeMiProblem p, float *out) {
   const long long Q3 = p->Q3;
   for (long long e = 0; e < p->nelem; e++)
     for (int c = 0; c < p->num_comps_jac_data; c++)
-      for (long long i = 0; i < Q3; i++) out[(e * p->num_comps_jac_data + c) * Q3 + i] = h[c * (long long)p->nelem * Q3 + e * Q3 + i];
+      for (long long i = 0; i < Q3; i++) out[(e * p->num_comps_jac_data + c) * Q3 + i] = h[(e * p->num_comps_jac_data + c) * Q3 + i];
   free(h);
   return 0;
 }

@@ -36,7 +36,7 @@ struct VolumeArgs {
   const double *u2;       // second L-vector input (q_dot), if any
   double       *v;        // output L-vector (summed into with atomics when ev is null)
   double       *ev;       // two-pass scatter: node-sorted element-node contributions [nelem * P^3][5] (replaces v)
-  const int    *perm;     // two-pass scatter: slot of element node e * P^3 + j in ev
+  const int    *perm;     // two-pass scatter: slot of element node e * P^3 + j in ev (null: slot e * P^3 + j)
   const double *qdata;
   QDataLayout   qd;
   double       *jac;      // FP64 jac_data written by the IFunction
@@ -68,8 +68,9 @@ hipError_t launch_reciprocal(double *x, long long n, hipStream_t s);
 hipError_t launch_norm(const double *x, long long n, int type, double *d_partial, int npartial, hipStream_t s);
 hipError_t launch_multiplicity(const int *offsets, long long n_offsets, int ncomp, int comp_stride, double *mult, hipStream_t s);
 hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *u, double *e, hipStream_t s);
-// Second pass of the deterministic scatter: v[t_offset[k] + c * comp_stride] (+)= sum over slots j of row k,
-// t_rowptr[k] <= j < t_rowptr[k+1], of ev[j * ncomp + c], in ascending slot order.
-hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_offset, const double *ev, double *v,
-                              bool overwrite, hipStream_t s);
+// Second pass of the deterministic scatter: v[t_offset[k] + c * comp_stride] (+)= sum over j in
+// [t_rowptr[k], t_rowptr[k+1]) of ev[slot(j) * ncomp + c] in ascending j, with slot(j) = t_index[j] for an
+// element-ordered E-vector or slot(j) = j for a node-sorted one (t_index null).
+hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
+                              const double *ev, double *v, bool overwrite, hipStream_t s);
 hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u, hipStream_t s);

@@ -72,15 +72,32 @@ __global__ void k_rstr_t(const int *off, int nelem, int esize, int ncomp, int cs
   }
 }
 
-// Deterministic second scatter pass: one thread per (L-vector node, component).  The node-sorted E-vector holds a
-// node's contributions in consecutive slots, so a wave reads one contiguous stretch of it.
-__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ toff,
-                              const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
+// Deterministic second scatter pass: one thread per (L-vector node, component).  Row k's contributions are the
+// E-vector slots t_rowptr[k] .. t_rowptr[k+1]-1 (node-sorted E-vector) or the slots index[] of that range
+// (element-ordered E-vector); they are summed in ascending order.  The loads of up to 8 contributions (the most a
+// hexahedral mesh node has) are issued together.
+template <bool INDEXED>
+__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
+                              const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
   const long long n = (long long)nrows * ncomp;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const int row = (int)(i / ncomp), c = (int)(i % ncomp);
+    const int b = rowptr[row], m = rowptr[row + 1] - b;
     double    acc = 0;
-    for (int j = rowptr[row]; j < rowptr[row + 1]; j++) acc += ev[(long long)j * ncomp + c];
+    for (int k0 = 0; k0 < m; k0 += 8) {
+      double x[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        x[k] = 0;
+        if (k0 + k < m) {
+          const long long slot = INDEXED ? (long long)index[b + k0 + k] : (long long)(b + k0 + k);
+          x[k]                 = ev[slot * ncomp + c];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k0 + k < m) acc += x[k];
+    }
     double *dst = &v[toff[row] + (long long)c * cs];
     *dst        = overwrite ? acc : *dst + acc;
   }
@@ -88,11 +105,16 @@ __global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restric
 
 }  // namespace
 
-hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_offset, const double *ev, double *v,
-                              bool overwrite, hipStream_t s) {
+hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
+                              const double *ev, double *v, bool overwrite, hipStream_t s) {
   const long long n = (long long)nrows * ncomp;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_node_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_offset, ev, v, overwrite);
+  if (t_index)
+    hipLaunchKernelGGL(k_node_gather<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev, v,
+                       overwrite);
+  else
+    hipLaunchKernelGGL(k_node_gather<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev,
+                       v, overwrite);
   return hipGetLastError();
 }
 

@@ -95,9 +95,9 @@ __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
   const double d = J[0][0] * Ai[0][0] + J[1][0] * Ai[0][1] + J[2][0] * Ai[0][2];
   const double s = 1 / d;
   const double w = a.w1d[qx] * a.w1d[qy] * a.w1d[qz];
-  a.qdata[nsop::qidx(a.qd, e, 0, i)] = w * d;
+  a.qdata[nsop::qidx<Q>(a.qd, e, 0, i)] = w * d;
 #pragma unroll
-  for (int k = 0; k < 9; k++) a.qdata[nsop::qidx(a.qd, e, 1 + k, i)] = (&Ai[0][0])[k] * s;
+  for (int k = 0; k < 9; k++) a.qdata[nsop::qidx<Q>(a.qd, e, 1 + k, i)] = (&Ai[0][0])[k] * s;
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -176,7 +176,8 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
       st.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {
         if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
-          double *dst = a.ev + (long long)a.perm[(long long)e * Dm::P3 + t] * 5;
+          const long long en  = (long long)e * Dm::P3 + t;
+          double         *dst = a.ev + (a.perm ? (long long)a.perm[en] : en) * 5;
 #pragma unroll
           for (int c = 0; c < 5; c++) dst[c] = r[c];
         } else {
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
       for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
     }
     typename Op::Point p;
-    Op::load_point(a, ctx, e, t, ok, u2, p);
+    Op::template load_point<Q>(a, ctx, e, t, ok, u2, p);
     for (int c = 0; c < 5; c++) {
       double M[4][4] = {{0}};
       for (int s = 0; s < Op::kInSlots; s++) {

@@ -12,16 +12,19 @@
 
 namespace nsop {
 
+// Index of component c at point i of element e in strided quadrature data of the backend layout [elem][comp][qpt]
+// (the operator binding checks sn = 1 and sc = Q^3): the component offset is a compile-time constant.
+template <int Q>
 __device__ __forceinline__ long long qidx(const QDataLayout &l, int e, int c, int i) {
-  return (long long)i * l.sn + (long long)c * l.sc + (long long)e * l.se;
+  return (long long)e * l.se + c * (Q * Q * Q) + i;
 }
 
-template <typename T>
+template <int Q, typename T>
 __device__ __forceinline__ void load_qdata(const VolumeArgs &a, int e, int t, bool ok, T &wdetJ, T dXdx[3][3]) {
   if (ok) {
-    wdetJ = (T)a.qdata[qidx(a.qd, e, 0, t)];
+    wdetJ = (T)a.qdata[qidx<Q>(a.qd, e, 0, t)];
 #pragma unroll
-    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)a.qdata[qidx(a.qd, e, 1 + k, t)];
+    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)a.qdata[qidx<Q>(a.qd, e, 1 + k, t)];
   } else {  // padding thread of a partial batch: harmless geometry, zero weight
     wdetJ = 0;
 #pragma unroll
@@ -70,7 +73,7 @@ struct IFunction {
                             double qdot[5], double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<VAR>(q);
 #pragma unroll
@@ -139,20 +142,20 @@ struct IFunction {
     if (ok) {  // jac_data (newtonian.h:251-254)
       if (a.jac32) {
 #pragma unroll
-        for (int j = 0; j < 5; j++) a.jac32[qidx(a.jd, e, j, t)] = (float)q[j];
+        for (int j = 0; j < 5; j++) a.jac32[qidx<Q>(a.jd, e, j, t)] = (float)q[j];
 #pragma unroll
-        for (int j = 0; j < 6; j++) a.jac32[qidx(a.jd, e, 5 + j, t)] = (float)sr[j];
+        for (int j = 0; j < 6; j++) a.jac32[qidx<Q>(a.jd, e, 5 + j, t)] = (float)sr[j];
 #pragma unroll
-        for (int j = 0; j < 3; j++) a.jac32[qidx(a.jd, e, 11 + j, t)] = (float)Tau[j];
-        if (ctx.idl_enable) a.jac32[qidx(a.jd, e, 14, t)] = (float)sigma;
+        for (int j = 0; j < 3; j++) a.jac32[qidx<Q>(a.jd, e, 11 + j, t)] = (float)Tau[j];
+        if (ctx.idl_enable) a.jac32[qidx<Q>(a.jd, e, 14, t)] = (float)sigma;
       } else if (a.jac) {
 #pragma unroll
-        for (int j = 0; j < 5; j++) a.jac[qidx(a.jd, e, j, t)] = q[j];
+        for (int j = 0; j < 5; j++) a.jac[qidx<Q>(a.jd, e, j, t)] = q[j];
 #pragma unroll
-        for (int j = 0; j < 6; j++) a.jac[qidx(a.jd, e, 5 + j, t)] = sr[j];
+        for (int j = 0; j < 6; j++) a.jac[qidx<Q>(a.jd, e, 5 + j, t)] = sr[j];
 #pragma unroll
-        for (int j = 0; j < 3; j++) a.jac[qidx(a.jd, e, 11 + j, t)] = Tau[j];
-        if (ctx.idl_enable) a.jac[qidx(a.jd, e, 14, t)] = sigma;
+        for (int j = 0; j < 3; j++) a.jac[qidx<Q>(a.jd, e, 11 + j, t)] = Tau[j];
+        if (ctx.idl_enable) a.jac[qidx<Q>(a.jd, e, 14, t)] = sigma;
       }
     }
 #pragma unroll
@@ -175,28 +178,29 @@ struct IJacobian {
   };
 
   static constexpr int kInSlots = 4;  // linear inputs: value and three reference derivatives
+  template <int Q>
   __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *, Point &p) {
     const ns::Gas<T> g = ns::Gas<T>::from(ctx.gas);
-    load_qdata(a, e, t, ok, p.wdetJ, p.dXdx);
+    load_qdata<Q>(a, e, t, ok, p.wdetJ, p.dXdx);
     T qi[5], sr[6];
     p.sigma = 0;
     if (ok) {
       if (a.jac32) {
 #pragma unroll
-        for (int j = 0; j < 5; j++) qi[j] = (T)a.jac32[qidx(a.jd, e, j, t)];
+        for (int j = 0; j < 5; j++) qi[j] = (T)a.jac32[qidx<Q>(a.jd, e, j, t)];
 #pragma unroll
-        for (int j = 0; j < 6; j++) sr[j] = (T)a.jac32[qidx(a.jd, e, 5 + j, t)];
+        for (int j = 0; j < 6; j++) sr[j] = (T)a.jac32[qidx<Q>(a.jd, e, 5 + j, t)];
 #pragma unroll
-        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jac32[qidx(a.jd, e, 11 + j, t)];
-        if (ctx.idl_enable) p.sigma = (T)a.jac32[qidx(a.jd, e, 14, t)];
+        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jac32[qidx<Q>(a.jd, e, 11 + j, t)];
+        if (ctx.idl_enable) p.sigma = (T)a.jac32[qidx<Q>(a.jd, e, 14, t)];
       } else {
 #pragma unroll
-        for (int j = 0; j < 5; j++) qi[j] = (T)a.jacr[qidx(a.jd, e, j, t)];
+        for (int j = 0; j < 5; j++) qi[j] = (T)a.jacr[qidx<Q>(a.jd, e, j, t)];
 #pragma unroll
-        for (int j = 0; j < 6; j++) sr[j] = (T)a.jacr[qidx(a.jd, e, 5 + j, t)];
+        for (int j = 0; j < 6; j++) sr[j] = (T)a.jacr[qidx<Q>(a.jd, e, 5 + j, t)];
 #pragma unroll
-        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jacr[qidx(a.jd, e, 11 + j, t)];
-        if (ctx.idl_enable) p.sigma = (T)a.jacr[qidx(a.jd, e, 14, t)];
+        for (int j = 0; j < 3; j++) p.Tau[j] = (T)a.jacr[qidx<Q>(a.jd, e, 11 + j, t)];
+        if (ctx.idl_enable) p.sigma = (T)a.jacr[qidx<Q>(a.jd, e, 14, t)];
       }
     } else {
       padding_state<VAR>(qi);
@@ -278,7 +282,7 @@ struct IJacobian {
   __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double u[5], double gu[3][5],
                             double *, double v[5], double gw[3][5]) {
     Point p;
-    load_point(a, ctx, e, t, ok, nullptr, p);
+    load_point<Q>(a, ctx, e, t, ok, nullptr, p);
     T dq[5], gref[3][5];
 #pragma unroll
     for (int c = 0; c < 5; c++) {
@@ -300,7 +304,7 @@ struct RHSFunction {
                             double *, double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<ns::VAR_CONS>(q);
 #pragma unroll
@@ -368,9 +372,10 @@ struct MassNewtonian {
     ns::St<double> s;
     double         wdetJ, dXdx[3][3], Tau[3];
   };
+  template <int Q>
   __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *q2, Point &p) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
-    load_qdata(a, e, t, ok, p.wdetJ, p.dXdx);
+    load_qdata<Q>(a, e, t, ok, p.wdetJ, p.dXdx);
     double q[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) q[c] = q2[c];
@@ -400,7 +405,7 @@ struct MassNewtonian {
                             double q[5], double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<ns::VAR_CONS>(q);
 #pragma unroll
@@ -431,8 +436,9 @@ struct Mass5 {
   struct Point {
     double wdetJ;
   };
+  template <int Q>
   __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, const double *, Point &p) {
-    p.wdetJ = ok ? a.qdata[qidx(a.qd, e, 0, t)] : 0.0;
+    p.wdetJ = ok ? a.qdata[qidx<Q>(a.qd, e, 0, t)] : 0.0;
   }
   __device__ static void action(const CeedMiNewtonianContext &, const Point &p, const double u[5], const double (*)[5], double v[5],
                                 double gw[3][5]) {
@@ -445,7 +451,7 @@ struct Mass5 {
   template <int Q>
   __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, double u[5], double (*)[5], double *,
                             double v[5], double gw[3][5]) {
-    const double wdetJ = ok ? a.qdata[qidx(a.qd, e, 0, t)] : 0.0;
+    const double wdetJ = ok ? a.qdata[qidx<Q>(a.qd, e, 0, t)] : 0.0;
 #pragma unroll
     for (int c = 0; c < 5; c++) v[c] = wdetJ * u[c];
   }
@@ -464,7 +470,7 @@ struct MonitorKE {
     if (!ok) return;
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata(a, e, t, ok, wdetJ, dXdx);
+    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
     const ns::St<double> s = ns::state_from_q<VAR>(g, q);
     ns::St<double>       gs[3];
     ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);
@@ -479,11 +485,11 @@ struct MonitorKE {
     // vorticity = curl u with grad_u[i][j] = d u_i / d x_j = gs[j].Y[1 + i]
     const double w0 = gs[1].Y[3] - gs[2].Y[2], w1 = gs[2].Y[1] - gs[0].Y[3], w2 = gs[0].Y[2] - gs[1].Y[1];
     double      *out = a.jac;  // strided output field
-    out[qidx(a.jd, e, 0, t)] = wdetJ * 0.5 * s.U[0] * (s.Y[1] * s.Y[1] + s.Y[2] * s.Y[2] + s.Y[3] * s.Y[3]);
-    out[qidx(a.jd, e, 1, t)] = wdetJ * -2 * g.mu * SS;
-    out[qidx(a.jd, e, 2, t)] = wdetJ * -g.lambda * g.mu * (div_u * div_u);
-    out[qidx(a.jd, e, 3, t)] = wdetJ * s.Y[0] * div_u;
-    out[qidx(a.jd, e, 4, t)] = wdetJ * g.mu * (w0 * w0 + w1 * w1 + w2 * w2);
+    out[qidx<Q>(a.jd, e, 0, t)] = wdetJ * 0.5 * s.U[0] * (s.Y[1] * s.Y[1] + s.Y[2] * s.Y[2] + s.Y[3] * s.Y[3]);
+    out[qidx<Q>(a.jd, e, 1, t)] = wdetJ * -2 * g.mu * SS;
+    out[qidx<Q>(a.jd, e, 2, t)] = wdetJ * -g.lambda * g.mu * (div_u * div_u);
+    out[qidx<Q>(a.jd, e, 3, t)] = wdetJ * s.Y[0] * div_u;
+    out[qidx<Q>(a.jd, e, 4, t)] = wdetJ * g.mu * (w0 * w0 + w1 * w1 + w2 * w2);
   }
 };
 

@@ -18,7 +18,7 @@ enum : int { STAB_NONE = 0, STAB_SU = 1, STAB_SUPG = 2 };
 
 template <typename T>
 struct Gas {
-  T lambda, mu, k, cv, cp, gamma, Rgas;
+  T lambda, mu, k, cv, cp, gamma, Rgas, rcv;
   __host__ __device__ static Gas from(const CeedMiGas &g) {
     Gas r;
     r.lambda = (T)g.lambda;
@@ -28,14 +28,20 @@ struct Gas {
     r.cp     = (T)g.cp;
     r.gamma  = (T)(g.cp / g.cv);  // HeatCapacityRatio (newtonian_state.h:42)
     r.Rgas   = (T)(g.cp - g.cv);  // GasConstant (newtonian_state.h:44)
+    r.rcv    = T(1) / r.cv;
     return r;
   }
 };
 
+// A state with the per-point quantities its tangent maps reuse.  The reference recomputes them in every _fwd call;
+// here the energies are computed once with the same operations, and the divisions by the state's density or by
+// Rgas T^2 become multiplications by a reciprocal (a rounding-level difference).
 template <typename T>
 struct St {
   T U[5];
   T Y[5];
+  T rRT2, etot;       // 1 / (Rgas T^2) and cv T + |u|^2 / 2 (StateConservativeFromPrimitive_fwd)
+  T rrho, et, ei_c;   // 1 / rho, E / rho, E / rho - |u|^2 / 2 (StatePrimitiveFromConservative_fwd)
 };
 
 template <typename T>
@@ -61,15 +67,12 @@ __device__ __forceinline__ void prim_from_cons(const Gas<T> &g, const T U[5], T 
 // StatePrimitiveFromConservative_fwd (newtonian_state.h:75-89)
 template <typename T>
 __device__ __forceinline__ void prim_from_cons_fwd(const Gas<T> &g, const St<T> &s, const T dU[5], T dY[5]) {
-  for (int i = 0; i < 3; i++) dY[1 + i] = (dU[1 + i] - s.Y[1 + i] * dU[0]) / s.U[0];
-  T ek  = T(.5) * dot3(&s.Y[1], &s.Y[1]);
+  for (int i = 0; i < 3; i++) dY[1 + i] = (dU[1 + i] - s.Y[1 + i] * dU[0]) * s.rrho;
   T dek = dot3(&dY[1], &s.Y[1]);
-  T et  = s.U[4] / s.U[0];
-  T det = (dU[4] - et * dU[0]) / s.U[0];
-  T ei  = et - ek;
+  T det = (dU[4] - s.et * dU[0]) * s.rrho;
   T dei = det - dek;
-  dY[4] = dei / g.cv;
-  dY[0] = (g.gamma - 1) * (dU[0] * ei + s.U[0] * dei);
+  dY[4] = dei * g.rcv;
+  dY[0] = (g.gamma - 1) * (dU[0] * s.ei_c + s.U[0] * dei);
 }
 
 // StateConservativeFromPrimitive (newtonian_state.h:148-157)
@@ -85,13 +88,11 @@ __device__ __forceinline__ void cons_from_prim(const Gas<T> &g, const T Y[5], T 
 // StateConservativeFromPrimitive_fwd (newtonian_state.h:159-173)
 template <typename T>
 __device__ __forceinline__ void cons_from_prim_fwd(const Gas<T> &g, const St<T> &s, const T dY[5], T dU[5]) {
-  dU[0] = (dY[0] * s.Y[4] - s.Y[0] * dY[4]) / (g.Rgas * s.Y[4] * s.Y[4]);
+  dU[0] = (dY[0] * s.Y[4] - s.Y[0] * dY[4]) * s.rRT2;
   for (int i = 0; i < 3; i++) dU[1 + i] = dU[0] * s.Y[1 + i] + s.U[0] * dY[1 + i];
-  T ek  = T(.5) * dot3(&s.Y[1], &s.Y[1]);
   T dek = dot3(&dY[1], &s.Y[1]);
-  T ei  = g.cv * s.Y[4];
   T dei = g.cv * dY[4];
-  dU[4] = dU[0] * (ei + ek) + s.U[0] * (dei + dek);
+  dU[4] = dU[0] * s.etot + s.U[0] * (dei + dek);
 }
 
 // StateEntropyFromPrimitive (newtonian_state.h:91-103)
@@ -162,7 +163,7 @@ __device__ __forceinline__ void state_from_entropy_fwd(const Gas<T> &g, const St
   ds.U[4]     = ds.U[0] * (ei + ek) + s.U[0] * (dei + dek);
 }
 
-// StateFromQ (newtonian_state.h:393-407)
+// StateFromQ (newtonian_state.h:393-407), plus the quantities the state's tangent maps reuse
 template <int VAR, typename T>
 __device__ __forceinline__ St<T> state_from_q(const Gas<T> &g, const T q[5]) {
   St<T> s;
@@ -175,6 +176,16 @@ __device__ __forceinline__ St<T> state_from_q(const Gas<T> &g, const T q[5]) {
   } else {
     cons_from_entropy(g, q, s.U);
     prim_from_entropy(g, q, s.Y);
+  }
+  const T ek = T(.5) * dot3(&s.Y[1], &s.Y[1]);
+  s.rRT2     = T(1) / (g.Rgas * s.Y[4] * s.Y[4]);
+  s.etot     = g.cv * s.Y[4] + ek;
+  if constexpr (VAR == VAR_CONS) {
+    s.rrho = T(1) / s.U[0];
+    s.et   = s.U[4] / s.U[0];
+    s.ei_c = s.et - ek;
+  } else {
+    s.rrho = s.et = s.ei_c = T(0);  // prim_from_cons_fwd is only used for conservative states
   }
   return s;
 }
@@ -252,11 +263,12 @@ __device__ __forceinline__ void tau_diag_prim(const CeedMiTauCoeffs &tc, const G
   T       gg = 0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) gg += gij[i][j] * gij[i][j];
-  const T tau  = sq(rho) * (T(4.) * sq(dts) + vt) + (T)tc.Ctau_v * sq(g.mu) * gg;
-  const T fact = sqrt(tau);
-  Tau[0]       = (T)tc.Ctau_C * fact / (rho * (gij[0][0] + gij[1][1] + gij[2][2])) * T(0.125);
-  Tau[1]       = (T)tc.Ctau_M / fact;
-  Tau[2]       = (T)tc.Ctau_E / (fact * g.cv);
+  const T tau   = sq(rho) * (T(4.) * sq(dts) + vt) + (T)tc.Ctau_v * sq(g.mu) * gg;
+  const T fact  = sqrt(tau);
+  const T rfact = T(1) / fact;
+  Tau[0]        = (T)tc.Ctau_C * fact / (rho * (gij[0][0] + gij[1][1] + gij[2][2])) * T(0.125);
+  Tau[1]        = (T)tc.Ctau_M * rfact;
+  Tau[2]        = (T)tc.Ctau_E * rfact * g.rcv;
 }
 
 // StabilizationMatrix with dYFromTau (stabilization.h:14-39): stab[var][dir] added into gv
