@@ -40,11 +40,24 @@ static int check_point_layout(CeedOperator op, CeedElemRestriction r, int Q) {
   return 0;
 }
 
+// Quadrature-data fields read by the fused kernels: 10 components ({wdetJ, dXdx}) per point of every element, held in
+// a vector of at least that length (the kernels copy whole elements' qdata blocks).
+static int check_qdata_field(CeedOperator op, const OpPlan *p) {
+  if (p->kind == QfKind::Setup || !p->r_qd) return 0;
+  const CeedSize need = (CeedSize)p->nelem * 10 * p->Q * p->Q * p->Q;
+  if (p->r_qd->num_comp != 10 || p->r_qd->num_elem != p->nelem || !p->v_qdata || p->v_qdata->length < need)
+    return MI_ERR(op->ceed, CEED_ERROR_DIMENSION, "qdata must be a 10-component strided field over the operator's %d elements", p->nelem);
+  if (p->r_jd && (p->r_jd->num_elem != p->nelem || (p->v_jac && p->v_jac->length < (CeedSize)p->nelem * p->r_jd->num_comp * p->Q * p->Q * p->Q)))
+    return MI_ERR(op->ceed, CEED_ERROR_DIMENSION, "jac_data must be a strided field over the operator's %d elements", p->nelem);
+  return 0;
+}
+
 static int plan_bind(CeedOperator op) {
   MI_CALL(plan_bind_fields(op));
   OpPlan *p    = op->plan;
   int     ierr = check_point_layout(op, p->r_qd, p->Q);
   if (!ierr) ierr = check_point_layout(op, p->r_jd, p->Q);
+  if (!ierr) ierr = check_qdata_field(op, p);
   if (ierr) {
     delete p;
     op->plan = nullptr;

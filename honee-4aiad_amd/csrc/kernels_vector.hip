@@ -72,34 +72,59 @@ __global__ void k_rstr_t(const int *off, int nelem, int esize, int ncomp, int cs
   }
 }
 
-// Deterministic second scatter pass: one thread per (L-vector node, component).  Row k's contributions are the
+// Deterministic second scatter pass: one thread per (L-vector node, component) item.  Row k's contributions are the
 // E-vector slots t_rowptr[k] .. t_rowptr[k+1]-1 (node-sorted E-vector) or the slots index[] of that range
-// (element-ordered E-vector); they are summed in ascending order.  The loads of up to 8 contributions (the most a
-// hexahedral mesh node has) are issued together.
-template <bool INDEXED>
-__global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
-                              const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v, bool overwrite) {
-  const long long n = (long long)nrows * ncomp;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int row = (int)(i / ncomp), c = (int)(i % ncomp);
-    const int b = rowptr[row], m = rowptr[row + 1] - b;
-    double    acc = 0;
-    for (int k0 = 0; k0 < m; k0 += 8) {
-      double x[8];
+// (element-ordered E-vector); they are summed in ascending order.  Each thread handles NPT items a grid stride apart
+// and issues all their row-pointer loads, then all their contribution loads (up to 8 per row, the most a hexahedral
+// mesh node has), so a thread waits on two memory round trips for NPT items.
+template <bool INDEXED, int NPT>
+__global__ void __launch_bounds__(256) k_node_gather(int nrows, int ncomp, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
+                                                     const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v,
+                                                     bool overwrite) {
+  const long long n      = (long long)nrows * ncomp;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += stride * NPT) {
+    int row[NPT], c[NPT], b[NPT], m[NPT];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        x[k] = 0;
-        if (k0 + k < m) {
-          const long long slot = INDEXED ? (long long)index[b + k0 + k] : (long long)(b + k0 + k);
-          x[k]                 = ev[slot * ncomp + c];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (k0 + k < m) acc += x[k];
+    for (int j = 0; j < NPT; j++) {
+      const long long i = i0 + j * stride;
+      row[j]            = i < n ? (int)(i / ncomp) : 0;
+      c[j]              = i < n ? (int)(i % ncomp) : 0;
+      b[j]              = rowptr[row[j]];
+      m[j]              = i < n ? rowptr[row[j] + 1] - b[j] : 0;
     }
-    double *dst = &v[toff[row] + (long long)c * cs];
-    *dst        = overwrite ? acc : *dst + acc;
+    double acc[NPT];
+#pragma unroll
+    for (int j = 0; j < NPT; j++) acc[j] = 0;
+    bool more = false;
+#pragma unroll
+    for (int j = 0; j < NPT; j++) more |= m[j] > 8;
+    if (!more) {  // every row has at most 8 contributions: one batch of loads
+      double x[NPT][8];
+#pragma unroll
+      for (int j = 0; j < NPT; j++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          x[j][k] = 0;
+          if (k < m[j]) x[j][k] = ev[(INDEXED ? (long long)index[b[j] + k] : (long long)(b[j] + k)) * ncomp + c[j]];
+        }
+#pragma unroll
+      for (int j = 0; j < NPT; j++)
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (k < m[j]) acc[j] += x[j][k];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NPT; j++)
+        for (int k = 0; k < m[j]; k++) acc[j] += ev[(INDEXED ? (long long)index[b[j] + k] : (long long)(b[j] + k)) * ncomp + c[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+      if (i0 + j * stride < n) {
+        double *dst = &v[toff[row[j]] + (long long)c[j] * cs];
+        *dst        = overwrite ? acc[j] : *dst + acc[j];
+      }
+    }
   }
 }
 
@@ -107,13 +132,15 @@ __global__ void k_node_gather(int nrows, int ncomp, int cs, const int *__restric
 
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
                               const double *ev, double *v, bool overwrite, hipStream_t s) {
-  const long long n = (long long)nrows * ncomp;
+  constexpr int   NPT = 1;
+  const long long n   = (long long)nrows * ncomp;
   if (n <= 0) return hipSuccess;
+  const int grid = grid_for((n + NPT - 1) / NPT);
   if (t_index)
-    hipLaunchKernelGGL(k_node_gather<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev, v,
+    hipLaunchKernelGGL((k_node_gather<true, NPT>), dim3(grid), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev, v,
                        overwrite);
   else
-    hipLaunchKernelGGL(k_node_gather<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev,
+    hipLaunchKernelGGL((k_node_gather<false, NPT>), dim3(grid), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev,
                        v, overwrite);
   return hipGetLastError();
 }

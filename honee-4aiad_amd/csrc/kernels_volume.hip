@@ -111,8 +111,44 @@ __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
 // Slot map of the stage: u -> 0..4, reference gradient of u -> 5 + 5 d + c (kGrad), u2 -> 20..24 (kGrad) or 5..9;
 // outputs: v -> 0..4, gradient test weights -> 5 + 5 d + c.  Interpolation scratch is chosen from the slots that are
 // not live at that point.
+// Asynchronous copy of the quadrature data of element batch bt (EPB consecutive elements, [elem][10][Q^3] and
+// contiguous) into sQD through the LDS-DMA path (global_load_lds_dwordx4, 16 bytes per lane): no registers are held
+// while the loads are in flight.  The caller waits with s_waitcnt vmcnt(0) and a barrier before reading sQD.
+template <int EPB, int Q3>
+__device__ __forceinline__ void stage_qdata_async(const VolumeArgs &a, long long bt, double (*sQD)[10][Q3]) {
+  constexpr int   NT   = EPB * Q3;                       // threads of the block
+  constexpr int   NB   = EPB * 10 * Q3 * 8;              // bytes of a full batch
+  constexpr int   NCH  = (NB + NT * 16 - 1) / (NT * 16);  // copy instructions per thread
+  const long long e0   = bt * EPB;
+  const long long ne   = (long long)a.nelem - e0 < EPB ? (long long)a.nelem - e0 : EPB;
+  const int       nb   = (int)(ne * 10 * Q3 * 8);
+  const char     *src  = (const char *)(a.qdata + e0 * a.qd.se);
+  char           *dst  = (char *)&sQD[0][0][0];
+  const int       tid  = threadIdx.x;
+  const int       wave0 = tid - (tid & 63);  // first thread of this wave: the DMA writes lane i at base + 16 i
+#pragma unroll
+  for (int k = 0; k < NCH; k++) {
+    const int chunk = k * NT + tid;
+    if (chunk * 16 < nb)
+      __builtin_amdgcn_global_load_lds((const void *)(src + chunk * 16), (__attribute__((address_space(3))) void *)(dst + (k * NT + wave0) * 16),
+                                       16, 0, 0);
+  }
+}
+
+// The interpolated inputs of every quadrature point and the quadrature-point outputs are staged in LDS (X), one slot
+// per (field component, point).  The basis stages treat the five components of a field together (basis3::Stage), so
+// a 1D coefficient row is read once per pass and five independent contractions are in flight; the quadrature-point
+// body then runs with only its own registers live.
+//
+// Slot map of the stage: u -> 0..4, reference gradient of u -> 5 + 5 d + c (kGrad), u2 -> 20..24 (kGrad) or 5..9;
+// outputs: v -> 0..4, gradient test weights -> 5 + 5 d + c.  Interpolation scratch is chosen from the slots that are
+// not live at that point.
+//
+// Software pipeline over a block's element batches: the next batch's quadrature data is copied into sQD by LDS DMA
+// while the current batch's quadrature-point body runs, and its node values are loaded into registers while the
+// current batch's basis transpose runs, so neither waits on HBM latency at the start of the next batch.
 template <int P, int Q, class Op>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm           = Dims<P, Q>;
   constexpr int EPB  = elems_per_block<Q>();
   constexpr int Q3   = Dm::Q3;
@@ -124,33 +160,64 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
   constexpr int SU2  = Op::kGrad ? 0 : 10;               // scratch of the u2 interpolation
   constexpr int SU   = (Op::kIn2 && !Op::kGrad) ? 10 : 5;  // scratch of the u interpolation
   static_assert(!Op::kIn2 || Op::kGrad || NST >= 20, "stage too small for the u2 scratch");
-  __shared__ double sB[Q * P], sDc[Q * Q];
-  __shared__ double sX[EPB][NST][Q3];
+  __shared__ double                 sB[Q * P], sDc[Q * Q];
+  __shared__ double                 sX[EPB][NST][Q3];
+  __shared__ __align__(16) double   sQD[EPB][10][Q3];
   static_assert(Dm::P3 <= Q3, "P <= Q is required");
   load_tables<P, Q>(a.bt, sB, sDc);
-  const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  basis3::Stage<P, Q> st(sB, sDc, sX[le], t);
+  const int           le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  const bool          nd = t < Dm::P3;
   double(*X)[Q3]         = sX[le];
   const BatchRange br    = batch_range(((long long)a.nelem + EPB - 1) / EPB);
+  const int        cs    = a.comp_stride;
+
+  // prologue: quadrature data and node values of the first batch
+  int    off = 0;
+  double un[5] = {0, 0, 0, 0, 0}, u2n[5] = {0, 0, 0, 0, 0};
+  if (br.first < br.end) {
+    stage_qdata_async<EPB, Q3>(a, br.first, sQD);
+    const long long e_ = br.first * EPB + le;
+    if (nd && e_ < a.nelem) {
+      off = a.offsets[e_ * Dm::P3 + t];
+#pragma unroll
+      for (int c = 0; c < 5; c++) un[c] = a.u[off + c * cs];
+      if constexpr (Op::kIn2)
+#pragma unroll
+        for (int c = 0; c < 5; c++) u2n[c] = a.u2[off + c * cs];
+    }
+  }
 
   for (long long bt = br.first; bt < br.end; bt += br.stride) {
-    const long long e_  = bt * EPB + le;
-    const bool      ok  = e_ < a.nelem;
-    const int       e   = ok ? (int)e_ : 0;
-    const bool      nd  = t < Dm::P3;
-    const int       off = nd ? a.offsets[(long long)e * Dm::P3 + t] : 0;
-    double          un[5], u2n[5];
-#pragma unroll
-    for (int c = 0; c < 5; c++) un[c] = nd ? a.u[off + c * a.comp_stride] : 0.0;
-    if constexpr (Op::kIn2) {
-#pragma unroll
-      for (int c = 0; c < 5; c++) u2n[c] = nd ? a.u2[off + c * a.comp_stride] : 0.0;
-      st.template interp<5>(u2n, SU2, SU2 + 5, U2);
-    }
+    const long long e_   = bt * EPB + le;
+    const bool      ok   = e_ < a.nelem;
+    const int       e    = ok ? (int)e_ : 0;
+    const long long btn  = bt + br.stride;
+    const bool      more = btn < br.end;
+    const long long en_  = btn * EPB + le;
+    const bool      okn  = more && en_ < a.nelem;
+    const long long enode = (long long)e * Dm::P3 + t;
+
+    // The basis stages index LDS with per-thread addresses derived from t.  An opaque copy of t per phase keeps the
+    // compiler from hoisting those addresses out of the loop, where they would stay live (or spill) across the
+    // quadrature-point body.
+    int tb = t;
+    asm volatile("" : "+v"(tb));
+    basis3::Stage<P, Q> st(sB, sDc, X, tb);
+    if constexpr (Op::kIn2) st.template interp<5>(u2n, SU2, SU2 + 5, U2);
     st.template interp<5>(un, SU, SU + 5, 0);
     if constexpr (Op::kGrad) st.template grad<5>(0, 5);
+    int off_next = 0;
     {
-      double u[5], gu[3][5], u2[5], v[5], gw[3][5];
+      __builtin_amdgcn_s_waitcnt(0);  // this batch's quadrature data has landed in sQD
+      __syncthreads();
+      double qd[10], u[5], gu[3][5], u2[5], v[5], gw[3][5];
+#pragma unroll
+      for (int k = 0; k < 10; k++) qd[k] = sQD[le][k][t];
+      __syncthreads();  // all of sQD has been read: the next batch's copy may overwrite it
+      if (more) {
+        stage_qdata_async<EPB, Q3>(a, btn, sQD);
+        if (nd && okn) off_next = a.offsets[en_ * Dm::P3 + t];
+      }
 #pragma unroll
       for (int c = 0; c < 5; c++) {
         u[c] = X[c][t];
@@ -159,7 +226,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
           for (int d = 0; d < 3; d++) gu[d][c] = X[5 + 5 * d + c][t];
         if constexpr (Op::kIn2) u2[c] = X[U2 + c][t];
       }
-      Op::template qf<Q>(a, ctx, e, t, ok, u, gu, u2, v, gw);
+      Op::template qf<Q>(a, ctx, e, t, ok, qd, u, gu, u2, v, gw);
       if constexpr (Op::kScatter) {
         __syncthreads();  // other threads' gradient passes read slots 0..4
 #pragma unroll
@@ -171,18 +238,35 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_volume(Vol
         }
       }
     }
+    // Node values of the next batch, consumed by its interpolation, and this batch's E-vector slots: loaded while
+    // the basis transpose runs.  The scheduling barrier keeps these loads (and their registers) out of the
+    // quadrature-point body above.
+    __builtin_amdgcn_sched_barrier(0);
+    const int off_cur = off;
+    const int slot    = (a.perm && ok && nd) ? a.perm[enode] : 0;
+    if (more) {
+      const bool ld = nd && okn;
+#pragma unroll
+      for (int c = 0; c < 5; c++) un[c] = ld ? a.u[off_next + c * cs] : 0.0;
+      if constexpr (Op::kIn2)
+#pragma unroll
+        for (int c = 0; c < 5; c++) u2n[c] = ld ? a.u2[off_next + c * cs] : 0.0;
+      off = off_next;
+    }
     if constexpr (Op::kScatter) {
-      double r[5];
-      st.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      basis3::Stage<P, Q> sto(sB, sDc, X, tt);
+      double              r[5];
+      sto.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {
         if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
-          const long long en  = (long long)e * Dm::P3 + t;
-          double         *dst = a.ev + (a.perm ? (long long)a.perm[en] : en) * 5;
+          double *dst = a.ev + (a.perm ? (long long)slot : enode) * 5;
 #pragma unroll
           for (int c = 0; c < 5; c++) dst[c] = r[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 5; c++) atomicAdd(&a.v[off + c * a.comp_stride], r[c]);
+          for (int c = 0; c < 5; c++) atomicAdd(&a.v[off_cur + c * cs], r[c]);
         }
       }
     } else {
@@ -223,7 +307,9 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
       for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
     }
     typename Op::Point p;
-    Op::template load_point<Q>(a, ctx, e, t, ok, u2, p);
+    double             qd[10];
+    nsop::fetch_qdata<Q>(a, e, t, ok, qd);
+    Op::template load_point<Q>(a, ctx, e, t, ok, qd, u2, p);
     for (int c = 0; c < 5; c++) {
       double M[4][4] = {{0}};
       for (int s = 0; s < Op::kInSlots; s++) {

@@ -19,12 +19,19 @@ __device__ __forceinline__ long long qidx(const QDataLayout &l, int e, int c, in
   return (long long)e * l.se + c * (Q * Q * Q) + i;
 }
 
-template <int Q, typename T>
-__device__ __forceinline__ void load_qdata(const VolumeArgs &a, int e, int t, bool ok, T &wdetJ, T dXdx[3][3]) {
-  if (ok) {
-    wdetJ = (T)a.qdata[qidx<Q>(a.qd, e, 0, t)];
+// The 10 quadrature-data values {wdetJ, dXdx} of point t of element e, read from the strided field
+template <int Q>
+__device__ __forceinline__ void fetch_qdata(const VolumeArgs &a, int e, int t, bool ok, double qd[10]) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)a.qdata[qidx<Q>(a.qd, e, 1 + k, t)];
+  for (int k = 0; k < 10; k++) qd[k] = ok ? a.qdata[qidx<Q>(a.qd, e, k, t)] : 0.0;
+}
+
+template <typename T>
+__device__ __forceinline__ void unpack_qdata(const double qd[10], bool ok, T &wdetJ, T dXdx[3][3]) {
+  if (ok) {
+    wdetJ = (T)qd[0];
+#pragma unroll
+    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = (T)qd[1 + k];
   } else {  // padding thread of a partial batch: harmless geometry, zero weight
     wdetJ = 0;
 #pragma unroll
@@ -48,6 +55,23 @@ __device__ __forceinline__ void map_grad_out(T wdetJ, const T dXdx[3][3], const 
     for (int d = 0; d < 3; d++) gw[d][c] = (double)(wdetJ * (gv[c][0] * dXdx[d][0] + gv[c][1] * dXdx[d][1] + gv[c][2] * dXdx[d][2]));
 }
 
+// The same mapping formed one direction j at a time: acc[d][c] += gv_j[c] dXdx[d][j] for j = 0, 1, 2 in order, then
+// gw = wdetJ acc (identical operation order to map_grad_out)
+template <typename T>
+__device__ __forceinline__ void accumulate_grad_out(int j, const T dXdx[3][3], const T gvj[5], T acc[3][5]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) acc[d][c] = (j == 0) ? gvj[c] * dXdx[d][0] : acc[d][c] + gvj[c] * dXdx[d][j];
+}
+template <typename T>
+__device__ __forceinline__ void finish_grad_out(T wdetJ, const T acc[3][5], double gw[3][5]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) gw[d][c] = (double)(wdetJ * acc[d][c]);
+}
+
 // x coordinate (first component) of the quadrature point from the trilinear coordinate field
 template <int Q>
 __device__ __forceinline__ double coord_x(const VolumeArgs &a, int e, int t) {
@@ -69,11 +93,11 @@ template <int VAR>
 struct IFunction {
   static constexpr bool kGrad = true, kIn2 = true, kOutV = true, kOutG = true, kScatter = true;
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double qdot[5], double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
+    unpack_qdata(qd, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<VAR>(q);
 #pragma unroll
@@ -82,8 +106,16 @@ struct IFunction {
         gref[0][c] = gref[1][c] = gref[2][c] = 0;
       }
     }
-    const ns::St<double> s  = ns::state_from_q<VAR>(g, q);
-    const ns::St<double> sd = ns::state_from_q_fwd<VAR>(g, s, qdot);
+    // jac_data (newtonian.h:251-254), each value written as soon as it is known so that it is not held in registers
+    auto put = [&](int j, double val) {
+      if (ok) {
+        if (a.jac32) a.jac32[qidx<Q>(a.jd, e, j, t)] = (float)val;
+        else if (a.jac) a.jac[qidx<Q>(a.jd, e, j, t)] = val;
+      }
+    };
+    const ns::St<double> s = ns::state_from_q<VAR>(g, q);
+#pragma unroll
+    for (int j = 0; j < 5; j++) put(j, q[j]);
     // Gradient states one direction at a time: the strong convective residual (FluxInviscidStrong,
     // newtonian_state.h:466-475) is accumulated in the same direction order, and only the primitive gradients
     // (velocity, temperature) are kept for the viscous fluxes.
@@ -98,39 +130,50 @@ struct IFunction {
 #pragma unroll
       for (int k = 0; k < 4; k++) gY[j][k] = gsj.Y[1 + k];
     }
-    const double bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2], s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
-    double       vi[5];
+    // KMStrainRate_State / NewtonianStress / ViscousEnergyFlux (newtonian_state.h:511-573)
+    const double w     = 1 / sqrt(2.);
+    const double sr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
+#pragma unroll
+    for (int j = 0; j < 6; j++) put(5 + j, sr[j]);
+    double st[3][3], Fe[3];
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gY[i][3];
+    // U_dot - body force
+    const ns::St<double> sd    = ns::state_from_q_fwd<VAR>(g, s, qdot);
+    const double         bf[5] = {0, s.U[0] * ctx.g[0], s.U[0] * ctx.g[1], s.U[0] * ctx.g[2], s.U[1] * ctx.g[0] + s.U[2] * ctx.g[1] + s.U[3] * ctx.g[2]};
+    double               vi[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) vi[j] = sd.U[j] - bf[j];
     double Tau[3];
     ns::tau_diag_prim<double>(ctx.tau_coeffs, g, s, dXdx, ctx.dt, Tau);
-    double stab[5][3] = {{0}};
-    if (ctx.stabilization != ns::STAB_NONE) {  // Stabilization (stabilization.h:41-58) with divFdiff = 0
+#pragma unroll
+    for (int j = 0; j < 3; j++) put(11 + j, Tau[j]);
+    // Stabilization (stabilization.h:14-58, divFdiff = 0): the state perturbation StateFromY_fwd(Tau o r)
+    const bool     stabilized = ctx.stabilization != ns::STAB_NONE;
+    ns::St<double> dst;
+    if (stabilized) {
       if (ctx.stabilization == ns::STAB_SUPG)
 #pragma unroll
         for (int j = 0; j < 5; j++) r[j] += sd.U[j] - bf[j];
-      ns::add_stab_matrix(g, s, Tau, r, stab);
+      ns::stab_state(g, s, Tau, r, dst);
     }
-    // KMStrainRate_State / NewtonianStress / ViscousEnergyFlux from the kept primitive gradients
-    const double w   = 1 / sqrt(2.);
-    double       sr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
-    double       st[3][3], Fe[3];
-    ns::stress_from_strain(g, sr, st);
-#pragma unroll
-    for (int i = 0; i < 3; i++) Fe[i] = -s.Y[1] * st[0][i] - s.Y[2] * st[1][i] - s.Y[3] * st[2][i] - g.k * gY[i][3];
-    double gv[5][3];  // grad_v_i[var][dir] = -F_total + stab
+    // grad_v_i[var][dir] = -F_total + stab, one direction at a time, mapped with wdetJ dXdx as it is formed
+    double acc[3][5];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      double F[5];
+      double F[5], sj[5] = {0, 0, 0, 0, 0}, gvj[5];
       ns::flux_inviscid_dir(s, j, F);
-      gv[0][j] = -F[0] + stab[0][j];
+      if (stabilized) ns::flux_inviscid_fwd_dir(s, dst, j, sj);
+      gvj[0] = -F[0] + sj[0];
 #pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(F[1 + k] - st[k][j]) + stab[k + 1][j];
-      gv[4][j] = -(F[4] + Fe[j]) + stab[4][j];
+      for (int k = 0; k < 3; k++) gvj[k + 1] = -(F[1 + k] - st[k][j]) + sj[k + 1];
+      gvj[4] = -(F[4] + Fe[j]) + sj[4];
+      accumulate_grad_out(j, dXdx, gvj, acc);
     }
-    double sigma = 0;
     if (ctx.idl_enable) {
-      sigma = ns::linear_ramp<double>(ctx.idl_amplitude, ctx.idl_length, ctx.idl_start, ok ? coord_x<Q>(a, e, t) : 0.0);
+      const double sigma = ns::linear_ramp<double>(ctx.idl_amplitude, ctx.idl_length, ctx.idl_start, ok ? coord_x<Q>(a, e, t) : 0.0);
+      put(14, sigma);
       ns::St<double> ds;
       ds.Y[0] = (s.Y[0] - ctx.idl_pressure) * sigma;
 #pragma unroll
@@ -139,28 +182,9 @@ struct IFunction {
 #pragma unroll
       for (int j = 0; j < 5; j++) vi[j] += ds.U[j];
     }
-    if (ok) {  // jac_data (newtonian.h:251-254)
-      if (a.jac32) {
-#pragma unroll
-        for (int j = 0; j < 5; j++) a.jac32[qidx<Q>(a.jd, e, j, t)] = (float)q[j];
-#pragma unroll
-        for (int j = 0; j < 6; j++) a.jac32[qidx<Q>(a.jd, e, 5 + j, t)] = (float)sr[j];
-#pragma unroll
-        for (int j = 0; j < 3; j++) a.jac32[qidx<Q>(a.jd, e, 11 + j, t)] = (float)Tau[j];
-        if (ctx.idl_enable) a.jac32[qidx<Q>(a.jd, e, 14, t)] = (float)sigma;
-      } else if (a.jac) {
-#pragma unroll
-        for (int j = 0; j < 5; j++) a.jac[qidx<Q>(a.jd, e, j, t)] = q[j];
-#pragma unroll
-        for (int j = 0; j < 6; j++) a.jac[qidx<Q>(a.jd, e, 5 + j, t)] = sr[j];
-#pragma unroll
-        for (int j = 0; j < 3; j++) a.jac[qidx<Q>(a.jd, e, 11 + j, t)] = Tau[j];
-        if (ctx.idl_enable) a.jac[qidx<Q>(a.jd, e, 14, t)] = sigma;
-      }
-    }
 #pragma unroll
     for (int j = 0; j < 5; j++) v[j] = wdetJ * vi[j];
-    map_grad_out(wdetJ, dXdx, gv, gw);
+    finish_grad_out(wdetJ, acc, gw);
   }
 };
 
@@ -179,9 +203,9 @@ struct IJacobian {
 
   static constexpr int kInSlots = 4;  // linear inputs: value and three reference derivatives
   template <int Q>
-  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *, Point &p) {
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], const double *, Point &p) {
     const ns::Gas<T> g = ns::Gas<T>::from(ctx.gas);
-    load_qdata<Q>(a, e, t, ok, p.wdetJ, p.dXdx);
+    unpack_qdata(qd, ok, p.wdetJ, p.dXdx);
     T qi[5], sr[6];
     p.sigma = 0;
     if (ok) {
@@ -231,38 +255,38 @@ struct IJacobian {
 #pragma unroll
       for (int k = 0; k < 4; k++) gY[j][k] = gdsj.Y[1 + k];
     }
-    const T gx = (T)ctx.g[0], gy = (T)ctx.g[1], gz = (T)ctx.g[2];
-    const T dbf[5] = {0, ds.U[0] * gx, ds.U[0] * gy, ds.U[0] * gz, ds.U[1] * gx + ds.U[2] * gy + ds.U[3] * gz};
-    T       vi[5], dUdot[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-      dUdot[j] = shift * ds.U[j];
-      vi[j]    = dUdot[j] - dbf[j];
-    }
-    T stab[5][3] = {{0}};
-    if (ctx.stabilization != ns::STAB_NONE) {
-      if (ctx.stabilization == ns::STAB_SUPG)
-#pragma unroll
-        for (int j = 0; j < 5; j++) r[j] += dUdot[j] - dbf[j];
-      ns::add_stab_matrix(g, s, p.Tau, r, stab);
-    }
     const T w      = T(1) / (T)sqrt(2.);
-    T       dsr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
-    T       dst[3][3], dFe[3];
-    ns::stress_from_strain(g, dsr, dst);
+    const T dsr[6] = {gY[0][0], gY[1][1], gY[2][2], w * (gY[2][1] + gY[1][2]), w * (gY[2][0] + gY[0][2]), w * (gY[1][0] + gY[0][1])};
+    T       dstr[3][3], dFe[3];
+    ns::stress_from_strain(g, dsr, dstr);
 #pragma unroll
     for (int i = 0; i < 3; i++)  // ViscousEnergyFlux_fwd (newtonian_state.h:567-573)
-      dFe[i] = -s.Y[1] * dst[0][i] - ds.Y[1] * p.st[0][i] - s.Y[2] * dst[1][i] - ds.Y[2] * p.st[1][i] - s.Y[3] * dst[2][i] -
+      dFe[i] = -s.Y[1] * dstr[0][i] - ds.Y[1] * p.st[0][i] - s.Y[2] * dstr[1][i] - ds.Y[2] * p.st[1][i] - s.Y[3] * dstr[2][i] -
                ds.Y[3] * p.st[2][i] - g.k * gY[i][3];
-    T gv[5][3];
+    const T gx = (T)ctx.g[0], gy = (T)ctx.g[1], gz = (T)ctx.g[2];
+    const T dbf[5] = {0, ds.U[0] * gx, ds.U[0] * gy, ds.U[0] * gz, ds.U[1] * gx + ds.U[2] * gy + ds.U[3] * gz};
+    T       vi[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) vi[j] = shift * ds.U[j] - dbf[j];
+    const bool stabilized = ctx.stabilization != ns::STAB_NONE;
+    ns::St<T>  dstab;
+    if (stabilized) {
+      if (ctx.stabilization == ns::STAB_SUPG)
+#pragma unroll
+        for (int j = 0; j < 5; j++) r[j] += shift * ds.U[j] - dbf[j];
+      ns::stab_state(g, s, p.Tau, r, dstab);
+    }
+    T acc[3][5];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      T dF[5];
+      T dF[5], sj[5] = {0, 0, 0, 0, 0}, gvj[5];
       ns::flux_inviscid_fwd_dir(s, ds, j, dF);
-      gv[0][j] = -dF[0] + stab[0][j];
+      if (stabilized) ns::flux_inviscid_fwd_dir(s, dstab, j, sj);
+      gvj[0] = -dF[0] + sj[0];
 #pragma unroll
-      for (int k = 0; k < 3; k++) gv[k + 1][j] = -(dF[1 + k] - dst[k][j]) + stab[k + 1][j];
-      gv[4][j] = -(dF[4] + dFe[j]) + stab[4][j];
+      for (int k = 0; k < 3; k++) gvj[k + 1] = -(dF[1 + k] - dstr[k][j]) + sj[k + 1];
+      gvj[4] = -(dF[4] + dFe[j]) + sj[4];
+      accumulate_grad_out(j, p.dXdx, gvj, acc);
     }
     if (ctx.idl_enable) {  // InternalDampingLayer_IJacobian_Integrand (newtonian_integrands.h:61-65)
       ns::St<T> dd;
@@ -275,14 +299,14 @@ struct IJacobian {
     }
 #pragma unroll
     for (int j = 0; j < 5; j++) v[j] = (double)(p.wdetJ * vi[j]);
-    map_grad_out(p.wdetJ, p.dXdx, gv, gw);
+    finish_grad_out(p.wdetJ, acc, gw);
   }
 
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double u[5], double gu[3][5],
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double u[5], double gu[3][5],
                             double *, double v[5], double gw[3][5]) {
     Point p;
-    load_point<Q>(a, ctx, e, t, ok, nullptr, p);
+    load_point<Q>(a, ctx, e, t, ok, qd, nullptr, p);
     T dq[5], gref[3][5];
 #pragma unroll
     for (int c = 0; c < 5; c++) {
@@ -300,11 +324,11 @@ struct IJacobian {
 struct RHSFunction {
   static constexpr bool kGrad = true, kIn2 = false, kOutV = true, kOutG = true, kScatter = true;
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double *, double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
+    unpack_qdata(qd, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<ns::VAR_CONS>(q);
 #pragma unroll
@@ -373,9 +397,9 @@ struct MassNewtonian {
     double         wdetJ, dXdx[3][3], Tau[3];
   };
   template <int Q>
-  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double *q2, Point &p) {
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], const double *q2, Point &p) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
-    load_qdata<Q>(a, e, t, ok, p.wdetJ, p.dXdx);
+    unpack_qdata(qd, ok, p.wdetJ, p.dXdx);
     double q[5];
 #pragma unroll
     for (int c = 0; c < 5; c++) q[c] = q2[c];
@@ -401,11 +425,11 @@ struct MassNewtonian {
       for (int d = 0; d < 3; d++) gw[d][c] = p.wdetJ * (stab[c][0] * p.dXdx[d][0] + stab[c][1] * p.dXdx[d][1] + stab[c][2] * p.dXdx[d][2]);
   }
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double qdot[5], double (*)[5],
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double qdot[5], double (*)[5],
                             double q[5], double v[5], double gw[3][5]) {
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
+    unpack_qdata(qd, ok, wdetJ, dXdx);
     if (!ok) {
       padding_state<ns::VAR_CONS>(q);
 #pragma unroll
@@ -437,8 +461,8 @@ struct Mass5 {
     double wdetJ;
   };
   template <int Q>
-  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, const double *, Point &p) {
-    p.wdetJ = ok ? a.qdata[qidx<Q>(a.qd, e, 0, t)] : 0.0;
+  __device__ static void load_point(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, const double qd[10], const double *, Point &p) {
+    p.wdetJ = ok ? qd[0] : 0.0;
   }
   __device__ static void action(const CeedMiNewtonianContext &, const Point &p, const double u[5], const double (*)[5], double v[5],
                                 double gw[3][5]) {
@@ -449,9 +473,9 @@ struct Mass5 {
     }
   }
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, double u[5], double (*)[5], double *,
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &, int e, int t, bool ok, const double qd[10], double u[5], double (*)[5], double *,
                             double v[5], double gw[3][5]) {
-    const double wdetJ = ok ? a.qdata[qidx<Q>(a.qd, e, 0, t)] : 0.0;
+    const double wdetJ = ok ? qd[0] : 0.0;
 #pragma unroll
     for (int c = 0; c < 5; c++) v[c] = wdetJ * u[c];
   }
@@ -465,12 +489,12 @@ template <int VAR>
 struct MonitorKE {
   static constexpr bool kGrad = true, kIn2 = false, kOutV = false, kOutG = false, kScatter = false;
   template <int Q>
-  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, double q[5], double gref[3][5],
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double *, double *, double (*)[5]) {
     if (!ok) return;
     const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
     double                wdetJ, dXdx[3][3];
-    load_qdata<Q>(a, e, t, ok, wdetJ, dXdx);
+    unpack_qdata(qd, ok, wdetJ, dXdx);
     const ns::St<double> s = ns::state_from_q<VAR>(g, q);
     ns::St<double>       gs[3];
     ns::phys_gradient<VAR>(g, s, gref, dXdx, gs);

@@ -285,6 +285,15 @@ __device__ __forceinline__ void add_stab_matrix(const Gas<T> &g, const St<T> &s,
   }
 }
 
+// The state perturbation of StabilizationMatrix: dY = Tau o r on the primitive slots (dYFromTau) mapped by
+// StateFromY_fwd (stabilization.h:14-39); the stabilization flux of direction i is FluxInviscid_fwd(s, ds, i).
+template <typename T>
+__device__ __forceinline__ void stab_state(const Gas<T> &g, const St<T> &s, const T Tau[3], const T r[5], St<T> &ds) {
+  const T dY[5] = {Tau[0] * r[0], Tau[1] * r[1], Tau[1] * r[2], Tau[1] * r[3], Tau[2] * r[4]};
+  for (int i = 0; i < 5; i++) ds.Y[i] = dY[i];
+  cons_from_prim_fwd(g, s, ds.Y, ds.U);
+}
+
 // Strong convective residual sum_i dF_i/dq . dq/dx_i (FluxInviscidStrong, newtonian_state.h:466-475)
 template <typename T>
 __device__ __forceinline__ void flux_inviscid_strong(const St<T> &s, const St<T> gs[3], T r[5]) {
