@@ -162,7 +162,9 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   static_assert(!Op::kIn2 || Op::kGrad || NST >= 20, "stage too small for the u2 scratch");
   __shared__ double                 sB[Q * P], sDc[Q * Q];
   __shared__ double                 sX[EPB][NST][Q3];
-  __shared__ __align__(16) double   sQD[EPB][10][Q3];
+  // quadrature data staged through LDS by DMA (Op::kStageQData) or read from global memory at the point body
+  constexpr bool                    SQ = Op::kStageQData;
+  __shared__ __align__(16) double   sQD[SQ ? EPB : 1][10][SQ ? Q3 : 1];
   static_assert(Dm::P3 <= Q3, "P <= Q is required");
   load_tables<P, Q>(a.bt, sB, sDc);
   const int           le = threadIdx.x / Q3, t = threadIdx.x % Q3;
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   int    off = 0;
   double un[5] = {0, 0, 0, 0, 0}, u2n[5] = {0, 0, 0, 0, 0};
   if (br.first < br.end) {
-    stage_qdata_async<EPB, Q3>(a, br.first, sQD);
+    if constexpr (SQ) stage_qdata_async<EPB, Q3>(a, br.first, sQD);
     const long long e_ = br.first * EPB + le;
     if (nd && e_ < a.nelem) {
       off = a.offsets[e_ * Dm::P3 + t];
@@ -208,16 +210,18 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
     if constexpr (Op::kGrad) st.template grad<5>(0, 5);
     int off_next = 0;
     {
-      __builtin_amdgcn_s_waitcnt(0);  // this batch's quadrature data has landed in sQD
-      __syncthreads();
       double qd[10], u[5], gu[3][5], u2[5], v[5], gw[3][5];
+      if constexpr (SQ) {
+        __builtin_amdgcn_s_waitcnt(0);  // this batch's quadrature data has landed in sQD
+        __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 10; k++) qd[k] = sQD[le][k][t];
-      __syncthreads();  // all of sQD has been read: the next batch's copy may overwrite it
-      if (more) {
-        stage_qdata_async<EPB, Q3>(a, btn, sQD);
-        if (nd && okn) off_next = a.offsets[en_ * Dm::P3 + t];
+        for (int k = 0; k < 10; k++) qd[k] = sQD[le][k][t];
+        __syncthreads();  // all of sQD has been read: the next batch's copy may overwrite it
+        if (more) stage_qdata_async<EPB, Q3>(a, btn, sQD);
+      } else {
+        nsop::fetch_qdata<Q>(a, e, t, ok, qd);
       }
+      if (more && nd && okn) off_next = a.offsets[en_ * Dm::P3 + t];
 #pragma unroll
       for (int c = 0; c < 5; c++) {
         u[c] = X[c][t];

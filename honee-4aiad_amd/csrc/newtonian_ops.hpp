@@ -92,6 +92,7 @@ __device__ __forceinline__ double coord_x(const VolumeArgs &a, int e, int t) {
 template <int VAR>
 struct IFunction {
   static constexpr bool kGrad = true, kIn2 = true, kOutV = true, kOutG = true, kScatter = true;
+  static constexpr bool kStageQData = true;  // prefetch quadrature data into LDS (measured faster for this body)
   template <int Q>
   __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double qdot[5], double v[5], double gw[3][5]) {
@@ -194,6 +195,7 @@ struct IFunction {
 template <int VAR, typename T>
 struct IJacobian {
   static constexpr bool kGrad = true, kIn2 = false, kOutV = true, kOutG = true, kScatter = true;
+  static constexpr bool kStageQData = false;
 
   // State-dependent part at one quadrature point (shared by apply and diagonal assembly)
   struct Point {
@@ -323,6 +325,7 @@ struct IJacobian {
 // ---------------------------------------------------------------------------------------------------------------------
 struct RHSFunction {
   static constexpr bool kGrad = true, kIn2 = false, kOutV = true, kOutG = true, kScatter = true;
+  static constexpr bool kStageQData = false;
   template <int Q>
   __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double *, double v[5], double gw[3][5]) {
@@ -390,6 +393,7 @@ struct RHSFunction {
 // ---------------------------------------------------------------------------------------------------------------------
 struct MassNewtonian {
   static constexpr bool kGrad = false, kIn2 = true, kOutV = true, kOutG = true, kScatter = true;
+  static constexpr bool kStageQData = false;
   static constexpr int  kInSlots = 1;  // linear in the value of q_dot only
 
   struct Point {
@@ -456,6 +460,7 @@ struct MassNewtonian {
 // Mass_5 (qfunctions/mass.h): v = wdetJ u
 struct Mass5 {
   static constexpr bool kGrad = false, kIn2 = false, kOutV = true, kOutG = false, kScatter = true;
+  static constexpr bool kStageQData = false;
   static constexpr int  kInSlots = 1;
   struct Point {
     double wdetJ;
@@ -488,6 +493,7 @@ struct Mass5 {
 template <int VAR>
 struct MonitorKE {
   static constexpr bool kGrad = true, kIn2 = false, kOutV = false, kOutG = false, kScatter = false;
+  static constexpr bool kStageQData = false;
   template <int Q>
   __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int e, int t, bool ok, const double qd[10], double q[5], double gref[3][5],
                             double *, double *, double (*)[5]) {
