@@ -440,3 +440,198 @@ static inline void o_idl_residual(OGas g, const OState *s, double sigma, double 
   OState ds = o_state_from_y_fwd(g, s, dampY);
   for (int i = 0; i < 5; i++) res[i] += ds.U[i];
 }
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Riemann fluxes for the freestream boundary (qfunctions/riemann_solver.h).  Left = interior state, right = exterior
+// state, normal = unit outward normal.
+// ---------------------------------------------------------------------------------------------------------------------
+
+// TotalSpecificEnthalpy (newtonian_state.h:52-56)
+static inline double o_total_enthalpy(OGas g, const OState *s) {
+  const double e_kinetic  = 0.5 * o_dot3(&s->Y[1], &s->Y[1]);
+  const double e_internal = g.cv * s->Y[4];
+  return e_internal + e_kinetic + s->Y[0] / s->U[0];
+}
+
+// TotalSpecificEnthalpy_fwd (newtonian_state.h:58-62)
+static inline double o_total_enthalpy_fwd(OGas g, const OState *s, const OState *ds) {
+  const double de_kinetic  = o_dot3(&ds->Y[1], &s->Y[1]);
+  const double de_internal = g.cv * ds->Y[4];
+  return de_internal + de_kinetic + ds->Y[0] / s->U[0] - s->Y[0] / o_sq(s->U[0]) * ds->U[0];
+}
+
+// RoeSetup / RoeSetup_fwd (riemann_solver.h:27-40): weights {left, right}
+static inline void o_roe_setup(double rho_l, double rho_r, double w[2]) {
+  const double sl = sqrt(rho_l), sr = sqrt(rho_r);
+  w[0] = sl / (sl + sr);
+  w[1] = sr / (sl + sr);
+}
+static inline void o_roe_setup_fwd(double rho_l, double rho_r, double drho_l, double drho_r, double dw[2]) {
+  const double sl = sqrt(rho_l), sr = sqrt(rho_r);
+  const double ss = o_sq(sl + sr);
+  const double r_right = (sl / (2 * sr * ss)) * drho_r - (sr / (2 * sl * ss)) * drho_l;
+  const double r_left  = (sr / (2 * sl * ss)) * drho_l - (sl / (2 * sr * ss)) * drho_r;
+  dw[0] = r_left;
+  dw[1] = r_right;
+}
+
+// RoeAverage / RoeAverage_fwd (riemann_solver.h:42-47)
+static inline double o_roe_avg(const double w[2], double ql, double qr) { return w[0] * ql + w[1] * qr; }
+static inline double o_roe_avg_fwd(const double w[2], const double dw[2], double ql, double qr, double dql, double dqr) {
+  return qr * dw[1] + ql * dw[0] + w[1] * dqr + w[0] * dql;
+}
+
+// ComputeHLLSpeeds_Roe (riemann_solver.h:87-106)
+static inline void o_hll_speeds_roe(OGas g, const OState *l, double ul, const OState *r, double ur, double *sl, double *sr) {
+  const double gamma = o_gamma(g);
+  double       w[2];
+  o_roe_setup(l->U[0], r->U[0], w);
+  const double u_roe = o_roe_avg(w, ul, ur);
+  const double H_l = o_total_enthalpy(g, l), H_r = o_total_enthalpy(g, r);
+  const double H_roe = o_roe_avg(w, H_l, H_r);
+  const double a_roe = sqrt((gamma - 1) * (H_roe - 0.5 * o_sq(u_roe)));
+  *sl                = u_roe - a_roe;
+  *sr                = u_roe + a_roe;
+}
+
+// ComputeHLLSpeeds_Roe_fwd (riemann_solver.h:108-137)
+static inline void o_hll_speeds_roe_fwd(OGas g, const OState *l, const OState *dl, double ul, double dul, const OState *r, const OState *dr,
+                                        double ur, double dur, double *sl, double *dsl, double *sr, double *dsr) {
+  const double gamma = o_gamma(g);
+  double       w[2], dw[2];
+  o_roe_setup(l->U[0], r->U[0], w);
+  o_roe_setup_fwd(l->U[0], r->U[0], dl->U[0], dr->U[0], dw);
+  const double u_roe  = o_roe_avg(w, ul, ur);
+  const double du_roe = o_roe_avg_fwd(w, dw, ul, ur, dul, dur);
+  const double H_l = o_total_enthalpy(g, l), H_r = o_total_enthalpy(g, r);
+  const double dH_l = o_total_enthalpy_fwd(g, l, dl), dH_r = o_total_enthalpy_fwd(g, r, dr);
+  const double H_roe  = o_roe_avg(w, H_l, H_r);
+  const double dH_roe = o_roe_avg_fwd(w, dw, H_l, H_r, dH_l, dH_r);
+  const double a_roe  = sqrt((gamma - 1) * (H_roe - 0.5 * o_sq(u_roe)));
+  double       da_roe = 0.5 * sqrt((gamma - 1) / (H_roe - 0.5 * o_sq(u_roe))) * dH_roe;
+  da_roe -= 0.5 * sqrt(gamma - 1) * u_roe / sqrt(H_roe - 0.5 * o_sq(u_roe)) * du_roe;
+  *sl  = u_roe - a_roe;
+  *dsl = du_roe - da_roe;
+  *sr  = u_roe + a_roe;
+  *dsr = du_roe + da_roe;
+}
+
+// Flux_HLL (riemann_solver.h:49-66)
+static inline void o_flux_hll(const OState *l, const OState *r, const double Fl[5], const double Fr[5], double s_l, double s_r, double F[5]) {
+  for (int i = 0; i < 5; i++) F[i] = (s_r * Fl[i] - s_l * Fr[i] + s_l * s_r * (r->U[i] - l->U[i])) / (s_r - s_l);
+}
+
+// Flux_HLL_fwd (riemann_solver.h:68-85)
+static inline void o_flux_hll_fwd(const OState *l, const OState *r, const OState *dl, const OState *dr, const double Fl[5], const double Fr[5],
+                                  const double dFl[5], const double dFr[5], double S_l, double S_r, double dS_l, double dS_r, double dF[5]) {
+  for (int i = 0; i < 5; i++) {
+    const double S_diff = S_r - S_l;
+    dF[i]               = 0.;
+    dF[i] += (S_l * (-Fl[i] + Fr[i] + S_l * l->U[i] - S_l * r->U[i]) / o_sq(S_diff)) * dS_r;
+    dF[i] += (S_r * (Fl[i] - Fr[i] - S_r * l->U[i] + S_r * r->U[i]) / o_sq(S_diff)) * dS_l;
+    dF[i] += (S_r * dFl[i] - S_l * dFr[i] + S_r * S_l * (dr->U[i] - dl->U[i])) / S_diff;
+  }
+}
+
+// RiemannFlux_HLL (riemann_solver.h:151-169)
+static inline void o_riemann_hll(OGas g, const OState *l, const OState *r, const double n[3], double F[5]) {
+  double Fl[5], Fr[5];
+  o_flux_inviscid_dot_normal(l, n, Fl);
+  o_flux_inviscid_dot_normal(r, n, Fr);
+  const double ul = o_dot3(&l->Y[1], n), ur = o_dot3(&r->Y[1], n);
+  double       s_l, s_r;
+  o_hll_speeds_roe(g, l, ul, r, ur, &s_l, &s_r);
+  if (0 <= s_l) memcpy(F, Fl, sizeof(Fl));
+  else if (s_r <= 0) memcpy(F, Fr, sizeof(Fr));
+  else o_flux_hll(l, r, Fl, Fr, s_l, s_r, F);
+}
+
+// RiemannFlux_HLL_fwd (riemann_solver.h:184-205)
+static inline void o_riemann_hll_fwd(OGas g, const OState *l, const OState *dl, const OState *r, const OState *dr, const double n[3], double dF[5]) {
+  double Fl[5], Fr[5], dFl[5], dFr[5];
+  o_flux_inviscid_dot_normal(l, n, Fl);
+  o_flux_inviscid_dot_normal(r, n, Fr);
+  o_flux_inviscid_dot_normal_fwd(l, dl, n, dFl);
+  o_flux_inviscid_dot_normal_fwd(r, dr, n, dFr);
+  const double ul = o_dot3(&l->Y[1], n), ur = o_dot3(&r->Y[1], n);
+  const double dul = o_dot3(&dl->Y[1], n), dur = o_dot3(&dr->Y[1], n);
+  double       s_l, ds_l, s_r, ds_r;
+  o_hll_speeds_roe_fwd(g, l, dl, ul, dul, r, dr, ur, dur, &s_l, &ds_l, &s_r, &ds_r);
+  if (0 <= s_l) memcpy(dF, dFl, sizeof(dFl));
+  else if (s_r <= 0) memcpy(dF, dFr, sizeof(dFr));
+  else o_flux_hll_fwd(l, r, dl, dr, Fl, Fr, dFl, dFr, s_l, s_r, ds_l, ds_r, dF);
+}
+
+// RiemannFlux_HLLC_Star (riemann_solver.h:207-224): StateConservativeAXPBYPCZ(1, F_side, s_side * fact, star, -s_side, U_side)
+static inline void o_hllc_star(const OState *side, const double F_side[5], const double n[3], double u_side, double s_side, double s_star, double F[5]) {
+  const double fact  = side->U[0] * (s_side - u_side) / (s_side - s_star);
+  const double denom = side->U[0] * (s_side - u_side);
+  const double star[5] = {1.0, side->Y[1] + (s_star - u_side) * n[0], side->Y[2] + (s_star - u_side) * n[1], side->Y[3] + (s_star - u_side) * n[2],
+                          side->U[4] / side->U[0] + (s_star - u_side) * (s_star + side->Y[0] / denom)};
+  const double b = s_side * fact, c = -s_side;
+  for (int i = 0; i < 5; i++) F[i] = 1 * F_side[i] + b * star[i] + c * side->U[i];
+}
+
+// RiemannFlux_HLLC_Star_fwd (riemann_solver.h:226-256): StateConservativeMult over {dF_side, star, dstar, U, dU}
+static inline void o_hllc_star_fwd(const OState *side, const OState *dside, const double F_side[5], const double dF_side[5], const double n[3],
+                                   double u_side, double du_side, double s_side, double ds_side, double s_star, double ds_star, double dF[5]) {
+  const double fact  = side->U[0] * (s_side - u_side) / (s_side - s_star);
+  const double dfact = (side->U[0] * (ds_side - du_side) + dside->U[0] * (s_side - u_side)) / (s_side - s_star) -
+                       fact / (s_side - s_star) * (ds_side - ds_star);
+  const double denom  = side->U[0] * (s_side - u_side);
+  const double ddenom = side->U[0] * (ds_side - du_side) + dside->U[0] * (s_side - u_side);
+  const double star[5] = {1.0, side->Y[1] + (s_star - u_side) * n[0], side->Y[2] + (s_star - u_side) * n[1], side->Y[3] + (s_star - u_side) * n[2],
+                          side->U[4] / side->U[0] + (s_star - u_side) * (s_star + side->Y[0] / denom)};
+  const double dstar[5] = {0., dside->Y[1] + (ds_star - du_side) * n[0], dside->Y[2] + (ds_star - du_side) * n[1],
+                           dside->Y[3] + (ds_star - du_side) * n[2],
+                           dside->U[4] / side->U[0] - side->U[4] / o_sq(side->U[0]) * dside->U[0] + (ds_star - du_side) * (s_star + side->Y[0] / denom) +
+                               (s_star - u_side) * (ds_star + dside->Y[0] / denom - side->Y[0] / o_sq(denom) * ddenom)};
+  const double  a[5] = {1, ds_side * fact + s_side * dfact, s_side * fact, -ds_side, -s_side};
+  const double *X[5] = {dF_side, star, dstar, side->U, dside->U};
+  for (int i = 0; i < 5; i++) dF[i] = 0;
+  for (int k = 0; k < 5; k++)
+    for (int i = 0; i < 5; i++) dF[i] += a[k] * X[k][i];
+}
+
+// RiemannFlux_HLLC (riemann_solver.h:258-284)
+static inline void o_riemann_hllc(OGas g, const OState *l, const OState *r, const double n[3], double F[5]) {
+  double Fl[5], Fr[5];
+  o_flux_inviscid_dot_normal(l, n, Fl);
+  o_flux_inviscid_dot_normal(r, n, Fr);
+  const double ul = o_dot3(&l->Y[1], n), ur = o_dot3(&r->Y[1], n);
+  double       s_l, s_r;
+  o_hll_speeds_roe(g, l, ul, r, ur, &s_l, &s_r);
+  const double rhou_l = l->U[0] * ul, rhou_r = r->U[0] * ur;
+  const double numer  = r->Y[0] - l->Y[0] + rhou_l * (s_l - ul) - rhou_r * (s_r - ur);
+  const double denom  = l->U[0] * (s_l - ul) - r->U[0] * (s_r - ur);
+  const double s_star = numer / denom;
+  if (0 <= s_l) memcpy(F, Fl, sizeof(Fl));
+  else if (0 <= s_star) o_hllc_star(l, Fl, n, ul, s_l, s_star, F);
+  else if (0 <= s_r) o_hllc_star(r, Fr, n, ur, s_r, s_star, F);
+  else memcpy(F, Fr, sizeof(Fr));
+}
+
+// RiemannFlux_HLLC_fwd (riemann_solver.h:286-338)
+static inline void o_riemann_hllc_fwd(OGas g, const OState *l, const OState *dl, const OState *r, const OState *dr, const double n[3], double dF[5]) {
+  double Fl[5], Fr[5], dFl[5], dFr[5];
+  o_flux_inviscid_dot_normal(l, n, Fl);
+  o_flux_inviscid_dot_normal(r, n, Fr);
+  o_flux_inviscid_dot_normal_fwd(l, dl, n, dFl);
+  o_flux_inviscid_dot_normal_fwd(r, dr, n, dFr);
+  const double ul = o_dot3(&l->Y[1], n), ur = o_dot3(&r->Y[1], n);
+  const double dul = o_dot3(&dl->Y[1], n), dur = o_dot3(&dr->Y[1], n);
+  double       s_l, ds_l, s_r, ds_r;
+  o_hll_speeds_roe_fwd(g, l, dl, ul, dul, r, dr, ur, dur, &s_l, &ds_l, &s_r, &ds_r);
+  const double rhou_l = l->U[0] * ul, drhou_l = l->U[0] * dul + dl->U[0] * ul;
+  const double rhou_r = r->U[0] * ur, drhou_r = r->U[0] * dur + dr->U[0] * ur;
+  const double numer  = r->Y[0] - l->Y[0] + rhou_l * (s_l - ul) - rhou_r * (s_r - ur);
+  const double dnumer = dr->Y[0] - dl->Y[0] + rhou_l * (ds_l - dul) + drhou_l * (s_l - ul) - rhou_r * (ds_r - dur) - drhou_r * (s_r - ur);
+  const double denom  = l->U[0] * (s_l - ul) - r->U[0] * (s_r - ur);
+  const double ddenom = l->U[0] * (ds_l - dul) + dl->U[0] * (s_l - ul) - r->U[0] * (ds_r - dur) - dr->U[0] * (s_r - ur);
+  const double s_star  = numer / denom;
+  const double ds_star = dnumer / denom - numer / o_sq(denom) * ddenom;
+  if (0 <= s_l) memcpy(dF, dFl, sizeof(dFl));
+  else if (0 <= s_star) o_hllc_star_fwd(l, dl, Fl, dFl, n, ul, dul, s_l, ds_l, s_star, ds_star, dF);
+  else if (0 <= s_r) o_hllc_star_fwd(r, dr, Fr, dFr, n, ur, dur, s_r, ds_r, s_star, ds_star, dF);
+  else memcpy(dF, dFr, sizeof(dFr));
+}

@@ -245,3 +245,94 @@ def qf_ijacobian(c: ONewt, dq, gref, wdetJ, dXdx, jd):
     a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
     lib().oracle_qf_ijacobian(ct.byref(c), _d(a(dq)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(jd)), _d(v), _d(gv))
     return v, gv.reshape(3, 5)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Boundary faces: consistent boundary integral and freestream Riemann boundary
+# ---------------------------------------------------------------------------------------------------------------------
+class OFreestream(ct.Structure):
+    _fields_ = [("newt", ONewt), ("S_inf_U", ct.c_double * 5), ("S_inf_Y", ct.c_double * 5), ("riemann", ct.c_int)]
+
+
+class OFaces(ct.Structure):
+    _fields_ = [("nfaces", ct.c_int), ("face_elem", _ip), ("face_id", _ip)]
+
+
+class Faces:
+    """Boundary faces given as (volume element, local face id 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)."""
+
+    def __init__(self, face_elem, face_id):
+        self.elem = np.ascontiguousarray(face_elem, dtype=np.int32)
+        self.fid = np.ascontiguousarray(face_id, dtype=np.int32)
+        self.s = OFaces(len(self.elem), _i(self.elem), _i(self.fid))
+        self.nfaces = len(self.elem)
+
+
+def freestream_ctx(c: ONewt, S_inf_U, S_inf_Y, hllc: bool) -> OFreestream:
+    f = OFreestream()
+    f.newt = c
+    for i in range(5):
+        f.S_inf_U[i] = S_inf_U[i]
+        f.S_inf_Y[i] = S_inf_Y[i]
+    f.riemann = 1 if hllc else 0
+    return f
+
+
+def setup_qdata_boundary(m: Mesh, f: Faces):
+    qd = np.zeros((f.nfaces, 10, m.tab.Q**2))
+    lib().oracle_setup_qdata_boundary(ct.byref(m.s), ct.byref(f.s), _d(qd))
+    return qd
+
+
+def boundary_integral(m: Mesh, f: Faces, c: ONewt, qdata_sur, q):
+    v = np.zeros(5 * m.nnodes)
+    jac = np.zeros((f.nfaces, 11, m.tab.Q**2))
+    lib().oracle_boundary_integral(ct.byref(m.s), ct.byref(f.s), ct.byref(c), _d(qdata_sur), _d(q), _d(v), _d(jac))
+    return v, jac
+
+
+def boundary_integral_jacobian(m: Mesh, f: Faces, c: ONewt, qdata_sur, jac_sur, dq):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_boundary_integral_jacobian(ct.byref(m.s), ct.byref(f.s), ct.byref(c), _d(qdata_sur), _d(jac_sur), _d(dq), _d(v))
+    return v
+
+
+def freestream(m: Mesh, f: Faces, fc: OFreestream, qdata_sur, q):
+    v = np.zeros(5 * m.nnodes)
+    jac = np.zeros((f.nfaces, 5, m.tab.Q**2))
+    lib().oracle_freestream(ct.byref(m.s), ct.byref(f.s), ct.byref(fc), _d(qdata_sur), _d(q), _d(v), _d(jac))
+    return v, jac
+
+
+def freestream_jacobian(m: Mesh, f: Faces, fc: OFreestream, qdata_sur, jac_sur, dq):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_freestream_jacobian(ct.byref(m.s), ct.byref(f.s), ct.byref(fc), _d(qdata_sur), _d(jac_sur), _d(dq), _d(v))
+    return v
+
+
+def _v(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def riemann_flux(gas: OGas, hllc: bool, qL, qR, var, n):
+    F = np.zeros(5)
+    lib().oracle_riemann_flux(ct.byref(gas), int(hllc), _d(_v(qL)), _d(_v(qR)), int(var), _d(_v(n)), _d(F))
+    return F
+
+
+def riemann_flux_fwd(gas: OGas, hllc: bool, qL, dqL, qR, dqR, var, n):
+    dF = np.zeros(5)
+    lib().oracle_riemann_flux_fwd(ct.byref(gas), int(hllc), _d(_v(qL)), _d(_v(dqL)), _d(_v(qR)), _d(_v(dqR)), int(var), _d(_v(n)), _d(dF))
+    return dF
+
+
+def hll_speeds(gas: OGas, qL, qR, var, n):
+    sL, sR = ct.c_double(), ct.c_double()
+    lib().oracle_hll_speeds(ct.byref(gas), _d(_v(qL)), _d(_v(qR)), int(var), _d(_v(n)), ct.byref(sL), ct.byref(sR))
+    return sL.value, sR.value
+
+
+def hll_speeds_fwd(gas: OGas, qL, dqL, qR, dqR, var, n):
+    out = [ct.c_double() for _ in range(4)]
+    lib().oracle_hll_speeds_fwd(ct.byref(gas), _d(_v(qL)), _d(_v(dqL)), _d(_v(qR)), _d(_v(dqR)), int(var), _d(_v(n)), *[ct.byref(o) for o in out])
+    return tuple(o.value for o in out)
