@@ -21,6 +21,8 @@ struct OpPlan {
   CeedBasis           b_q        = nullptr, b_x = nullptr;
   CeedVector          v_qdot     = nullptr, v_qdata = nullptr, v_x = nullptr, v_jac = nullptr;
   QDataLayout         qd{}, jd{};
+  bool                face = false;  // boundary-face operator: 2D tensor basis, Q^2 points per face
+  bool                hllc = false;
   std::string         kernel_name;
 };
 
@@ -30,12 +32,12 @@ static int plan_bind_fields(CeedOperator op);
 
 // The fused kernels address strided quadrature data in the backend layout [elem][comp][qpt], i.e. strides
 // {1, Q^3, ncomp Q^3} with one point per Q^3 slot; a restriction with other strides is not bound.
-static int check_point_layout(CeedOperator op, CeedElemRestriction r, int Q) {
+static int check_point_layout(CeedOperator op, CeedElemRestriction r, int Q, bool face) {
   if (!r) return 0;
-  const CeedInt Q3 = Q * Q * Q;
+  const CeedInt Q3 = face ? Q * Q : Q * Q * Q;
   if (!(r->elem_size == Q3 && r->strides[0] == 1 && r->strides[1] == Q3 && r->strides[2] == (CeedInt)r->num_comp * Q3))
     return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "strided quadrature data must use CEED_STRIDES_BACKEND (or strides {1, Q^3, ncomp Q^3}) "
-                                                    "with Q^3 = %d points per element",
+                                                    "with %d points per element",
                   Q3);
   return 0;
 }
@@ -43,11 +45,12 @@ static int check_point_layout(CeedOperator op, CeedElemRestriction r, int Q) {
 // Quadrature-data fields read by the fused kernels: 10 components ({wdetJ, dXdx}) per point of every element, held in
 // a vector of at least that length (the kernels copy whole elements' qdata blocks).
 static int check_qdata_field(CeedOperator op, const OpPlan *p) {
-  if (p->kind == QfKind::Setup || !p->r_qd) return 0;
-  const CeedSize need = (CeedSize)p->nelem * 10 * p->Q * p->Q * p->Q;
+  if (p->kind == QfKind::Setup || p->kind == QfKind::SetupBoundary || !p->r_qd) return 0;
+  const CeedSize npts = p->face ? (CeedSize)p->Q * p->Q : (CeedSize)p->Q * p->Q * p->Q;
+  const CeedSize need = (CeedSize)p->nelem * 10 * npts;
   if (p->r_qd->num_comp != 10 || p->r_qd->num_elem != p->nelem || !p->v_qdata || p->v_qdata->length < need)
     return MI_ERR(op->ceed, CEED_ERROR_DIMENSION, "qdata must be a 10-component strided field over the operator's %d elements", p->nelem);
-  if (p->r_jd && (p->r_jd->num_elem != p->nelem || (p->v_jac && p->v_jac->length < (CeedSize)p->nelem * p->r_jd->num_comp * p->Q * p->Q * p->Q)))
+  if (p->r_jd && (p->r_jd->num_elem != p->nelem || (p->v_jac && p->v_jac->length < (CeedSize)p->nelem * p->r_jd->num_comp * npts)))
     return MI_ERR(op->ceed, CEED_ERROR_DIMENSION, "jac_data must be a strided field over the operator's %d elements", p->nelem);
   return 0;
 }
@@ -55,8 +58,8 @@ static int check_qdata_field(CeedOperator op, const OpPlan *p) {
 static int plan_bind(CeedOperator op) {
   MI_CALL(plan_bind_fields(op));
   OpPlan *p    = op->plan;
-  int     ierr = check_point_layout(op, p->r_qd, p->Q);
-  if (!ierr) ierr = check_point_layout(op, p->r_jd, p->Q);
+  int     ierr = check_point_layout(op, p->r_qd, p->Q, p->face);
+  if (!ierr) ierr = check_point_layout(op, p->r_jd, p->Q, p->face);
   if (!ierr) ierr = check_qdata_field(op, p);
   if (ierr) {
     delete p;
@@ -73,6 +76,7 @@ static bool is_active(CeedVector v) { return v == CEED_VECTOR_ACTIVE; }
 static bool is_strided(CeedElemRestriction r) { return r && r != CEED_ELEMRESTRICTION_NONE && r->strided; }
 static bool is_offsets(CeedElemRestriction r) { return r && r != CEED_ELEMRESTRICTION_NONE && !r->strided; }
 static bool is_tensor3(CeedBasis b) { return b && b != CEED_BASIS_NONE && b->tensor && b->dim == 3; }
+static bool is_tensor2(CeedBasis b) { return b && b != CEED_BASIS_NONE && b->tensor && b->dim == 2; }
 
 #define BIND_CHECK(cond, ...)                                                       \
   do {                                                                              \
@@ -100,6 +104,59 @@ static int bind_solution_basis(CeedOperator op, OpPlan *p, CeedOperatorField_pri
   BIND_CHECK(volume_pq_supported(p->P, p->Q), "no fused kernel for P_1d=%d, Q_1d=%d", p->P, p->Q);
   BIND_CHECK(f->rstr->elem_size == p->P * p->P * p->P, "restriction element size does not match basis");
   MI_CALL(basis_device_tables(f->basis));
+  return 0;
+}
+
+// Face solution field: 5-component field on the boundary faces with a 2D tensor basis (P_1d = Q_1d)
+static int bind_face_basis(CeedOperator op, OpPlan *p, CeedOperatorField_private *f) {
+  BIND_CHECK(is_offsets(f->rstr) && is_tensor2(f->basis), "face field '%s' needs an offset restriction and a 2D tensor basis", f->name.c_str());
+  BIND_CHECK(f->rstr->num_comp == 5 && f->basis->num_comp == 5, "face field '%s' must have 5 components", f->name.c_str());
+  p->face  = true;
+  p->r_q   = f->rstr;
+  p->b_q   = f->basis;
+  p->P     = f->basis->P_1d;
+  p->Q     = f->basis->Q_1d;
+  p->nelem = f->rstr->num_elem;
+  BIND_CHECK(face_pq_supported(p->P, p->Q), "no fused face kernel for P_1d=%d, Q_1d=%d", p->P, p->Q);
+  BIND_CHECK(f->rstr->elem_size == p->P * p->P, "face restriction element size does not match the face basis");
+  MI_CALL(basis_device_tables(f->basis));
+  return 0;
+}
+
+// Face operators (boundary_condition.c:300-360): q/dq (INTERP), Grad_q/Grad_dq (GRAD, 2 reference directions), surface
+// qdata, x; output v (INTERP) and, for the residual of an implicit problem, the surface Jacobian data
+static int bind_face_op(CeedOperator op, OpPlan *p, bool jacobian, int njac) {
+  CeedQFunction qf       = op->qf;
+  const char   *names[5] = {jacobian ? "dq" : "q", jacobian ? "Grad_dq" : "Grad_q", "surface qdata", "x", "surface jacobian data"};
+  CeedEvalMode  modes[5] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_NONE, CEED_EVAL_INTERP, CEED_EVAL_NONE};
+  CeedInt       sizes[5] = {5, 10, 10, 3, njac};
+  const size_t  nin      = jacobian ? 5 : 4;
+  BIND_CHECK(qf->inputs.size() == nin, "%s: unexpected input fields", qf->name.c_str());
+  for (size_t i = 0; i < nin; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+  MI_CALL(check_field(op, 0, false, "v", CEED_EVAL_INTERP, 5));
+  auto **fi = op->in_fields.data(), **fo = op->out_fields.data();
+  MI_CALL(bind_face_basis(op, p, fi[0]));
+  BIND_CHECK(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q,
+             "%s: the solution value and gradient must share the active face restriction/basis", qf->name.c_str());
+  BIND_CHECK(is_strided(fi[2]->rstr) && !is_active(fi[2]->vec), "%s: surface qdata must be a passive strided field", qf->name.c_str());
+  p->r_qd    = fi[2]->rstr;
+  p->v_qdata = fi[2]->vec;
+  p->qd      = strided_layout(p->r_qd);
+  BIND_CHECK(is_active(fo[0]->vec) && fo[0]->rstr == p->r_q, "%s: v must be active on the face space", qf->name.c_str());
+  if (jacobian) {
+    BIND_CHECK(is_strided(fi[4]->rstr) && !is_active(fi[4]->vec), "%s: surface jacobian data must be a passive strided field", qf->name.c_str());
+    p->r_jd  = fi[4]->rstr;
+    p->v_jac = fi[4]->vec;
+    p->jd    = strided_layout(p->r_jd);
+    p->njac  = njac;
+  } else if (qf->outputs.size() == 2) {
+    MI_CALL(check_field(op, 1, false, "surface jacobian data", CEED_EVAL_NONE, njac));
+    BIND_CHECK(is_strided(fo[1]->rstr) && !is_active(fo[1]->vec), "%s: surface jacobian data must be a passive strided field", qf->name.c_str());
+    p->r_jd  = fo[1]->rstr;
+    p->v_jac = fo[1]->vec;
+    p->jd    = strided_layout(p->r_jd);
+    p->njac  = njac;
+  }
   return 0;
 }
 
@@ -291,6 +348,40 @@ static int plan_bind_fields(CeedOperator op) {
       p->kernel_name = "monitor_total_ke";
       return 0;
     }
+    case QfKind::SetupBoundary: {  // setupgeo.h:100-117 wired by the boundary qdata setup (src/qdata.c)
+      if ((ierr = check_field(op, 0, true, "dx", CEED_EVAL_GRAD, 6))) return fail(ierr);
+      if ((ierr = check_field(op, 1, true, "weight", CEED_EVAL_WEIGHT, 1))) return fail(ierr);
+      if ((ierr = check_field(op, 0, false, "surface qdata", CEED_EVAL_NONE, 10))) return fail(ierr);
+      auto *fx = op->in_fields[0], *fq = op->out_fields[0];
+      if (!(is_offsets(fx->rstr) && is_tensor2(fx->basis) && fx->basis->P_1d == 2 && fx->rstr->num_comp == 3 && fx->rstr->elem_size == 4) ||
+          !is_active(fx->vec))
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "SetupBoundary: 'dx' must be the active bilinear face coordinate field"));
+      if (!is_strided(fq->rstr) || !is_active(fq->vec) || fq->rstr->num_comp != 10)
+        return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "SetupBoundary: surface qdata must be an active strided 10-component field"));
+      p->face  = true;
+      p->r_x   = fx->rstr;
+      p->b_x   = fx->basis;
+      p->Q     = fx->basis->Q_1d;
+      p->Px    = 2;
+      p->nelem = fx->rstr->num_elem;
+      p->r_qd  = fq->rstr;
+      p->qd    = strided_layout(fq->rstr);
+      if (!face_pq_supported(p->Q, p->Q)) return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "SetupBoundary: Q_1d=%d not supported", p->Q));
+      if ((ierr = basis_device_tables(fx->basis))) return fail(ierr);
+      p->kernel_name = "setup_boundary";
+      return 0;
+    }
+    case QfKind::BoundaryIntegral:  // newtonian.h:328-375 wired by problems/newtonian.c:513-569
+    case QfKind::BoundaryIntegralJacobian:
+      if ((ierr = bind_face_op(op, p, qf->kind == QfKind::BoundaryIntegralJacobian, 11))) return fail(ierr);
+      p->kernel_name = qf->kind == QfKind::BoundaryIntegral ? "boundary_integral" : "boundary_integral_jacobian";
+      return 0;
+    case QfKind::Freestream:  // bc_freestream.h:14-114 wired by problems/bc_freestream.c:30-180
+    case QfKind::FreestreamJacobian:
+      if ((ierr = bind_face_op(op, p, qf->kind == QfKind::FreestreamJacobian, 5))) return fail(ierr);
+      p->hllc        = qf->hllc;
+      p->kernel_name = qf->kind == QfKind::Freestream ? "freestream" : "freestream_jacobian";
+      return 0;
     default:
       break;
   }
@@ -323,7 +414,7 @@ static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
   a.perm = nullptr;
   if (ceed->scatter != 1) return 0;
   MI_CALL(rstr_transpose_map(p->r_q));
-  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->P * p->P * p->P * 5, &a.ev));
+  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->r_q->elem_size * 5, &a.ev));
   a.perm = kSortedEvec ? p->r_q->d_t_perm : nullptr;
   return 0;
 }
@@ -463,6 +554,73 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       if (p->kind == QfKind::RHSFunction) MI_HIP(ceed, launch_rhs(p->P, p->Q, a, ctx, s));
       else if (p->kind == QfKind::MonitorKE) MI_HIP(ceed, launch_monitor_ke(p->P, p->Q, p->var, a, ctx, s));
       else MI_HIP(ceed, launch_mass(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, s));
+      MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      MI_CALL(scatter_finish(p, ceed, a, overwrite));
+      return 0;
+    }
+    case QfKind::SetupBoundary: {
+      SetupArgs a{};
+      a.nelem = p->nelem;
+      MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
+      a.xcomp_stride = p->r_x->comp_stride;
+      MI_CALL(vec_device_read(in, &a.x));
+      a.Bx  = p->b_x->d_interp_1d;
+      a.Dx  = p->b_x->d_grad_1d;
+      a.w1d = p->b_x->d_weight_1d;
+      if (out->length < (CeedSize)p->nelem * p->Q * p->Q * 10) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "surface qdata vector too short");
+      MI_CALL(vec_device_write(out, &a.qdata, true));
+      a.qd = p->qd;
+      MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+      MI_HIP(ceed, launch_setup_boundary(p->Q, a, s));
+      MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      return 0;
+    }
+    case QfKind::BoundaryIntegral:
+    case QfKind::BoundaryIntegralJacobian:
+    case QfKind::Freestream:
+    case QfKind::FreestreamJacobian: {
+      const bool jacobian   = p->kind == QfKind::BoundaryIntegralJacobian || p->kind == QfKind::FreestreamJacobian;
+      const bool freestream = p->kind == QfKind::Freestream || p->kind == QfKind::FreestreamJacobian;
+      if (in->length != p->r_q->l_size || out->length != p->r_q->l_size)
+        return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: active vectors must have the face restriction's L-vector size", op->qf->name.c_str());
+      CeedMiNewtonianContext  nctx{};
+      CeedMiFreestreamContext fctx{};
+      if (freestream) {
+        CeedQFunctionContext c = op->qf->ctx;
+        if (!c || !c->data) return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "%s: missing QFunction context", op->qf->name.c_str());
+        if (c->size != sizeof(CeedMiFreestreamContext))
+          return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: context size %zu does not match struct FreestreamContext_ (%zu)", op->qf->name.c_str(),
+                        c->size, sizeof(CeedMiFreestreamContext));
+        memcpy(&fctx, c->data, sizeof(fctx));
+      } else {
+        MI_CALL(newtonian_context(op, &nctx));
+      }
+      const bool implicit = freestream ? fctx.newt_ctx.is_implicit : nctx.is_implicit;
+      VolumeArgs a{};
+      a.nelem = p->nelem;
+      MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+      a.comp_stride = p->r_q->comp_stride;
+      MI_CALL(vec_device_read(in, &a.u));
+      MI_CALL(vec_device_write(out, &a.v, true));
+      MI_CALL(scatter_prepare(p, ceed, a));
+      MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+      a.qd = p->qd;
+      a.jd = p->jd;
+      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, nullptr, nullptr, nullptr};
+      if (p->v_jac) {
+        if (jacobian) {
+          if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, false));
+          else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
+        } else if (implicit) {
+          if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, true));
+          else MI_CALL(vec_device_write(p->v_jac, &a.jac, true));
+        }
+      } else if (jacobian) {
+        return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "%s: surface jacobian data is required", op->qf->name.c_str());
+      }
+      MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+      if (freestream) MI_HIP(ceed, launch_face_freestream(p->P, p->Q, p->var, p->hllc, jacobian, ceed->jac_fp32, a, fctx, s));
+      else MI_HIP(ceed, launch_face_boundary_integral(p->P, p->Q, p->var, jacobian, ceed->jac_fp32, a, nctx, s));
       MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
       MI_CALL(scatter_finish(p, ceed, a, overwrite));
       return 0;

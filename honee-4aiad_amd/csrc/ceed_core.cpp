@@ -1052,6 +1052,23 @@ static void qf_identify(CeedQFunction qf) {
   } else if (n.rfind("MonitorTotalKineticEnergy_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::MonitorKE;
     qf->state_var = var_of(n);
+  } else if (n == "SetupBoundary") {
+    qf->kind = QfKind::SetupBoundary;
+  } else if (n.rfind("BoundaryIntegral_Jacobian_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::BoundaryIntegralJacobian;
+    qf->state_var = var_of(n);
+  } else if (n.rfind("BoundaryIntegral_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::BoundaryIntegral;
+    qf->state_var = var_of(n);
+  } else if (n.rfind("Freestream_", 0) == 0) {  // Freestream[_Jacobian]_{Conserv,Prim,Entropy}_{HLL,HLLC}
+    const bool        hllc = n.size() > 5 && n.compare(n.size() - 5, 5, "_HLLC") == 0;
+    const bool        hll  = !hllc && n.size() > 4 && n.compare(n.size() - 4, 4, "_HLL") == 0;
+    const std::string base = hllc ? n.substr(0, n.size() - 5) : hll ? n.substr(0, n.size() - 4) : n;
+    if ((hll || hllc) && var_of(base) >= 0) {
+      qf->kind      = base.rfind("Freestream_Jacobian_", 0) == 0 ? QfKind::FreestreamJacobian : QfKind::Freestream;
+      qf->state_var = var_of(base);
+      qf->hllc      = hllc;
+    }
   }
 }
 

@@ -94,6 +94,11 @@ enum class QfKind {
   MassNewtonian,
   Mass5,
   MonitorKE,
+  SetupBoundary,
+  BoundaryIntegral,
+  BoundaryIntegralJacobian,
+  Freestream,
+  FreestreamJacobian,
 };
 
 struct CeedQFunction_private {
@@ -107,6 +112,7 @@ struct CeedQFunction_private {
   CeedSize             user_flops = 0;
   QfKind               kind       = QfKind::Unknown;
   int                  state_var  = 0;
+  bool                 hllc       = false;  // Freestream*: HLLC (true) or HLL Riemann flux
 };
 
 struct CtxField {

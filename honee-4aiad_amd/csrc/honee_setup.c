@@ -22,6 +22,19 @@ static const char *const kIFunctionLoc[3] = {"qfunctions/newtonian.h:IFunction_N
 static const char *const kIJacobianLoc[3] = {"qfunctions/newtonian.h:IJacobian_Newtonian_Conserv", "qfunctions/newtonian.h:IJacobian_Newtonian_Prim",
                                              "qfunctions/newtonian.h:IJacobian_Newtonian_Entropy"};
 static const char *const kSetupLoc        = "qfunctions/setupgeo.h:Setup";
+static const char *const kSetupBoundaryLoc = "qfunctions/setupgeo.h:SetupBoundary";
+static const char *const kVarName[3]       = {"Conserv", "Prim", "Entropy"};
+
+#define HONEE_MI_MAX_BOUNDARIES 8
+
+// One boundary definition: its face restrictions, bases, quadrature data and Jacobian data
+typedef struct {
+  int                  kind, nfaces, njac;
+  CeedElemRestriction  restr_q, restr_x, restr_qd, restr_jd;
+  CeedBasis            basis_q, basis_x;
+  CeedVector           x_coord, q_data, jac_data;
+  CeedQFunctionContext fs_qfctx;
+} HoneeMiBoundary;
 
 struct HoneeMiProblem_ {
   Ceed                  ceed;
@@ -32,7 +45,10 @@ struct HoneeMiProblem_ {
   CeedContextFieldLabel solution_time_label, timestep_size_label, ijacobian_shift_label;
   CeedQFunctionContext  newt_qfctx;
   CeedMiNewtonianContext *ctx;
-  int                   nelem, Q3, nnodes, num_comps_jac_data;
+  int                   nelem, Q3, nnodes, num_comps_jac_data, state_var;
+  HoneeMiMesh           mesh;  // tabulation pointers stay owned by the caller for the problem's lifetime
+  HoneeMiBoundary       bnd[HONEE_MI_MAX_BOUNDARIES];
+  int                   nbnd;
   char                  err[512];
 };
 
@@ -58,7 +74,9 @@ static int FreeContext(void *data) {
 
 static int Setup(HoneeMiProblem p, const char *resource, const HoneeMiMesh *m, const CeedMiNewtonianContext *ctx_in, int state_var) {
   const int P = m->P, Q = m->Q, P3 = P * P * P;
-  p->nelem  = m->nelem;
+  p->mesh      = *m;
+  p->state_var = state_var;
+  p->nelem     = m->nelem;
   p->Q3     = Q * Q * Q;
   p->nnodes = m->nnodes;
   CHK(p, CeedInit(resource, &p->ceed));
@@ -202,6 +220,145 @@ Ceed HoneeMiProblemGetCeed(HoneeMiProblem p) { return p->ceed; }
 
 CeedOperator HoneeMiProblemGetOperator(HoneeMiProblem p, int which) { return which == 0 ? p->op_ifunction : p->op_ijacobian; }
 
+// StateFromPrimitive (newtonian_state.h:175-180) for the exterior state of the freestream context
+static void StateFromPrimitive(const CeedMiGas *gas, const double Y[5], CeedMiState *s) {
+  const double Rgas = gas->cp - gas->cv;
+  for (int i = 0; i < 5; i++) s->Y[i] = Y[i];
+  s->U[0]         = Y[0] / (Rgas * Y[4]);
+  for (int i = 0; i < 3; i++) s->U[1 + i] = s->U[0] * Y[1 + i];
+  const double ei = gas->cv * Y[4];
+  const double ek = .5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
+  s->U[4]         = s->U[0] * (ei + ek);
+}
+
+// A boundary definition (HoneeBCAddIFunctionOp / HoneeBCAddIJacobianOp, src/boundary_condition.c:290-490)
+static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y) {
+  if (p->nbnd >= HONEE_MI_MAX_BOUNDARIES || kind < 0 || kind > 2 || nfaces <= 0) return CEED_ERROR_DIMENSION;
+  HoneeMiBoundary *b = &p->bnd[p->nbnd++];
+  const HoneeMiMesh *m = &p->mesh;
+  const int P = m->P, Q = m->Q, P2 = P * P, Q2 = Q * Q;
+  b->kind   = kind;
+  b->nfaces = nfaces;
+  b->njac   = kind == 0 ? 11 : 5;
+  CHK(p, CeedBasisCreateTensorH1(p->ceed, 2, 5, P, Q, m->B, m->D, m->qref, m->qweight, &b->basis_q));
+  CHK(p, CeedBasisCreateTensorH1(p->ceed, 2, 3, 2, Q, m->Bx, m->Dx, m->qref, m->qweight, &b->basis_x));
+  {
+    CeedInt *off = malloc(sizeof(CeedInt) * (size_t)nfaces * P2);
+    for (size_t i = 0; i < (size_t)nfaces * P2; i++) off[i] = 5 * face_nodes[i];
+    CHK(p, CeedElemRestrictionCreate(p->ceed, nfaces, P2, 5, 1, 5 * (CeedSize)m->nnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off, &b->restr_q));
+    for (size_t i = 0; i < (size_t)nfaces * 4; i++) off[i] = 3 * face_xnodes[i];
+    CHK(p, CeedElemRestrictionCreate(p->ceed, nfaces, 4, 3, 1, 3 * (CeedSize)m->nxnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off, &b->restr_x));
+    free(off);
+  }
+  CHK(p, CeedElemRestrictionCreateVector(b->restr_x, &b->x_coord, NULL));
+  CHK(p, CeedVectorSetArray(b->x_coord, CEED_MEM_HOST, CEED_COPY_VALUES, (CeedScalar *)m->xcoord));
+  {  // boundary quadrature data (QDataBoundaryGet, src/qdata.c)
+    CeedQFunction qf_setup;
+    CeedOperator  op_setup;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kSetupBoundaryLoc, &qf_setup));
+    CHK(p, CeedQFunctionAddInput(qf_setup, "dx", 6, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf_setup, "weight", 1, CEED_EVAL_WEIGHT));
+    CHK(p, CeedQFunctionAddOutput(qf_setup, "surface qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedElemRestrictionCreateStrided(p->ceed, nfaces, Q2, 10, (CeedSize)nfaces * Q2 * 10, CEED_STRIDES_BACKEND, &b->restr_qd));
+    CHK(p, CeedElemRestrictionCreateVector(b->restr_qd, &b->q_data, NULL));
+    CHK(p, CeedOperatorCreate(p->ceed, qf_setup, NULL, NULL, &op_setup));
+    CHK(p, CeedOperatorSetField(op_setup, "dx", b->restr_x, b->basis_x, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_setup, "weight", CEED_ELEMRESTRICTION_NONE, b->basis_x, CEED_VECTOR_NONE));
+    CHK(p, CeedOperatorSetField(op_setup, "surface qdata", b->restr_qd, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorApply(op_setup, b->x_coord, b->q_data, CEED_REQUEST_IMMEDIATE));
+    CHK(p, CeedOperatorDestroy(&op_setup));
+    CHK(p, CeedQFunctionDestroy(&qf_setup));
+  }
+  const int implicit = p->ctx->is_implicit;
+  if (implicit) {
+    CHK(p, CeedElemRestrictionCreateStrided(p->ceed, nfaces, Q2, b->njac, (CeedSize)nfaces * Q2 * b->njac, CEED_STRIDES_BACKEND, &b->restr_jd));
+    CHK(p, CeedElemRestrictionCreateVector(b->restr_jd, &b->jac_data, NULL));
+  }
+  CeedQFunctionContext qfctx = p->newt_qfctx;
+  if (kind > 0) {  // FreestreamContext_ (bc_freestream_type.h:9-12), filled as problems/bc_freestream.c:156-163 does
+    CeedMiFreestreamContext *fs = malloc(sizeof(*fs));
+    fs->newt_ctx                = *p->ctx;
+    StateFromPrimitive(&p->ctx->gas, S_inf_Y, &fs->S_infty);
+    CHK(p, CeedQFunctionContextCreate(p->ceed, &b->fs_qfctx));
+    CHK(p, CeedQFunctionContextSetData(b->fs_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, sizeof(*fs), fs));
+    CHK(p, CeedQFunctionContextSetDataDestroy(b->fs_qfctx, CEED_MEM_HOST, FreeContext));
+    qfctx = b->fs_qfctx;
+  }
+  char loc_f[128], loc_j[128];
+  if (kind == 0) {
+    snprintf(loc_f, sizeof(loc_f), "qfunctions/newtonian.h:BoundaryIntegral_%s", kVarName[p->state_var]);
+    snprintf(loc_j, sizeof(loc_j), "qfunctions/newtonian.h:BoundaryIntegral_Jacobian_%s", kVarName[p->state_var]);
+  } else {
+    const char *flux = kind == 1 ? "HLL" : "HLLC";
+    snprintf(loc_f, sizeof(loc_f), "qfunctions/bc_freestream.h:Freestream_%s_%s", kVarName[p->state_var], flux);
+    snprintf(loc_j, sizeof(loc_j), "qfunctions/bc_freestream.h:Freestream_Jacobian_%s_%s", kVarName[p->state_var], flux);
+  }
+  {  // IFunction face operator (src/boundary_condition.c:300-310, 400-410)
+    CeedQFunction qf;
+    CeedOperator  op;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, loc_f, &qf));
+    CHK(p, CeedQFunctionSetContext(qf, qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 10, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "surface qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+    if (implicit) CHK(p, CeedQFunctionAddOutput(qf, "surface jacobian data", b->njac, CEED_EVAL_NONE));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op));
+    CHK(p, CeedOperatorSetField(op, "q", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op, "Grad_q", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op, "surface qdata", b->restr_qd, CEED_BASIS_NONE, b->q_data));
+    CHK(p, CeedOperatorSetField(op, "x", b->restr_x, b->basis_x, b->x_coord));
+    CHK(p, CeedOperatorSetField(op, "v", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    if (implicit) CHK(p, CeedOperatorSetField(op, "surface jacobian data", b->restr_jd, CEED_BASIS_NONE, b->jac_data));
+    CHK(p, CeedOperatorCompositeAddSub(p->op_ifunction, op));
+    CHK(p, CeedOperatorDestroy(&op));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  if (implicit) {  // IJacobian face operator (src/boundary_condition.c:350-360, 475-485)
+    CeedQFunction qf;
+    CeedOperator  op;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, loc_j, &qf));
+    CHK(p, CeedQFunctionSetContext(qf, qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "dq", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_dq", 10, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "surface qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "surface jacobian data", b->njac, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op));
+    CHK(p, CeedOperatorSetField(op, "dq", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op, "Grad_dq", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op, "surface qdata", b->restr_qd, CEED_BASIS_NONE, b->q_data));
+    CHK(p, CeedOperatorSetField(op, "x", b->restr_x, b->basis_x, b->x_coord));
+    CHK(p, CeedOperatorSetField(op, "surface jacobian data", b->restr_jd, CEED_BASIS_NONE, b->jac_data));
+    CHK(p, CeedOperatorSetField(op, "v", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorCompositeAddSub(p->op_ijacobian, op));
+    CHK(p, CeedOperatorDestroy(&op));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  return 0;
+}
+
+int HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y) {
+  if (kind > 0 && !S_inf_Y) return CEED_ERROR_INCOMPLETE;
+  return AddBoundary(p, kind, nfaces, face_nodes, face_xnodes, S_inf_Y);
+}
+
+int HoneeMiGetBoundaryData(HoneeMiProblem p, int boundary, int which, double *out) {
+  if (boundary < 0 || boundary >= p->nbnd) return CEED_ERROR_DIMENSION;
+  HoneeMiBoundary *b = &p->bnd[boundary];
+  CeedVector       v = which == 0 ? b->q_data : b->jac_data;
+  if (!v) return CEED_ERROR_INCOMPLETE;
+  const CeedScalar *a;
+  CeedSize          n;
+  CHK(p, CeedVectorGetLength(v, &n));
+  CHK(p, CeedVectorGetArrayRead(v, CEED_MEM_HOST, &a));
+  memcpy(out, a, sizeof(double) * (size_t)n);  // backend layout [face][comp][qpt]
+  CHK(p, CeedVectorRestoreArrayRead(v, &a));
+  return 0;
+}
+
 // IFunction_NS's operator section (src/setupts.c:90-111): update context, hand over arrays, apply, take arrays back.
 int HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g) {
   CeedScalar *arr;
@@ -290,6 +447,19 @@ int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   CeedElemRestrictionDestroy(&p->elem_restr_jd);
   CeedBasisDestroy(&p->basis_q);
   CeedBasisDestroy(&p->basis_x);
+  for (int i = 0; i < p->nbnd; i++) {
+    HoneeMiBoundary *b = &p->bnd[i];
+    CeedVectorDestroy(&b->x_coord);
+    CeedVectorDestroy(&b->q_data);
+    CeedVectorDestroy(&b->jac_data);
+    CeedElemRestrictionDestroy(&b->restr_q);
+    CeedElemRestrictionDestroy(&b->restr_x);
+    CeedElemRestrictionDestroy(&b->restr_qd);
+    CeedElemRestrictionDestroy(&b->restr_jd);
+    CeedBasisDestroy(&b->basis_q);
+    CeedBasisDestroy(&b->basis_x);
+    if (b->fs_qfctx) CeedQFunctionContextDestroy(&b->fs_qfctx);
+  }
   if (p->newt_qfctx) CeedQFunctionContextDestroy(&p->newt_qfctx);
   else free(p->ctx);
   CeedDestroy(&p->ceed);

@@ -32,6 +32,13 @@ CeedOperator HoneeMiProblemGetOperator(HoneeMiProblem p, int which);  // 0: IFun
 int          HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g);
 int          HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const double *dq, double *v);
 int          HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, double *diag);
+// Boundary faces of the problem (HONEE's BC definitions, problems/newtonian.c:555-569 and problems/bc_freestream.c:30-180):
+// kind 0: consistent boundary integral; 1: freestream with HLL flux; 2: freestream with HLLC flux (S_inf_Y is the
+// exterior primitive state).  face_nodes [nfaces][P^2] are the face nodes in face-local (X1 fastest) order with
+// outward dx/dX1 x dx/dX2, face_xnodes [nfaces][4] the coordinate nodes likewise.  The face operators are added to the
+// IFunction and IJacobian composites.
+int          HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y);
+int          HoneeMiGetBoundaryData(HoneeMiProblem p, int boundary, int which, double *out);  // which 0: surface qdata, 1: jac data
 int          HoneeMiGetQData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData32(HoneeMiProblem p, float *out);

@@ -60,6 +60,14 @@ hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs 
 hipError_t launch_monitor_ke(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 
 // Vector kernels (kernels_vector.hip)
+// Boundary faces (2D tensor faces of the hexahedral mesh); VolumeArgs::nelem is the number of faces
+bool       face_pq_supported(int P, int Q);
+hipError_t launch_setup_boundary(int Q, const SetupArgs &a, hipStream_t stream);
+hipError_t launch_face_boundary_integral(int P, int Q, int var, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx,
+                                         hipStream_t s);
+hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiFreestreamContext &ctx,
+                                  hipStream_t s);
+
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);
 hipError_t launch_axpby(double *y, double alpha, double beta, const double *x, long long n, hipStream_t s);
 hipError_t launch_scale(double *x, double alpha, long long n, hipStream_t s);

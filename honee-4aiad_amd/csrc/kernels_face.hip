@@ -1,0 +1,287 @@
+// Fused kernels of the boundary-face operators (2D tensor faces of the hexahedral mesh) on MI355X.
+//
+// k_face<P, Q, Op, Ctx> applies one face sub-operator in one launch: per face it gathers the L-vector through the face
+// restriction, interpolates values (and the two reference tangential derivatives, for the boundary integral) to the
+// Q x Q face quadrature points in LDS, evaluates the face body (face_ops.hpp), applies the interpolation transpose and
+// scatters (FP64 atomics, or the E-vector of the deterministic two-pass scatter).  Each face owns Q^2 consecutive
+// threads; a 64-thread block holds FPB faces.
+//
+// k_setup_boundary<Q> computes the face quadrature data {w detJb, normal, dXdx} from the bilinear coordinate field
+// (SetupBoundary, qfunctions/setupgeo.h:100-117).
+#include <hip/hip_runtime.h>
+
+#include "face_ops.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+template <int Q>
+constexpr int faces_per_block() {
+  return (Q * Q >= 64) ? 1 : 64 / (Q * Q);
+}
+
+// SetupBoundary with NormalVectorFromdxdX_3D (setupgeo_helpers.h:78-91) and InvertBoundaryMappingJacobian_3D
+// (setupgeo_helpers.h:143-171); one thread per face quadrature point.
+template <int Q>
+__global__ void __launch_bounds__(256) k_setup_boundary(SetupArgs a) {
+  constexpr int   Q2 = Q * Q;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)a.nelem * Q2) return;
+  const int f = (int)(gid / Q2), i = (int)(gid % Q2), qx = i % Q, qy = i / Q;
+  double    dxdX[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const int    xo = a.xoffsets[(long long)f * 4 + n];
+    const int    nx = n & 1, ny = n >> 1;
+    const double b1 = a.Bx[qx * 2 + nx], d1 = a.Dx[qx * 2 + nx], b2 = a.Bx[qy * 2 + ny], d2 = a.Dx[qy * 2 + ny];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double xv = a.x[xo + c * a.xcomp_stride];
+      dxdX[c][0] += d1 * b2 * xv;
+      dxdX[c][1] += b1 * d2 * xv;
+    }
+  }
+  double normal[3];
+  normal[0]          = dxdX[1][0] * dxdX[2][1] - dxdX[2][0] * dxdX[1][1];
+  normal[1]          = dxdX[2][0] * dxdX[0][1] - dxdX[0][0] * dxdX[2][1];
+  normal[2]          = dxdX[0][0] * dxdX[1][1] - dxdX[1][0] * dxdX[0][1];
+  const double detJb = sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+#pragma unroll
+  for (int d = 0; d < 3; d++) normal[d] *= 1 / detJb;
+  double JTJ[2][2] = {{0., 0.}, {0., 0.}};
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int l = 0; l < 3; l++) JTJ[j][k] += dxdX[l][j] * dxdX[l][k];
+  const double det       = JTJ[0][0] * JTJ[1][1] - JTJ[1][0] * JTJ[0][1];
+  const double inv[2][2] = {{JTJ[1][1] / det, -JTJ[0][1] / det}, {-JTJ[1][0] / det, JTJ[0][0] / det}};
+  const double w         = a.w1d[qx] * a.w1d[qy];
+  double      *qd        = a.qdata + (long long)f * a.qd.se + i;
+  qd[0]                  = w * detJb;
+#pragma unroll
+  for (int d = 0; d < 3; d++) qd[(1 + d) * Q2] = normal[d];
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int l = 0; l < 2; l++) acc += inv[l][j] * dxdX[k][l];
+      qd[(4 + 3 * j + k) * Q2] = acc;
+    }
+}
+
+// Per-face LDS stage: X[slot][Q^2].  Slot map: values 0..4, reference tangential derivatives 5 + 5 d + c (d = 0, 1);
+// scratch 15..19.  Outputs (interpolation test weights) reuse slots 0..4.
+template <int P, int Q, class Op, class Ctx>
+__global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face(VolumeArgs a, const Ctx ctx) {
+  constexpr int FPB = faces_per_block<Q>();
+  constexpr int Q2 = Q * Q, P2 = P * P, PQ = P * Q;
+  constexpr int NST = 20;
+  __shared__ double sB[Q * P], sDc[Q * Q];
+  __shared__ double sX[FPB][NST][Q2];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) sB[i] = a.bt.B[i];
+  for (int i = threadIdx.x; i < Q * Q; i += blockDim.x) sDc[i] = a.bt.Dc[i];
+  const int le = threadIdx.x / Q2, t = threadIdx.x % Q2, qx = t % Q, qy = t / Q;
+  const bool nd = t < P2;
+  double(*X)[Q2] = sX[le];
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long f_  = bt * FPB + le;
+    const bool      ok  = f_ < a.nelem;
+    const int       f   = ok ? (int)f_ : 0;
+    const int       off = (ok && nd) ? a.offsets[(long long)f * P2 + t] : 0;
+    double          un[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) un[c] = (ok && nd) ? a.u[off + c * a.comp_stride] : 0.0;
+    __syncthreads();  // the previous batch's transpose has finished reading the stage
+    if (nd) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) X[15 + c][t] = un[c];
+    }
+    __syncthreads();
+    if (t < PQ) {  // x: X[5 + c][j][ix] = sum_i B[ix][i] u[j][i]
+      const int ix = t % Q, j = t / Q;
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < P; i++) acc += sB[ix * P + i] * X[15 + c][j * P + i];
+        X[5 + c][t] = acc;
+      }
+    }
+    __syncthreads();
+    double u[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {  // y: value at (qx, qy)
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < P; j++) acc += sB[qy * P + j] * X[5 + c][j * Q + qx];
+      u[c] = acc;
+    }
+    double gref[2][5];
+    if constexpr (Op::kGrad) {  // collocated tangential derivatives
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 5; c++) X[c][t] = u[c];
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        double g0 = 0, g1 = 0;
+#pragma unroll
+        for (int m = 0; m < Q; m++) {
+          g0 += sDc[qx * Q + m] * X[c][qy * Q + m];
+          g1 += sDc[qy * Q + m] * X[c][m * Q + qx];
+        }
+        gref[0][c] = g0;
+        gref[1][c] = g1;
+      }
+    }
+    double v[5];
+    Op::template qf<Q>(a, ctx, f, t, ok, u, gref, v);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 5; c++) X[c][t] = v[c];
+    __syncthreads();
+    if (t < PQ) {  // y^T: X[5 + c][j][ix] = sum_qy B[qy][j] v[qy][ix]
+      const int ix = t % Q, j = t / Q;
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int y = 0; y < Q; y++) acc += sB[y * P + j] * X[c][y * Q + ix];
+        X[5 + c][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (ok && nd) {  // x^T and scatter: node (i, j) = t
+      const int i = t % P, j = t / P;
+      double    r[5];
+#pragma unroll
+      for (int c = 0; c < 5; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int x = 0; x < Q; x++) acc += sB[x * P + i] * X[5 + c][j * Q + x];
+        r[c] = acc;
+      }
+      if (a.ev) {
+        const long long en  = (long long)f * P2 + t;
+        double         *dst = a.ev + (a.perm ? (long long)a.perm[en] : en) * 5;
+#pragma unroll
+        for (int c = 0; c < 5; c++) dst[c] = r[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 5; c++) atomicAdd(&a.v[off + c * a.comp_stride], r[c]);
+      }
+    }
+  }
+}
+
+template <int P, int Q, class Op, class Ctx>
+hipError_t launch_face_op(const VolumeArgs &a, const Ctx &ctx, hipStream_t stream) {
+  constexpr int   FPB    = faces_per_block<Q>();
+  const int       block  = FPB * Q * Q;
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+  int             dev = 0, ncu = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  long long grid = (long long)ncu * 8;
+  if (grid > nbatch) grid = nbatch;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_face<P, Q, Op, Ctx>), dim3((unsigned)grid), dim3(block), 0, stream, a, ctx);
+  return hipGetLastError();
+}
+
+template <int P, int Q, bool JAC>
+hipError_t face_boundary_integral_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+  if constexpr (!JAC) {
+    switch (var) {
+      case 0: return launch_face_op<P, Q, fop::BoundaryIntegral<0>>(a, ctx, s);
+      case 1: return launch_face_op<P, Q, fop::BoundaryIntegral<1>>(a, ctx, s);
+      default: return launch_face_op<P, Q, fop::BoundaryIntegral<2>>(a, ctx, s);
+    }
+  } else {
+    if (fp32) {
+      switch (var) {
+        case 0: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<0, float>>(a, ctx, s);
+        case 1: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<1, float>>(a, ctx, s);
+        default: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<2, float>>(a, ctx, s);
+      }
+    }
+    switch (var) {
+      case 0: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<0, double>>(a, ctx, s);
+      case 1: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<1, double>>(a, ctx, s);
+      default: return launch_face_op<P, Q, fop::BoundaryIntegralJacobian<2, double>>(a, ctx, s);
+    }
+  }
+}
+
+template <int P, int Q, bool HLLC, bool JAC>
+hipError_t face_freestream_var(int var, bool fp32, const VolumeArgs &a, const CeedMiFreestreamContext &ctx, hipStream_t s) {
+  if constexpr (!JAC) {
+    switch (var) {
+      case 0: return launch_face_op<P, Q, fop::Freestream<0, HLLC>>(a, ctx, s);
+      case 1: return launch_face_op<P, Q, fop::Freestream<1, HLLC>>(a, ctx, s);
+      default: return launch_face_op<P, Q, fop::Freestream<2, HLLC>>(a, ctx, s);
+    }
+  } else {
+    if (fp32) {
+      switch (var) {
+        case 0: return launch_face_op<P, Q, fop::FreestreamJacobian<0, HLLC, float>>(a, ctx, s);
+        case 1: return launch_face_op<P, Q, fop::FreestreamJacobian<1, HLLC, float>>(a, ctx, s);
+        default: return launch_face_op<P, Q, fop::FreestreamJacobian<2, HLLC, float>>(a, ctx, s);
+      }
+    }
+    switch (var) {
+      case 0: return launch_face_op<P, Q, fop::FreestreamJacobian<0, HLLC, double>>(a, ctx, s);
+      case 1: return launch_face_op<P, Q, fop::FreestreamJacobian<1, HLLC, double>>(a, ctx, s);
+      default: return launch_face_op<P, Q, fop::FreestreamJacobian<2, HLLC, double>>(a, ctx, s);
+    }
+  }
+}
+
+}  // namespace
+
+#define HONEE_FACE_PQ(CALL)                   \
+  if (P == 2 && Q == 2) return CALL(2, 2);   \
+  if (P == 3 && Q == 3) return CALL(3, 3);   \
+  if (P == 4 && Q == 4) return CALL(4, 4);   \
+  if (P == 5 && Q == 5) return CALL(5, 5);
+
+bool face_pq_supported(int P, int Q) { return P == Q && P >= 2 && P <= 5; }
+
+hipError_t launch_setup_boundary(int Q, const SetupArgs &a, hipStream_t stream) {
+  const long long n     = (long long)a.nelem * Q * Q;
+  const int       block = 256;
+  const int       grid  = (int)((n + block - 1) / block);
+  if (n == 0) return hipSuccess;
+  switch (Q) {
+    case 2: hipLaunchKernelGGL((k_setup_boundary<2>), dim3(grid), dim3(block), 0, stream, a); break;
+    case 3: hipLaunchKernelGGL((k_setup_boundary<3>), dim3(grid), dim3(block), 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((k_setup_boundary<4>), dim3(grid), dim3(block), 0, stream, a); break;
+    case 5: hipLaunchKernelGGL((k_setup_boundary<5>), dim3(grid), dim3(block), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_face_boundary_integral(int P, int Q, int var, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx,
+                                         hipStream_t s) {
+#define CALL_BI(P_, Q_) (jacobian ? face_boundary_integral_var<P_, Q_, true>(var, fp32, a, ctx, s) : face_boundary_integral_var<P_, Q_, false>(var, fp32, a, ctx, s))
+  HONEE_FACE_PQ(CALL_BI)
+#undef CALL_BI
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiFreestreamContext &ctx,
+                                  hipStream_t s) {
+#define CALL_FS(P_, Q_)                                                                                                    \
+  (hllc ? (jacobian ? face_freestream_var<P_, Q_, true, true>(var, fp32, a, ctx, s) : face_freestream_var<P_, Q_, true, false>(var, fp32, a, ctx, s)) \
+        : (jacobian ? face_freestream_var<P_, Q_, false, true>(var, fp32, a, ctx, s) : face_freestream_var<P_, Q_, false, false>(var, fp32, a, ctx, s)))
+  HONEE_FACE_PQ(CALL_FS)
+#undef CALL_FS
+  return hipErrorInvalidValue;
+}

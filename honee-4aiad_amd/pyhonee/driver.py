@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import LIB_DIR
-from .mesh import BoxMesh, Tabulation
+from .mesh import BoxMesh, Tabulation, face_node_tables
 
 _dp, _ip, _fp = ct.POINTER(ct.c_double), ct.POINTER(ct.c_int), ct.POINTER(ct.c_float)
 
@@ -115,6 +115,8 @@ def load_libs():
         lib.HoneeMiApplyIFunction.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiApplyIJacobian.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiIJacobianDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
+        lib.HoneeMiProblemAddBoundary.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiGetBoundaryData.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
         lib.HoneeMiGetQData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData32.argtypes = [ct.c_void_p, ct.c_void_p]
@@ -178,6 +180,29 @@ class NewtonianProblem:
 
     def ijacobian_diagonal(self, diag, shift, mem=CEED_MEM_HOST):
         self._check(self.lib.HoneeMiIJacobianDiagonal(self.handle, mem, shift, _ptr(diag)), "IJacobian diagonal")
+
+    BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
+
+    def add_boundary(self, kind: int, face_elem, face_id, S_inf_Y=None) -> int:
+        """Add boundary faces (volume element, local face id) of the given kind to the IFunction/IJacobian composites."""
+        nodes, xnodes = face_node_tables(self.mesh, face_elem, face_id)
+        self._keep.setdefault("faces", []).append((nodes, xnodes))
+        yinf = None if S_inf_Y is None else np.ascontiguousarray(S_inf_Y, dtype=np.float64)
+        if yinf is not None:
+            self._keep["faces"].append(yinf)
+        self._check(self.lib.HoneeMiProblemAddBoundary(self.handle, int(kind), len(nodes), _ptr(nodes), _ptr(xnodes),
+                                                       None if yinf is None else _ptr(yinf)), "AddBoundary")
+        self.nbnd = getattr(self, "nbnd", 0) + 1
+        self._bnd_sizes = getattr(self, "_bnd_sizes", []) + [(len(nodes), 11 if kind == 0 else 5)]
+        return self.nbnd - 1
+
+    def boundary_data(self, boundary: int, which: int):
+        """Surface qdata (which=0, [nfaces][10][Q^2]) or surface Jacobian data (which=1) of a boundary."""
+        nf, nj = self._bnd_sizes[boundary]
+        Q2 = self.tab.Q ** 2
+        out = np.zeros((nf, 10 if which == 0 else nj, Q2))
+        self._check(self.lib.HoneeMiGetBoundaryData(self.handle, int(boundary), int(which), _ptr(out)), "GetBoundaryData")
+        return out
 
     def qdata(self):
         out = np.zeros((self.nelem, 10, self.Q3))

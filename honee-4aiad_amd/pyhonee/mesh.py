@@ -213,3 +213,44 @@ def boundary_faces(mesh: BoxMesh, face_id: int) -> np.ndarray:
     idx = [ex, ey, ez][d]
     target = 0 if side == 0 else mesh.shape[d] - 1
     return np.sort(eid[idx == target].ravel()).astype(np.int32)
+
+
+# Face-local node (i, j) -> element node (a, b, c) for face ids 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, with the first face
+# coordinate X1 along i and X2 along j chosen so that dx/dX1 x dx/dX2 points out of the element.
+_FACE_ABC = {
+    0: lambda i, j, L: (0, j, i),
+    1: lambda i, j, L: (L, i, j),
+    2: lambda i, j, L: (i, 0, j),
+    3: lambda i, j, L: (j, L, i),
+    4: lambda i, j, L: (j, i, 0),
+    5: lambda i, j, L: (i, j, L),
+}
+
+
+def face_node_tables(mesh: BoxMesh, face_elem, face_id) -> tuple[np.ndarray, np.ndarray]:
+    """Face node indices [nfaces][P^2] and coordinate node indices [nfaces][4] (face-local, X1 fastest)."""
+    P = mesh.P
+    face_elem = np.asarray(face_elem, dtype=np.int64)
+    face_id = np.asarray(face_id, dtype=np.int64)
+    nodes = np.zeros((len(face_elem), P * P), dtype=np.int32)
+    xnodes = np.zeros((len(face_elem), 4), dtype=np.int32)
+    for k, (e, f) in enumerate(zip(face_elem, face_id)):
+        for j in range(P):
+            for i in range(P):
+                a, b, c = _FACE_ABC[int(f)](i, j, P - 1)
+                nodes[k, i + P * j] = mesh.offsets[e, a + P * (b + P * c)]
+        for j in range(2):
+            for i in range(2):
+                a, b, c = _FACE_ABC[int(f)](i, j, 1)
+                xnodes[k, i + 2 * j] = mesh.xoffsets[e, a + 2 * (b + 2 * c)]
+    return nodes, xnodes
+
+
+def box_boundary(mesh: BoxMesh, sides) -> tuple[np.ndarray, np.ndarray]:
+    """(face_elem, face_id) of all faces on the given box sides."""
+    elems, ids = [], []
+    for f in sides:
+        e = boundary_faces(mesh, f)
+        elems.append(e)
+        ids.append(np.full(len(e), f, dtype=np.int32))
+    return np.concatenate(elems).astype(np.int32), np.concatenate(ids).astype(np.int32)
