@@ -655,6 +655,37 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
     MI_CALL(timing_end(ceed, "mass_diagonal", ev));
     return 0;
   }
+  if (p->kind == QfKind::BoundaryIntegralJacobian || p->kind == QfKind::FreestreamJacobian) {  // face sub-operators
+    const bool              freestream = p->kind == QfKind::FreestreamJacobian;
+    CeedMiNewtonianContext  nctx{};
+    CeedMiFreestreamContext fctx{};
+    if (freestream) {
+      CeedQFunctionContext c = op->qf->ctx;
+      if (!c || !c->data || c->size != sizeof(CeedMiFreestreamContext))
+        return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: missing or mismatched FreestreamContext", op->qf->name.c_str());
+      memcpy(&fctx, c->data, sizeof(fctx));
+    } else {
+      MI_CALL(newtonian_context(op, &nctx));
+    }
+    if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+    VolumeArgs a{};
+    a.nelem = p->nelem;
+    MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
+    a.comp_stride = p->r_q->comp_stride;
+    MI_CALL(vec_device_write(diag, &a.v, true));
+    MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+    a.qd         = p->qd;
+    a.jd         = p->jd;
+    const int PQ = p->P * p->Q;
+    a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+    if (ceed->jac_fp32) MI_CALL(vec_device_shadow32(p->v_jac, &a.jac32, false));
+    else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
+    hipEvent_t ev;
+    MI_CALL(timing_begin(ceed, "face_jacobian_diagonal", &ev));
+    MI_HIP(ceed, launch_face_jacobian_diagonal(p->P, p->Q, p->var, freestream, p->hllc, ceed->jac_fp32, a, nctx, fctx, ceed->stream));
+    MI_CALL(timing_end(ceed, "face_jacobian_diagonal", ev));
+    return 0;
+  }
   if (p->kind != QfKind::IJacobian)
     return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly of '%s' is not provided", op->qf->name.c_str());
   CeedMiNewtonianContext ctx;

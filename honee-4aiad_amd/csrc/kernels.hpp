@@ -67,6 +67,8 @@ hipError_t launch_face_boundary_integral(int P, int Q, int var, bool jacobian, b
                                          hipStream_t s);
 hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiFreestreamContext &ctx,
                                   hipStream_t s);
+hipError_t launch_face_jacobian_diagonal(int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
+                                         const CeedMiNewtonianContext &nctx, const CeedMiFreestreamContext &fctx, hipStream_t s);
 
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);
 hipError_t launch_axpby(double *y, double alpha, double beta, const double *x, long long n, hipStream_t s);

@@ -180,6 +180,88 @@ __global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face(VolumeArg
   }
 }
 
+// Diagonal of a face Jacobian (CeedOperatorLinearAssembleDiagonal for the face sub-operators).  At each point and for
+// each component c, the linear body is applied to unit inputs in slot s (value, d/dX1, d/dX2 of component c), giving
+// M_s = dv_c/d(slot s); the node diagonal is sum_q M_0 (B.B)(B.B) + M_1 (G.B)(B.B) + M_2 (B.B)(G.B) with the 1D
+// products of the face basis, contracted in two LDS passes.
+template <int P, int Q, class Op, class Ctx>
+__global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face_diagonal(VolumeArgs a, const Ctx ctx) {
+  constexpr int FPB = faces_per_block<Q>();
+  constexpr int Q2 = Q * Q, P2 = P * P, PQ = P * Q;
+  __shared__ double sBB[Q * P], sGB[Q * P];
+  __shared__ double sM[FPB][3][Q2], sT[FPB][3][PQ];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    sBB[i] = a.bt.BB[i];
+    sGB[i] = a.bt.GB[i];
+  }
+  const int  le = threadIdx.x / Q2, t = threadIdx.x % Q2;
+  const bool nd = t < P2;
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long f_  = bt * FPB + le;
+    const bool      ok  = f_ < a.nelem;
+    const int       f   = ok ? (int)f_ : 0;
+    const int       off = (ok && nd) ? a.offsets[(long long)f * P2 + t] : 0;
+    for (int c = 0; c < 5; c++) {
+      double M[3];
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        double u[5] = {0, 0, 0, 0, 0}, gref[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}}, v[5];
+        if (s == 0) u[c] = 1;
+        else gref[s - 1][c] = 1;
+        if (s > 0 && !Op::kGrad) {
+          M[s] = 0;
+          continue;
+        }
+        Op::template qf<Q>(a, ctx, f, t, ok, u, gref, v);
+        M[s] = ok ? v[c] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 3; s++) sM[le][s][t] = M[s];
+      __syncthreads();
+      if (t < PQ) {  // contract X2: T_s[j][qx] = sum_qy F2_s[qy][j] M_s[qy][qx]
+        const int qx = t % Q, j = t / Q;
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+          const double *F2  = (s == 2) ? sGB : sBB;
+          double        acc = 0;
+#pragma unroll
+          for (int y = 0; y < Q; y++) acc += F2[y * P + j] * sM[le][s][y * Q + qx];
+          sT[le][s][t] = acc;
+        }
+      }
+      __syncthreads();
+      if (ok && nd) {  // contract X1 and sum the slots: node (i, j) = t
+        const int i = t % P, j = t / P;
+        double    r = 0;
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+          const double *F1 = (s == 1) ? sGB : sBB;
+#pragma unroll
+          for (int x = 0; x < Q; x++) r += F1[x * P + i] * sT[le][s][j * Q + x];
+        }
+        atomicAdd(&a.v[off + c * a.comp_stride], r);
+      }
+    }
+  }
+}
+
+template <int P, int Q, class Op, class Ctx>
+hipError_t launch_face_diag(const VolumeArgs &a, const Ctx &ctx, hipStream_t stream) {
+  constexpr int   FPB    = faces_per_block<Q>();
+  const int       block  = FPB * Q * Q;
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+  int             dev = 0, ncu = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  long long grid = (long long)ncu * 8;
+  if (grid > nbatch) grid = nbatch;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_face_diagonal<P, Q, Op, Ctx>), dim3((unsigned)grid), dim3(block), 0, stream, a, ctx);
+  return hipGetLastError();
+}
+
 template <int P, int Q, class Op, class Ctx>
 hipError_t launch_face_op(const VolumeArgs &a, const Ctx &ctx, hipStream_t stream) {
   constexpr int   FPB    = faces_per_block<Q>();
@@ -243,6 +325,40 @@ hipError_t face_freestream_var(int var, bool fp32, const VolumeArgs &a, const Ce
   }
 }
 
+template <int P, int Q>
+hipError_t face_diag_var(bool freestream, bool hllc, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &nctx,
+                         const CeedMiFreestreamContext &fctx, hipStream_t s) {
+#define FD_VARS(OPT, CTX)                                                          \
+  switch (var) {                                                                   \
+    case 0: return launch_face_diag<P, Q, OPT(0), decltype(CTX)>(a, CTX, s);      \
+    case 1: return launch_face_diag<P, Q, OPT(1), decltype(CTX)>(a, CTX, s);      \
+    default: return launch_face_diag<P, Q, OPT(2), decltype(CTX)>(a, CTX, s);     \
+  }
+#define BI_D(V) fop::BoundaryIntegralJacobian<V, double>
+#define BI_F(V) fop::BoundaryIntegralJacobian<V, float>
+#define FS_HLL_D(V) fop::FreestreamJacobian<V, false, double>
+#define FS_HLL_F(V) fop::FreestreamJacobian<V, false, float>
+#define FS_HLLC_D(V) fop::FreestreamJacobian<V, true, double>
+#define FS_HLLC_F(V) fop::FreestreamJacobian<V, true, float>
+  if (!freestream) {
+    if (fp32) FD_VARS(BI_F, nctx)
+    FD_VARS(BI_D, nctx)
+  }
+  if (hllc) {
+    if (fp32) FD_VARS(FS_HLLC_F, fctx)
+    FD_VARS(FS_HLLC_D, fctx)
+  }
+  if (fp32) FD_VARS(FS_HLL_F, fctx)
+  FD_VARS(FS_HLL_D, fctx)
+#undef FD_VARS
+#undef BI_D
+#undef BI_F
+#undef FS_HLL_D
+#undef FS_HLL_F
+#undef FS_HLLC_D
+#undef FS_HLLC_F
+}
+
 }  // namespace
 
 #define HONEE_FACE_PQ(CALL)                   \
@@ -283,5 +399,13 @@ hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobia
         : (jacobian ? face_freestream_var<P_, Q_, false, true>(var, fp32, a, ctx, s) : face_freestream_var<P_, Q_, false, false>(var, fp32, a, ctx, s)))
   HONEE_FACE_PQ(CALL_FS)
 #undef CALL_FS
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_face_jacobian_diagonal(int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
+                                         const CeedMiNewtonianContext &nctx, const CeedMiFreestreamContext &fctx, hipStream_t s) {
+#define CALL_FD(P_, Q_) (face_diag_var<P_, Q_>(freestream, hllc, var, fp32, a, nctx, fctx, s))
+  HONEE_FACE_PQ(CALL_FD)
+#undef CALL_FD
   return hipErrorInvalidValue;
 }
