@@ -125,3 +125,32 @@ def test_closed_box_uniform_state_balances():
     scale = 71.0 * 2 * np.pi  # pressure x face length scale of the residual
     assert np.abs(g).max() < 1e-10 * scale, np.abs(g).max()
     prob.close()
+
+
+@pytest.mark.parametrize("kind,degree", [(0, 1), (1, 2), (2, 1)])
+def test_jacobian_diagonal_with_faces(kind, degree):
+    """Point-Jacobi diagonal of the IJacobian composite (volume + faces) against the oracle's unit columns."""
+    _say(f"diagonal with faces kind={kind} degree={degree}")
+    case = make_case(shape=(2, 2, 2), degree=degree, periodic=(False, True, True), perturb=0.1, state_var=1, stabilization=2, seed=150 + kind)
+    om, oc, qd = oracle_setup(case)
+    fe, fi, f = _faces_setup(case, [0, 1])
+    q, qdot = case["q"], case["qdot"]
+    _, jac_vol = O.ifunction(om, oc, qd, q, qdot)
+    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q)
+    shift = 120.0
+    oc.ijacobian_time_shift = shift
+    d_ref = O.ijacobian_diagonal(om, oc, qd, jac_vol)
+    n = q.size
+    for i in range(n):  # face part by unit columns
+        e = np.zeros(n)
+        e[i] = 1.0
+        d_ref[i] += _oracle_boundary_jac(om, oc, f, kind, qds, jac_bnd, fc, e)[i]
+    prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), 1)
+    prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    g = np.zeros_like(q)
+    prob.ifunction(q, qdot, g)
+    d = np.zeros_like(q)
+    prob.ijacobian_diagonal(d, shift)
+    e2, emax = rel_err(d, d_ref)
+    assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+    prob.close()
