@@ -10,6 +10,8 @@
 // (SetupBoundary, qfunctions/setupgeo.h:100-117).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "face_ops.hpp"
 #include "kernels.hpp"
 
@@ -77,6 +79,7 @@ __global__ void __launch_bounds__(256) k_setup_boundary(SetupArgs a) {
 // scratch 15..19.  Outputs (interpolation test weights) reuse slots 0..4.
 template <int P, int Q, class Op, class Ctx>
 __global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face(VolumeArgs a, const Ctx ctx) {
+  static_assert(!std::is_reference<Ctx>::value, "the context is passed to the kernel by value");
   constexpr int FPB = faces_per_block<Q>();
   constexpr int Q2 = Q * Q, P2 = P * P, PQ = P * Q;
   constexpr int NST = 20;
@@ -186,6 +189,7 @@ __global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face(VolumeArg
 // products of the face basis, contracted in two LDS passes.
 template <int P, int Q, class Op, class Ctx>
 __global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face_diagonal(VolumeArgs a, const Ctx ctx) {
+  static_assert(!std::is_reference<Ctx>::value, "the context is passed to the kernel by value");
   constexpr int FPB = faces_per_block<Q>();
   constexpr int Q2 = Q * Q, P2 = P * P, PQ = P * Q;
   __shared__ double sBB[Q * P], sGB[Q * P];
@@ -330,9 +334,9 @@ hipError_t face_diag_var(bool freestream, bool hllc, int var, bool fp32, const V
                          const CeedMiFreestreamContext &fctx, hipStream_t s) {
 #define FD_VARS(OPT, CTX)                                                          \
   switch (var) {                                                                   \
-    case 0: return launch_face_diag<P, Q, OPT(0), decltype(CTX)>(a, CTX, s);      \
-    case 1: return launch_face_diag<P, Q, OPT(1), decltype(CTX)>(a, CTX, s);      \
-    default: return launch_face_diag<P, Q, OPT(2), decltype(CTX)>(a, CTX, s);     \
+    case 0: return launch_face_diag<P, Q, OPT(0)>(a, CTX, s);  \
+    case 1: return launch_face_diag<P, Q, OPT(1)>(a, CTX, s);  \
+    default: return launch_face_diag<P, Q, OPT(2)>(a, CTX, s); \
   }
 #define BI_D(V) fop::BoundaryIntegralJacobian<V, double>
 #define BI_F(V) fop::BoundaryIntegralJacobian<V, float>
