@@ -21,6 +21,12 @@ static const char *const kIFunctionLoc[3] = {"qfunctions/newtonian.h:IFunction_N
                                              "qfunctions/newtonian.h:IFunction_Newtonian_Entropy"};
 static const char *const kIJacobianLoc[3] = {"qfunctions/newtonian.h:IJacobian_Newtonian_Conserv", "qfunctions/newtonian.h:IJacobian_Newtonian_Prim",
                                              "qfunctions/newtonian.h:IJacobian_Newtonian_Entropy"};
+static const char *const kRHSLoc          = "qfunctions/newtonian.h:RHSFunction_Newtonian";
+static const char *const kMassStabLoc     = "qfunctions/newtonian.h:MassFunction_Newtonian_Conserv";
+static const char *const kMass5Loc        = "qfunctions/mass.h:Mass_5";
+static const char *const kMonitorKELoc[3] = {"qfunctions/monitor_totalkineticenergy.h:MonitorTotalKineticEnergy_Conserv",
+                                             "qfunctions/monitor_totalkineticenergy.h:MonitorTotalKineticEnergy_Prim",
+                                             "qfunctions/monitor_totalkineticenergy.h:MonitorTotalKineticEnergy_Entropy"};
 static const char *const kSetupLoc        = "qfunctions/setupgeo.h:Setup";
 static const char *const kSetupBoundaryLoc = "qfunctions/setupgeo.h:SetupBoundary";
 static const char *const kVarName[3]       = {"Conserv", "Prim", "Entropy"};
@@ -42,6 +48,12 @@ struct HoneeMiProblem_ {
   CeedBasis             basis_q, basis_x;
   CeedVector            q_ceed, q_dot_ceed, g_ceed, x_coord, q_data, jac_data;
   CeedOperator          op_ifunction, op_ijacobian;
+  CeedOperator          op_rhs, op_mass, op_monitor;  // explicit problems: RHS composite and mass; KE monitor
+  CeedElemRestriction   elem_restr_ke;
+  CeedVector            ke_values;
+  CeedContextFieldLabel rhs_time_label, rhs_dt_label;
+  int                   mass_stabilized;
+  double                volume;
   CeedContextFieldLabel solution_time_label, timestep_size_label, ijacobian_shift_label;
   CeedQFunctionContext  newt_qfctx;
   CeedMiNewtonianContext *ctx;
@@ -196,6 +208,92 @@ static int Setup(HoneeMiProblem p, const char *resource, const HoneeMiMesh *m, c
   CHK(p, CeedOperatorGetContextFieldLabel(p->op_ijacobian, "ijacobian time shift", &p->ijacobian_shift_label));
   CHK(p, CeedOperatorDestroy(&op_ifunction_vol));
   CHK(p, CeedOperatorDestroy(&op_ijacobian_vol));
+
+  if (!ctx_in->is_implicit) {  // explicit problems: RHS composite (src/setuplibceed.c:176-203, 272-287) and the mass operator
+    CeedQFunction qf;
+    CeedOperator  op_rhs_vol;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kRHSLoc, &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+    CHK(p, CeedQFunctionSetUserFlopsEstimate(qf, 0));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op_rhs_vol));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "Grad_q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(op_rhs_vol, "Grad_v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+    CHK(p, CeedOperatorCreateComposite(p->ceed, &p->op_rhs));
+    CHK(p, CeedOperatorCompositeAddSub(p->op_rhs, op_rhs_vol));
+    CHK(p, CeedOperatorDestroy(&op_rhs_vol));
+    CHK(p, CeedOperatorGetContextFieldLabel(p->op_rhs, "solution time", &p->rhs_time_label));
+    CHK(p, CeedOperatorGetContextFieldLabel(p->op_rhs, "timestep size", &p->rhs_dt_label));
+
+    // Mass: SUPG-stabilized for explicit SUPG (problems/newtonian.c:795, 212-265), else Mass_5 (src/setuplibceed.c:18-50).
+    // The stabilized mass reads the current state through the passive field "q", bound to q_ceed as in the reference.
+    p->mass_stabilized = ctx_in->stabilization == 2;
+    if (p->mass_stabilized) {
+      CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kMassStabLoc, &qf));
+      CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+      CHK(p, CeedQFunctionSetUserFlopsEstimate(qf, 0));
+      CHK(p, CeedQFunctionAddInput(qf, "q_dot", 5, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+      CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 15, CEED_EVAL_GRAD));
+      CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_mass));
+      CHK(p, CeedOperatorSetName(p->op_mass, "RHS Mass Operator, Newtonian Stabilized"));
+      CHK(p, CeedOperatorSetField(p->op_mass, "q_dot", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(p->op_mass, "q", p->elem_restr_q, p->basis_q, p->q_ceed));
+      CHK(p, CeedOperatorSetField(p->op_mass, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+      CHK(p, CeedOperatorSetField(p->op_mass, "v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(p->op_mass, "Grad_v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    } else {
+      CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kMass5Loc, &qf));
+      CHK(p, CeedQFunctionAddInput(qf, "u", 5, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+      CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_INTERP));
+      CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_mass));
+      CHK(p, CeedOperatorSetName(p->op_mass, "RHS Mass Operator"));
+      CHK(p, CeedOperatorSetField(p->op_mass, "u", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(p->op_mass, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+      CHK(p, CeedOperatorSetField(p->op_mass, "v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    }
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+
+  {  // Total kinetic energy monitor (src/monitor_totalkineticenergy.c:66-110), values at quadrature points
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kMonitorKELoc[state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedQFunctionAddInput(qf, "q_data", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 5, CEED_EVAL_NONE));
+    CHK(p, CeedElemRestrictionCreateStrided(p->ceed, m->nelem, p->Q3, 5, (CeedSize)m->nelem * p->Q3 * 5, CEED_STRIDES_BACKEND, &p->elem_restr_ke));
+    CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_ke, &p->ke_values, NULL));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_monitor));
+    CHK(p, CeedOperatorSetField(p->op_monitor, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_monitor, "Grad_q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_monitor, "q_data", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(p->op_monitor, "v", p->elem_restr_ke, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  {  // domain volume: sum of w det J (HoneeCalculateDomainSize, src/misc.c)
+    const CeedScalar *a;
+    CHK(p, CeedVectorGetArrayRead(p->q_data, CEED_MEM_HOST, &a));
+    double vol = 0;
+    for (long long e = 0; e < p->nelem; e++)
+      for (long long i = 0; i < p->Q3; i++) vol += a[e * 10 * p->Q3 + i];
+    CHK(p, CeedVectorRestoreArrayRead(p->q_data, &a));
+    p->volume = vol;
+  }
   return 0;
 }
 
@@ -312,6 +410,7 @@ static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_n
     CHK(p, CeedOperatorSetField(op, "v", b->restr_q, b->basis_q, CEED_VECTOR_ACTIVE));
     if (implicit) CHK(p, CeedOperatorSetField(op, "surface jacobian data", b->restr_jd, CEED_BASIS_NONE, b->jac_data));
     CHK(p, CeedOperatorCompositeAddSub(p->op_ifunction, op));
+    if (p->op_rhs) CHK(p, CeedOperatorCompositeAddSub(p->op_rhs, op));  // explicit: BCDefinitionAddOperators(op_rhs)
     CHK(p, CeedOperatorDestroy(&op));
     CHK(p, CeedQFunctionDestroy(&qf));
   }
@@ -396,6 +495,69 @@ int HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, doubl
   return 0;
 }
 
+// RHS_NS's operator section (src/setupts.c:31-52): update time and dt, apply the RHS composite
+int HoneeMiApplyRHS(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, double *r) {
+  CeedScalar *arr;
+  if (!p->op_rhs) return CEED_ERROR_INCOMPATIBLE;
+  if (p->rhs_time_label) CHK(p, CeedOperatorSetContextDouble(p->op_rhs, p->rhs_time_label, &time));
+  if (p->rhs_dt_label) CHK(p, CeedOperatorSetContextDouble(p->op_rhs, p->rhs_dt_label, &dt));
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, r));
+  CHK(p, CeedOperatorApply(p->op_rhs, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+// Mass matrix action y = M(q) x of the explicit mass KSP (MatMult of the "mass_" KSP's MatCeed, src/setuplibceed.c:56-95);
+// q is the state the SUPG-stabilized mass is evaluated at (ignored by Mass_5).
+int HoneeMiApplyMass(HoneeMiProblem p, int mem_type, const double *q, const double *x, double *y) {
+  CeedScalar *arr;
+  if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)x));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, y));
+  CHK(p, CeedOperatorApply(p->op_mass, p->q_dot_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+// Diagonal of the mass matrix (PCJACOBI "diagonal" of the mass KSP)
+int HoneeMiMassDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *diag) {
+  CeedScalar *arr;
+  if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, diag));
+  CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_mass, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+int HoneeMiProblemIsExplicit(HoneeMiProblem p) { return p->op_rhs != NULL; }
+
+double HoneeMiProblemVolume(HoneeMiProblem p) { return p->volume; }
+
+// TSMonitor_TotalKineticEnergy (src/monitor_totalkineticenergy.c:113-140): sums over the quadrature-point values of
+// the monitor QFunction; sums[] are not divided by the domain volume.
+int HoneeMiMonitorKE(HoneeMiProblem p, int mem_type, const double *q, double sums[5]) {
+  CeedScalar       *arr;
+  const CeedScalar *a;
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedOperatorApply(p->op_monitor, p->q_ceed, p->ke_values, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorGetArrayRead(p->ke_values, CEED_MEM_HOST, &a));
+  const long long Q3 = p->Q3;
+  for (int c = 0; c < 5; c++) sums[c] = 0;
+  for (long long e = 0; e < p->nelem; e++)
+    for (int c = 0; c < 5; c++)
+      for (long long i = 0; i < Q3; i++) sums[c] += a[(e * 5 + c) * Q3 + i];
+  CHK(p, CeedVectorRestoreArrayRead(p->ke_values, &a));
+  return 0;
+}
+
 // Copy backend-strided quadrature data ([elem][comp][qpt], CEED_STRIDES_BACKEND of this backend) to host
 static int CopyStrided(HoneeMiProblem p, CeedVector v, CeedElemRestriction r, int ncomp, double *out) {
   const CeedScalar *a;
@@ -435,6 +597,11 @@ int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   if (!p) return 0;
   CeedOperatorDestroy(&p->op_ifunction);
   CeedOperatorDestroy(&p->op_ijacobian);
+  CeedOperatorDestroy(&p->op_rhs);
+  CeedOperatorDestroy(&p->op_mass);
+  CeedOperatorDestroy(&p->op_monitor);
+  CeedVectorDestroy(&p->ke_values);
+  CeedElemRestrictionDestroy(&p->elem_restr_ke);
   CeedVectorDestroy(&p->q_ceed);
   CeedVectorDestroy(&p->q_dot_ceed);
   CeedVectorDestroy(&p->g_ceed);

@@ -39,6 +39,16 @@ int          HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shi
 // IFunction and IJacobian composites.
 int          HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y);
 int          HoneeMiGetBoundaryData(HoneeMiProblem p, int boundary, int which, double *out);  // which 0: surface qdata, 1: jac data
+// Explicit problems (is_implicit false): RHS composite apply (RHS_NS, src/setupts.c:31-52), mass matrix action and
+// diagonal at state q (the "mass_" KSP operator, src/setuplibceed.c:56-95; SUPG-stabilized when stabilization is SUPG)
+int          HoneeMiApplyRHS(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, double *r);
+int          HoneeMiApplyMass(HoneeMiProblem p, int mem_type, const double *q, const double *x, double *y);
+int          HoneeMiMassDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *diag);
+int          HoneeMiProblemIsExplicit(HoneeMiProblem p);
+double       HoneeMiProblemVolume(HoneeMiProblem p);
+// Total kinetic energy monitor sums (TotalKineticEnergy, StrainDissipation, DivergenceDissipation, VolumeExpansion,
+// MuVorticity2), not divided by the volume (src/monitor_totalkineticenergy.c:113-140)
+int          HoneeMiMonitorKE(HoneeMiProblem p, int mem_type, const double *q, double sums[5]);
 int          HoneeMiGetQData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData32(HoneeMiProblem p, float *out);

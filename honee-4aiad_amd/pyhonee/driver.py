@@ -117,6 +117,13 @@ def load_libs():
         lib.HoneeMiIJacobianDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
         lib.HoneeMiProblemAddBoundary.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetBoundaryData.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
+        lib.HoneeMiApplyRHS.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiApplyMass.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiMassDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiProblemIsExplicit.argtypes = [ct.c_void_p]
+        lib.HoneeMiProblemVolume.argtypes = [ct.c_void_p]
+        lib.HoneeMiProblemVolume.restype = ct.c_double
+        lib.HoneeMiMonitorKE.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, _dp]
         lib.HoneeMiGetQData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData32.argtypes = [ct.c_void_p, ct.c_void_p]
@@ -180,6 +187,32 @@ class NewtonianProblem:
 
     def ijacobian_diagonal(self, diag, shift, mem=CEED_MEM_HOST):
         self._check(self.lib.HoneeMiIJacobianDiagonal(self.handle, mem, shift, _ptr(diag)), "IJacobian diagonal")
+
+    def rhs(self, q, r, time=0.0, dt=None, mem=CEED_MEM_HOST):
+        """Explicit RHS composite (volume RHSFunction_Newtonian + boundary faces); explicit problems only."""
+        dt = self.ctx.dt if dt is None else dt
+        self._check(self.lib.HoneeMiApplyRHS(self.handle, mem, time, dt, _ptr(q), _ptr(r)), "RHS")
+
+    def mass(self, q, x, y, mem=CEED_MEM_HOST):
+        """Mass matrix action y = M(q) x (SUPG-stabilized for SUPG problems, else Mass_5); explicit problems only."""
+        self._check(self.lib.HoneeMiApplyMass(self.handle, mem, _ptr(q), _ptr(x), _ptr(y)), "Mass")
+
+    def mass_diagonal(self, q, diag, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiMassDiagonal(self.handle, mem, _ptr(q), _ptr(diag)), "Mass diagonal")
+
+    @property
+    def explicit(self) -> bool:
+        return bool(self.lib.HoneeMiProblemIsExplicit(self.handle))
+
+    @property
+    def volume(self) -> float:
+        return float(self.lib.HoneeMiProblemVolume(self.handle))
+
+    def monitor_ke(self, q, mem=CEED_MEM_HOST) -> np.ndarray:
+        """Sums of the total kinetic energy monitor QFunction over the domain (not divided by the volume)."""
+        out = np.zeros(5)
+        self._check(self.lib.HoneeMiMonitorKE(self.handle, mem, _ptr(q), out.ctypes.data_as(_dp)), "MonitorKE")
+        return out
 
     BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
 

@@ -174,6 +174,12 @@ def mass_newtonian(m: Mesh, c: ONewt, qdata, q, qdot):
     return v
 
 
+def mass_newtonian_diagonal(m: Mesh, c: ONewt, qdata, q):
+    d = np.zeros(5 * m.nnodes)
+    lib().oracle_mass_newtonian_diagonal(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(d))
+    return d
+
+
 def monitor_ke(m: Mesh, c: ONewt, qdata, q):
     s = np.zeros(5)
     lib().oracle_monitor_ke(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(s))

@@ -11,12 +11,13 @@ from pyhonee.driver import context_from_dict
 
 
 def make_case(shape=(3, 2, 2), degree=3, periodic=(True, True, True), perturb=0.15, seed=1, state_var=1, stabilization=2,
-              idl=False, g=(0.0, 0.0, 0.0), noise=0.05, dt=1e-3, tau=None, lengths=(2 * np.pi, 2 * np.pi, 2 * np.pi)):
+              idl=False, g=(0.0, 0.0, 0.0), noise=0.05, dt=1e-3, tau=None, lengths=(2 * np.pi, 2 * np.pi, 2 * np.pi),
+              implicit=True):
     """A seeded, non-trivial Newtonian state on a (possibly non-affine) box mesh."""
     mesh = M.box_mesh(shape, degree, lengths=lengths, periodic=periodic, perturb=perturb, seed=seed)
     tab = M.tabulate(degree, 0)
     rng = np.random.default_rng(seed)
-    d = PR.tgv_context_dict(state_var=state_var, stabilization=stabilization, dt=dt, tau=tau or PR.default_tau(degree))
+    d = PR.tgv_context_dict(state_var=state_var, stabilization=stabilization, implicit=implicit, dt=dt, tau=tau or PR.default_tau(degree))
     d["g"] = g
     d["ijacobian_time_shift"] = 0.0
     if idl:
