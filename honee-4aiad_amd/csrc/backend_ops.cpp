@@ -630,15 +630,20 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
   }
 }
 
-// Point-Jacobi diagonal (MatGetDiagonal_Ceed -> CeedOperatorLinearAssembleDiagonal, src/mat-ceed.c:1603-1630)
-int op_plan_diagonal(CeedOperator op, CeedVector diag) {
+// Point-Jacobi diagonal (MatGetDiagonal_Ceed -> CeedOperatorLinearAssembleDiagonal, src/mat-ceed.c:1603-1630) or, with
+// point_block, the [node][c_out][c_in] point-block diagonal (PCVPBJACOBI's MatInvertVariableBlockDiagonal input,
+// CeedOperatorLinearAssemblePointBlockDiagonal) of an interleaved (comp_stride 1) 5-component operator.
+int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
   if (!op->plan) MI_CALL(plan_bind(op));
   OpPlan *p    = op->plan;
   Ceed    ceed = op->ceed;
+  const CeedSize want = point_block ? 5 * p->r_q->l_size : p->r_q->l_size;
+  if (point_block && p->r_q->comp_stride != 1)
+    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly needs an interleaved (comp_stride 1) restriction");
   if (p->kind == QfKind::MassNewtonian || p->kind == QfKind::Mass5) {  // mass matrices for the explicit path
     CeedMiNewtonianContext ctx{};
     if (p->kind == QfKind::MassNewtonian) MI_CALL(newtonian_context(op, &ctx));
-    if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+    if (diag->length != want) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
     VolumeArgs a{};
     a.nelem = p->nelem;
     MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
@@ -651,7 +656,8 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
     a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "mass_diagonal", &ev));
-    MI_HIP(ceed, launch_mass_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
+    if (point_block) MI_HIP(ceed, launch_mass_point_block_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
+    else MI_HIP(ceed, launch_mass_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
     MI_CALL(timing_end(ceed, "mass_diagonal", ev));
     return 0;
   }
@@ -667,7 +673,7 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
     } else {
       MI_CALL(newtonian_context(op, &nctx));
     }
-    if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+    if (diag->length != want) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
     VolumeArgs a{};
     a.nelem = p->nelem;
     MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
@@ -682,7 +688,7 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
     else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "face_jacobian_diagonal", &ev));
-    MI_HIP(ceed, launch_face_jacobian_diagonal(p->P, p->Q, p->var, freestream, p->hllc, ceed->jac_fp32, a, nctx, fctx, ceed->stream));
+    MI_HIP(ceed, launch_face_jacobian_diagonal(point_block, p->P, p->Q, p->var, freestream, p->hllc, ceed->jac_fp32, a, nctx, fctx, ceed->stream));
     MI_CALL(timing_end(ceed, "face_jacobian_diagonal", ev));
     return 0;
   }
@@ -690,7 +696,7 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
     return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly of '%s' is not provided", op->qf->name.c_str());
   CeedMiNewtonianContext ctx;
   MI_CALL(newtonian_context(op, &ctx));
-  if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector must have the L-vector size");
+  if (diag->length != want) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
   VolumeArgs a{};
   a.nelem = p->nelem;
   MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
@@ -705,7 +711,8 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag) {
   else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
   hipEvent_t ev;
   MI_CALL(timing_begin(ceed, "ijacobian_diagonal", &ev));
-  MI_HIP(ceed, launch_ijacobian_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
+  if (point_block) MI_HIP(ceed, launch_ijacobian_point_block_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
+  else MI_HIP(ceed, launch_ijacobian_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
   MI_CALL(timing_end(ceed, "ijacobian_diagonal", ev));
   return 0;
 }

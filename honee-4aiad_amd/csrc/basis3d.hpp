@@ -162,6 +162,54 @@ struct Elem {
 };
 
 
+// Five-component form of Elem::contract_t for point-block diagonal assembly: r[k] += sum_q Fz[qz][c] Fy[qy][b]
+// Fx[qx][a] w[k](q) for node t = (a, b, c), the five components contracted together in each pass.  s0, s1, s2 are the
+// element's [5][MAX3] scratch arrays.
+template <int P, int Q>
+struct Contract5 {
+  using D = Dims<P, Q>;
+  double (*s0)[D::MAX3];
+  double (*s1)[D::MAX3];
+  double (*s2)[D::MAX3];
+  int t;
+
+  __device__ void add(const double (&w)[5], const double *Fx, const double *Fy, const double *Fz, double (&r)[5]) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 5; k++) s0[k][t] = w[k];
+    __syncthreads();
+    if (t < D::PQQ) {
+      const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int z = 0; z < Q; z++) acc += Fz[z * P + c] * s0[k][(z * Q + iy) * Q + ix];
+        s1[k][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (t < D::PPQ) {
+      const int ix = t % Q, b = (t / Q) % P, c = t / (Q * P);
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int y = 0; y < Q; y++) acc += Fy[y * P + b] * s1[k][(c * Q + y) * Q + ix];
+        s2[k][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (t < D::P3) {
+      const int a = t % P, cb = t / P;
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int x = 0; x < Q; x++) r[k] += Fx[x * P + a] * s2[k][cb * Q + x];
+    }
+  }
+};
+
 // Multi-component basis stage: the element's LDS stage X[slot][Q^3] holds one slot per (field component, point);
 // every pass treats NC components together, so each 1D coefficient row is read from LDS once per pass and the NC
 // independent contractions overlap their LDS latency.  Barriers are block-wide; a pass reads other threads' slots only

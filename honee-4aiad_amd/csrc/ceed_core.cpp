@@ -1509,7 +1509,15 @@ extern "C" int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector
     for (CeedOperator s : op->subs) MI_CALL(CeedOperatorLinearAssembleAddDiagonal(s, assembled, request));
     return CEED_ERROR_SUCCESS;
   }
-  return op_plan_diagonal(op, assembled);
+  return op_plan_diagonal(op, assembled, false);
+}
+
+extern "C" int CeedOperatorLinearAssembleAddPointBlockDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  if (op->composite) {
+    for (CeedOperator s : op->subs) MI_CALL(CeedOperatorLinearAssembleAddPointBlockDiagonal(s, assembled, request));
+    return CEED_ERROR_SUCCESS;
+  }
+  return op_plan_diagonal(op, assembled, true);
 }
 
 extern "C" int CeedOperatorLinearAssembleDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
@@ -1520,10 +1528,31 @@ extern "C" int CeedOperatorLinearAssembleDiagonal(CeedOperator op, CeedVector as
 }
 
 extern "C" int CeedOperatorLinearAssemblePointBlockDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
-  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly is not provided by the MI355X backend");
+  double *d;
+  MI_CALL(vec_device_write(assembled, &d, false));
+  MI_HIP(op->ceed, launch_fill(d, 0.0, assembled->length, op->ceed->stream));
+  return CeedOperatorLinearAssembleAddPointBlockDiagonal(op, assembled, request);
 }
+// COO pattern of the point-block diagonal values (MatCeedAssemblePointBlockDiagonalCOO, src/mat-ceed.c:76-141): entry
+// 25 n + 5 c_out + c_in sits at row 5 n + c_out, column 5 n + c_in of the interleaved 5-component L-vector space.  The
+// arrays are allocated with malloc (the caller frees them with free).
 extern "C" int CeedOperatorLinearAssemblePointBlockDiagonalSymbolic(CeedOperator op, CeedSize *num_entries, CeedInt **rows, CeedInt **cols) {
-  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly is not provided by the MI355X backend");
+  CeedSize lin = 0, lout = 0;
+  MI_CALL(CeedOperatorGetActiveVectorLengths(op, &lin, &lout));
+  if (lout <= 0 || lout % 5 != 0 || lin != lout)
+    return MI_ERR(op->ceed, CEED_ERROR_DIMENSION, "point-block diagonal needs a square operator on a 5-component L-vector space");
+  const CeedSize nn = lout / 5;
+  *num_entries      = nn * 25;
+  *rows             = (CeedInt *)malloc(sizeof(CeedInt) * (size_t)*num_entries);
+  *cols             = (CeedInt *)malloc(sizeof(CeedInt) * (size_t)*num_entries);
+  if (!*rows || !*cols) return MI_ERR(op->ceed, CEED_ERROR_MAJOR, "out of host memory");
+  for (CeedSize n = 0; n < nn; n++)
+    for (int co = 0; co < 5; co++)
+      for (int ci = 0; ci < 5; ci++) {
+        (*rows)[n * 25 + co * 5 + ci] = (CeedInt)(n * 5 + co);
+        (*cols)[n * 25 + co * 5 + ci] = (CeedInt)(n * 5 + ci);
+      }
+  return CEED_ERROR_SUCCESS;
 }
 extern "C" int CeedOperatorLinearAssembleSymbolic(CeedOperator op, CeedSize *num_entries, CeedInt **rows, CeedInt **cols) {
   return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "full matrix assembly is not provided by the MI355X backend");

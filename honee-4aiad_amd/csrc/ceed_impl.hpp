@@ -184,7 +184,7 @@ int rstr_device_offsets(CeedElemRestriction r, const int **d);
 int basis_device_tables(CeedBasis b);
 int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite);
 bool op_plan_overwrites(CeedOperator op, CeedVector out);
-int op_plan_diagonal(CeedOperator op, CeedVector diag);
+int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block);
 void op_plan_destroy(OpPlan *plan);
 int timing_begin(Ceed ceed, const char *name, hipEvent_t *start);
 int timing_end(Ceed ceed, const char *name, hipEvent_t start);

@@ -50,7 +50,7 @@ struct HoneeMiProblem_ {
   CeedOperator          op_ifunction, op_ijacobian;
   CeedOperator          op_rhs, op_mass, op_monitor;  // explicit problems: RHS composite and mass; KE monitor
   CeedElemRestriction   elem_restr_ke;
-  CeedVector            ke_values;
+  CeedVector            ke_values, pb_ceed;
   CeedContextFieldLabel rhs_time_label, rhs_dt_label;
   int                   mass_stabilized;
   double                volume;
@@ -558,6 +558,31 @@ int HoneeMiMonitorKE(HoneeMiProblem p, int mem_type, const double *q, double sum
   return 0;
 }
 
+// Point-block diagonal [node][c_out][c_in] of the shifted IJacobian (MatCeedAssemblePointBlockDiagonalCOO's values,
+// src/mat-ceed.c:118-141), as PCVPBJACOBI uses it
+int HoneeMiIJacobianPointBlockDiagonal(HoneeMiProblem p, int mem_type, double shift, double *pb) {
+  CeedScalar *arr;
+  if (!p->pb_ceed) CHK(p, CeedVectorCreate(p->ceed, 25 * (CeedSize)p->nnodes, &p->pb_ceed));
+  if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
+  CHK(p, CeedVectorSetArray(p->pb_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, pb));
+  CHK(p, CeedOperatorLinearAssemblePointBlockDiagonal(p->op_ijacobian, p->pb_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->pb_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
+// Point-block diagonal of the explicit mass matrix at state q
+int HoneeMiMassPointBlockDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *pb) {
+  CeedScalar *arr;
+  if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
+  if (!p->pb_ceed) CHK(p, CeedVectorCreate(p->ceed, 25 * (CeedSize)p->nnodes, &p->pb_ceed));
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->pb_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, pb));
+  CHK(p, CeedOperatorLinearAssemblePointBlockDiagonal(p->op_mass, p->pb_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->pb_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
 // Copy backend-strided quadrature data ([elem][comp][qpt], CEED_STRIDES_BACKEND of this backend) to host
 static int CopyStrided(HoneeMiProblem p, CeedVector v, CeedElemRestriction r, int ncomp, double *out) {
   const CeedScalar *a;
@@ -601,6 +626,7 @@ int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   CeedOperatorDestroy(&p->op_mass);
   CeedOperatorDestroy(&p->op_monitor);
   CeedVectorDestroy(&p->ke_values);
+  CeedVectorDestroy(&p->pb_ceed);
   CeedElemRestrictionDestroy(&p->elem_restr_ke);
   CeedVectorDestroy(&p->q_ceed);
   CeedVectorDestroy(&p->q_dot_ceed);

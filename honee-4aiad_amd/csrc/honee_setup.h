@@ -32,6 +32,10 @@ CeedOperator HoneeMiProblemGetOperator(HoneeMiProblem p, int which);  // 0: IFun
 int          HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g);
 int          HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const double *dq, double *v);
 int          HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, double *diag);
+// Point-block diagonals [nnodes][5 (c_out)][5 (c_in)] (CeedOperatorLinearAssemblePointBlockDiagonal) of the shifted
+// IJacobian and of the explicit mass at state q
+int          HoneeMiIJacobianPointBlockDiagonal(HoneeMiProblem p, int mem_type, double shift, double *pb);
+int          HoneeMiMassPointBlockDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *pb);
 // Boundary faces of the problem (HONEE's BC definitions, problems/newtonian.c:555-569 and problems/bc_freestream.c:30-180):
 // kind 0: consistent boundary integral; 1: freestream with HLL flux; 2: freestream with HLLC flux (S_inf_Y is the
 // exterior primitive state).  face_nodes [nfaces][P^2] are the face nodes in face-local (X1 fastest) order with

@@ -54,6 +54,10 @@ bool       volume_pq_supported(int P, int Q);
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_ijacobian_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
+// Point-block diagonals [node][c_out][c_in] (output index 25 * (offset / 5) + 5 c_out + c_in; interleaved L-vectors)
+hipError_t launch_ijacobian_point_block_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx,
+                                                 hipStream_t stream);
+hipError_t launch_mass_point_block_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_rhs(int P, int Q, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_mass(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
 hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
@@ -67,7 +71,8 @@ hipError_t launch_face_boundary_integral(int P, int Q, int var, bool jacobian, b
                                          hipStream_t s);
 hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiFreestreamContext &ctx,
                                   hipStream_t s);
-hipError_t launch_face_jacobian_diagonal(int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
+// Diagonal (point_block false) or point-block diagonal [node][c_out][c_in] of a face Jacobian
+hipError_t launch_face_jacobian_diagonal(bool point_block, int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
                                          const CeedMiNewtonianContext &nctx, const CeedMiFreestreamContext &fctx, hipStream_t s);
 
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);

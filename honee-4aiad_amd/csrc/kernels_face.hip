@@ -251,6 +251,97 @@ __global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face_diagonal(
   }
 }
 
+// Point-block diagonal of a face Jacobian: as k_face_diagonal with the five output components of each unit-input action
+// contracted together; output [node][c_out][c_in] at 25 * (offset / 5).
+template <int P, int Q, class Op, class Ctx>
+__global__ void __launch_bounds__(faces_per_block<Q>() * Q * Q) k_face_pblock(VolumeArgs a, const Ctx ctx) {
+  static_assert(!std::is_reference<Ctx>::value, "the context is passed to the kernel by value");
+  constexpr int FPB = faces_per_block<Q>();
+  constexpr int Q2 = Q * Q, P2 = P * P, PQ = P * Q;
+  __shared__ double sBB[Q * P], sGB[Q * P];
+  __shared__ double sM[FPB][3][5][Q2], sT[FPB][3][5][PQ];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    sBB[i] = a.bt.BB[i];
+    sGB[i] = a.bt.GB[i];
+  }
+  const int       le = threadIdx.x / Q2, t = threadIdx.x % Q2;
+  const bool      nd = t < P2;
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long f_  = bt * FPB + le;
+    const bool      ok  = f_ < a.nelem;
+    const int       f   = ok ? (int)f_ : 0;
+    const int       off = (ok && nd) ? a.offsets[(long long)f * P2 + t] : 0;
+    for (int ci = 0; ci < 5; ci++) {
+      double M[3][5];
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        double u[5] = {0, 0, 0, 0, 0}, gref[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}}, v[5];
+        if (s == 0) u[ci] = 1;
+        else gref[s - 1][ci] = 1;
+        if (s > 0 && !Op::kGrad) {
+#pragma unroll
+          for (int co = 0; co < 5; co++) M[s][co] = 0;
+          continue;
+        }
+        Op::template qf<Q>(a, ctx, f, t, ok, u, gref, v);
+#pragma unroll
+        for (int co = 0; co < 5; co++) M[s][co] = ok ? v[co] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int co = 0; co < 5; co++) sM[le][s][co][t] = M[s][co];
+      __syncthreads();
+      if (t < PQ) {  // contract X2
+        const int qx = t % Q, j = t / Q;
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+          const double *F2 = (s == 2) ? sGB : sBB;
+#pragma unroll
+          for (int co = 0; co < 5; co++) {
+            double acc = 0;
+#pragma unroll
+            for (int y = 0; y < Q; y++) acc += F2[y * P + j] * sM[le][s][co][y * Q + qx];
+            sT[le][s][co][t] = acc;
+          }
+        }
+      }
+      __syncthreads();
+      if (ok && nd) {  // contract X1, sum the slots
+        const int i = t % P, j = t / P;
+#pragma unroll
+        for (int co = 0; co < 5; co++) {
+          double r = 0;
+#pragma unroll
+          for (int s = 0; s < 3; s++) {
+            const double *F1 = (s == 1) ? sGB : sBB;
+#pragma unroll
+            for (int x = 0; x < Q; x++) r += F1[x * P + i] * sT[le][s][co][j * Q + x];
+          }
+          atomicAdd(&a.v[(long long)off * 5 + co * 5 + ci], r);
+        }
+      }
+    }
+  }
+}
+
+template <int P, int Q, class Op, class Ctx>
+hipError_t launch_face_pblock(const VolumeArgs &a, const Ctx &ctx, hipStream_t stream) {
+  constexpr int   FPB    = faces_per_block<Q>();
+  const int       block  = FPB * Q * Q;
+  const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
+  int             dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  long long grid = (long long)ncu * 8;
+  if (grid > nbatch) grid = nbatch;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_face_pblock<P, Q, Op, Ctx>), dim3((unsigned)grid), dim3(block), 0, stream, a, ctx);
+  return hipGetLastError();
+}
+
 template <int P, int Q, class Op, class Ctx>
 hipError_t launch_face_diag(const VolumeArgs &a, const Ctx &ctx, hipStream_t stream) {
   constexpr int   FPB    = faces_per_block<Q>();
@@ -330,13 +421,13 @@ hipError_t face_freestream_var(int var, bool fp32, const VolumeArgs &a, const Ce
 }
 
 template <int P, int Q>
-hipError_t face_diag_var(bool freestream, bool hllc, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &nctx,
+hipError_t face_diag_var(bool pblock, bool freestream, bool hllc, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &nctx,
                          const CeedMiFreestreamContext &fctx, hipStream_t s) {
-#define FD_VARS(OPT, CTX)                                                          \
-  switch (var) {                                                                   \
-    case 0: return launch_face_diag<P, Q, OPT(0)>(a, CTX, s);  \
-    case 1: return launch_face_diag<P, Q, OPT(1)>(a, CTX, s);  \
-    default: return launch_face_diag<P, Q, OPT(2)>(a, CTX, s); \
+#define FD_VARS(OPT, CTX)                                                                                              \
+  switch (var) {                                                                                                       \
+    case 0: return pblock ? launch_face_pblock<P, Q, OPT(0)>(a, CTX, s) : launch_face_diag<P, Q, OPT(0)>(a, CTX, s);  \
+    case 1: return pblock ? launch_face_pblock<P, Q, OPT(1)>(a, CTX, s) : launch_face_diag<P, Q, OPT(1)>(a, CTX, s);  \
+    default: return pblock ? launch_face_pblock<P, Q, OPT(2)>(a, CTX, s) : launch_face_diag<P, Q, OPT(2)>(a, CTX, s); \
   }
 #define BI_D(V) fop::BoundaryIntegralJacobian<V, double>
 #define BI_F(V) fop::BoundaryIntegralJacobian<V, float>
@@ -406,9 +497,9 @@ hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobia
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_face_jacobian_diagonal(int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
+hipError_t launch_face_jacobian_diagonal(bool point_block, int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
                                          const CeedMiNewtonianContext &nctx, const CeedMiFreestreamContext &fctx, hipStream_t s) {
-#define CALL_FD(P_, Q_) (face_diag_var<P_, Q_>(freestream, hllc, var, fp32, a, nctx, fctx, s))
+#define CALL_FD(P_, Q_) (face_diag_var<P_, Q_>(point_block, freestream, hllc, var, fp32, a, nctx, fctx, s))
   HONEE_FACE_PQ(CALL_FD)
 #undef CALL_FD
   return hipErrorInvalidValue;

@@ -341,6 +341,89 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
   }
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Point-block diagonal of a linear operator whose quadrature-point action is Op::action: for each input component
+// c_in and input slot s (value, three reference derivatives), one action gives the test-slot weights of all five
+// output components; the weights of each unordered slot pair are summed (the diagonal's 1D products are symmetric in
+// the pair) and contracted five components at a time.  Output layout [node][c_out][c_in] at 25 * (offset / 5).
+// ------------------------------------------------------------------------------------------------------------------
+template <int P, int Q, class Op, typename T>
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+  using Dm          = Dims<P, Q>;
+  constexpr int EPB = elems_per_block<Q>();
+  constexpr int Q3  = Dm::Q3;
+  constexpr int NS  = Op::kInSlots;
+  __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
+  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
+  __shared__ double c0[EPB][5][Dm::MAX3], c1[EPB][5][Dm::MAX3], c2[EPB][5][Dm::MAX3];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    sBB[i] = a.bt.BB[i];
+    sGB[i] = a.bt.GB[i];
+    sGG[i] = a.bt.GG[i];
+  }
+  __shared__ double sB[Q * P], sDc[Q * Q];
+  load_tables<P, Q>(a.bt, sB, sDc);
+  const int           le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  Elem<P, Q>          el(sB, sDc, s0[le], s1[le], s2[le], t);
+  basis3::Contract5<P, Q> pb{c0[le], c1[le], c2[le], t};
+  const long long     nbatch = ((long long)a.nelem + EPB - 1) / EPB;
+  // slot pairs (s, b), s <= b, and their 1D factors (x, y, z) as in k_diagonal
+  constexpr int kPair[4][4] = {{0, 4, 5, 6}, {4, 1, 7, 8}, {5, 7, 2, 9}, {6, 8, 9, 3}};
+
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long e_  = bt * EPB + le;
+    const bool      ok  = e_ < a.nelem;
+    const int       e   = ok ? (int)e_ : 0;
+    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
+    double          u2[5];
+    if constexpr (Op::kIn2) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
+    }
+    typename Op::Point p;
+    double             qd[10];
+    nsop::fetch_qdata<Q>(a, e, t, ok, qd);
+    Op::template load_point<Q>(a, ctx, e, t, ok, qd, u2, p);
+    for (int ci = 0; ci < 5; ci++) {
+      double W[10][5];
+#pragma unroll
+      for (int k = 0; k < 10; k++)
+#pragma unroll
+        for (int co = 0; co < 5; co++) W[k][co] = 0;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        T dq[5] = {0, 0, 0, 0, 0}, gref[3][5] = {{0}};
+        if (s == 0) dq[ci] = 1;
+        else gref[s - 1][ci] = 1;
+        double v[5], gw[3][5];
+        Op::action(ctx, p, dq, gref, v, gw);
+#pragma unroll
+        for (int co = 0; co < 5; co++) {
+          W[kPair[s][0]][co] += ok ? v[co] : 0.0;
+#pragma unroll
+          for (int d = 0; d < 3; d++) W[kPair[s][1 + d]][co] += ok ? gw[d][co] : 0.0;
+        }
+      }
+      double r[5] = {0, 0, 0, 0, 0};
+      pb.add(W[0], sBB, sBB, sBB, r);
+      pb.add(W[4], sGB, sBB, sBB, r);
+      pb.add(W[5], sBB, sGB, sBB, r);
+      pb.add(W[6], sBB, sBB, sGB, r);
+      if constexpr (NS > 1) {
+        pb.add(W[1], sGG, sBB, sBB, r);
+        pb.add(W[2], sBB, sGG, sBB, r);
+        pb.add(W[3], sBB, sBB, sGG, r);
+        pb.add(W[7], sGB, sGB, sBB, r);
+        pb.add(W[8], sGB, sBB, sGB, r);
+        pb.add(W[9], sBB, sGB, sGB, r);
+      }
+      if (ok && t < Dm::P3)
+#pragma unroll
+        for (int co = 0; co < 5; co++) atomicAdd(&a.v[(long long)off * 5 + co * 5 + ci], r[co]);
+    }
+  }
+}
+
 template <typename K>
 int grid_for(K kernel, int block, long long nbatch) {
   int dev = 0, ncu = 0, per_cu = 0;
@@ -372,6 +455,32 @@ hipError_t launch_diag(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, h
   const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
   return hipGetLastError();
+}
+
+template <int P, int Q, class Op, typename T>
+hipError_t launch_pblock(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
+  constexpr int EPB   = elems_per_block<Q>();
+  const int     block = EPB * Q * Q * Q;
+  auto          k     = k_pblock_diagonal<P, Q, Op, T>;
+  const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
+  return hipGetLastError();
+}
+
+template <int P, int Q>
+hipError_t launch_ijacobian_pblock_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+  if (fp32) {
+    switch (var) {
+      case 0: return launch_pblock<P, Q, nsop::IJacobian<0, float>, float>(a, ctx, s);
+      case 1: return launch_pblock<P, Q, nsop::IJacobian<1, float>, float>(a, ctx, s);
+      default: return launch_pblock<P, Q, nsop::IJacobian<2, float>, float>(a, ctx, s);
+    }
+  }
+  switch (var) {
+    case 0: return launch_pblock<P, Q, nsop::IJacobian<0, double>, double>(a, ctx, s);
+    case 1: return launch_pblock<P, Q, nsop::IJacobian<1, double>, double>(a, ctx, s);
+    default: return launch_pblock<P, Q, nsop::IJacobian<2, double>, double>(a, ctx, s);
+  }
 }
 
 template <int P, int Q>
@@ -484,6 +593,22 @@ hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs 
   (stabilized ? launch_diag<P_, Q_, nsop::MassNewtonian, double>(a, ctx, s) : launch_diag<P_, Q_, nsop::Mass5, double>(a, ctx, s))
   HONEE_PQ_ALL(CALL_MD)
 #undef CALL_MD
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ijacobian_point_block_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx,
+                                                 hipStream_t s) {
+#define CALL_PB(P_, Q_) (launch_ijacobian_pblock_var<P_, Q_>(var, fp32, a, ctx, s))
+  HONEE_PQ_ALL(CALL_PB)
+#undef CALL_PB
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mass_point_block_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+#define CALL_MPB(P_, Q_) \
+  (stabilized ? launch_pblock<P_, Q_, nsop::MassNewtonian, double>(a, ctx, s) : launch_pblock<P_, Q_, nsop::Mass5, double>(a, ctx, s))
+  HONEE_PQ_ALL(CALL_MPB)
+#undef CALL_MPB
   return hipErrorInvalidValue;
 }
 

@@ -117,6 +117,8 @@ def load_libs():
         lib.HoneeMiIJacobianDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
         lib.HoneeMiProblemAddBoundary.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetBoundaryData.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
+        lib.HoneeMiIJacobianPointBlockDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
+        lib.HoneeMiMassPointBlockDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiApplyRHS.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiApplyMass.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiMassDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
@@ -213,6 +215,13 @@ class NewtonianProblem:
         out = np.zeros(5)
         self._check(self.lib.HoneeMiMonitorKE(self.handle, mem, _ptr(q), out.ctypes.data_as(_dp)), "MonitorKE")
         return out
+
+    def ijacobian_point_block_diagonal(self, pb, shift, mem=CEED_MEM_HOST):
+        """Point-block diagonal [nnodes][5 (out)][5 (in)] of the shifted IJacobian composite."""
+        self._check(self.lib.HoneeMiIJacobianPointBlockDiagonal(self.handle, mem, shift, _ptr(pb)), "IJacobian point-block diagonal")
+
+    def mass_point_block_diagonal(self, q, pb, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiMassPointBlockDiagonal(self.handle, mem, _ptr(q), _ptr(pb)), "Mass point-block diagonal")
 
     BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
 

@@ -369,6 +369,32 @@ int oracle_ijacobian_diagonal(const OMesh *m, const ONewt *c, const double *qdat
   return 0;
 }
 
+// Point-block diagonal (CeedOperatorLinearAssemblePointBlockDiagonal semantics): pb[node][c_out][c_in] is the sum over
+// elements of the element matrix entries coupling (node, c_in) to (node, c_out), by local unit vectors.
+int oracle_ijacobian_point_block_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, double *pb) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q), nj = njac(c);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 25 * P3);  // [e][node][c_out][c_in]
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], re[5 * 512];
+    for (int n = 0; n < P3; n++)
+      for (int ci = 0; ci < 5; ci++) {
+        memset(ue, 0, sizeof(double) * 5 * P3);
+        ue[ci * P3 + n] = 1.0;
+        elem_ijacobian(m, c, &qdata[(size_t)e * 10 * Q3], &jac[(size_t)e * nj * Q3], ue, re);
+        for (int co = 0; co < 5; co++) E[((size_t)e * P3 + n) * 25 + co * 5 + ci] = re[co * P3 + n];
+      }
+  }
+  memset(pb, 0, sizeof(double) * 25 * m->nnodes);
+  for (int e = 0; e < m->nelem; e++)
+    for (int n = 0; n < P3; n++) {
+      const int node = m->offsets[e * P3 + n];
+      for (int k = 0; k < 25; k++) pb[node * 25 + k] += E[((size_t)e * P3 + n) * 25 + k];
+    }
+  free(E);
+  return 0;
+}
+
 // ----------------------------------------------------------------------------------------------------------------------
 // RHSFunction_Newtonian (newtonian.h:133-202), conservative variables
 // ----------------------------------------------------------------------------------------------------------------------

@@ -60,12 +60,46 @@ int    oracle_setup_qdata(const OMesh *m, double *qdata);
 int    oracle_ifunction(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v, double *jac);
 int    oracle_ijacobian(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, const double *dq, double *v);
 int    oracle_ijacobian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, double *diag);
+int    oracle_ijacobian_point_block_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *jac, double *pb);
 int    oracle_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *v);
 int    oracle_mass_newtonian(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *qdot, double *v);
 int    oracle_mass_newtonian_diagonal(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *diag);
 int    oracle_monitor_ke(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double sums[5]);
 void   oracle_ic_taylorgreen(const ONewt *c, const double ref_Y[5], const double bg_u[3], const double L[3], int n, const double *xyz, double *q);
 double oracle_volume(const OMesh *m, const double *qdata);
+
+// Time integration (honee_oracle_solve.c)
+typedef struct {
+  int    gmres;          // 0: KSPPREONLY, 1: GMRES
+  int    jacobi_rowsum;  // PCJACOBI type: 1 row sum, 0 diagonal
+  double rtol;
+  int    restart, max_it;
+} ORKMass;
+
+typedef struct {
+  int    order;  // 1 or 2
+  double dt;
+  double snes_rtol, snes_atol, snes_stol;
+  int    snes_max_it;
+  double ksp_rtol, ksp_atol;
+  int    ksp_restart, ksp_max_it;
+  int    pc_pbjacobi;  // 0 point-Jacobi, 1 point-block Jacobi
+} OBDFOptions;
+
+typedef struct {
+  int       steps;
+  double    time;
+  long long ifunction_evals, newton_its, krylov_its, line_search_backtracks;
+  int       last_newton_its, last_krylov_its;
+  double    last_fnorm0, last_fnorm;
+} OBDFStats;
+
+// Explicit Runge-Kutta (Butcher tableau A[s][s], b[s], c[s]) of the RHS with the mass KSP; optional kinetic-energy
+// rows (step, time, 5 monitor sums / volume) every record_every steps
+int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
+              const double *cs, const ORKMass *mo, int record_every, double *ke_rows, int *nrows, long long *mass_its);
+// BDF(order) with Newton-GMRES and point-Jacobi; returns nonzero when a nonlinear solve fails
+int oracle_bdf(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, int nsteps, const OBDFOptions *o, OBDFStats *st);
 
 // Explicit time stepping of the RHS with the SUPG-stabilized consistent mass (problems/newtonian.c:212-265) solved by
 // left-Jacobi-preconditioned GMRES; used to reproduce the reference's Taylor-Green kinetic-energy fixture.

@@ -1,27 +1,95 @@
 // CPU ORACLE -- test infrastructure only (see ns_pointwise.h).
 //
-// Explicit time stepping used to reproduce the reference's Taylor-Green kinetic-energy fixture
-// (examples/navierstokes.c:222, examples/taylor_green_vortex.yaml):
-//   * PETSc TSRK type "4" (classic 4-stage RK) over RHS_NS (src/setupts.c:31-75),
-//   * each stage solves M(q) G = R with the SUPG-stabilized consistent mass (problems/newtonian.c:212-265),
-//   * the mass KSP is GMRES(restart) with left point-Jacobi preconditioning, zero initial guess, classical Gram-Schmidt
-//     and PETSc's default convergence test ||B r|| <= rtol ||B b|| (the "mass_" KSP set up in src/setuplibceed.c:56-95).
-//     As in PETSc, the Jacobi diagonal is computed when the preconditioner is first set up and then reused, because the
-//     shell matrix's object state never changes between solves.
+// Time integration as HONEE configures PETSc's TS/SNES/KSP (TSSolve_NS, src/setupts.c:334-360; the "mass_" KSP,
+// src/setuplibceed.c:56-95).  PETSc is a third-party dependency absent from the reference tree (v3.24.5 per
+// tests/output/taylorgreen_totalke.csv:5); its published algorithms are restated here:
+//   * TSRK with a Butcher tableau: Y_s = q + h sum_{j<s} a_sj K_j, K_s = M(Y_s)^{-1} R(t + c_s h, Y_s) (RHS_NS,
+//     src/setupts.c:31-75), q += h sum_s b_s K_s.  The mass solve is PCJACOBI (row sum = M 1, or the diagonal) alone
+//     (KSPPREONLY, HONEE's lumped default) or inside GMRES.  As in PETSc, the Jacobi vector is computed when the
+//     preconditioner is first set up and then reused, because the shell matrix's object state never changes between
+//     solves.  The SUPG-stabilized mass (problems/newtonian.c:212-265) depends on the stage state Y_s.
+//   * KSPGMRES: left preconditioning, zero initial guess, classical Gram-Schmidt without refinement
+//     (KSPGMRESCycle/KSPGMRESUpdateHessenberg/KSPGMRESBuildSoln, src/ksp/ksp/impls/gmres/gmres.c), PETSc's default
+//     convergence test ||B r|| <= max(rtol ||B b||, atol).
+//   * TSBDF(k): q_dot = sum_i alpha_i X_i with alpha_i the derivative at t_{n+1} of the Lagrange basis on the nodes
+//     (t_{n+1}, history); the Jacobian shift is alpha_0.  An order-2 start takes a BDF1 half step, then a
+//     variable-step BDF2 step to t_0 + h (TSBDF_Restart).  The Newton initial guess extrapolates the history
+//     (Lagrange through up to k + 1 points).  The "timestep size" context field stays the nominal h.
+//   * PCJACOBI / PCVPBJACOBI: the point diagonal (zero entries replaced by 1) or the 5x5 node blocks inverted by
+//     Gauss-Jordan elimination with partial pivoting (a singular block is an error).
+//   * SNESNEWTONLS with backtracking on f = ||F||^2/2 along the Newton direction (slope -||F||^2, sufficient-decrease
+//     constant 1e-4, step cut by the quadratic model clamped to [0.1, 0.5] of the previous step); the IJacobian
+//     (src/setupts.c:121-156) uses the Jacobian data of the last IFunction evaluation and the point-Jacobi diagonal is
+//     reassembled at every Newton iteration (MatGetDiagonal_Ceed, src/mat-ceed.c:1603-1630).  Convergence:
+//     ||F|| <= atol, ||F|| <= max(atol, rtol ||F_0||) or ||step|| <= stol ||x||.
+// Parity unpinned against PETSc itself (no PETSc here): these restatements are pinned by the reference's Taylor-Green
+// kinetic-energy fixture (RK4 + GMRES mass), by the order conditions and convergence rates of the schemes, and by
+// closed-form linear test problems in the tests.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "honee_oracle.h"
 
-typedef struct {
-  const OMesh  *m;
-  const ONewt  *c;
-  const double *qdata;
-  const double *q;  // state the stabilized mass depends on
-} MassCtx;
+typedef void (*ApplyFn)(void *ctx, const double *x, double *y);
 
-static void mass_apply(const MassCtx *mc, const double *x, double *y) { oracle_mass_newtonian(mc->m, mc->c, mc->qdata, mc->q, x, y); }
+// Preconditioner z = B (x - y) (y may be NULL): B is the inverted point diagonal (nb == 1) or the inverted 5x5 node
+// blocks [c_out][c_in] (nb == 5)
+typedef struct {
+  const double *d;
+  int           nb;
+} Prec;
+
+static void prec_apply(const Prec *pc, int n, const double *x, const double *y, double *z) {
+  if (pc->nb == 1) {
+    for (int i = 0; i < n; i++) z[i] = pc->d[i] * (y ? x[i] - y[i] : x[i]);
+    return;
+  }
+  for (int node = 0; node < n / 5; node++) {
+    double r[5];
+    for (int j = 0; j < 5; j++) r[j] = y ? x[node * 5 + j] - y[node * 5 + j] : x[node * 5 + j];
+    for (int i = 0; i < 5; i++) {
+      double acc = 0;
+      for (int j = 0; j < 5; j++) acc += pc->d[node * 25 + i * 5 + j] * r[j];
+      z[node * 5 + i] = acc;
+    }
+  }
+}
+
+// In-place inverse of each 5x5 block by Gauss-Jordan elimination with partial pivoting; returns the number of
+// singular blocks
+static int pb_invert(int nnodes, double *pb) {
+  int bad = 0;
+  for (int node = 0; node < nnodes; node++) {
+    double A[5][5], X[5][5];
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) A[i][j] = pb[node * 25 + i * 5 + j], X[i][j] = i == j;
+    for (int k = 0; k < 5; k++) {
+      int piv = k;
+      for (int i = k + 1; i < 5; i++)
+        if (fabs(A[i][k]) > fabs(A[piv][k])) piv = i;
+      for (int j = 0; j < 5; j++) {
+        double ta = A[k][j], tx = X[k][j];
+        A[k][j] = A[piv][j], X[k][j] = X[piv][j];
+        A[piv][j] = ta, X[piv][j] = tx;
+      }
+      if (A[k][k] == 0) {
+        bad++;
+        continue;
+      }
+      const double inv = 1.0 / A[k][k];
+      for (int j = 0; j < 5; j++) A[k][j] *= inv, X[k][j] *= inv;
+      for (int i = 0; i < 5; i++) {
+        if (i == k) continue;
+        const double f = A[i][k];
+        for (int j = 0; j < 5; j++) A[i][j] -= f * A[k][j], X[i][j] -= f * X[k][j];
+      }
+    }
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) pb[node * 25 + i * 5 + j] = X[i][j];
+  }
+  return bad;
+}
 
 static double vdot(int n, const double *a, const double *b) {
   double s = 0;
@@ -29,9 +97,9 @@ static double vdot(int n, const double *a, const double *b) {
   return s;
 }
 
-// Left-preconditioned GMRES(restart) with zero initial guess, mirroring KSPGMRESCycle/KSPGMRESUpdateHessenberg/
-// KSPGMRESBuildSoln (PETSc src/ksp/ksp/impls/gmres/gmres.c) with classical Gram-Schmidt and no refinement.
-static int gmres(int n, const MassCtx *mc, const double *dinv, const double *b, double *x, double rtol, int restart, int maxit) {
+// Left-preconditioned GMRES(restart) with zero initial guess; returns the iteration count
+static int gmres(int n, ApplyFn apply, void *actx, const Prec *pc, const double *b, double *x, double rtol, double atol, int restart,
+                 int maxit) {
   const int K   = restart;
   double   *V   = malloc(sizeof(double) * (size_t)n * (K + 1));
   double   *w   = malloc(sizeof(double) * n);
@@ -45,18 +113,18 @@ static int gmres(int n, const MassCtx *mc, const double *dinv, const double *b, 
   memset(x, 0, sizeof(double) * n);
   while (!converged && its < maxit) {
     // r = B (b - A x)
-    mass_apply(mc, x, Ax);
-    for (int i = 0; i < n; i++) V[i] = dinv[i] * (b[i] - Ax[i]);
+    apply(actx, x, Ax);
+    prec_apply(pc, n, b, Ax, V);
     double res = sqrt(vdot(n, V, V));
-    if (its == 0) ttol = fmax(rtol * res, 1e-50);
+    if (its == 0) ttol = fmax(rtol * res, atol);
     if (res <= ttol) break;
     for (int i = 0; i < n; i++) V[i] /= res;
     grs[0] = res;
     int it = 0;
     while (it < K && its < maxit) {
       double *v_new = &V[(size_t)(it + 1) * n];
-      mass_apply(mc, &V[(size_t)it * n], w);
-      for (int i = 0; i < n; i++) v_new[i] = dinv[i] * w[i];
+      apply(actx, &V[(size_t)it * n], w);
+      prec_apply(pc, n, w, NULL, v_new);
       // classical Gram-Schmidt: all dot products first, then subtract
       for (int j = 0; j <= it; j++) HH(j, it) = vdot(n, v_new, &V[(size_t)j * n]);
       for (int j = 0; j <= it; j++)
@@ -66,13 +134,13 @@ static int gmres(int n, const MassCtx *mc, const double *dinv, const double *b, 
         for (int i = 0; i < n; i++) v_new[i] /= HH(it + 1, it);
       // apply previous rotations to column it, then form the new one
       for (int j = 0; j < it; j++) {
-        double tt    = HH(j, it);
+        double tt     = HH(j, it);
         HH(j, it)     = cc[j] * tt + ss[j] * HH(j + 1, it);
         HH(j + 1, it) = cc[j] * HH(j + 1, it) - ss[j] * tt;
       }
-      double tt = sqrt(HH(it, it) * HH(it, it) + HH(it + 1, it) * HH(it + 1, it));
-      cc[it]    = HH(it, it) / tt;
-      ss[it]    = HH(it + 1, it) / tt;
+      double tt   = sqrt(HH(it, it) * HH(it, it) + HH(it + 1, it) * HH(it + 1, it));
+      cc[it]      = HH(it, it) / tt;
+      ss[it]      = HH(it + 1, it) / tt;
       grs[it + 1] = -(ss[it] * grs[it]);
       grs[it]     = cc[it] * grs[it];
       HH(it, it)  = cc[it] * HH(it, it) + ss[it] * HH(it + 1, it);
@@ -107,6 +175,23 @@ static int gmres(int n, const MassCtx *mc, const double *dinv, const double *b, 
   return its;
 }
 
+static void jacobi_invert(int n, double *d) {
+  for (int i = 0; i < n; i++) d[i] = d[i] != 0 ? 1.0 / d[i] : 1.0;
+}
+
+// ---- explicit ----
+typedef struct {
+  const OMesh  *m;
+  const ONewt  *c;
+  const double *qdata;
+  const double *q;  // state the stabilized mass depends on
+} MassCtx;
+
+static void mass_apply(void *ctx, const double *x, double *y) {
+  const MassCtx *mc = ctx;
+  oracle_mass_newtonian(mc->m, mc->c, mc->qdata, mc->q, x, y);
+}
+
 static void record_row(const OMesh *m, const ONewt *c, const double *qdata, const double *q, int step, double t, double vol, double *row) {
   double sums[5];
   oracle_monitor_ke(m, c, qdata, q, sums);
@@ -115,49 +200,249 @@ static void record_row(const OMesh *m, const ONewt *c, const double *qdata, cons
   for (int k = 0; k < 5; k++) row[2 + k] = sums[k] / vol;
 }
 
-int oracle_rk4_stabilized(const OMesh *m, ONewt *c, const double *qdata, double *q, double dt, int nsteps, double ksp_rtol, int ksp_restart,
-                          int record_every, double *ke_rows, int *nrows) {
+int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
+              const double *cs, const ORKMass *mo, int record_every, double *ke_rows, int *nrows, long long *mass_its) {
   const int    n   = 5 * m->nnodes;
   const double vol = oracle_volume(m, qdata);
-  // TSRK4 tableau
-  const double A[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1, 0}};
-  const double bw[4]   = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
-  const double cst[4]  = {0, 0.5, 0.5, 1};
-  double      *K[4], *Y = malloc(sizeof(double) * n), *R = malloc(sizeof(double) * n), *dinv = NULL;
-  for (int i = 0; i < 4; i++) K[i] = malloc(sizeof(double) * n);
-  double t     = 0;
-  int    rows  = 0;
-  int    total = 0;
-  c->dt        = dt;
-  record_row(m, c, qdata, q, 0, t, vol, &ke_rows[7 * rows++]);
+  double     **K = malloc(sizeof(double *) * s), *Y = malloc(sizeof(double) * n), *R = malloc(sizeof(double) * n), *dinv = NULL;
+  for (int i = 0; i < s; i++) K[i] = malloc(sizeof(double) * n);
+  double    t     = t0;
+  int       rows  = 0;
+  long long total = 0;
+  c->dt           = dt;
+  if (ke_rows) record_row(m, c, qdata, q, 0, t, vol, &ke_rows[7 * rows++]);
   for (int step = 1; step <= nsteps; step++) {
-    for (int s = 0; s < 4; s++) {
+    for (int i = 0; i < s; i++) {
       memcpy(Y, q, sizeof(double) * n);
-      for (int j = 0; j < s; j++) {
-        const double wj = dt * A[s][j];
-        for (int i = 0; i < n; i++) Y[i] += wj * K[j][i];
+      for (int j = 0; j < i; j++) {
+        const double wj = dt * A[i * s + j];
+        if (wj != 0)
+          for (int l = 0; l < n; l++) Y[l] += wj * K[j][l];
       }
-      c->time = t + dt * cst[s];
+      c->time = t + dt * cs[i];
       oracle_rhs(m, c, qdata, Y, R);
       MassCtx mc = {m, c, qdata, Y};
       if (!dinv) {  // PCSetUp at the first solve
         dinv = malloc(sizeof(double) * n);
-        oracle_mass_newtonian_diagonal(m, c, qdata, Y, dinv);
-        for (int i = 0; i < n; i++) dinv[i] = dinv[i] != 0 ? 1.0 / dinv[i] : 1.0;
+        if (mo->jacobi_rowsum) {
+          double *ones = malloc(sizeof(double) * n);
+          for (int l = 0; l < n; l++) ones[l] = 1.0;
+          mass_apply(&mc, ones, dinv);
+          free(ones);
+        } else {
+          oracle_mass_newtonian_diagonal(m, c, qdata, Y, dinv);
+        }
+        jacobi_invert(n, dinv);
       }
-      total += gmres(n, &mc, dinv, R, K[s], ksp_rtol, ksp_restart, 10000);
+      const Prec pc = {dinv, 1};
+      if (mo->gmres) total += gmres(n, mass_apply, &mc, &pc, R, K[i], mo->rtol, 1e-50, mo->restart, mo->max_it);
+      else
+        for (int l = 0; l < n; l++) K[i][l] = dinv[l] * R[l];
     }
-    for (int s = 0; s < 4; s++) {
-      const double ws = dt * bw[s];
-      for (int i = 0; i < n; i++) q[i] += ws * K[s][i];
+    for (int i = 0; i < s; i++) {
+      const double ws = dt * b[i];
+      if (ws != 0)
+        for (int l = 0; l < n; l++) q[l] += ws * K[i][l];
     }
     t += dt;
-    if (step % record_every == 0 || step == nsteps) record_row(m, c, qdata, q, step, t, vol, &ke_rows[7 * rows++]);
+    if (ke_rows && (step % record_every == 0 || step == nsteps)) record_row(m, c, qdata, q, step, t, vol, &ke_rows[7 * rows++]);
   }
-  *nrows = rows;
-  for (int i = 0; i < 4; i++) free(K[i]);
+  if (nrows) *nrows = rows;
+  if (mass_its) *mass_its = total;
+  for (int i = 0; i < s; i++) free(K[i]);
+  free(K);
   free(Y);
   free(R);
   free(dinv);
-  return total;
+  return 0;
+}
+
+int oracle_rk4_stabilized(const OMesh *m, ONewt *c, const double *qdata, double *q, double dt, int nsteps, double ksp_rtol, int ksp_restart,
+                          int record_every, double *ke_rows, int *nrows) {
+  // TSRK "4" tableau, GMRES mass with the diagonal Jacobi preconditioner (examples/taylor_green_vortex.yaml:41-43)
+  const double A[16] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1, 0};
+  const double b[4]  = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
+  const double cs[4] = {0, 0.5, 0.5, 1};
+  ORKMass      mo    = {1, 0, ksp_rtol, ksp_restart, 10000};
+  long long    its   = 0;
+  oracle_rk(m, c, qdata, q, 0.0, dt, nsteps, 4, A, b, cs, &mo, record_every, ke_rows, nrows, &its);
+  return (int)its;
+}
+
+// ---- implicit ----
+typedef struct {
+  const OMesh  *m;
+  ONewt        *c;
+  const double *qdata;
+  const double *jac;
+} JacCtx;
+
+static void jac_apply(void *ctx, const double *x, double *y) {
+  const JacCtx *jc = ctx;
+  oracle_ijacobian(jc->m, jc->c, jc->qdata, jc->jac, x, y);
+}
+
+static void bdf_coefficients(const double *T, int k, double *alpha) {
+  alpha[0] = 0;
+  for (int m = 1; m <= k; m++) alpha[0] += 1.0 / (T[0] - T[m]);
+  for (int i = 1; i <= k; i++) {
+    double v = 1.0 / (T[i] - T[0]);
+    for (int j = 1; j <= k; j++)
+      if (j != i) v *= (T[0] - T[j]) / (T[i] - T[j]);
+    alpha[i] = v;
+  }
+}
+
+static void lagrange_values(const double *X, int m, double t, double *L) {
+  for (int i = 0; i < m; i++) {
+    double v = 1;
+    for (int j = 0; j < m; j++)
+      if (j != i) v *= (t - X[j]) / (X[i] - X[j]);
+    L[i] = v;
+  }
+}
+
+typedef struct {
+  const OMesh  *m;
+  ONewt        *c;
+  const double *qdata;
+  const OBDFOptions *o;
+  int           n, nhist, cap;
+  double      **hist, *hist_t;
+  double       *x, *xdot, *hsum, *F, *Ftry, *dx, *xtry, *dinv, *jac, *jac_try;
+  OBDFStats    *st;
+} BDF;
+
+static void residual(BDF *b, double tnew, double shift, const double *x, double *F, double *jac) {
+  for (int i = 0; i < b->n; i++) b->xdot[i] = shift * x[i] + b->hsum[i];
+  b->c->time = tnew;
+  oracle_ifunction(b->m, b->c, b->qdata, x, b->xdot, F, jac);
+  b->st->ifunction_evals++;
+}
+
+static int bdf_solve(BDF *b, double tnew, int k) {
+  const int n = b->n;
+  double    T[4], alpha[4];
+  T[0] = tnew;
+  for (int i = 1; i <= k; i++) T[i] = b->hist_t[i - 1];
+  bdf_coefficients(T, k, alpha);
+  const double shift = alpha[0];
+  for (int l = 0; l < n; l++) {
+    double s = 0;
+    for (int i = 1; i <= k; i++) s += alpha[i] * b->hist[i - 1][l];
+    b->hsum[l] = s;
+  }
+  const int m = k + 1 < b->nhist ? k + 1 : b->nhist;
+  double    L[4];
+  lagrange_values(b->hist_t, m, tnew, L);
+  for (int l = 0; l < n; l++) {
+    double s = 0;
+    for (int i = 0; i < m; i++) s += L[i] * b->hist[i][l];
+    b->x[l] = s;
+  }
+  const OBDFOptions *o = b->o;
+  residual(b, tnew, shift, b->x, b->F, b->jac);
+  double fnorm = sqrt(vdot(n, b->F, b->F)), fnorm0 = fnorm;
+  b->st->last_fnorm0     = fnorm0;
+  b->st->last_newton_its = 0;
+  b->st->last_krylov_its = 0;
+  if (!isfinite(fnorm)) return -3;
+  const double ttol      = fmax(o->snes_atol, o->snes_rtol * fnorm0);
+  int          converged = fnorm <= o->snes_atol;
+  b->c->ijacobian_time_shift = shift;
+  for (int it = 1; it <= o->snes_max_it && !converged; it++) {
+    if (o->pc_pbjacobi) {
+      oracle_ijacobian_point_block_diagonal(b->m, b->c, b->qdata, b->jac, b->dinv);
+      if (pb_invert(n / 5, b->dinv)) return -6;
+    } else {
+      oracle_ijacobian_diagonal(b->m, b->c, b->qdata, b->jac, b->dinv);
+      jacobi_invert(n, b->dinv);
+    }
+    JacCtx     jc   = {b->m, b->c, b->qdata, b->jac};
+    const Prec pc   = {b->dinv, o->pc_pbjacobi ? 5 : 1};
+    const int  kits = gmres(n, jac_apply, &jc, &pc, b->F, b->dx, o->ksp_rtol, o->ksp_atol, o->ksp_restart, o->ksp_max_it);
+    b->st->krylov_its += kits;
+    b->st->last_krylov_its += kits;
+    const double ynorm = sqrt(vdot(n, b->dx, b->dx));
+    const double initslope = -fnorm * fnorm, alpha_ls = 1e-4;
+    double       lambda = 1.0, gnorm = 0;
+    for (int bt = 0;; bt++) {
+      for (int l = 0; l < n; l++) b->xtry[l] = b->x[l] - lambda * b->dx[l];
+      residual(b, tnew, shift, b->xtry, b->Ftry, b->jac_try);
+      gnorm = sqrt(vdot(n, b->Ftry, b->Ftry));
+      if (isfinite(gnorm) && 0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope) break;
+      if (bt >= 40) return -4;
+      b->st->line_search_backtracks++;
+      double lq = isfinite(gnorm) ? -initslope * lambda * lambda / (2.0 * (0.5 * gnorm * gnorm - 0.5 * fnorm * fnorm - lambda * initslope)) : 0;
+      lambda    = fmin(fmax(lq, 0.1 * lambda), 0.5 * lambda);
+    }
+    double *sw;
+    sw = b->x, b->x = b->xtry, b->xtry = sw;
+    sw = b->F, b->F = b->Ftry, b->Ftry = sw;
+    sw = b->jac, b->jac = b->jac_try, b->jac_try = sw;
+    fnorm = gnorm;
+    b->st->newton_its++;
+    b->st->last_newton_its = it;
+    const double xnorm = sqrt(vdot(n, b->x, b->x));
+    if (fnorm <= ttol || lambda * ynorm <= o->snes_stol * xnorm) converged = 1;
+  }
+  b->st->last_fnorm = fnorm;
+  return converged ? 0 : -4;
+}
+
+static void hist_push(BDF *b, const double *x, double tx) {
+  double *last = b->hist[b->cap - 1];
+  for (int i = b->cap - 1; i > 0; i--) {
+    b->hist[i]   = b->hist[i - 1];
+    b->hist_t[i] = b->hist_t[i - 1];
+  }
+  b->hist[0]   = last;
+  b->hist_t[0] = tx;
+  memcpy(b->hist[0], x, sizeof(double) * b->n);
+  b->nhist = b->nhist + 1 < b->cap ? b->nhist + 1 : b->cap;
+}
+
+int oracle_bdf(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, int nsteps, const OBDFOptions *o, OBDFStats *st) {
+  const int n  = 5 * m->nnodes;
+  const int nj = 14 + (c->idl_enable ? 1 : 0);
+  BDF       b  = {0};
+  b.m = m, b.c = c, b.qdata = qdata, b.o = o, b.n = n, b.cap = o->order + 1, b.st = st;
+  memset(st, 0, sizeof(*st));
+  b.hist   = malloc(sizeof(double *) * b.cap);
+  b.hist_t = calloc(b.cap, sizeof(double));
+  for (int i = 0; i < b.cap; i++) b.hist[i] = malloc(sizeof(double) * n);
+  double **bufs[] = {&b.x, &b.xdot, &b.hsum, &b.F, &b.Ftry, &b.dx, &b.xtry, &b.dinv};
+  for (int i = 0; i < 8; i++) *bufs[i] = malloc(sizeof(double) * n * (i == 7 ? 5 : 1));
+  b.jac     = malloc(sizeof(double) * (size_t)m->nelem * nj * m->Q * m->Q * m->Q);
+  b.jac_try = malloc(sizeof(double) * (size_t)m->nelem * nj * m->Q * m->Q * m->Q);
+  c->dt      = o->dt;
+  double t   = t0;
+  int    k   = 1, ierr = 0;
+  for (int step = 0; step < nsteps && !ierr; step++) {
+    if (b.nhist == 0) {
+      hist_push(&b, q, t);
+      if (o->order > 1) {
+        if ((ierr = bdf_solve(&b, t + o->dt / 2, 1))) break;
+        hist_push(&b, b.x, t + o->dt / 2);
+        k = 2;
+      } else {
+        k = 1;
+      }
+    } else {
+      k = k + 1 < o->order ? k + 1 : o->order;
+    }
+    if ((ierr = bdf_solve(&b, t + o->dt, k))) break;
+    hist_push(&b, b.x, t + o->dt);
+    memcpy(q, b.x, sizeof(double) * n);
+    t += o->dt;
+    st->steps++;
+  }
+  st->time = t;
+  for (int i = 0; i < b.cap; i++) free(b.hist[i]);
+  free(b.hist);
+  free(b.hist_t);
+  for (int i = 0; i < 8; i++) free(*bufs[i]);
+  free(b.jac);
+  free(b.jac_try);
+  return ierr;
 }

@@ -162,6 +162,13 @@ def ijacobian_diagonal(m: Mesh, c: ONewt, qdata, jac):
     return v
 
 
+def ijacobian_point_block_diagonal(m: Mesh, c: ONewt, qdata, jac):
+    """[nnodes][5 (out)][5 (in)] point-block diagonal of the IJacobian."""
+    pb = np.zeros(25 * m.nnodes)
+    lib().oracle_ijacobian_point_block_diagonal(ct.byref(m.s), ct.byref(c), _d(qdata), _d(jac), _d(pb))
+    return pb.reshape(m.nnodes, 5, 5)
+
+
 def rhs(m: Mesh, c: ONewt, qdata, q):
     v = np.zeros(5 * m.nnodes)
     lib().oracle_rhs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(v))
@@ -342,3 +349,91 @@ def hll_speeds_fwd(gas: OGas, qL, dqL, qR, dqR, var, n):
     out = [ct.c_double() for _ in range(4)]
     lib().oracle_hll_speeds_fwd(ct.byref(gas), _d(_v(qL)), _d(_v(dqL)), _d(_v(qR)), _d(_v(dqR)), int(var), _d(_v(n)), *[ct.byref(o) for o in out])
     return tuple(o.value for o in out)
+
+
+# ---- time integration (honee_oracle_solve.c) ----
+class ORKMass(ct.Structure):
+    _fields_ = [("gmres", ct.c_int), ("jacobi_rowsum", ct.c_int), ("rtol", ct.c_double), ("restart", ct.c_int), ("max_it", ct.c_int)]
+
+
+class OBDFOptions(ct.Structure):
+    _fields_ = [("order", ct.c_int), ("dt", ct.c_double), ("snes_rtol", ct.c_double), ("snes_atol", ct.c_double), ("snes_stol", ct.c_double),
+                ("snes_max_it", ct.c_int), ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_restart", ct.c_int),
+                ("ksp_max_it", ct.c_int), ("pc_pbjacobi", ct.c_int)]
+
+
+class OBDFStats(ct.Structure):
+    _fields_ = [("steps", ct.c_int), ("time", ct.c_double), ("ifunction_evals", ct.c_longlong), ("newton_its", ct.c_longlong),
+                ("krylov_its", ct.c_longlong), ("line_search_backtracks", ct.c_longlong), ("last_newton_its", ct.c_int),
+                ("last_krylov_its", ct.c_int), ("last_fnorm0", ct.c_double), ("last_fnorm", ct.c_double)]
+
+
+def tableau_ref(name: str, stages: int = 4):
+    """Butcher tableaus restated from their published definitions: classic RK4; Fehlberg's RK5(4) propagating the
+    fifth-order weights; Shu-Osher SSP(3,3); forward Euler; Ketcheson's optimal third-order SSP with n^2 stages,
+    whose stages are derived here by running its low-storage recurrence on symbolic stage coefficients."""
+    if name == "4":
+        A = [[0, 0, 0, 0], [0.5, 0, 0, 0], [0, 0.5, 0, 0], [0, 0, 1, 0]]
+        b = [1 / 6, 1 / 3, 1 / 3, 1 / 6]
+    elif name == "5f":
+        A = [[0] * 6, [1 / 4, 0, 0, 0, 0, 0], [3 / 32, 9 / 32, 0, 0, 0, 0], [1932 / 2197, -7200 / 2197, 7296 / 2197, 0, 0, 0],
+             [439 / 216, -8, 3680 / 513, -845 / 4104, 0, 0], [-8 / 27, 2, -3544 / 2565, 1859 / 4104, -11 / 40, 0]]
+        b = [16 / 135, 0, 6656 / 12825, 28561 / 56430, -9 / 50, 2 / 55]
+    elif name == "ssp33":
+        A = [[0, 0, 0], [1, 0, 0], [0.25, 0.25, 0]]
+        b = [1 / 6, 1 / 6, 2 / 3]
+    elif name == "euler":
+        A, b = [[0]], [1]
+    elif name == "ssp3":
+        n = int(round(np.sqrt(stages)))
+        assert n * n == stages and n >= 2
+        r = stages - n
+        A = np.zeros((stages, stages))
+        cur = np.zeros(stages)  # coefficients of the current register over h K_j (the u coefficient is 1)
+        k = 0
+
+        def lstep(cur, k):
+            A[k] = cur
+            cur = cur.copy()
+            cur[k] += 1.0 / r
+            return cur, k + 1
+
+        for _ in range((n - 1) * (n - 2) // 2):
+            cur, k = lstep(cur, k)
+        saved = cur.copy()
+        for _ in range((n - 1) * (n - 2) // 2 + 1, n * (n + 1) // 2):
+            cur, k = lstep(cur, k)
+        cur, k = lstep(cur, k)
+        cur = (n * saved + (n - 1) * cur) / (2 * n - 1)
+        for _ in range(n * (n + 1) // 2 + 1, n * n + 1):
+            cur, k = lstep(cur, k)
+        assert k == stages
+        b = cur
+    else:
+        raise KeyError(name)
+    A, b = np.array(A, dtype=np.float64), np.array(b, dtype=np.float64)
+    return A, b, A.sum(axis=1)
+
+
+def rk(m: Mesh, c: ONewt, qdata, q, dt, nsteps, A, b, cc, t0=0.0, gmres=False, rowsum=True, rtol=1e-5, restart=30, max_it=10000,
+       record_every=0):
+    q = np.array(q, dtype=np.float64)
+    A, b, cc = (np.ascontiguousarray(x, dtype=np.float64) for x in (A, b, cc))
+    mo = ORKMass(int(gmres), int(rowsum), rtol, restart, max_it)
+    rows = np.zeros((nsteps + 2, 7)) if record_every else None
+    nrows = ct.c_int(0)
+    its = ct.c_longlong(0)
+    lib().oracle_rk(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(t0), ct.c_double(dt), nsteps, len(b), _d(A), _d(b), _d(cc),
+                    ct.byref(mo), max(record_every, 1), _d(rows) if rows is not None else None, ct.byref(nrows), ct.byref(its))
+    return q, (rows[: nrows.value] if rows is not None else None), its.value
+
+
+def bdf(m: Mesh, c: ONewt, qdata, q, dt, nsteps, order=2, t0=0.0, snes_rtol=1e-8, snes_atol=1e-50, snes_stol=1e-8, snes_max_it=50,
+        ksp_rtol=1e-5, ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, pbjacobi=False):
+    q = np.array(q, dtype=np.float64)
+    o = OBDFOptions(order, dt, snes_rtol, snes_atol, snes_stol, snes_max_it, ksp_rtol, ksp_atol, ksp_restart, ksp_max_it, int(pbjacobi))
+    st = OBDFStats()
+    ierr = lib().oracle_bdf(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(t0), nsteps, ct.byref(o), ct.byref(st))
+    if ierr:
+        raise RuntimeError(f"oracle BDF failed ({ierr})")
+    return q, {k: getattr(st, k) for k, _ in OBDFStats._fields_}

@@ -108,3 +108,24 @@ def test_monitor_ke_matches_oracle(state_var):
     for k in (2, 3):
         assert abs(s[k] - s_ref[k]) <= 1e-12 * abs(s_ref[0]) + 1e-12 * abs(s_ref[k]), (k, s[k], s_ref[k])
     prob.close()
+
+
+@pytest.mark.parametrize("stabilization", [2, 0])
+def test_mass_point_block_diagonal_matches_unit_columns(stabilization):
+    _say(f"mass point-block stab={stabilization}")
+    case = make_case(shape=(2, 2, 2), degree=2, perturb=0.1, state_var=0, stabilization=stabilization, implicit=False, seed=310 + stabilization)
+    om, oc, qd = oracle_setup(case)
+    q = case["q"]
+    nn = om.nnodes
+    pb_ref = np.zeros((nn, 5, 5))
+    for node in range(nn):
+        for ci in range(5):
+            e = np.zeros(5 * nn)
+            e[node * 5 + ci] = 1.0
+            pb_ref[node, :, ci] = O.mass_newtonian(om, oc, qd, q, e)[node * 5: node * 5 + 5]
+    prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), 0)
+    pb = np.zeros((nn, 5, 5))
+    prob.mass_point_block_diagonal(q, pb)
+    e2, emax = rel_err(pb, pb_ref)
+    assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+    prob.close()

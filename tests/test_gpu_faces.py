@@ -154,3 +154,44 @@ def test_jacobian_diagonal_with_faces(kind, degree):
     e2, emax = rel_err(d, d_ref)
     assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
     prob.close()
+
+
+@pytest.mark.parametrize("kind,state_var,fp32", [(0, 1, False), (2, 2, False), (1, 0, True)])
+def test_point_block_diagonal_with_faces(kind, state_var, fp32):
+    """Point-block diagonal [node][c_out][c_in] of the IJacobian composite (volume + faces) against the oracle's
+    volume point blocks plus the face operator's unit columns; FP32 Jacobian mode at 1e-5."""
+    _say(f"point-block diagonal with faces kind={kind} var={state_var} fp32={fp32}")
+    case = make_case(shape=(2, 2, 2), degree=2, periodic=(False, True, True), perturb=0.1, state_var=state_var, stabilization=2,
+                     seed=170 + kind)
+    om, oc, qd = oracle_setup(case)
+    fe, fi, f = _faces_setup(case, [0, 1])
+    q, qdot = case["q"], case["qdot"]
+    _, jac_vol = O.ifunction(om, oc, qd, q, qdot)
+    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q)
+    shift = 80.0
+    oc.ijacobian_time_shift = shift
+    pb_ref = O.ijacobian_point_block_diagonal(om, oc, qd, jac_vol)
+    nn = om.nnodes
+    for node in range(nn):  # face part by unit columns
+        for ci in range(5):
+            e = np.zeros(5 * nn)
+            e[node * 5 + ci] = 1.0
+            pb_ref[node, :, ci] += _oracle_boundary_jac(om, oc, f, kind, qds, jac_bnd, fc, e)[node * 5: node * 5 + 5]
+    res = "/gpu/hip/mi355x:jacobian_precision=fp32" if fp32 else "/gpu/hip/mi355x"
+    prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), state_var, res)
+    prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    g = np.zeros_like(q)
+    prob.ifunction(q, qdot, g)
+    pb = np.zeros((nn, 5, 5))
+    prob.ijacobian_point_block_diagonal(pb, shift)
+    e2, emax = rel_err(pb, pb_ref)
+    print(f"point-block rel2 {e2:.3e} max {emax:.3e}", flush=True)
+    if fp32:
+        assert e2 < 1e-5
+    else:
+        assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
+    # the diagonal of each block is the point diagonal
+    d = np.zeros_like(q)
+    prob.ijacobian_diagonal(d, shift)
+    assert rel_err(np.einsum("nkk->nk", pb).ravel(), d)[0] < (1e-6 if fp32 else 1e-13)
+    prob.close()

@@ -249,3 +249,24 @@ def test_ijacobian_diagonal_matches_columns():
         e[i] = 1.0
         cols[i] = O.ijacobian(om, oc, qd, jac, e)[i]
     np.testing.assert_allclose(diag, cols, rtol=1e-12, atol=1e-12 * np.abs(cols).max())
+
+
+def test_oracle_point_block_diagonal_brute_force():
+    """The oracle's point-block diagonal equals the node blocks of the assembled operator's unit columns, and its
+    block diagonals equal the point diagonal."""
+    from helpers import make_case, oracle_setup
+
+    case = make_case(shape=(2, 2, 2), degree=1, state_var=1, seed=3)
+    om, oc, qd = oracle_setup(case)
+    _, jac = O.ifunction(om, oc, qd, case["q"], case["qdot"])
+    oc.ijacobian_time_shift = 50.0
+    pb = O.ijacobian_point_block_diagonal(om, oc, qd, jac)
+    nn = om.nnodes
+    ref = np.zeros((nn, 5, 5))
+    for node in range(nn):
+        for ci in range(5):
+            e = np.zeros(5 * nn)
+            e[node * 5 + ci] = 1
+            ref[node, :, ci] = O.ijacobian(om, oc, qd, jac, e)[node * 5: node * 5 + 5]
+    assert np.abs(ref - pb).max() <= 1e-13 * np.abs(ref).max()
+    np.testing.assert_allclose(np.einsum("nkk->nk", pb).ravel(), O.ijacobian_diagonal(om, oc, qd, jac), rtol=0, atol=1e-13 * np.abs(ref).max())
