@@ -18,7 +18,7 @@ struct Ceed_private {
   int              device   = 0;
   hipStream_t      stream   = nullptr;  // null stream unless CeedSetStream
   bool             jac_fp32 = false;
-  int              scatter  = 0;  // ScatterMode: 0 atomic, 1 two-pass (deterministic)
+  int              scatter  = 1;  // ScatterMode: 0 atomic, 1 two-pass (deterministic, the default)
   double          *d_work   = nullptr;  // E-vector workspace of the two-pass scatter
   size_t           work_len = 0;
   CeedErrorHandler handler  = nullptr;
