@@ -177,6 +177,11 @@ struct Contract5 {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 5; k++) s0[k][t] = w[k];
+    add_staged(s0, Fx, Fy, Fz, r);
+  }
+
+  // As add(), with the five weights already in src[k][q] (written by every thread before the call)
+  __device__ void add_staged(const double (*src)[D::MAX3], const double *Fx, const double *Fy, const double *Fz, double (&r)[5]) {
     __syncthreads();
     if (t < D::PQQ) {
       const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
@@ -184,7 +189,7 @@ struct Contract5 {
       for (int k = 0; k < 5; k++) {
         double acc = 0;
 #pragma unroll
-        for (int z = 0; z < Q; z++) acc += Fz[z * P + c] * s0[k][(z * Q + iy) * Q + ix];
+        for (int z = 0; z < Q; z++) acc += Fz[z * P + c] * src[k][(z * Q + iy) * Q + ix];
         s1[k][t] = acc;
       }
     }

@@ -353,9 +353,13 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
   constexpr int EPB = elems_per_block<Q>();
   constexpr int Q3  = Dm::Q3;
   constexpr int NS  = Op::kInSlots;
+  constexpr int NP  = 10;  // slot pairs (with one input slot only pairs 0, 4, 5, 6 carry weights)
   __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
   __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
-  __shared__ double c0[EPB][5][Dm::MAX3], c1[EPB][5][Dm::MAX3], c2[EPB][5][Dm::MAX3];
+  // pair weights W[pair][c_out][q] (each thread owns its q column, so the accumulation needs no barrier) and the
+  // contraction's intermediate arrays
+  __shared__ double sW[EPB][NP][5][Dm::MAX3];
+  __shared__ double c1[EPB][5][Dm::MAX3], c2[EPB][5][Dm::MAX3];
   for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
     sBB[i] = a.bt.BB[i];
     sGB[i] = a.bt.GB[i];
@@ -363,11 +367,12 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
   }
   __shared__ double sB[Q * P], sDc[Q * Q];
   load_tables<P, Q>(a.bt, sB, sDc);
-  const int           le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>          el(sB, sDc, s0[le], s1[le], s2[le], t);
-  basis3::Contract5<P, Q> pb{c0[le], c1[le], c2[le], t};
-  const long long     nbatch = ((long long)a.nelem + EPB - 1) / EPB;
-  // slot pairs (s, b), s <= b, and their 1D factors (x, y, z) as in k_diagonal
+  const int               le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  Elem<P, Q>              el(sB, sDc, s0[le], s1[le], s2[le], t);
+  basis3::Contract5<P, Q> pb{nullptr, c1[le], c2[le], t};
+  double(*W)[5][Dm::MAX3] = sW[le];
+  const long long         nbatch = ((long long)a.nelem + EPB - 1) / EPB;
+  // slot pairs (s, b), s <= b, as in k_diagonal: 0..3 the diagonal pairs, then (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
   constexpr int kPair[4][4] = {{0, 4, 5, 6}, {4, 1, 7, 8}, {5, 7, 2, 9}, {6, 8, 9, 3}};
 
   for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
@@ -380,42 +385,50 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
 #pragma unroll
       for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
     }
-    typename Op::Point p;
-    double             qd[10];
-    nsop::fetch_qdata<Q>(a, e, t, ok, qd);
-    Op::template load_point<Q>(a, ctx, e, t, ok, qd, u2, p);
     for (int ci = 0; ci < 5; ci++) {
-      double W[10][5];
+      __syncthreads();  // the previous component's contractions have read W
+      for (int k = 0; k < NP; k++)
+        if (NS > 1 || k == 0 || (k >= 4 && k <= 6))
 #pragma unroll
-      for (int k = 0; k < 10; k++)
-#pragma unroll
-        for (int co = 0; co < 5; co++) W[k][co] = 0;
+          for (int co = 0; co < 5; co++) W[k][co][t] = 0;
+      // The point data is reloaded for each input component (an opaque element index keeps the compiler from hoisting
+      // it out of the loop), so that it is not live across the contractions below.
+      int eo = e;
+      asm volatile("" : "+v"(eo));
+      typename Op::Point p;
+      double             qd[10];
+      nsop::fetch_qdata<Q>(a, eo, t, ok, qd);
+      Op::template load_point<Q>(a, ctx, eo, t, ok, qd, u2, p);
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         T dq[5] = {0, 0, 0, 0, 0}, gref[3][5] = {{0}};
-        if (s == 0) dq[ci] = 1;
-        else gref[s - 1][ci] = 1;
+#pragma unroll
+        for (int c = 0; c < 5; c++) {  // unit input (s, ci) without a runtime array index
+          if (s == 0) dq[c] = c == ci ? T(1) : T(0);
+          else gref[s - 1][c] = c == ci ? T(1) : T(0);
+        }
         double v[5], gw[3][5];
         Op::action(ctx, p, dq, gref, v, gw);
+        if (!ok) continue;
 #pragma unroll
         for (int co = 0; co < 5; co++) {
-          W[kPair[s][0]][co] += ok ? v[co] : 0.0;
+          W[kPair[s][0]][co][t] += v[co];
 #pragma unroll
-          for (int d = 0; d < 3; d++) W[kPair[s][1 + d]][co] += ok ? gw[d][co] : 0.0;
+          for (int d = 0; d < 3; d++) W[kPair[s][1 + d]][co][t] += gw[d][co];
         }
       }
       double r[5] = {0, 0, 0, 0, 0};
-      pb.add(W[0], sBB, sBB, sBB, r);
-      pb.add(W[4], sGB, sBB, sBB, r);
-      pb.add(W[5], sBB, sGB, sBB, r);
-      pb.add(W[6], sBB, sBB, sGB, r);
+      pb.add_staged(W[0], sBB, sBB, sBB, r);
+      pb.add_staged(W[4], sGB, sBB, sBB, r);
+      pb.add_staged(W[5], sBB, sGB, sBB, r);
+      pb.add_staged(W[6], sBB, sBB, sGB, r);
       if constexpr (NS > 1) {
-        pb.add(W[1], sGG, sBB, sBB, r);
-        pb.add(W[2], sBB, sGG, sBB, r);
-        pb.add(W[3], sBB, sBB, sGG, r);
-        pb.add(W[7], sGB, sGB, sBB, r);
-        pb.add(W[8], sGB, sBB, sGB, r);
-        pb.add(W[9], sBB, sGB, sGB, r);
+        pb.add_staged(W[1], sGG, sBB, sBB, r);
+        pb.add_staged(W[2], sBB, sGG, sBB, r);
+        pb.add_staged(W[3], sBB, sBB, sGG, r);
+        pb.add_staged(W[7], sGB, sGB, sBB, r);
+        pb.add_staged(W[8], sGB, sBB, sGB, r);
+        pb.add_staged(W[9], sBB, sGB, sGB, r);
       }
       if (ok && t < Dm::P3)
 #pragma unroll
