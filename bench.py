@@ -73,6 +73,89 @@ def cpu_baseline(mesh_shape_full, degree, seconds=10.0):
                       f"workload), {n} residual+Jacobian pairs in {n * dt:.1f} s, oracle (FP64 C restatement of /cpu/self)"}
 
 
+def time_integration_c2(mesh, tab, Y, steps, warm_steps=1):
+    """BDF2 on the implicit SUPG workload: Newton (rtol 1e-8) with GMRES(30) (rtol 1e-5) and point-block Jacobi, dt 2e-3,
+    all-FP64 and with the FP32 Jacobian inside GMRES.  Timed after warm_steps (which include the BDF start-up) with the
+    state resident on the GPU; returns seconds per step and iteration counts per step."""
+    from pyhonee import problems as PR
+    from pyhonee import ts as TS
+    from pyhonee.driver import NewtonianProblem, context_from_dict
+
+    d = PR.tgv_context_dict(state_var=1, stabilization=2, implicit=True, dt=2e-3)
+    out, sols = {}, {}
+    for label, resource in (("fp64", "/gpu/hip/mi355x"), ("fp32_jacobian", "/gpu/hip/mi355x:jacobian_precision=fp32")):
+        print(f"# time integration {label}", file=sys.stderr, flush=True)
+        prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1, resource)
+        ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_BDF, bdf_order=2, dt=2e-3, ksp_pc=TS.PC_PBJACOBI))
+        ts.set_solution(Y.ravel(), 0.0)
+        ts.step(warm_steps)
+        s0 = ts.stats()
+        t0 = time.perf_counter()
+        ts.step(steps)
+        elapsed = time.perf_counter() - t0
+        s1 = ts.stats()
+        sols[label] = ts.solution()
+        out[label] = {"s_per_step": elapsed / steps, "newton_its_per_step": (s1["newton_its"] - s0["newton_its"]) / steps,
+                      "krylov_its_per_step": (s1["krylov_its"] - s0["krylov_its"]) / steps, "final_fnorm": s1["last_fnorm"]}
+        ts.close()
+        prob.close()
+    out["fp32_jacobian_vs_fp64_solution_rel"] = float(np.linalg.norm(sols["fp32_jacobian"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+    out["config"] = (f"TGV {mesh.nelem} hex p={tab.P - 1}, primitive, SUPG, BDF2 dt=2e-3, Newton rtol 1e-8 + GMRES(30) rtol 1e-5 + "
+                     f"point-block Jacobi; {warm_steps} warm-up step(s), {steps} timed")
+    return out
+
+
+def explicit_c1(steps=100, cfl=0.3, cpu=True):
+    """The CPU-runnable explicit configuration: TGV 8^3 periodic hex p=2 Q=3, conservative, SUPG off, Shu-Osher SSP-RK3
+    with the lumped mass (HONEE's default mass KSP), `steps` steps at CFL 0.3 with
+    dt = CFL / max_q (sqrt(u^T g u) + c sqrt(lambda_max(g))), g = dXdx^T dXdx / 4 (monitor_cfl.h plus the acoustic speed).
+    GPU seconds for the run, and the oracle's seconds on the host for the same run."""
+    from pyhonee import mesh as M
+    from pyhonee import problems as PR
+    from pyhonee import ts as TS
+    from pyhonee.driver import NewtonianProblem, context_from_dict
+
+    L = 2 * np.pi
+    mesh = M.box_mesh((8, 8, 8), 2, lengths=(L, L, L))
+    tab = M.tabulate(2, 0)
+    q0 = PR.taylor_green_ic(mesh, 0).ravel()
+    gas = PR.TGV_GAS
+    Yn = PR.nodal_field(mesh, lambda x: PR.taylor_green_prim(x))
+    c_max = float(np.sqrt(gas["cp"] / gas["cv"] * (gas["cp"] - gas["cv"]) * Yn[:, 4]).max())
+    u_max = float(np.linalg.norm(Yn[:, 1:4], axis=1).max())
+    sqrt_g = 0.5 * 2.0 / (L / 8)  # affine cube elements: dXdx = 2/h I
+    dt = cfl / (sqrt_g * (u_max + c_max))
+    d = PR.tgv_context_dict(state_var=0, stabilization=0, implicit=False, dt=dt)
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 0)
+    ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_RK, rk_type=TS.RK_SSP33, dt=dt))
+    ts.set_solution(q0, 0.0)
+    ts.step(1)
+    ts.set_solution(q0, 0.0)
+    t0 = time.perf_counter()
+    ts.step(steps)
+    gpu_s = time.perf_counter() - t0
+    ke = ts.monitor_ke()[0] / prob.volume
+    out = {"config": f"TGV 8^3 p=2 conservative, SUPG off, SSP-RK3, lumped mass, {steps} steps, dt={dt:.5g} (CFL {cfl})",
+           "gpu_s": gpu_s, "gpu_ms_per_step": gpu_s / steps * 1e3, "final_kinetic_energy": ke}
+    q_gpu = ts.solution()
+    ts.close()
+    prob.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        om = O.Mesh(mesh, tab)
+        oc = O.newt_from_dict(d)
+        qd = om.setup_qdata()
+        A, b, cc = O.tableau_ref("ssp33")
+        t0 = time.perf_counter()
+        q_cpu, _, _ = O.rk(om, oc, qd, q0, dt, steps, A, b, cc, gmres=False, rowsum=True)
+        out["cpu_oracle_s"] = time.perf_counter() - t0
+        out["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["gpu_vs_oracle_rel"] = float(np.linalg.norm(q_gpu - q_cpu) / np.linalg.norm(q_cpu))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +167,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--scatter", choices=("atomic", "twopass"), default="twopass",
                     help="element scatter-add: FP64 atomics, or the deterministic two-pass node gather")
+    ap.add_argument("--ts-steps", type=int, default=5, help="timed BDF2 steps of the implicit workload (0 skips time integration)")
+    ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
@@ -183,6 +268,11 @@ def main():
         line["ijacobian_fp32_gdofs"] = dof / t32 / 1e9
         line["ijacobian_fp32_roofline_frac"] = b32 / t32 / 1e9 / HBM_PEAK_GBS
     line["ijacobian_roofline_frac"] = b_ij / t_ij / 1e9 / HBM_PEAK_GBS
+    if args.ts_steps > 0:
+        line["time_integration"] = time_integration_c2(mesh, tab, Y, args.ts_steps)
+    if not args.no_explicit:
+        print("# explicit 8^3 p=2 run", file=sys.stderr, flush=True)
+        line["explicit_c1"] = explicit_c1(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         print("# timing CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(shape, degree)
