@@ -25,6 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "honee-4aiad_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one IFunction apply at the default workload (fused kernel 595.0 MB + node gather 168.5 MB, two-pass
+# scatter), from two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0, the factors the counters show for
+# the known-traffic kernels of --calibrate-traffic); the counts and commands are in
+# profiles/ifunction_newtonian_tgv32_p3_traffic.md.  Other workloads report traffic null.
+PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 763.4e6}
 METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
 
 
@@ -169,7 +174,21 @@ def main():
                     help="element scatter-add: FP64 atomics, or the deterministic two-pass node gather")
     ap.add_argument("--ts-steps", type=int, default=5, help="timed BDF2 steps of the implicit workload (0 skips time integration)")
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
+    ap.add_argument("--calibrate-traffic", action="store_true",
+                    help="only run kernels of known HBM traffic (3 reductions reading 1 GiB, 3 fills writing 1 GiB) to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc")
     args = ap.parse_args()
+    if args.calibrate_traffic:
+        import torch
+
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            s_ = x.sum()
+        for _ in range(3):
+            x.fill_(2.0)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "3 x sum reading 1073741824 B, 3 x fill writing 1073741824 B", "check": float(s_)}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
         sys.exit(2)
@@ -258,7 +277,7 @@ def main():
         "ifunction_gdofs": dof / t_if / 1e9,
         "ijacobian_gdofs": dof / t_ij / 1e9,
         "roofline": {"bound": "hbm", "kernel": "ifunction_newtonian", "achieved": b_if / t_if / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
+                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((shape, degree, args.scatter)), "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
                      "avg_launch_us": t_if_k * 1e6, "avg_node_gather_us": t_gather * 1e6, "scatter": args.scatter},
     }
     if "fp32" in results:
