@@ -663,3 +663,247 @@ void oracle_qf_ijacobian(const ONewt *c, const double *dq, const double *gref, d
   qf_ijacobian(c, dq, G, wdetJ, J, jd, v, GV);
   memcpy(gv, GV, sizeof(GV));
 }
+
+// ----------------------------------------------------------------------------------------------------------------------
+// Data-driven SGS closure and its nodal inputs (problems/sgs_dd_model.c, src/velocity_gradient_projection.c,
+// src/grid_anisotropy_tensor.c).  Nodal fields are interleaved L-vectors [nnodes][ncomp] on the solution nodes.
+// ----------------------------------------------------------------------------------------------------------------------
+#include "sgs_pointwise.h"
+
+static void gatherN(const OMesh *m, int e, const double *L, int ncomp, double *u_e) {
+  const int P3 = ipow3(m->P);
+  for (int n = 0; n < P3; n++) {
+    const int node = m->offsets[e * P3 + n];
+    for (int c = 0; c < ncomp; c++) u_e[c * P3 + n] = L[(size_t)node * ncomp + c];
+  }
+}
+
+// Serial scatter-add of an E-vector [nelem][ncomp][P^3] in element order
+static void scatterN_add(const OMesh *m, const double *E, int ncomp, double *L) {
+  const int P3 = ipow3(m->P);
+  for (int e = 0; e < m->nelem; e++)
+    for (int n = 0; n < P3; n++) {
+      const int node = m->offsets[e * P3 + n];
+      for (int c = 0; c < ncomp; c++) L[(size_t)node * ncomp + c] += E[((size_t)e * ncomp + c) * P3 + n];
+    }
+}
+
+// VelocityGradientProjectionRHS (qfunctions/velocity_gradient_projection.h:9-36): r = sum_e B^T (wdetJ du_j/dx_k)
+int oracle_velocity_gradient_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *r) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 9 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], duq[15 * 512], vq[9 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, duq);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5], gref[3][5], wdetJ, dXdx[3][3];
+      for (int k = 0; k < 5; k++) {
+        qi[k] = uq[k * Q3 + i];
+        for (int d = 0; d < 3; d++) gref[d][k] = duq[(d * 5 + k) * Q3 + i];
+      }
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      const OState s = o_state_from_q(c->gas, qi, c->state_var);
+      OState       gs[3];
+      o_phys_gradient(c->gas, &s, c->state_var, gref, dXdx, gs);
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) vq[(j * 3 + k) * Q3 + i] = wdetJ * gs[k].Y[1 + j];  // grad_velocity[j][k] = d u_j / d x_k
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 9, vq, NULL, &E[(size_t)e * 9 * P3]);
+  }
+  memset(r, 0, sizeof(double) * 9 * (size_t)m->nnodes);
+  scatterN_add(m, E, 9, r);
+  free(E);
+  return 0;
+}
+
+// AnisotropyTensorProjection (qfunctions/grid_anisotropy_tensor.h:34-54): r = sum_e B^T (wdetJ [KM(A_ij), delta])
+int oracle_grid_anisotropy_rhs(const OMesh *m, const double *qdata, double *r) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 7 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double        vq[7 * 512];
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double wdetJ, dXdx[3][3], kmA[6], delta;
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      o_anisotropy_tensor(dXdx, kmA, &delta);
+      for (int k = 0; k < 6; k++) vq[k * Q3 + i] = wdetJ * kmA[k];
+      vq[6 * Q3 + i] = wdetJ * delta;
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 7, vq, NULL, &E[(size_t)e * 7 * P3]);
+  }
+  memset(r, 0, sizeof(double) * 7 * (size_t)m->nnodes);
+  scatterN_add(m, E, 7, r);
+  free(E);
+  return 0;
+}
+
+// Mass_N (qfunctions/mass.h): v = sum_e B^T (wdetJ B u), ncomp interleaved components
+int oracle_mass_n(const OMesh *m, const double *qdata, int ncomp, const double *u, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * ncomp * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[9 * 512], uq[9 * 512];
+    gatherN(m, e, u, ncomp, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, ncomp, ue, uq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int k = 0; k < ncomp; k++)
+      for (int i = 0; i < Q3; i++) uq[k * Q3 + i] *= qd[i];
+    interp_grad_t(m->P, m->Q, m->B, m->D, ncomp, uq, NULL, &E[(size_t)e * ncomp * P3]);
+  }
+  memset(v, 0, sizeof(double) * ncomp * (size_t)m->nnodes);
+  scatterN_add(m, E, ncomp, v);
+  free(E);
+  return 0;
+}
+
+int oracle_mass_n_diagonal(const OMesh *m, const double *qdata, int ncomp, double *d) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * ncomp * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double        ue[512], uq[512], re[512];
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int n = 0; n < P3; n++) {
+      memset(ue, 0, sizeof(double) * P3);
+      ue[n] = 1.0;
+      interp_grad(m->P, m->Q, m->B, m->D, 1, ue, uq, NULL);
+      for (int i = 0; i < Q3; i++) uq[i] *= qd[i];
+      interp_grad_t(m->P, m->Q, m->B, m->D, 1, uq, NULL, re);
+      for (int k = 0; k < ncomp; k++) E[((size_t)e * ncomp + k) * P3 + n] = re[n];
+    }
+  }
+  memset(d, 0, sizeof(double) * ncomp * (size_t)m->nnodes);
+  scatterN_add(m, E, ncomp, d);
+  free(E);
+  return 0;
+}
+
+// Velocity gradient projection as HONEE configures it (src/velocity_gradient_projection.c:122-155): KSPPREONLY with
+// PCJACOBI row sum, i.e. G = R / (M 1) per entry.
+int oracle_velocity_gradient_projection(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *G) {
+  const size_t n    = 9 * (size_t)m->nnodes;
+  double      *ones = calloc(n, sizeof(double)), *ML = calloc(n, sizeof(double));
+  for (size_t i = 0; i < n; i++) ones[i] = 1.0;
+  oracle_mass_n(m, qdata, 9, ones, ML);
+  oracle_velocity_gradient_rhs(m, c, qdata, q, G);
+  for (size_t i = 0; i < n; i++) G[i] *= ML[i] != 0 ? 1.0 / ML[i] : 1.0;
+  free(ones);
+  free(ML);
+  return 0;
+}
+
+// Grid anisotropy projection (src/grid_anisotropy_tensor.c:84-110): KSPCG with PCJACOBI (diagonal), natural norm,
+// rtol 1e-10, zero initial guess, on the 7-component system M X = R.  Returns the CG iteration count.
+int oracle_grid_anisotropy_projection(const OMesh *m, const double *qdata, double *X) {
+  const size_t n = 7 * (size_t)m->nnodes;
+  double *R = malloc(sizeof(double) * n), *dinv = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n),
+         *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  oracle_grid_anisotropy_rhs(m, qdata, R);
+  oracle_mass_n_diagonal(m, qdata, 7, dinv);
+  for (size_t i = 0; i < n; i++) dinv[i] = dinv[i] != 0 ? 1.0 / dinv[i] : 1.0;
+  memset(X, 0, sizeof(double) * n);
+  memcpy(r, R, sizeof(double) * n);
+  for (size_t i = 0; i < n; i++) z[i] = dinv[i] * r[i];
+  memcpy(p, z, sizeof(double) * n);
+  double rz = 0;
+  for (size_t i = 0; i < n; i++) rz += r[i] * z[i];
+  const double ttol = fmax(1e-10 * sqrt(fabs(rz)), 1e-50);
+  int          its  = 0;
+  while (sqrt(fabs(rz)) > ttol && its < 10000) {
+    oracle_mass_n(m, qdata, 7, p, Ap);
+    double pAp = 0;
+    for (size_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+    const double alpha = rz / pAp;
+    for (size_t i = 0; i < n; i++) {
+      X[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+      z[i] = dinv[i] * r[i];
+    }
+    double rz_new = 0;
+    for (size_t i = 0; i < n; i++) rz_new += r[i] * z[i];
+    const double beta = rz_new / rz;
+    for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    rz = rz_new;
+    its++;
+  }
+  free(R);
+  free(dinv);
+  free(r);
+  free(z);
+  free(p);
+  free(Ap);
+  return its;
+}
+
+// ComputeSgsDDNodal_Fused (qfunctions/sgs_dd_model.h:65-96) at every node: q [nnodes][5], G [nnodes][9] (du_j/dx_k at
+// j*3+k), A [nnodes][7] (KM anisotropy tensor, delta) -> km_sgs [nnodes][6].  The reference evaluates it at each
+// element node and sums with the inverse multiplicity; every element sees the same nodal inputs, so that sum is the
+// nodal value itself.
+int oracle_sgs_nodal(const OGas *g, int state_var, const OSgsModel *model, int nnodes, const double *q, const double *G, const double *A,
+                     double *km_sgs) {
+#pragma omp parallel for schedule(static)
+  for (int nd = 0; nd < nnodes; nd++) {
+    const OState s = o_state_from_q(*g, &q[(size_t)nd * 5], state_var);
+    double       gu[3][3];
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) gu[j][k] = G[(size_t)nd * 9 + j * 3 + k];
+    o_sgs_nodal(model, gu, &A[(size_t)nd * 7], A[(size_t)nd * 7 + 6], g->mu / s.U[0], &km_sgs[(size_t)nd * 6]);
+  }
+  return 0;
+}
+
+// IFunction_NodalSgs (qfunctions/sgs_dd_model.h:215-241): v += sum_e G^T (-wdetJ dXdx F_sgs(Y, B km_sgs))
+int oracle_ifunction_nodal_sgs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *km_sgs, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 5 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], ke[6 * 512], kq[6 * 512], gvq[15 * 512];
+    gather5(m, e, q, ue);
+    gatherN(m, e, km_sgs, 6, ke);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, NULL);
+    interp_grad(m->P, m->Q, m->B, m->D, 6, ke, kq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5], ki[6], wdetJ, dXdx[3][3], F[5][3];
+      for (int k = 0; k < 5; k++) qi[k] = uq[k * Q3 + i];
+      for (int k = 0; k < 6; k++) ki[k] = kq[k * Q3 + i];
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      const OState s = o_state_from_q(c->gas, qi, c->state_var);
+      o_flux_sgs(s.Y, ki, F);
+      for (int d = 0; d < 3; d++)
+        for (int j = 0; j < 5; j++) gvq[(d * 5 + j) * Q3 + i] = -wdetJ * (dXdx[d][0] * F[j][0] + dXdx[d][1] * F[j][1] + dXdx[d][2] * F[j][2]);
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 5, NULL, gvq, &E[(size_t)e * 5 * P3]);
+  }
+  scatterN_add(m, E, 5, v);
+  free(E);
+  return 0;
+}
+
+// Pointwise entry points (tests of the eigen solver, anisotropy tensor and model)
+int oracle_diagonalize3(const double *A_in, int max_sweeps, double *eval, double *evec) {
+  double A[3][3], E[3][3];
+  memcpy(A, A_in, sizeof(A));
+  const int it = o_diagonalize3(A, eval, E, max_sweeps);
+  memcpy(evec, E, sizeof(E));
+  return it;
+}
+
+void oracle_anisotropy_tensor(const double *dXdx, double *km_A, double *delta) { o_anisotropy_tensor((const double(*)[3])dXdx, km_A, delta); }
+
+void oracle_sgs_point(const OSgsModel *model, const double *gu, const double *km_A, double delta, double nu, double *inputs, double *outputs,
+                      double *evec, double *gmag, double *km_sgs) {
+  double E[3][3], in[6], out[6];
+  o_sgs_inputs((const double(*)[3])gu, km_A, delta, nu, E, in, gmag);
+  o_dd_inference(model, in, out);
+  for (int k = 0; k < 6; k++) inputs[k] = in[k], outputs[k] = out[k];
+  memcpy(evec, E, sizeof(E));
+  o_sgs_outputs(model, out, delta, E, *gmag, km_sgs);
+}

@@ -106,6 +106,17 @@ int oracle_bdf(const OMesh *m, ONewt *c, const double *qdata, double *q, double 
 int oracle_rk4_stabilized(const OMesh *m, ONewt *c, const double *qdata, double *q, double dt, int nsteps, double ksp_rtol, int ksp_restart,
                           int record_every, double *ke_rows /* [(nsteps/record_every+2)][7] */, int *nrows);
 
+// Data-driven SGS closure (honee_oracle.c; pointwise restatement in sgs_pointwise.h)
+int oracle_velocity_gradient_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *r);
+int oracle_grid_anisotropy_rhs(const OMesh *m, const double *qdata, double *r);
+int oracle_mass_n(const OMesh *m, const double *qdata, int ncomp, const double *u, double *v);
+int oracle_mass_n_diagonal(const OMesh *m, const double *qdata, int ncomp, double *d);
+int oracle_velocity_gradient_projection(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *G);
+int oracle_grid_anisotropy_projection(const OMesh *m, const double *qdata, double *X);
+int oracle_ifunction_nodal_sgs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *km_sgs, double *v);
+int oracle_diagonalize3(const double *A, int max_sweeps, double *eval, double *evec);
+void oracle_anisotropy_tensor(const double *dXdx, double *km_A, double *delta);
+
 // Faces
 int oracle_setup_qdata_boundary(const OMesh *m, const OFaces *f, double *qdata_sur);
 int oracle_freestream(const OMesh *m, const OFaces *f, const OFreestream *c, const double *qdata_sur, const double *q, double *v, double *jac_sur);

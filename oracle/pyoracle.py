@@ -437,3 +437,91 @@ def bdf(m: Mesh, c: ONewt, qdata, q, dt, nsteps, order=2, t0=0.0, snes_rtol=1e-8
     if ierr:
         raise RuntimeError(f"oracle BDF failed ({ierr})")
     return q, {k: getattr(st, k) for k, _ in OBDFStats._fields_}
+
+
+# ---- data-driven SGS closure (honee_oracle.c, sgs_pointwise.h) ----
+class OSgsModel(ct.Structure):
+    _fields_ = [("num_inputs", ct.c_int), ("num_outputs", ct.c_int), ("num_neurons", ct.c_int), ("alpha", ct.c_double),
+                ("b1", _dp), ("b2", _dp), ("W1", _dp), ("W2", _dp), ("bounds", _dp)]
+
+
+class SgsModel:
+    """Keeps the parameter arrays alive for the C struct."""
+
+    def __init__(self, params: dict):
+        self.arrays = {k: np.ascontiguousarray(params[k], dtype=np.float64) for k in ("b1", "b2", "W1", "W2", "bounds")}
+        a = self.arrays
+        self.s = OSgsModel(a["W1"].shape[1], a["W2"].shape[0], a["W1"].shape[0], float(params["alpha"]), _d(a["b1"]), _d(a["b2"]),
+                           _d(a["W1"]), _d(a["W2"]), _d(a["bounds"]))
+
+
+def diagonalize3(A, max_sweeps=5):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    ev, E = np.zeros(3), np.zeros((3, 3))
+    it = lib().oracle_diagonalize3(_d(A), max_sweeps, _d(ev), _d(E))
+    return ev, E, it
+
+
+def anisotropy_tensor(dXdx):
+    dXdx = np.ascontiguousarray(dXdx, dtype=np.float64)
+    kmA, delta = np.zeros(6), ct.c_double(0)
+    lib().oracle_anisotropy_tensor(_d(dXdx), _d(kmA), ct.byref(delta))
+    return kmA, delta.value
+
+
+def sgs_point(model: SgsModel, gu, kmA, delta, nu):
+    gu = np.ascontiguousarray(gu, dtype=np.float64)
+    kmA = np.ascontiguousarray(kmA, dtype=np.float64)
+    inp, out, E, km = np.zeros(6), np.zeros(6), np.zeros((3, 3)), np.zeros(6)
+    gmag = ct.c_double(0)
+    lib().oracle_sgs_point(ct.byref(model.s), _d(gu), _d(kmA), ct.c_double(delta), ct.c_double(nu), _d(inp), _d(out), _d(E), ct.byref(gmag), _d(km))
+    return dict(inputs=inp, outputs=out, evec=E, gmag=gmag.value, km_sgs=km)
+
+
+def velocity_gradient_rhs(m: Mesh, c: ONewt, qdata, q):
+    r = np.zeros(9 * m.nnodes)
+    lib().oracle_velocity_gradient_rhs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(r))
+    return r
+
+
+def velocity_gradient_projection(m: Mesh, c: ONewt, qdata, q):
+    G = np.zeros(9 * m.nnodes)
+    lib().oracle_velocity_gradient_projection(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(G))
+    return G
+
+
+def grid_anisotropy_rhs(m: Mesh, qdata):
+    r = np.zeros(7 * m.nnodes)
+    lib().oracle_grid_anisotropy_rhs(ct.byref(m.s), _d(qdata), _d(r))
+    return r
+
+
+def grid_anisotropy_projection(m: Mesh, qdata):
+    X = np.zeros(7 * m.nnodes)
+    its = lib().oracle_grid_anisotropy_projection(ct.byref(m.s), _d(qdata), _d(X))
+    return X, its
+
+
+def mass_n(m: Mesh, qdata, ncomp, u):
+    v = np.zeros(ncomp * m.nnodes)
+    lib().oracle_mass_n(ct.byref(m.s), _d(qdata), ncomp, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(v))
+    return v
+
+
+def mass_n_diagonal(m: Mesh, qdata, ncomp):
+    d = np.zeros(ncomp * m.nnodes)
+    lib().oracle_mass_n_diagonal(ct.byref(m.s), _d(qdata), ncomp, _d(d))
+    return d
+
+
+def sgs_nodal(gas: OGas, state_var: int, model: SgsModel, q, G, A):
+    n = q.size // 5
+    km = np.zeros(6 * n)
+    lib().oracle_sgs_nodal(ct.byref(gas), state_var, ct.byref(model.s), n, _d(q), _d(G), _d(A), _d(km))
+    return km
+
+
+def ifunction_nodal_sgs(m: Mesh, c: ONewt, qdata, q, km_sgs):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_ifunction_nodal_sgs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(km_sgs), _d(v))
+    return v
