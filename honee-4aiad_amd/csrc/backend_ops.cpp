@@ -6,9 +6,12 @@
 // src/qdata.c:154-169) and records the restriction/basis/vector objects; device pointers and the QFunction context are
 // read again at every application, because HONEE swaps vector arrays and context values between calls.
 // QFunctions without a fused kernel are rejected with CEED_ERROR_UNSUPPORTED: there is no host execution path.
+#include <stddef.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "ceed_impl.hpp"
 
@@ -24,9 +27,19 @@ struct OpPlan {
   bool                face = false;  // boundary-face operator: 2D tensor basis, Q^2 points per face
   bool                hllc = false;
   std::string         kernel_name;
+  // field operators of the SGS closure (own output restriction, second inputs) and the model parameters
+  int                 ncomp_out = 0;
+  CeedElemRestriction r_out = nullptr, r_in2 = nullptr, r_aux = nullptr, r_aux2 = nullptr, r_im = nullptr;
+  CeedBasis           b_out = nullptr, b_in2 = nullptr;
+  CeedVector          v_in2 = nullptr, v_aux = nullptr, v_aux2 = nullptr, v_im = nullptr;
+  double             *d_model = nullptr;  // packed sgs model parameters (device)
+  std::vector<double> model_key;          // host copy of what d_model holds
 };
 
-void op_plan_destroy(OpPlan *plan) { delete plan; }
+void op_plan_destroy(OpPlan *plan) {
+  if (plan && plan->d_model) (void)hipFree(plan->d_model);
+  delete plan;
+}
 
 static int plan_bind_fields(CeedOperator op);
 
@@ -62,7 +75,7 @@ static int plan_bind(CeedOperator op) {
   if (!ierr) ierr = check_point_layout(op, p->r_jd, p->Q, p->face);
   if (!ierr) ierr = check_qdata_field(op, p);
   if (ierr) {
-    delete p;
+    op_plan_destroy(p);
     op->plan = nullptr;
   }
   return ierr;
@@ -172,6 +185,184 @@ static int bind_coords(CeedOperator op, OpPlan *p, CeedOperatorField_private *f)
   return 0;
 }
 
+// A field with an offset restriction and a 3D tensor basis of ncomp components (ncomp < 0: any), interpolating to
+// the operator's quadrature points; the first such field fixes P, Q and the element count of the plan.
+static int bind_field_basis(CeedOperator op, OpPlan *p, CeedOperatorField_private *f, int ncomp) {
+  BIND_CHECK(is_offsets(f->rstr) && is_tensor3(f->basis), "field '%s' needs an offset restriction and a 3D tensor basis", f->name.c_str());
+  BIND_CHECK(ncomp < 0 || (f->rstr->num_comp == ncomp && f->basis->num_comp == ncomp), "field '%s' must have %d components", f->name.c_str(), ncomp);
+  BIND_CHECK(f->rstr->num_comp == f->basis->num_comp, "field '%s': restriction and basis component counts differ", f->name.c_str());
+  if (!p->b_q && !p->b_out) {
+    p->P     = f->basis->P_1d;
+    p->Q     = f->basis->Q_1d;
+    p->nelem = f->rstr->num_elem;
+    BIND_CHECK(sgs_pq_supported(p->P, p->Q), "no fused kernel for P_1d=%d, Q_1d=%d", p->P, p->Q);
+  } else {
+    const CeedBasis b0 = p->b_q ? p->b_q : p->b_out;
+    BIND_CHECK(f->basis->P_1d == p->P && f->basis->Q_1d == p->Q && f->rstr->num_elem == p->nelem && f->basis->interp_1d == b0->interp_1d &&
+                   f->basis->grad_1d == b0->grad_1d,
+               "field '%s' must use the same 1D basis tables and elements as the operator's other fields", f->name.c_str());
+  }
+  BIND_CHECK(f->rstr->elem_size == p->P * p->P * p->P, "restriction element size does not match basis");
+  MI_CALL(basis_device_tables(f->basis));
+  return 0;
+}
+
+static int bind_qdata_in(CeedOperator op, OpPlan *p, CeedOperatorField_private *f) {
+  BIND_CHECK(is_strided(f->rstr) && !is_active(f->vec), "%s: qdata must be a passive strided field", op->qf->name.c_str());
+  p->r_qd    = f->rstr;
+  p->v_qdata = f->vec;
+  p->qd      = strided_layout(p->r_qd);
+  return 0;
+}
+
+// Fields of the SGS closure and its projections (problems/sgs_dd_model.c:114-203, 498-558;
+// src/velocity_gradient_projection.c:60-130; src/grid_anisotropy_tensor.c:44-88; src/inverse_multiplicity.c:24-80)
+static int bind_sgs_op(CeedOperator op, OpPlan *p) {
+  CeedQFunction qf = op->qf;
+  auto        **fi = op->in_fields.data(), **fo = op->out_fields.data();
+  switch (qf->kind) {
+    case QfKind::VelocityGradientRHS: {
+      const char  *names[4] = {"q", "Grad_q", "qdata", "x"};
+      CeedEvalMode modes[4] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_NONE, CEED_EVAL_INTERP};
+      CeedInt      sizes[4] = {5, 15, 10, 3};
+      BIND_CHECK(qf->inputs.size() == 4 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      for (int i = 0; i < 4; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+      MI_CALL(check_field(op, 0, false, "velocity gradient", CEED_EVAL_INTERP, 9));
+      MI_CALL(bind_field_basis(op, p, fi[0], 5));
+      p->r_q = fi[0]->rstr;
+      p->b_q = fi[0]->basis;
+      BIND_CHECK(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q,
+                 "%s: q and Grad_q must share the active restriction/basis", qf->name.c_str());
+      MI_CALL(bind_qdata_in(op, p, fi[2]));
+      MI_CALL(bind_coords(op, p, fi[3]));
+      BIND_CHECK(is_active(fo[0]->vec), "%s: the velocity gradient must be active", qf->name.c_str());
+      MI_CALL(bind_field_basis(op, p, fo[0], 9));
+      p->r_out       = fo[0]->rstr;
+      p->b_out       = fo[0]->basis;
+      p->ncomp_out   = 9;
+      p->kernel_name = "velocity_gradient_rhs";
+      return 0;
+    }
+    case QfKind::AnisotropyRHS: {
+      BIND_CHECK(qf->inputs.size() == 1 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      MI_CALL(check_field(op, 0, true, "qdata", CEED_EVAL_NONE, 10));
+      MI_CALL(check_field(op, 0, false, "v", CEED_EVAL_INTERP, 7));
+      BIND_CHECK(is_active(fo[0]->vec), "%s: the output must be active", qf->name.c_str());
+      MI_CALL(bind_field_basis(op, p, fo[0], 7));
+      p->r_out     = fo[0]->rstr;
+      p->b_out     = fo[0]->basis;
+      p->ncomp_out = 7;
+      MI_CALL(bind_qdata_in(op, p, fi[0]));
+      p->kernel_name = "anisotropy_rhs";
+      return 0;
+    }
+    case QfKind::MassN: {
+      const int nc = qf->ncomp;
+      BIND_CHECK(mass_n_supported(nc), "%s: no fused kernel for %d components", qf->name.c_str(), nc);
+      BIND_CHECK(qf->inputs.size() == 2 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      MI_CALL(check_field(op, 0, true, "u", CEED_EVAL_INTERP, nc));
+      MI_CALL(check_field(op, 1, true, "qdata", CEED_EVAL_NONE, -1));
+      MI_CALL(check_field(op, 0, false, "v", CEED_EVAL_INTERP, nc));
+      MI_CALL(bind_field_basis(op, p, fi[0], nc));
+      p->r_q = p->r_out = fi[0]->rstr;
+      p->b_q = p->b_out = fi[0]->basis;
+      BIND_CHECK(is_active(fi[0]->vec) && is_active(fo[0]->vec) && fo[0]->rstr == p->r_q && fo[0]->basis == p->b_q,
+                 "%s: u and v must be active on the same restriction/basis", qf->name.c_str());
+      MI_CALL(bind_qdata_in(op, p, fi[1]));
+      p->ncomp_out   = nc;
+      p->kernel_name = "mass_n";
+      return 0;
+    }
+    case QfKind::InverseMultiplicity: {
+      BIND_CHECK(qf->inputs.size() == 1 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      MI_CALL(check_field(op, 0, true, "multiplicity", CEED_EVAL_NONE, -1));
+      MI_CALL(check_field(op, 0, false, "inverse multiplicity", CEED_EVAL_NONE, 1));
+      BIND_CHECK(is_offsets(fi[0]->rstr) && is_active(fi[0]->vec), "%s: multiplicity must be active with an offset restriction", qf->name.c_str());
+      BIND_CHECK(is_strided(fo[0]->rstr) && is_active(fo[0]->vec) && fo[0]->rstr->num_comp == 1, "%s: output must be an active strided field",
+                 qf->name.c_str());
+      BIND_CHECK(fo[0]->rstr->num_elem == fi[0]->rstr->num_elem && fo[0]->rstr->elem_size == fi[0]->rstr->elem_size,
+                 "%s: input and output restrictions must have the same elements", qf->name.c_str());
+      p->r_q         = fi[0]->rstr;
+      p->r_out       = fo[0]->rstr;
+      p->nelem       = fi[0]->rstr->num_elem;
+      p->kernel_name = "inverse_multiplicity";
+      return 0;
+    }
+    case QfKind::SgsNodal: {
+      const char  *names[5] = {"q", "x", "gradient velocity", "anisotropy tensor", "inverse multiplicity"};
+      CeedEvalMode modes[5] = {CEED_EVAL_NONE, CEED_EVAL_INTERP, CEED_EVAL_NONE, CEED_EVAL_NONE, CEED_EVAL_NONE};
+      CeedInt      sizes[5] = {5, 3, 9, 7, 1};
+      BIND_CHECK(qf->inputs.size() == 5 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      for (int i = 0; i < 5; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+      MI_CALL(check_field(op, 0, false, "km_sgs", CEED_EVAL_NONE, 6));
+      BIND_CHECK(is_offsets(fi[0]->rstr) && !is_active(fi[0]->vec) && fi[0]->vec, "%s: q must be a passive nodal field", qf->name.c_str());
+      BIND_CHECK(!is_active(fi[1]->vec), "%s: x must be passive", qf->name.c_str());  // not read by the closure
+      BIND_CHECK(is_offsets(fi[2]->rstr) && is_active(fi[2]->vec), "%s: the velocity gradient must be the active nodal input", qf->name.c_str());
+      BIND_CHECK(is_offsets(fi[3]->rstr) && !is_active(fi[3]->vec) && fi[3]->vec, "%s: the anisotropy tensor must be a passive nodal field",
+                 qf->name.c_str());
+      BIND_CHECK(is_strided(fi[4]->rstr) && !is_active(fi[4]->vec) && fi[4]->vec, "%s: the inverse multiplicity must be a passive strided field",
+                 qf->name.c_str());
+      BIND_CHECK(is_offsets(fo[0]->rstr) && is_active(fo[0]->vec), "%s: km_sgs must be the active nodal output", qf->name.c_str());
+      p->r_out  = fo[0]->rstr;
+      p->r_q    = fi[2]->rstr;
+      p->r_aux  = fi[0]->rstr;
+      p->v_aux  = fi[0]->vec;
+      p->r_aux2 = fi[3]->rstr;
+      p->v_aux2 = fi[3]->vec;
+      p->r_im   = fi[4]->rstr;
+      p->v_im   = fi[4]->vec;
+      p->nelem  = p->r_out->num_elem;
+      const CeedInt            esz = p->r_out->elem_size;
+      const CeedElemRestriction ins[4] = {p->r_q, p->r_aux, p->r_aux2, p->r_im};
+      for (CeedElemRestriction r : ins)
+        BIND_CHECK(r->num_elem == p->nelem && r->elem_size == esz, "%s: nodal fields must share the element nodes of km_sgs", qf->name.c_str());
+      {  // one closure evaluation per node: the element nodes of a km_sgs node must read the same input nodes
+        const auto                  &oo = p->r_out->h_offsets;
+        std::vector<std::pair<CeedInt, int>> first(oo.size());
+        for (size_t i = 0; i < oo.size(); i++) first[i] = {oo[i], (int)i};
+        std::sort(first.begin(), first.end());
+        for (size_t j = 1; j < first.size(); j++) {
+          if (first[j].first != first[j - 1].first) continue;
+          const int i0 = first[j - 1].second, i1 = first[j].second;
+          for (CeedElemRestriction r : {p->r_q, p->r_aux, p->r_aux2})
+            BIND_CHECK(r->h_offsets[i0] == r->h_offsets[i1], "%s: element nodes sharing a km_sgs node read different input nodes",
+                       qf->name.c_str());
+        }
+      }
+      p->kernel_name = "sgs_dd_nodal";
+      return 0;
+    }
+    case QfKind::NodalSgsResidual: {
+      const char  *names[3] = {"q", "qdata", "km_sgs"};
+      CeedEvalMode modes[3] = {CEED_EVAL_INTERP, CEED_EVAL_NONE, CEED_EVAL_INTERP};
+      CeedInt      sizes[3] = {5, 10, 6};
+      BIND_CHECK(qf->inputs.size() == 3 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      for (int i = 0; i < 3; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+      MI_CALL(check_field(op, 0, false, "Grad_v", CEED_EVAL_GRAD, 15));
+      MI_CALL(bind_field_basis(op, p, fi[0], 5));
+      p->r_q = fi[0]->rstr;
+      p->b_q = fi[0]->basis;
+      BIND_CHECK(is_active(fi[0]->vec), "%s: q must be active", qf->name.c_str());
+      MI_CALL(bind_qdata_in(op, p, fi[1]));
+      BIND_CHECK(!is_active(fi[2]->vec) && fi[2]->vec, "%s: km_sgs must be a passive field", qf->name.c_str());
+      MI_CALL(bind_field_basis(op, p, fi[2], 6));
+      p->r_in2 = fi[2]->rstr;
+      p->b_in2 = fi[2]->basis;
+      p->v_in2 = fi[2]->vec;
+      BIND_CHECK(is_active(fo[0]->vec) && fo[0]->rstr == p->r_q && fo[0]->basis == p->b_q, "%s: Grad_v must be active on the solution space",
+                 qf->name.c_str());
+      p->r_out       = p->r_q;
+      p->b_out       = p->b_q;
+      p->ncomp_out   = 5;
+      p->kernel_name = "ifunction_nodal_sgs";
+      return 0;
+    }
+    default:
+      break;
+  }
+  return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: not an SGS field operator", qf->name.c_str());
+}
+
 static int plan_bind_fields(CeedOperator op) {
   CeedQFunction qf = op->qf;
   auto          p  = new OpPlan();
@@ -179,7 +370,7 @@ static int plan_bind_fields(CeedOperator op) {
   p->var           = qf->state_var;
   op->plan         = p;
   auto fail        = [&](int ierr) {
-    delete p;
+    op_plan_destroy(p);
     op->plan = nullptr;
     return ierr;
   };
@@ -382,6 +573,14 @@ static int plan_bind_fields(CeedOperator op) {
       p->hllc        = qf->hllc;
       p->kernel_name = qf->kind == QfKind::Freestream ? "freestream" : "freestream_jacobian";
       return 0;
+    case QfKind::VelocityGradientRHS:
+    case QfKind::AnisotropyRHS:
+    case QfKind::MassN:
+    case QfKind::InverseMultiplicity:
+    case QfKind::SgsNodal:
+    case QfKind::NodalSgsResidual:
+      if ((ierr = bind_sgs_op(op, p))) return fail(ierr);
+      return 0;
     default:
       break;
   }
@@ -448,6 +647,148 @@ bool op_plan_overwrites(CeedOperator op, CeedVector out) {
   }
   if (rstr_transpose_map(p->r_q)) return false;
   return p->r_q->t_covers_all && out->length == p->r_q->l_size;
+}
+
+// Packed parameters of the data-driven SGS model from its QFunction context (struct SgsDDContext_,
+// qfunctions/sgs_dd_model.h:15-31): b1[20], b2[6], W1[20][6], W2[6][20], bounds[6][2], alpha.  Uploaded when they
+// differ from what the plan's device copy holds.
+static int sgs_model_upload(CeedOperator op, OpPlan *p, CeedMiNewtonianContext *newt) {
+  Ceed                 ceed = op->ceed;
+  CeedQFunctionContext c    = op->qf->ctx;
+  if (!c || !c->data) return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "%s: missing SgsDDContext", op->qf->name.c_str());
+  if (c->size < offsetof(CeedMiSgsDDContext, data))
+    return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: context is smaller than struct SgsDDContext_", op->qf->name.c_str());
+  const auto *sc = (const CeedMiSgsDDContext *)c->data;
+  if (sc->num_inputs != 6 || sc->num_outputs != 6 || sc->num_neurons != 20 || sc->num_layers != 1)
+    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "%s: only the 6-20-6 single-hidden-layer model has a fused kernel", op->qf->name.c_str());
+  const size_t ends[5] = {sc->offsets.bias1 + 20, sc->offsets.bias2 + 6, sc->offsets.weight1 + 120, sc->offsets.weight2 + 120,
+                          sc->offsets.out_scaling + 12};
+  for (size_t e : ends)
+    if (offsetof(CeedMiSgsDDContext, data) + e * sizeof(double) > c->size)
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: model parameters extend past the context", op->qf->name.c_str());
+  if (sc->newt_ctx.divFdiff_method != 0)
+    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "%s: divergence of diffusive flux projection is not supported", op->qf->name.c_str());
+  *newt = sc->newt_ctx;
+  std::vector<double> packed;
+  packed.reserve(sgs_model_doubles());
+  auto put = [&](size_t off, size_t n) { packed.insert(packed.end(), sc->data + off, sc->data + off + n); };
+  put(sc->offsets.bias1, 20);
+  put(sc->offsets.bias2, 6);
+  put(sc->offsets.weight1, 120);
+  put(sc->offsets.weight2, 120);
+  put(sc->offsets.out_scaling, 12);
+  packed.push_back(sc->alpha);
+  if (packed.size() != sgs_model_doubles()) return MI_ERR(ceed, CEED_ERROR_BACKEND, "sgs model packing mismatch");
+  if (p->d_model && packed == p->model_key) return 0;
+  if (!p->d_model) MI_HIP(ceed, hipMalloc((void **)&p->d_model, sizeof(double) * packed.size()));
+  MI_HIP(ceed, hipMemcpyAsync(p->d_model, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice, ceed->stream));
+  MI_HIP(ceed, hipStreamSynchronize(ceed->stream));  // the host buffer is released on return
+  p->model_key = std::move(packed);
+  return 0;
+}
+
+static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector out, bool overwrite) {
+  Ceed        ceed = op->ceed;
+  hipStream_t s    = ceed->stream;
+  hipEvent_t  ev;
+  if (p->kind == QfKind::InverseMultiplicity) {
+    if (in->length != p->r_q->l_size || out->length < (CeedSize)p->nelem * p->r_out->elem_size)
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: vector sizes do not match the restrictions", op->qf->name.c_str());
+    const int    *off;
+    const double *d_in;
+    double       *d_out;
+    MI_CALL(rstr_device_offsets(p->r_q, &off));
+    MI_CALL(vec_device_read(in, &d_in));
+    MI_CALL(vec_device_write(out, &d_out, true));
+    MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+    MI_HIP(ceed, launch_inverse_multiplicity(off, p->nelem, p->r_out->elem_size, d_in, d_out, strided_layout(p->r_out), s));
+    MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+    return 0;
+  }
+  if (p->kind == QfKind::SgsNodal) {
+    CeedMiNewtonianContext ctx;
+    MI_CALL(sgs_model_upload(op, p, &ctx));
+    if (in->length != p->r_q->l_size || out->length != p->r_out->l_size)
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: active vectors must have the restrictions' L-vector sizes", op->qf->name.c_str());
+    CeedElemRestriction r = p->r_out;
+    MI_CALL(rstr_transpose_map(r));
+    SgsNodalArgs a{};
+    a.nrows      = r->t_nrows;
+    a.rowptr     = r->d_t_rowptr;
+    a.index      = r->d_t_index;
+    a.row_offset = r->d_t_offset;
+    a.out_comp_stride = r->comp_stride;
+    MI_CALL(vec_device_write(out, &a.v, true));
+    MI_CALL(rstr_device_offsets(p->r_aux, &a.q_offsets));
+    MI_CALL(rstr_device_offsets(p->r_q, &a.g_offsets));
+    MI_CALL(rstr_device_offsets(p->r_aux2, &a.a_offsets));
+    a.q_comp_stride = p->r_aux->comp_stride;
+    a.g_comp_stride = p->r_q->comp_stride;
+    a.a_comp_stride = p->r_aux2->comp_stride;
+    MI_CALL(vec_device_read(p->v_aux, &a.q));
+    MI_CALL(vec_device_read(in, &a.g));
+    MI_CALL(vec_device_read(p->v_aux2, &a.aniso));
+    MI_CALL(vec_device_read(p->v_im, &a.inv_mult));
+    a.im        = strided_layout(p->r_im);
+    a.elem_size = r->elem_size;
+    a.model     = p->d_model;
+    MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+    MI_HIP(ceed, launch_sgs_nodal(p->var, a, ctx, s));
+    MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+    return 0;
+  }
+  CeedMiNewtonianContext ctx{};
+  if (p->kind == QfKind::VelocityGradientRHS || p->kind == QfKind::NodalSgsResidual) MI_CALL(newtonian_context(op, &ctx));
+  FieldArgs a{};
+  a.nelem = p->nelem;
+  if (p->kind != QfKind::AnisotropyRHS) {
+    if (in->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: input must have the L-vector size", op->qf->name.c_str());
+    MI_CALL(rstr_device_offsets(p->r_q, &a.in_offsets));
+    a.in_comp_stride = p->r_q->comp_stride;
+    MI_CALL(vec_device_read(in, &a.u));
+  }
+  if (p->kind == QfKind::NodalSgsResidual) {
+    MI_CALL(rstr_device_offsets(p->r_in2, &a.in2_offsets));
+    a.in2_comp_stride = p->r_in2->comp_stride;
+    MI_CALL(vec_device_read(p->v_in2, &a.u2));
+  }
+  CeedElemRestriction r = p->r_out;
+  if (out->length != r->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: output must have the L-vector size", op->qf->name.c_str());
+  MI_CALL(rstr_device_offsets(r, &a.out_offsets));
+  a.out_comp_stride = r->comp_stride;
+  MI_CALL(vec_device_write(out, &a.v, true));
+  if (ceed->scatter == 1) {
+    MI_CALL(rstr_transpose_map(r));
+    MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * r->elem_size * p->ncomp_out, &a.ev));
+    a.perm = r->d_t_perm;
+  }
+  MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+  a.qd         = p->qd;
+  const int PQ = p->P * p->Q;
+  CeedBasis b  = p->b_out;
+  a.bt         = BasisTables{b->d_interp_1d, b->d_collo, b->d_prod, b->d_prod + PQ, b->d_prod + 2 * PQ};
+  MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+  switch (p->kind) {
+    case QfKind::VelocityGradientRHS:
+      MI_HIP(ceed, launch_velocity_gradient_rhs(p->P, p->Q, p->var, a, ctx, s));
+      break;
+    case QfKind::AnisotropyRHS:
+      MI_HIP(ceed, launch_anisotropy_rhs(p->P, p->Q, a, s));
+      break;
+    case QfKind::MassN:
+      MI_HIP(ceed, launch_mass_n(p->P, p->Q, p->ncomp_out, a, s));
+      break;
+    default:
+      MI_HIP(ceed, launch_ifunction_nodal_sgs(p->P, p->Q, p->var, a, ctx, s));
+      break;
+  }
+  MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+  if (a.ev) {
+    MI_CALL(timing_begin(ceed, "scatter_node_gather", &ev));
+    MI_HIP(ceed, launch_node_gather(r->t_nrows, p->ncomp_out, r->comp_stride, r->d_t_rowptr, nullptr, r->d_t_offset, a.ev, a.v, overwrite, s));
+    MI_CALL(timing_end(ceed, "scatter_node_gather", ev));
+  }
+  return 0;
 }
 
 int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite) {
@@ -625,6 +966,13 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       MI_CALL(scatter_finish(p, ceed, a, overwrite));
       return 0;
     }
+    case QfKind::VelocityGradientRHS:
+    case QfKind::AnisotropyRHS:
+    case QfKind::MassN:
+    case QfKind::InverseMultiplicity:
+    case QfKind::SgsNodal:
+    case QfKind::NodalSgsResidual:
+      return sgs_field_apply(op, p, in, out, overwrite);
     default:
       return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "operator kind not executable");
   }
@@ -637,6 +985,24 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
   if (!op->plan) MI_CALL(plan_bind(op));
   OpPlan *p    = op->plan;
   Ceed    ceed = op->ceed;
+  if (p->kind == QfKind::MassN) {  // Mass_N of the SGS projections: the scalar mass diagonal in every component
+    if (point_block) return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal of '%s' is not provided", op->qf->name.c_str());
+    if (diag->length != p->r_out->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
+    FieldArgs a{};
+    a.nelem = p->nelem;
+    MI_CALL(rstr_device_offsets(p->r_out, &a.out_offsets));
+    a.out_comp_stride = p->r_out->comp_stride;
+    MI_CALL(vec_device_write(diag, &a.v, true));
+    MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+    a.qd         = p->qd;
+    const int PQ = p->P * p->Q;
+    a.bt         = BasisTables{p->b_out->d_interp_1d, p->b_out->d_collo, p->b_out->d_prod, p->b_out->d_prod + PQ, p->b_out->d_prod + 2 * PQ};
+    hipEvent_t ev;
+    MI_CALL(timing_begin(ceed, "mass_n_diagonal", &ev));
+    MI_HIP(ceed, launch_mass_n_diagonal(p->P, p->Q, p->ncomp_out, a, ceed->stream));
+    MI_CALL(timing_end(ceed, "mass_n_diagonal", ev));
+    return 0;
+  }
   const CeedSize want = point_block ? 5 * p->r_q->l_size : p->r_q->l_size;
   if (point_block && p->r_q->comp_stride != 1)
     return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal assembly needs an interleaved (comp_stride 1) restriction");

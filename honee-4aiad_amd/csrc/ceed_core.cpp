@@ -558,6 +558,29 @@ extern "C" int CeedVectorNorm(CeedVector v, CeedNormType type, CeedScalar *norm)
   return CEED_ERROR_SUCCESS;
 }
 
+extern "C" int CeedMiVectorDot(CeedVector x, CeedVector y, double *result) {
+  Ceed ceed = x->ceed;
+  if (x->length != y->length) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "vector lengths differ");
+  const double *dx, *dy;
+  MI_CALL(vec_device_read(x, &dx));
+  MI_CALL(vec_device_read(y, &dy));
+  const int npart = 1024;
+  if (ceed->scratch_len < npart) {
+    if (ceed->d_scratch) hipFree(ceed->d_scratch);
+    if (ceed->h_scratch) hipHostFree(ceed->h_scratch);
+    MI_HIP(ceed, hipMalloc((void **)&ceed->d_scratch, sizeof(double) * npart));
+    MI_HIP(ceed, hipHostMalloc((void **)&ceed->h_scratch, sizeof(double) * npart));
+    ceed->scratch_len = npart;
+  }
+  MI_HIP(ceed, launch_dot(dx, dy, x->length, ceed->d_scratch, npart, ceed->stream));
+  MI_HIP(ceed, hipMemcpyAsync(ceed->h_scratch, ceed->d_scratch, sizeof(double) * npart, hipMemcpyDeviceToHost, ceed->stream));
+  MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
+  double acc = 0;
+  for (int i = 0; i < npart; i++) acc += ceed->h_scratch[i];
+  *result = acc;
+  return CEED_ERROR_SUCCESS;
+}
+
 extern "C" int CeedVectorScale(CeedVector x, CeedScalar alpha) {
   double *d;
   MI_CALL(vec_device_write(x, &d, true));
@@ -1049,6 +1072,22 @@ static void qf_identify(CeedQFunction qf) {
     qf->kind = QfKind::MassNewtonian;
   } else if (n == "Mass_5") {
     qf->kind = QfKind::Mass5;
+  } else if (n.rfind("Mass_", 0) == 0 && n.size() == 6 && n[5] >= '1' && n[5] <= '9') {  // qfunctions/mass.h:28-42
+    qf->kind  = QfKind::MassN;
+    qf->ncomp = n[5] - '0';
+  } else if (n.rfind("VelocityGradientProjectionRHS_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::VelocityGradientRHS;
+    qf->state_var = var_of(n);
+  } else if (n == "AnisotropyTensorProjection") {
+    qf->kind = QfKind::AnisotropyRHS;
+  } else if (n == "InverseMultiplicity") {
+    qf->kind = QfKind::InverseMultiplicity;
+  } else if (n.rfind("ComputeSgsDDNodal_", 0) == 0 && var_of(n) >= 0 && n.find("Sequential") == std::string::npos) {
+    qf->kind      = QfKind::SgsNodal;
+    qf->state_var = var_of(n);
+  } else if (n.rfind("IFunction_NodalSgs_", 0) == 0 && var_of(n) >= 0) {
+    qf->kind      = QfKind::NodalSgsResidual;
+    qf->state_var = var_of(n);
   } else if (n.rfind("MonitorTotalKineticEnergy_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::MonitorKE;
     qf->state_var = var_of(n);

@@ -99,6 +99,12 @@ enum class QfKind {
   BoundaryIntegralJacobian,
   Freestream,
   FreestreamJacobian,
+  VelocityGradientRHS,
+  AnisotropyRHS,
+  MassN,
+  InverseMultiplicity,
+  SgsNodal,
+  NodalSgsResidual,
 };
 
 struct CeedQFunction_private {
@@ -113,6 +119,7 @@ struct CeedQFunction_private {
   QfKind               kind       = QfKind::Unknown;
   int                  state_var  = 0;
   bool                 hllc       = false;  // Freestream*: HLLC (true) or HLL Riemann flux
+  int                  ncomp      = 0;      // Mass_N: N
 };
 
 struct CtxField {

@@ -6,6 +6,7 @@
 // apply, as VecReadPetscToCeed/VecReadCeedToPetsc do (include/petsc-ceed-utils.h:127-160).
 #include "honee_setup.h"
 
+#include <math.h>
 #include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -61,6 +62,13 @@ struct HoneeMiProblem_ {
   HoneeMiMesh           mesh;  // tabulation pointers stay owned by the caller for the problem's lifetime
   HoneeMiBoundary       bnd[HONEE_MI_MAX_BOUNDARIES];
   int                   nbnd;
+  // data-driven SGS model (problems/sgs_dd_model.c): nodal fields, projections and operators
+  int                   sgs_enabled, grid_aniso_its;
+  CeedElemRestriction   elem_restr_grad_velo, elem_restr_grid_aniso, elem_restr_sgs, elem_restr_inv_mult;
+  CeedBasis             basis_grad_velo, basis_grid_aniso, basis_sgs, basis_x_to_q;
+  CeedVector            grad_velo, grad_velo_inv_rowsum, grid_aniso, sgs_nodal, inv_mult;
+  CeedOperator          op_grad_velo_rhs, op_sgs_nodal, op_sgs_apply;
+  CeedQFunctionContext  sgsdd_qfctx;
   char                  err[512];
 };
 
@@ -458,6 +466,322 @@ int HoneeMiGetBoundaryData(HoneeMiProblem p, int boundary, int which, double *ou
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// Data-driven subgrid-stress model (problems/sgs_dd_model.c, src/velocity_gradient_projection.c,
+// src/grid_anisotropy_tensor.c, src/inverse_multiplicity.c).  The SGS DM of the reference is a clone of the solution
+// DM with 6 components; on one rank its nodes are the solution nodes, so each nodal field here has the solution
+// offsets scaled by its component count and the solution basis tables.
+// ---------------------------------------------------------------------------------------------------------------------
+static const char *const kVelocityGradientRHSLoc[3] = {"qfunctions/velocity_gradient_projection.h:VelocityGradientProjectionRHS_Conserv",
+                                                       "qfunctions/velocity_gradient_projection.h:VelocityGradientProjectionRHS_Prim",
+                                                       "qfunctions/velocity_gradient_projection.h:VelocityGradientProjectionRHS_Entropy"};
+static const char *const kSgsNodalLoc[3]            = {"qfunctions/sgs_dd_model.h:ComputeSgsDDNodal_Conserv", "qfunctions/sgs_dd_model.h:ComputeSgsDDNodal_Prim",
+                                                       "qfunctions/sgs_dd_model.h:ComputeSgsDDNodal_Entropy"};
+static const char *const kSgsApplyLoc[3]            = {"qfunctions/sgs_dd_model.h:IFunction_NodalSgs_Conserv", "qfunctions/sgs_dd_model.h:IFunction_NodalSgs_Prim",
+                                                       "qfunctions/sgs_dd_model.h:IFunction_NodalSgs_Entropy"};
+
+// Nodal restriction and basis of an ncomp-component field on the solution nodes (DMPlexCeedElemRestrictionCreate and
+// DMPlexCeedBasisCreate of a cloned DM)
+static int NodalField(HoneeMiProblem p, int ncomp, CeedElemRestriction *restr, CeedBasis *basis) {
+  const HoneeMiMesh *m  = &p->mesh;
+  const int          P3 = m->P * m->P * m->P;
+  CeedInt           *off = malloc(sizeof(CeedInt) * (size_t)m->nelem * P3);
+  for (size_t i = 0; i < (size_t)m->nelem * P3; i++) off[i] = ncomp * m->offsets[i];
+  int ierr = CeedElemRestrictionCreate(p->ceed, m->nelem, P3, ncomp, 1, ncomp * (CeedSize)m->nnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off, restr);
+  free(off);
+  CHK(p, ierr);
+  if (basis) CHK(p, CeedBasisCreateTensorH1(p->ceed, 3, ncomp, m->P, m->Q, m->B, m->D, m->qref, m->qweight, basis));
+  return 0;
+}
+
+// HoneeMassQFunctionCreate (src/setuplibceed.c) with Mass_N on a nodal field
+static int MassOperator(HoneeMiProblem p, int ncomp, CeedElemRestriction restr, CeedBasis basis, CeedOperator *op) {
+  static const char *const kMassLoc[10] = {NULL, "qfunctions/mass.h:Mass_1", "qfunctions/mass.h:Mass_2", "qfunctions/mass.h:Mass_3",
+                                           "qfunctions/mass.h:Mass_4", "qfunctions/mass.h:Mass_5", "qfunctions/mass.h:Mass_6",
+                                           "qfunctions/mass.h:Mass_7", NULL, "qfunctions/mass.h:Mass_9"};
+  CeedQFunction qf;
+  if (ncomp < 1 || ncomp > 9 || !kMassLoc[ncomp]) return CEED_ERROR_UNSUPPORTED;
+  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kMassLoc[ncomp], &qf));
+  CHK(p, CeedQFunctionAddInput(qf, "u", ncomp, CEED_EVAL_INTERP));
+  CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+  CHK(p, CeedQFunctionAddOutput(qf, "v", ncomp, CEED_EVAL_INTERP));
+  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, op));
+  CHK(p, CeedOperatorSetField(*op, "u", restr, basis, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(*op, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+  CHK(p, CeedOperatorSetField(*op, "v", restr, basis, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedQFunctionDestroy(&qf));
+  return 0;
+}
+
+// PCJACOBI's inverse diagonal.  PETSc maps a zero diagonal entry to 1; the mass diagonals and row sums here are
+// strictly positive (positive w detJ), where that rule and CeedVectorReciprocal agree.
+static int JacobiInverse(HoneeMiProblem p, CeedVector d) {
+  CHK(p, CeedVectorReciprocal(d));
+  return 0;
+}
+
+// VelocityGradientProjectionSetup (src/velocity_gradient_projection.c:36-130): RHS operator and the lumped mass
+// (PCJACOBI row sum of Mass_9, KSPPREONLY), i.e. G = R ./ (M 1)
+static int VelocityGradientProjectionSetup(HoneeMiProblem p) {
+  CeedQFunction qf;
+  CHK(p, NodalField(p, 9, &p->elem_restr_grad_velo, &p->basis_grad_velo));
+  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kVelocityGradientRHSLoc[p->state_var], &qf));
+  CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+  CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+  CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 15, CEED_EVAL_GRAD));
+  CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+  CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+  CHK(p, CeedQFunctionAddOutput(qf, "velocity gradient", 9, CEED_EVAL_INTERP));
+  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_grad_velo_rhs));
+  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "Grad_q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "velocity gradient", p->elem_restr_grad_velo, p->basis_grad_velo, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedQFunctionDestroy(&qf));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grad_velo, &p->grad_velo, NULL));
+  {  // PCJacobiSetType(PC_JACOBI_ROWSUM): MatGetRowSum = M 1
+    CeedOperator op_mass;
+    CeedVector   ones;
+    CHK(p, MassOperator(p, 9, p->elem_restr_grad_velo, p->basis_grad_velo, &op_mass));
+    CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grad_velo, &ones, NULL));
+    CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grad_velo, &p->grad_velo_inv_rowsum, NULL));
+    CHK(p, CeedVectorSetValue(ones, 1.0));
+    CHK(p, CeedOperatorApply(op_mass, ones, p->grad_velo_inv_rowsum, CEED_REQUEST_IMMEDIATE));
+    CHK(p, JacobiInverse(p, p->grad_velo_inv_rowsum));
+    CHK(p, CeedVectorDestroy(&ones));
+    CHK(p, CeedOperatorDestroy(&op_mass));
+  }
+  return 0;
+}
+
+// GridAnisotropyTensorProjectionSetupApply (src/grid_anisotropy_tensor.c:14-111): R = AnisotropyTensorProjection,
+// KSPCG with PCJACOBI (diagonal of Mass_7), natural norm, rtol 1e-10, zero initial guess; the CG recurrences of
+// KSPSolve_CG with KSP_NORM_NATURAL.
+static int GridAnisotropyTensorProjection(HoneeMiProblem p) {
+  CeedQFunction qf;
+  CeedOperator  op_rhs, op_mass;
+  CeedVector    R, dinv, r, z, pv, Ap;
+  CHK(p, NodalField(p, 7, &p->elem_restr_grid_aniso, &p->basis_grid_aniso));
+  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/grid_anisotropy_tensor.h:AnisotropyTensorProjection", &qf));
+  CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+  CHK(p, CeedQFunctionAddOutput(qf, "v", 7, CEED_EVAL_INTERP));
+  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op_rhs));
+  CHK(p, CeedOperatorSetField(op_rhs, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+  CHK(p, CeedOperatorSetField(op_rhs, "v", p->elem_restr_grid_aniso, p->basis_grid_aniso, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedQFunctionDestroy(&qf));
+  CHK(p, MassOperator(p, 7, p->elem_restr_grid_aniso, p->basis_grid_aniso, &op_mass));
+  CeedVector *vs[7] = {&R, &dinv, &r, &z, &pv, &Ap, &p->grid_aniso};
+  for (int i = 0; i < 7; i++) CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grid_aniso, vs[i], NULL));
+  CHK(p, CeedOperatorApply(op_rhs, CEED_VECTOR_NONE, R, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedOperatorLinearAssembleDiagonal(op_mass, dinv, CEED_REQUEST_IMMEDIATE));
+  CHK(p, JacobiInverse(p, dinv));
+  double rz, pAp, rz_new;
+  CHK(p, CeedVectorSetValue(p->grid_aniso, 0.0));
+  CHK(p, CeedVectorCopy(R, r));
+  CHK(p, CeedVectorPointwiseMult(z, dinv, r));
+  CHK(p, CeedVectorCopy(z, pv));
+  CHK(p, CeedMiVectorDot(r, z, &rz));
+  const double ttol = fmax(1e-10 * sqrt(fabs(rz)), 1e-50);
+  int          its  = 0;
+  while (sqrt(fabs(rz)) > ttol && its < 10000) {
+    CHK(p, CeedOperatorApply(op_mass, pv, Ap, CEED_REQUEST_IMMEDIATE));
+    CHK(p, CeedMiVectorDot(pv, Ap, &pAp));
+    const double alpha = rz / pAp;
+    CHK(p, CeedVectorAXPY(p->grid_aniso, alpha, pv));
+    CHK(p, CeedVectorAXPY(r, -alpha, Ap));
+    CHK(p, CeedVectorPointwiseMult(z, dinv, r));
+    CHK(p, CeedMiVectorDot(r, z, &rz_new));
+    const double beta = rz_new / rz;
+    CHK(p, CeedVectorAXPBY(pv, 1.0, beta, z));
+    rz = rz_new;
+    its++;
+  }
+  p->grid_aniso_its = its;
+  for (int i = 0; i < 6; i++) CHK(p, CeedVectorDestroy(vs[i]));
+  CHK(p, CeedOperatorDestroy(&op_rhs));
+  CHK(p, CeedOperatorDestroy(&op_mass));
+  return 0;
+}
+
+// GetInverseMultiplicity (src/inverse_multiplicity.c:24-80) with a local multiplicity: element-node (collocated)
+// 1 / multiplicity of the SGS field
+static int InverseMultiplicity(HoneeMiProblem p) {
+  const int     P3 = p->mesh.P * p->mesh.P * p->mesh.P;
+  CeedQFunction qf;
+  CeedOperator  op;
+  CeedVector    mult;
+  CHK(p, CeedElemRestrictionCreateStrided(p->ceed, p->nelem, P3, 1, (CeedSize)p->nelem * P3, CEED_STRIDES_BACKEND, &p->elem_restr_inv_mult));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_inv_mult, &p->inv_mult, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_sgs, &mult, NULL));
+  CHK(p, CeedElemRestrictionGetMultiplicity(p->elem_restr_sgs, mult));
+  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/inverse_multiplicity.h:InverseMultiplicity", &qf));
+  CHK(p, CeedQFunctionAddInput(qf, "multiplicity", 6, CEED_EVAL_NONE));
+  CHK(p, CeedQFunctionAddOutput(qf, "inverse multiplicity", 1, CEED_EVAL_NONE));
+  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op));
+  CHK(p, CeedOperatorSetName(op, "InverseMultiplicity"));
+  CHK(p, CeedOperatorSetField(op, "multiplicity", p->elem_restr_sgs, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(op, "inverse multiplicity", p->elem_restr_inv_mult, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorApply(op, mult, p->inv_mult, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorDestroy(&mult));
+  CHK(p, CeedOperatorDestroy(&op));
+  CHK(p, CeedQFunctionDestroy(&qf));
+  return 0;
+}
+
+static int FreeSgsContext(void *data) {
+  free(data);
+  return 0;
+}
+
+// SgsDDContextFill (problems/sgs_dd_model.c:602-650): parameters at the reference's offsets, with w1.dat
+// [input][neuron] and w2.dat [neuron][output] transposed to [neuron][input] and [output][neuron]
+static int SgsDDContextCreate(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
+  const int    ni = 6, no = 6, nn = 20;
+  const size_t bias1 = 0, bias2 = bias1 + nn, weight1 = bias2 + nn, weight2 = weight1 + (size_t)nn * ni, out_scaling = weight2 + (size_t)ni * nn;
+  const size_t total_scalars = out_scaling + 2 * (size_t)no;
+  const size_t total_bytes   = sizeof(CeedMiSgsDDContext) + total_scalars * sizeof(double);
+  CeedMiSgsDDContext *c      = calloc(1, total_bytes);
+  c->num_inputs              = ni;
+  c->num_outputs             = no;
+  c->num_layers              = 1;
+  c->num_neurons             = nn;
+  c->alpha                   = prm->alpha;
+  c->newt_ctx                = *p->ctx;
+  c->offsets.bias1           = bias1;
+  c->offsets.bias2           = bias2;
+  c->offsets.weight1         = weight1;
+  c->offsets.weight2         = weight2;
+  c->offsets.out_scaling     = out_scaling;
+  c->total_bytes             = total_bytes;
+  memcpy(&c->data[bias1], prm->b1, sizeof(double) * nn);
+  memcpy(&c->data[bias2], prm->b2, sizeof(double) * no);
+  memcpy(&c->data[out_scaling], prm->out_scaling, sizeof(double) * 2 * no);
+  for (int i = 0; i < ni; i++)  // TransposeMatrix(w1, N = num_inputs, M = num_neurons)
+    for (int j = 0; j < nn; j++) c->data[weight1 + (size_t)j * ni + i] = prm->w1[(size_t)i * nn + j];
+  for (int i = 0; i < nn; i++)  // TransposeMatrix(w2, N = num_neurons, M = num_outputs)
+    for (int j = 0; j < no; j++) c->data[weight2 + (size_t)j * nn + i] = prm->w2[(size_t)i * no + j];
+  CHK(p, CeedQFunctionContextCreate(p->ceed, &p->sgsdd_qfctx));
+  CHK(p, CeedQFunctionContextSetData(p->sgsdd_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, total_bytes, c));
+  CHK(p, CeedQFunctionContextSetDataDestroy(p->sgsdd_qfctx, CEED_MEM_HOST, FreeSgsContext));
+  return 0;
+}
+
+// SgsDDSetup (problems/sgs_dd_model.c:652-737) with the fused nodal evaluation (SgsDDSetupNodalEvaluation_Fused,
+// :114-203) and the nodal IFunction (SgsSetupNodalIFunction, :498-558)
+int HoneeMiProblemSetupSgsDD(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
+  if (p->sgs_enabled) return CEED_ERROR_MINOR;
+  CHK(p, VelocityGradientProjectionSetup(p));
+  CHK(p, SgsDDContextCreate(p, prm));
+  CHK(p, GridAnisotropyTensorProjection(p));
+  CHK(p, NodalField(p, 6, &p->elem_restr_sgs, &p->basis_sgs));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_sgs, &p->sgs_nodal, NULL));
+  CHK(p, InverseMultiplicity(p));
+  CHK(p, CeedBasisCreateProjection(p->basis_x, p->basis_q, &p->basis_x_to_q));
+  {
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kSgsNodalLoc[p->state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->sgsdd_qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "gradient velocity", 9, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "anisotropy tensor", 7, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "inverse multiplicity", 1, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "km_sgs", 6, CEED_EVAL_NONE));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_sgs_nodal));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "q", p->elem_restr_q, CEED_BASIS_NONE, p->q_ceed));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "x", p->elem_restr_x, p->basis_x_to_q, p->x_coord));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "gradient velocity", p->elem_restr_grad_velo, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "anisotropy tensor", p->elem_restr_grid_aniso, CEED_BASIS_NONE, p->grid_aniso));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "inverse multiplicity", p->elem_restr_inv_mult, CEED_BASIS_NONE, p->inv_mult));
+    CHK(p, CeedOperatorSetField(p->op_sgs_nodal, "km_sgs", p->elem_restr_sgs, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  {
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kSgsApplyLoc[p->state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "km_sgs", 6, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 15, CEED_EVAL_GRAD));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_sgs_apply));
+    CHK(p, CeedOperatorSetField(p->op_sgs_apply, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_sgs_apply, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(p->op_sgs_apply, "km_sgs", p->elem_restr_sgs, p->basis_sgs, p->sgs_nodal));
+    CHK(p, CeedOperatorSetField(p->op_sgs_apply, "Grad_v", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  p->sgs_enabled = 1;
+  return 0;
+}
+
+// VelocityGradientProjectionApply (src/velocity_gradient_projection.c:145-155): q_ceed holds the state
+static int VelocityGradientProjectionApply(HoneeMiProblem p) {
+  CHK(p, CeedOperatorApply(p->op_grad_velo_rhs, p->q_ceed, p->grad_velo, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorPointwiseMult(p->grad_velo, p->grad_velo, p->grad_velo_inv_rowsum));
+  return 0;
+}
+
+// SgsDDApplyIFunction (problems/sgs_dd_model.c:561-588): q_ceed holds the state, g_ceed the residual
+static int SgsDDApplyIFunction(HoneeMiProblem p) {
+  CHK(p, VelocityGradientProjectionApply(p));
+  CHK(p, CeedOperatorApply(p->op_sgs_nodal, p->grad_velo, p->sgs_nodal, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedOperatorApplyAdd(p->op_sgs_apply, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  return 0;
+}
+
+static int CopyVectorOut(HoneeMiProblem p, CeedVector v, int mem_type, double *out) {
+  CeedSize          n;
+  const CeedScalar *a;
+  CHK(p, CeedVectorGetLength(v, &n));
+  CHK(p, CeedVectorGetArrayRead(v, (CeedMemType)mem_type, &a));
+  const hipError_t herr = hipMemcpy(out, a, sizeof(double) * (size_t)n, mem_type == CEED_MEM_HOST ? hipMemcpyHostToHost : hipMemcpyDeviceToDevice);
+  CHK(p, CeedVectorRestoreArrayRead(v, &a));
+  if (herr != hipSuccess) {
+    snprintf(p->err, sizeof(p->err), "hipMemcpy failed: %d", (int)herr);
+    return CEED_ERROR_BACKEND;
+  }
+  return 0;
+}
+
+int HoneeMiSgsGridAnisotropy(HoneeMiProblem p, double *out, int *cg_iterations) {
+  if (!p->sgs_enabled) return CEED_ERROR_INCOMPATIBLE;
+  if (cg_iterations) *cg_iterations = p->grid_aniso_its;
+  return CopyVectorOut(p, p->grid_aniso, CEED_MEM_HOST, out);
+}
+
+int HoneeMiVelocityGradientProjection(HoneeMiProblem p, int mem_type, const double *q, double *grad_velo) {
+  CeedScalar *arr;
+  if (!p->sgs_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, VelocityGradientProjectionApply(p));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  return CopyVectorOut(p, p->grad_velo, mem_type, grad_velo);
+}
+
+int HoneeMiSgsNodalStress(HoneeMiProblem p, int mem_type, const double *q, const double *grad_velo, double *km_sgs) {
+  CeedScalar *arr;
+  if (!p->sgs_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->grad_velo, (CeedMemType)mem_type, CEED_COPY_VALUES, (CeedScalar *)grad_velo));
+  CHK(p, CeedOperatorApply(p->op_sgs_nodal, p->grad_velo, p->sgs_nodal, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  return CopyVectorOut(p, p->sgs_nodal, mem_type, km_sgs);
+}
+
+int HoneeMiApplySgsResidual(HoneeMiProblem p, int mem_type, const double *q, const double *km_sgs, double *g) {
+  CeedScalar *arr;
+  if (!p->sgs_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedVectorSetArray(p->sgs_nodal, (CeedMemType)mem_type, CEED_COPY_VALUES, (CeedScalar *)km_sgs));
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, g));
+  CHK(p, CeedOperatorApplyAdd(p->op_sgs_apply, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
+  return 0;
+}
+
 // IFunction_NS's operator section (src/setupts.c:90-111): update context, hand over arrays, apply, take arrays back.
 int HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g) {
   CeedScalar *arr;
@@ -467,6 +791,7 @@ int HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt
   CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q_dot));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, g));
   CHK(p, CeedOperatorApply(p->op_ifunction, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->sgs_enabled) CHK(p, SgsDDApplyIFunction(p));  // src/setupts.c:118-120
   CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
   CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mem_type, &arr));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
@@ -622,6 +947,23 @@ int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   if (!p) return 0;
   CeedOperatorDestroy(&p->op_ifunction);
   CeedOperatorDestroy(&p->op_ijacobian);
+  CeedOperatorDestroy(&p->op_grad_velo_rhs);
+  CeedOperatorDestroy(&p->op_sgs_nodal);
+  CeedOperatorDestroy(&p->op_sgs_apply);
+  CeedVectorDestroy(&p->grad_velo);
+  CeedVectorDestroy(&p->grad_velo_inv_rowsum);
+  CeedVectorDestroy(&p->grid_aniso);
+  CeedVectorDestroy(&p->sgs_nodal);
+  CeedVectorDestroy(&p->inv_mult);
+  CeedElemRestrictionDestroy(&p->elem_restr_grad_velo);
+  CeedElemRestrictionDestroy(&p->elem_restr_grid_aniso);
+  CeedElemRestrictionDestroy(&p->elem_restr_sgs);
+  CeedElemRestrictionDestroy(&p->elem_restr_inv_mult);
+  CeedBasisDestroy(&p->basis_grad_velo);
+  CeedBasisDestroy(&p->basis_grid_aniso);
+  CeedBasisDestroy(&p->basis_sgs);
+  CeedBasisDestroy(&p->basis_x_to_q);
+  if (p->sgsdd_qfctx) CeedQFunctionContextDestroy(&p->sgsdd_qfctx);
   CeedOperatorDestroy(&p->op_rhs);
   CeedOperatorDestroy(&p->op_mass);
   CeedOperatorDestroy(&p->op_monitor);

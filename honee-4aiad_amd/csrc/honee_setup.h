@@ -56,6 +56,19 @@ int          HoneeMiMonitorKE(HoneeMiProblem p, int mem_type, const double *q, d
 int          HoneeMiGetQData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData(HoneeMiProblem p, double *out);
 int          HoneeMiGetJacData32(HoneeMiProblem p, float *out);
+// Data-driven SGS model (problems/sgs_dd_model.c:652-737): the parameter arrays as the reference's .dat files hold
+// them (b1.dat [20], b2.dat [6], w1.dat [6 inputs][20 neurons], w2.dat [20 neurons][6 outputs], OutScaling.dat [6][2])
+// and -sgs_model_dd_leakyrelu_alpha.  Setup projects the grid anisotropy tensor (CG) and adds the SGS residual to
+// every HoneeMiApplyIFunction (implicit problems).
+typedef struct {
+  double        alpha;
+  const double *b1, *b2, *w1, *w2, *out_scaling;
+} HoneeMiSgsDDParams;
+int HoneeMiProblemSetupSgsDD(HoneeMiProblem p, const HoneeMiSgsDDParams *params);
+int HoneeMiSgsGridAnisotropy(HoneeMiProblem p, double *out, int *cg_iterations);  // [nnodes][7], host
+int HoneeMiVelocityGradientProjection(HoneeMiProblem p, int mem_type, const double *q, double *grad_velo);  // [nnodes][9]
+int HoneeMiSgsNodalStress(HoneeMiProblem p, int mem_type, const double *q, const double *grad_velo, double *km_sgs);  // [nnodes][6]
+int HoneeMiApplySgsResidual(HoneeMiProblem p, int mem_type, const double *q, const double *km_sgs, double *g);  // g += SGS residual
 const char  *HoneeMiProblemError(HoneeMiProblem p);
 const char  *HoneeMiLastError(void);
 

@@ -665,9 +665,9 @@ const char *HoneeMiTSError(HoneeMiTS ts) { return ts ? ts->err.c_str() : g_ts_er
 int HoneeMiTSDestroy(HoneeMiTS *pts) {
   HoneeMiTS ts = *pts;
   if (!ts) return 0;
-  if (ts->s) hipStreamSynchronize(ts->s);
-  for (void *a : ts->allocs) hipFree(a);
-  if (ts->h_red) hipHostFree(ts->h_red);
+  if (ts->s) (void)hipStreamSynchronize(ts->s);
+  for (void *a : ts->allocs) (void)hipFree(a);
+  if (ts->h_red) (void)hipHostFree(ts->h_red);
   delete ts;
   *pts = nullptr;
   return 0;

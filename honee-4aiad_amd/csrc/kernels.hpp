@@ -49,6 +49,43 @@ struct VolumeArgs {
   const double *Bx;        // [Q][2]
 };
 
+// Volume operator whose fields have their own restrictions (SGS closure inputs and residual, Mass_N): an active input
+// u (in_offsets), a passive interpolated input u2 (in2_offsets), strided qdata, and the output v (out_offsets), or the
+// two-pass E-vector ev with slots perm[] of the output restriction's transpose map.
+struct FieldArgs {
+  int           nelem;
+  const int    *in_offsets;
+  int           in_comp_stride;
+  const double *u;
+  const int    *in2_offsets;
+  int           in2_comp_stride;
+  const double *u2;
+  const int    *out_offsets;
+  int           out_comp_stride;
+  double       *v;
+  double       *ev;
+  const int    *perm;
+  const double *qdata;
+  QDataLayout   qd;
+  BasisTables   bt;
+};
+
+// Nodal SGS closure: one evaluation per row k of the output restriction's transpose map (rowptr, index, row_offset),
+// summed over the row's element nodes with their inverse multiplicities (strided layout im, elem_size nodes each).
+struct SgsNodalArgs {
+  long long     nrows;
+  const int    *rowptr, *index, *row_offset;
+  int           out_comp_stride;
+  double       *v;
+  const int    *q_offsets, *g_offsets, *a_offsets;
+  int           q_comp_stride, g_comp_stride, a_comp_stride;
+  const double *q, *g, *aniso;
+  const double *inv_mult;
+  QDataLayout   im;
+  int           elem_size;
+  const double *model;  // b1[20], b2[6], W1[20][6], W2[6][20], bounds[6][2], alpha
+};
+
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream);
 bool       volume_pq_supported(int P, int Q);
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);
@@ -89,3 +126,17 @@ hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int ele
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
                               const double *ev, double *v, bool overwrite, hipStream_t s);
 hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u, hipStream_t s);
+
+// SGS closure and projections (kernels_sgs.hip)
+bool       sgs_pq_supported(int P, int Q);
+bool       mass_n_supported(int ncomp);
+size_t     sgs_model_doubles();
+hipError_t launch_velocity_gradient_rhs(int P, int Q, int var, const FieldArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
+hipError_t launch_anisotropy_rhs(int P, int Q, const FieldArgs &a, hipStream_t s);
+hipError_t launch_mass_n(int P, int Q, int ncomp, const FieldArgs &a, hipStream_t s);
+hipError_t launch_mass_n_diagonal(int P, int Q, int ncomp, const FieldArgs &a, hipStream_t s);
+hipError_t launch_ifunction_nodal_sgs(int P, int Q, int var, const FieldArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
+hipError_t launch_inverse_multiplicity(const int *offsets, int nelem, int elem_size, const double *mult, double *out, QDataLayout ol, hipStream_t s);
+hipError_t launch_sgs_nodal(int var, const SgsNodalArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
+// Deterministic dot product: npartial fixed per-block partial sums (summed in order by the caller)
+hipError_t launch_dot(const double *x, const double *y, long long n, double *d_partial, int npartial, hipStream_t s);

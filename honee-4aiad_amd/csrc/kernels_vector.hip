@@ -128,6 +128,20 @@ __global__ void __launch_bounds__(256) k_node_gather(int nrows, int ncomp, int c
   }
 }
 
+// Per-block partial sums of x . y over a fixed grid (deterministic for a given n and npartial)
+__global__ void k_dot(const double *x, const double *y, long long n, double *partial) {
+  __shared__ double red[kBlock];
+  double            acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) acc += x[i] * y[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 }  // namespace
 
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
@@ -168,6 +182,10 @@ hipError_t launch_pointwise_mult(double *w, const double *x, const double *y, lo
 hipError_t launch_reciprocal(double *x, long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, x, n);
+  return hipGetLastError();
+}
+hipError_t launch_dot(const double *x, const double *y, long long n, double *d_partial, int npartial, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot, dim3(npartial), dim3(kBlock), 0, s, x, y, n, d_partial);
   return hipGetLastError();
 }
 hipError_t launch_norm(const double *x, long long n, int type, double *d_partial, int npartial, hipStream_t s) {

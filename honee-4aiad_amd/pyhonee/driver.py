@@ -129,6 +129,11 @@ def load_libs():
         lib.HoneeMiGetQData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData.argtypes = [ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetJacData32.argtypes = [ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiProblemSetupSgsDD.argtypes = [ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiSgsGridAnisotropy.argtypes = [ct.c_void_p, ct.c_void_p, _ip]
+        lib.HoneeMiVelocityGradientProjection.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiSgsNodalStress.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiApplySgsResidual.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         c = _libs["ceed"]
         c.CeedMiSetKernelTiming.argtypes = [ct.c_void_p, ct.c_bool]
         c.CeedMiResetKernelTimes.argtypes = [ct.c_void_p]
@@ -136,6 +141,10 @@ def load_libs():
         c.CeedMiSynchronize.argtypes = [ct.c_void_p]
         _libs["setup"] = lib
     return _libs["ceed"], _libs["setup"]
+
+
+class HoneeMiSgsDDParams(ct.Structure):
+    _fields_ = [("alpha", ct.c_double), ("b1", _dp), ("b2", _dp), ("w1", _dp), ("w2", _dp), ("out_scaling", _dp)]
 
 
 def _ptr(a):
@@ -222,6 +231,33 @@ class NewtonianProblem:
 
     def mass_point_block_diagonal(self, q, pb, mem=CEED_MEM_HOST):
         self._check(self.lib.HoneeMiMassPointBlockDiagonal(self.handle, mem, _ptr(q), _ptr(pb)), "Mass point-block diagonal")
+
+    def setup_sgs_dd(self, model: dict):
+        """Enable the data-driven SGS model (problems/sgs_dd_model.c:652-737) with the parameters of
+        pyhonee.sgs.load_dd_model; every later IFunction adds the SGS residual."""
+        arrs = dict(b1=np.ascontiguousarray(model["b1"], dtype=np.float64), b2=np.ascontiguousarray(model["b2"], dtype=np.float64),
+                    w1=np.ascontiguousarray(np.asarray(model["W1"]).T, dtype=np.float64),  # w1.dat order [input][neuron]
+                    w2=np.ascontiguousarray(np.asarray(model["W2"]).T, dtype=np.float64),  # w2.dat order [neuron][output]
+                    out_scaling=np.ascontiguousarray(model["bounds"], dtype=np.float64))
+        prm = HoneeMiSgsDDParams(float(model["alpha"]), *(arrs[k].ctypes.data_as(_dp) for k in ("b1", "b2", "w1", "w2", "out_scaling")))
+        self._check(self.lib.HoneeMiProblemSetupSgsDD(self.handle, ct.byref(prm)), "SetupSgsDD")
+
+    def sgs_grid_anisotropy(self):
+        """Projected grid anisotropy field [nnodes][7] (KM tensor, delta) and the CG iteration count."""
+        out = np.zeros((self.nnodes, 7))
+        its = ct.c_int(0)
+        self._check(self.lib.HoneeMiSgsGridAnisotropy(self.handle, _ptr(out), ct.byref(its)), "SgsGridAnisotropy")
+        return out, its.value
+
+    def velocity_gradient_projection(self, q, grad_velo, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiVelocityGradientProjection(self.handle, mem, _ptr(q), _ptr(grad_velo)), "VelocityGradientProjection")
+
+    def sgs_nodal_stress(self, q, grad_velo, km_sgs, mem=CEED_MEM_HOST):
+        self._check(self.lib.HoneeMiSgsNodalStress(self.handle, mem, _ptr(q), _ptr(grad_velo), _ptr(km_sgs)), "SgsNodalStress")
+
+    def apply_sgs_residual(self, q, km_sgs, g, mem=CEED_MEM_HOST):
+        """g += the SGS residual of IFunction_NodalSgs for the nodal stress km_sgs."""
+        self._check(self.lib.HoneeMiApplySgsResidual(self.handle, mem, _ptr(q), _ptr(km_sgs), _ptr(g)), "ApplySgsResidual")
 
     BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
 

@@ -58,6 +58,24 @@ typedef struct {
   double                 u[3];
 } CeedMiTaylorGreenContext;
 
+// struct SgsDDContext_ (qfunctions/sgs_dd_model.h:15-31): the data-driven SGS model, its parameters in data[] at the
+// given offsets (bias1[num_neurons], bias2[num_outputs], weight1[num_neurons][num_inputs],
+// weight2[num_outputs][num_neurons], out_scaling[num_outputs][2]); total_bytes is the size of the whole allocation.
+typedef struct {
+  CeedInt                num_inputs, num_outputs;
+  CeedInt                num_layers;
+  CeedInt                num_neurons;
+  double                 alpha;
+  CeedMiNewtonianContext newt_ctx;
+  struct {
+    size_t bias1, bias2;
+    size_t weight1, weight2;
+    size_t out_scaling;
+  } offsets;
+  size_t total_bytes;
+  double data[1];
+} CeedMiSgsDDContext;
+
 // ---- Backend extensions (no counterpart in libCEED; used by this repository's driver, tests and benchmark) ----
 
 // HIP stream the backend launches on (a hipStream_t).
@@ -72,6 +90,10 @@ int CeedMiGetKernelTimes(Ceed ceed, int max_kernels, char *names, size_t names_l
 
 // Device pointer of a vector's FP32 shadow array (the jac_data mirror written in jacobian_precision=fp32 mode).
 int CeedMiVectorGetFp32ShadowRead(CeedVector vec, const float **d_array, CeedSize *length);
+
+// Dot product x . y of two vectors of equal length, reduced on the device over a fixed grid and summed on the host in
+// a fixed order (the same result for the same inputs on every call).
+int CeedMiVectorDot(CeedVector x, CeedVector y, double *result);
 
 // Synchronize the backend stream.
 int CeedMiSynchronize(Ceed ceed);
