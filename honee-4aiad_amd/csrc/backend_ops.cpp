@@ -357,6 +357,128 @@ static int bind_sgs_op(CeedOperator op, OpPlan *p) {
       p->kernel_name = "ifunction_nodal_sgs";
       return 0;
     }
+    case QfKind::DiffFilterRHS: {  // src/differential_filter.c:51-88
+      const char  *names[3] = {"q", "qdata", "x"};
+      CeedEvalMode modes[3] = {CEED_EVAL_INTERP, CEED_EVAL_NONE, CEED_EVAL_INTERP};
+      CeedInt      sizes[3] = {5, 10, 3};
+      BIND_CHECK(qf->inputs.size() == 3 && qf->outputs.size() == 2, "%s: unexpected fields (filtered state and velocity products)", qf->name.c_str());
+      for (int i = 0; i < 3; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+      MI_CALL(check_field(op, 0, false, "v0", CEED_EVAL_INTERP, 5));
+      MI_CALL(check_field(op, 1, false, "v1", CEED_EVAL_INTERP, 6));
+      MI_CALL(bind_field_basis(op, p, fi[0], 5));
+      p->r_q = fi[0]->rstr;
+      p->b_q = fi[0]->basis;
+      BIND_CHECK(is_active(fi[0]->vec), "%s: q must be active", qf->name.c_str());
+      MI_CALL(bind_qdata_in(op, p, fi[1]));
+      MI_CALL(bind_coords(op, p, fi[2]));
+      BIND_CHECK(is_active(fo[0]->vec) && is_active(fo[1]->vec), "%s: v0 and v1 must be active", qf->name.c_str());
+      MI_CALL(bind_field_basis(op, p, fo[0], 5));
+      MI_CALL(bind_field_basis(op, p, fo[1], 6));
+      {  // the two filtered fields interleave per node: [5 state | 6 products], one 11-component output
+        CeedElemRestriction r0 = fo[0]->rstr, r1 = fo[1]->rstr;
+        bool                ok = r0->comp_stride == 1 && r1->comp_stride == 1 && r0->l_size == r1->l_size && r0->h_offsets.size() == r1->h_offsets.size();
+        for (size_t i = 0; ok && i < r0->h_offsets.size(); i++) ok = r1->h_offsets[i] == r0->h_offsets[i] + 5;
+        BIND_CHECK(ok, "%s: v1 must follow v0 in each node of one interleaved vector (offsets + 5, component stride 1)", qf->name.c_str());
+      }
+      p->r_out       = fo[0]->rstr;
+      p->b_out       = fo[0]->basis;
+      p->ncomp_out   = 11;
+      p->kernel_name = "diff_filter_rhs";
+      return 0;
+    }
+    case QfKind::DiffFilterLHS: {  // src/differential_filter.c:90-149
+      const int    N        = qf->ncomp;
+      const char  *names[5] = {"q", "Grad_q", "anisotropy tensor", "x", "qdata"};
+      CeedEvalMode modes[5] = {CEED_EVAL_INTERP, CEED_EVAL_GRAD, CEED_EVAL_NONE, CEED_EVAL_INTERP, CEED_EVAL_NONE};
+      CeedInt      sizes[5] = {N, 3 * N, 7, 3, 10};
+      BIND_CHECK(diff_filter_ncomp_supported(N), "%s: no fused kernel for %d components", qf->name.c_str(), N);
+      BIND_CHECK(qf->inputs.size() == 5 && qf->outputs.size() == 2, "%s: unexpected fields", qf->name.c_str());
+      for (int i = 0; i < 5; i++) MI_CALL(check_field(op, i, true, names[i], modes[i], sizes[i]));
+      MI_CALL(check_field(op, 0, false, "v", CEED_EVAL_INTERP, N));
+      MI_CALL(check_field(op, 1, false, "Grad_v", CEED_EVAL_GRAD, 3 * N));
+      MI_CALL(bind_field_basis(op, p, fi[0], N));
+      p->r_q = p->r_out = fi[0]->rstr;
+      p->b_q = p->b_out = fi[0]->basis;
+      BIND_CHECK(is_active(fi[0]->vec) && is_active(fi[1]->vec) && fi[1]->rstr == p->r_q && fi[1]->basis == p->b_q,
+                 "%s: q and Grad_q must share the active restriction/basis", qf->name.c_str());
+      BIND_CHECK(is_strided(fi[2]->rstr) && !is_active(fi[2]->vec) && fi[2]->vec, "%s: the anisotropy tensor must be a passive strided field",
+                 qf->name.c_str());
+      MI_CALL(check_point_layout(op, fi[2]->rstr, p->Q, false));
+      BIND_CHECK(fi[2]->rstr->num_elem == p->nelem && fi[2]->rstr->num_comp == 7 && fi[2]->vec->length >= (CeedSize)p->nelem * 7 * p->Q * p->Q * p->Q,
+                 "%s: the anisotropy tensor must hold 7 components at every point", qf->name.c_str());
+      p->r_aux = fi[2]->rstr;
+      p->v_aux = fi[2]->vec;
+      MI_CALL(bind_coords(op, p, fi[3]));
+      MI_CALL(bind_qdata_in(op, p, fi[4]));
+      BIND_CHECK(is_active(fo[0]->vec) && is_active(fo[1]->vec) && fo[0]->rstr == p->r_q && fo[1]->rstr == p->r_q && fo[0]->basis == p->b_q &&
+                     fo[1]->basis == p->b_q,
+                 "%s: v and Grad_v must be active on the filtered field", qf->name.c_str());
+      p->ncomp_out   = N;
+      p->kernel_name = "diff_filter_lhs";
+      return 0;
+    }
+    case QfKind::AnisotropyCollocate: {  // src/grid_anisotropy_tensor.c:114-160
+      BIND_CHECK(qf->inputs.size() == 1 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      MI_CALL(check_field(op, 0, true, "qdata", CEED_EVAL_NONE, 10));
+      MI_CALL(check_field(op, 0, false, "v", CEED_EVAL_NONE, 7));
+      BIND_CHECK(is_strided(fi[0]->rstr) && !is_active(fi[0]->vec), "%s: qdata must be a passive strided field", qf->name.c_str());
+      BIND_CHECK(is_strided(fo[0]->rstr) && is_active(fo[0]->vec) && fo[0]->rstr->num_comp == 7, "%s: output must be an active strided field",
+                 qf->name.c_str());
+      int q = 1;
+      while (q * q * q < fi[0]->rstr->elem_size) q++;
+      BIND_CHECK(q * q * q == fi[0]->rstr->elem_size && fo[0]->rstr->elem_size == fi[0]->rstr->elem_size &&
+                     fo[0]->rstr->num_elem == fi[0]->rstr->num_elem,
+                 "%s: qdata and output must share the element points", qf->name.c_str());
+      p->Q     = q;
+      p->nelem = fi[0]->rstr->num_elem;
+      MI_CALL(bind_qdata_in(op, p, fi[0]));
+      p->r_out       = fo[0]->rstr;
+      p->kernel_name = "anisotropy_collocate";
+      return 0;
+    }
+    case QfKind::SgsTrainingNodal: {  // src/smartsim/sgs_dd_training.c:93-180
+      const char  *names[5] = {"q", "velocity product", "gradient velocity", "anisotropy tensor", "inverse multiplicity"};
+      CeedInt      sizes[5] = {5, 6, 9, 7, 1};
+      BIND_CHECK(qf->inputs.size() == 5 && qf->outputs.size() == 1, "%s: unexpected fields", qf->name.c_str());
+      for (int i = 0; i < 5; i++) MI_CALL(check_field(op, i, true, names[i], CEED_EVAL_NONE, sizes[i]));
+      MI_CALL(check_field(op, 0, false, "training data", CEED_EVAL_NONE, 12));
+      BIND_CHECK(is_offsets(fi[0]->rstr) && is_offsets(fi[1]->rstr) && !is_active(fi[0]->vec) && fi[0]->vec && fi[1]->vec == fi[0]->vec,
+                 "%s: q and the velocity products must be passive nodal fields of one filtered vector", qf->name.c_str());
+      BIND_CHECK(is_offsets(fi[2]->rstr) && is_active(fi[2]->vec), "%s: the velocity gradient must be the active nodal input", qf->name.c_str());
+      BIND_CHECK(is_offsets(fi[3]->rstr) && !is_active(fi[3]->vec) && fi[3]->vec, "%s: the anisotropy tensor must be a passive nodal field",
+                 qf->name.c_str());
+      BIND_CHECK(is_strided(fi[4]->rstr) && !is_active(fi[4]->vec) && fi[4]->vec, "%s: the inverse multiplicity must be a passive strided field",
+                 qf->name.c_str());
+      BIND_CHECK(is_offsets(fo[0]->rstr) && is_active(fo[0]->vec), "%s: the training data must be the active nodal output", qf->name.c_str());
+      p->r_out  = fo[0]->rstr;
+      p->r_q    = fi[2]->rstr;
+      p->r_aux  = fi[0]->rstr;
+      p->v_aux  = fi[0]->vec;
+      p->r_in2  = fi[1]->rstr;
+      p->r_aux2 = fi[3]->rstr;
+      p->v_aux2 = fi[3]->vec;
+      p->r_im   = fi[4]->rstr;
+      p->v_im   = fi[4]->vec;
+      p->nelem  = p->r_out->num_elem;
+      const CeedInt             esz    = p->r_out->elem_size;
+      const CeedElemRestriction ins[5] = {p->r_q, p->r_aux, p->r_in2, p->r_aux2, p->r_im};
+      for (CeedElemRestriction r : ins)
+        BIND_CHECK(r->num_elem == p->nelem && r->elem_size == esz, "%s: nodal fields must share the element nodes of the output", qf->name.c_str());
+      {
+        const auto                          &oo = p->r_out->h_offsets;
+        std::vector<std::pair<CeedInt, int>> first(oo.size());
+        for (size_t i = 0; i < oo.size(); i++) first[i] = {oo[i], (int)i};
+        std::sort(first.begin(), first.end());
+        for (size_t j = 1; j < first.size(); j++) {
+          if (first[j].first != first[j - 1].first) continue;
+          const int i0 = first[j - 1].second, i1 = first[j].second;
+          for (CeedElemRestriction r : {p->r_q, p->r_aux, p->r_in2, p->r_aux2})
+            BIND_CHECK(r->h_offsets[i0] == r->h_offsets[i1], "%s: element nodes sharing an output node read different input nodes", qf->name.c_str());
+        }
+      }
+      p->kernel_name = "sgs_training_nodal";
+      return 0;
+    }
     default:
       break;
   }
@@ -579,6 +701,10 @@ static int plan_bind_fields(CeedOperator op) {
     case QfKind::InverseMultiplicity:
     case QfKind::SgsNodal:
     case QfKind::NodalSgsResidual:
+    case QfKind::DiffFilterRHS:
+    case QfKind::DiffFilterLHS:
+    case QfKind::AnisotropyCollocate:
+    case QfKind::SgsTrainingNodal:
       if ((ierr = bind_sgs_op(op, p))) return fail(ierr);
       return 0;
     default:
@@ -687,6 +813,30 @@ static int sgs_model_upload(CeedOperator op, OpPlan *p, CeedMiNewtonianContext *
   return 0;
 }
 
+// struct DifferentialFilterContext_ (qfunctions/differential_filter.h:16-25)
+static int diff_filter_context(CeedOperator op, CeedMiDiffFilterContext *ctx) {
+  CeedQFunctionContext c = op->qf->ctx;
+  if (!c || !c->data) return MI_ERR(op->ceed, CEED_ERROR_INCOMPLETE, "%s: missing QFunction context", op->qf->name.c_str());
+  if (c->size != sizeof(CeedMiDiffFilterContext))
+    return MI_ERR(op->ceed, CEED_ERROR_INCOMPATIBLE, "%s: context size %zu does not match struct DifferentialFilterContext_ (%zu)",
+                  op->qf->name.c_str(), c->size, sizeof(CeedMiDiffFilterContext));
+  memcpy(ctx, c->data, sizeof(*ctx));
+  if (ctx->damping_function < 0 || ctx->damping_function > 2)
+    return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: unknown wall damping function %d", op->qf->name.c_str(), ctx->damping_function);
+  return 0;
+}
+
+// Collocated anisotropy tensor and coordinate field of the filter operator
+static int filter_aux_args(OpPlan *p, FieldArgs &a) {
+  MI_CALL(vec_device_read(p->v_aux, &a.aux));
+  a.auxl = strided_layout(p->r_aux);
+  MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
+  a.xcomp_stride = p->r_x->comp_stride;
+  MI_CALL(vec_device_read(p->v_x, &a.x));
+  a.Bx = p->b_x->d_interp_1d;
+  return 0;
+}
+
 static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector out, bool overwrite) {
   Ceed        ceed = op->ceed;
   hipStream_t s    = ceed->stream;
@@ -702,6 +852,53 @@ static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector
     MI_CALL(vec_device_write(out, &d_out, true));
     MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
     MI_HIP(ceed, launch_inverse_multiplicity(off, p->nelem, p->r_out->elem_size, d_in, d_out, strided_layout(p->r_out), s));
+    MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+    return 0;
+  }
+  if (p->kind == QfKind::AnisotropyCollocate) {
+    if (out->length < (CeedSize)p->nelem * 7 * p->Q * p->Q * p->Q)
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: output vector too short", op->qf->name.c_str());
+    const double *qd;
+    double       *d_out;
+    MI_CALL(vec_device_read(p->v_qdata, &qd));
+    MI_CALL(vec_device_write(out, &d_out, true));
+    MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+    MI_HIP(ceed, launch_anisotropy_collocate(p->Q, p->nelem, qd, p->qd, d_out, strided_layout(p->r_out), s));
+    MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+    return 0;
+  }
+  if (p->kind == QfKind::SgsTrainingNodal) {
+    CeedQFunctionContext c = op->qf->ctx;
+    if (!c || !c->data || c->size != sizeof(CeedMiSgsTrainingContext))
+      return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: context must be struct SGS_DD_TrainingContext_", op->qf->name.c_str());
+    CeedMiNewtonianContext ctx = ((const CeedMiSgsTrainingContext *)c->data)->newt_ctx;
+    if (in->length != p->r_q->l_size || out->length != p->r_out->l_size)
+      return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: active vectors must have the restrictions' L-vector sizes", op->qf->name.c_str());
+    CeedElemRestriction r = p->r_out;
+    MI_CALL(rstr_transpose_map(r));
+    SgsNodalArgs a{};
+    a.nrows           = r->t_nrows;
+    a.rowptr          = r->d_t_rowptr;
+    a.index           = r->d_t_index;
+    a.row_offset      = r->d_t_offset;
+    a.out_comp_stride = r->comp_stride;
+    MI_CALL(vec_device_write(out, &a.v, true));
+    MI_CALL(rstr_device_offsets(p->r_aux, &a.q_offsets));
+    MI_CALL(rstr_device_offsets(p->r_in2, &a.p_offsets));
+    MI_CALL(rstr_device_offsets(p->r_q, &a.g_offsets));
+    MI_CALL(rstr_device_offsets(p->r_aux2, &a.a_offsets));
+    a.q_comp_stride = p->r_aux->comp_stride;
+    a.p_comp_stride = p->r_in2->comp_stride;
+    a.g_comp_stride = p->r_q->comp_stride;
+    a.a_comp_stride = p->r_aux2->comp_stride;
+    MI_CALL(vec_device_read(p->v_aux, &a.q));
+    MI_CALL(vec_device_read(in, &a.g));
+    MI_CALL(vec_device_read(p->v_aux2, &a.aniso));
+    MI_CALL(vec_device_read(p->v_im, &a.inv_mult));
+    a.im        = strided_layout(p->r_im);
+    a.elem_size = r->elem_size;
+    MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
+    MI_HIP(ceed, launch_sgs_training_nodal(a, ctx, s));
     MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
     return 0;
   }
@@ -737,10 +934,13 @@ static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector
     MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
     return 0;
   }
-  CeedMiNewtonianContext ctx{};
+  CeedMiNewtonianContext  ctx{};
+  CeedMiDiffFilterContext fctx{};
   if (p->kind == QfKind::VelocityGradientRHS || p->kind == QfKind::NodalSgsResidual) MI_CALL(newtonian_context(op, &ctx));
+  if (p->kind == QfKind::DiffFilterRHS || p->kind == QfKind::DiffFilterLHS) MI_CALL(diff_filter_context(op, &fctx));
   FieldArgs a{};
   a.nelem = p->nelem;
+  if (p->kind == QfKind::DiffFilterLHS) MI_CALL(filter_aux_args(p, a));
   if (p->kind != QfKind::AnisotropyRHS) {
     if (in->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: input must have the L-vector size", op->qf->name.c_str());
     MI_CALL(rstr_device_offsets(p->r_q, &a.in_offsets));
@@ -777,6 +977,12 @@ static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector
       break;
     case QfKind::MassN:
       MI_HIP(ceed, launch_mass_n(p->P, p->Q, p->ncomp_out, a, s));
+      break;
+    case QfKind::DiffFilterRHS:
+      MI_HIP(ceed, launch_diff_filter_rhs(p->P, p->Q, p->var, a, fctx, s));
+      break;
+    case QfKind::DiffFilterLHS:
+      MI_HIP(ceed, launch_diff_filter_lhs(p->P, p->Q, p->ncomp_out, a, fctx, s));
       break;
     default:
       MI_HIP(ceed, launch_ifunction_nodal_sgs(p->P, p->Q, p->var, a, ctx, s));
@@ -972,6 +1178,10 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
     case QfKind::InverseMultiplicity:
     case QfKind::SgsNodal:
     case QfKind::NodalSgsResidual:
+    case QfKind::DiffFilterRHS:
+    case QfKind::DiffFilterLHS:
+    case QfKind::AnisotropyCollocate:
+    case QfKind::SgsTrainingNodal:
       return sgs_field_apply(op, p, in, out, overwrite);
     default:
       return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "operator kind not executable");
@@ -1001,6 +1211,27 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     MI_CALL(timing_begin(ceed, "mass_n_diagonal", &ev));
     MI_HIP(ceed, launch_mass_n_diagonal(p->P, p->Q, p->ncomp_out, a, ceed->stream));
     MI_CALL(timing_end(ceed, "mass_n_diagonal", ev));
+    return 0;
+  }
+  if (p->kind == QfKind::DiffFilterLHS) {  // Jacobi diagonal of the filter solve (PCJACOBI, src/differential_filter.c:178-181)
+    if (point_block) return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "point-block diagonal of '%s' is not provided", op->qf->name.c_str());
+    if (diag->length != p->r_q->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
+    CeedMiDiffFilterContext fctx;
+    MI_CALL(diff_filter_context(op, &fctx));
+    FieldArgs a{};
+    a.nelem = p->nelem;
+    MI_CALL(filter_aux_args(p, a));
+    MI_CALL(rstr_device_offsets(p->r_q, &a.out_offsets));
+    a.out_comp_stride = p->r_q->comp_stride;
+    MI_CALL(vec_device_write(diag, &a.v, true));
+    MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
+    a.qd         = p->qd;
+    const int PQ = p->P * p->Q;
+    a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+    hipEvent_t ev;
+    MI_CALL(timing_begin(ceed, "diff_filter_lhs_diagonal", &ev));
+    MI_HIP(ceed, launch_diff_filter_lhs_diagonal(p->P, p->Q, p->ncomp_out, a, fctx, ceed->stream));
+    MI_CALL(timing_end(ceed, "diff_filter_lhs_diagonal", ev));
     return 0;
   }
   const CeedSize want = point_block ? 5 * p->r_q->l_size : p->r_q->l_size;

@@ -1088,6 +1088,20 @@ static void qf_identify(CeedQFunction qf) {
   } else if (n.rfind("IFunction_NodalSgs_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::NodalSgsResidual;
     qf->state_var = var_of(n);
+  } else if (n.rfind("DifferentialFilter_RHS_", 0) == 0 && var_of(n) >= 0) {  // qfunctions/differential_filter.h:54-64
+    qf->kind      = QfKind::DiffFilterRHS;
+    qf->state_var = var_of(n);
+  } else if (n.rfind("DifferentialFilter_LHS_", 0) == 0) {  // qfunctions/differential_filter.h:140-154
+    const std::string tail = n.substr(23);
+    if (tail == "1" || tail == "5" || tail == "6" || tail == "11") {
+      qf->kind  = QfKind::DiffFilterLHS;
+      qf->ncomp = std::stoi(tail);
+    }
+  } else if (n == "AnisotropyTensorCollocate") {
+    qf->kind = QfKind::AnisotropyCollocate;
+  } else if (n == "ComputeSGS_DDAnisotropicTrainingDataNodal_Prim") {
+    qf->kind      = QfKind::SgsTrainingNodal;
+    qf->state_var = 1;
   } else if (n.rfind("MonitorTotalKineticEnergy_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::MonitorKE;
     qf->state_var = var_of(n);

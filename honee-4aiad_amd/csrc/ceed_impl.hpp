@@ -105,6 +105,10 @@ enum class QfKind {
   InverseMultiplicity,
   SgsNodal,
   NodalSgsResidual,
+  DiffFilterRHS,
+  DiffFilterLHS,
+  AnisotropyCollocate,
+  SgsTrainingNodal,
 };
 
 struct CeedQFunction_private {
@@ -119,7 +123,7 @@ struct CeedQFunction_private {
   QfKind               kind       = QfKind::Unknown;
   int                  state_var  = 0;
   bool                 hllc       = false;  // Freestream*: HLLC (true) or HLL Riemann flux
-  int                  ncomp      = 0;      // Mass_N: N
+  int                  ncomp      = 0;      // Mass_N, DifferentialFilter_LHS_N: N
 };
 
 struct CtxField {

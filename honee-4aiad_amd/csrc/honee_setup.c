@@ -69,6 +69,16 @@ struct HoneeMiProblem_ {
   CeedVector            grad_velo, grad_velo_inv_rowsum, grid_aniso, sgs_nodal, inv_mult;
   CeedOperator          op_grad_velo_rhs, op_sgs_nodal, op_sgs_apply;
   CeedQFunctionContext  sgsdd_qfctx;
+  // differential filter (src/differential_filter.c) and SGS training pairs (src/smartsim/sgs_dd_training.c)
+  int                   filter_enabled, filter_pc_ready, train_enabled, filter_its;
+  double                filter_rtol, filter_atol;
+  int                   filter_max_it;
+  CeedElemRestriction   elem_restr_filter[2], elem_restr_aniso_col, elem_restr_train;
+  CeedBasis             basis_filter[2];
+  CeedVector            filter_rhs, filtered, filter_dinv, aniso_col, train_data, cg_work[3];
+  CeedOperator          op_filter_rhs, op_filter_lhs, op_filtered_grad_velo_rhs, op_sgs_train;
+  CeedQFunctionContext  diff_filter_qfctx, train_qfctx;
+  CeedContextFieldLabel filter_width_label;
   char                  err[512];
 };
 
@@ -522,23 +532,30 @@ static int JacobiInverse(HoneeMiProblem p, CeedVector d) {
 
 // VelocityGradientProjectionSetup (src/velocity_gradient_projection.c:36-130): RHS operator and the lumped mass
 // (PCJACOBI row sum of Mass_9, KSPPREONLY), i.e. G = R ./ (M 1)
-static int VelocityGradientProjectionSetup(HoneeMiProblem p) {
+// The velocity-gradient RHS operator on a 5-component state field (restriction/basis), state variables var
+static int VelocityGradientRHSOperator(HoneeMiProblem p, int var, CeedElemRestriction restr, CeedBasis basis, CeedOperator *op) {
   CeedQFunction qf;
-  CHK(p, NodalField(p, 9, &p->elem_restr_grad_velo, &p->basis_grad_velo));
-  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kVelocityGradientRHSLoc[p->state_var], &qf));
+  CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kVelocityGradientRHSLoc[var], &qf));
   CHK(p, CeedQFunctionSetContext(qf, p->newt_qfctx));
   CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
   CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 15, CEED_EVAL_GRAD));
   CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
   CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
   CHK(p, CeedQFunctionAddOutput(qf, "velocity gradient", 9, CEED_EVAL_INTERP));
-  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_grad_velo_rhs));
-  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
-  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "Grad_q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
-  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
-  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "x", p->elem_restr_x, p->basis_x, p->x_coord));
-  CHK(p, CeedOperatorSetField(p->op_grad_velo_rhs, "velocity gradient", p->elem_restr_grad_velo, p->basis_grad_velo, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, op));
+  CHK(p, CeedOperatorSetField(*op, "q", restr, basis, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(*op, "Grad_q", restr, basis, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(*op, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+  CHK(p, CeedOperatorSetField(*op, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+  CHK(p, CeedOperatorSetField(*op, "velocity gradient", p->elem_restr_grad_velo, p->basis_grad_velo, CEED_VECTOR_ACTIVE));
   CHK(p, CeedQFunctionDestroy(&qf));
+  return 0;
+}
+
+static int VelocityGradientProjectionSetup(HoneeMiProblem p) {
+  if (p->grad_velo) return 0;
+  CHK(p, NodalField(p, 9, &p->elem_restr_grad_velo, &p->basis_grad_velo));
+  CHK(p, VelocityGradientRHSOperator(p, p->state_var, p->elem_restr_q, p->basis_q, &p->op_grad_velo_rhs));
   CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grad_velo, &p->grad_velo, NULL));
   {  // PCJacobiSetType(PC_JACOBI_ROWSUM): MatGetRowSum = M 1
     CeedOperator op_mass;
@@ -555,13 +572,44 @@ static int VelocityGradientProjectionSetup(HoneeMiProblem p) {
   return 0;
 }
 
+// KSPCG with PCJACOBI (inverse diagonal dinv) and KSP_NORM_NATURAL from a zero initial guess: converged when
+// sqrt(r.z) <= max(rtol sqrt(r0.z0), atol), at most max_it iterations (KSPSolve_CG recurrences).  work holds three
+// vectors of the system size.
+static int CgJacobi(HoneeMiProblem p, CeedOperator op, CeedVector dinv, CeedVector R, CeedVector X, CeedVector work[3], CeedVector Ap, double rtol,
+                    double atol, int max_it, int *iterations) {
+  CeedVector r = work[0], z = work[1], pv = work[2];
+  double     rz, pAp, rz_new;
+  CHK(p, CeedVectorSetValue(X, 0.0));
+  CHK(p, CeedVectorCopy(R, r));
+  CHK(p, CeedVectorPointwiseMult(z, dinv, r));
+  CHK(p, CeedVectorCopy(z, pv));
+  CHK(p, CeedMiVectorDot(r, z, &rz));
+  const double ttol = fmax(rtol * sqrt(fabs(rz)), atol);
+  int          its  = 0;
+  while (sqrt(fabs(rz)) > ttol && its < max_it) {
+    CHK(p, CeedOperatorApply(op, pv, Ap, CEED_REQUEST_IMMEDIATE));
+    CHK(p, CeedMiVectorDot(pv, Ap, &pAp));
+    const double alpha = rz / pAp;
+    CHK(p, CeedVectorAXPY(X, alpha, pv));
+    CHK(p, CeedVectorAXPY(r, -alpha, Ap));
+    CHK(p, CeedVectorPointwiseMult(z, dinv, r));
+    CHK(p, CeedMiVectorDot(r, z, &rz_new));
+    const double beta = rz_new / rz;
+    CHK(p, CeedVectorAXPBY(pv, 1.0, beta, z));
+    rz = rz_new;
+    its++;
+  }
+  *iterations = its;
+  return 0;
+}
+
 // GridAnisotropyTensorProjectionSetupApply (src/grid_anisotropy_tensor.c:14-111): R = AnisotropyTensorProjection,
-// KSPCG with PCJACOBI (diagonal of Mass_7), natural norm, rtol 1e-10, zero initial guess; the CG recurrences of
-// KSPSolve_CG with KSP_NORM_NATURAL.
+// KSPCG with PCJACOBI (diagonal of Mass_7), natural norm, rtol 1e-10, zero initial guess.
 static int GridAnisotropyTensorProjection(HoneeMiProblem p) {
+  if (p->grid_aniso) return 0;
   CeedQFunction qf;
   CeedOperator  op_rhs, op_mass;
-  CeedVector    R, dinv, r, z, pv, Ap;
+  CeedVector    R, dinv, Ap, work[3];
   CHK(p, NodalField(p, 7, &p->elem_restr_grid_aniso, &p->basis_grid_aniso));
   CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/grid_anisotropy_tensor.h:AnisotropyTensorProjection", &qf));
   CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
@@ -571,56 +619,38 @@ static int GridAnisotropyTensorProjection(HoneeMiProblem p) {
   CHK(p, CeedOperatorSetField(op_rhs, "v", p->elem_restr_grid_aniso, p->basis_grid_aniso, CEED_VECTOR_ACTIVE));
   CHK(p, CeedQFunctionDestroy(&qf));
   CHK(p, MassOperator(p, 7, p->elem_restr_grid_aniso, p->basis_grid_aniso, &op_mass));
-  CeedVector *vs[7] = {&R, &dinv, &r, &z, &pv, &Ap, &p->grid_aniso};
+  CeedVector *vs[7] = {&R, &dinv, &Ap, &work[0], &work[1], &work[2], &p->grid_aniso};
   for (int i = 0; i < 7; i++) CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_grid_aniso, vs[i], NULL));
   CHK(p, CeedOperatorApply(op_rhs, CEED_VECTOR_NONE, R, CEED_REQUEST_IMMEDIATE));
   CHK(p, CeedOperatorLinearAssembleDiagonal(op_mass, dinv, CEED_REQUEST_IMMEDIATE));
   CHK(p, JacobiInverse(p, dinv));
-  double rz, pAp, rz_new;
-  CHK(p, CeedVectorSetValue(p->grid_aniso, 0.0));
-  CHK(p, CeedVectorCopy(R, r));
-  CHK(p, CeedVectorPointwiseMult(z, dinv, r));
-  CHK(p, CeedVectorCopy(z, pv));
-  CHK(p, CeedMiVectorDot(r, z, &rz));
-  const double ttol = fmax(1e-10 * sqrt(fabs(rz)), 1e-50);
-  int          its  = 0;
-  while (sqrt(fabs(rz)) > ttol && its < 10000) {
-    CHK(p, CeedOperatorApply(op_mass, pv, Ap, CEED_REQUEST_IMMEDIATE));
-    CHK(p, CeedMiVectorDot(pv, Ap, &pAp));
-    const double alpha = rz / pAp;
-    CHK(p, CeedVectorAXPY(p->grid_aniso, alpha, pv));
-    CHK(p, CeedVectorAXPY(r, -alpha, Ap));
-    CHK(p, CeedVectorPointwiseMult(z, dinv, r));
-    CHK(p, CeedMiVectorDot(r, z, &rz_new));
-    const double beta = rz_new / rz;
-    CHK(p, CeedVectorAXPBY(pv, 1.0, beta, z));
-    rz = rz_new;
-    its++;
-  }
-  p->grid_aniso_its = its;
+  CHK(p, CgJacobi(p, op_mass, dinv, R, p->grid_aniso, work, Ap, 1e-10, 1e-50, 10000, &p->grid_aniso_its));
   for (int i = 0; i < 6; i++) CHK(p, CeedVectorDestroy(vs[i]));
   CHK(p, CeedOperatorDestroy(&op_rhs));
   CHK(p, CeedOperatorDestroy(&op_mass));
   return 0;
 }
 
-// GetInverseMultiplicity (src/inverse_multiplicity.c:24-80) with a local multiplicity: element-node (collocated)
-// 1 / multiplicity of the SGS field
-static int InverseMultiplicity(HoneeMiProblem p) {
+// GetInverseMultiplicity (src/inverse_multiplicity.c:24-80): element-node (collocated) 1 / multiplicity of the nodes
+// of restr_src (every nodal field here shares the solution nodes, so one vector serves all of them)
+static int InverseMultiplicity(HoneeMiProblem p, CeedElemRestriction restr_src) {
+  if (p->inv_mult) return 0;
   const int     P3 = p->mesh.P * p->mesh.P * p->mesh.P;
   CeedQFunction qf;
   CeedOperator  op;
   CeedVector    mult;
+  CeedInt       ncomp;
+  CHK(p, CeedElemRestrictionGetNumComponents(restr_src, &ncomp));
   CHK(p, CeedElemRestrictionCreateStrided(p->ceed, p->nelem, P3, 1, (CeedSize)p->nelem * P3, CEED_STRIDES_BACKEND, &p->elem_restr_inv_mult));
   CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_inv_mult, &p->inv_mult, NULL));
-  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_sgs, &mult, NULL));
-  CHK(p, CeedElemRestrictionGetMultiplicity(p->elem_restr_sgs, mult));
+  CHK(p, CeedElemRestrictionCreateVector(restr_src, &mult, NULL));
+  CHK(p, CeedElemRestrictionGetMultiplicity(restr_src, mult));
   CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/inverse_multiplicity.h:InverseMultiplicity", &qf));
-  CHK(p, CeedQFunctionAddInput(qf, "multiplicity", 6, CEED_EVAL_NONE));
+  CHK(p, CeedQFunctionAddInput(qf, "multiplicity", ncomp, CEED_EVAL_NONE));
   CHK(p, CeedQFunctionAddOutput(qf, "inverse multiplicity", 1, CEED_EVAL_NONE));
   CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op));
   CHK(p, CeedOperatorSetName(op, "InverseMultiplicity"));
-  CHK(p, CeedOperatorSetField(op, "multiplicity", p->elem_restr_sgs, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+  CHK(p, CeedOperatorSetField(op, "multiplicity", restr_src, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
   CHK(p, CeedOperatorSetField(op, "inverse multiplicity", p->elem_restr_inv_mult, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
   CHK(p, CeedOperatorApply(op, mult, p->inv_mult, CEED_REQUEST_IMMEDIATE));
   CHK(p, CeedVectorDestroy(&mult));
@@ -676,7 +706,7 @@ int HoneeMiProblemSetupSgsDD(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
   CHK(p, GridAnisotropyTensorProjection(p));
   CHK(p, NodalField(p, 6, &p->elem_restr_sgs, &p->basis_sgs));
   CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_sgs, &p->sgs_nodal, NULL));
-  CHK(p, InverseMultiplicity(p));
+  CHK(p, InverseMultiplicity(p, p->elem_restr_sgs));
   CHK(p, CeedBasisCreateProjection(p->basis_x, p->basis_q, &p->basis_x_to_q));
   {
     CeedQFunction qf;
@@ -714,6 +744,241 @@ int HoneeMiProblemSetupSgsDD(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
   }
   p->sgs_enabled = 1;
   return 0;
+}
+
+static int CopyVectorOut(HoneeMiProblem p, CeedVector v, int mem_type, double *out);
+
+// ---- Differential filter (src/differential_filter.c:17-305) ----
+static const char *const kFilterRHSLoc[3] = {"qfunctions/differential_filter.h:DifferentialFilter_RHS_Conserv",
+                                             "qfunctions/differential_filter.h:DifferentialFilter_RHS_Prim",
+                                             "qfunctions/differential_filter.h:DifferentialFilter_RHS_Entropy"};
+
+// DifferentialFilterSetup + DifferentialFilterCreateOperators: the filtered field holds per node the primitive state
+// (field 0, 5 components) then the velocity products (field 1, 6 components); the LHS is the composite of
+// DifferentialFilter_LHS_5 and _6 over those fields with the collocated grid anisotropy tensor; the solve is KSPCG +
+// PCJACOBI (diagonal), natural norm.
+int HoneeMiProblemSetupDiffFilter(HoneeMiProblem p, const HoneeMiDiffFilterOptions *o) {
+  if (p->filter_enabled) return CEED_ERROR_MINOR;
+  const HoneeMiMesh *m = &p->mesh;
+  const int          P3 = m->P * m->P * m->P, ncomps[2] = {5, 6}, field_off[2] = {0, 5};
+  for (int f = 0; f < 2; f++) {
+    CeedInt *off = malloc(sizeof(CeedInt) * (size_t)m->nelem * P3);
+    for (size_t i = 0; i < (size_t)m->nelem * P3; i++) off[i] = 11 * m->offsets[i] + field_off[f];
+    int ierr = CeedElemRestrictionCreate(p->ceed, m->nelem, P3, ncomps[f], 1, 11 * (CeedSize)m->nnodes, CEED_MEM_HOST, CEED_COPY_VALUES, off,
+                                         &p->elem_restr_filter[f]);
+    free(off);
+    CHK(p, ierr);
+    CHK(p, CeedBasisCreateTensorH1(p->ceed, 3, ncomps[f], m->P, m->Q, m->B, m->D, m->qref, m->qweight, &p->basis_filter[f]));
+  }
+  {  // context (src/differential_filter.c:236-275), units of 1 meter
+    CeedMiDiffFilterContext *c = calloc(1, sizeof(*c));
+    c->grid_based_width        = o->grid_based_width != 0;
+    for (int i = 0; i < 3; i++) c->width_scaling[i] = o->width_scaling[i];
+    c->kernel_scaling   = o->kernel_scaling;
+    c->friction_length  = o->friction_length;
+    c->damping_function = o->damping_function;
+    c->damping_constant = o->damping_constant;
+    c->newt_ctx         = *p->ctx;
+    CHK(p, CeedQFunctionContextCreate(p->ceed, &p->diff_filter_qfctx));
+    CHK(p, CeedQFunctionContextSetData(p->diff_filter_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, sizeof(*c), c));
+    CHK(p, CeedQFunctionContextSetDataDestroy(p->diff_filter_qfctx, CEED_MEM_HOST, FreeContext));
+    CHK(p, CeedQFunctionContextRegisterDouble(p->diff_filter_qfctx, "filter width scaling", offsetof(CeedMiDiffFilterContext, width_scaling), 3,
+                                              "Filter width scaling"));
+  }
+  {  // RHS operator
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kFilterRHSLoc[p->state_var], &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->diff_filter_qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "v0", 5, CEED_EVAL_INTERP));
+    CHK(p, CeedQFunctionAddOutput(qf, "v1", 6, CEED_EVAL_INTERP));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_filter_rhs));
+    CHK(p, CeedOperatorSetField(p->op_filter_rhs, "q", p->elem_restr_q, p->basis_q, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_filter_rhs, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(p->op_filter_rhs, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+    CHK(p, CeedOperatorSetField(p->op_filter_rhs, "v0", p->elem_restr_filter[0], p->basis_filter[0], CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_filter_rhs, "v1", p->elem_restr_filter[1], p->basis_filter[1], CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  {  // GridAnisotropyTensorCalculateCollocatedVector (src/grid_anisotropy_tensor.c:114-160)
+    CeedQFunction qf;
+    CeedOperator  op;
+    CHK(p, CeedElemRestrictionCreateStrided(p->ceed, p->nelem, p->Q3, 7, (CeedSize)p->nelem * p->Q3 * 7, CEED_STRIDES_BACKEND, &p->elem_restr_aniso_col));
+    CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_aniso_col, &p->aniso_col, NULL));
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/grid_anisotropy_tensor.h:AnisotropyTensorCollocate", &qf));
+    CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "v", 7, CEED_EVAL_NONE));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &op));
+    CHK(p, CeedOperatorSetField(op, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+    CHK(p, CeedOperatorSetField(op, "v", p->elem_restr_aniso_col, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorApply(op, CEED_VECTOR_NONE, p->aniso_col, CEED_REQUEST_IMMEDIATE));
+    CHK(p, CeedOperatorDestroy(&op));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  {  // LHS composite
+    static const char *const kLHSLoc[2] = {"qfunctions/differential_filter.h:DifferentialFilter_LHS_5",
+                                           "qfunctions/differential_filter.h:DifferentialFilter_LHS_6"};
+    CHK(p, CeedOperatorCreateComposite(p->ceed, &p->op_filter_lhs));
+    for (int f = 0; f < 2; f++) {
+      CeedQFunction qf;
+      CeedOperator  sub;
+      const int     N = ncomps[f];
+      CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, kLHSLoc[f], &qf));
+      CHK(p, CeedQFunctionSetContext(qf, p->diff_filter_qfctx));
+      CHK(p, CeedQFunctionSetUserFlopsEstimate(qf, 0));
+      CHK(p, CeedQFunctionAddInput(qf, "q", N, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddInput(qf, "Grad_q", 3 * N, CEED_EVAL_GRAD));
+      CHK(p, CeedQFunctionAddInput(qf, "anisotropy tensor", 7, CEED_EVAL_NONE));
+      CHK(p, CeedQFunctionAddInput(qf, "x", 3, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddInput(qf, "qdata", 10, CEED_EVAL_NONE));
+      CHK(p, CeedQFunctionAddOutput(qf, "v", N, CEED_EVAL_INTERP));
+      CHK(p, CeedQFunctionAddOutput(qf, "Grad_v", 3 * N, CEED_EVAL_GRAD));
+      CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &sub));
+      CHK(p, CeedOperatorSetField(sub, "q", p->elem_restr_filter[f], p->basis_filter[f], CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(sub, "Grad_q", p->elem_restr_filter[f], p->basis_filter[f], CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(sub, "anisotropy tensor", p->elem_restr_aniso_col, CEED_BASIS_NONE, p->aniso_col));
+      CHK(p, CeedOperatorSetField(sub, "x", p->elem_restr_x, p->basis_x, p->x_coord));
+      CHK(p, CeedOperatorSetField(sub, "qdata", p->elem_restr_qd, CEED_BASIS_NONE, p->q_data));
+      CHK(p, CeedOperatorSetField(sub, "v", p->elem_restr_filter[f], p->basis_filter[f], CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorSetField(sub, "Grad_v", p->elem_restr_filter[f], p->basis_filter[f], CEED_VECTOR_ACTIVE));
+      CHK(p, CeedOperatorCompositeAddSub(p->op_filter_lhs, sub));
+      CHK(p, CeedOperatorDestroy(&sub));
+      CHK(p, CeedQFunctionDestroy(&qf));
+    }
+    CHK(p, CeedOperatorGetContextFieldLabel(p->op_filter_lhs, "filter width scaling", &p->filter_width_label));
+  }
+  CeedVector *vs[6] = {&p->filter_rhs, &p->filtered, &p->filter_dinv, &p->cg_work[0], &p->cg_work[1], &p->cg_work[2]};
+  for (int i = 0; i < 6; i++) CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_filter[0], vs[i], NULL));
+  p->filter_rtol    = o->ksp_rtol > 0 ? o->ksp_rtol : 1e-10;
+  p->filter_atol    = o->ksp_atol > 0 ? o->ksp_atol : 1e-50;
+  p->filter_max_it  = o->ksp_max_it > 0 ? o->ksp_max_it : 10000;
+  p->filter_enabled = 1;
+  return 0;
+}
+
+// DifferentialFilterApply (src/differential_filter.c:283-305): q_ceed holds the state; result in p->filtered.  The
+// Jacobi preconditioner is set up at the first solve and kept: a later change of the filter width through the
+// operator context does not set up the KSP's preconditioner again.
+static int DiffFilterApply(HoneeMiProblem p) {
+  CHK(p, CeedOperatorApply(p->op_filter_rhs, p->q_ceed, p->filter_rhs, CEED_REQUEST_IMMEDIATE));
+  if (!p->filter_pc_ready) {
+    CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_filter_lhs, p->filter_dinv, CEED_REQUEST_IMMEDIATE));
+    CHK(p, JacobiInverse(p, p->filter_dinv));
+    p->filter_pc_ready = 1;
+  }
+  CeedVector ApV;
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_filter[0], &ApV, NULL));
+  int ierr = CgJacobi(p, p->op_filter_lhs, p->filter_dinv, p->filter_rhs, p->filtered, p->cg_work, ApV, p->filter_rtol, p->filter_atol,
+                      p->filter_max_it, &p->filter_its);
+  CeedVectorDestroy(&ApV);
+  return ierr;
+}
+
+int HoneeMiSetFilterWidthScaling(HoneeMiProblem p, const double scaling[3]) {
+  if (!p->filter_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedOperatorSetContextDouble(p->op_filter_lhs, p->filter_width_label, (double *)scaling));
+  return 0;
+}
+
+int HoneeMiDiffFilterApply(HoneeMiProblem p, int mem_type, const double *q, double *filtered, int *iterations) {
+  CeedScalar *arr;
+  if (!p->filter_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  int ierr = DiffFilterApply(p);
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, ierr);
+  if (iterations) *iterations = p->filter_its;
+  return CopyVectorOut(p, p->filtered, mem_type, filtered);
+}
+
+int HoneeMiDiffFilterRHS(HoneeMiProblem p, int mem_type, const double *q, double *rhs) {
+  CeedScalar *arr;
+  if (!p->filter_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  int ierr = CeedOperatorApply(p->op_filter_rhs, p->q_ceed, p->filter_rhs, CEED_REQUEST_IMMEDIATE);
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, ierr);
+  return CopyVectorOut(p, p->filter_rhs, mem_type, rhs);
+}
+
+// The filter LHS operator and its diagonal on a host array [nnodes][11] (tests)
+int HoneeMiDiffFilterLHS(HoneeMiProblem p, const double *u, double *v, double *diag) {
+  if (!p->filter_enabled) return CEED_ERROR_INCOMPATIBLE;
+  CeedVector uv, vv;
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_filter[0], &uv, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_filter[0], &vv, NULL));
+  CHK(p, CeedVectorSetArray(uv, CEED_MEM_HOST, CEED_COPY_VALUES, (CeedScalar *)u));
+  CHK(p, CeedOperatorApply(p->op_filter_lhs, uv, vv, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CopyVectorOut(p, vv, CEED_MEM_HOST, v));
+  CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_filter_lhs, vv, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CopyVectorOut(p, vv, CEED_MEM_HOST, diag));
+  CHK(p, CeedVectorDestroy(&uv));
+  CHK(p, CeedVectorDestroy(&vv));
+  return 0;
+}
+
+// SGS_DD_TrainingSetup (src/smartsim/sgs_dd_training.c:183-282) and SetupTrainingDataCalculation (:93-180): the
+// projected grid anisotropy, the velocity-gradient projection of the filtered primitive state, and the nodal
+// training-pair operator (12 components per node)
+int HoneeMiProblemSetupSgsTraining(HoneeMiProblem p) {
+  if (!p->filter_enabled) return CEED_ERROR_INCOMPATIBLE;
+  if (p->train_enabled) return CEED_ERROR_MINOR;
+  CHK(p, VelocityGradientProjectionSetup(p));
+  CHK(p, GridAnisotropyTensorProjection(p));
+  CHK(p, NodalField(p, 12, &p->elem_restr_train, NULL));
+  CHK(p, CeedElemRestrictionCreateVector(p->elem_restr_train, &p->train_data, NULL));
+  CHK(p, InverseMultiplicity(p, p->elem_restr_train));
+  CHK(p, VelocityGradientRHSOperator(p, 1, p->elem_restr_filter[0], p->basis_filter[0], &p->op_filtered_grad_velo_rhs));
+  {
+    CeedMiSgsTrainingContext *c = calloc(1, sizeof(*c));
+    c->newt_ctx                 = *p->ctx;
+    CHK(p, CeedQFunctionContextCreate(p->ceed, &p->train_qfctx));
+    CHK(p, CeedQFunctionContextSetData(p->train_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, sizeof(*c), c));
+    CHK(p, CeedQFunctionContextSetDataDestroy(p->train_qfctx, CEED_MEM_HOST, FreeContext));
+  }
+  {
+    CeedQFunction qf;
+    CHK(p, CeedQFunctionCreateInterior(p->ceed, 1, HoneeMiQFunctionNotOnHost, "qfunctions/sgs_dd_training.h:ComputeSGS_DDAnisotropicTrainingDataNodal_Prim",
+                                       &qf));
+    CHK(p, CeedQFunctionSetContext(qf, p->train_qfctx));
+    CHK(p, CeedQFunctionAddInput(qf, "q", 5, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "velocity product", 6, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "gradient velocity", 9, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "anisotropy tensor", 7, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddInput(qf, "inverse multiplicity", 1, CEED_EVAL_NONE));
+    CHK(p, CeedQFunctionAddOutput(qf, "training data", 12, CEED_EVAL_NONE));
+    CHK(p, CeedOperatorCreate(p->ceed, qf, NULL, NULL, &p->op_sgs_train));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "q", p->elem_restr_filter[0], CEED_BASIS_NONE, p->filtered));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "velocity product", p->elem_restr_filter[1], CEED_BASIS_NONE, p->filtered));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "gradient velocity", p->elem_restr_grad_velo, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "anisotropy tensor", p->elem_restr_grid_aniso, CEED_BASIS_NONE, p->grid_aniso));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "inverse multiplicity", p->elem_restr_inv_mult, CEED_BASIS_NONE, p->inv_mult));
+    CHK(p, CeedOperatorSetField(p->op_sgs_train, "training data", p->elem_restr_train, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE));
+    CHK(p, CeedQFunctionDestroy(&qf));
+  }
+  p->train_enabled = 1;
+  return 0;
+}
+
+// TSMonitor_SGS_DD_Training's computation for one filter width (src/smartsim/sgs_dd_training.c:313-349): filter the
+// state with the width scaled by width_scale in every direction, project the filtered velocity gradient, evaluate the
+// training pairs [nnodes][12] (six model inputs, six filtered SGS stress components)
+int HoneeMiSgsTrainingData(HoneeMiProblem p, int mem_type, const double *q, double width_scale, double *out, int *filter_iterations) {
+  CeedScalar *arr;
+  if (!p->train_enabled) return CEED_ERROR_INCOMPATIBLE;
+  const double scaling[3] = {width_scale, width_scale, width_scale};
+  CHK(p, HoneeMiSetFilterWidthScaling(p, scaling));
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  int ierr = DiffFilterApply(p);
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  CHK(p, ierr);
+  if (filter_iterations) *filter_iterations = p->filter_its;
+  CHK(p, CeedOperatorApply(p->op_filtered_grad_velo_rhs, p->filtered, p->grad_velo, CEED_REQUEST_IMMEDIATE));
+  CHK(p, CeedVectorPointwiseMult(p->grad_velo, p->grad_velo, p->grad_velo_inv_rowsum));
+  CHK(p, CeedOperatorApply(p->op_sgs_train, p->grad_velo, p->train_data, CEED_REQUEST_IMMEDIATE));
+  return CopyVectorOut(p, p->train_data, mem_type, out);
 }
 
 // VelocityGradientProjectionApply (src/velocity_gradient_projection.c:145-155): q_ceed holds the state
@@ -964,6 +1229,21 @@ int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   CeedBasisDestroy(&p->basis_sgs);
   CeedBasisDestroy(&p->basis_x_to_q);
   if (p->sgsdd_qfctx) CeedQFunctionContextDestroy(&p->sgsdd_qfctx);
+  CeedOperatorDestroy(&p->op_filter_rhs);
+  CeedOperatorDestroy(&p->op_filter_lhs);
+  CeedOperatorDestroy(&p->op_filtered_grad_velo_rhs);
+  CeedOperatorDestroy(&p->op_sgs_train);
+  for (int i = 0; i < 2; i++) {
+    CeedElemRestrictionDestroy(&p->elem_restr_filter[i]);
+    CeedBasisDestroy(&p->basis_filter[i]);
+  }
+  CeedElemRestrictionDestroy(&p->elem_restr_aniso_col);
+  CeedElemRestrictionDestroy(&p->elem_restr_train);
+  CeedVector *fv[9] = {&p->filter_rhs, &p->filtered, &p->filter_dinv, &p->aniso_col, &p->train_data, &p->cg_work[0], &p->cg_work[1], &p->cg_work[2],
+                       NULL};
+  for (int i = 0; fv[i]; i++) CeedVectorDestroy(fv[i]);
+  if (p->diff_filter_qfctx) CeedQFunctionContextDestroy(&p->diff_filter_qfctx);
+  if (p->train_qfctx) CeedQFunctionContextDestroy(&p->train_qfctx);
   CeedOperatorDestroy(&p->op_rhs);
   CeedOperatorDestroy(&p->op_mass);
   CeedOperatorDestroy(&p->op_monitor);

@@ -69,6 +69,25 @@ int HoneeMiSgsGridAnisotropy(HoneeMiProblem p, double *out, int *cg_iterations);
 int HoneeMiVelocityGradientProjection(HoneeMiProblem p, int mem_type, const double *q, double *grad_velo);  // [nnodes][9]
 int HoneeMiSgsNodalStress(HoneeMiProblem p, int mem_type, const double *q, const double *grad_velo, double *km_sgs);  // [nnodes][6]
 int HoneeMiApplySgsResidual(HoneeMiProblem p, int mem_type, const double *q, const double *km_sgs, double *g);  // g += SGS residual
+// Differential filter (src/differential_filter.c:177-275): the options of DifferentialFilterSetup (damping_function
+// 0 none, 1 van Driest, 2 MMS) and of its KSP (0 selects PETSc's defaults: rtol 1e-10 as HONEE sets it, atol 1e-50,
+// max_it 10000).  The filtered field is [nnodes][11]: primitive state, then velocity products XX YY ZZ YZ XZ XY.
+typedef struct {
+  int    grid_based_width;
+  double width_scaling[3], kernel_scaling, friction_length;
+  int    damping_function;
+  double damping_constant;
+  double ksp_rtol, ksp_atol;
+  int    ksp_max_it;
+} HoneeMiDiffFilterOptions;
+int HoneeMiProblemSetupDiffFilter(HoneeMiProblem p, const HoneeMiDiffFilterOptions *options);
+int HoneeMiSetFilterWidthScaling(HoneeMiProblem p, const double scaling[3]);
+int HoneeMiDiffFilterRHS(HoneeMiProblem p, int mem_type, const double *q, double *rhs);
+int HoneeMiDiffFilterLHS(HoneeMiProblem p, const double *u, double *v, double *diag);  // host arrays [nnodes][11]
+int HoneeMiDiffFilterApply(HoneeMiProblem p, int mem_type, const double *q, double *filtered, int *iterations);
+// SGS training pairs (src/smartsim/sgs_dd_training.c): [nnodes][12] per filter width scale
+int HoneeMiProblemSetupSgsTraining(HoneeMiProblem p);
+int HoneeMiSgsTrainingData(HoneeMiProblem p, int mem_type, const double *q, double width_scale, double *out, int *filter_iterations);
 const char  *HoneeMiProblemError(HoneeMiProblem p);
 const char  *HoneeMiLastError(void);
 

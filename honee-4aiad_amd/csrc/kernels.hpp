@@ -68,6 +68,12 @@ struct FieldArgs {
   const double *qdata;
   QDataLayout   qd;
   BasisTables   bt;
+  const double *aux;  // per-point auxiliary data (collocated anisotropy tensor [elem][7][Q^3])
+  QDataLayout   auxl;
+  const int    *xoffsets;  // trilinear coordinate field: [nelem][8] offsets of component 0
+  int           xcomp_stride;
+  const double *x;
+  const double *Bx;  // [Q][2]
 };
 
 // Nodal SGS closure: one evaluation per row k of the output restriction's transpose map (rowptr, index, row_offset),
@@ -84,6 +90,8 @@ struct SgsNodalArgs {
   QDataLayout   im;
   int           elem_size;
   const double *model;  // b1[20], b2[6], W1[20][6], W2[6][20], bounds[6][2], alpha
+  const int    *p_offsets;  // training pairs: filtered velocity products (offsets into the filtered field q)
+  int           p_comp_stride;
 };
 
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream);
@@ -138,5 +146,12 @@ hipError_t launch_mass_n_diagonal(int P, int Q, int ncomp, const FieldArgs &a, h
 hipError_t launch_ifunction_nodal_sgs(int P, int Q, int var, const FieldArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
 hipError_t launch_inverse_multiplicity(const int *offsets, int nelem, int elem_size, const double *mult, double *out, QDataLayout ol, hipStream_t s);
 hipError_t launch_sgs_nodal(int var, const SgsNodalArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
+// Differential filter and SGS training pairs (kernels_sgs.hip)
+hipError_t launch_diff_filter_rhs(int P, int Q, int var, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s);
+bool       diff_filter_ncomp_supported(int ncomp);
+hipError_t launch_diff_filter_lhs(int P, int Q, int ncomp, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s);
+hipError_t launch_diff_filter_lhs_diagonal(int P, int Q, int ncomp, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s);
+hipError_t launch_anisotropy_collocate(int Q, int nelem, const double *qdata, QDataLayout qd, double *out, QDataLayout ol, hipStream_t s);
+hipError_t launch_sgs_training_nodal(const SgsNodalArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
 // Deterministic dot product: npartial fixed per-block partial sums (summed in order by the caller)
 hipError_t launch_dot(const double *x, const double *y, long long n, double *d_partial, int npartial, hipStream_t s);

@@ -348,8 +348,8 @@ hipError_t launch_face_diag(const VolumeArgs &a, const Ctx &ctx, hipStream_t str
   const int       block  = FPB * Q * Q;
   const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
   int             dev = 0, ncu = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   long long grid = (long long)ncu * 8;
   if (grid > nbatch) grid = nbatch;
   if (grid < 1) grid = 1;
@@ -363,8 +363,8 @@ hipError_t launch_face_op(const VolumeArgs &a, const Ctx &ctx, hipStream_t strea
   const int       block  = FPB * Q * Q;
   const long long nbatch = ((long long)a.nelem + FPB - 1) / FPB;
   int             dev = 0, ncu = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   long long grid = (long long)ncu * 8;
   if (grid > nbatch) grid = nbatch;
   if (grid < 1) grid = 1;

@@ -37,9 +37,10 @@ constexpr int sgs_elems_per_block() {
 
 template <int VAR>
 struct VelocityGradientRHS {  // v[j*3+k] = wdetJ du_j/dx_k
+  using Ctx                       = CeedMiNewtonianContext;
   static constexpr int  NU = 5, NU2 = 0, NV = 9, NST = 27;
-  static constexpr bool GRAD = true, OUTG = false;
-  __device__ static void qf(const CeedMiNewtonianContext &ctx, const double qd[10], bool ok, const double (&u)[5], const double (&gu)[3][5],
+  static constexpr bool GRAD = true, OUTV = true, OUTG = false;
+  __device__ static void qf(const FieldArgs &, const Ctx &ctx, int, int, const double qd[10], bool ok, const double (&u)[5], const double (&gu)[3][5],
                             const double (&)[1], double (&out)[9]) {
     const auto g = ns::Gas<double>::from(ctx.gas);
     double     wdetJ, dXdx[3][3], qi[5];
@@ -58,9 +59,10 @@ struct VelocityGradientRHS {  // v[j*3+k] = wdetJ du_j/dx_k
 };
 
 struct AnisotropyRHS {  // v = wdetJ [KM(A_ij), delta]
+  using Ctx                       = CeedMiNewtonianContext;
   static constexpr int  NU = 0, NU2 = 0, NV = 7, NST = 21;
-  static constexpr bool GRAD = false, OUTG = false;
-  __device__ static void qf(const CeedMiNewtonianContext &, const double qd[10], bool ok, const double (&)[1], const double (&)[3][1],
+  static constexpr bool GRAD = false, OUTV = true, OUTG = false;
+  __device__ static void qf(const FieldArgs &, const Ctx &, int, int, const double qd[10], bool ok, const double (&)[1], const double (&)[3][1],
                             const double (&)[1], double (&out)[7]) {
     double wdetJ, dXdx[3][3], kmA[6], delta;
     nsop::unpack_qdata(qd, ok, wdetJ, dXdx);
@@ -73,9 +75,10 @@ struct AnisotropyRHS {  // v = wdetJ [KM(A_ij), delta]
 
 template <int NC>
 struct MassN {  // v = wdetJ u
+  using Ctx                       = CeedMiNewtonianContext;
   static constexpr int  NU = NC, NU2 = 0, NV = NC, NST = 3 * NC;
-  static constexpr bool GRAD = false, OUTG = false;
-  __device__ static void qf(const CeedMiNewtonianContext &, const double qd[10], bool ok, const double (&u)[NC], const double (&)[3][NC],
+  static constexpr bool GRAD = false, OUTV = true, OUTG = false;
+  __device__ static void qf(const FieldArgs &, const Ctx &, int, int, const double qd[10], bool ok, const double (&u)[NC], const double (&)[3][NC],
                             const double (&)[1], double (&out)[NC]) {
     const double w = ok ? qd[0] : 0.0;
 #pragma unroll
@@ -85,9 +88,10 @@ struct MassN {  // v = wdetJ u
 
 template <int VAR>
 struct NodalSgsResidual {  // Grad_v[k][j] = -wdetJ (dXdx[k] . F_sgs[j])
+  using Ctx                       = CeedMiNewtonianContext;
   static constexpr int  NU = 5, NU2 = 6, NV = 5, NST = 20;
-  static constexpr bool GRAD = false, OUTG = true;
-  __device__ static void qf(const CeedMiNewtonianContext &ctx, const double qd[10], bool ok, const double (&u)[5], const double (&)[3][5],
+  static constexpr bool GRAD = false, OUTV = false, OUTG = true;
+  __device__ static void qf(const FieldArgs &, const Ctx &ctx, int, int, const double qd[10], bool ok, const double (&u)[5], const double (&)[3][5],
                             const double (&km)[6], double (&out)[15]) {
     const auto g = ns::Gas<double>::from(ctx.gas);
     double     wdetJ, dXdx[3][3], qi[5], F[5][3];
@@ -104,19 +108,130 @@ struct NodalSgsResidual {  // Grad_v[k][j] = -wdetJ (dXdx[k] . F_sgs[j])
   }
 };
 
+template <int VAR>
+struct DiffFilterRHS {  // v0 = wdetJ Y, v1 = wdetJ u_i u_j (XX YY ZZ YZ XZ XY): one 11-component output
+  using Ctx                       = CeedMiDiffFilterContext;
+  static constexpr int  NU = 5, NU2 = 0, NV = 11, NST = 33;
+  static constexpr bool GRAD = false, OUTV = true, OUTG = false;
+  __device__ static void qf(const FieldArgs &, const Ctx &ctx, int, int, const double qd[10], bool ok, const double (&u)[5], const double (&)[3][5],
+                            const double (&)[1], double (&out)[11]) {
+    const auto   g     = ns::Gas<double>::from(ctx.newt_ctx.gas);
+    const double wdetJ = ok ? qd[0] : 0.0;
+    double       qi[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) qi[c] = u[c];
+    if (!ok) nsop::padding_state<VAR>(qi);
+    const ns::St<double> s = ns::state_from_q<VAR>(g, qi);
+#pragma unroll
+    for (int c = 0; c < 5; c++) out[c] = wdetJ * s.Y[c];
+    const double *v = &s.Y[1];
+    out[5]          = wdetJ * v[0] * v[0];
+    out[6]          = wdetJ * v[1] * v[1];
+    out[7]          = wdetJ * v[2] * v[2];
+    out[8]          = wdetJ * v[1] * v[2];
+    out[9]          = wdetJ * v[0] * v[2];
+    out[10]         = wdetJ * v[0] * v[1];
+  }
+};
+
+// alpha_ij of DifferentialFilter_LHS_N (qfunctions/differential_filter.h:91-125) at point t of element e: the
+// collocated anisotropy tensor (aux, [elem][7][Q^3]) when the width is grid based, the coordinates for wall damping
+template <int Q>
+__device__ __forceinline__ void filter_alpha(const FieldArgs &a, const CeedMiDiffFilterContext &f, int e, int t, bool ok, double alpha[3][3]) {
+  constexpr int Q3      = Q * Q * Q;
+  double        D[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (f.grid_based_width && ok) {
+    double km[6];
+    const double *A = a.aux + (long long)e * a.auxl.se + t;
+    const double  delta = A[6 * Q3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) km[k] = A[k * Q3] * delta;
+    sgs::km_unpack(km, D);
+  }
+  double S[3] = {f.width_scaling[0], f.width_scaling[1], f.width_scaling[2]};
+  if (f.damping_function != 0) {
+    double x1 = 0.0;  // y coordinate of the point (trilinear coordinate field)
+    if (ok) {
+      const int qx = t % Q, qy = (t / Q) % Q, qz = t / (Q * Q);
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        const int ia = v & 1, ib = (v >> 1) & 1, ic = v >> 2;
+        x1 += a.Bx[qx * 2 + ia] * a.Bx[qy * 2 + ib] * a.Bx[qz * 2 + ic] * a.x[a.xoffsets[(long long)e * 8 + v] + a.xcomp_stride];
+      }
+    }
+    if (f.damping_function == 1) {  // van Driest: -expm1(-y+ / A+)
+      const double dc = -expm1(-(x1 / f.friction_length) / f.damping_constant);
+      const double s0 = dc * f.width_scaling[0], s2 = dc * f.width_scaling[2];
+      S[0]            = 1 < s0 ? s0 : 1;
+      S[1]            = dc * f.width_scaling[1];
+      S[2]            = 1 < s2 ? s2 : 1;
+    } else {  // MMS
+      S[0] = 1;
+      S[1] = tanh(60 * x1);
+      S[2] = 1;
+    }
+  }
+  double SD[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) SD[i][j] = S[i] * D[i][j];  // diag(S) . D
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) alpha[i][j] = (SD[i][0] * SD[0][j] + SD[i][1] * SD[1][j] + SD[i][2] * SD[2][j]) * f.kernel_scaling;
+}
+
+template <int N>
+struct DiffFilterLHS {  // v = wdetJ q, Grad_v[k] = wdetJ (alpha (dXdx grad_x q))[k], every component alike
+  using Ctx                       = CeedMiDiffFilterContext;
+  static constexpr int  NU = N, NU2 = 0, NV = N, NST = 4 * N;
+  static constexpr bool GRAD = true, OUTV = true, OUTG = true;
+  template <int Q>
+  __device__ static void qfq(const FieldArgs &a, const Ctx &ctx, int e, int t, const double qd[10], bool ok, const double (&u)[N],
+                             const double (&gu)[3][N], const double (&)[1], double (&out)[4 * N]) {
+    double wdetJ, dXdx[3][3], alpha[3][3];
+    nsop::unpack_qdata(qd, ok, wdetJ, dXdx);
+    filter_alpha<Q>(a, ctx, e, t, ok, alpha);
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      out[j] = wdetJ * u[j];
+      double dq[3], dqX[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) dq[k] = gu[0][j] * dXdx[0][k] + gu[1][j] * dXdx[1][k] + gu[2][j] * dXdx[2][k];
+#pragma unroll
+      for (int k = 0; k < 3; k++) dqX[k] = dXdx[k][0] * dq[0] + dXdx[k][1] * dq[1] + dXdx[k][2] * dq[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) out[N + k * N + j] = wdetJ * (alpha[k][0] * dqX[0] + alpha[k][1] * dqX[1] + alpha[k][2] * dqX[2]);
+    }
+  }
+};
+
 // Slot map of the stage: active input u -> 0..NU-1 (interpolation scratch NU..3NU-1), its reference gradient ->
 // NU + NU d + c, passive input -> 5..10 (interpolation scratch 11..16 and 5..10; only with NU = 5 and no gradient).
 // Outputs: values -> 0..NV-1 (transpose scratch up to 3NV), gradient test weights -> NV + NV d + c.
+// Op::qf(a, ctx, e, t, ...) or, for bodies that depend on Q, Op::qfq<Q>(...)
+template <class Op, int Q, class = void>
+struct HasQfq {
+  static constexpr bool value = false;
+};
+template <class Op, int Q>
+struct HasQfq<Op, Q, decltype((void)&Op::template qfq<Q>)> {
+  static constexpr bool value = true;
+};
+
 template <int P, int Q, class Op>
-__global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_field(FieldArgs a, const CeedMiNewtonianContext ctx) {
+__global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_field(FieldArgs a, const typename Op::Ctx ctx) {
   using Dm          = Dims<P, Q>;
   constexpr int EPB = sgs_elems_per_block<Q>();
   constexpr int Q3  = Dm::Q3;
   constexpr int NU = Op::NU, NU2 = Op::NU2, NV = Op::NV;
-  constexpr int NUX = NU > 0 ? NU : 1, NU2X = NU2 > 0 ? NU2 : 1, NOUT = Op::OUTG ? 3 * NV : NV;
+  constexpr int NUX = NU > 0 ? NU : 1, NU2X = NU2 > 0 ? NU2 : 1, NOUT = (Op::OUTV ? NV : 0) + (Op::OUTG ? 3 * NV : 0);
+  constexpr int G0  = Op::OUTV ? NV : 0;  // first gradient-weight entry of out[]
   static_assert(NU2 == 0 || (NU == 5 && !Op::GRAD && NU2 == 6), "passive input slot map assumes a 5-component value input");
   static_assert(Op::NST >= 3 * NU && Op::NST >= (Op::GRAD ? 4 * NU : 0) && Op::NST >= (NU2 ? 17 : 0), "stage too small for the inputs");
   static_assert(Op::NST >= (Op::OUTG ? 4 * NV : 3 * NV), "stage too small for the output transpose");
+  static_assert(Op::OUTV || Op::OUTG, "no output");
   static_assert(Dm::P3 <= Q3, "P <= Q is required");
   __shared__ double sB[Q * P], sDc[Q * Q];
   __shared__ double sX[EPB][Op::NST][Q3];
@@ -172,20 +287,21 @@ __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_field(
     }
 #pragma unroll
     for (int c = 0; c < NU2X; c++) u2[c] = NU2 > 0 ? X[5 + c][t] : 0.0;
-    Op::qf(ctx, qd, ok, u, gu, u2, out);
+    if constexpr (HasQfq<Op, Q>::value) Op::template qfq<Q>(a, ctx, e, t, qd, ok, u, gu, u2, out);
+    else Op::qf(a, ctx, e, t, qd, ok, u, gu, u2, out);
     __syncthreads();  // other threads' last interpolation pass may still read the stage
     double r[NV];
+    if constexpr (Op::OUTV) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) X[c][t] = out[c];
+    }
     if constexpr (Op::OUTG) {
 #pragma unroll
       for (int d = 0; d < 3; d++)
 #pragma unroll
-        for (int c = 0; c < NV; c++) X[NV + NV * d + c][t] = out[d * NV + c];
-      st.template transpose<NV, false, true>(0, NV, r);
-    } else {
-#pragma unroll
-      for (int c = 0; c < NV; c++) X[c][t] = out[c];
-      st.template transpose<NV, true, false>(0, NV, r);
+        for (int c = 0; c < NV; c++) X[NV + NV * d + c][t] = out[G0 + d * NV + c];
     }
+    st.template transpose<NV, Op::OUTV, Op::OUTG>(0, NV, r);
     if (ok && nd) {
       if (a.ev) {  // two-pass scatter: node-sorted element-node contributions, summed per node by launch_node_gather
         double *dst = a.ev + (long long)a.perm[enode] * NV;
@@ -285,6 +401,128 @@ __global__ void __launch_bounds__(kNodalBlock) k_sgs_nodal(SgsNodalArgs a, const
   }
 }
 
+// Diagonal of the filter operator DifferentialFilter_LHS_N: per point the mass weight wdetJ and the gradient coupling
+// K = wdetJ alpha (dXdx dXdx^T) (Grad_v[k] = sum_d K[k][d] dq/dX_d), contracted with products of the 1D tables
+// (B.B, G.B, G.G) by sum factorization; the same value in each of the ncomp components of a node.
+template <int P, int Q>
+__global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_filter_lhs_diagonal(FieldArgs a, const CeedMiDiffFilterContext ctx,
+                                                                                             int ncomp) {
+  using Dm          = Dims<P, Q>;
+  constexpr int EPB = sgs_elems_per_block<Q>();
+  constexpr int Q3  = Dm::Q3;
+  __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
+  __shared__ double sS[EPB][3][Dm::MAX3];
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    sBB[i] = a.bt.BB[i];
+    sGB[i] = a.bt.GB[i];
+    sGG[i] = a.bt.GG[i];
+  }
+  const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
+  const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
+  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long long e_ = bt * EPB + le;
+    const bool      ok = e_ < a.nelem;
+    const int       e  = ok ? (int)e_ : 0;
+    double          qd[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) qd[k] = ok ? a.qdata[(long long)e * a.qd.se + k * Q3 + t] : 0.0;
+    double wdetJ, dXdx[3][3], alpha[3][3], G[3][3], K[3][3];
+    nsop::unpack_qdata(qd, ok, wdetJ, dXdx);
+    filter_alpha<Q>(a, ctx, e, t, ok, alpha);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) G[i][d] = dXdx[i][0] * dXdx[d][0] + dXdx[i][1] * dXdx[d][1] + dXdx[i][2] * dXdx[d][2];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) K[k][d] = wdetJ * (alpha[k][0] * G[0][d] + alpha[k][1] * G[1][d] + alpha[k][2] * G[2][d]);
+    basis3::Elem<P, Q> el(nullptr, nullptr, sS[le][0], sS[le][1], sS[le][2], t);
+    double             r = el.contract_t(wdetJ, sBB, sBB, sBB);
+    r += el.contract_t(K[0][0], sGG, sBB, sBB);
+    r += el.contract_t(K[1][1], sBB, sGG, sBB);
+    r += el.contract_t(K[2][2], sBB, sBB, sGG);
+    r += el.contract_t(K[0][1] + K[1][0], sGB, sGB, sBB);
+    r += el.contract_t(K[0][2] + K[2][0], sGB, sBB, sGB);
+    r += el.contract_t(K[1][2] + K[2][1], sBB, sGB, sGB);
+    if (ok && t < Dm::P3) {
+      const int off = a.out_offsets[(long long)e * Dm::P3 + t];
+      for (int c = 0; c < ncomp; c++) atomicAdd(&a.v[off + c * a.out_comp_stride], r);
+    }
+    __syncthreads();
+  }
+}
+
+// AnisotropyTensorCollocate (qfunctions/grid_anisotropy_tensor.h:57-77): [KM(A_ij), delta] at every point
+__global__ void k_anisotropy_collocate(int Q3, long long n, const double *qdata, QDataLayout qd, double *out, QDataLayout ol) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long e = i / Q3, t = i % Q3;
+    const double   *q = qdata + e * qd.se + t * qd.sn;
+    double          dXdx[3][3], kmA[6], delta;
+#pragma unroll
+    for (int k = 0; k < 9; k++) (&dXdx[0][0])[k] = q[(1 + k) * qd.sc];
+    sgs::anisotropy_tensor(dXdx, kmA, delta);
+    double *o = out + e * ol.se + t * ol.sn;
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k * ol.sc] = kmA[k];
+    o[6 * ol.sc] = delta;
+  }
+}
+
+// ComputeSGS_DDAnisotropicTrainingDataNodal_Prim (qfunctions/sgs_dd_training.h:23-60), once per node of the training
+// field, summed over the node's element nodes with their inverse multiplicities like k_sgs_nodal
+__global__ void __launch_bounds__(kNodalBlock) k_sgs_training_nodal(SgsNodalArgs a, const CeedMiNewtonianContext ctx) {
+  const auto      g = ns::Gas<double>::from(ctx.gas);
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nrows) return;
+  const int j0 = a.rowptr[k], j1 = a.rowptr[k + 1];
+  const int i0 = a.index[j0];
+  double    yi[5], vp[6], gu[3][3], kmA[6];
+  {
+    const int off = a.q_offsets[i0];
+#pragma unroll
+    for (int c = 0; c < 5; c++) yi[c] = a.q[off + c * a.q_comp_stride];
+  }
+  {
+    const int off = a.p_offsets[i0];
+#pragma unroll
+    for (int c = 0; c < 6; c++) vp[c] = a.q[off + c * a.p_comp_stride];
+  }
+  {
+    const int off = a.g_offsets[i0];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++) gu[j][kk] = a.g[off + (j * 3 + kk) * a.g_comp_stride];
+  }
+  const int aoff = a.a_offsets[i0];
+#pragma unroll
+  for (int c = 0; c < 6; c++) kmA[c] = a.aniso[aoff + c * a.a_comp_stride];
+  const double         delta = a.aniso[aoff + 6 * a.a_comp_stride];
+  const ns::St<double> s     = ns::state_from_q<ns::VAR_PRIM>(g, yi);
+  double               E[3][3], val[12], gmag;
+  sgs::sgs_inputs(gu, kmA, delta, g.mu / s.U[0], E, val, gmag);
+  const double *u = &s.Y[1];
+  val[6]          = vp[0] - u[0] * u[0];
+  val[7]          = vp[1] - u[1] * u[1];
+  val[8]          = vp[2] - u[2] * u[2];
+  val[9]          = vp[3] - u[1] * u[2];
+  val[10]         = vp[4] - u[0] * u[2];
+  val[11]         = vp[5] - u[0] * u[1];
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; c++) acc[c] = 0;
+  for (int j = j0; j < j1; j++) {
+    const int    ie = a.index[j];
+    const double w  = a.inv_mult[(long long)(ie % a.elem_size) * a.im.sn + (long long)(ie / a.elem_size) * a.im.se];
+#pragma unroll
+    for (int c = 0; c < 12; c++) acc[c] += val[c] * w;
+  }
+  const int off = a.row_offset[k];
+#pragma unroll
+  for (int c = 0; c < 12; c++) a.v[off + c * a.out_comp_stride] += acc[c];
+}
+
 template <typename K>
 int grid_for_batches(K kernel, int block, long long nbatch) {
   int dev = 0, ncu = 0, per_cu = 0;
@@ -298,7 +536,7 @@ int grid_for_batches(K kernel, int block, long long nbatch) {
 }
 
 template <int P, int Q, class Op>
-hipError_t launch_field_pq(const FieldArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+hipError_t launch_field_pq(const FieldArgs &a, const typename Op::Ctx &ctx, hipStream_t s) {
   if (a.nelem <= 0) return hipSuccess;
   constexpr int EPB   = sgs_elems_per_block<Q>();
   constexpr int block = EPB * Q * Q * Q;
@@ -333,7 +571,7 @@ hipError_t dispatch_pq(int P, int Q, Args &&...args) {
 
 template <int P, int Q, class Op>
 struct RunField {
-  static hipError_t run(const FieldArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) { return launch_field_pq<P, Q, Op>(a, ctx, s); }
+  static hipError_t run(const FieldArgs &a, const typename Op::Ctx &ctx, hipStream_t s) { return launch_field_pq<P, Q, Op>(a, ctx, s); }
 };
 
 template <template <int> class OpT, class... Args>
@@ -435,3 +673,55 @@ hipError_t launch_sgs_nodal(int var, const SgsNodalArgs &a, const CeedMiNewtonia
 }
 
 size_t sgs_model_doubles() { return sizeof(sgs::Model) / sizeof(double); }
+
+hipError_t launch_diff_filter_rhs(int P, int Q, int var, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s) {
+  return dispatch_var<DiffFilterRHS>(var, P, Q, a, ctx, s);
+}
+
+bool diff_filter_ncomp_supported(int ncomp) { return ncomp == 1 || ncomp == 5 || ncomp == 6 || ncomp == 11; }
+
+hipError_t launch_diff_filter_lhs(int P, int Q, int ncomp, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s) {
+  switch (ncomp) {
+    case 1:
+      return dispatch_pq<RunField, DiffFilterLHS<1>>(P, Q, a, ctx, s);
+    case 5:
+      return dispatch_pq<RunField, DiffFilterLHS<5>>(P, Q, a, ctx, s);
+    case 6:
+      return dispatch_pq<RunField, DiffFilterLHS<6>>(P, Q, a, ctx, s);
+    case 11:
+      return dispatch_pq<RunField, DiffFilterLHS<11>>(P, Q, a, ctx, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_diff_filter_lhs_diagonal(int P, int Q, int ncomp, const FieldArgs &a, const CeedMiDiffFilterContext &ctx, hipStream_t s) {
+  if (a.nelem <= 0) return hipSuccess;
+#define FD_DIAG(p, q)                                                                        \
+  if (P == p && Q == q) {                                                                    \
+    constexpr int EPB   = sgs_elems_per_block<q>();                                          \
+    constexpr int block = EPB * q * q * q;                                                   \
+    auto          kern  = k_filter_lhs_diagonal<p, q>;                                       \
+    const int     grid  = grid_for_batches(kern, block, ((long long)a.nelem + EPB - 1) / EPB); \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, a, ctx, ncomp);                  \
+    return hipGetLastError();                                                                \
+  }
+  SGS_PQ_LIST(FD_DIAG)
+#undef FD_DIAG
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_anisotropy_collocate(int Q, int nelem, const double *qdata, QDataLayout qd, double *out, QDataLayout ol, hipStream_t s) {
+  const long long n = (long long)nelem * Q * Q * Q;
+  if (n <= 0) return hipSuccess;
+  const int grid = (int)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_anisotropy_collocate, dim3(grid), dim3(256), 0, s, Q * Q * Q, n, qdata, qd, out, ol);
+  return hipGetLastError();
+}
+
+hipError_t launch_sgs_training_nodal(const SgsNodalArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
+  if (a.nrows <= 0) return hipSuccess;
+  const long long nb = (a.nrows + kNodalBlock - 1) / kNodalBlock;
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sgs_training_nodal, dim3((int)nb), dim3(kNodalBlock), 0, s, a, ctx);
+  return hipGetLastError();
+}

@@ -440,8 +440,8 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
 template <typename K>
 int grid_for(K kernel, int block, long long nbatch) {
   int dev = 0, ncu = 0, per_cu = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
   // one persistent block per resident slot: each XCD's blocks then sweep a contiguous window of its batch range
   long long g = (long long)ncu * (per_cu > 0 ? per_cu : 1);

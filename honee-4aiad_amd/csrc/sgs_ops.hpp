@@ -172,8 +172,10 @@ __device__ __forceinline__ void orient_basis(double E[3][3], const double v[3]) 
     for (int k = 0; k < 3; k++) E[1][k] *= -1;
 }
 
-// ComputeSgsDD_Fused: nodal inputs -> Kelvin-Mandel SGS stress
-__device__ __forceinline__ void sgs_stress(const Model &m, const double gu[3][3], const double km_A[6], double delta, double nu, double km_sgs[6]) {
+// ComputeSgsDDInputs (qfunctions/sgs_dd_utils.h:58-90): eigenframe strain rates, vorticity components and the viscous
+// scale, normalized by |grad u| + epsilon; E holds the oriented eigenvectors (rows), gmag |grad u|
+__device__ __forceinline__ void sgs_inputs(const double gu[3][3], const double km_A[6], double delta, double nu, double E[3][3], double in[6],
+                                           double &gmag) {
   double A[3][3], gi[3][3];
   km_unpack(km_A, A);
 #pragma unroll
@@ -182,7 +184,7 @@ __device__ __forceinline__ void sgs_stress(const Model &m, const double gu[3][3]
     for (int j = 0; j < 3; j++) gi[i][j] = gu[i][0] * A[0][j] + gu[i][1] * A[1][j] + gu[i][2] * A[2][j];
   const double w      = 1 / sqrt(2.);
   const double kms[6] = {gi[0][0], gi[1][1], gi[2][2], w * (gi[1][2] + gi[2][1]), w * (gi[0][2] + gi[2][0]), w * (gi[0][1] + gi[1][0])};
-  double       S[3][3], sev[3], E[3][3];
+  double       S[3][3], sev[3];
   km_unpack(kms, S);
   diagonalize3(S, 5, sev, E);
   const double r12 = 0.5 * (gi[1][2] - gi[2][1]), r02 = 0.5 * (gi[0][2] - gi[2][0]), r01 = 0.5 * (gi[0][1] - gi[1][0]);
@@ -191,12 +193,16 @@ __device__ __forceinline__ void sgs_stress(const Model &m, const double gu[3][3]
   double vs[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) vs[i] = E[i][0] * om[0] + E[i][1] * om[1] + E[i][2] * om[2];
-  const double gmag = sqrt((sev[0] * sev[0] + sev[1] * sev[1] + sev[2] * sev[2]) + 0.5 * (vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2]));
-  double       in[6] = {sev[0], sev[1], sev[2], vs[0], vs[1], nu / (delta * delta)};
-  const double sc    = 1 / (gmag + 0x1p-52);
+  gmag = sqrt((sev[0] * sev[0] + sev[1] * sev[1] + sev[2] * sev[2]) + 0.5 * (vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2]));
+  in[0] = sev[0], in[1] = sev[1], in[2] = sev[2], in[3] = vs[0], in[4] = vs[1], in[5] = nu / (delta * delta);
+  const double sc = 1 / (gmag + 0x1p-52);
 #pragma unroll
   for (int k = 0; k < 6; k++) in[k] *= sc;
-  // MLP
+}
+
+// MLP inference (DataDrivenInference, qfunctions/sgs_dd_model.h:37-53), de-normalization and rotation back to the
+// physical frame (ComputeSgsDDOutputs, qfunctions/sgs_dd_utils.h:92-127)
+__device__ __forceinline__ void sgs_finish(const Model &m, const double in[6], double delta, const double E[3][3], double gmag, double km_sgs[6]) {
   double V[kNeurons];
 #pragma unroll
   for (int i = 0; i < kNeurons; i++) {
@@ -234,6 +240,13 @@ __device__ __forceinline__ void sgs_stress(const Model &m, const double gu[3][3]
 #pragma unroll
     for (int j = 0; j < 3; j++) T[i][j] = ES[i][0] * E[0][j] + ES[i][1] * E[1][j] + ES[i][2] * E[2][j];
   km_pack(T, km_sgs);
+}
+
+// ComputeSgsDD_Fused (qfunctions/sgs_dd_model.h:55-63): nodal inputs -> Kelvin-Mandel SGS stress
+__device__ __forceinline__ void sgs_stress(const Model &m, const double gu[3][3], const double km_A[6], double delta, double nu, double km_sgs[6]) {
+  double E[3][3], in[6], gmag;
+  sgs_inputs(gu, km_A, delta, nu, E, in, gmag);
+  sgs_finish(m, in, delta, E, gmag, km_sgs);
 }
 
 // FluxSubgridStress: F[0] = 0, F[1+k][j] = sgs[k][j], F[4][j] = u . sgs[:, j]

@@ -134,6 +134,13 @@ def load_libs():
         lib.HoneeMiVelocityGradientProjection.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiSgsNodalStress.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiApplySgsResidual.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiProblemSetupDiffFilter.argtypes = [ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiSetFilterWidthScaling.argtypes = [ct.c_void_p, _dp]
+        lib.HoneeMiDiffFilterRHS.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiDiffFilterLHS.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiDiffFilterApply.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, _ip]
+        lib.HoneeMiProblemSetupSgsTraining.argtypes = [ct.c_void_p]
+        lib.HoneeMiSgsTrainingData.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_double, ct.c_void_p, _ip]
         c = _libs["ceed"]
         c.CeedMiSetKernelTiming.argtypes = [ct.c_void_p, ct.c_bool]
         c.CeedMiResetKernelTimes.argtypes = [ct.c_void_p]
@@ -145,6 +152,12 @@ def load_libs():
 
 class HoneeMiSgsDDParams(ct.Structure):
     _fields_ = [("alpha", ct.c_double), ("b1", _dp), ("b2", _dp), ("w1", _dp), ("w2", _dp), ("out_scaling", _dp)]
+
+
+class HoneeMiDiffFilterOptions(ct.Structure):
+    _fields_ = [("grid_based_width", ct.c_int), ("width_scaling", ct.c_double * 3), ("kernel_scaling", ct.c_double),
+                ("friction_length", ct.c_double), ("damping_function", ct.c_int), ("damping_constant", ct.c_double),
+                ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_max_it", ct.c_int)]
 
 
 def _ptr(a):
@@ -258,6 +271,45 @@ class NewtonianProblem:
     def apply_sgs_residual(self, q, km_sgs, g, mem=CEED_MEM_HOST):
         """g += the SGS residual of IFunction_NodalSgs for the nodal stress km_sgs."""
         self._check(self.lib.HoneeMiApplySgsResidual(self.handle, mem, _ptr(q), _ptr(km_sgs), _ptr(g)), "ApplySgsResidual")
+
+    def setup_diff_filter(self, grid_based_width=False, width_scaling=(1.0, 1.0, 1.0), kernel_scaling=0.1, friction_length=0.0,
+                          damping_function=0, damping_constant=25.0, ksp_rtol=0.0, ksp_atol=0.0, ksp_max_it=0):
+        """Differential filter (src/differential_filter.c:177-275); defaults as DifferentialFilterSetup's."""
+        o = HoneeMiDiffFilterOptions(int(grid_based_width), (ct.c_double * 3)(*width_scaling), kernel_scaling, friction_length,
+                                     int(damping_function), damping_constant, ksp_rtol, ksp_atol, int(ksp_max_it))
+        self._check(self.lib.HoneeMiProblemSetupDiffFilter(self.handle, ct.byref(o)), "SetupDiffFilter")
+
+    def set_filter_width_scaling(self, scaling):
+        arr = np.ascontiguousarray(scaling, dtype=np.float64)
+        self._check(self.lib.HoneeMiSetFilterWidthScaling(self.handle, arr.ctypes.data_as(_dp)), "SetFilterWidthScaling")
+
+    def diff_filter_rhs(self, q):
+        out = np.zeros(11 * self.nnodes)
+        self._check(self.lib.HoneeMiDiffFilterRHS(self.handle, CEED_MEM_HOST, _ptr(q), _ptr(out)), "DiffFilterRHS")
+        return out
+
+    def diff_filter_lhs(self, u):
+        """The filter operator applied to u [nnodes][11], and its diagonal."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v, d = np.zeros(11 * self.nnodes), np.zeros(11 * self.nnodes)
+        self._check(self.lib.HoneeMiDiffFilterLHS(self.handle, _ptr(u), _ptr(v), _ptr(d)), "DiffFilterLHS")
+        return v, d
+
+    def diff_filter_apply(self, q, mem=CEED_MEM_HOST):
+        out = np.zeros(11 * self.nnodes)
+        its = ct.c_int(0)
+        self._check(self.lib.HoneeMiDiffFilterApply(self.handle, mem, _ptr(q), _ptr(out), ct.byref(its)), "DiffFilterApply")
+        return out, its.value
+
+    def setup_sgs_training(self):
+        self._check(self.lib.HoneeMiProblemSetupSgsTraining(self.handle), "SetupSgsTraining")
+
+    def sgs_training_data(self, q, width_scale, mem=CEED_MEM_HOST):
+        """Training pairs [nnodes][12] (six model inputs, six filtered SGS stress components) for one filter width."""
+        out = np.zeros(12 * self.nnodes)
+        its = ct.c_int(0)
+        self._check(self.lib.HoneeMiSgsTrainingData(self.handle, mem, _ptr(q), float(width_scale), _ptr(out), ct.byref(its)), "SgsTrainingData")
+        return out.reshape(self.nnodes, 12), its.value
 
     BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
 

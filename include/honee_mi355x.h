@@ -76,6 +76,23 @@ typedef struct {
   double data[1];
 } CeedMiSgsDDContext;
 
+// struct DifferentialFilterContext_ (qfunctions/differential_filter.h:16-25); damping_function is
+// enum DifferentialFilterDampingFunction: 0 none, 1 van Driest, 2 MMS
+typedef struct {
+  bool                   grid_based_width;
+  double                 width_scaling[3];
+  double                 kernel_scaling;
+  double                 friction_length;
+  int                    damping_function;
+  double                 damping_constant;
+  CeedMiNewtonianContext newt_ctx;
+} CeedMiDiffFilterContext;
+
+// struct SGS_DD_TrainingContext_ (qfunctions/sgs_dd_training.h:17-20)
+typedef struct {
+  CeedMiNewtonianContext newt_ctx;
+} CeedMiSgsTrainingContext;
+
 // ---- Backend extensions (no counterpart in libCEED; used by this repository's driver, tests and benchmark) ----
 
 // HIP stream the backend launches on (a hipStream_t).

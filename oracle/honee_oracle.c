@@ -887,6 +887,235 @@ int oracle_ifunction_nodal_sgs(const OMesh *m, const ONewt *c, const double *qda
   return 0;
 }
 
+// ----------------------------------------------------------------------------------------------------------------------
+// Differential filter (qfunctions/differential_filter.h, src/differential_filter.c) and SGS training pairs
+// (qfunctions/sgs_dd_training.h, src/smartsim/sgs_dd_training.c).  The filtered field has 11 interleaved components
+// per node: the primitive state (5, DIFF_FILTER_STATE_NUM) then the velocity products (6, order XX YY ZZ YZ XZ XY).
+// ----------------------------------------------------------------------------------------------------------------------
+// AnisotropyTensorCollocate (qfunctions/grid_anisotropy_tensor.h:57-77): A [nelem][7][Q^3] = [KM(A_ij), delta]
+int oracle_anisotropy_collocate(const OMesh *m, const double *qdata, double *A) {
+  const int Q3 = ipow3(m->Q);
+  for (int e = 0; e < m->nelem; e++) {
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double wdetJ, dXdx[3][3], kmA[6], delta;
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      o_anisotropy_tensor(dXdx, kmA, &delta);
+      for (int k = 0; k < 6; k++) A[((size_t)e * 7 + k) * Q3 + i] = kmA[k];
+      A[((size_t)e * 7 + 6) * Q3 + i] = delta;
+    }
+  }
+  return 0;
+}
+
+static inline double fmax_ref(double a, double b) { return a < b ? b : a; }  // Max (qfunctions/utils.h:14)
+
+// alpha_ij of DifferentialFilter_LHS_N (qfunctions/differential_filter.h:91-125) at one point
+static void o_filter_alpha(const ODiffFilter *f, const double *A7, const double x[3], double alpha[3][3]) {
+  double D[3][3] = {{0}};
+  if (f->grid_based_width) {
+    double km[6];
+    for (int k = 0; k < 6; k++) km[k] = A7[k] * A7[6];  // dimensionalize the normalized anisotropy tensor
+    o_km_unpack(km, D);
+  } else {
+    D[0][0] = D[1][1] = D[2][2] = 1;
+  }
+  double S[3][3] = {{0}};
+  if (f->damping == 1) {  // van Driest: -expm1(-y+ / A+)
+    const double dc = -expm1(-(x[1] / f->friction_length) / f->damping_constant);
+    S[0][0]         = fmax_ref(1, dc * f->width_scaling[0]);
+    S[1][1]         = dc * f->width_scaling[1];
+    S[2][2]         = fmax_ref(1, dc * f->width_scaling[2]);
+  } else if (f->damping == 0) {
+    S[0][0] = f->width_scaling[0];
+    S[1][1] = f->width_scaling[1];
+    S[2][2] = f->width_scaling[2];
+  } else {
+    S[0][0] = 1;
+    S[1][1] = tanh(60 * x[1]);
+    S[2][2] = 1;
+  }
+  double SD[3][3] = {{0}};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) SD[i][j] += S[i][k] * D[k][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      alpha[i][j] = 0;
+      for (int k = 0; k < 3; k++) alpha[i][j] += SD[i][k] * SD[k][j];
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) alpha[i][j] *= f->kernel_scaling;
+}
+
+// DifferentialFilter_RHS (qfunctions/differential_filter.h:28-52): r [nnodes][11] = sum_e B^T wdetJ [Y, u_i u_j]
+int oracle_diff_filter_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *r) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * 11 * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[5 * 512], uq[5 * 512], vq[11 * 512];
+    gather5(m, e, q, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, 5, ue, uq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double qi[5];
+      for (int k = 0; k < 5; k++) qi[k] = uq[k * Q3 + i];
+      const double wdetJ = qd[i];
+      const OState s     = o_state_from_q(c->gas, qi, c->state_var);
+      const double *u    = &s.Y[1];
+      for (int k = 0; k < 5; k++) vq[k * Q3 + i] = wdetJ * s.Y[k];
+      vq[5 * Q3 + i]  = wdetJ * u[0] * u[0];
+      vq[6 * Q3 + i]  = wdetJ * u[1] * u[1];
+      vq[7 * Q3 + i]  = wdetJ * u[2] * u[2];
+      vq[8 * Q3 + i]  = wdetJ * u[1] * u[2];
+      vq[9 * Q3 + i]  = wdetJ * u[0] * u[2];
+      vq[10 * Q3 + i] = wdetJ * u[0] * u[1];
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, 11, vq, NULL, &E[(size_t)e * 11 * P3]);
+  }
+  memset(r, 0, sizeof(double) * 11 * (size_t)m->nnodes);
+  scatterN_add(m, E, 11, r);
+  free(E);
+  return 0;
+}
+
+// DifferentialFilter_LHS_N (qfunctions/differential_filter.h:63-138) on an ncomp-component nodal field, the same
+// operator for every component; Acol is AnisotropyTensorCollocate's [nelem][7][Q^3] (read when grid_based_width)
+int oracle_diff_filter_lhs(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, const double *u, double *v) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * ncomp * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double *ue = malloc(sizeof(double) * ncomp * P3), *uq = malloc(sizeof(double) * ncomp * Q3), *duq = malloc(sizeof(double) * 3 * ncomp * Q3);
+    double *vq = malloc(sizeof(double) * ncomp * Q3), *gvq = malloc(sizeof(double) * 3 * ncomp * Q3), xq[3 * 512];
+    gatherN(m, e, u, ncomp, ue);
+    interp_grad(m->P, m->Q, m->B, m->D, ncomp, ue, uq, duq);
+    elem_coords(m, e, xq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int i = 0; i < Q3; i++) {
+      double wdetJ, dXdx[3][3], A7[7], x[3], alpha[3][3];
+      unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+      for (int k = 0; k < 7; k++) A7[k] = Acol[((size_t)e * 7 + k) * Q3 + i];
+      for (int d = 0; d < 3; d++) x[d] = xq[d * Q3 + i];
+      o_filter_alpha(f, A7, x, alpha);
+      for (int j = 0; j < ncomp; j++) {
+        vq[j * Q3 + i] = wdetJ * uq[j * Q3 + i];
+        double dq[3], dqX[3], dqa[3];
+        for (int k = 0; k < 3; k++)
+          dq[k] = duq[(0 * ncomp + j) * Q3 + i] * dXdx[0][k] + duq[(1 * ncomp + j) * Q3 + i] * dXdx[1][k] + duq[(2 * ncomp + j) * Q3 + i] * dXdx[2][k];
+        for (int k = 0; k < 3; k++) dqX[k] = dXdx[k][0] * dq[0] + dXdx[k][1] * dq[1] + dXdx[k][2] * dq[2];
+        for (int k = 0; k < 3; k++) dqa[k] = alpha[k][0] * dqX[0] + alpha[k][1] * dqX[1] + alpha[k][2] * dqX[2];
+        for (int k = 0; k < 3; k++) gvq[(k * ncomp + j) * Q3 + i] = wdetJ * dqa[k];
+      }
+    }
+    interp_grad_t(m->P, m->Q, m->B, m->D, ncomp, vq, gvq, &E[(size_t)e * ncomp * P3]);
+    free(ue), free(uq), free(duq), free(vq), free(gvq);
+  }
+  memset(v, 0, sizeof(double) * ncomp * (size_t)m->nnodes);
+  scatterN_add(m, E, ncomp, v);
+  free(E);
+  return 0;
+}
+
+// Diagonal of the filter operator (every component the same): element operator applied to unit node vectors
+int oracle_diff_filter_lhs_diagonal(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, double *d) {
+  const int P3 = ipow3(m->P), Q3 = ipow3(m->Q);
+  double   *E  = malloc(sizeof(double) * (size_t)m->nelem * ncomp * P3);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < m->nelem; e++) {
+    double ue[512], uq[512], duq[3 * 512], vq[512], gvq[3 * 512], re[512], xq[3 * 512];
+    elem_coords(m, e, xq, NULL);
+    const double *qd = &qdata[(size_t)e * 10 * Q3];
+    for (int n = 0; n < P3; n++) {
+      memset(ue, 0, sizeof(double) * P3);
+      ue[n] = 1.0;
+      interp_grad(m->P, m->Q, m->B, m->D, 1, ue, uq, duq);
+      for (int i = 0; i < Q3; i++) {
+        double wdetJ, dXdx[3][3], A7[7], x[3], alpha[3][3];
+        unpack_qdata(qd, Q3, i, &wdetJ, dXdx);
+        for (int k = 0; k < 7; k++) A7[k] = Acol[((size_t)e * 7 + k) * Q3 + i];
+        for (int dd = 0; dd < 3; dd++) x[dd] = xq[dd * Q3 + i];
+        o_filter_alpha(f, A7, x, alpha);
+        vq[i] = wdetJ * uq[i];
+        double dq[3], dqX[3];
+        for (int k = 0; k < 3; k++) dq[k] = duq[0 * Q3 + i] * dXdx[0][k] + duq[1 * Q3 + i] * dXdx[1][k] + duq[2 * Q3 + i] * dXdx[2][k];
+        for (int k = 0; k < 3; k++) dqX[k] = dXdx[k][0] * dq[0] + dXdx[k][1] * dq[1] + dXdx[k][2] * dq[2];
+        for (int k = 0; k < 3; k++) gvq[k * Q3 + i] = wdetJ * (alpha[k][0] * dqX[0] + alpha[k][1] * dqX[1] + alpha[k][2] * dqX[2]);
+      }
+      interp_grad_t(m->P, m->Q, m->B, m->D, 1, vq, gvq, re);
+      for (int k = 0; k < ncomp; k++) E[((size_t)e * ncomp + k) * P3 + n] = re[n];
+    }
+  }
+  memset(d, 0, sizeof(double) * ncomp * (size_t)m->nnodes);
+  scatterN_add(m, E, ncomp, d);
+  free(E);
+  return 0;
+}
+
+// KSPCG with PCJACOBI (diagonal) and KSP_NORM_NATURAL from a zero initial guess (src/differential_filter.c:176-185):
+// converged when sqrt(r.z) <= max(rtol sqrt(r0.z0), atol); at most max_it iterations.  Returns the iteration count.
+int oracle_diff_filter_solve(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, const double *R, double rtol,
+                             double atol, int max_it, double *X) {
+  const size_t n    = (size_t)ncomp * m->nnodes;
+  double      *dinv = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n),
+         *Ap = malloc(sizeof(double) * n);
+  oracle_diff_filter_lhs_diagonal(m, f, qdata, Acol, ncomp, dinv);
+  for (size_t i = 0; i < n; i++) dinv[i] = dinv[i] != 0 ? 1.0 / dinv[i] : 1.0;
+  memset(X, 0, sizeof(double) * n);
+  memcpy(r, R, sizeof(double) * n);
+  for (size_t i = 0; i < n; i++) z[i] = dinv[i] * r[i];
+  memcpy(p, z, sizeof(double) * n);
+  double rz = 0;
+  for (size_t i = 0; i < n; i++) rz += r[i] * z[i];
+  const double ttol = fmax(rtol * sqrt(fabs(rz)), atol);
+  int          its  = 0;
+  while (sqrt(fabs(rz)) > ttol && its < max_it) {
+    oracle_diff_filter_lhs(m, f, qdata, Acol, ncomp, p, Ap);
+    double pAp = 0;
+    for (size_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+    const double alpha = rz / pAp;
+    for (size_t i = 0; i < n; i++) {
+      X[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+      z[i] = dinv[i] * r[i];
+    }
+    double rz_new = 0;
+    for (size_t i = 0; i < n; i++) rz_new += r[i] * z[i];
+    const double beta = rz_new / rz;
+    for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    rz = rz_new;
+    its++;
+  }
+  free(dinv), free(r), free(z), free(p), free(Ap);
+  return its;
+}
+
+// ComputeSGS_DDAnisotropicTrainingDataNodal_Prim (qfunctions/sgs_dd_training.h:23-60) at every node: Yf [nnodes][11]
+// (filtered primitive state, filtered velocity products), G [nnodes][9], A [nnodes][7] -> [nnodes][12]: the six model
+// inputs, then the filtered SGS stress components XX YY ZZ YZ XZ XY
+int oracle_sgs_training_nodal(const OGas *g, int nnodes, const double *Yf, const double *G, const double *A, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int nd = 0; nd < nnodes; nd++) {
+    const double *yf = &Yf[(size_t)nd * 11], *vp = yf + 5;
+    const OState  s  = o_state_from_q(*g, yf, 1);
+    double        gu[3][3], E[3][3], in[6], gmag;
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) gu[j][k] = G[(size_t)nd * 9 + j * 3 + k];
+    o_sgs_inputs(gu, &A[(size_t)nd * 7], A[(size_t)nd * 7 + 6], g->mu / s.U[0], E, in, &gmag);
+    double *o = &out[(size_t)nd * 12];
+    for (int k = 0; k < 6; k++) o[k] = in[k];
+    const double *u = &s.Y[1];
+    o[6]            = vp[0] - u[0] * u[0];
+    o[7]            = vp[1] - u[1] * u[1];
+    o[8]            = vp[2] - u[2] * u[2];
+    o[9]            = vp[3] - u[1] * u[2];
+    o[10]           = vp[4] - u[0] * u[2];
+    o[11]           = vp[5] - u[0] * u[1];
+  }
+  return 0;
+}
+
 // Pointwise entry points (tests of the eigen solver, anisotropy tensor and model)
 int oracle_diagonalize3(const double *A_in, int max_sweeps, double *eval, double *evec) {
   double A[3][3], E[3][3];

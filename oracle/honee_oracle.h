@@ -114,6 +114,20 @@ int oracle_mass_n_diagonal(const OMesh *m, const double *qdata, int ncomp, doubl
 int oracle_velocity_gradient_projection(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *G);
 int oracle_grid_anisotropy_projection(const OMesh *m, const double *qdata, double *X);
 int oracle_ifunction_nodal_sgs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, const double *km_sgs, double *v);
+// Differential filter options (struct DifferentialFilterContext_, qfunctions/differential_filter.h:16-25)
+typedef struct {
+  int    grid_based_width;
+  double width_scaling[3], kernel_scaling, friction_length;
+  int    damping;  // 0 none, 1 van Driest, 2 MMS
+  double damping_constant;
+} ODiffFilter;
+int oracle_anisotropy_collocate(const OMesh *m, const double *qdata, double *A);
+int oracle_diff_filter_rhs(const OMesh *m, const ONewt *c, const double *qdata, const double *q, double *r);
+int oracle_diff_filter_lhs(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, const double *u, double *v);
+int oracle_diff_filter_lhs_diagonal(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, double *d);
+int oracle_diff_filter_solve(const OMesh *m, const ODiffFilter *f, const double *qdata, const double *Acol, int ncomp, const double *R, double rtol,
+                             double atol, int max_it, double *X);
+int oracle_sgs_training_nodal(const OGas *g, int nnodes, const double *Yf, const double *G, const double *A, double *out);
 int oracle_diagonalize3(const double *A, int max_sweeps, double *eval, double *evec);
 void oracle_anisotropy_tensor(const double *dXdx, double *km_A, double *delta);
 

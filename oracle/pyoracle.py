@@ -525,3 +525,54 @@ def ifunction_nodal_sgs(m: Mesh, c: ONewt, qdata, q, km_sgs):
     v = np.zeros(5 * m.nnodes)
     lib().oracle_ifunction_nodal_sgs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(km_sgs), _d(v))
     return v
+
+
+# ---- differential filter and SGS training pairs (honee_oracle.c) ----
+class ODiffFilter(ct.Structure):
+    _fields_ = [("grid_based_width", ct.c_int), ("width_scaling", ct.c_double * 3), ("kernel_scaling", ct.c_double),
+                ("friction_length", ct.c_double), ("damping", ct.c_int), ("damping_constant", ct.c_double)]
+
+
+def diff_filter(grid_based_width=False, width_scaling=(1.0, 1.0, 1.0), kernel_scaling=0.1, friction_length=0.0, damping=0,
+                damping_constant=25.0) -> ODiffFilter:
+    """Options of DifferentialFilterSetup (src/differential_filter.c:236-260), defaults included."""
+    return ODiffFilter(int(grid_based_width), (ct.c_double * 3)(*width_scaling), kernel_scaling, friction_length, int(damping), damping_constant)
+
+
+def anisotropy_collocate(m: Mesh, qdata):
+    A = np.zeros(m.nelem * 7 * m.Q3)
+    lib().oracle_anisotropy_collocate(ct.byref(m.s), _d(qdata), _d(A))
+    return A
+
+
+def diff_filter_rhs(m: Mesh, c: ONewt, qdata, q):
+    r = np.zeros(11 * m.nnodes)
+    lib().oracle_diff_filter_rhs(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), _d(r))
+    return r
+
+
+def diff_filter_lhs(m: Mesh, f: ODiffFilter, qdata, Acol, ncomp, u):
+    v = np.zeros(ncomp * m.nnodes)
+    lib().oracle_diff_filter_lhs(ct.byref(m.s), ct.byref(f), _d(qdata), _d(Acol), ncomp, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(v))
+    return v
+
+
+def diff_filter_lhs_diagonal(m: Mesh, f: ODiffFilter, qdata, Acol, ncomp):
+    d = np.zeros(ncomp * m.nnodes)
+    lib().oracle_diff_filter_lhs_diagonal(ct.byref(m.s), ct.byref(f), _d(qdata), _d(Acol), ncomp, _d(d))
+    return d
+
+
+def diff_filter_solve(m: Mesh, f: ODiffFilter, qdata, Acol, ncomp, R, rtol=1e-10, atol=1e-50, max_it=10000):
+    X = np.zeros(ncomp * m.nnodes)
+    its = lib().oracle_diff_filter_solve(ct.byref(m.s), ct.byref(f), _d(qdata), _d(Acol), ncomp, _d(np.ascontiguousarray(R, dtype=np.float64)),
+                                         ct.c_double(rtol), ct.c_double(atol), int(max_it), _d(X))
+    return X, its
+
+
+def sgs_training_nodal(gas: OGas, Yf, G, A):
+    n = Yf.size // 11
+    out = np.zeros(12 * n)
+    lib().oracle_sgs_training_nodal(ct.byref(gas), n, _d(np.ascontiguousarray(Yf, dtype=np.float64)), _d(np.ascontiguousarray(G, dtype=np.float64)),
+                                    _d(np.ascontiguousarray(A, dtype=np.float64)), _d(out))
+    return out
