@@ -120,7 +120,7 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
     fprintf(stderr, "CeedInit: no HIP device available for /gpu/hip/mi355x\n");
     return CEED_ERROR_BACKEND;
   }
-  hipGetDevice(&c->device);
+  (void)hipGetDevice(&c->device);
   // options: ":key=value" pairs
   size_t pos = res.find(':');
   while (pos != std::string::npos) {
@@ -198,13 +198,13 @@ extern "C" int CeedDestroy(Ceed *ceed) {
   }
   Ceed c = *ceed;
   for (auto &pe : c->pending_events) {
-    hipEventDestroy(pe.second.first);
-    hipEventDestroy(pe.second.second);
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
   }
-  for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
-  if (c->d_scratch) hipFree(c->d_scratch);
-  if (c->h_scratch) hipHostFree(c->h_scratch);
-  if (c->d_work) hipFree(c->d_work);
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->h_scratch) (void)hipHostFree(c->h_scratch);
+  if (c->d_work) (void)hipFree(c->d_work);
   delete c;
   *ceed = nullptr;
   return CEED_ERROR_SUCCESS;
@@ -227,7 +227,7 @@ static hipEvent_t event_get(Ceed ceed) {
     return e;
   }
   hipEvent_t e;
-  hipEventCreate(&e);
+  (void)hipEventCreate(&e);
   return e;
 }
 
@@ -540,8 +540,8 @@ extern "C" int CeedVectorNorm(CeedVector v, CeedNormType type, CeedScalar *norm)
   MI_CALL(vec_device_read(v, &d));
   const int npart = 1024;
   if (ceed->scratch_len < npart) {
-    if (ceed->d_scratch) hipFree(ceed->d_scratch);
-    if (ceed->h_scratch) hipHostFree(ceed->h_scratch);
+    if (ceed->d_scratch) (void)hipFree(ceed->d_scratch);
+    if (ceed->h_scratch) (void)hipHostFree(ceed->h_scratch);
     MI_HIP(ceed, hipMalloc((void **)&ceed->d_scratch, sizeof(double) * npart));
     MI_HIP(ceed, hipHostMalloc((void **)&ceed->h_scratch, sizeof(double) * npart));
     ceed->scratch_len = npart;
@@ -566,8 +566,8 @@ extern "C" int CeedMiVectorDot(CeedVector x, CeedVector y, double *result) {
   MI_CALL(vec_device_read(y, &dy));
   const int npart = 1024;
   if (ceed->scratch_len < npart) {
-    if (ceed->d_scratch) hipFree(ceed->d_scratch);
-    if (ceed->h_scratch) hipHostFree(ceed->h_scratch);
+    if (ceed->d_scratch) (void)hipFree(ceed->d_scratch);
+    if (ceed->h_scratch) (void)hipHostFree(ceed->h_scratch);
     MI_HIP(ceed, hipMalloc((void **)&ceed->d_scratch, sizeof(double) * npart));
     MI_HIP(ceed, hipHostMalloc((void **)&ceed->h_scratch, sizeof(double) * npart));
     ceed->scratch_len = npart;
@@ -648,8 +648,8 @@ extern "C" int CeedVectorDestroy(CeedVector *vec) {
     return CEED_ERROR_SUCCESS;
   }
   if (v->h_owned) free(v->h_array);
-  if (v->d_owned) hipFree(v->d_array);
-  if (v->d_shadow32) hipFree(v->d_shadow32);
+  if (v->d_owned) (void)hipFree(v->d_array);
+  if (v->d_shadow32) (void)hipFree(v->d_shadow32);
   Ceed c = v->ceed;
   delete v;
   CeedDestroy(&c);
@@ -844,11 +844,11 @@ extern "C" int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
     *rstr = nullptr;
     return CEED_ERROR_SUCCESS;
   }
-  if (r->d_offsets) hipFree(r->d_offsets);
-  if (r->d_t_rowptr) hipFree(r->d_t_rowptr);
-  if (r->d_t_perm) hipFree(r->d_t_perm);
-  if (r->d_t_index) hipFree(r->d_t_index);
-  if (r->d_t_offset) hipFree(r->d_t_offset);
+  if (r->d_offsets) (void)hipFree(r->d_offsets);
+  if (r->d_t_rowptr) (void)hipFree(r->d_t_rowptr);
+  if (r->d_t_perm) (void)hipFree(r->d_t_perm);
+  if (r->d_t_index) (void)hipFree(r->d_t_index);
+  if (r->d_t_offset) (void)hipFree(r->d_t_offset);
   Ceed c = r->ceed;
   delete r;
   CeedDestroy(&c);
@@ -1034,11 +1034,11 @@ extern "C" int CeedBasisDestroy(CeedBasis *basis) {
     *basis = nullptr;
     return CEED_ERROR_SUCCESS;
   }
-  if (b->d_interp_1d) hipFree(b->d_interp_1d);
-  if (b->d_grad_1d) hipFree(b->d_grad_1d);
-  if (b->d_collo) hipFree(b->d_collo);
-  if (b->d_weight_1d) hipFree(b->d_weight_1d);
-  if (b->d_prod) hipFree(b->d_prod);
+  if (b->d_interp_1d) (void)hipFree(b->d_interp_1d);
+  if (b->d_grad_1d) (void)hipFree(b->d_grad_1d);
+  if (b->d_collo) (void)hipFree(b->d_collo);
+  if (b->d_weight_1d) (void)hipFree(b->d_weight_1d);
+  if (b->d_prod) (void)hipFree(b->d_prod);
   Ceed c = b->ceed;
   delete b;
   CeedDestroy(&c);

@@ -981,6 +981,16 @@ int HoneeMiSgsTrainingData(HoneeMiProblem p, int mem_type, const double *q, doub
   return CopyVectorOut(p, p->train_data, mem_type, out);
 }
 
+// Fields of the last HoneeMiSgsTrainingData call (host copies): filtered [nnodes][11], projected filtered velocity
+// gradient [nnodes][9] and the projected grid anisotropy [nnodes][7]
+int HoneeMiSgsTrainingFields(HoneeMiProblem p, double *filtered, double *grad_velo, double *aniso) {
+  if (!p->train_enabled) return CEED_ERROR_INCOMPATIBLE;
+  if (filtered) CHK(p, CopyVectorOut(p, p->filtered, CEED_MEM_HOST, filtered));
+  if (grad_velo) CHK(p, CopyVectorOut(p, p->grad_velo, CEED_MEM_HOST, grad_velo));
+  if (aniso) CHK(p, CopyVectorOut(p, p->grid_aniso, CEED_MEM_HOST, aniso));
+  return 0;
+}
+
 // VelocityGradientProjectionApply (src/velocity_gradient_projection.c:145-155): q_ceed holds the state
 static int VelocityGradientProjectionApply(HoneeMiProblem p) {
   CHK(p, CeedOperatorApply(p->op_grad_velo_rhs, p->q_ceed, p->grad_velo, CEED_REQUEST_IMMEDIATE));

@@ -88,6 +88,7 @@ int HoneeMiDiffFilterApply(HoneeMiProblem p, int mem_type, const double *q, doub
 // SGS training pairs (src/smartsim/sgs_dd_training.c): [nnodes][12] per filter width scale
 int HoneeMiProblemSetupSgsTraining(HoneeMiProblem p);
 int HoneeMiSgsTrainingData(HoneeMiProblem p, int mem_type, const double *q, double width_scale, double *out, int *filter_iterations);
+int HoneeMiSgsTrainingFields(HoneeMiProblem p, double *filtered, double *grad_velo, double *aniso);  // host [nnodes][11], [9], [7]
 const char  *HoneeMiProblemError(HoneeMiProblem p);
 const char  *HoneeMiLastError(void);
 

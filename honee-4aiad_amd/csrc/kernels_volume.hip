@@ -442,7 +442,7 @@ int grid_for(K kernel, int block, long long nbatch) {
   int dev = 0, ncu = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
   // one persistent block per resident slot: each XCD's blocks then sweep a contiguous window of its batch range
   long long g = (long long)ncu * (per_cu > 0 ? per_cu : 1);
   if (g > nbatch) g = nbatch >= kNumXcd ? nbatch / kNumXcd * kNumXcd : nbatch;

@@ -141,6 +141,7 @@ def load_libs():
         lib.HoneeMiDiffFilterApply.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, _ip]
         lib.HoneeMiProblemSetupSgsTraining.argtypes = [ct.c_void_p]
         lib.HoneeMiSgsTrainingData.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_double, ct.c_void_p, _ip]
+        lib.HoneeMiSgsTrainingFields.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         c = _libs["ceed"]
         c.CeedMiSetKernelTiming.argtypes = [ct.c_void_p, ct.c_bool]
         c.CeedMiResetKernelTimes.argtypes = [ct.c_void_p]
@@ -310,6 +311,13 @@ class NewtonianProblem:
         its = ct.c_int(0)
         self._check(self.lib.HoneeMiSgsTrainingData(self.handle, mem, _ptr(q), float(width_scale), _ptr(out), ct.byref(its)), "SgsTrainingData")
         return out.reshape(self.nnodes, 12), its.value
+
+    def sgs_training_fields(self):
+        """Filtered field [nnodes][11], projected filtered velocity gradient [nnodes][9] and grid anisotropy [nnodes][7]
+        of the last sgs_training_data call."""
+        f, g, a = np.zeros((self.nnodes, 11)), np.zeros((self.nnodes, 9)), np.zeros((self.nnodes, 7))
+        self._check(self.lib.HoneeMiSgsTrainingFields(self.handle, _ptr(f), _ptr(g), _ptr(a)), "SgsTrainingFields")
+        return f, g, a
 
     BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
 

@@ -576,3 +576,79 @@ def sgs_training_nodal(gas: OGas, Yf, G, A):
     lib().oracle_sgs_training_nodal(ct.byref(gas), n, _d(np.ascontiguousarray(Yf, dtype=np.float64)), _d(np.ascontiguousarray(G, dtype=np.float64)),
                                     _d(np.ascontiguousarray(A, dtype=np.float64)), _d(out))
     return out
+
+
+# ---- SGS model training (FP64 restatement of the MLP, its gradient and Adam; the GPU trains in FP32) ----
+MLP_DIMS = (6, 20, 6)
+
+
+def mlp_unpack(theta, dims=MLP_DIMS):
+    """Parameter vector layout W1 [nh][ni], b1 [nh], W2 [no][nh], b2 [no]."""
+    ni, nh, no = dims
+    theta = np.asarray(theta, dtype=np.float64)
+    o = 0
+    W1 = theta[o:o + nh * ni].reshape(nh, ni)
+    o += nh * ni
+    b1 = theta[o:o + nh]
+    o += nh
+    W2 = theta[o:o + no * nh].reshape(no, nh)
+    o += no * nh
+    b2 = theta[o:o + no]
+    return W1, b1, W2, b2
+
+
+def mlp_forward(theta, X, alpha, dims=MLP_DIMS):
+    W1, b1, W2, b2 = mlp_unpack(theta, dims)
+    h = np.asarray(X, dtype=np.float64) @ W1.T + b1
+    a = np.where(h < 0, alpha * h, h)
+    return a @ W2.T + b2
+
+
+def mlp_loss_grad(theta, X, T, alpha, dims=MLP_DIMS):
+    """Mean squared error over samples and outputs and its gradient by back-propagation."""
+    W1, b1, W2, b2 = mlp_unpack(theta, dims)
+    X = np.asarray(X, dtype=np.float64)
+    T = np.asarray(T, dtype=np.float64)
+    n, no = T.shape
+    h = X @ W1.T + b1
+    a = np.where(h < 0, alpha * h, h)
+    y = a @ W2.T + b2
+    e = y - T
+    loss = float(np.sum(e * e) / (n * no))
+    dy = 2 * e / (n * no)
+    gW2 = dy.T @ a
+    gb2 = dy.sum(0)
+    dh = (dy @ W2) * np.where(h < 0, alpha, 1.0)
+    gW1 = dh.T @ X
+    gb1 = dh.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2])
+
+
+def adam_step(theta, grad, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step t (1-based) with bias-corrected moments."""
+    m = beta1 * m + (1 - beta1) * grad
+    v = beta2 * v + (1 - beta2) * grad * grad
+    mh = m / (1 - beta1 ** t)
+    vh = v / (1 - beta2 ** t)
+    return theta - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def sgs_model_samples(gas: OGas, train, filtered, G, A):
+    """Model inputs and the eigenframe stress S = E tau E^T / (delta^2 |grad u|^2) (order S00 S11 S22 S01 S02 S12) of
+    nodal training fields, with E and |grad u| from the oracle's closure inputs."""
+    zero = SgsModel({"b1": np.zeros(20), "b2": np.zeros(6), "W1": np.zeros((20, 6)), "W2": np.zeros((6, 20)), "bounds": np.zeros((6, 2)), "alpha": 0.0})
+    train, filtered, G, A = (np.asarray(a, dtype=np.float64).reshape(len(np.asarray(a).reshape(-1, k)), k)
+                             for a, k in ((train, 12), (filtered, 11), (G, 9), (A, 7)))
+    n = train.shape[0]
+    X, S = np.zeros((n, 6)), np.zeros((n, 6))
+    R = gas.cp - gas.cv
+    for i in range(n):
+        rho = filtered[i, 0] / (R * filtered[i, 4])
+        r = sgs_point(zero, G[i].reshape(3, 3), A[i, :6], A[i, 6], gas.mu / rho)
+        t = train[i, 6:]
+        tau = np.array([[t[0], t[5], t[4]], [t[5], t[1], t[3]], [t[4], t[3], t[2]]])
+        E = r["evec"]
+        Sm = E @ tau @ E.T / (A[i, 6] ** 2 * r["gmag"] ** 2)
+        X[i] = r["inputs"]
+        S[i] = [Sm[0, 0], Sm[1, 1], Sm[2, 2], Sm[0, 1], Sm[0, 2], Sm[1, 2]]
+    return X, S
