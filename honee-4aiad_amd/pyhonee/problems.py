@@ -96,3 +96,45 @@ def nodal_field(mesh: BoxMesh, per_node_fn) -> np.ndarray:
 
 def taylor_green_ic(mesh: BoxMesh, state_var=VAR_CONS, gas=TGV_GAS, ref_Y=TGV_REFERENCE_Y, background=(0.0, 0.0, 0.0), lx=2 * np.pi):
     return nodal_field(mesh, lambda x: q_from_prim(gas, taylor_green_prim(x, ref_Y, gas, background, lx), state_var))
+
+
+def random_fourier_modes(seed: int, kmax: int = 8, k0: float = 4.0, u_rms: float = 1.0, lx: float = 2 * np.pi):
+    """Random-phase solenoidal Fourier modes of a periodic box [0, lx]^3 with energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2),
+    wavevectors with 1 <= |k| <= kmax (one of each +-k pair), normalized to the given per-component RMS velocity.  Returns the
+    wavevectors [m][3] (physical), unit directions perpendicular to them [m][3], amplitudes [m] and phases [m]."""
+    rng = np.random.default_rng(seed)
+    rng_k = np.arange(-kmax, kmax + 1)
+    kk = np.stack(np.meshgrid(rng_k, rng_k, rng_k, indexing="ij"), -1).reshape(-1, 3)
+    kn = np.linalg.norm(kk, axis=1)
+    half = (kk[:, 0] > 0) | ((kk[:, 0] == 0) & (kk[:, 1] > 0)) | ((kk[:, 0] == 0) & (kk[:, 1] == 0) & (kk[:, 2] > 0))
+    sel = half & (kn >= 1) & (kn <= kmax)
+    kk, kn = kk[sel].astype(np.float64), kn[sel]
+    shell = np.rint(kn).astype(int)
+    counts = np.bincount(shell)
+    E = kn ** 4 * np.exp(-2 * (kn / k0) ** 2)
+    amp = np.sqrt(E / counts[shell])
+    d = rng.normal(size=kk.shape)
+    d -= (np.sum(d * kk, axis=1) / kn ** 2)[:, None] * kk
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    phase = rng.uniform(0, 2 * np.pi, size=len(kk))
+    amp *= u_rms / np.sqrt(np.sum(amp ** 2) / 6.0)  # per-component mean square: sum amp^2 / 2 over three components
+    return kk * (2 * np.pi / lx), d, amp, phase
+
+
+def random_fourier_velocity(xyz, modes) -> np.ndarray:
+    """Velocity of random_fourier_modes at points xyz [n][3]: sum_m amp_m d_m cos(k_m . x + phase_m)."""
+    kk, d, amp, phase = modes
+    u = np.zeros((len(xyz), 3))
+    for s in range(0, len(kk), 256):
+        arg = xyz @ kk[s:s + 256].T + phase[s:s + 256]
+        u += (np.cos(arg) * amp[s:s + 256]) @ d[s:s + 256]
+    return u
+
+
+def turbulence_prim(mesh: BoxMesh, modes, ref_Y=TGV_REFERENCE_Y) -> np.ndarray:
+    """Primitive state [nnodes][5]: reference pressure and temperature with the random Fourier velocity."""
+    def f(x):
+        Y = np.tile(np.asarray(ref_Y, dtype=np.float64), (len(x), 1))
+        Y[:, 1:4] = random_fourier_velocity(x, modes)
+        return Y
+    return nodal_field(mesh, f)
