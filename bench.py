@@ -161,6 +161,26 @@ def explicit_c1(steps=100, cfl=0.3, cpu=True):
     return out
 
 
+def les_study():
+    """The learned closure a posteriori: decaying isotropic turbulence (E(k) ~ k^4 exp(-2 (k/4)^2), u_rms 1, Mach 0.1),
+    plain fine run 96^3 p=2 (SSP-RK3) differentially filtered to the coarse element size, network trained on the GPU
+    on a second flow's filtered pairs at the coarse nodes, coarse 24^3 p=2 BDF2 LES with the FP32 Jacobian with and
+    without the closure.  Reports the coarse runs' shell-spectrum error against the filtered fine run at t = 1 (relative
+    l2 over 1 <= k <= 12), their GPU seconds against the fine run's, and LES error x GPU seconds."""
+    from pyhonee import les as LES
+
+    out = LES.study(log=lambda m: print(m, file=sys.stderr, flush=True))
+    res = {"config": out["config"], "fine_gpu_s": out["fine_gpu_s"], "fine_steps": out["fine_steps"],
+           "training_samples": out["training_samples"], "training_s": out["training_s"], "training_loss": out["training_loss"],
+           "filtered_ke_t_end": out["filtered_ke"][-1], "spectra_t_end": out["spectra_t_end"]}
+    for name in ("no_model", "closure"):
+        r = out[name]
+        res[name] = {"gpu_s": r["gpu_s"], "spectrum_error_t_end": r["spectrum_error"][-1], "spectrum_error": r["spectrum_error"],
+                     "ke_error_t_end": r["ke_error"][-1], "speedup_vs_fine": r["speedup_vs_fine"],
+                     "les_error_x_gpu_s": r["les_error_x_gpu_s"], "newton_its": r["newton_its"], "krylov_its": r["krylov_its"]}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +194,7 @@ def main():
                     help="element scatter-add: FP64 atomics, or the deterministic two-pass node gather")
     ap.add_argument("--ts-steps", type=int, default=5, help="timed BDF2 steps of the implicit workload (0 skips time integration)")
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
+    ap.add_argument("--no-les", action="store_true", help="skip the a-posteriori LES study of the learned closure")
     ap.add_argument("--calibrate-traffic", action="store_true",
                     help="only run kernels of known HBM traffic (3 reductions reading 1 GiB, 3 fills writing 1 GiB) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc")
@@ -292,6 +313,8 @@ def main():
     if not args.no_explicit:
         print("# explicit 8^3 p=2 run", file=sys.stderr, flush=True)
         line["explicit_c1"] = explicit_c1(cpu=not args.no_cpu_baseline)
+    if not args.no_les:
+        line["les"] = les_study()
     if not args.no_cpu_baseline:
         print("# timing CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(shape, degree)

@@ -442,10 +442,12 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
   };
   for (int it = 1; it <= o.snes_max_it && !converged; it++) {
     const bool block = o.ksp_pc == 1;
-    if (block) TS_OP(ts, HoneeMiIJacobianPointBlockDiagonal(ts->p, kMemDevice, shift, ts->dinv));
-    else TS_OP(ts, HoneeMiIJacobianDiagonal(ts->p, kMemDevice, shift, ts->dinv));
-    ts->st.diagonal_assemblies++;
-    TS_CALL(invert_diagonal(ts, ts->dinv, block, "Jacobian preconditioner"));
+    if ((it - 1) % std::max(1, o.pc_lag) == 0) {
+      if (block) TS_OP(ts, HoneeMiIJacobianPointBlockDiagonal(ts->p, kMemDevice, shift, ts->dinv));
+      else TS_OP(ts, HoneeMiIJacobianDiagonal(ts->p, kMemDevice, shift, ts->dinv));
+      ts->st.diagonal_assemblies++;
+      TS_CALL(invert_diagonal(ts, ts->dinv, block, "Jacobian preconditioner"));
+    }
     int kits = 0;
     TS_CALL(gmres(ts, apply, Precond{ts, ts->dinv, block}, ts->F, ts->dx, o.ksp_rtol, o.ksp_atol, o.ksp_restart, o.ksp_max_it, &kits));
     ts->st.krylov_its += kits;
@@ -540,6 +542,7 @@ void HoneeMiTSOptionsDefault(HoneeMiTSOptions *o) {
   o->ksp_restart        = 30;
   o->ksp_max_it         = 10000;
   o->ksp_pc             = 0;
+  o->pc_lag             = 1;
   o->mass_pc            = 0;
 }
 

@@ -40,6 +40,9 @@ typedef struct {
   int    ksp_restart, ksp_max_it;
   // preconditioners: 0 point-Jacobi (PCJACOBI), 1 point-block Jacobi on the 5x5 node blocks (PCVPBJACOBI)
   int    ksp_pc, mass_pc;
+  // rebuild the Jacobian preconditioner every pc_lag-th Newton iteration of a solve (the first always; PETSc's
+  // -snes_lag_preconditioner; 1 rebuilds every iteration)
+  int    pc_lag;
 } HoneeMiTSOptions;
 
 typedef struct {
