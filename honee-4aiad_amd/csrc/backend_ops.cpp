@@ -695,6 +695,21 @@ static int plan_bind_fields(CeedOperator op) {
       p->hllc        = qf->hllc;
       p->kernel_name = qf->kind == QfKind::Freestream ? "freestream" : "freestream_jacobian";
       return 0;
+    case QfKind::RiemannOutflow:  // bc_outflow.h:37-305 wired by problems/bc_outflow.c:114-180
+    case QfKind::RiemannOutflowJacobian:
+    case QfKind::PressureOutflow:
+    case QfKind::PressureOutflowJacobian: {
+      const bool jac = qf->kind == QfKind::RiemannOutflowJacobian || qf->kind == QfKind::PressureOutflowJacobian;
+      if ((ierr = bind_face_op(op, p, jac, 11))) return fail(ierr);
+      const bool pressure = qf->kind == QfKind::PressureOutflow || qf->kind == QfKind::PressureOutflowJacobian;
+      p->kernel_name      = std::string(pressure ? "pressure_outflow" : "riemann_outflow") + (jac ? "_jacobian" : "");
+      return 0;
+    }
+    case QfKind::Slip:  // bc_slip.h:11-111 wired by problems/bc_slip.c:59-73
+    case QfKind::SlipJacobian:
+      if ((ierr = bind_face_op(op, p, qf->kind == QfKind::SlipJacobian, 5))) return fail(ierr);
+      p->kernel_name = qf->kind == QfKind::Slip ? "slip" : "slip_jacobian";
+      return 0;
     case QfKind::VelocityGradientRHS:
     case QfKind::AnisotropyRHS:
     case QfKind::MassN:
@@ -724,6 +739,56 @@ static int newtonian_context(CeedOperator op, CeedMiNewtonianContext *ctx) {
   memcpy(ctx, c->data, sizeof(*ctx));
   if (ctx->divFdiff_method != 0)
     return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "%s: divergence of diffusive flux projection is not supported", op->qf->name.c_str());
+  return 0;
+}
+
+// Face sub-operators: whether the body is a Jacobian, and the context it reads (0 NewtonianIdealGasContext_,
+// 1 FreestreamContext_, 2 OutflowContext_)
+static bool face_kind(QfKind k, bool *jacobian, int *ctx_type) {
+  switch (k) {
+    case QfKind::BoundaryIntegral: *jacobian = false, *ctx_type = 0; return true;
+    case QfKind::BoundaryIntegralJacobian: *jacobian = true, *ctx_type = 0; return true;
+    case QfKind::Slip: *jacobian = false, *ctx_type = 0; return true;
+    case QfKind::SlipJacobian: *jacobian = true, *ctx_type = 0; return true;
+    case QfKind::Freestream: *jacobian = false, *ctx_type = 1; return true;
+    case QfKind::FreestreamJacobian: *jacobian = true, *ctx_type = 1; return true;
+    case QfKind::RiemannOutflow:
+    case QfKind::PressureOutflow: *jacobian = false, *ctx_type = 2; return true;
+    case QfKind::RiemannOutflowJacobian:
+    case QfKind::PressureOutflowJacobian: *jacobian = true, *ctx_type = 2; return true;
+    default: return false;
+  }
+}
+
+template <class C>
+static int sized_context(CeedOperator op, const char *what, C *out) {
+  CeedQFunctionContext c = op->qf->ctx;
+  if (!c || !c->data) return MI_ERR(op->ceed, CEED_ERROR_INCOMPLETE, "%s: missing QFunction context", op->qf->name.c_str());
+  if (c->size != sizeof(C))
+    return MI_ERR(op->ceed, CEED_ERROR_INCOMPATIBLE, "%s: context size %zu does not match struct %s (%zu)", op->qf->name.c_str(), c->size, what,
+                  sizeof(C));
+  memcpy(out, c->data, sizeof(C));
+  return 0;
+}
+
+struct FaceContexts {
+  CeedMiNewtonianContext  nctx{};
+  CeedMiFreestreamContext fctx{};
+  CeedMiOutflowContext    octx{};
+  bool                    implicit = false;
+};
+
+static int face_contexts(CeedOperator op, int ctx_type, FaceContexts *fc) {
+  if (ctx_type == 1) {
+    MI_CALL(sized_context(op, "FreestreamContext_", &fc->fctx));
+    fc->implicit = fc->fctx.newt_ctx.is_implicit;
+  } else if (ctx_type == 2) {
+    MI_CALL(sized_context(op, "OutflowContext_", &fc->octx));
+    fc->implicit = fc->octx.newt_ctx.is_implicit;
+  } else {
+    MI_CALL(newtonian_context(op, &fc->nctx));
+    fc->implicit = fc->nctx.is_implicit;
+  }
   return 0;
 }
 
@@ -1125,24 +1190,21 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
     case QfKind::BoundaryIntegral:
     case QfKind::BoundaryIntegralJacobian:
     case QfKind::Freestream:
-    case QfKind::FreestreamJacobian: {
-      const bool jacobian   = p->kind == QfKind::BoundaryIntegralJacobian || p->kind == QfKind::FreestreamJacobian;
-      const bool freestream = p->kind == QfKind::Freestream || p->kind == QfKind::FreestreamJacobian;
+    case QfKind::FreestreamJacobian:
+    case QfKind::RiemannOutflow:
+    case QfKind::RiemannOutflowJacobian:
+    case QfKind::PressureOutflow:
+    case QfKind::PressureOutflowJacobian:
+    case QfKind::Slip:
+    case QfKind::SlipJacobian: {
+      bool jacobian = false;
+      int  ctx_type = 0;
+      face_kind(p->kind, &jacobian, &ctx_type);
       if (in->length != p->r_q->l_size || out->length != p->r_q->l_size)
         return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: active vectors must have the face restriction's L-vector size", op->qf->name.c_str());
-      CeedMiNewtonianContext  nctx{};
-      CeedMiFreestreamContext fctx{};
-      if (freestream) {
-        CeedQFunctionContext c = op->qf->ctx;
-        if (!c || !c->data) return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "%s: missing QFunction context", op->qf->name.c_str());
-        if (c->size != sizeof(CeedMiFreestreamContext))
-          return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: context size %zu does not match struct FreestreamContext_ (%zu)", op->qf->name.c_str(),
-                        c->size, sizeof(CeedMiFreestreamContext));
-        memcpy(&fctx, c->data, sizeof(fctx));
-      } else {
-        MI_CALL(newtonian_context(op, &nctx));
-      }
-      const bool implicit = freestream ? fctx.newt_ctx.is_implicit : nctx.is_implicit;
+      FaceContexts fc;
+      MI_CALL(face_contexts(op, ctx_type, &fc));
+      const bool implicit = fc.implicit;
       VolumeArgs a{};
       a.nelem = p->nelem;
       MI_CALL(rstr_device_offsets(p->r_q, &a.offsets));
@@ -1165,9 +1227,19 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       } else if (jacobian) {
         return MI_ERR(ceed, CEED_ERROR_INCOMPLETE, "%s: surface jacobian data is required", op->qf->name.c_str());
       }
+      const bool fp32 = ceed->jac_fp32;
       MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
-      if (freestream) MI_HIP(ceed, launch_face_freestream(p->P, p->Q, p->var, p->hllc, jacobian, ceed->jac_fp32, a, fctx, s));
-      else MI_HIP(ceed, launch_face_boundary_integral(p->P, p->Q, p->var, jacobian, ceed->jac_fp32, a, nctx, s));
+      switch (p->kind) {
+        case QfKind::Freestream:
+        case QfKind::FreestreamJacobian: MI_HIP(ceed, launch_face_freestream(p->P, p->Q, p->var, p->hllc, jacobian, fp32, a, fc.fctx, s)); break;
+        case QfKind::RiemannOutflow:
+        case QfKind::RiemannOutflowJacobian: MI_HIP(ceed, launch_face_outflow(p->P, p->Q, p->var, false, jacobian, fp32, a, fc.octx, s)); break;
+        case QfKind::PressureOutflow:
+        case QfKind::PressureOutflowJacobian: MI_HIP(ceed, launch_face_outflow(p->P, p->Q, p->var, true, jacobian, fp32, a, fc.octx, s)); break;
+        case QfKind::Slip:
+        case QfKind::SlipJacobian: MI_HIP(ceed, launch_face_slip(p->P, p->Q, p->var, jacobian, fp32, a, fc.nctx, s)); break;
+        default: MI_HIP(ceed, launch_face_boundary_integral(p->P, p->Q, p->var, jacobian, fp32, a, fc.nctx, s)); break;
+      }
       MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
       MI_CALL(scatter_finish(p, ceed, a, overwrite));
       return 0;
@@ -1258,18 +1330,11 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     MI_CALL(timing_end(ceed, "mass_diagonal", ev));
     return 0;
   }
-  if (p->kind == QfKind::BoundaryIntegralJacobian || p->kind == QfKind::FreestreamJacobian) {  // face sub-operators
-    const bool              freestream = p->kind == QfKind::FreestreamJacobian;
-    CeedMiNewtonianContext  nctx{};
-    CeedMiFreestreamContext fctx{};
-    if (freestream) {
-      CeedQFunctionContext c = op->qf->ctx;
-      if (!c || !c->data || c->size != sizeof(CeedMiFreestreamContext))
-        return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: missing or mismatched FreestreamContext", op->qf->name.c_str());
-      memcpy(&fctx, c->data, sizeof(fctx));
-    } else {
-      MI_CALL(newtonian_context(op, &nctx));
-    }
+  bool face_jac = false;
+  int  face_ctx = 0;
+  if (face_kind(p->kind, &face_jac, &face_ctx) && face_jac) {  // face sub-operators
+    FaceContexts fc;
+    MI_CALL(face_contexts(op, face_ctx, &fc));
     if (diag->length != want) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "diagonal vector has the wrong size");
     VolumeArgs a{};
     a.nelem = p->nelem;
@@ -1285,7 +1350,23 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "face_jacobian_diagonal", &ev));
-    MI_HIP(ceed, launch_face_jacobian_diagonal(point_block, p->P, p->Q, p->var, freestream, p->hllc, ceed->jac_fp32, a, nctx, fctx, ceed->stream));
+    switch (p->kind) {
+      case QfKind::RiemannOutflowJacobian:
+        MI_HIP(ceed, launch_face_bc_jacobian_diagonal(point_block, p->P, p->Q, p->var, kFaceRiemannOutflow, ceed->jac_fp32, a, fc.nctx, fc.octx,
+                                                      ceed->stream));
+        break;
+      case QfKind::PressureOutflowJacobian:
+        MI_HIP(ceed, launch_face_bc_jacobian_diagonal(point_block, p->P, p->Q, p->var, kFacePressureOutflow, ceed->jac_fp32, a, fc.nctx, fc.octx,
+                                                      ceed->stream));
+        break;
+      case QfKind::SlipJacobian:
+        MI_HIP(ceed, launch_face_bc_jacobian_diagonal(point_block, p->P, p->Q, p->var, kFaceSlip, ceed->jac_fp32, a, fc.nctx, fc.octx, ceed->stream));
+        break;
+      default:
+        MI_HIP(ceed, launch_face_jacobian_diagonal(point_block, p->P, p->Q, p->var, p->kind == QfKind::FreestreamJacobian, p->hllc, ceed->jac_fp32, a,
+                                                   fc.nctx, fc.fctx, ceed->stream));
+        break;
+    }
     MI_CALL(timing_end(ceed, "face_jacobian_diagonal", ev));
     return 0;
   }

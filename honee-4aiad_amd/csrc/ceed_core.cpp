@@ -1113,6 +1113,13 @@ static void qf_identify(CeedQFunction qf) {
   } else if (n.rfind("BoundaryIntegral_", 0) == 0 && var_of(n) >= 0) {
     qf->kind      = QfKind::BoundaryIntegral;
     qf->state_var = var_of(n);
+  } else if (var_of(n) >= 0 && (n.rfind("RiemannOutflow_", 0) == 0 || n.rfind("PressureOutflow_", 0) == 0 || n.rfind("Slip_", 0) == 0)) {
+    // {RiemannOutflow,PressureOutflow,Slip}[_Jacobian]_{Conserv,Prim,Entropy} (bc_outflow.h, bc_slip.h)
+    const bool jac = n.find("_Jacobian_") != std::string::npos;
+    if (n.rfind("RiemannOutflow_", 0) == 0) qf->kind = jac ? QfKind::RiemannOutflowJacobian : QfKind::RiemannOutflow;
+    else if (n.rfind("PressureOutflow_", 0) == 0) qf->kind = jac ? QfKind::PressureOutflowJacobian : QfKind::PressureOutflow;
+    else qf->kind = jac ? QfKind::SlipJacobian : QfKind::Slip;
+    qf->state_var = var_of(n);
   } else if (n.rfind("Freestream_", 0) == 0) {  // Freestream[_Jacobian]_{Conserv,Prim,Entropy}_{HLL,HLLC}
     const bool        hllc = n.size() > 5 && n.compare(n.size() - 5, 5, "_HLLC") == 0;
     const bool        hll  = !hllc && n.size() > 4 && n.compare(n.size() - 4, 4, "_HLL") == 0;

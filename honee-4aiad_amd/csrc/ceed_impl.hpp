@@ -99,6 +99,12 @@ enum class QfKind {
   BoundaryIntegralJacobian,
   Freestream,
   FreestreamJacobian,
+  RiemannOutflow,
+  RiemannOutflowJacobian,
+  PressureOutflow,
+  PressureOutflowJacobian,
+  Slip,
+  SlipJacobian,
   VelocityGradientRHS,
   AnisotropyRHS,
   MassN,

@@ -1,6 +1,7 @@
 // Quadrature-point bodies of the fused boundary-face operators: HONEE's consistent boundary integral
-// (qfunctions/newtonian.h:328-431) and the freestream Riemann boundary (qfunctions/bc_freestream.h:14-114), each with
-// its Jacobian.  A body maps the interpolated face values (and, for the boundary integral, the two reference tangential
+// (qfunctions/newtonian.h:328-431), the freestream Riemann boundary (qfunctions/bc_freestream.h:14-114), the Riemann
+// and weak-pressure outflows (qfunctions/bc_outflow.h:37-305) and the slip wall (qfunctions/bc_slip.h:11-111), each
+// with its Jacobian.  A body maps the interpolated face values (and, for the boundary integral, the two reference tangential
 // derivatives) at one face quadrature point to the test-function weights v (interpolation transpose only), reading and
 // writing its strided face data directly.
 //
@@ -251,6 +252,355 @@ struct FreestreamJacobian {
     T dF[5];
     if constexpr (HLLC) ns::riemann_hllc_fwd(g, s, ds, r, dr, n, dF);
     else ns::riemann_hll_fwd(g, s, ds, r, dr, n, dF);
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = (double)(wdetJb * dF[c]);
+  }
+};
+
+// ViscousEnergyFlux (newtonian_state.h:560-565), direction j
+template <typename T>
+__device__ __forceinline__ T viscous_energy_flux_dir(const ns::Gas<T> &g, const ns::St<T> &s, const ns::St<T> gs[3], const T st[3][3], int j) {
+  return -s.Y[1] * st[0][j] - s.Y[2] * st[1][j] - s.Y[3] * st[2][j] - g.k * gs[j].Y[4];
+}
+
+// ViscousEnergyFlux_fwd (newtonian_state.h:567-576), direction j
+template <typename T>
+__device__ __forceinline__ T viscous_energy_flux_fwd_dir(const ns::Gas<T> &g, const ns::St<T> &s, const ns::St<T> &ds, const ns::St<T> gds[3],
+                                                         const T st[3][3], const T dst[3][3], int j) {
+  return -s.Y[1] * dst[0][j] - ds.Y[1] * st[0][j] - s.Y[2] * dst[1][j] - ds.Y[2] * st[1][j] - s.Y[3] * dst[2][j] - ds.Y[3] * st[2][j] -
+         g.k * gds[j].Y[4];
+}
+
+// FluxTotal_RiemannBoundary (newtonian_state.h:498-509): normal Riemann flux plus the viscous flux through n
+template <typename T>
+__device__ __forceinline__ void flux_total_riemann_boundary(const T Fn[5], const T st[3][3], const T Fe[3], const T n[3], T Flux[5]) {
+  Flux[0] = Fn[0];
+#pragma unroll
+  for (int k = 0; k < 3; k++) Flux[k + 1] = Fn[k + 1];
+  Flux[4] = Fn[4];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) Flux[k + 1] -= st[k][j] * n[j];
+    Flux[4] += Fe[j] * n[j];
+  }
+}
+
+// Softplus and the factor its forward mode returns (bc_outflow.h:18-27; the reference's Softplus_fwd returns the
+// derivative dsoftplus/dx without multiplying by dx, and the outflow Jacobian uses that factor as is)
+template <typename T>
+__device__ __forceinline__ T softplus(T x, T width) {
+  if (x > 40 * width) return x;
+  return width * (T)log1p(exp(x / width));
+}
+template <typename T>
+__device__ __forceinline__ T softplus_fwd(T x, T /*dx*/, T width) {
+  if (x > 40 * width) return T(1);
+  const T t = (T)exp(x / width);
+  return t / (1 + t);
+}
+
+template <int Q, typename T>
+__device__ __forceinline__ void store_q_strain(const VolumeArgs &a, int f, int t, bool ok, const double q[5], const T sr[6]) {
+  if (!ok) return;
+  if (a.jac32) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) a.jac32[fidx<Q>(a.jd, f, c, t)] = (float)q[c];
+#pragma unroll
+    for (int c = 0; c < 6; c++) a.jac32[fidx<Q>(a.jd, f, 5 + c, t)] = (float)sr[c];
+  } else if (a.jac) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) a.jac[fidx<Q>(a.jd, f, c, t)] = q[c];
+#pragma unroll
+    for (int c = 0; c < 6; c++) a.jac[fidx<Q>(a.jd, f, 5 + c, t)] = (double)sr[c];
+  }
+}
+
+template <int Q, typename T>
+__device__ __forceinline__ void load_q_strain(const VolumeArgs &a, int f, int t, bool ok, T qi[5], T sr[6], int var) {
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) qi[c] = a.jac32 ? (T)a.jac32[fidx<Q>(a.jd, f, c, t)] : (T)a.jacr[fidx<Q>(a.jd, f, c, t)];
+#pragma unroll
+    for (int c = 0; c < 6; c++) sr[c] = a.jac32 ? (T)a.jac32[fidx<Q>(a.jd, f, 5 + c, t)] : (T)a.jacr[fidx<Q>(a.jd, f, 5 + c, t)];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; c++) qi[c] = (var == ns::VAR_ENTROPY) ? (c == 4 ? T(-1) : T(0)) : T(1);
+#pragma unroll
+    for (int c = 0; c < 6; c++) sr[c] = 0;
+  }
+}
+
+// Exterior state of the Riemann outflow (bc_outflow.h:52-61): the interior velocity (plus the softplus-regularized
+// normal correction) at the prescribed pressure and temperature
+template <typename T>
+__device__ __forceinline__ void outflow_exterior_prim(const CeedMiOutflowContext &o, const ns::St<T> &s_int, const T n[3], T ye[5]) {
+#pragma unroll
+  for (int c = 0; c < 5; c++) ye[c] = s_int.Y[c];
+  ye[0]        = (T)o.pressure;
+  ye[4]        = (T)o.temperature;
+  const T un   = ye[1] * n[0] + ye[2] * n[1] + ye[3] * n[2];
+  const T proj = (T)(1 - o.recirc) * softplus(-un, (T)o.softplus_velocity);
+#pragma unroll
+  for (int j = 0; j < 3; j++) ye[1 + j] += n[j] * proj;
+}
+
+// RiemannOutflow (bc_outflow.h:37-89): v = wdetJb (HLLC(s_int, s_ext) + viscous flux) . n; jac_data_sur = {q, strain
+// rate} when implicit, v negated when explicit
+template <int VAR>
+struct RiemannOutflow {
+  static constexpr bool kGrad     = true;
+  static constexpr int  kJacComps = 11;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiOutflowContext &octx, int f, int t, bool ok, double q[5], double gref[2][5],
+                            double v[5]) {
+    const CeedMiNewtonianContext &ctx = octx.newt_ctx;
+    const ns::Gas<double>         g   = ns::Gas<double>::from(ctx.gas);
+    double                        wdetJb, dXdx[2][3], n[3];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    if (!ok) {
+      padding_q<VAR>(q);
+#pragma unroll
+      for (int c = 0; c < 5; c++) gref[0][c] = gref[1][c] = 0;
+    }
+    const ns::St<double> s_int = ns::state_from_q<VAR>(g, q);
+    double               ye[5];
+    outflow_exterior_prim(octx, s_int, n, ye);
+    const ns::St<double> s_ext = ns::state_from_q<ns::VAR_PRIM>(g, ye);
+    ns::St<double>       gs[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) gs[k] = phys_gradient_boundary_dir<VAR>(g, s_int, gref, dXdx, k);
+    double sr[6], st[3][3], Fe[3], Fn[5], Flux[5];
+    strain_rate_from_states(gs, sr);
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int j = 0; j < 3; j++) Fe[j] = viscous_energy_flux_dir(g, s_int, gs, st, j);
+    ns::riemann_hllc(g, s_int, s_ext, n, Fn);
+    flux_total_riemann_boundary(Fn, st, Fe, n, Flux);
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = wdetJb * Flux[c];
+    if (ctx.is_implicit) store_q_strain<Q>(a, f, t, ok, q, sr);
+    else
+#pragma unroll
+      for (int c = 0; c < 5; c++) v[c] *= -1;
+  }
+};
+
+// RiemannOutflow_Jacobian (bc_outflow.h:106-164), in T: exterior pressure and temperature held fixed
+template <int VAR, typename T>
+struct RiemannOutflowJacobian {
+  static constexpr bool kGrad     = true;
+  static constexpr int  kJacComps = 11;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiOutflowContext &octx, int f, int t, bool ok, double dqd[5], double grefd[2][5],
+                            double v[5]) {
+    const ns::Gas<T> g = ns::Gas<T>::from(octx.newt_ctx.gas);
+    T                wdetJb, dXdx[2][3], n[3], qi[5], sr[6], dq[5], gref[2][5];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    load_q_strain<Q>(a, f, t, ok, qi, sr, VAR);
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      dq[c]      = ok ? (T)dqd[c] : T(0);
+      gref[0][c] = ok ? (T)grefd[0][c] : T(0);
+      gref[1][c] = ok ? (T)grefd[1][c] : T(0);
+    }
+    const ns::St<T> s_int  = ns::state_from_q<VAR>(g, qi);
+    const ns::St<T> ds_int = ns::state_from_q_fwd<VAR>(g, s_int, dq);
+    T               ye[5], dye[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      ye[c]  = s_int.Y[c];
+      dye[c] = ds_int.Y[c];
+    }
+    ye[0] = (T)octx.pressure, ye[4] = (T)octx.temperature;
+    dye[0] = 0, dye[4] = 0;
+    const T un    = s_int.Y[1] * n[0] + s_int.Y[2] * n[1] + s_int.Y[3] * n[2];
+    const T dun   = ds_int.Y[1] * n[0] + ds_int.Y[2] * n[1] + ds_int.Y[3] * n[2];
+    const T proj  = (T)(1 - octx.recirc) * softplus(-un, (T)octx.softplus_velocity);
+    const T dproj = (T)(1 - octx.recirc) * softplus_fwd(-un, -dun, (T)octx.softplus_velocity);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      ye[1 + j] += n[j] * proj;
+      dye[1 + j] += n[j] * dproj;
+    }
+    const ns::St<T> s_ext  = ns::state_from_q<ns::VAR_PRIM>(g, ye);
+    const ns::St<T> ds_ext = ns::state_from_q_fwd<ns::VAR_PRIM>(g, s_ext, dye);
+    ns::St<T>       gds[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) gds[k] = phys_gradient_boundary_dir<VAR>(g, s_int, gref, dXdx, k);
+    T dsr[6], dst[3][3], st[3][3], dFe[3], dFn[5], dFlux[5];
+    strain_rate_from_states(gds, dsr);
+    ns::stress_from_strain(g, dsr, dst);
+    ns::stress_from_strain(g, sr, st);
+#pragma unroll
+    for (int j = 0; j < 3; j++) dFe[j] = viscous_energy_flux_fwd_dir(g, s_int, ds_int, gds, st, dst, j);
+    ns::riemann_hllc_fwd(g, s_int, ds_int, s_ext, ds_ext, n, dFn);
+    flux_total_riemann_boundary(dFn, dst, dFe, n, dFlux);
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = (double)(wdetJb * dFlux[c]);
+  }
+};
+
+// PressureOutflow (bc_outflow.h:185-229): the interior state at the prescribed pressure, v = -wdetJb F_total . n with
+// wdetJb negated when implicit; jac_data_sur = {q, strain rate} when implicit
+template <int VAR>
+struct PressureOutflow {
+  static constexpr bool kGrad     = true;
+  static constexpr int  kJacComps = 11;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiOutflowContext &octx, int f, int t, bool ok, double q[5], double gref[2][5],
+                            double v[5]) {
+    const CeedMiNewtonianContext &ctx = octx.newt_ctx;
+    const ns::Gas<double>         g   = ns::Gas<double>::from(ctx.gas);
+    double                        wdetJb, dXdx[2][3], n[3];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    if (!ok) {
+      padding_q<VAR>(q);
+#pragma unroll
+      for (int c = 0; c < 5; c++) gref[0][c] = gref[1][c] = 0;
+    }
+    ns::St<double> s = ns::state_from_q<VAR>(g, q);
+    s.Y[0]           = octx.pressure;
+    wdetJb *= ctx.is_implicit ? -1. : 1.;
+    ns::St<double> gs[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) gs[k] = phys_gradient_boundary_dir<VAR>(g, s, gref, dXdx, k);
+    double sr[6], st[3][3];
+    strain_rate_from_states(gs, sr);
+    ns::stress_from_strain(g, sr, st);
+    double Flux[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double F[5];
+      ns::flux_inviscid_dir(s, j, F);
+      add_total_flux_dir(F, st, viscous_energy_flux_dir(g, s, gs, st, j), n[j], j, Flux);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = -wdetJb * Flux[c];
+    if (ctx.is_implicit) store_q_strain<Q>(a, f, t, ok, q, sr);
+  }
+};
+
+// PressureOutflow_Jacobian (bc_outflow.h:246-293), in T: dq's pressure part dropped after the state tangent
+template <int VAR, typename T>
+struct PressureOutflowJacobian {
+  static constexpr bool kGrad     = true;
+  static constexpr int  kJacComps = 11;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiOutflowContext &octx, int f, int t, bool ok, double dqd[5], double grefd[2][5],
+                            double v[5]) {
+    const ns::Gas<T> g = ns::Gas<T>::from(octx.newt_ctx.gas);
+    T                wdetJb, dXdx[2][3], n[3], qi[5], sr[6], dq[5], gref[2][5];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    wdetJb *= octx.newt_ctx.is_implicit ? T(-1) : T(1);
+    load_q_strain<Q>(a, f, t, ok, qi, sr, VAR);
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      dq[c]      = ok ? (T)dqd[c] : T(0);
+      gref[0][c] = ok ? (T)grefd[0][c] : T(0);
+      gref[1][c] = ok ? (T)grefd[1][c] : T(0);
+    }
+    ns::St<T> s  = ns::state_from_q<VAR>(g, qi);
+    ns::St<T> ds = ns::state_from_q_fwd<VAR>(g, s, dq);
+    s.Y[0]       = (T)octx.pressure;
+    ds.Y[0]      = 0;
+    ns::St<T> gds[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) gds[k] = phys_gradient_boundary_dir<VAR>(g, s, gref, dXdx, k);
+    T dsr[6], dst[3][3], st[3][3];
+    strain_rate_from_states(gds, dsr);
+    ns::stress_from_strain(g, dsr, dst);
+    ns::stress_from_strain(g, sr, st);
+    T dFlux[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T dF[5];
+      ns::flux_inviscid_fwd_dir(s, ds, j, dF);
+      add_total_flux_dir(dF, dst, viscous_energy_flux_fwd_dir(g, s, ds, gds, st, dst, j), n[j], j, dFlux);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = (double)(-wdetJb * dFlux[c]);
+  }
+};
+
+// Slip (bc_slip.h:11-46): HLLC flux against the mirror state (normal velocity reflected); jac_data_sur = q when
+// implicit, v negated when explicit
+template <int VAR>
+struct Slip {
+  static constexpr bool kGrad     = false;
+  static constexpr int  kJacComps = 5;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int f, int t, bool ok, double q[5], double (*)[5], double v[5]) {
+    const ns::Gas<double> g = ns::Gas<double>::from(ctx.gas);
+    double                wdetJb, dXdx[2][3], n[3];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    if (!ok) padding_q<VAR>(q);
+    const ns::St<double> s  = ns::state_from_q<VAR>(g, q);
+    const double         vn = s.Y[1] * n[0] + s.Y[2] * n[1] + s.Y[3] * n[2];
+    double               yr[5];
+    yr[0] = s.Y[0];
+#pragma unroll
+    for (int j = 0; j < 3; j++) yr[1 + j] = s.Y[1 + j] - 2. * n[j] * vn;
+    yr[4]                    = s.Y[4];
+    const ns::St<double> s_r = ns::state_from_q<ns::VAR_PRIM>(g, yr);
+    double               F[5];
+    ns::riemann_hllc(g, s, s_r, n, F);
+#pragma unroll
+    for (int c = 0; c < 5; c++) v[c] = wdetJb * F[c];
+    if (ctx.is_implicit) {
+      if (ok) {
+        if (a.jac32) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) a.jac32[fidx<Q>(a.jd, f, c, t)] = (float)q[c];
+        } else if (a.jac) {
+#pragma unroll
+          for (int c = 0; c < 5; c++) a.jac[fidx<Q>(a.jd, f, c, t)] = q[c];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 5; c++) v[c] *= -1;
+    }
+  }
+};
+
+// Slip_Jacobian (bc_slip.h:60-99), in T
+template <int VAR, typename T>
+struct SlipJacobian {
+  static constexpr bool kGrad     = false;
+  static constexpr int  kJacComps = 5;
+  template <int Q>
+  __device__ static void qf(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, int f, int t, bool ok, double dqd[5], double (*)[5],
+                            double v[5]) {
+    const ns::Gas<T> g = ns::Gas<T>::from(ctx.gas);
+    T                wdetJb, dXdx[2][3], n[3], qi[5], dq[5];
+    unpack_face_qdata<Q>(a, f, t, ok, wdetJb, dXdx, n);
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) qi[c] = a.jac32 ? (T)a.jac32[fidx<Q>(a.jd, f, c, t)] : (T)a.jacr[fidx<Q>(a.jd, f, c, t)];
+    } else {
+      padding_q<VAR>(qi);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) dq[c] = ok ? (T)dqd[c] : T(0);
+    const ns::St<T> s   = ns::state_from_q<VAR>(g, qi);
+    const ns::St<T> ds  = ns::state_from_q_fwd<VAR>(g, s, dq);
+    const T         vn  = s.Y[1] * n[0] + s.Y[2] * n[1] + s.Y[3] * n[2];
+    const T         dvn = ds.Y[1] * n[0] + ds.Y[2] * n[1] + ds.Y[3] * n[2];
+    T               yr[5], dyr[5];
+    yr[0]  = s.Y[0];
+    dyr[0] = ds.Y[0];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      yr[1 + j]  = s.Y[1 + j] - T(2) * n[j] * vn;
+      dyr[1 + j] = ds.Y[1 + j] - T(2) * n[j] * dvn;
+    }
+    yr[4]                 = s.Y[4];
+    dyr[4]                = ds.Y[4];
+    const ns::St<T> s_r   = ns::state_from_q<ns::VAR_PRIM>(g, yr);
+    const ns::St<T> ds_r  = ns::state_from_q_fwd<ns::VAR_PRIM>(g, s_r, dyr);
+    T               dF[5];
+    ns::riemann_hllc_fwd(g, s, ds, s_r, ds_r, n, dF);
 #pragma unroll
     for (int c = 0; c < 5; c++) v[c] = (double)(wdetJb * dF[c]);
   }

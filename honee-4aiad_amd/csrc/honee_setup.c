@@ -349,13 +349,13 @@ static void StateFromPrimitive(const CeedMiGas *gas, const double Y[5], CeedMiSt
 
 // A boundary definition (HoneeBCAddIFunctionOp / HoneeBCAddIJacobianOp, src/boundary_condition.c:290-490)
 static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y) {
-  if (p->nbnd >= HONEE_MI_MAX_BOUNDARIES || kind < 0 || kind > 2 || nfaces <= 0) return CEED_ERROR_DIMENSION;
+  if (p->nbnd >= HONEE_MI_MAX_BOUNDARIES || kind < 0 || kind > 5 || nfaces <= 0) return CEED_ERROR_DIMENSION;
   HoneeMiBoundary *b = &p->bnd[p->nbnd++];
   const HoneeMiMesh *m = &p->mesh;
   const int P = m->P, Q = m->Q, P2 = P * P, Q2 = Q * Q;
   b->kind   = kind;
   b->nfaces = nfaces;
-  b->njac   = kind == 0 ? 11 : 5;
+  b->njac   = (kind == 0 || kind == 3 || kind == 4) ? 11 : 5;  // boundary integral and outflows store {q, strain rate}
   CHK(p, CeedBasisCreateTensorH1(p->ceed, 2, 5, P, Q, m->B, m->D, m->qref, m->qweight, &b->basis_q));
   CHK(p, CeedBasisCreateTensorH1(p->ceed, 2, 3, 2, Q, m->Bx, m->Dx, m->qref, m->qweight, &b->basis_x));
   {
@@ -391,7 +391,18 @@ static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_n
     CHK(p, CeedElemRestrictionCreateVector(b->restr_jd, &b->jac_data, NULL));
   }
   CeedQFunctionContext qfctx = p->newt_qfctx;
-  if (kind > 0) {  // FreestreamContext_ (bc_freestream_type.h:9-12), filled as problems/bc_freestream.c:156-163 does
+  if (kind == 3 || kind == 4) {  // OutflowContext_ (bc_freestream_type.h:14-21), filled as problems/bc_outflow.c:154-160 does
+    CeedMiOutflowContext *oc = malloc(sizeof(*oc));
+    oc->newt_ctx             = *p->ctx;
+    oc->pressure             = S_inf_Y[0];
+    oc->temperature          = S_inf_Y[1];
+    oc->recirc               = S_inf_Y[2];
+    oc->softplus_velocity    = S_inf_Y[3];
+    CHK(p, CeedQFunctionContextCreate(p->ceed, &b->fs_qfctx));
+    CHK(p, CeedQFunctionContextSetData(b->fs_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, sizeof(*oc), oc));
+    CHK(p, CeedQFunctionContextSetDataDestroy(b->fs_qfctx, CEED_MEM_HOST, FreeContext));
+    qfctx = b->fs_qfctx;
+  } else if (kind == 1 || kind == 2) {  // FreestreamContext_ (bc_freestream_type.h:9-12), filled as problems/bc_freestream.c:156-163 does
     CeedMiFreestreamContext *fs = malloc(sizeof(*fs));
     fs->newt_ctx                = *p->ctx;
     StateFromPrimitive(&p->ctx->gas, S_inf_Y, &fs->S_infty);
@@ -404,6 +415,13 @@ static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_n
   if (kind == 0) {
     snprintf(loc_f, sizeof(loc_f), "qfunctions/newtonian.h:BoundaryIntegral_%s", kVarName[p->state_var]);
     snprintf(loc_j, sizeof(loc_j), "qfunctions/newtonian.h:BoundaryIntegral_Jacobian_%s", kVarName[p->state_var]);
+  } else if (kind == 3 || kind == 4) {  // problems/bc_outflow.c:26-112
+    const char *name = kind == 3 ? "RiemannOutflow" : "PressureOutflow";
+    snprintf(loc_f, sizeof(loc_f), "qfunctions/bc_outflow.h:%s_%s", name, kVarName[p->state_var]);
+    snprintf(loc_j, sizeof(loc_j), "qfunctions/bc_outflow.h:%s_Jacobian_%s", name, kVarName[p->state_var]);
+  } else if (kind == 5) {  // problems/bc_slip.c:16-57
+    snprintf(loc_f, sizeof(loc_f), "qfunctions/bc_slip.h:Slip_%s", kVarName[p->state_var]);
+    snprintf(loc_j, sizeof(loc_j), "qfunctions/bc_slip.h:Slip_Jacobian_%s", kVarName[p->state_var]);
   } else {
     const char *flux = kind == 1 ? "HLL" : "HLLC";
     snprintf(loc_f, sizeof(loc_f), "qfunctions/bc_freestream.h:Freestream_%s_%s", kVarName[p->state_var], flux);
@@ -458,7 +476,7 @@ static int AddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_n
 }
 
 int HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y) {
-  if (kind > 0 && !S_inf_Y) return CEED_ERROR_INCOMPLETE;
+  if (kind > 0 && kind != 5 && !S_inf_Y) return CEED_ERROR_INCOMPLETE;
   return AddBoundary(p, kind, nfaces, face_nodes, face_xnodes, S_inf_Y);
 }
 

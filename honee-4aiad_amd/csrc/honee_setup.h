@@ -36,9 +36,10 @@ int          HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shi
 // IJacobian and of the explicit mass at state q
 int          HoneeMiIJacobianPointBlockDiagonal(HoneeMiProblem p, int mem_type, double shift, double *pb);
 int          HoneeMiMassPointBlockDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *pb);
-// Boundary faces of the problem (HONEE's BC definitions, problems/newtonian.c:555-569 and problems/bc_freestream.c:30-180):
-// kind 0: consistent boundary integral; 1: freestream with HLL flux; 2: freestream with HLLC flux (S_inf_Y is the
-// exterior primitive state).  face_nodes [nfaces][P^2] are the face nodes in face-local (X1 fastest) order with
+// Boundary faces of the problem (HONEE's BC definitions, problems/newtonian.c:555-569, problems/bc_freestream.c:30-180,
+// problems/bc_outflow.c:114-180, problems/bc_slip.c:59-73): kind 0: consistent boundary integral; 1: freestream with
+// HLL flux; 2: freestream with HLLC flux (S_inf_Y is the exterior primitive state); 3: Riemann outflow; 4: weak-pressure
+// outflow (S_inf_Y holds {pressure, temperature, recirc, softplus_velocity}); 5: slip wall (S_inf_Y unused).  face_nodes [nfaces][P^2] are the face nodes in face-local (X1 fastest) order with
 // outward dx/dX1 x dx/dX2, face_xnodes [nfaces][4] the coordinate nodes likewise.  The face operators are added to the
 // IFunction and IJacobian composites.
 int          HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y);

@@ -119,6 +119,13 @@ hipError_t launch_face_freestream(int P, int Q, int var, bool hllc, bool jacobia
 // Diagonal (point_block false) or point-block diagonal [node][c_out][c_in] of a face Jacobian
 hipError_t launch_face_jacobian_diagonal(bool point_block, int P, int Q, int var, bool freestream, bool hllc, bool fp32, const VolumeArgs &a,
                                          const CeedMiNewtonianContext &nctx, const CeedMiFreestreamContext &fctx, hipStream_t s);
+// Outflow (Riemann or weak pressure) and slip faces (kernels_face_bc.hip); kind of the diagonal: one of kFace*
+constexpr int kFaceRiemannOutflow = 2, kFacePressureOutflow = 3, kFaceSlip = 4;
+hipError_t launch_face_outflow(int P, int Q, int var, bool pressure, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiOutflowContext &ctx,
+                               hipStream_t s);
+hipError_t launch_face_slip(int P, int Q, int var, bool jacobian, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s);
+hipError_t launch_face_bc_jacobian_diagonal(bool point_block, int P, int Q, int var, int kind, bool fp32, const VolumeArgs &a,
+                                            const CeedMiNewtonianContext &nctx, const CeedMiOutflowContext &octx, hipStream_t s);
 
 hipError_t launch_fill(double *x, double value, long long n, hipStream_t s);
 hipError_t launch_axpby(double *y, double alpha, double beta, const double *x, long long n, hipStream_t s);

@@ -319,10 +319,12 @@ class NewtonianProblem:
         self._check(self.lib.HoneeMiSgsTrainingFields(self.handle, _ptr(f), _ptr(g), _ptr(a)), "SgsTrainingFields")
         return f, g, a
 
-    BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC = 0, 1, 2
+    BOUNDARY_INTEGRAL, FREESTREAM_HLL, FREESTREAM_HLLC, RIEMANN_OUTFLOW, PRESSURE_OUTFLOW, SLIP = 0, 1, 2, 3, 4, 5
 
     def add_boundary(self, kind: int, face_elem, face_id, S_inf_Y=None) -> int:
-        """Add boundary faces (volume element, local face id) of the given kind to the IFunction/IJacobian composites."""
+        """Add boundary faces (volume element, local face id) of the given kind to the IFunction/IJacobian composites.
+        S_inf_Y: the exterior primitive state (freestream), or {pressure, temperature, recirc, softplus_velocity}
+        (outflows)."""
         nodes, xnodes = face_node_tables(self.mesh, face_elem, face_id)
         self._keep.setdefault("faces", []).append((nodes, xnodes))
         yinf = None if S_inf_Y is None else np.ascontiguousarray(S_inf_Y, dtype=np.float64)
@@ -331,7 +333,7 @@ class NewtonianProblem:
         self._check(self.lib.HoneeMiProblemAddBoundary(self.handle, int(kind), len(nodes), _ptr(nodes), _ptr(xnodes),
                                                        None if yinf is None else _ptr(yinf)), "AddBoundary")
         self.nbnd = getattr(self, "nbnd", 0) + 1
-        self._bnd_sizes = getattr(self, "_bnd_sizes", []) + [(len(nodes), 11 if kind == 0 else 5)]
+        self._bnd_sizes = getattr(self, "_bnd_sizes", []) + [(len(nodes), 11 if kind in (0, 3, 4) else 5)]
         return self.nbnd - 1
 
     def boundary_data(self, boundary: int, which: int):

@@ -50,6 +50,15 @@ typedef struct {
   CeedMiState            S_infty;
 } CeedMiFreestreamContext;
 
+// struct OutflowContext_ (qfunctions/bc_freestream_type.h:14-21)
+typedef struct {
+  CeedMiNewtonianContext newt_ctx;
+  double                 recirc;             // amount of recirculation allowed in the exterior state [0, 1]
+  double                 softplus_velocity;  // width of the softplus velocity regularization
+  double                 pressure;
+  double                 temperature;
+} CeedMiOutflowContext;
+
 // struct TaylorGreenContext_ (qfunctions/taylorgreen.h:13-20)
 typedef struct {
   double                 reference[5];  // StatePrimitive: pressure, velocity[3], temperature

@@ -48,6 +48,14 @@ typedef struct {
   int    riemann;  // 0 HLL, 1 HLLC
 } OFreestream;
 
+// Outflow context (bc_freestream_type.h:14-21) and the outflow variant: 0 Riemann (bc_outflow.h:37-164), 1 weak
+// pressure (bc_outflow.h:185-293)
+typedef struct {
+  ONewt  newt;
+  double recirc, softplus_velocity, pressure, temperature;
+  int    pressure_outflow;
+} OOutflow;
+
 // Boundary faces of a hex mesh: each face is described by its volume element and local face id
 // (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z).
 typedef struct {
@@ -136,6 +144,12 @@ int oracle_setup_qdata_boundary(const OMesh *m, const OFaces *f, double *qdata_s
 int oracle_freestream(const OMesh *m, const OFaces *f, const OFreestream *c, const double *qdata_sur, const double *q, double *v, double *jac_sur);
 int oracle_freestream_jacobian(const OMesh *m, const OFaces *f, const OFreestream *c, const double *qdata_sur, const double *jac_sur,
                                const double *dq, double *v);
+int oracle_outflow(const OMesh *m, const OFaces *f, const OOutflow *c, const double *qdata_sur, const double *q, double *v, double *jac_sur);
+int oracle_outflow_jacobian(const OMesh *m, const OFaces *f, const OOutflow *c, const double *qdata_sur, const double *jac_sur, const double *dq,
+                            double *v);
+int oracle_slip(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *q, double *v, double *jac_sur);
+int oracle_slip_jacobian(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *jac_sur, const double *dq,
+                         double *v);
 int oracle_boundary_integral(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *q, double *v,
                              double *jac_sur);
 int oracle_boundary_integral_jacobian(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *jac_sur,

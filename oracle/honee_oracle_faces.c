@@ -1,8 +1,9 @@
 // CPU ORACLE -- test infrastructure only (see ns_pointwise.h).
 //
 // Boundary-face operators of the HONEE Newtonian problem, restated with libCEED /cpu/self semantics on the faces of a
-// hexahedral mesh: the consistent boundary integral (qfunctions/newtonian.h:328-443) and the freestream Riemann
-// boundary (qfunctions/bc_freestream.h:14-138), with their Jacobians and the boundary quadrature data
+// hexahedral mesh: the consistent boundary integral (qfunctions/newtonian.h:328-443), the freestream Riemann
+// boundary (qfunctions/bc_freestream.h:14-138), the Riemann and weak-pressure outflows (qfunctions/bc_outflow.h:37-305)
+// and the slip wall (qfunctions/bc_slip.h:11-111), with their Jacobians and the boundary quadrature data
 // (qfunctions/setupgeo.h:100-117).
 //
 // A face is given by its volume element and local face id (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z).  Face-local node (i, j),
@@ -12,6 +13,7 @@
 //   -y: b = 0,   X1 = x, X2 = z      +y: b = P-1, X1 = z, X2 = x
 //   -z: c = 0,   X1 = y, X2 = x      +z: c = P-1, X1 = x, X2 = y
 // Face data layouts: qdata_sur [nfaces][10][Q^2] = {wdetJb, normal[3], dXdx[2][3]}; jac_sur [nfaces][ncomp][Q^2].
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -299,6 +301,228 @@ int oracle_freestream_jacobian(const OMesh *m, const OFaces *f, const OFreestrea
       const OState ds = o_state_from_q_fwd(c->newt.gas, &s, dqi, c->newt.state_var);
       if (c->riemann == 0) o_riemann_hll_fwd(c->newt.gas, &s, &ds, &S_inf, &dS_inf, normal, dF);
       else o_riemann_hllc_fwd(c->newt.gas, &s, &ds, &S_inf, &dS_inf, normal, dF);
+      for (int j = 0; j < 5; j++) vq[j * Q2 + p] = wdetJb * dF[j];
+    }
+    memset(r_f, 0, sizeof(double) * 5 * P2);
+    for (int cc = 0; cc < 5; cc++) tensor2_t_add(m->P, m->Q, m->B, m->B, &vq[cc * Q2], &r_f[cc * P2]);
+    face_scatter5(m, f, k, r_f, v);
+  }
+  return 0;
+}
+
+// Softplus (bc_outflow.h:18-21) and the value its forward mode returns (bc_outflow.h:23-27): the derivative factor,
+// not multiplied by dx
+static double softplus(double x, double width) { return x > 40 * width ? x : width * log1p(exp(x / width)); }
+static double softplus_fwd(double x, double dx, double width) {
+  (void)dx;
+  if (x > 40 * width) return 1;
+  const double t = exp(x / width);
+  return t / (1 + t);
+}
+
+// FluxTotal_RiemannBoundary (newtonian_state.h:498-509)
+static void flux_total_riemann_boundary(const double Fn[5], const double st[3][3], const double Fe[3], const double n[3], double Flux[5]) {
+  for (int j = 0; j < 5; j++) Flux[j] = Fn[j];
+  for (int j = 0; j < 3; j++) {
+    for (int k = 0; k < 3; k++) Flux[k + 1] -= st[k][j] * n[j];
+    Flux[4] += Fe[j] * n[j];
+  }
+}
+
+// RiemannOutflow (bc_outflow.h:37-89) and PressureOutflow (bc_outflow.h:185-229)
+int oracle_outflow(const OMesh *m, const OFaces *f, const OOutflow *c, const double *qdata_sur, const double *q, double *v, double *jac_sur) {
+  const int  P2 = m->P * m->P, Q2 = m->Q * m->Q;
+  const OGas g   = c->newt.gas;
+  const int  var = c->newt.state_var;
+  for (int k = 0; k < f->nfaces; k++) {
+    double u_f[5 * 64], uq[5 * 64], gq[10 * 64], vq[5 * 64], r_f[5 * 64];
+    face_gather5(m, f, k, q, u_f);
+    face_interp_grad(m, u_f, uq, gq);
+    const double *qd = &qdata_sur[(long)k * 10 * Q2];
+    for (int p = 0; p < Q2; p++) {
+      double qi[5], wdetJb, dXdx[2][3], normal[3];
+      for (int j = 0; j < 5; j++) qi[j] = uq[j * Q2 + p];
+      unpack_qdata_sur(qd, Q2, p, &wdetJb, dXdx, normal);
+      OState s = o_state_from_q(g, qi, var);
+      OState gs[3];
+      double sr[6], km[6], st[3][3], Fe[3], Flux[5];
+      if (c->pressure_outflow) {
+        s.Y[0] = c->pressure;
+        wdetJb *= c->newt.is_implicit ? -1. : 1.;
+        phys_gradient_boundary(g, &s, var, gq, Q2, p, dXdx, gs);
+        o_km_strain_rate(gs, sr);
+        o_newtonian_stress(g, sr, km);
+        o_km_unpack(km, st);
+        o_viscous_energy_flux(g, s.Y, gs, st, Fe);
+        double Finv[3][5];
+        o_flux_inviscid(&s, Finv);
+        o_flux_total_boundary(Finv, st, Fe, normal, Flux);
+        for (int j = 0; j < 5; j++) vq[j * Q2 + p] = -wdetJb * Flux[j];
+      } else {
+        double ye[5];
+        memcpy(ye, s.Y, sizeof(ye));
+        ye[0]              = c->pressure;
+        ye[4]              = c->temperature;
+        const double un    = o_dot3(&ye[1], normal);
+        const double proj  = (1 - c->recirc) * softplus(-un, c->softplus_velocity);
+        for (int j = 0; j < 3; j++) ye[1 + j] += normal[j] * proj;
+        const OState s_ext = o_state_from_q(g, ye, OVAR_PRIM);
+        phys_gradient_boundary(g, &s, var, gq, Q2, p, dXdx, gs);
+        o_km_strain_rate(gs, sr);
+        o_newtonian_stress(g, sr, km);
+        o_km_unpack(km, st);
+        o_viscous_energy_flux(g, s.Y, gs, st, Fe);
+        double Fn[5];
+        o_riemann_hllc(g, &s, &s_ext, normal, Fn);
+        flux_total_riemann_boundary(Fn, st, Fe, normal, Flux);
+        for (int j = 0; j < 5; j++) vq[j * Q2 + p] = wdetJb * Flux[j];
+        if (!c->newt.is_implicit)
+          for (int j = 0; j < 5; j++) vq[j * Q2 + p] *= -1;
+      }
+      if (c->newt.is_implicit) {
+        double *jd = &jac_sur[(long)k * 11 * Q2];
+        for (int j = 0; j < 5; j++) jd[j * Q2 + p] = qi[j];
+        for (int j = 0; j < 6; j++) jd[(5 + j) * Q2 + p] = sr[j];
+      }
+    }
+    memset(r_f, 0, sizeof(double) * 5 * P2);
+    for (int cc = 0; cc < 5; cc++) tensor2_t_add(m->P, m->Q, m->B, m->B, &vq[cc * Q2], &r_f[cc * P2]);
+    face_scatter5(m, f, k, r_f, v);
+  }
+  return 0;
+}
+
+// RiemannOutflow_Jacobian (bc_outflow.h:106-164) and PressureOutflow_Jacobian (bc_outflow.h:246-293)
+int oracle_outflow_jacobian(const OMesh *m, const OFaces *f, const OOutflow *c, const double *qdata_sur, const double *jac_sur, const double *dq,
+                            double *v) {
+  const int  P2 = m->P * m->P, Q2 = m->Q * m->Q;
+  const OGas g   = c->newt.gas;
+  const int  var = c->newt.state_var;
+  for (int k = 0; k < f->nfaces; k++) {
+    double u_f[5 * 64], uq[5 * 64], gq[10 * 64], vq[5 * 64], r_f[5 * 64];
+    face_gather5(m, f, k, dq, u_f);
+    face_interp_grad(m, u_f, uq, gq);
+    const double *qd = &qdata_sur[(long)k * 10 * Q2];
+    const double *jd = &jac_sur[(long)k * 11 * Q2];
+    for (int p = 0; p < Q2; p++) {
+      double wdetJb, dXdx[2][3], normal[3], qi[5], sr[6], dqi[5];
+      unpack_qdata_sur(qd, Q2, p, &wdetJb, dXdx, normal);
+      for (int j = 0; j < 5; j++) qi[j] = jd[j * Q2 + p];
+      for (int j = 0; j < 6; j++) sr[j] = jd[(5 + j) * Q2 + p];
+      for (int j = 0; j < 5; j++) dqi[j] = uq[j * Q2 + p];
+      double km[6], st[3][3], dsr[6], dkm[6], dst[3][3], dFe[3], dFlux[5];
+      o_newtonian_stress(g, sr, km);
+      OState s  = o_state_from_q(g, qi, var);
+      OState ds = o_state_from_q_fwd(g, &s, dqi, var);
+      OState gds[3];
+      if (c->pressure_outflow) {
+        wdetJb *= c->newt.is_implicit ? -1. : 1.;
+        s.Y[0]  = c->pressure;
+        ds.Y[0] = 0.;
+        phys_gradient_boundary(g, &s, var, gq, Q2, p, dXdx, gds);
+        o_km_strain_rate(gds, dsr);
+        o_newtonian_stress(g, dsr, dkm);
+        o_km_unpack(dkm, dst);
+        o_km_unpack(km, st);
+        o_viscous_energy_flux_fwd(g, s.Y, ds.Y, gds, st, dst, dFe);
+        double dFinv[3][5];
+        o_flux_inviscid_fwd(&s, &ds, dFinv);
+        o_flux_total_boundary(dFinv, dst, dFe, normal, dFlux);
+        for (int j = 0; j < 5; j++) vq[j * Q2 + p] = -wdetJb * dFlux[j];
+      } else {
+        double ye[5], dye[5];
+        memcpy(ye, s.Y, sizeof(ye));
+        memcpy(dye, ds.Y, sizeof(dye));
+        ye[0] = c->pressure, ye[4] = c->temperature;
+        dye[0] = 0, dye[4] = 0;
+        const double un    = o_dot3(&s.Y[1], normal);
+        const double dun   = o_dot3(&ds.Y[1], normal);
+        const double proj  = (1 - c->recirc) * softplus(-un, c->softplus_velocity);
+        const double dproj = (1 - c->recirc) * softplus_fwd(-un, -dun, c->softplus_velocity);
+        for (int j = 0; j < 3; j++) {
+          ye[1 + j] += normal[j] * proj;
+          dye[1 + j] += normal[j] * dproj;
+        }
+        const OState s_ext  = o_state_from_q(g, ye, OVAR_PRIM);
+        const OState ds_ext = o_state_from_y_fwd(g, &s_ext, dye);
+        phys_gradient_boundary(g, &s, var, gq, Q2, p, dXdx, gds);
+        o_km_strain_rate(gds, dsr);
+        o_newtonian_stress(g, dsr, dkm);
+        o_km_unpack(dkm, dst);
+        o_km_unpack(km, st);
+        o_viscous_energy_flux_fwd(g, s.Y, ds.Y, gds, st, dst, dFe);
+        double dFn[5];
+        o_riemann_hllc_fwd(g, &s, &ds, &s_ext, &ds_ext, normal, dFn);
+        flux_total_riemann_boundary(dFn, dst, dFe, normal, dFlux);
+        for (int j = 0; j < 5; j++) vq[j * Q2 + p] = wdetJb * dFlux[j];
+      }
+    }
+    memset(r_f, 0, sizeof(double) * 5 * P2);
+    for (int cc = 0; cc < 5; cc++) tensor2_t_add(m->P, m->Q, m->B, m->B, &vq[cc * Q2], &r_f[cc * P2]);
+    face_scatter5(m, f, k, r_f, v);
+  }
+  return 0;
+}
+
+// Mirror state of the slip wall (bc_slip.h:29-33): the normal velocity component reflected
+static void slip_mirror(const double Y[5], const double n[3], double Yr[5]) {
+  const double vn = o_dot3(&Y[1], n);
+  Yr[0]           = Y[0];
+  for (int j = 0; j < 3; j++) Yr[1 + j] = Y[1 + j] - 2. * n[j] * vn;
+  Yr[4] = Y[4];
+}
+
+// Slip (bc_slip.h:11-46)
+int oracle_slip(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *q, double *v, double *jac_sur) {
+  const int P2 = m->P * m->P, Q2 = m->Q * m->Q;
+  for (int k = 0; k < f->nfaces; k++) {
+    double u_f[5 * 64], uq[5 * 64], vq[5 * 64], r_f[5 * 64];
+    face_gather5(m, f, k, q, u_f);
+    face_interp_grad(m, u_f, uq, NULL);
+    const double *qd = &qdata_sur[(long)k * 10 * Q2];
+    for (int p = 0; p < Q2; p++) {
+      double qi[5], wdetJb, dXdx[2][3], normal[3], Yr[5], F[5];
+      for (int j = 0; j < 5; j++) qi[j] = uq[j * Q2 + p];
+      unpack_qdata_sur(qd, Q2, p, &wdetJb, dXdx, normal);
+      const OState s = o_state_from_q(c->gas, qi, c->state_var);
+      slip_mirror(s.Y, normal, Yr);
+      const OState s_r = o_state_from_q(c->gas, Yr, OVAR_PRIM);
+      o_riemann_hllc(c->gas, &s, &s_r, normal, F);
+      for (int j = 0; j < 5; j++) vq[j * Q2 + p] = wdetJb * F[j];
+      if (c->is_implicit) {
+        for (int j = 0; j < 5; j++) jac_sur[((long)k * 5 + j) * Q2 + p] = qi[j];
+      } else {
+        for (int j = 0; j < 5; j++) vq[j * Q2 + p] *= -1;
+      }
+    }
+    memset(r_f, 0, sizeof(double) * 5 * P2);
+    for (int cc = 0; cc < 5; cc++) tensor2_t_add(m->P, m->Q, m->B, m->B, &vq[cc * Q2], &r_f[cc * P2]);
+    face_scatter5(m, f, k, r_f, v);
+  }
+  return 0;
+}
+
+// Slip_Jacobian (bc_slip.h:60-99)
+int oracle_slip_jacobian(const OMesh *m, const OFaces *f, const ONewt *c, const double *qdata_sur, const double *jac_sur, const double *dq,
+                         double *v) {
+  const int P2 = m->P * m->P, Q2 = m->Q * m->Q;
+  for (int k = 0; k < f->nfaces; k++) {
+    double u_f[5 * 64], uq[5 * 64], vq[5 * 64], r_f[5 * 64];
+    face_gather5(m, f, k, dq, u_f);
+    face_interp_grad(m, u_f, uq, NULL);
+    const double *qd = &qdata_sur[(long)k * 10 * Q2];
+    for (int p = 0; p < Q2; p++) {
+      double qi[5], dqi[5], wdetJb, dXdx[2][3], normal[3], Yr[5], dYr[5], dF[5];
+      unpack_qdata_sur(qd, Q2, p, &wdetJb, dXdx, normal);
+      for (int j = 0; j < 5; j++) qi[j] = jac_sur[((long)k * 5 + j) * Q2 + p];
+      for (int j = 0; j < 5; j++) dqi[j] = uq[j * Q2 + p];
+      const OState s  = o_state_from_q(c->gas, qi, c->state_var);
+      const OState ds = o_state_from_q_fwd(c->gas, &s, dqi, c->state_var);
+      slip_mirror(s.Y, normal, Yr);
+      slip_mirror(ds.Y, normal, dYr);
+      const OState s_r  = o_state_from_q(c->gas, Yr, OVAR_PRIM);
+      const OState ds_r = o_state_from_y_fwd(c->gas, &s_r, dYr);
+      o_riemann_hllc_fwd(c->gas, &s, &ds, &s_r, &ds_r, normal, dF);
       for (int j = 0; j < 5; j++) vq[j * Q2 + p] = wdetJb * dF[j];
     }
     memset(r_f, 0, sizeof(double) * 5 * P2);

@@ -267,6 +267,11 @@ class OFreestream(ct.Structure):
     _fields_ = [("newt", ONewt), ("S_inf_U", ct.c_double * 5), ("S_inf_Y", ct.c_double * 5), ("riemann", ct.c_int)]
 
 
+class OOutflow(ct.Structure):
+    _fields_ = [("newt", ONewt), ("recirc", ct.c_double), ("softplus_velocity", ct.c_double), ("pressure", ct.c_double),
+                ("temperature", ct.c_double), ("pressure_outflow", ct.c_int)]
+
+
 class OFaces(ct.Structure):
     _fields_ = [("nfaces", ct.c_int), ("face_elem", _ip), ("face_id", _ip)]
 
@@ -320,6 +325,37 @@ def freestream(m: Mesh, f: Faces, fc: OFreestream, qdata_sur, q):
 def freestream_jacobian(m: Mesh, f: Faces, fc: OFreestream, qdata_sur, jac_sur, dq):
     v = np.zeros(5 * m.nnodes)
     lib().oracle_freestream_jacobian(ct.byref(m.s), ct.byref(f.s), ct.byref(fc), _d(qdata_sur), _d(jac_sur), _d(dq), _d(v))
+    return v
+
+
+def outflow_ctx(c: ONewt, pressure, temperature, recirc=1.0, softplus_velocity=1e-2, pressure_outflow=False) -> OOutflow:
+    """OutflowContext_ as problems/bc_outflow.c:154-160 fills it (defaults recirc 1, softplus width 1e-2)."""
+    return OOutflow(c, float(recirc), float(softplus_velocity), float(pressure), float(temperature), int(bool(pressure_outflow)))
+
+
+def outflow(m: Mesh, f: Faces, oc: OOutflow, qdata_sur, q):
+    v = np.zeros(5 * m.nnodes)
+    jac = np.zeros((f.nfaces, 11, m.tab.Q**2))
+    lib().oracle_outflow(ct.byref(m.s), ct.byref(f.s), ct.byref(oc), _d(qdata_sur), _d(q), _d(v), _d(jac))
+    return v, jac
+
+
+def outflow_jacobian(m: Mesh, f: Faces, oc: OOutflow, qdata_sur, jac_sur, dq):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_outflow_jacobian(ct.byref(m.s), ct.byref(f.s), ct.byref(oc), _d(qdata_sur), _d(jac_sur), _d(dq), _d(v))
+    return v
+
+
+def slip(m: Mesh, f: Faces, c: ONewt, qdata_sur, q):
+    v = np.zeros(5 * m.nnodes)
+    jac = np.zeros((f.nfaces, 5, m.tab.Q**2))
+    lib().oracle_slip(ct.byref(m.s), ct.byref(f.s), ct.byref(c), _d(qdata_sur), _d(q), _d(v), _d(jac))
+    return v, jac
+
+
+def slip_jacobian(m: Mesh, f: Faces, c: ONewt, qdata_sur, jac_sur, dq):
+    v = np.zeros(5 * m.nnodes)
+    lib().oracle_slip_jacobian(ct.byref(m.s), ct.byref(f.s), ct.byref(c), _d(qdata_sur), _d(jac_sur), _d(dq), _d(v))
     return v
 
 

@@ -1,4 +1,5 @@
-"""Boundary-face operators on the HIP path (consistent boundary integral, freestream HLL/HLLC) against the CPU oracle.
+"""Boundary-face operators on the HIP path (consistent boundary integral, freestream HLL/HLLC, Riemann and weak-pressure
+outflow, slip wall) against the CPU oracle.
 
 The IFunction and IJacobian composites (volume + faces) are applied through the C-ABI driver and compared with the
 oracle's volume operator plus its face operators on the same seeded states.  Tolerances as for the volume parity
@@ -18,6 +19,16 @@ from pyhonee.driver import NewtonianProblem
 pytestmark = pytest.mark.gpu
 
 Y_INF = np.array([71.4, 0.9, 0.05, -0.02, 0.25])
+# outflow {pressure, temperature, recirc, softplus_velocity}: recirc below 1 exercises the softplus regularization
+OUTFLOW = np.array([71.0, 0.25, 0.5, 1e-2])
+# recirc 1 for assembled diagonals: below 1 the reference's outflow Jacobian body is affine, not linear, in dq (its
+# Softplus_fwd returns the derivative factor without the dx it is passed, bc_outflow.h:23-27), so diagonals assembled
+# from per-point unit inputs and the operator's global unit columns legitimately differ
+OUTFLOW_LINEAR = np.array([71.0, 0.25, 1.0, 1e-2])
+
+
+def _bc_params(kind, linear=False):
+    return {0: None, 1: Y_INF, 2: Y_INF, 3: OUTFLOW_LINEAR if linear else OUTFLOW, 4: OUTFLOW, 5: None}[kind]
 
 
 def _say(msg):
@@ -30,10 +41,17 @@ def _faces_setup(case, sides):
     return fe, fi, O.Faces(fe, fi)
 
 
-def _oracle_boundary(om, oc, f, kind, q):
+def _oracle_boundary(om, oc, f, kind, q, linear=False):
     qds = O.setup_qdata_boundary(om, f)
     if kind == 0:
         v, jac = O.boundary_integral(om, f, oc, qds, q)
+        return v, jac, qds, None
+    if kind in (3, 4):
+        c = O.outflow_ctx(oc, *_bc_params(kind, linear), pressure_outflow=kind == 4)
+        v, jac = O.outflow(om, f, c, qds, q)
+        return v, jac, qds, c
+    if kind == 5:
+        v, jac = O.slip(om, f, oc, qds, q)
         return v, jac, qds, None
     U_inf, _ = O.state_from_q(oc.gas, Y_INF, 1)
     fc = O.freestream_ctx(oc, U_inf, Y_INF, kind == 2)
@@ -44,6 +62,11 @@ def _oracle_boundary(om, oc, f, kind, q):
 def _oracle_boundary_jac(om, oc, f, kind, qds, jac, fc, dq):
     if kind == 0:
         return O.boundary_integral_jacobian(om, f, oc, qds, jac, dq)
+    if kind in (3, 4):
+        fc.newt.ijacobian_time_shift = oc.ijacobian_time_shift
+        return O.outflow_jacobian(om, f, fc, qds, jac, dq)
+    if kind == 5:
+        return O.slip_jacobian(om, f, oc, qds, jac, dq)
     return O.freestream_jacobian(om, f, fc, qds, jac, dq)
 
 
@@ -55,10 +78,18 @@ def _oracle_boundary_jac(om, oc, f, kind, qds, jac, fc, dq):
     (2, 1, "/gpu/hip/mi355x"),
     (2, 0, "/gpu/hip/mi355x:scatter=atomic"),
     (1, 2, "/gpu/hip/mi355x"),
+    (3, 1, "/gpu/hip/mi355x"),
+    (3, 0, "/gpu/hip/mi355x:scatter=atomic"),
+    (3, 2, "/gpu/hip/mi355x"),
+    (4, 1, "/gpu/hip/mi355x"),
+    (4, 0, "/gpu/hip/mi355x"),
+    (5, 1, "/gpu/hip/mi355x"),
+    (5, 2, "/gpu/hip/mi355x:scatter=atomic"),
+    (5, 0, "/gpu/hip/mi355x"),
 ])
 def test_face_operators_match_oracle(kind, state_var, resource):
     _say(f"faces kind={kind} var={state_var} {resource}")
-    sides = [0, 1, 4] if kind == 0 else [0, 1]
+    sides = [0, 1, 4] if kind in (0, 3, 4, 5) else [0, 1]
     case = make_case(shape=(3, 2, 2), degree=3, periodic=(False, True, False), perturb=0.12, state_var=state_var, stabilization=2,
                      seed=100 + 10 * kind + state_var)
     om, oc, qd = oracle_setup(case)
@@ -68,7 +99,7 @@ def test_face_operators_match_oracle(kind, state_var, resource):
     v_bnd, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q)
 
     prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), state_var, resource)
-    b = prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    b = prob.add_boundary(kind, fe, fi, _bc_params(kind))
     assert rel_err(prob.boundary_data(b, 0), qds)[0] < 1e-13
     g = np.zeros_like(q)
     prob.ifunction(q, qdot, g, time=0.0)
@@ -87,7 +118,7 @@ def test_face_operators_match_oracle(kind, state_var, resource):
     prob.close()
 
 
-@pytest.mark.parametrize("kind", [0, 2])
+@pytest.mark.parametrize("kind", [0, 2, 3, 4, 5])
 def test_face_operators_fp32_jacobian(kind):
     _say(f"faces fp32 jacobian kind={kind}")
     case = make_case(shape=(2, 2, 2), degree=3, periodic=(False, True, True), perturb=0.1, state_var=1, stabilization=2, seed=130 + kind)
@@ -101,7 +132,7 @@ def test_face_operators_fp32_jacobian(kind):
     dq = case["rng"].uniform(-1e-3, 1e-3, q.shape)
     vj_ref = O.ijacobian(om, oc, qd, jac_vol, dq) + _oracle_boundary_jac(om, oc, f, kind, qds, jac_bnd, fc, dq)
     prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), 1, "/gpu/hip/mi355x:jacobian_precision=fp32")
-    prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    prob.add_boundary(kind, fe, fi, _bc_params(kind))
     g = np.zeros_like(q)
     prob.ifunction(q, qdot, g)
     vj = np.zeros_like(dq)
@@ -127,7 +158,7 @@ def test_closed_box_uniform_state_balances():
     prob.close()
 
 
-@pytest.mark.parametrize("kind,degree", [(0, 1), (1, 2), (2, 1)])
+@pytest.mark.parametrize("kind,degree", [(0, 1), (1, 2), (2, 1), (3, 2), (4, 1), (5, 2)])
 def test_jacobian_diagonal_with_faces(kind, degree):
     """Point-Jacobi diagonal of the IJacobian composite (volume + faces) against the oracle's unit columns."""
     _say(f"diagonal with faces kind={kind} degree={degree}")
@@ -136,7 +167,7 @@ def test_jacobian_diagonal_with_faces(kind, degree):
     fe, fi, f = _faces_setup(case, [0, 1])
     q, qdot = case["q"], case["qdot"]
     _, jac_vol = O.ifunction(om, oc, qd, q, qdot)
-    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q)
+    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q, linear=True)
     shift = 120.0
     oc.ijacobian_time_shift = shift
     d_ref = O.ijacobian_diagonal(om, oc, qd, jac_vol)
@@ -146,7 +177,7 @@ def test_jacobian_diagonal_with_faces(kind, degree):
         e[i] = 1.0
         d_ref[i] += _oracle_boundary_jac(om, oc, f, kind, qds, jac_bnd, fc, e)[i]
     prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), 1)
-    prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    prob.add_boundary(kind, fe, fi, _bc_params(kind, linear=True))
     g = np.zeros_like(q)
     prob.ifunction(q, qdot, g)
     d = np.zeros_like(q)
@@ -156,7 +187,8 @@ def test_jacobian_diagonal_with_faces(kind, degree):
     prob.close()
 
 
-@pytest.mark.parametrize("kind,state_var,fp32", [(0, 1, False), (2, 2, False), (1, 0, True)])
+@pytest.mark.parametrize("kind,state_var,fp32", [(0, 1, False), (2, 2, False), (1, 0, True), (3, 1, False), (4, 2, False), (5, 0, False),
+                                                 (3, 1, True)])
 def test_point_block_diagonal_with_faces(kind, state_var, fp32):
     """Point-block diagonal [node][c_out][c_in] of the IJacobian composite (volume + faces) against the oracle's
     volume point blocks plus the face operator's unit columns; FP32 Jacobian mode at 1e-5."""
@@ -167,7 +199,7 @@ def test_point_block_diagonal_with_faces(kind, state_var, fp32):
     fe, fi, f = _faces_setup(case, [0, 1])
     q, qdot = case["q"], case["qdot"]
     _, jac_vol = O.ifunction(om, oc, qd, q, qdot)
-    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q)
+    _, jac_bnd, qds, fc = _oracle_boundary(om, oc, f, kind, q, linear=True)
     shift = 80.0
     oc.ijacobian_time_shift = shift
     pb_ref = O.ijacobian_point_block_diagonal(om, oc, qd, jac_vol)
@@ -179,7 +211,7 @@ def test_point_block_diagonal_with_faces(kind, state_var, fp32):
             pb_ref[node, :, ci] += _oracle_boundary_jac(om, oc, f, kind, qds, jac_bnd, fc, e)[node * 5: node * 5 + 5]
     res = "/gpu/hip/mi355x:jacobian_precision=fp32" if fp32 else "/gpu/hip/mi355x"
     prob = NewtonianProblem(case["mesh"], case["tab"], product_context(case), state_var, res)
-    prob.add_boundary(kind, fe, fi, Y_INF if kind else None)
+    prob.add_boundary(kind, fe, fi, _bc_params(kind, linear=True))
     g = np.zeros_like(q)
     prob.ifunction(q, qdot, g)
     pb = np.zeros((nn, 5, 5))
