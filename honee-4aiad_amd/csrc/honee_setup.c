@@ -52,6 +52,10 @@ struct HoneeMiProblem_ {
   CeedOperator          op_rhs, op_mass, op_monitor;  // explicit problems: RHS composite and mass; KE monitor
   CeedElemRestriction   elem_restr_ke;
   CeedVector            ke_values, pb_ceed;
+  // essential (strong) boundary conditions: mask (0 at constrained entries), its complement, boundary values, the
+  // point-block mask and identity, and work copies of the operator inputs
+  int                   ess_enabled;
+  CeedVector            ess_mask, ess_cmask, ess_qbc, ess_pb_mask, ess_pb_eye, ess_q, ess_qdot;
   CeedContextFieldLabel rhs_time_label, rhs_dt_label;
   int                   mass_stabilized;
   double                volume;
@@ -1075,30 +1079,122 @@ int HoneeMiApplySgsResidual(HoneeMiProblem p, int mem_type, const double *q, con
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// Essential (strong) boundary conditions.  HONEE constrains them through the DMPlex section: the solver's vectors
+// exclude the constrained entries, every operator input is a local vector into which DMPlexInsertBoundaryValues
+// inserts the boundary values (SetBCsFromICs and DMPlexInsertBoundaryValues_FromICs, src/misc.c:51-95: zero the
+// constrained entries with the boundary mask, then add Qbc), the time derivative's constrained entries stay zero, and
+// local-to-global drops the constrained residual rows.  Over the full L-vector the same system is: inputs
+// q .* mask + Qbc and q_dot .* mask, residual rows masked, and the Jacobian (and the mass matrix of explicit problems)
+// the identity on the constrained entries, with diagonals and point blocks to match.
+// ---------------------------------------------------------------------------------------------------------------------
+static int HostToVector(HoneeMiProblem p, CeedSize n, const double *h, CeedVector *v) {
+  if (!*v) CHK(p, CeedVectorCreate(p->ceed, n, v));
+  CHK(p, CeedVectorSetArray(*v, CEED_MEM_HOST, CEED_COPY_VALUES, (CeedScalar *)h));
+  return 0;
+}
+
+int HoneeMiProblemSetEssential(HoneeMiProblem p, int n, const int *dofs, const double *values) {
+  const CeedSize N = 5 * (CeedSize)p->nnodes;
+  if (n < 0 || (n > 0 && !dofs)) return CEED_ERROR_INCOMPLETE;
+  for (int i = 0; i < n; i++)
+    if (dofs[i] < 0 || dofs[i] >= N) return CEED_ERROR_DIMENSION;
+  double *mask = malloc(sizeof(double) * (size_t)N), *qbc = calloc((size_t)N, sizeof(double));
+  double *pbm = malloc(sizeof(double) * 25 * (size_t)p->nnodes), *pbe = calloc(25 * (size_t)p->nnodes, sizeof(double));
+  if (!mask || !qbc || !pbm || !pbe) {
+    free(mask);
+    free(qbc);
+    free(pbm);
+    free(pbe);
+    return CEED_ERROR_MAJOR;
+  }
+  for (CeedSize i = 0; i < N; i++) mask[i] = 1.0;
+  for (int i = 0; i < n; i++) {
+    mask[dofs[i]] = 0.0;
+    qbc[dofs[i]]  = values ? values[i] : 0.0;
+  }
+  for (CeedSize node = 0; node < p->nnodes; node++)
+    for (int r = 0; r < 5; r++)
+      for (int c = 0; c < 5; c++) {
+        pbm[node * 25 + r * 5 + c] = mask[node * 5 + r] * mask[node * 5 + c];
+        pbe[node * 25 + r * 5 + c] = (r == c) ? 1.0 - mask[node * 5 + r] : 0.0;
+      }
+  int ierr = HostToVector(p, N, mask, &p->ess_mask);
+  for (CeedSize i = 0; i < N; i++) mask[i] = 1.0 - mask[i];
+  if (!ierr) ierr = HostToVector(p, N, mask, &p->ess_cmask);
+  if (!ierr) ierr = HostToVector(p, N, qbc, &p->ess_qbc);
+  if (!ierr) ierr = HostToVector(p, 25 * (CeedSize)p->nnodes, pbm, &p->ess_pb_mask);
+  if (!ierr) ierr = HostToVector(p, 25 * (CeedSize)p->nnodes, pbe, &p->ess_pb_eye);
+  if (!ierr && !p->ess_q) ierr = CeedVectorCreate(p->ceed, N, &p->ess_q);
+  if (!ierr && !p->ess_qdot) ierr = CeedVectorCreate(p->ceed, N, &p->ess_qdot);
+  free(mask);
+  free(qbc);
+  free(pbm);
+  free(pbe);
+  if (ierr) return ierr;
+  p->ess_enabled = n > 0;
+  return 0;
+}
+
+// The local vector of an operator input: work = x .* mask (+ Qbc when insert); *dev is its device array
+static int EssentialLocal(HoneeMiProblem p, int mem_type, const double *x, CeedVector work, int insert, const double **dev) {
+  CHK(p, CeedVectorSetArray(work, (CeedMemType)mem_type, CEED_COPY_VALUES, (CeedScalar *)x));
+  CHK(p, CeedVectorPointwiseMult(work, work, p->ess_mask));
+  if (insert) CHK(p, CeedVectorAXPY(work, 1.0, p->ess_qbc));
+  const double *tmp;
+  CHK(p, CeedVectorGetArrayRead(work, CEED_MEM_DEVICE, &tmp));
+  *dev = tmp;
+  CHK(p, CeedVectorRestoreArrayRead(work, &tmp));
+  return 0;
+}
+
+// y = y .* mask + cmask .* x (identity on the constrained entries); x given as the caller's array
+static int EssentialIdentityRows(HoneeMiProblem p, int mem_type, CeedVector y, const double *x) {
+  CHK(p, CeedVectorPointwiseMult(y, y, p->ess_mask));
+  CHK(p, CeedVectorSetArray(p->ess_qdot, (CeedMemType)mem_type, CEED_COPY_VALUES, (CeedScalar *)x));
+  CHK(p, CeedVectorPointwiseMult(p->ess_qdot, p->ess_qdot, p->ess_cmask));
+  CHK(p, CeedVectorAXPY(y, 1.0, p->ess_qdot));
+  return 0;
+}
+
 // IFunction_NS's operator section (src/setupts.c:90-111): update context, hand over arrays, apply, take arrays back.
 int HoneeMiApplyIFunction(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, const double *q_dot, double *g) {
   CeedScalar *arr;
+  int         mt_in = mem_type;
   if (p->solution_time_label) CHK(p, CeedOperatorSetContextDouble(p->op_ifunction, p->solution_time_label, &time));
   if (p->timestep_size_label) CHK(p, CeedOperatorSetContextDouble(p->op_ifunction, p->timestep_size_label, &dt));
-  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
-  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q_dot));
+  if (p->ess_enabled) {  // UpdateBoundaryValues and the local time derivative (src/setupts.c:86-101)
+    CHK(p, EssentialLocal(p, mem_type, q, p->ess_q, 1, &q));
+    CHK(p, EssentialLocal(p, mem_type, q_dot, p->ess_qdot, 0, &q_dot));
+    mt_in = CEED_MEM_DEVICE;
+  }
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_in, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mt_in, CEED_USE_POINTER, (CeedScalar *)q_dot));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, g));
   CHK(p, CeedOperatorApply(p->op_ifunction, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
   if (p->sgs_enabled) CHK(p, SgsDDApplyIFunction(p));  // src/setupts.c:118-120
-  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
-  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mem_type, &arr));
+  if (p->ess_enabled) CHK(p, CeedVectorPointwiseMult(p->g_ceed, p->g_ceed, p->ess_mask));  // constrained rows dropped
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_in, &arr));
+  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mt_in, &arr));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
 
 // MatMult_Ceed's operator section (src/mat-ceed.c:1643-1694) with FormIJacobian_NS's shift (src/setupts.c:141)
 int HoneeMiApplyIJacobian(HoneeMiProblem p, int mem_type, double shift, const double *dq, double *v) {
-  CeedScalar *arr;
+  CeedScalar   *arr;
+  const double *dq_in = dq;
+  int           mt_in = mem_type;
   if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
-  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)dq));
+  if (p->ess_enabled) {
+    CHK(p, EssentialLocal(p, mem_type, dq, p->ess_q, 0, &dq_in));
+    mt_in = CEED_MEM_DEVICE;
+  }
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_in, CEED_USE_POINTER, (CeedScalar *)dq_in));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, v));
   CHK(p, CeedOperatorApply(p->op_ijacobian, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
-  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  if (p->ess_enabled) CHK(p, EssentialIdentityRows(p, mem_type, p->g_ceed, dq));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_in, &arr));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1109,6 +1205,10 @@ int HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, doubl
   if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, diag));
   CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_ijacobian, p->g_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->ess_enabled) {  // unit diagonal on the constrained entries
+    CHK(p, CeedVectorPointwiseMult(p->g_ceed, p->g_ceed, p->ess_mask));
+    CHK(p, CeedVectorAXPY(p->g_ceed, 1.0, p->ess_cmask));
+  }
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1116,13 +1216,19 @@ int HoneeMiIJacobianDiagonal(HoneeMiProblem p, int mem_type, double shift, doubl
 // RHS_NS's operator section (src/setupts.c:31-52): update time and dt, apply the RHS composite
 int HoneeMiApplyRHS(HoneeMiProblem p, int mem_type, double time, double dt, const double *q, double *r) {
   CeedScalar *arr;
+  int         mt_in = mem_type;
   if (!p->op_rhs) return CEED_ERROR_INCOMPATIBLE;
   if (p->rhs_time_label) CHK(p, CeedOperatorSetContextDouble(p->op_rhs, p->rhs_time_label, &time));
   if (p->rhs_dt_label) CHK(p, CeedOperatorSetContextDouble(p->op_rhs, p->rhs_dt_label, &dt));
-  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  if (p->ess_enabled) {  // UpdateBoundaryValues (src/setupts.c:40)
+    CHK(p, EssentialLocal(p, mem_type, q, p->ess_q, 1, &q));
+    mt_in = CEED_MEM_DEVICE;
+  }
+  CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_in, CEED_USE_POINTER, (CeedScalar *)q));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, r));
   CHK(p, CeedOperatorApply(p->op_rhs, p->q_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
-  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  if (p->ess_enabled) CHK(p, CeedVectorPointwiseMult(p->g_ceed, p->g_ceed, p->ess_mask));
+  CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_in, &arr));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1130,14 +1236,25 @@ int HoneeMiApplyRHS(HoneeMiProblem p, int mem_type, double time, double dt, cons
 // Mass matrix action y = M(q) x of the explicit mass KSP (MatMult of the "mass_" KSP's MatCeed, src/setuplibceed.c:56-95);
 // q is the state the SUPG-stabilized mass is evaluated at (ignored by Mass_5).
 int HoneeMiApplyMass(HoneeMiProblem p, int mem_type, const double *q, const double *x, double *y) {
-  CeedScalar *arr;
+  CeedScalar   *arr;
+  const double *x_in  = x;
+  int           mt_q  = mem_type, mt_x = mem_type;
   if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
-  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
-  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)x));
+  if (p->ess_enabled) {
+    if (p->mass_stabilized) {
+      CHK(p, EssentialLocal(p, mem_type, q, p->ess_q, 1, &q));
+      mt_q = CEED_MEM_DEVICE;
+    }
+    CHK(p, EssentialLocal(p, mem_type, x, p->ess_qdot, 0, &x_in));
+    mt_x = CEED_MEM_DEVICE;
+  }
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_q, CEED_USE_POINTER, (CeedScalar *)q));
+  CHK(p, CeedVectorSetArray(p->q_dot_ceed, (CeedMemType)mt_x, CEED_USE_POINTER, (CeedScalar *)x_in));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, y));
   CHK(p, CeedOperatorApply(p->op_mass, p->q_dot_ceed, p->g_ceed, CEED_REQUEST_IMMEDIATE));
-  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
-  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mem_type, &arr));
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_q, &arr));
+  CHK(p, CeedVectorTakeArray(p->q_dot_ceed, (CeedMemType)mt_x, &arr));
+  if (p->ess_enabled) CHK(p, EssentialIdentityRows(p, mem_type, p->g_ceed, x));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1145,11 +1262,20 @@ int HoneeMiApplyMass(HoneeMiProblem p, int mem_type, const double *q, const doub
 // Diagonal of the mass matrix (PCJACOBI "diagonal" of the mass KSP)
 int HoneeMiMassDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *diag) {
   CeedScalar *arr;
+  int         mt_q = mem_type;
   if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
-  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  if (p->ess_enabled && p->mass_stabilized) {
+    CHK(p, EssentialLocal(p, mem_type, q, p->ess_q, 1, &q));
+    mt_q = CEED_MEM_DEVICE;
+  }
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_q, CEED_USE_POINTER, (CeedScalar *)q));
   CHK(p, CeedVectorSetArray(p->g_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, diag));
   CHK(p, CeedOperatorLinearAssembleDiagonal(p->op_mass, p->g_ceed, CEED_REQUEST_IMMEDIATE));
-  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  if (p->ess_enabled) {
+    CHK(p, CeedVectorPointwiseMult(p->g_ceed, p->g_ceed, p->ess_mask));
+    CHK(p, CeedVectorAXPY(p->g_ceed, 1.0, p->ess_cmask));
+  }
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_q, &arr));
   CHK(p, CeedVectorTakeArray(p->g_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1184,6 +1310,10 @@ int HoneeMiIJacobianPointBlockDiagonal(HoneeMiProblem p, int mem_type, double sh
   if (p->ijacobian_shift_label) CHK(p, CeedOperatorSetContextDouble(p->op_ijacobian, p->ijacobian_shift_label, &shift));
   CHK(p, CeedVectorSetArray(p->pb_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, pb));
   CHK(p, CeedOperatorLinearAssemblePointBlockDiagonal(p->op_ijacobian, p->pb_ceed, CEED_REQUEST_IMMEDIATE));
+  if (p->ess_enabled) {  // constrained rows and columns of each block replaced by the identity
+    CHK(p, CeedVectorPointwiseMult(p->pb_ceed, p->pb_ceed, p->ess_pb_mask));
+    CHK(p, CeedVectorAXPY(p->pb_ceed, 1.0, p->ess_pb_eye));
+  }
   CHK(p, CeedVectorTakeArray(p->pb_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1191,12 +1321,21 @@ int HoneeMiIJacobianPointBlockDiagonal(HoneeMiProblem p, int mem_type, double sh
 // Point-block diagonal of the explicit mass matrix at state q
 int HoneeMiMassPointBlockDiagonal(HoneeMiProblem p, int mem_type, const double *q, double *pb) {
   CeedScalar *arr;
+  int         mt_q = mem_type;
   if (!p->op_mass) return CEED_ERROR_INCOMPATIBLE;
   if (!p->pb_ceed) CHK(p, CeedVectorCreate(p->ceed, 25 * (CeedSize)p->nnodes, &p->pb_ceed));
-  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, (CeedScalar *)q));
+  if (p->ess_enabled && p->mass_stabilized) {
+    CHK(p, EssentialLocal(p, mem_type, q, p->ess_q, 1, &q));
+    mt_q = CEED_MEM_DEVICE;
+  }
+  if (p->mass_stabilized) CHK(p, CeedVectorSetArray(p->q_ceed, (CeedMemType)mt_q, CEED_USE_POINTER, (CeedScalar *)q));
   CHK(p, CeedVectorSetArray(p->pb_ceed, (CeedMemType)mem_type, CEED_USE_POINTER, pb));
   CHK(p, CeedOperatorLinearAssemblePointBlockDiagonal(p->op_mass, p->pb_ceed, CEED_REQUEST_IMMEDIATE));
-  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mem_type, &arr));
+  if (p->ess_enabled) {
+    CHK(p, CeedVectorPointwiseMult(p->pb_ceed, p->pb_ceed, p->ess_pb_mask));
+    CHK(p, CeedVectorAXPY(p->pb_ceed, 1.0, p->ess_pb_eye));
+  }
+  if (p->mass_stabilized) CHK(p, CeedVectorTakeArray(p->q_ceed, (CeedMemType)mt_q, &arr));
   CHK(p, CeedVectorTakeArray(p->pb_ceed, (CeedMemType)mem_type, &arr));
   return 0;
 }
@@ -1238,6 +1377,13 @@ const char *HoneeMiProblemError(HoneeMiProblem p) { return p->err; }
 int HoneeMiProblemDestroy(HoneeMiProblem *problem) {
   HoneeMiProblem p = *problem;
   if (!p) return 0;
+  CeedVectorDestroy(&p->ess_mask);
+  CeedVectorDestroy(&p->ess_cmask);
+  CeedVectorDestroy(&p->ess_qbc);
+  CeedVectorDestroy(&p->ess_pb_mask);
+  CeedVectorDestroy(&p->ess_pb_eye);
+  CeedVectorDestroy(&p->ess_q);
+  CeedVectorDestroy(&p->ess_qdot);
   CeedOperatorDestroy(&p->op_ifunction);
   CeedOperatorDestroy(&p->op_ijacobian);
   CeedOperatorDestroy(&p->op_grad_velo_rhs);

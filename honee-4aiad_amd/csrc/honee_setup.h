@@ -43,6 +43,12 @@ int          HoneeMiMassPointBlockDiagonal(HoneeMiProblem p, int mem_type, const
 // outward dx/dX1 x dx/dX2, face_xnodes [nfaces][4] the coordinate nodes likewise.  The face operators are added to the
 // IFunction and IJacobian composites.
 int          HoneeMiProblemAddBoundary(HoneeMiProblem p, int kind, int nfaces, const int *face_nodes, const int *face_xnodes, const double *S_inf_Y);
+// Essential (strong) boundary conditions on n L-vector entries dofs[i] = 5 node + component, with the boundary values
+// values[i] (NULL: zero), as HONEE's DM_BC_ESSENTIAL boundaries with values from the initial condition or prescribed
+// constants (SetBCsFromICs, src/misc.c:51-95; essential_comps, src/bc_definition.c:249-280).  From then on every
+// operator input gets the boundary values inserted (and a zero time derivative) on those entries, their residual rows
+// are zero, and the Jacobian, mass matrix, diagonals and point blocks are the identity on them.  n = 0 removes them.
+int          HoneeMiProblemSetEssential(HoneeMiProblem p, int n, const int *dofs, const double *values);
 int          HoneeMiGetBoundaryData(HoneeMiProblem p, int boundary, int which, double *out);  // which 0: surface qdata, 1: jac data
 // Explicit problems (is_implicit false): RHS composite apply (RHS_NS, src/setupts.c:31-52), mass matrix action and
 // diagonal at state q (the "mass_" KSP operator, src/setuplibceed.c:56-95; SUPG-stabilized when stabilization is SUPG)

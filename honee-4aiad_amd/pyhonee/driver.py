@@ -116,6 +116,7 @@ def load_libs():
         lib.HoneeMiApplyIJacobian.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiIJacobianDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
         lib.HoneeMiProblemAddBoundary.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        lib.HoneeMiProblemSetEssential.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
         lib.HoneeMiGetBoundaryData.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
         lib.HoneeMiIJacobianPointBlockDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p]
         lib.HoneeMiMassPointBlockDiagonal.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
@@ -335,6 +336,14 @@ class NewtonianProblem:
         self.nbnd = getattr(self, "nbnd", 0) + 1
         self._bnd_sizes = getattr(self, "_bnd_sizes", []) + [(len(nodes), 11 if kind in (0, 3, 4) else 5)]
         return self.nbnd - 1
+
+    def set_essential(self, dofs, values=None):
+        """Essential (strong) boundary conditions on the L-vector entries dofs (5 node + component) with the given
+        values (default zero); see HoneeMiProblemSetEssential."""
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        v = None if values is None else np.ascontiguousarray(np.broadcast_to(values, d.shape), dtype=np.float64)
+        self._keep["essential"] = (d, v)
+        self._check(self.lib.HoneeMiProblemSetEssential(self.handle, len(d), _ptr(d), None if v is None else _ptr(v)), "SetEssential")
 
     def boundary_data(self, boundary: int, which: int):
         """Surface qdata (which=0, [nfaces][10][Q^2]) or surface Jacobian data (which=1) of a boundary."""

@@ -254,3 +254,17 @@ def box_boundary(mesh: BoxMesh, sides) -> tuple[np.ndarray, np.ndarray]:
         elems.append(e)
         ids.append(np.full(len(e), f, dtype=np.int32))
     return np.concatenate(elems).astype(np.int32), np.concatenate(ids).astype(np.int32)
+
+
+def boundary_nodes(mesh: BoxMesh, sides) -> np.ndarray:
+    """Sorted indices of the nodes on the given box sides (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)."""
+    fe, fi = box_boundary(mesh, sides)
+    nodes, _ = face_node_tables(mesh, fe, fi)
+    return np.unique(nodes.ravel()).astype(np.int32)
+
+
+def essential_dofs(mesh: BoxMesh, sides, comps) -> np.ndarray:
+    """L-vector entries 5 node + c of the components comps on the nodes of the given sides (HONEE's essential_comps
+    of a DM_BC_ESSENTIAL boundary, src/bc_definition.c:249-280)."""
+    nodes = boundary_nodes(mesh, sides)
+    return np.sort((5 * nodes[:, None] + np.asarray(comps, dtype=np.int64)[None, :]).ravel()).astype(np.int32)
