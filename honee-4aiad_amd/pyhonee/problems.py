@@ -138,3 +138,60 @@ def turbulence_prim(mesh: BoxMesh, modes, ref_Y=TGV_REFERENCE_Y) -> np.ndarray:
         Y[:, 1:4] = random_fourier_velocity(x, modes)
         return Y
     return nodal_field(mesh, f)
+
+
+# ---- channel (problems/channel.c:75-140, qfunctions/channel.h:24-53) ----
+def channel_context_dict(mu, umax=1.0, theta0=TGV_REFERENCE_Y[4], P0=TGV_REFERENCE_Y[0], H=1.0, body_force_scale=1.0, state_var=VAR_PRIM,
+                         stabilization=STAB_SUPG, dt=1e-2, tau=None):
+    """NS_CHANNEL's context on the Taylor-Green gas with viscosity mu: the body force B = 2 umax mu / H^2 enters as the
+    gravity vector g = (B / rho0, 0, 0) with rho0 = P0 / (R theta0) (problems/channel.c:126-137)."""
+    gas = dict(TGV_GAS)
+    gas["mu"] = float(mu)
+    rho0 = P0 / (_rgas(gas) * theta0)
+    B = body_force_scale * 2 * umax * gas["mu"] / (H * H)
+    d = {"gas": gas, "tau": dict(tau or TGV_TAU), "state_var": state_var, "stabilization": stabilization, "is_implicit": True, "dt": dt,
+         "g": (B / rho0, 0.0, 0.0)}
+    return d
+
+
+def channel_prim(xyz, gas, umax=1.0, theta0=TGV_REFERENCE_Y[4], P0=TGV_REFERENCE_Y[0], center=1.0, H=1.0):
+    """Exact_Channel (qfunctions/channel.h:24-53): Poiseuille velocity and the reference's temperature profile
+    theta0 (1 + (Pr Ec / 3)(1 - ((y - center) / H)^4)) with Pr = mu / (cp k) and Ec = umax^2 / (cp theta0), as the
+    reference writes them."""
+    y = np.asarray(xyz, dtype=np.float64)[..., 1]
+    eta2 = ((y - center) / H) ** 2
+    Pr = gas["mu"] / (gas["cp"] * gas["k"])
+    Ec = umax * umax / (gas["cp"] * theta0)
+    Y = np.zeros(np.shape(xyz)[:-1] + (5,))
+    Y[..., 0] = P0
+    Y[..., 1] = umax * (1 - eta2)
+    Y[..., 4] = theta0 * (1 + (Pr * Ec / 3) * (1 - eta2 * eta2))
+    return Y
+
+
+def channel_perturbation(xyz, seed, lengths, n_modes=20, amplitude=0.1, center=1.0, H=1.0):
+    """Seeded velocity perturbation of the channel: per component, n_modes random Fourier modes in each periodic
+    direction (x and z, integer wavenumbers 1..n_modes on the box lengths, random phases and unit amplitudes), damped by
+    (1 - ((y - center) / H)^2)^2 so it vanishes at the walls, scaled to a maximum of `amplitude` (the centerline
+    velocity's fraction)."""
+    rng = np.random.default_rng(seed)
+    x = np.asarray(xyz, dtype=np.float64)
+    damp = (1 - ((x[:, 1] - center) / H) ** 2) ** 2
+    u = np.zeros((len(x), 3))
+    for c in range(3):
+        for d, L in ((0, lengths[0]), (2, lengths[2])):
+            k = rng.integers(1, n_modes + 1, size=n_modes)
+            ph = rng.uniform(0, 2 * np.pi, size=n_modes)
+            u[:, c] += np.sin(np.outer(x[:, d], 2 * np.pi * k / L) + ph).sum(axis=1)
+    u *= damp[:, None]
+    return u * (amplitude / max(np.abs(u).max(), 1e-300))
+
+
+def channel_ic(mesh: BoxMesh, gas, seed=None, umax=1.0, amplitude=0.1, lengths=(4 * np.pi, 2.0, 4 * np.pi / 3)):
+    """Primitive channel state [nnodes][5]: Exact_Channel plus, with a seed, the perturbation at amplitude x umax."""
+    def f(x):
+        Y = channel_prim(x, gas, umax=umax)
+        if seed is not None:
+            Y[:, 1:4] += umax * channel_perturbation(x, seed, lengths, amplitude=amplitude)
+        return Y
+    return nodal_field(mesh, f)
