@@ -161,6 +161,60 @@ def explicit_c1(steps=100, cfl=0.3, cpu=True):
     return out
 
 
+def faces_c3(steps=20, warmup=3):
+    """Applies with boundary faces on the wall-bounded configuration: channel 4pi x 2 x 4pi/3 on 48 x 32 x 24 hexes,
+    wall-normal stretching 1.1, p=3 Q=4, primitive FP64, SUPG, Re_tau 180 (mu = 2 / 180^2 at umax = H = 1), Poiseuille
+    state with 10% seeded Fourier perturbations, consistent boundary-integral faces on both walls.  Reports the
+    IFunction and IJacobian composite applies (volume + faces) in GDoF/s and the per-kernel times; the volume-only
+    figures are the periodic workload's."""
+    import torch
+
+    from pyhonee import mesh as M
+    from pyhonee import problems as PR
+    from pyhonee.driver import CEED_MEM_DEVICE, NewtonianProblem, context_from_dict
+
+    lengths = (4 * np.pi, 2.0, 4 * np.pi / 3)
+    mesh = M.box_mesh((48, 32, 24), 3, lengths=lengths, periodic=(True, False, True), stretch=(1.0, 1.1, 1.0),
+                      symmetric_stretch=(False, True, False))
+    tab = M.tabulate(3, 0)
+    d = PR.channel_context_dict(2.0 / 180**2, tau=PR.default_tau(3), dt=2e-3)
+    Y = PR.channel_ic(mesh, d["gas"], seed=180, lengths=lengths)
+    dof = Y.size
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(180)
+    q = torch.tensor(Y.ravel(), dtype=torch.float64, device=dev)
+    qdot = torch.tensor(rng.uniform(-1e-3, 1e-3, Y.size), dtype=torch.float64, device=dev)
+    dq = torch.tensor(rng.uniform(-1e-3, 1e-3, Y.size), dtype=torch.float64, device=dev)
+    g, v = torch.zeros_like(q), torch.zeros_like(q)
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    fe, fi = M.box_boundary(mesh, [2, 3])
+    prob.add_boundary(0, fe, fi)
+    out = {"config": "channel 48x32x24 p=3 Q=4, stretching 1.1, primitive, SUPG, boundary-integral faces on both walls (3072 faces)",
+           "dof": dof}
+    for name, fn in (("ifunction", lambda: prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)),
+                     ("ijacobian", lambda: prob.ijacobian(dq, v, 500.0, mem=CEED_MEM_DEVICE))):
+        for _ in range(warmup):
+            fn()
+        prob.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        prob.synchronize()
+        t_apply = (time.perf_counter() - t0) / steps
+        prob.set_timing(True)
+        prob.reset_times()
+        for _ in range(steps):
+            fn()
+        prob.synchronize()
+        kt = prob.kernel_times()
+        prob.set_timing(False)
+        # ms per launch by kernel: the fused volume kernel, the face kernel and the node gathers of both scatters
+        per = {k: v_[1] / v_[0] for k, v_ in kt.items()}
+        out[name] = {"apply_ms": t_apply * 1e3, "gdofs_with_faces": dof / t_apply / 1e9, "kernel_ms": per}
+    prob.close()
+    return out
+
+
 def les_study():
     """The learned closure a posteriori: decaying isotropic turbulence (E(k) ~ k^4 exp(-2 (k/4)^2), u_rms 1, Mach 0.1),
     plain fine run 96^3 p=2 (SSP-RK3) differentially filtered to the coarse element size, network trained on the GPU
@@ -195,6 +249,7 @@ def main():
     ap.add_argument("--ts-steps", type=int, default=5, help="timed BDF2 steps of the implicit workload (0 skips time integration)")
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
     ap.add_argument("--no-les", action="store_true", help="skip the a-posteriori LES study of the learned closure")
+    ap.add_argument("--no-faces", action="store_true", help="skip the channel applies with boundary faces")
     ap.add_argument("--calibrate-traffic", action="store_true",
                     help="only run kernels of known HBM traffic (3 reductions reading 1 GiB, 3 fills writing 1 GiB) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc")
@@ -313,6 +368,9 @@ def main():
     if not args.no_explicit:
         print("# explicit 8^3 p=2 run", file=sys.stderr, flush=True)
         line["explicit_c1"] = explicit_c1(cpu=not args.no_cpu_baseline)
+    if not args.no_faces:
+        print("# channel applies with faces", file=sys.stderr, flush=True)
+        line["faces_c3"] = faces_c3()
     if not args.no_les:
         line["les"] = les_study()
     if not args.no_cpu_baseline:
