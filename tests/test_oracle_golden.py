@@ -106,3 +106,14 @@ def test_product_ic_matches_oracle_ic():
         qo = PR.nodal_field(mesh, lambda x: O.ic_taylorgreen(c, PR.TGV_REFERENCE_Y, [0.1, 0.2, 0.3], [L, L, L], x))
         qp = PR.taylor_green_ic(mesh, var, background=(0.1, 0.2, 0.3))
         np.testing.assert_allclose(qp, qo, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["taylor-green.bin", "taylor-green-IC.bin"])
+def test_writer_round_trips_reference_bin(name, tmp_path):
+    """HoneeWriteBinaryVec's layout (src/honee-file.c:196-217): writing back what the reader returns reproduces the
+    reference's file byte for byte (HONEE token, int32 step, float64 time, PETSc Vec record)."""
+    path = os.path.join(GOLDEN, name)
+    vals, step, t = honee_file.read_vec(path)
+    out = tmp_path / name
+    honee_file.write_vec(str(out), vals, step=step, time=t)
+    assert out.read_bytes() == open(path, "rb").read()
