@@ -25,11 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "honee-4aiad_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes of one IFunction apply at the default workload (fused kernel 595.0 MB + node gather 168.5 MB, two-pass
+# HBM bytes of one IFunction apply at the default workload (fused kernel 600.2 MB + node gather 168.5 MB, two-pass
 # scatter), from two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0, the factors the counters show for
 # the known-traffic kernels of --calibrate-traffic); the counts and commands are in
 # profiles/ifunction_newtonian_tgv32_p3_traffic.md.  Other workloads report traffic null.
-PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 763.4e6}
+PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 768.7e6}
 METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
 
 
