@@ -146,7 +146,7 @@ __global__ void k_dot(const double *x, const double *y, long long n, double *par
 
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
                               const double *ev, double *v, bool overwrite, hipStream_t s) {
-  constexpr int   NPT = 1;
+  constexpr int   NPT = 2;  // measured: 2 items per thread 31.8 us against 39.1 us (1) and 39.0 us (4) at 884,736 rows
   const long long n   = (long long)nrows * ncomp;
   if (n <= 0) return hipSuccess;
   const int grid = grid_for((n + NPT - 1) / NPT);
