@@ -219,6 +219,51 @@ struct Contract5 {
 // every pass treats NC components together, so each 1D coefficient row is read from LDS once per pass and the NC
 // independent contractions overlap their LDS latency.  Barriers are block-wide; a pass reads other threads' slots only
 // after a barrier that follows their writes.
+// Cross-lane access for the 4 x 4 x 4 element (P = Q = 4, one 64-lane wave per element, lane = qx + 4 qy + 16 qz):
+// the four lanes of an x line form a DPP quad and the sixteen lanes of an xy slab form a DPP row.  dpp_d<CTRL> returns
+// the double held by the lane that the DPP control CTRL selects (quad_perm broadcast 0x00, 0x55, 0xAA, 0xFF of quad
+// lane 0..3; row_ror 0x124, 0x128, 0x12C by 4, 8 and 12 lanes within the row), moving the two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b  = __double_as_longlong(v);
+  const int       lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int       hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// sum_a c[a] v(lane a of this quad), a ascending (the x line's contraction)
+__device__ __forceinline__ double quad_contract(const double c[4], double v) {
+  double acc = 0;
+  acc += c[0] * dpp_d<0x00>(v);
+  acc += c[1] * dpp_d<0x55>(v);
+  acc += c[2] * dpp_d<0xAA>(v);
+  acc += c[3] * dpp_d<0xFF>(v);
+  return acc;
+}
+
+// sum_n c[n] v(lane rotated by 4 n within this row): the y line's contraction with coefficients already permuted to the
+// rotation order (see RowRot)
+__device__ __forceinline__ double row_contract(const double c[4], double v) {
+  double acc = 0;
+  acc += c[0] * v;
+  acc += c[1] * dpp_d<0x124>(v);
+  acc += c[2] * dpp_d<0x128>(v);
+  acc += c[3] * dpp_d<0x12C>(v);
+  return acc;
+}
+
+// The y index of the lane each row rotation reads, found by rotating the lanes' own y index with the same DPP controls
+// (independent of the rotation's direction)
+struct RowRot {
+  int y[4];
+  __device__ explicit RowRot(int qy) {
+    y[0] = qy;
+    y[1] = __builtin_amdgcn_mov_dpp(qy, 0x124, 0xF, 0xF, true);
+    y[2] = __builtin_amdgcn_mov_dpp(qy, 0x128, 0xF, 0xF, true);
+    y[3] = __builtin_amdgcn_mov_dpp(qy, 0x12C, 0xF, 0xF, true);
+  }
+};
+
 template <int P, int Q>
 struct Stage {
   using D = Dims<P, Q>;
@@ -239,6 +284,30 @@ struct Stage {
   // Ends without a barrier: only this thread's own dst slots are written by the last pass.
   template <int NC>
   __device__ void interp(const double (&u)[NC], int s0, int s1, int dst) {
+    if constexpr (P == 4 && Q == 4) {  // x and y passes across lanes, z through LDS (slot s0)
+      const RowRot rr(qy);
+      double       bx[4], by[4], bz[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        bx[i] = B[qx * 4 + i];
+        by[i] = B[qy * 4 + rr.y[i]];
+        bz[i] = B[qz * 4 + i];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[s0 + c][t] = row_contract(by, quad_contract(bx, u[c]));
+      __syncthreads();
+      double r[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc += bz[i] * X[s0 + c][(i * 4 + qy) * 4 + qx];
+        r[c] = acc;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[dst + c][t] = r[c];
+      return;
+    }
     if (t < D::P3) {
 #pragma unroll
       for (int c = 0; c < NC; c++) X[s0 + c][t] = u[c];
@@ -294,6 +363,32 @@ struct Stage {
   template <int NC>
   __device__ void grad(int src, int gdst) {
     __syncthreads();
+    if constexpr (P == 4 && Q == 4) {  // x and y derivatives across lanes from this lane's value, z through LDS
+      const RowRot rr(qy);
+      double       dx[4], dy[4], dz[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        dx[m] = Dc[qx * 4 + m];
+        dy[m] = Dc[qy * 4 + rr.y[m]];
+        dz[m] = Dc[qz * 4 + m];
+      }
+      double g[3][NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const double v  = X[src + c][t];
+        double       gz = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) gz += dz[m] * X[src + c][(m * 4 + qy) * 4 + qx];
+        g[0][c] = quad_contract(dx, v);
+        g[1][c] = row_contract(dy, v);
+        g[2][c] = gz;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) X[gdst + NC * d + c][t] = g[d][c];
+      return;
+    }
     double dx[Q], dy[Q], dz[Q];
 #pragma unroll
     for (int m = 0; m < Q; m++) {
@@ -328,6 +423,50 @@ struct Stage {
   template <int NC, bool HAS_V, bool HAS_G>
   __device__ void transpose(int v0, int g0, double (&r)[NC]) {
     __syncthreads();
+    if constexpr (P == 4 && Q == 4) {  // z passes through LDS (slot v0 as scratch), x and y passes across lanes
+      const RowRot rr(qy);
+      double       w[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) w[c] = HAS_V ? X[v0 + c][t] : 0.0;
+      if (HAS_G) {
+        double dxT[4], dyT[4], dz[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          dxT[m] = Dc[m * 4 + qx];
+          dyT[m] = Dc[rr.y[m] * 4 + qy];
+          dz[m]  = Dc[m * 4 + qz];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          double acc = w[c];
+#pragma unroll
+          for (int m = 0; m < 4; m++) acc += dz[m] * X[g0 + 2 * NC + c][(m * 4 + qy) * 4 + qx];
+          acc += quad_contract(dxT, X[g0 + c][t]);
+          acc += row_contract(dyT, X[g0 + NC + c][t]);
+          w[c] = acc;
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[v0 + c][t] = w[c];
+      __syncthreads();
+      double bz[4], byT[4], bxT[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        bz[i]  = B[i * 4 + qz];
+        byT[i] = B[rr.y[i] * 4 + qy];
+        bxT[i] = B[i * 4 + qx];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0;
+#pragma unroll
+        for (int z = 0; z < 4; z++) acc += bz[z] * X[v0 + c][(z * 4 + qy) * 4 + qx];
+        r[c] = quad_contract(bxT, row_contract(byT, acc));
+      }
+      __syncthreads();
+      return;
+    }
     double w[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) w[c] = HAS_V ? X[v0 + c][t] : 0.0;
