@@ -252,6 +252,34 @@ __device__ __forceinline__ double row_contract(const double c[4], double v) {
   return acc;
 }
 
+// The four values of this lane's z line (the lanes with the same qx and qy), z ascending: a 32-lane swap gives every lane
+// the values of lanes L & 31 and (L & 31) + 32, and a 16-lane swap of each gives lanes with bit 4 cleared and set
+// (v_permlane32_swap, v_permlane16_swap; both operands the same register)
+__device__ __forceinline__ void z_line(double v, double z[4]) {
+  const long long b = __double_as_longlong(v);
+  unsigned        h[2] = {(unsigned)b, (unsigned)(b >> 32)};
+  unsigned        zz[2][4];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const auto a  = __builtin_amdgcn_permlane32_swap(h[k], h[k], false, false);
+    const auto lo = __builtin_amdgcn_permlane16_swap(a[0], a[0], false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(a[1], a[1], false, false);
+    zz[k][0] = lo[0], zz[k][1] = lo[1], zz[k][2] = hi[0], zz[k][3] = hi[1];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) z[i] = __longlong_as_double(((long long)zz[1][i] << 32) | (long long)zz[0][i]);
+}
+
+// sum_z c[z] v(lane of this z line at z), z ascending
+__device__ __forceinline__ double z_contract(const double c[4], double v) {
+  double z[4];
+  z_line(v, z);
+  double acc = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc += c[i] * z[i];
+  return acc;
+}
+
 // The y index of the lane each row rotation reads, found by rotating the lanes' own y index with the same DPP controls
 // (independent of the rotation's direction)
 struct RowRot {
@@ -264,7 +292,9 @@ struct RowRot {
   }
 };
 
-template <int P, int Q>
+// ZX: the z passes of the 4 x 4 x 4 element also across lanes (v_permlane swaps) instead of through LDS; measured
+// faster for the FP32 Jacobian and slower for the FP64 operators
+template <int P, int Q, bool ZX = false>
 struct Stage {
   using D = Dims<P, Q>;
   static constexpr int Q3 = D::Q3;
@@ -284,7 +314,19 @@ struct Stage {
   // Ends without a barrier: only this thread's own dst slots are written by the last pass.
   template <int NC>
   __device__ void interp(const double (&u)[NC], int s0, int s1, int dst) {
-    if constexpr (P == 4 && Q == 4) {  // x and y passes across lanes, z through LDS (slot s0)
+    if constexpr (P == 4 && Q == 4 && ZX) {  // all three passes across the lanes of the element's wave
+      const RowRot rr(qy);
+      double       bx[4], by[4], bz[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        bx[i] = B[qx * 4 + i];
+        by[i] = B[qy * 4 + rr.y[i]];
+        bz[i] = B[qz * 4 + i];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) X[dst + c][t] = z_contract(bz, row_contract(by, quad_contract(bx, u[c])));
+      return;
+    } else if constexpr (P == 4 && Q == 4) {  // x and y passes across lanes, z through LDS (slot s0)
       const RowRot rr(qy);
       double       bx[4], by[4], bz[4];
 #pragma unroll
@@ -363,7 +405,29 @@ struct Stage {
   template <int NC>
   __device__ void grad(int src, int gdst) {
     __syncthreads();
-    if constexpr (P == 4 && Q == 4) {  // x and y derivatives across lanes from this lane's value, z through LDS
+    if constexpr (P == 4 && Q == 4 && ZX) {  // derivatives across the lanes of the element's wave from this lane's value
+      const RowRot rr(qy);
+      double       dx[4], dy[4], dz[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        dx[m] = Dc[qx * 4 + m];
+        dy[m] = Dc[qy * 4 + rr.y[m]];
+        dz[m] = Dc[qz * 4 + m];
+      }
+      double g[3][NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const double v = X[src + c][t];
+        g[0][c]        = quad_contract(dx, v);
+        g[1][c]        = row_contract(dy, v);
+        g[2][c]        = z_contract(dz, v);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) X[gdst + NC * d + c][t] = g[d][c];
+      return;
+    } else if constexpr (P == 4 && Q == 4) {  // x and y derivatives across lanes from this lane's value, z through LDS
       const RowRot rr(qy);
       double       dx[4], dy[4], dz[4];
 #pragma unroll
@@ -423,7 +487,37 @@ struct Stage {
   template <int NC, bool HAS_V, bool HAS_G>
   __device__ void transpose(int v0, int g0, double (&r)[NC]) {
     __syncthreads();
-    if constexpr (P == 4 && Q == 4) {  // z passes through LDS (slot v0 as scratch), x and y passes across lanes
+    if constexpr (P == 4 && Q == 4 && ZX) {  // all passes across the lanes of the element's wave
+      const RowRot rr(qy);
+      double       bxT[4], byT[4], bzT[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        bxT[i] = B[i * 4 + qx];
+        byT[i] = B[rr.y[i] * 4 + qy];
+        bzT[i] = B[i * 4 + qz];
+      }
+      double dxT[4], dyT[4], dzT[4];
+      if (HAS_G) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          dxT[m] = Dc[m * 4 + qx];
+          dyT[m] = Dc[rr.y[m] * 4 + qy];
+          dzT[m] = Dc[m * 4 + qz];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double w = HAS_V ? X[v0 + c][t] : 0.0;
+        if (HAS_G) {
+          w += z_contract(dzT, X[g0 + 2 * NC + c][t]);
+          w += quad_contract(dxT, X[g0 + c][t]);
+          w += row_contract(dyT, X[g0 + NC + c][t]);
+        }
+        r[c] = quad_contract(bxT, row_contract(byT, z_contract(bzT, w)));
+      }
+      __syncthreads();
+      return;
+    } else if constexpr (P == 4 && Q == 4) {  // z passes through LDS (slot v0 as scratch), x and y passes across lanes
       const RowRot rr(qy);
       double       w[NC];
 #pragma unroll

@@ -12,6 +12,8 @@
 // (B.B, G.B, G.G) with sum factorization.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "basis3d.hpp"
 #include "kernels.hpp"
 #include "newtonian_ops.hpp"
@@ -44,6 +46,16 @@ __device__ __forceinline__ BatchRange batch_range(long long nbatch) {
   const long long lo = x * per, hi = (x + 1) * per < nbatch ? (x + 1) * per : nbatch;
   return {lo + j, hi, G / kNumXcd};
 }
+
+// Op::kZCrossLane selects the basis stage whose z passes also run across lanes (basis3::Stage's ZX); default off
+template <class Op, class = void>
+struct ZCross {
+  static constexpr bool value = false;
+};
+template <class Op>
+struct ZCross<Op, std::void_t<decltype(Op::kZCrossLane)>> {
+  static constexpr bool value = Op::kZCrossLane;
+};
 
 template <int P, int Q>
 __device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc) {
@@ -204,7 +216,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
     // quadrature-point body.
     int tb = t;
     asm volatile("" : "+v"(tb));
-    basis3::Stage<P, Q> st(sB, sDc, X, tb);
+    basis3::Stage<P, Q, ZCross<Op>::value> st(sB, sDc, X, tb);
     if constexpr (Op::kIn2) st.template interp<5>(u2n, SU2, SU2 + 5, U2);
     st.template interp<5>(un, SU, SU + 5, 0);
     if constexpr (Op::kGrad) st.template grad<5>(0, 5);
@@ -260,7 +272,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
     if constexpr (Op::kScatter) {
       int tt = t;
       asm volatile("" : "+v"(tt));
-      basis3::Stage<P, Q> sto(sB, sDc, X, tt);
+      basis3::Stage<P, Q, ZCross<Op>::value> sto(sB, sDc, X, tt);
       double              r[5];
       sto.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {
