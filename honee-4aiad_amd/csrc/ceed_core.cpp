@@ -226,8 +226,10 @@ static hipEvent_t event_get(Ceed ceed) {
     ceed->event_pool.pop_back();
     return e;
   }
+  // timing only (read after a stream synchronize): without the system-scope fence an event record does not write back
+  // and invalidate the caches, which would slow the kernel that follows it
   hipEvent_t e;
-  (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 
@@ -270,6 +272,30 @@ extern "C" int CeedMiGetStream(Ceed ceed, void **hip_stream) {
 
 extern "C" int CeedMiSetKernelTiming(Ceed ceed, bool enable) {
   ceed->timing = enable;
+  // events are created here, not between launches: an event created on the launch path delays the host, and the
+  // kernels then start on an idle GPU and run measurably slower than in steady state
+  constexpr size_t kTimingEvents = 1024;
+  while (enable && ceed->event_pool.size() < kTimingEvents) {
+    hipEvent_t e;
+    MI_HIP(ceed, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    ceed->event_pool.push_back(e);
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiTimeRegionBegin(Ceed ceed) {
+  if (ceed->region_start) return MI_ERR(ceed, CEED_ERROR_ACCESS, "a timed region is already open");
+  ceed->region_start = event_get(ceed);
+  MI_HIP(ceed, hipEventRecord(ceed->region_start, ceed->stream));
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedMiTimeRegionEnd(Ceed ceed, const char *name) {
+  if (!ceed->region_start) return MI_ERR(ceed, CEED_ERROR_ACCESS, "no timed region is open");
+  hipEvent_t stop = event_get(ceed);
+  MI_HIP(ceed, hipEventRecord(stop, ceed->stream));
+  ceed->pending_events.push_back({name, {ceed->region_start, stop}});
+  ceed->region_start = nullptr;
   return CEED_ERROR_SUCCESS;
 }
 

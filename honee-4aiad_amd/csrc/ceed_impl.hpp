@@ -28,6 +28,7 @@ struct Ceed_private {
   std::map<std::string, std::pair<int, double>> kernel_times;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events;
   std::vector<hipEvent_t>                  event_pool;
+  hipEvent_t                               region_start = nullptr;  // open CeedMiTimeRegionBegin
   double                                  *d_scratch = nullptr;  // reduction scratch (norms)
   double                                  *h_scratch = nullptr;
   int                                      scratch_len = 0;

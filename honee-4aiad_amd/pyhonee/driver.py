@@ -147,6 +147,8 @@ def load_libs():
         c.CeedMiSetKernelTiming.argtypes = [ct.c_void_p, ct.c_bool]
         c.CeedMiResetKernelTimes.argtypes = [ct.c_void_p]
         c.CeedMiGetKernelTimes.argtypes = [ct.c_void_p, ct.c_int, ct.c_char_p, ct.c_size_t, _ip, _dp, _ip]
+        c.CeedMiTimeRegionBegin.argtypes = [ct.c_void_p]
+        c.CeedMiTimeRegionEnd.argtypes = [ct.c_void_p, ct.c_char_p]
         c.CeedMiSynchronize.argtypes = [ct.c_void_p]
         _libs["setup"] = lib
     return _libs["ceed"], _libs["setup"]
@@ -370,6 +372,15 @@ class NewtonianProblem:
 
     def set_timing(self, enable: bool):
         self.ceedlib.CeedMiSetKernelTiming(self.ceed, enable)
+
+    def region_begin(self):
+        """Start a timed region on the backend stream (two events around back-to-back work; see kernel_times)."""
+        if self.ceedlib.CeedMiTimeRegionBegin(self.ceed):
+            raise RuntimeError("CeedMiTimeRegionBegin failed")
+
+    def region_end(self, name: str):
+        if self.ceedlib.CeedMiTimeRegionEnd(self.ceed, name.encode()):
+            raise RuntimeError("CeedMiTimeRegionEnd failed")
 
     def reset_times(self):
         self.ceedlib.CeedMiResetKernelTimes(self.ceed)

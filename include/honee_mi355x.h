@@ -113,6 +113,11 @@ int CeedMiGetStream(Ceed ceed, void **hip_stream);
 int CeedMiSetKernelTiming(Ceed ceed, bool enable);
 int CeedMiResetKernelTimes(Ceed ceed);
 int CeedMiGetKernelTimes(Ceed ceed, int max_kernels, char *names, size_t names_len, int *launches, double *total_ms, int *num_kernels);
+// Region timing on the backend stream, independent of per-kernel timing: the work launched between Begin and End is
+// bracketed by two events (no events between its kernels) and reported by CeedMiGetKernelTimes under name, one
+// launch per region.  For steady-state durations of back-to-back operator applies.
+int CeedMiTimeRegionBegin(Ceed ceed);
+int CeedMiTimeRegionEnd(Ceed ceed, const char *name);
 
 // Device pointer of a vector's FP32 shadow array (the jac_data mirror written in jacobian_precision=fp32 mode).
 int CeedMiVectorGetFp32ShadowRead(CeedVector vec, const float **d_array, CeedSize *length);
