@@ -191,8 +191,40 @@ def faces_c3(steps=20, warmup=3):
     prob.add_boundary(0, fe, fi)
     out = {"config": "channel 48x32x24 p=3 Q=4, stretching 1.1, primitive, SUPG, boundary-integral faces on both walls (3072 faces)",
            "dof": dof}
-    for name, fn in (("ifunction", lambda: prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)),
-                     ("ijacobian", lambda: prob.ijacobian(dq, v, 500.0, mem=CEED_MEM_DEVICE))):
+    out.update(_apply_times(prob, q, qdot, g, dq, v, dof, steps, warmup))
+    prob.close()
+    # every face kind on the same walls: per-launch times of the face kernels inside the composite applies and of the
+    # composite's diagonal assembly (volume + faces)
+    P0, theta0 = float(Y[..., 0].mean()), float(Y[..., 4].mean())
+    params = {1: np.array([P0, 1.0, 0.0, 0.0, theta0]), 2: np.array([P0, 1.0, 0.0, 0.0, theta0]),
+              3: np.array([P0, theta0, 1.0, 1e-2]), 4: np.array([P0, theta0, 1.0, 1e-2]), 5: None}
+    names = {0: "boundary_integral", 1: "freestream_hll", 2: "freestream_hllc", 3: "riemann_outflow", 4: "pressure_outflow",
+             5: "slip"}
+    diag = torch.zeros_like(q)
+    kinds = {}
+    for kind in range(6):
+        prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+        prob.add_boundary(kind, fe, fi, params.get(kind))
+        r = _apply_times(prob, q, qdot, g, dq, v, dof, steps, warmup, diag=diag)
+        kinds[names[kind]] = {k: {n: t for n, t in r[k]["kernel_ms"].items() if "newtonian" not in n and "gather" not in n}
+                              for k in ("ifunction", "ijacobian", "jacobi_diagonal")}
+        prob.close()
+    out["face_kernel_ms"] = kinds
+    return out
+
+
+def _apply_times(prob, q, qdot, g, dq, v, dof, steps, warmup, diag=None):
+    """Steady-state composite apply times (host clock over `steps` back-to-back applies) and per-launch kernel times
+    (per-kernel events, a separate pass) of the IFunction and IJacobian, and of the Jacobi diagonal when diag is
+    given."""
+    from pyhonee.driver import CEED_MEM_DEVICE
+
+    out = {}
+    fns = [("ifunction", lambda: prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)),
+           ("ijacobian", lambda: prob.ijacobian(dq, v, 500.0, mem=CEED_MEM_DEVICE))]
+    if diag is not None:
+        fns.append(("jacobi_diagonal", lambda: prob.ijacobian_diagonal(diag, 500.0, mem=CEED_MEM_DEVICE)))
+    for name, fn in fns:
         for _ in range(warmup):
             fn()
         prob.synchronize()
@@ -211,7 +243,6 @@ def faces_c3(steps=20, warmup=3):
         # ms per launch by kernel: the fused volume kernel, the face kernel and the node gathers of both scatters
         per = {k: v_[1] / v_[0] for k, v_ in kt.items()}
         out[name] = {"apply_ms": t_apply * 1e3, "gdofs_with_faces": dof / t_apply / 1e9, "kernel_ms": per}
-    prob.close()
     return out
 
 
@@ -327,6 +358,20 @@ def main():
                 fn()
             prob.region_end(name)
             apply_s[name] = prob.kernel_times()[name][1] / args.steps * 1e-3
+        # Jacobi and point-block diagonal assembly of the IJacobian (the preconditioners of the implicit solve)
+        diag = torch.zeros_like(q)
+        pb = torch.zeros(25 * mesh.nnodes, dtype=torch.float64, device=dev)
+        for name, fn, n in (("ijacobian_diagonal", lambda: prob.ijacobian_diagonal(diag, shift, mem=CEED_MEM_DEVICE), args.steps),
+                            ("ijacobian_point_block", lambda: prob.ijacobian_point_block_diagonal(pb, shift, mem=CEED_MEM_DEVICE),
+                             max(2, args.steps // 10))):
+            fn()
+            prob.synchronize()
+            prob.reset_times()
+            prob.region_begin()
+            for _ in range(n):
+                fn()
+            prob.region_end(name + "_assembly")
+            apply_s[name] = prob.kernel_times()[name + "_assembly"][1] / n * 1e-3
         results[label]["apply_s"] = apply_s
         if label == "fp64":
             # timing-free pass for the whole-step rate (events add host work between launches)
@@ -377,6 +422,14 @@ def main():
         line["ijacobian_fp32_gdofs"] = dof / t32 / 1e9
         line["ijacobian_fp32_roofline_frac"] = b32 / t32 / 1e9 / HBM_PEAK_GBS
     line["ijacobian_roofline_frac"] = b_ij / t_ij / 1e9 / HBM_PEAK_GBS
+    # diagonal assembly reads what the IJacobian apply reads except the input vector
+    line["assembly"] = {}
+    for label, r in results.items():
+        a = r["apply_s"]
+        b_dg = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes, jac_bytes=4 if label == "fp32" else 8)[1] - 8 * dof
+        line["assembly"][label] = {"jacobi_diagonal_ms": a["ijacobian_diagonal"] * 1e3,
+                                   "jacobi_diagonal_hbm_frac": b_dg / a["ijacobian_diagonal"] / 1e9 / HBM_PEAK_GBS,
+                                   "point_block_diagonal_ms": a["ijacobian_point_block"] * 1e3}
     if args.ts_steps > 0:
         line["time_integration"] = time_integration_c2(mesh, tab, Y, args.ts_steps)
     if not args.no_explicit:

@@ -215,6 +215,102 @@ struct Contract5 {
   }
 };
 
+// Diagonal-assembly contraction of the ten slot-pair weights W[pair][k][q] (k < NC components; pairs 0..3 the diagonal
+// pairs of {value, d/dX0, d/dX1, d/dX2}, then (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)) with their 1D factor triples
+// (Fx, Fy, Fz) drawn from BB = B.B, GB = G.B and GG = G.G:
+//   0 (BB,BB,BB)  1 (GG,BB,BB)  2 (BB,GG,BB)  3 (BB,BB,GG)  4 (GB,BB,BB)
+//   5 (BB,GB,BB)  6 (BB,BB,GB)  7 (GB,GB,BB)  8 (GB,BB,GB)  9 (BB,GB,GB).
+// r[k] += sum_pair sum_q Fz Fy Fx W[pair][k](q) for node t.  Sum factorization with grouping: after the z pass the
+// pairs that share (Fx, Fy) are summed (10 z passes -> 6 arrays), after the y pass those that share Fx (6 y passes
+// -> 3 arrays), so 19 passes replace 30.  W is overwritten.  Barriers are block-wide; W complete before the call.
+template <int P, int Q, int NC>
+struct ContractPairs {
+  using D = Dims<P, Q>;
+  int t;
+
+  __device__ void zacc(const double (*src)[D::MAX3], const double *Fz, double (&acc)[NC]) const {
+    if (t < D::PQQ) {
+      const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int z = 0; z < Q; z++) acc[k] += Fz[z * P + c] * src[k][(z * Q + iy) * Q + ix];
+    }
+  }
+  __device__ void yacc(const double (*src)[D::MAX3], const double *Fy, double (&acc)[NC]) const {
+    if (t < D::PPQ) {
+      const int ix = t % Q, b = (t / Q) % P, c = t / (Q * P);
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int y = 0; y < Q; y++) acc[k] += Fy[y * P + b] * src[k][(c * Q + y) * Q + ix];
+    }
+  }
+  __device__ void xacc(const double (*src)[D::MAX3], const double *Fx, double (&acc)[NC]) const {
+    if (t < D::P3) {
+      const int a = t % P, cb = t / P;
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int x = 0; x < Q; x++) acc[k] += Fx[x * P + a] * src[k][cb * Q + x];
+    }
+  }
+  // every thread has read the group's inputs before any thread overwrites the first of them
+  __device__ void put(double (*dst)[D::MAX3], const double (&acc)[NC], int n) const {
+    __syncthreads();
+    if (t < n)
+#pragma unroll
+      for (int k = 0; k < NC; k++) dst[k][t] = acc[k];
+  }
+
+  __device__ void add(double (*W)[NC][D::MAX3], const double *BB, const double *GB, const double *GG, double (&r)[NC]) const {
+    double a[NC];
+    __syncthreads();
+    // z pass, grouped by (Fx, Fy); each group's sum goes to its first pair's slot (read by no other group)
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[0], BB, a), zacc(W[6], GB, a), zacc(W[3], GG, a);
+    put(W[0], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[4], BB, a), zacc(W[8], GB, a);
+    put(W[4], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[5], BB, a), zacc(W[9], GB, a);
+    put(W[5], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[1], BB, a);
+    put(W[1], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[2], BB, a);
+    put(W[2], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[7], BB, a);
+    put(W[7], a, D::PQQ);
+    __syncthreads();
+    // y pass, grouped by Fx: (BB,BB) (BB,GB) (BB,GG) in slots 0, 5, 2; (GB,BB) (GB,GB) in 4, 7; (GG,BB) in 1
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[0], BB, a), yacc(W[5], GB, a), yacc(W[2], GG, a);
+    put(W[0], a, D::PPQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[4], BB, a), yacc(W[7], GB, a);
+    put(W[4], a, D::PPQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[1], BB, a);
+    put(W[1], a, D::PPQ);
+    __syncthreads();
+    // x pass
+    xacc(W[0], BB, r), xacc(W[4], GB, r), xacc(W[1], GG, r);
+  }
+};
+
 // Multi-component basis stage: the element's LDS stage X[slot][Q^3] holds one slot per (field component, point);
 // every pass treats NC components together, so each 1D coefficient row is read from LDS once per pass and the NC
 // independent contractions overlap their LDS latency.  Barriers are block-wide; a pass reads other threads' slots only

@@ -295,12 +295,13 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
 // Diagonal of a linear operator whose quadrature-point action is Op::action (IJacobian)
 // ------------------------------------------------------------------------------------------------------------------
 template <int P, int Q, class Op, typename T>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm          = Dims<P, Q>;
   constexpr int EPB = elems_per_block<Q>();
   constexpr int Q3  = Dm::Q3;
   __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
   __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
+  __shared__ double sWd[EPB][10][1][Dm::MAX3];  // slot-pair weights of one component
   for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
     sBB[i] = a.bt.BB[i];
     sGB[i] = a.bt.GB[i];
@@ -310,6 +311,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
   load_tables<P, Q>(a.bt, sB, sDc);
   const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
   Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
+  double(*Wd)[1][Dm::MAX3] = sWd[le];
   const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
 
   for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
@@ -326,29 +328,50 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
     double             qd[10];
     nsop::fetch_qdata<Q>(a, e, t, ok, qd);
     Op::template load_point<Q>(a, ctx, e, t, ok, qd, u2, p);
+    // One action body in the code: the unit input (s, c) and the rows of M are selected by comparisons with the
+    // runtime loop indices (no runtime array index), so only one action's temporaries are live at a time.
+#pragma unroll 1
     for (int c = 0; c < 5; c++) {
       double M[4][4] = {{0}};
+#pragma unroll 1
       for (int s = 0; s < Op::kInSlots; s++) {
-        T dq[5] = {0, 0, 0, 0, 0}, gref[3][5] = {{0}};
-        if (s == 0) dq[c] = 1;
-        else gref[s - 1][c] = 1;
+        T dq[5], gref[3][5];
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++) {
+          dq[cc] = (s == 0 && cc == c) ? T(1) : T(0);
+#pragma unroll
+          for (int d = 0; d < 3; d++) gref[d][cc] = (s == 1 + d && cc == c) ? T(1) : T(0);
+        }
         double v[5], gw[3][5];
         Op::action(ctx, p, dq, gref, v, gw);
-        M[s][0] = ok ? v[c] : 0.0;
-        for (int d = 0; d < 3; d++) M[s][1 + d] = ok ? gw[d][c] : 0.0;
+        double vc = 0, gc[3] = {0, 0, 0};
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++) {
+          vc = cc == c ? v[cc] : vc;
+#pragma unroll
+          for (int d = 0; d < 3; d++) gc[d] = cc == c ? gw[d][cc] : gc[d];
+        }
+#pragma unroll
+        for (int ss = 0; ss < 4; ss++) {
+          if (ss == s) {
+            M[ss][0] = ok ? vc : 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) M[ss][1 + d] = ok ? gc[d] : 0.0;
+          }
+        }
       }
-      // sum over input slot a and test slot b of M[a][b] phi_a phi_b, grouped by 1D factor triples
-      double r = el.contract_t(M[0][0], sBB, sBB, sBB);
-      r += el.contract_t(M[0][1] + M[1][0], sGB, sBB, sBB);
-      r += el.contract_t(M[0][2] + M[2][0], sBB, sGB, sBB);
-      r += el.contract_t(M[0][3] + M[3][0], sBB, sBB, sGB);
-      r += el.contract_t(M[1][1], sGG, sBB, sBB);
-      r += el.contract_t(M[2][2], sBB, sGG, sBB);
-      r += el.contract_t(M[3][3], sBB, sBB, sGG);
-      r += el.contract_t(M[1][2] + M[2][1], sGB, sGB, sBB);
-      r += el.contract_t(M[1][3] + M[3][1], sGB, sBB, sGB);
-      r += el.contract_t(M[2][3] + M[3][2], sBB, sGB, sGB);
-      if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r);
+      // sum over input slot a and test slot b of M[a][b] phi_a phi_b: the weights of the unordered slot pairs, each with
+      // its 1D factor triple, contracted together (basis3::ContractPairs); the contraction's coefficient addresses
+      // derive from an opaque copy of t, so that its table loads are not hoisted out of the component loop
+      int tc = t;
+      asm volatile("" : "+v"(tc));
+      __syncthreads();  // the previous component's contraction has read Wd
+      Wd[0][0][t] = M[0][0], Wd[1][0][t] = M[1][1], Wd[2][0][t] = M[2][2], Wd[3][0][t] = M[3][3];
+      Wd[4][0][t] = M[0][1] + M[1][0], Wd[5][0][t] = M[0][2] + M[2][0], Wd[6][0][t] = M[0][3] + M[3][0];
+      Wd[7][0][t] = M[1][2] + M[2][1], Wd[8][0][t] = M[1][3] + M[3][1], Wd[9][0][t] = M[2][3] + M[3][2];
+      double r[1] = {0};
+      basis3::ContractPairs<P, Q, 1>{tc}.add(Wd, sBB, sGB, sGG, r);
+      if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r[0]);
     }
   }
 }
@@ -357,10 +380,18 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_diagonal(V
 // Point-block diagonal of a linear operator whose quadrature-point action is Op::action: for each input component
 // c_in and input slot s (value, three reference derivatives), one action gives the test-slot weights of all five
 // output components; the weights of each unordered slot pair are summed (the diagonal's 1D products are symmetric in
-// the pair) and contracted five components at a time.  Output layout [node][c_out][c_in] at 25 * (offset / 5).
+// the pair) and contracted five components at a time, grouped by shared 1D factors (basis3::ContractPairs).  Output
+// layout [node][c_out][c_in] at 25 * (offset / 5).
 // ------------------------------------------------------------------------------------------------------------------
+// The weight slot of the unordered input/test slot pair (s, b), as in k_diagonal: 0..3 the diagonal pairs, then
+// (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) -> 4..9
+__device__ __forceinline__ int pair_slot(int s, int b) {
+  const int i = s < b ? s : b, j = s < b ? b : s;
+  return i == j ? i : (i == 0 ? 3 + j : (i == 1 ? 5 + j : 9));
+}
+
 template <int P, int Q, class Op, typename T>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
+__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_pblock_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm          = Dims<P, Q>;
   constexpr int EPB = elems_per_block<Q>();
   constexpr int Q3  = Dm::Q3;
@@ -381,11 +412,8 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
   load_tables<P, Q>(a.bt, sB, sDc);
   const int               le = threadIdx.x / Q3, t = threadIdx.x % Q3;
   Elem<P, Q>              el(sB, sDc, s0[le], s1[le], s2[le], t);
-  basis3::Contract5<P, Q> pb{nullptr, c1[le], c2[le], t};
   double(*W)[5][Dm::MAX3] = sW[le];
   const long long         nbatch = ((long long)a.nelem + EPB - 1) / EPB;
-  // slot pairs (s, b), s <= b, as in k_diagonal: 0..3 the diagonal pairs, then (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
-  constexpr int kPair[4][4] = {{0, 4, 5, 6}, {4, 1, 7, 8}, {5, 7, 2, 9}, {6, 8, 9, 3}};
 
   for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
     const long long e_  = bt * EPB + le;
@@ -397,6 +425,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
 #pragma unroll
       for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
     }
+#pragma unroll 1
     for (int ci = 0; ci < 5; ci++) {
       __syncthreads();  // the previous component's contractions have read W
       for (int k = 0; k < NP; k++)
@@ -411,36 +440,39 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q) k_pblock_dia
       double             qd[10];
       nsop::fetch_qdata<Q>(a, eo, t, ok, qd);
       Op::template load_point<Q>(a, ctx, eo, t, ok, qd, u2, p);
+      // the slot loop unrolled (the four actions overlap; measured faster than one action body here, unlike k_diagonal)
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        T dq[5] = {0, 0, 0, 0, 0}, gref[3][5] = {{0}};
+        T dq[5], gref[3][5];
 #pragma unroll
         for (int c = 0; c < 5; c++) {  // unit input (s, ci) without a runtime array index
-          if (s == 0) dq[c] = c == ci ? T(1) : T(0);
-          else gref[s - 1][c] = c == ci ? T(1) : T(0);
+          dq[c] = (s == 0 && c == ci) ? T(1) : T(0);
+#pragma unroll
+          for (int d = 0; d < 3; d++) gref[d][c] = (s == 1 + d && c == ci) ? T(1) : T(0);
         }
         double v[5], gw[3][5];
         Op::action(ctx, p, dq, gref, v, gw);
         if (!ok) continue;
 #pragma unroll
         for (int co = 0; co < 5; co++) {
-          W[kPair[s][0]][co][t] += v[co];
+          W[pair_slot(s, 0)][co][t] += v[co];
 #pragma unroll
-          for (int d = 0; d < 3; d++) W[kPair[s][1 + d]][co][t] += gw[d][co];
+          for (int d = 0; d < 3; d++) W[pair_slot(s, 1 + d)][co][t] += gw[d][co];
         }
       }
+      // the contraction's coefficient addresses derive from an opaque copy of t, so that the compiler does not hoist
+      // the ten contractions' table loads out of the component loop (they would stay live across the actions)
+      int tc = t;
+      asm volatile("" : "+v"(tc));
       double r[5] = {0, 0, 0, 0, 0};
-      pb.add_staged(W[0], sBB, sBB, sBB, r);
-      pb.add_staged(W[4], sGB, sBB, sBB, r);
-      pb.add_staged(W[5], sBB, sGB, sBB, r);
-      pb.add_staged(W[6], sBB, sBB, sGB, r);
       if constexpr (NS > 1) {
-        pb.add_staged(W[1], sGG, sBB, sBB, r);
-        pb.add_staged(W[2], sBB, sGG, sBB, r);
-        pb.add_staged(W[3], sBB, sBB, sGG, r);
-        pb.add_staged(W[7], sGB, sGB, sBB, r);
-        pb.add_staged(W[8], sGB, sBB, sGB, r);
-        pb.add_staged(W[9], sBB, sGB, sGB, r);
+        basis3::ContractPairs<P, Q, 5>{tc}.add(W, sBB, sGB, sGG, r);
+      } else {
+        basis3::Contract5<P, Q> pb{nullptr, c1[le], c2[le], tc};
+        pb.add_staged(W[0], sBB, sBB, sBB, r);
+        pb.add_staged(W[4], sGB, sBB, sBB, r);
+        pb.add_staged(W[5], sBB, sGB, sBB, r);
+        pb.add_staged(W[6], sBB, sBB, sGB, r);
       }
       if (ok && t < Dm::P3)
 #pragma unroll

@@ -222,8 +222,9 @@ def test_point_block_diagonal_with_faces(kind, state_var, fp32):
         assert e2 < 1e-5
     else:
         assert e2 < 1e-12 and emax < 1e-11, (e2, emax)
-    # the diagonal of each block is the point diagonal
+    # the diagonal of each block is the point diagonal; in FP32 mode the two kernels evaluate the FP32 actions with
+    # different instruction sequences, and each is within 1e-5 of the FP64 oracle
     d = np.zeros_like(q)
     prob.ijacobian_diagonal(d, shift)
-    assert rel_err(np.einsum("nkk->nk", pb).ravel(), d)[0] < (1e-6 if fp32 else 1e-13)
+    assert rel_err(np.einsum("nkk->nk", pb).ravel(), d)[0] < (2e-5 if fp32 else 1e-13)
     prob.close()
