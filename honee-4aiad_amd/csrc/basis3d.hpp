@@ -215,102 +215,6 @@ struct Contract5 {
   }
 };
 
-// Diagonal-assembly contraction of the ten slot-pair weights W[pair][k][q] (k < NC components; pairs 0..3 the diagonal
-// pairs of {value, d/dX0, d/dX1, d/dX2}, then (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)) with their 1D factor triples
-// (Fx, Fy, Fz) drawn from BB = B.B, GB = G.B and GG = G.G:
-//   0 (BB,BB,BB)  1 (GG,BB,BB)  2 (BB,GG,BB)  3 (BB,BB,GG)  4 (GB,BB,BB)
-//   5 (BB,GB,BB)  6 (BB,BB,GB)  7 (GB,GB,BB)  8 (GB,BB,GB)  9 (BB,GB,GB).
-// r[k] += sum_pair sum_q Fz Fy Fx W[pair][k](q) for node t.  Sum factorization with grouping: after the z pass the
-// pairs that share (Fx, Fy) are summed (10 z passes -> 6 arrays), after the y pass those that share Fx (6 y passes
-// -> 3 arrays), so 19 passes replace 30.  W is overwritten.  Barriers are block-wide; W complete before the call.
-template <int P, int Q, int NC>
-struct ContractPairs {
-  using D = Dims<P, Q>;
-  int t;
-
-  __device__ void zacc(const double (*src)[D::MAX3], const double *Fz, double (&acc)[NC]) const {
-    if (t < D::PQQ) {
-      const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
-#pragma unroll
-      for (int k = 0; k < NC; k++)
-#pragma unroll
-        for (int z = 0; z < Q; z++) acc[k] += Fz[z * P + c] * src[k][(z * Q + iy) * Q + ix];
-    }
-  }
-  __device__ void yacc(const double (*src)[D::MAX3], const double *Fy, double (&acc)[NC]) const {
-    if (t < D::PPQ) {
-      const int ix = t % Q, b = (t / Q) % P, c = t / (Q * P);
-#pragma unroll
-      for (int k = 0; k < NC; k++)
-#pragma unroll
-        for (int y = 0; y < Q; y++) acc[k] += Fy[y * P + b] * src[k][(c * Q + y) * Q + ix];
-    }
-  }
-  __device__ void xacc(const double (*src)[D::MAX3], const double *Fx, double (&acc)[NC]) const {
-    if (t < D::P3) {
-      const int a = t % P, cb = t / P;
-#pragma unroll
-      for (int k = 0; k < NC; k++)
-#pragma unroll
-        for (int x = 0; x < Q; x++) acc[k] += Fx[x * P + a] * src[k][cb * Q + x];
-    }
-  }
-  // every thread has read the group's inputs before any thread overwrites the first of them
-  __device__ void put(double (*dst)[D::MAX3], const double (&acc)[NC], int n) const {
-    __syncthreads();
-    if (t < n)
-#pragma unroll
-      for (int k = 0; k < NC; k++) dst[k][t] = acc[k];
-  }
-
-  __device__ void add(double (*W)[NC][D::MAX3], const double *BB, const double *GB, const double *GG, double (&r)[NC]) const {
-    double a[NC];
-    __syncthreads();
-    // z pass, grouped by (Fx, Fy); each group's sum goes to its first pair's slot (read by no other group)
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[0], BB, a), zacc(W[6], GB, a), zacc(W[3], GG, a);
-    put(W[0], a, D::PQQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[4], BB, a), zacc(W[8], GB, a);
-    put(W[4], a, D::PQQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[5], BB, a), zacc(W[9], GB, a);
-    put(W[5], a, D::PQQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[1], BB, a);
-    put(W[1], a, D::PQQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[2], BB, a);
-    put(W[2], a, D::PQQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    zacc(W[7], BB, a);
-    put(W[7], a, D::PQQ);
-    __syncthreads();
-    // y pass, grouped by Fx: (BB,BB) (BB,GB) (BB,GG) in slots 0, 5, 2; (GB,BB) (GB,GB) in 4, 7; (GG,BB) in 1
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    yacc(W[0], BB, a), yacc(W[5], GB, a), yacc(W[2], GG, a);
-    put(W[0], a, D::PPQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    yacc(W[4], BB, a), yacc(W[7], GB, a);
-    put(W[4], a, D::PPQ);
-#pragma unroll
-    for (int k = 0; k < NC; k++) a[k] = 0;
-    yacc(W[1], BB, a);
-    put(W[1], a, D::PPQ);
-    __syncthreads();
-    // x pass
-    xacc(W[0], BB, r), xacc(W[4], GB, r), xacc(W[1], GG, r);
-  }
-};
-
 // Multi-component basis stage: the element's LDS stage X[slot][Q^3] holds one slot per (field component, point);
 // every pass treats NC components together, so each 1D coefficient row is read from LDS once per pass and the NC
 // independent contractions overlap their LDS latency.  Barriers are block-wide; a pass reads other threads' slots only
@@ -385,6 +289,140 @@ struct RowRot {
     y[1] = __builtin_amdgcn_mov_dpp(qy, 0x124, 0xF, 0xF, true);
     y[2] = __builtin_amdgcn_mov_dpp(qy, 0x128, 0xF, 0xF, true);
     y[3] = __builtin_amdgcn_mov_dpp(qy, 0x12C, 0xF, 0xF, true);
+  }
+};
+
+// Diagonal-assembly contraction of the ten slot-pair weights W[pair][k][q] (k < NC components; pairs 0..3 the diagonal
+// pairs of {value, d/dX0, d/dX1, d/dX2}, then (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)) with their 1D factor triples
+// (Fx, Fy, Fz) drawn from BB = B.B, GB = G.B and GG = G.G:
+//   0 (BB,BB,BB)  1 (GG,BB,BB)  2 (BB,GG,BB)  3 (BB,BB,GG)  4 (GB,BB,BB)
+//   5 (BB,GB,BB)  6 (BB,BB,GB)  7 (GB,GB,BB)  8 (GB,BB,GB)  9 (BB,GB,GB).
+// r[k] += sum_pair sum_q Fz Fy Fx W[pair][k](q) for node t.  Sum factorization with grouping: after the z pass the
+// pairs that share (Fx, Fy) are summed (10 z passes -> 6 arrays), after the y pass those that share Fx (6 y passes
+// -> 3 arrays), so 19 passes replace 30.  W is overwritten.  Barriers are block-wide; W complete before the call.
+template <int P, int Q, int NC>
+struct ContractPairs {
+  using D = Dims<P, Q>;
+  int t;
+
+  __device__ void zacc(const double (*src)[D::MAX3], const double *Fz, double (&acc)[NC]) const {
+    if (t < D::PQQ) {
+      const int ix = t % Q, iy = (t / Q) % Q, c = t / (Q * Q);
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int z = 0; z < Q; z++) acc[k] += Fz[z * P + c] * src[k][(z * Q + iy) * Q + ix];
+    }
+  }
+  __device__ void yacc(const double (*src)[D::MAX3], const double *Fy, double (&acc)[NC]) const {
+    if (t < D::PPQ) {
+      const int ix = t % Q, b = (t / Q) % P, c = t / (Q * P);
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int y = 0; y < Q; y++) acc[k] += Fy[y * P + b] * src[k][(c * Q + y) * Q + ix];
+    }
+  }
+  __device__ void xacc(const double (*src)[D::MAX3], const double *Fx, double (&acc)[NC]) const {
+    if (t < D::P3) {
+      const int a = t % P, cb = t / P;
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int x = 0; x < Q; x++) acc[k] += Fx[x * P + a] * src[k][cb * Q + x];
+    }
+  }
+  // every thread has read the group's inputs before any thread overwrites the first of them
+  __device__ void put(double (*dst)[D::MAX3], const double (&acc)[NC], int n) const {
+    __syncthreads();
+    if (t < n)
+#pragma unroll
+      for (int k = 0; k < NC; k++) dst[k][t] = acc[k];
+  }
+
+  __device__ void add(double (*W)[NC][D::MAX3], const double *BB, const double *GB, const double *GG, double (&r)[NC]) const {
+    if constexpr (P == 4 && Q == 4) {  // one wave per element: the y and x passes across lanes, on registers
+      add_cross_lane(W, BB, GB, GG, r);
+      return;
+    }
+    double a[NC];
+    __syncthreads();
+    // z pass, grouped by (Fx, Fy); each group's sum goes to its first pair's slot (read by no other group)
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[0], BB, a), zacc(W[6], GB, a), zacc(W[3], GG, a);
+    put(W[0], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[4], BB, a), zacc(W[8], GB, a);
+    put(W[4], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[5], BB, a), zacc(W[9], GB, a);
+    put(W[5], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[1], BB, a);
+    put(W[1], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[2], BB, a);
+    put(W[2], a, D::PQQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    zacc(W[7], BB, a);
+    put(W[7], a, D::PQQ);
+    __syncthreads();
+    // y pass, grouped by Fx: (BB,BB) (BB,GB) (BB,GG) in slots 0, 5, 2; (GB,BB) (GB,GB) in 4, 7; (GG,BB) in 1
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[0], BB, a), yacc(W[5], GB, a), yacc(W[2], GG, a);
+    put(W[0], a, D::PPQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[4], BB, a), yacc(W[7], GB, a);
+    put(W[4], a, D::PPQ);
+#pragma unroll
+    for (int k = 0; k < NC; k++) a[k] = 0;
+    yacc(W[1], BB, a);
+    put(W[1], a, D::PPQ);
+    __syncthreads();
+    // x pass
+    xacc(W[0], BB, r), xacc(W[4], GB, r), xacc(W[1], GG, r);
+  }
+
+  // The 4 x 4 x 4 element (lane = qx + 4 qy + 16 qz): the z passes read W from LDS into registers, the six (Fx, Fy)
+  // group sums stay in registers, and the y and x passes run across lanes (DPP row rotations and quad broadcasts, as in
+  // Stage's transpose).  All 64 lanes must be active.
+  __device__ void add_cross_lane(double (*W)[NC][D::MAX3], const double *BB, const double *GB, const double *GG, double (&r)[NC]) const {
+    static_assert(P == 4 && Q == 4, "cross-lane passes need the 4 x 4 x 4 element");
+    const int qx = t & 3, qy = (t >> 2) & 3;
+    double    g[6][NC];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+      for (int k = 0; k < NC; k++) g[m][k] = 0;
+    __syncthreads();  // W complete
+    zacc(W[0], BB, g[0]), zacc(W[6], GB, g[0]), zacc(W[3], GG, g[0]);  // (BB, BB)
+    zacc(W[4], BB, g[1]), zacc(W[8], GB, g[1]);                        // (GB, BB)
+    zacc(W[5], BB, g[2]), zacc(W[9], GB, g[2]);                        // (BB, GB)
+    zacc(W[1], BB, g[3]);                                              // (GG, BB)
+    zacc(W[2], BB, g[4]);                                              // (BB, GG)
+    zacc(W[7], BB, g[5]);                                              // (GB, GB)
+    const RowRot rr(qy);
+    double       yBB[4], yGB[4], yGG[4], xBB[4], xGB[4], xGG[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      yBB[i] = BB[rr.y[i] * 4 + qy], yGB[i] = GB[rr.y[i] * 4 + qy], yGG[i] = GG[rr.y[i] * 4 + qy];
+      xBB[i] = BB[i * 4 + qx], xGB[i] = GB[i * 4 + qx], xGG[i] = GG[i * 4 + qx];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const double h0 = row_contract(yBB, g[0][k]) + row_contract(yGB, g[2][k]) + row_contract(yGG, g[4][k]);  // Fx = BB
+      const double h1 = row_contract(yBB, g[1][k]) + row_contract(yGB, g[5][k]);                                // Fx = GB
+      const double h2 = row_contract(yBB, g[3][k]);                                                             // Fx = GG
+      r[k] += quad_contract(xBB, h0) + quad_contract(xGB, h1) + quad_contract(xGG, h2);
+    }
   }
 };
 
