@@ -166,7 +166,9 @@ def faces_c3(steps=20, warmup=3):
     wall-normal stretching 1.1, p=3 Q=4, primitive FP64, SUPG, Re_tau 180 (mu = 2 / 180^2 at umax = H = 1), Poiseuille
     state with 10% seeded Fourier perturbations, consistent boundary-integral faces on both walls.  Reports the
     IFunction and IJacobian composite applies (volume + faces) in GDoF/s and the per-kernel times; the volume-only
-    figures are the periodic workload's."""
+    figures are the periodic workload's.  Then, for every face kind (boundary integral, freestream HLL and HLLC,
+    Riemann and pressure outflow, slip) on the same walls, the per-launch times of its face kernels in the IFunction,
+    the IJacobian and the Jacobi diagonal assembly."""
     import torch
 
     from pyhonee import mesh as M
