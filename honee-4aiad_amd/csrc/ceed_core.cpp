@@ -30,6 +30,7 @@ const CeedQFunction       CEED_QFUNCTION_NONE       = &s_qf_none;
 const CeedInt             CEED_STRIDES_BACKEND[3]   = {0, 0, 0};
 CeedRequest *const        CEED_REQUEST_IMMEDIATE    = &s_req_immediate;
 CeedRequest *const        CEED_REQUEST_ORDERED      = &s_req_ordered;
+const char *const         CeedMemTypes[]            = {"host", "device"};
 }
 
 static bool is_active(CeedVector v) { return v == CEED_VECTOR_ACTIVE; }
@@ -38,6 +39,10 @@ static bool is_none_vec(CeedVector v) { return v == CEED_VECTOR_NONE; }
 // ---------------------------------------------------------------------------------------------------------------------
 // Errors
 // ---------------------------------------------------------------------------------------------------------------------
+// Message of the last error raised without a Ceed (CeedInit failures): CeedGetErrorMessage(NULL, ...) returns it.
+// Nothing is printed: HONEE's test harness treats any stderr output as a failure.
+static thread_local std::string init_err_msg;
+
 int ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int ecode, const char *fmt, ...) {
   va_list args;
   va_start(args, fmt);
@@ -47,8 +52,9 @@ int ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int 
   } else if (ceed) {
     ret = CeedErrorStore(ceed, file, line, func, ecode, fmt, &args);
   } else {
-    vfprintf(stderr, fmt, args);
-    fprintf(stderr, "\n");
+    char buf[1024];
+    vsnprintf(buf, sizeof(buf), fmt, args);
+    init_err_msg = buf;
   }
   va_end(args);
   return ret;
@@ -74,6 +80,7 @@ extern "C" int CeedErrorStore(Ceed ceed, const char *filename, int lineno, const
   if (n < 0) n = 0;
   vsnprintf(buf + n, sizeof(buf) - n, format, *args);
   if (ceed) ceed->err_msg = buf;
+  else init_err_msg = buf;
   return ecode;
 }
 
@@ -90,8 +97,18 @@ extern "C" int CeedSetErrorHandler(Ceed ceed, CeedErrorHandler handler) {
   return CEED_ERROR_SUCCESS;
 }
 
+static int init_error(int ecode, const char *fmt, ...) {
+  char    buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  init_err_msg = buf;
+  return ecode;
+}
+
 extern "C" int CeedGetErrorMessage(Ceed ceed, const char **err_msg) {
-  *err_msg = ceed->err_msg.c_str();
+  *err_msg = ceed ? ceed->err_msg.c_str() : init_err_msg.c_str();
   return CEED_ERROR_SUCCESS;
 }
 
@@ -105,52 +122,47 @@ extern "C" int CeedResetErrorMessage(Ceed ceed, const char **err_msg) {
 // Ceed
 // ---------------------------------------------------------------------------------------------------------------------
 extern "C" int CeedInit(const char *resource, Ceed *ceed) {
-  if (!resource || !ceed) return CEED_ERROR_MAJOR;
+  if (!resource || !ceed) return init_error(CEED_ERROR_MAJOR, "CeedInit: null resource or output pointer");
   std::string res(resource);
   std::string base = res.substr(0, res.find(':'));
-  if (base != "/gpu/hip/mi355x" && base != "/gpu/hip" && base != "/gpu/hip/gen" && base != "/gpu/hip/shared") {
-    fprintf(stderr, "CeedInit: resource '%s' is not provided by the MI355X library (use /gpu/hip/mi355x)\n", resource);
-    return CEED_ERROR_UNSUPPORTED;
-  }
-  Ceed c      = new Ceed_private();
-  c->resource = "/gpu/hip/mi355x";
-  int ndev    = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    delete c;
-    fprintf(stderr, "CeedInit: no HIP device available for /gpu/hip/mi355x\n");
-    return CEED_ERROR_BACKEND;
-  }
-  (void)hipGetDevice(&c->device);
-  // options: ":key=value" pairs
-  size_t pos = res.find(':');
+  if (base != "/gpu/hip/mi355x" && base != "/gpu/hip" && base != "/gpu/hip/gen" && base != "/gpu/hip/shared")
+    return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: resource '%s' is not provided by the MI355X library (use /gpu/hip/mi355x)", resource);
+  // options: ":key=value" pairs, validated before the device is touched
+  int    device = -1, scatter = 1;
+  bool   jac_fp32 = false;
+  size_t pos      = res.find(':');
   while (pos != std::string::npos) {
     size_t      next = res.find_first_of(":,", pos + 1);
     std::string kv   = res.substr(pos + 1, next == std::string::npos ? std::string::npos : next - pos - 1);
     size_t      eq   = kv.find('=');
     std::string k = kv.substr(0, eq), v = eq == std::string::npos ? "" : kv.substr(eq + 1);
     if (k == "device_id") {
-      c->device = atoi(v.c_str());
+      device = atoi(v.c_str());
     } else if (k == "jacobian_precision") {
-      c->jac_fp32 = (v == "fp32" || v == "float");
+      if (v == "fp32" || v == "float") jac_fp32 = true;
+      else if (v == "fp64" || v == "double") jac_fp32 = false;
+      else return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: jacobian_precision must be fp64 or fp32, got '%s'", v.c_str());
     } else if (k == "scatter") {
-      if (v == "atomic") c->scatter = 0;
-      else if (v == "twopass") c->scatter = 1;
-      else {
-        fprintf(stderr, "CeedInit: scatter must be atomic or twopass, got '%s'\n", v.c_str());
-        delete c;
-        return CEED_ERROR_UNSUPPORTED;
-      }
+      if (v == "atomic") scatter = 0;
+      else if (v == "twopass") scatter = 1;
+      else return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: scatter must be twopass or atomic, got '%s'", v.c_str());
     } else if (!k.empty()) {
-      fprintf(stderr, "CeedInit: unknown resource option '%s'\n", k.c_str());
-      delete c;
-      return CEED_ERROR_UNSUPPORTED;
+      return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: unknown resource option '%s'", k.c_str());
     }
     pos = next;
   }
-  if (hipSetDevice(c->device) != hipSuccess) {
-    delete c;
-    return CEED_ERROR_BACKEND;
-  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return init_error(CEED_ERROR_BACKEND, "CeedInit: no HIP device available for /gpu/hip/mi355x");
+  if (device < 0) (void)hipGetDevice(&device);
+  if (device >= ndev || hipSetDevice(device) != hipSuccess)
+    return init_error(CEED_ERROR_BACKEND, "CeedInit: cannot select HIP device %d (%d available)", device, ndev);
+  Ceed c      = new Ceed_private();
+  c->self     = c;
+  c->resource = "/gpu/hip/mi355x";
+  c->device   = device;
+  c->jac_fp32 = jac_fp32;
+  c->scatter  = scatter;
+  init_err_msg.clear();
   *ceed = c;
   return CEED_ERROR_SUCCESS;
 }
@@ -1373,6 +1385,51 @@ extern "C" int CeedOperatorCompositeGetNumSub(CeedOperator op, CeedInt *num_sub)
   return CEED_ERROR_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// Backend-header entry points (ceed/backend.h)
+// ---------------------------------------------------------------------------------------------------------------------
+static_assert(offsetof(Ceed_private, self) == 0, "a Ceed must start with itself");
+static_assert(offsetof(CeedVector_private, ceed) == 0 && offsetof(CeedElemRestriction_private, ceed) == 0 &&
+                  offsetof(CeedBasis_private, ceed) == 0 && offsetof(CeedQFunction_private, ceed) == 0 &&
+                  offsetof(CeedQFunctionContext_private, ceed) == 0 && offsetof(CeedOperator_private, ceed) == 0,
+              "every object must start with its Ceed");
+
+extern "C" Ceed CeedObjectReturnCeed(CeedObject obj) { return obj ? *(Ceed *)obj : nullptr; }
+
+extern "C" int CeedOperatorGetOperatorAssemblyData(CeedOperator op, CeedOperatorAssemblyData *data) {
+  if (op->composite) return MI_ERR(op->ceed, CEED_ERROR_MINOR, "assembly data is defined for non-composite operators");
+  if (!op->asm_data) {
+    auto *d   = new CeedOperatorAssemblyData_private();
+    auto  add = [](std::vector<CeedBasis> &v, CeedBasis b) {
+      if (b && b != CEED_BASIS_NONE && std::find(v.begin(), v.end(), b) == v.end()) v.push_back(b);
+    };
+    for (auto *f : op->in_fields)
+      if (is_active(f->vec)) add(d->bases_in, f->basis);
+    for (auto *f : op->out_fields)
+      if (is_active(f->vec)) add(d->bases_out, f->basis);
+    op->asm_data = d;
+  }
+  *data = op->asm_data;
+  return CEED_ERROR_SUCCESS;
+}
+
+extern "C" int CeedOperatorAssemblyDataGetBases(CeedOperatorAssemblyData data, CeedInt *num_active_bases_in, CeedBasis **active_bases_in,
+                                                const CeedScalar ***assembled_bases_in, CeedInt *num_active_bases_out,
+                                                CeedBasis **active_bases_out, const CeedScalar ***assembled_bases_out) {
+  if (!data) return CEED_ERROR_MAJOR;
+  if (assembled_bases_in || assembled_bases_out) {
+    Ceed c = nullptr;
+    if (!data->bases_in.empty()) c = data->bases_in[0]->ceed;
+    else if (!data->bases_out.empty()) c = data->bases_out[0]->ceed;
+    return MI_ERR(c, CEED_ERROR_UNSUPPORTED, "assembled basis matrices are not formed by the MI355X backend");
+  }
+  if (num_active_bases_in) *num_active_bases_in = (CeedInt)data->bases_in.size();
+  if (active_bases_in) *active_bases_in = data->bases_in.data();
+  if (num_active_bases_out) *num_active_bases_out = (CeedInt)data->bases_out.size();
+  if (active_bases_out) *active_bases_out = data->bases_out.data();
+  return CEED_ERROR_SUCCESS;
+}
+
 extern "C" int CeedOperatorCompositeGetSubList(CeedOperator op, CeedOperator **sub_ops) {
   if (!op->composite) return MI_ERR(op->ceed, CEED_ERROR_MINOR, "not a composite operator");
   *sub_ops = op->subs.data();
@@ -1406,6 +1463,8 @@ extern "C" int CeedOperatorSetField(CeedOperator op, const char *field_name, Cee
     op_plan_destroy(op->plan);
     op->plan = nullptr;
   }
+  delete op->asm_data;
+  op->asm_data = nullptr;
   return CEED_ERROR_SUCCESS;
 }
 
@@ -1724,6 +1783,7 @@ extern "C" int CeedOperatorDestroy(CeedOperator *op) {
     return CEED_ERROR_SUCCESS;
   }
   if (o->plan) op_plan_destroy(o->plan);
+  delete o->asm_data;
   for (auto *fields : {&o->in_fields, &o->out_fields}) {
     for (auto *f : *fields) {
       CeedElemRestrictionDestroy(&f->rstr);

@@ -9,10 +9,14 @@
 #include <vector>
 
 #include "ceed.h"
+#include "ceed/backend.h"
 #include "honee_mi355x.h"
 #include "kernels.hpp"
 
+// Every object starts with the Ceed it belongs to, so that a cast to CeedObject reads it (CeedObjectReturnCeed); a
+// Ceed starts with itself.
 struct Ceed_private {
+  Ceed             self     = nullptr;
   int              refcount = 1;
   std::string      resource;
   int              device   = 0;
@@ -166,6 +170,11 @@ struct CeedOperatorField_private {
 
 struct OpPlan;  // backend binding of a non-composite operator
 
+// CeedOperatorGetOperatorAssemblyData: the distinct active bases of the operator's input and output fields
+struct CeedOperatorAssemblyData_private {
+  std::vector<CeedBasis> bases_in, bases_out;
+};
+
 struct CeedOperator_private {
   Ceed                                     ceed;
   int                                      refcount = 1;
@@ -177,6 +186,7 @@ struct CeedOperator_private {
   CeedInt                                  num_tabs = 0;
   std::vector<std::unique_ptr<CeedContextFieldLabel_private>> labels;
   OpPlan                                  *plan = nullptr;
+  CeedOperatorAssemblyData_private        *asm_data = nullptr;
 };
 
 // ---- helpers shared by the translation units ----

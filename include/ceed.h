@@ -4,7 +4,10 @@
 //   :device_id=<n>              HIP device ordinal (default: current device)
 //   :jacobian_precision=fp32    run Newtonian IJacobian operators in FP32 (FP32 jac_data mirror, FP32 Tau and flux
 //                               Jacobians); FP64 is the default
-//   :scatter=atomic|colored     scatter-add strategy of volume operators (colored = deterministic element colouring)
+//   :scatter=twopass|atomic     scatter-add strategy of volume operators: twopass (the default) writes element-node
+//                               contributions and sums them per node in a fixed order (deterministic); atomic adds
+//                               them with FP64 atomics
+// A CeedInit failure stores its message without printing; CeedGetErrorMessage(NULL, &msg) returns it.
 // The resource name sits under /gpu/hip so HONEE selects device vectors (examples/navierstokes.c:95-103).
 //
 // Every entry point returns an int error code (CEED_ERROR_SUCCESS == 0) and never aborts; the message of the last
@@ -39,6 +42,8 @@ typedef struct CeedContextFieldLabel_private *CeedContextFieldLabel;
 typedef struct CeedRequest_private          *CeedRequest;
 
 typedef enum { CEED_MEM_HOST, CEED_MEM_DEVICE } CeedMemType;
+// Names of the memory types, indexed by CeedMemType (src/misc.c:330)
+extern const char *const CeedMemTypes[];
 typedef enum { CEED_COPY_VALUES, CEED_USE_POINTER, CEED_OWN_POINTER } CeedCopyMode;
 typedef enum { CEED_NORM_1, CEED_NORM_2, CEED_NORM_MAX } CeedNormType;
 typedef enum {

@@ -12,6 +12,9 @@
 
 typedef int32_t CeedInt;
 typedef ptrdiff_t CeedSize;
+// printf conversions of CeedInt and CeedSize (src/petsc_ops.c:69, src/misc.c:398, src/mat-ceed.c:1497)
+#define CeedInt_FMT "d"
+#define CeedSize_FMT "td"
 typedef int8_t  CeedInt8;
 typedef double  CeedScalar;
 #define CEED_SCALAR_TYPE CEED_SCALAR_FP64

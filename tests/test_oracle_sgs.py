@@ -8,6 +8,11 @@
   weights (tests/golden/dd_sgs_data, read with the product's reader); frame invariance of the whole closure (a proper
   rotation R of the velocity gradient and of the anisotropy tensor rotates the stress to R tau R^T); the stress is
   symmetric and traceless-free (no constraint), and vanishes with the velocity gradient.
+* Input normalization and output de-normalization (sgs_dd_utils.h:40-127) by closed forms: a diagonal (pure strain)
+  gradient gives its sorted entries, zero vorticity and nu / delta^2 over |grad u|; a simple shear through a diagonal
+  anisotropy tensor gives (1/2, 0, -1/2, 0, +-1, nu / (delta^2 s)); a rigid rotation gives the oriented vorticity
+  (|w0|, sign(w0) sign(w2) w1) over |w| / sqrt 2; a zero-weight network gives the stress
+  delta^2 |grad u|^2 ((hi - lo)(b2 - 1) + hi) on the permuted diagonal.
 * Projections: the lumped velocity-gradient projection is exact for a linear velocity field; the CG anisotropy
   projection is exact (constant) on a uniform box mesh.
 * IFunction_NodalSgs: discrete conservation (the residual sums to zero over a periodic box), zero residual for a
@@ -178,3 +183,103 @@ def test_sgs_nodal_is_pointwise(model):
     for i in range(n):
         r = O.sgs_point(om, G[9 * i: 9 * i + 9].reshape(3, 3), A[7 * i: 7 * i + 6], A[7 * i + 6], gas.mu / rho[i])
         assert np.allclose(km[i], r["km_sgs"], rtol=1e-14, atol=1e-16)
+
+
+# ---- closed forms of the model's input normalization and output de-normalization (sgs_dd_utils.h:40-127) ----
+def _zero_net(b2, bounds, alpha=0.3):
+    """A 6-20-6 network whose output is its bias b2 (all weights zero), with min-max bounds [6][2]."""
+    return O.SgsModel({"b1": np.linspace(-1, 1, 20), "b2": np.asarray(b2, float), "W1": np.zeros((20, 6)), "W2": np.zeros((6, 20)),
+                       "bounds": np.asarray(bounds, float).ravel(), "alpha": alpha})
+
+
+_B2 = [0.15, 0.35, 0.55, 0.75, 0.95, 1.25]
+_BOUNDS = [[-0.7, 0.4], [-0.2, 0.9], [-1.3, -0.1], [0.05, 0.6], [-0.4, 0.2], [-0.9, 1.7]]
+
+
+def test_inputs_pure_strain_closed_form():
+    """Diagonal velocity gradient G = gamma diag(1, 3, 2) with grid anisotropy A = diag(0.9, 1.1, 0.7): the isotropic
+    gradient G A is diagonal, so the strain-rate eigenvalues are its entries sorted decreasing, the vorticity vanishes,
+    |grad u| is their Euclidean norm and the sixth input is nu / delta^2 over it (a slip to nu / delta, or to A G for a
+    non-commuting pair, is caught by the shear case below)."""
+    gamma, h, delta, nu = 2.5, np.array([0.9, 1.1, 0.7]), 0.3, 1.7e-3
+    G = gamma * np.diag([1.0, 3.0, 2.0])
+    r = O.sgs_point(_zero_net(_B2, _BOUNDS), G, [*h, 0, 0, 0], delta, nu)
+    lam = np.sort(gamma * np.array([1.0, 3.0, 2.0]) * h)[::-1]
+    gmag = np.linalg.norm(lam)
+    assert r["gmag"] == pytest.approx(gmag, rel=1e-14)
+    expect = np.array([*lam, 0.0, 0.0, nu / delta**2]) / (gmag + 2.0**-52)
+    np.testing.assert_allclose(r["inputs"], expect, rtol=1e-13, atol=1e-15)
+    # eigenvector rows: e0 = +-y, e1 = +-z, e2 = +-x (sorted 3.3, 1.4, 0.9 times gamma)
+    np.testing.assert_allclose(np.abs(r["evec"]), np.eye(3)[[1, 2, 0]], atol=1e-15)
+
+
+@pytest.mark.parametrize("g", [0.8, -1.9])
+def test_inputs_simple_shear_closed_form(g):
+    """du/dy = g with A = diag(0.8, 1.25, 1): the isotropic gradient is G A, whose only entry is s = g * 1.25 (A G would
+    give g * 0.8).  Its strain rate 1/2 (G A + (G A)^T) has eigenvalues |s|/2, 0, -|s|/2; the vorticity -2 R_01 = -s lies
+    along the zero-strain eigenvector; |grad u| = sqrt(|S|^2 + |omega|^2 / 2) = |s|.  So the inputs are (1/2, 0, -1/2,
+    0, +-1, nu / (delta^2 |s|)): a missing 1/2 in the strain rate, a missing 2 in the vorticity or a weight other than 1/2 on
+    the vorticity in |grad u| changes the first three, the order of the product changes the last."""
+    h, delta, nu = np.array([0.8, 1.25, 1.0]), 0.05, 1.5e-5
+    G = np.zeros((3, 3))
+    G[0, 1] = g
+    r = O.sgs_point(_zero_net(_B2, _BOUNDS), G, [*h, 0, 0, 0], delta, nu)
+    s = abs(g * h[1])
+    assert r["gmag"] == pytest.approx(s, rel=1e-14)
+    inp = r["inputs"]
+    np.testing.assert_allclose(inp[[0, 1, 2, 3]], np.array([0.5, 0.0, -0.5, 0.0]) * s / (s + 2.0**-52), atol=1e-14)
+    assert abs(abs(inp[4]) - s / (s + 2.0**-52)) < 1e-14
+    assert inp[5] == pytest.approx(nu / delta**2 / (s + 2.0**-52), rel=1e-13)
+
+
+@pytest.mark.parametrize("omega", [(0.3, -0.7, 1.1), (-0.3, -0.7, 1.1), (0.4, 0.9, -0.6)])
+def test_inputs_rigid_rotation_closed_form(omega):
+    """u = Omega x x: the strain rate vanishes (the eigen solver leaves the identity frame, evec rows x, y, z), the
+    vorticity is curl u = 2 Omega, and OrientBasisWithVector flips e0 and e2 to align with it and sets e1 = e2 x e0.  So
+    the S-frame vorticity is (|w0|, sign(w0) sign(w2) w1, |w2|), |grad u| = |w| / sqrt 2, and the inputs are
+    (0, 0, 0, |w0|, sign(w0) sign(w2) w1, nu / delta^2) / |grad u|: a swapped or sign-flipped vorticity component fails."""
+    Om = np.asarray(omega)
+    G = np.array([[0, -Om[2], Om[1]], [Om[2], 0, -Om[0]], [-Om[1], Om[0], 0]])
+    delta, nu = 0.2, 3e-3
+    r = O.sgs_point(_zero_net(_B2, _BOUNDS), G, [1, 1, 1, 0, 0, 0], delta, nu)
+    w = 2 * Om
+    gmag = np.linalg.norm(w) / np.sqrt(2)
+    assert r["gmag"] == pytest.approx(gmag, rel=1e-14)
+    sg = lambda v: 1.0 if v >= 0 else -1.0  # noqa: E731
+    vs = [abs(w[0]), sg(w[0]) * sg(w[2]) * w[1]]
+    expect = np.array([0, 0, 0, *vs, nu / delta**2]) / (gmag + 2.0**-52)
+    np.testing.assert_allclose(r["inputs"], expect, rtol=1e-13, atol=1e-15)
+    e0, e2 = sg(w[0]) * np.array([1.0, 0, 0]), sg(w[2]) * np.array([0, 0, 1.0])
+    np.testing.assert_allclose(r["evec"], np.array([e0, np.cross(e2, e0), e2]), atol=1e-15)
+
+
+def test_outputs_denormalization_and_rotation_closed_form():
+    """With a zero-weight network the outputs are b2; ComputeSgsDDOutputs maps them to (hi - lo)(b2 - 1) + hi, scales
+    by delta^2 |grad u|^2 and rotates the S-frame tensor [[o0, o3, o4], [o3, o1, o5], [o4, o5, o2]] back with E^T S E.
+    For the diagonal gradient of the pure-strain case E's rows are +-y, +-z, +-x, so tau_yy = o0, tau_zz = o1,
+    tau_xx = o2 exactly and the shear components are +-o3 (yz), +-o4 (xy), +-o5 (xz).  delta instead of delta^2, |grad u|
+    instead of its square, a different de-normalization or E S E^T (which puts o1 at xx) all fail."""
+    gamma, delta, nu = 1.3, 0.11, 1e-3
+    G = gamma * np.diag([1.0, 3.0, 2.0])
+    r = O.sgs_point(_zero_net(_B2, _BOUNDS), G, [1, 1, 1, 0, 0, 0], delta, nu)
+    np.testing.assert_allclose(r["outputs"], _B2, rtol=0, atol=0)
+    gmag = gamma * np.sqrt(14.0)
+    lo, hi = np.array(_BOUNDS)[:, 0], np.array(_BOUNDS)[:, 1]
+    o = ((hi - lo) * (np.array(_B2) - 1) + hi) * delta**2 * gmag**2
+    tau = _km_unpack(r["km_sgs"])
+    np.testing.assert_allclose([tau[1, 1], tau[2, 2], tau[0, 0]], o[:3], rtol=1e-13)
+    np.testing.assert_allclose(np.abs([tau[1, 2], tau[0, 1], tau[0, 2]]), np.abs(o[3:]), rtol=1e-13)
+    np.testing.assert_allclose(tau, tau.T, atol=0)
+
+
+def test_closed_forms_catch_a_planted_delta_slip():
+    """The pins must bite: evaluating the de-normalized stress with delta in place of delta^2 (a plausible slip in the
+    re-dimensionalization) moves tau_xx off the closed form by the factor 1 / delta."""
+    gamma, delta = 1.3, 0.11
+    G = gamma * np.diag([1.0, 3.0, 2.0])
+    r = O.sgs_point(_zero_net(_B2, _BOUNDS), G, [1, 1, 1, 0, 0, 0], delta, 1e-3)
+    lo, hi = np.array(_BOUNDS)[2]
+    right = ((hi - lo) * (_B2[2] - 1) + hi) * delta**2 * 14 * gamma**2
+    slipped = ((hi - lo) * (_B2[2] - 1) + hi) * delta * 14 * gamma**2
+    tau_xx = _km_unpack(r["km_sgs"])[0, 0]
+    assert tau_xx == pytest.approx(right, rel=1e-13) and abs(tau_xx - slipped) > 1e-3 * abs(right)
