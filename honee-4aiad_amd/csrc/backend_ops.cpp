@@ -824,7 +824,7 @@ static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a, bool overwr
 // a scattering volume operator in two-pass mode whose restriction rows cover the whole L-vector.
 bool op_plan_overwrites(CeedOperator op, CeedVector out) {
   if (op->composite || op->ceed->scatter != 1) return false;
-  if (!op->plan && plan_bind(op)) return false;
+  if (!op->plan && (g_no_alloc || plan_bind(op))) return false;
   const OpPlan *p = op->plan;
   switch (p->kind) {
     case QfKind::IFunction:
@@ -871,6 +871,7 @@ static int sgs_model_upload(CeedOperator op, OpPlan *p, CeedMiNewtonianContext *
   packed.push_back(sc->alpha);
   if (packed.size() != sgs_model_doubles()) return MI_ERR(ceed, CEED_ERROR_BACKEND, "sgs model packing mismatch");
   if (p->d_model && packed == p->model_key) return 0;
+  if (!p->d_model && g_no_alloc) return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the SGS model is not on the device yet");
   if (!p->d_model) MI_HIP(ceed, hipMalloc((void **)&p->d_model, sizeof(double) * packed.size()));
   MI_HIP(ceed, hipMemcpyAsync(p->d_model, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice, ceed->stream));
   MI_HIP(ceed, hipStreamSynchronize(ceed->stream));  // the host buffer is released on return
@@ -1063,7 +1064,10 @@ static int sgs_field_apply(CeedOperator op, OpPlan *p, CeedVector in, CeedVector
 }
 
 int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite) {
-  if (!op->plan) MI_CALL(plan_bind(op));
+  if (!op->plan) {
+    if (g_no_alloc) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: apply the operator once with CeedOperatorApply first");
+    MI_CALL(plan_bind(op));
+  }
   OpPlan      *p    = op->plan;
   Ceed         ceed = op->ceed;
   hipStream_t  s    = ceed->stream;

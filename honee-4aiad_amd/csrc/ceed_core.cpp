@@ -12,6 +12,8 @@
 
 #include "ceed_impl.hpp"
 
+#include <mutex>
+
 // ---------------------------------------------------------------------------------------------------------------------
 // Sentinels
 // ---------------------------------------------------------------------------------------------------------------------
@@ -121,6 +123,32 @@ extern "C" int CeedResetErrorMessage(Ceed ceed, const char **err_msg) {
 // ---------------------------------------------------------------------------------------------------------------------
 // Ceed
 // ---------------------------------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------------------------------
+// Fixed scratch of the allocation-free apply (CeedMiOperatorApplyNoAlloc): the two-pass scatter's E-vector lives in this
+// file-scope device array instead of the Ceed's workspace.  Every device holds its own copy until the process ends; the
+// host uses only the address hipGetSymbolAddress returns for that device (looked up once per device in CeedInit).
+// ---------------------------------------------------------------------------------------------------------------------
+constexpr size_t kFixedScratchLen = (size_t)12 << 20;  // doubles: 96 MiB
+static __device__ __align__(256) double scratch[kFixedScratchLen];
+static std::mutex                      g_scratch_mutex;
+static std::map<int, double *>         g_scratch_ptr;  // device -> address of scratch there (null if the lookup failed)
+thread_local bool                      g_no_alloc = false;
+
+// With device c->device current
+static void fixed_scratch_setup(Ceed c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto                        it = g_scratch_ptr.find(c->device);
+  if (it == g_scratch_ptr.end()) {
+    double *ptr = nullptr;
+    if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      ptr = nullptr;
+    }
+    it = g_scratch_ptr.emplace(c->device, ptr).first;
+  }
+  c->d_fixed = it->second;
+}
+
 extern "C" int CeedInit(const char *resource, Ceed *ceed) {
   if (!resource || !ceed) return init_error(CEED_ERROR_MAJOR, "CeedInit: null resource or output pointer");
   std::string res(resource);
@@ -162,6 +190,7 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
   c->device   = device;
   c->jac_fp32 = jac_fp32;
   c->scatter  = scatter;
+  fixed_scratch_setup(c);
   init_err_msg.clear();
   *ceed = c;
   return CEED_ERROR_SUCCESS;
@@ -247,7 +276,7 @@ static hipEvent_t event_get(Ceed ceed) {
 
 int timing_begin(Ceed ceed, const char *name, hipEvent_t *start) {
   *start = nullptr;
-  if (!ceed->timing) return 0;
+  if (!ceed->timing || g_no_alloc) return 0;  // (events come from a pool that grows: the allocation-free apply is untimed)
   *start = event_get(ceed);
   MI_HIP(ceed, hipEventRecord(*start, ceed->stream));
   return 0;
@@ -364,6 +393,7 @@ extern "C" int CeedVectorReferenceCopy(CeedVector vec, CeedVector *vec_copy) {
 
 static int vec_alloc_device(CeedVector v) {
   if (!v->d_array) {
+    if (g_no_alloc) return MI_ERR(v->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: a vector has no device array (length %lld)", (long long)v->length);
     MI_HIP(v->ceed, hipMalloc((void **)&v->d_array, sizeof(double) * std::max<CeedSize>(v->length, 1)));
     v->d_owned = true;
   }
@@ -372,6 +402,7 @@ static int vec_alloc_device(CeedVector v) {
 
 static int vec_alloc_host(CeedVector v) {
   if (!v->h_array) {
+    if (g_no_alloc) return MI_ERR(v->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: a vector has no host array");
     v->h_array = (double *)malloc(sizeof(double) * std::max<CeedSize>(v->length, 1));
     if (!v->h_array) return MI_ERR(v->ceed, CEED_ERROR_MAJOR, "host allocation failed");
     v->h_owned = true;
@@ -426,6 +457,7 @@ int vec_device_write(CeedVector v, double **d, bool keep_contents) {
 int vec_device_shadow32(CeedVector v, float **d, bool create) {
   if (!v->d_shadow32) {
     if (!create) return MI_ERR(v->ceed, CEED_ERROR_INCOMPLETE, "vector has no FP32 shadow");
+    if (g_no_alloc) return MI_ERR(v->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the FP32 jac_data shadow does not exist yet");
     MI_HIP(v->ceed, hipMalloc((void **)&v->d_shadow32, sizeof(float) * std::max<CeedSize>(v->length, 1)));
     v->shadow_len = v->length;
   }
@@ -748,6 +780,7 @@ extern "C" int CeedElemRestrictionCreateStrided(Ceed ceed, CeedInt num_elem, Cee
 int rstr_device_offsets(CeedElemRestriction r, const int **d) {
   if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no offsets");
   if (!r->d_offsets) {
+    if (g_no_alloc) return MI_ERR(r->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: restriction offsets are not on the device yet");
     MI_HIP(r->ceed, hipMalloc((void **)&r->d_offsets, sizeof(int) * std::max<size_t>(r->h_offsets.size(), 1)));
     MI_HIP(r->ceed, hipMemcpy(r->d_offsets, r->h_offsets.data(), sizeof(int) * r->h_offsets.size(), hipMemcpyHostToDevice));
   }
@@ -761,6 +794,7 @@ int rstr_device_offsets(CeedElemRestriction r, const int **d) {
 int rstr_transpose_map(CeedElemRestriction r) {
   if (r->d_t_rowptr) return 0;
   if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no transpose map");
+  if (g_no_alloc) return MI_ERR(r->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the scatter's transpose map is not built yet");
   const size_t                         n = r->h_offsets.size();
   std::vector<std::pair<CeedInt, int>> pairs(n);
   for (size_t i = 0; i < n; i++) pairs[i] = {r->h_offsets[i], (int)i};
@@ -792,6 +826,14 @@ int rstr_transpose_map(CeedElemRestriction r) {
 
 // Device workspace of at least len doubles, owned by the Ceed and reused by every operator apply on its stream.
 int ceed_workspace(Ceed ceed, size_t len, double **d) {
+  if (g_no_alloc) {
+    if (!ceed->d_fixed) return MI_ERR(ceed, CEED_ERROR_BACKEND, "allocation-free apply: the fixed scratch has no address on device %d", ceed->device);
+    if (len > kFixedScratchLen)
+      return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the E-vector needs %zu doubles, the fixed scratch holds %zu (%zu MiB)", len,
+                    kFixedScratchLen, kFixedScratchLen * sizeof(double) >> 20);
+    *d = ceed->d_fixed;
+    return 0;
+  }
   if (ceed->work_len < len) {
     if (ceed->d_work) {
       MI_HIP(ceed, hipStreamSynchronize(ceed->stream));
@@ -1647,6 +1689,16 @@ extern "C" int CeedOperatorApply(CeedOperator op, CeedVector in, CeedVector out,
     }
   }
   return apply_add(op, in, out, overwrite);
+}
+
+// CeedOperatorApply without any allocation: the operator must have been applied once (its plan, device offsets and
+// transpose maps exist) and its vectors must hold device arrays; the two-pass scatter's E-vector is the fixed scratch.
+extern "C" int CeedMiOperatorApplyNoAlloc(CeedOperator op, CeedVector in, CeedVector out) {
+  if (!op) return init_error(CEED_ERROR_MAJOR, "CeedMiOperatorApplyNoAlloc: null operator");
+  g_no_alloc     = true;
+  const int ierr = CeedOperatorApply(op, in, out, CEED_REQUEST_IMMEDIATE);
+  g_no_alloc     = false;
+  return ierr;
 }
 
 extern "C" int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {

@@ -36,7 +36,11 @@ struct Ceed_private {
   double                                  *d_scratch = nullptr;  // reduction scratch (norms)
   double                                  *h_scratch = nullptr;
   int                                      scratch_len = 0;
+  double *d_fixed = nullptr;  // this device's copy of the fixed E-vector scratch (CeedMiOperatorApplyNoAlloc), or null
 };
+
+// True on this thread while CeedMiOperatorApplyNoAlloc runs: every helper that would allocate returns an error instead
+extern thread_local bool g_no_alloc;
 
 struct CeedVector_private {
   Ceed     ceed;

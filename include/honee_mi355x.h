@@ -129,6 +129,15 @@ int CeedMiVectorDot(CeedVector x, CeedVector y, double *result);
 // Synchronize the backend stream.
 int CeedMiSynchronize(Ceed ceed);
 
+// CeedOperatorApply that allocates nothing.  It has no counterpart in libCEED or HONEE: it is for callers that cannot
+// allocate device memory, such as code under HIP graph capture or a tight serving loop.  The operator must have been
+// applied once with CeedOperatorApply (which binds it and uploads its offsets and scatter maps), and in and out must
+// hold device arrays.  The two-pass scatter's E-vector then lives in a fixed device array of 96 MiB
+// (12 Mi doubles: elements x nodes per element x 5 must fit, about 39,000 p=3 elements); larger operators, and any
+// step that would allocate, return an error instead.  Every call on one GPU shares that array and it keeps the last
+// call's contents: calls that overlap on one GPU (other streams or threads) race.  Kernel timing is skipped.
+int CeedMiOperatorApplyNoAlloc(CeedOperator op, CeedVector in, CeedVector out);
+
 #ifdef __cplusplus
 }
 #endif
