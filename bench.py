@@ -25,11 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "honee-4aiad_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes of one IFunction apply at the default workload (fused kernel 598.2 MB + node gather 133.1 MB, two-pass
+# HBM bytes of one IFunction apply at the default workload (fused kernel 551.9 MB + node gather 139.2 MB, two-pass
 # scatter), from two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0, the factors the counters show for
 # the known-traffic kernels of --calibrate-traffic); the counts and commands are in
-# profiles/ifunction_newtonian_tgv32_p3_traffic.md.  Other workloads report traffic null.
-PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 731.3e6}
+# profiles/ifunction_newtonian_tgv32_p3_traffic.md.  A stored figure, not measured by the run that reports it (the
+# line says so); other workloads report traffic null.
+PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 691.1e6}
 METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
 
 
@@ -413,7 +414,10 @@ def main():
         "ifunction_gdofs": dof / t_if / 1e9,
         "ijacobian_gdofs": dof / t_ij / 1e9,
         "roofline": {"bound": "hbm", "kernel": "ifunction_newtonian", "achieved": b_if / t_if / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((shape, degree, args.scatter)), "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
+                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((shape, degree, args.scatter)),
+                     "traffic_source": "stored: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
+                                       "(profiles/ifunction_newtonian_tgv32_p3_traffic.md), not measured by this run",
+                     "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
                      "avg_launch_us": t_if_k * 1e6, "avg_node_gather_us": t_gather * 1e6, "scatter": args.scatter,
                      "timing": "avg_apply_us: back-to-back applies between two HIP events on the backend stream (fused kernel + node "
                                "gather); avg_launch_us / avg_node_gather_us: per-kernel events"},

@@ -796,8 +796,10 @@ static int face_contexts(CeedOperator op, int ctx_type, FaceContexts *fc) {
 // (deterministic two-pass) writes the element-node contributions to the Ceed's E-vector workspace, and
 // scatter_finish sums them per node in a fixed order with launch_node_gather.
 // E-vector layout of the two-pass scatter: node-sorted (the fused kernel writes each element node to its slot in
-// node order; the gather streams) or element-ordered (coalesced fused-kernel writes; the gather reads by index).
-constexpr bool kSortedEvec = true;
+// node order; the gather streams) or element-ordered (coalesced fused-kernel writes; the gather reads each node's
+// contributions by index, whole).  Measured at 32^3 p=3 (IFunction apply): element-ordered 176 us (fused kernel
+// 136 us + gather 50 us, per-kernel events), node-sorted 197 us (166 + 30).
+constexpr bool kSortedEvec = false;
 
 static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
   a.ev   = nullptr;
@@ -1107,7 +1109,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       a.qd = p->qd;
       a.jd = p->jd;
       const int PQ = p->P * p->Q;
-      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+      a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ, p->b_q->d_grad_1d};
       if (p->kind == QfKind::IFunction) {
         const int need = 14 + (ctx.idl_enable ? 1 : 0);
         if (p->v_jac && p->njac < need) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "jac_data needs %d components", need);
@@ -1150,7 +1152,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
       a.qd         = p->qd;
       const int PQ = p->P * p->Q;
-      a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+      a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ, p->b_q->d_grad_1d};
       if (p->kind == QfKind::MonitorKE) {
         if (out->length < p->r_jd->l_size) return MI_ERR(ceed, CEED_ERROR_DIMENSION, "monitor output too short");
         MI_CALL(vec_device_write(out, &a.jac, true));

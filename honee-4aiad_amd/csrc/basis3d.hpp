@@ -437,10 +437,17 @@ struct Stage {
   double (*X)[Q3];
   int t, qx, qy, qz;
 
-  __device__ Stage(const double *B_, const double *Dc_, double (*X_)[Q3], int t_) : B(B_), Dc(Dc_), X(X_), t(t_) {
+  __device__ Stage(const double *B_, const double *Dc_, const double *, double (*X_)[Q3], int t_) : B(B_), Dc(Dc_), X(X_), t(t_) {
     qx = t % Q;
     qy = (t / Q) % Q;
     qz = t / (Q * Q);
+  }
+
+  // interp() followed by grad(dst, gdst)
+  template <int NC>
+  __device__ void interp_grad(const double (&u)[NC], int s0, int s1, int dst, int gdst) {
+    interp<NC>(u, s0, s1, dst);
+    grad<NC>(dst, gdst);
   }
 
   // Interpolate NC components from node values u[c] (threads t < P^3) to this thread's quadrature point, leaving the
@@ -767,6 +774,92 @@ struct Stage {
       }
     }
     __syncthreads();
+  }
+};
+
+// FP64 matrix-core passes for the 4 x 4 x 4 element.  v_mfma_f64_4x4x4_4b on one 64-lane wave: lane L = 4 b + 16 r + c
+// (block b = (L >> 2) & 3); operand A holds A[i][k] at (c = i, r = k), operand B holds B[k][j] at (r = k, c = j), and the
+// result D[i][j] + C lands at (r = i, c = j): the sum runs over r.  In the element layout L = qx + 4 qy + 16 qz (c = qx,
+// b = qy, r = qz):
+//   mf(M, u, acc) with the coefficients as A: w(qz') = sum_qz M[qz'][qz] u(qz) + acc, in the same layout (a z pass);
+//   mf(u, M, acc) with the data as A: sum over qz with M[qz][j], the result's r taking the old qx and its c the new j:
+//                 x and z trade places, so two such passes contract z and then x and return to the element layout.
+// A 4 x 4 table T (row-major) is held per lane as cr(T) = T[c][r] or rc(T) = T[r][c].  The y passes (across the four
+// blocks) stay DPP row rotations.  No pass reads LDS or needs a barrier; X only parks values across the point body.
+__device__ __forceinline__ double mfma44(double a, double b, double c) { return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0); }
+
+struct StageMF {
+  static constexpr int Q3 = 64;
+  const double *B;   // [4][4] interpolation B[q][n]
+  const double *Dc;  // [4][4] collocated derivative
+  const double *G;   // [4][4] derivative of the interpolation G[q][n]
+  double (*X)[Q3];
+  int t, qy;
+
+  __device__ StageMF(const double *B_, const double *Dc_, const double *G_, double (*X_)[Q3], int t_) : B(B_), Dc(Dc_), G(G_), X(X_), t(t_) {
+    qy = (t >> 2) & 3;
+  }
+  __device__ double cr(const double *T) const { return T[(t & 3) * 4 + (t >> 4)]; }
+  __device__ double rc(const double *T) const { return T[(t >> 4) * 4 + (t & 3)]; }
+
+  // Node values u[c] (lane = node a + 4 b + 16 c) -> quadrature-point values in slots dst..
+  template <int NC>
+  __device__ void interp(const double (&u)[NC], int, int, int dst) {
+    const RowRot rr(qy);
+    double       by[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) by[i] = B[qy * 4 + rr.y[i]];
+    const double bcr = cr(B);
+#pragma unroll
+    for (int c = 0; c < NC; c++) X[dst + c][t] = mfma44(mfma44(row_contract(by, u[c]), bcr, 0.0), bcr, 0.0);
+  }
+
+  // Values (slots dst..) and reference gradients (slots gdst + NC d + c): d/dX0 from the z-contracted intermediate with
+  // G (G_x B_y B_z), d/dX1 and d/dX2 collocated from the values (Dc_y, Dc_z)
+  template <int NC>
+  __device__ void interp_grad(const double (&u)[NC], int, int, int dst, int gdst) {
+    const RowRot rr(qy);
+    double       by[4], dy[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      by[i] = B[qy * 4 + rr.y[i]];
+      dy[i] = Dc[qy * 4 + rr.y[i]];
+    }
+    const double bcr = cr(B), gcr = cr(G), dcr = cr(Dc);
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const double s   = mfma44(row_contract(by, u[c]), bcr, 0.0);  // y, then z (x and z traded)
+      const double val = mfma44(s, bcr, 0.0);                       // x: element layout
+      X[dst + c][t]             = val;
+      X[gdst + c][t]            = mfma44(s, gcr, 0.0);
+      X[gdst + NC + c][t]       = row_contract(dy, val);
+      X[gdst + 2 * NC + c][t]   = mfma44(dcr, val, 0.0);
+    }
+  }
+
+  // Transpose: node contributions r[c] = B3^T v_c + sum_d G3_d^T g_{d,c} from slots v0.. and g0 + NC d + c, in the
+  // collocated form w = v + sum_d Dc_d^T g_d, then B^T along y (DPP), z and x (matrix passes)
+  template <int NC, bool HAS_V, bool HAS_G>
+  __device__ void transpose(int v0, int g0, double (&r)[NC]) {
+    const RowRot rr(qy);
+    double       byT[4], dyT[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      byT[i] = B[rr.y[i] * 4 + qy];
+      dyT[i] = Dc[rr.y[i] * 4 + qy];
+    }
+    const double brc = rc(B), drc = rc(Dc), eye = ((t >> 4) == (t & 3)) ? 1.0 : 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      double w = HAS_V ? X[v0 + c][t] : 0.0;
+      if (HAS_G) {
+        w                = mfma44(drc, X[g0 + 2 * NC + c][t], w);         // + Dc_z^T g_z
+        w               += row_contract(dyT, X[g0 + NC + c][t]);          // + Dc_y^T g_y
+        const double gxf = mfma44(X[g0 + c][t], eye, 0.0);                // g_x with x and z traded
+        w                = mfma44(mfma44(drc, gxf, 0.0), eye, w);         // + Dc_x^T g_x, traded back
+      }
+      r[c] = mfma44(mfma44(row_contract(byT, w), brc, 0.0), brc, 0.0);  // B^T along y, then z and x
+    }
   }
 };
 

@@ -15,6 +15,7 @@ struct BasisTables {
   const double *B;
   const double *Dc;
   const double *BB, *GB, *GG;  // elementwise products B.B, G.B, G.G of the 1D tables (diagonal assembly)
+  const double *G = nullptr;   // derivative of the interpolation [Q][P] (null: formed as Dc B)
 };
 
 struct SetupArgs {

@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_field(
     const bool      ok    = e_ < a.nelem;
     const int       e     = ok ? (int)e_ : 0;
     const long long enode = (long long)e * Dm::P3 + t;
-    basis3::Stage<P, Q> st(sB, sDc, X, t);
+    basis3::Stage<P, Q> st(sB, sDc, nullptr, X, t);
     if constexpr (NU > 0) {
       double un[NUX];
       if (ok && nd) {

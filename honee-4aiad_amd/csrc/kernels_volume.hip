@@ -47,6 +47,18 @@ __device__ __forceinline__ BatchRange batch_range(long long nbatch) {
   return {lo + j, hi, G / kNumXcd};
 }
 
+// The basis stage of the fused kernel: for the 4 x 4 x 4 element the x and z passes run on the FP64 matrix cores and the
+// y passes across lanes (basis3::StageMF), except for ops with Op::kZCrossLane (the FP32 Jacobian, measured faster with
+// all passes across lanes); otherwise sum factorization through LDS (basis3::Stage, ZX: z passes across lanes)
+template <int P, int Q, bool ZX>
+struct StageFor {
+  using type = basis3::Stage<P, Q, ZX>;
+};
+template <>
+struct StageFor<4, 4, false> {
+  using type = basis3::StageMF;
+};
+
 // Op::kZCrossLane selects the basis stage whose z passes also run across lanes (basis3::Stage's ZX); default off
 template <class Op, class = void>
 struct ZCross {
@@ -61,6 +73,21 @@ template <int P, int Q>
 __device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc) {
   for (int i = threadIdx.x; i < Q * P; i += blockDim.x) sB[i] = bt.B[i];
   for (int i = threadIdx.x; i < Q * Q; i += blockDim.x) sDc[i] = bt.Dc[i];
+}
+// with the derivative table G [Q][P] (formed as Dc B when the basis does not provide it)
+template <int P, int Q>
+__device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc, double *sG) {
+  load_tables<P, Q>(bt, sB, sDc);
+  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
+    if (bt.G) {
+      sG[i] = bt.G[i];
+    } else {
+      const int q = i / P, n = i % P;
+      double    g = 0;
+      for (int m = 0; m < Q; m++) g += bt.Dc[q * Q + m] * bt.B[m * P + n];
+      sG[i] = g;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -172,13 +199,14 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   constexpr int SU2  = Op::kGrad ? 0 : 10;               // scratch of the u2 interpolation
   constexpr int SU   = (Op::kIn2 && !Op::kGrad) ? 10 : 5;  // scratch of the u interpolation
   static_assert(!Op::kIn2 || Op::kGrad || NST >= 20, "stage too small for the u2 scratch");
-  __shared__ double                 sB[Q * P], sDc[Q * Q];
+  __shared__ double                 sB[Q * P], sDc[Q * Q], sG[Q * P];
   __shared__ double                 sX[EPB][NST][Q3];
   // quadrature data staged through LDS by DMA (Op::kStageQData) or read from global memory at the point body
   constexpr bool                    SQ = Op::kStageQData;
   __shared__ __align__(16) double   sQD[SQ ? EPB : 1][10][SQ ? Q3 : 1];
   static_assert(Dm::P3 <= Q3, "P <= Q is required");
-  load_tables<P, Q>(a.bt, sB, sDc);
+  using Stage = typename StageFor<P, Q, ZCross<Op>::value>::type;
+  load_tables<P, Q>(a.bt, sB, sDc, sG);
   const int           le = threadIdx.x / Q3, t = threadIdx.x % Q3;
   const bool          nd = t < Dm::P3;
   double(*X)[Q3]         = sX[le];
@@ -216,10 +244,10 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
     // quadrature-point body.
     int tb = t;
     asm volatile("" : "+v"(tb));
-    basis3::Stage<P, Q, ZCross<Op>::value> st(sB, sDc, X, tb);
+    Stage st(sB, sDc, sG, X, tb);
     if constexpr (Op::kIn2) st.template interp<5>(u2n, SU2, SU2 + 5, U2);
-    st.template interp<5>(un, SU, SU + 5, 0);
-    if constexpr (Op::kGrad) st.template grad<5>(0, 5);
+    if constexpr (Op::kGrad) st.template interp_grad<5>(un, SU, SU + 5, 0, 5);
+    else st.template interp<5>(un, SU, SU + 5, 0);
     int off_next = 0;
     {
       double qd[10], u[5], gu[3][5], u2[5], v[5], gw[3][5];
@@ -272,7 +300,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
     if constexpr (Op::kScatter) {
       int tt = t;
       asm volatile("" : "+v"(tt));
-      basis3::Stage<P, Q, ZCross<Op>::value> sto(sB, sDc, X, tt);
+      Stage               sto(sB, sDc, sG, X, tt);
       double              r[5];
       sto.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {

@@ -107,11 +107,12 @@ struct IFunction {
         gref[0][c] = gref[1][c] = gref[2][c] = 0;
       }
     }
-    // jac_data (newtonian.h:251-254), each value written as soon as it is known so that it is not held in registers
+    // jac_data (newtonian.h:251-254), each value written as soon as it is known so that it is not held in registers;
+    // streamed past the caches (nontemporal), measured faster: nothing in this apply reads it back
     auto put = [&](int j, double val) {
       if (ok) {
-        if (a.jac32) a.jac32[qidx<Q>(a.jd, e, j, t)] = (float)val;
-        else if (a.jac) a.jac[qidx<Q>(a.jd, e, j, t)] = val;
+        if (a.jac32) __builtin_nontemporal_store((float)val, &a.jac32[qidx<Q>(a.jd, e, j, t)]);
+        else if (a.jac) __builtin_nontemporal_store(val, &a.jac[qidx<Q>(a.jd, e, j, t)]);
       }
     };
     const ns::St<double> s = ns::state_from_q<VAR>(g, q);
