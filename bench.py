@@ -249,6 +249,88 @@ def _apply_times(prob, q, qdot, g, dq, v, dof, steps, warmup, diag=None):
     return out
 
 
+def c5_scale(steps=10, ts_steps=2, warm_steps=1):
+    """The scale configuration's residual and Jacobian: TGV at Re 1600 on 96^3 periodic hexahedra, p=4, Q=5
+    (56,623,104 nodes, 283,115,520 DoF), primitive FP64 state with seeded velocity noise 1e-3 U0, SUPG.  Steady-state
+    IFunction and IJacobian (FP64 and FP32 Jacobian) applies in GDoF/s with the IFunction's roofline fraction, then
+    BDF2 (dt 1e-3) with the FP32 Jacobian inside GMRES(30) and point-block Jacobi: seconds per step after a warm-up
+    step, iteration counts, and property checks (every Newton solve converged, the state finite, the kinetic energy
+    not increasing)."""
+    import torch
+
+    from pyhonee import mesh as M
+    from pyhonee import problems as PR
+    from pyhonee import ts as TS
+    from pyhonee.driver import CEED_MEM_DEVICE, NewtonianProblem, context_from_dict
+
+    shape, degree, dt = (96, 96, 96), 4, 1e-3
+    t0 = time.perf_counter()
+    mesh = M.box_mesh(shape, degree)
+    tab = M.tabulate(degree, 0)
+    rng = np.random.default_rng(1600)
+    Y = PR.nodal_field(mesh, lambda x: PR.taylor_green_prim(x), average=False)
+    Y[:, 1:4] += rng.uniform(-1e-3, 1e-3, (mesh.nnodes, 3))
+    dof = Y.size
+    print(f"# C5 mesh and state: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    d = PR.tgv_context_dict(state_var=1, stabilization=2, dt=dt, tau=PR.default_tau(degree))
+    dev = torch.device("cuda:0")
+    q = torch.tensor(Y.ravel(), dtype=torch.float64, device=dev)
+    qdot = torch.empty_like(q).uniform_(-1e-3, 1e-3, generator=torch.Generator(device=dev).manual_seed(96))
+    dq = torch.empty_like(q).uniform_(-1e-3, 1e-3, generator=torch.Generator(device=dev).manual_seed(97))
+    g, v = torch.zeros_like(q), torch.zeros_like(q)
+    out = {"config": f"TGV 96^3 periodic hex p=4 Q=5, primitive FP64, SUPG, dt {dt}", "dof": dof, "elements": mesh.nelem}
+    b_if, b_ij = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes)
+    _, b32 = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes, jac_bytes=4)
+    shift = 1.0 / dt
+    for label, resource in (("fp64", "/gpu/hip/mi355x"), ("fp32_jacobian", "/gpu/hip/mi355x:jacobian_precision=fp32")):
+        print(f"# C5 {label} applies", file=sys.stderr, flush=True)
+        prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1, resource)
+        r = {}
+        for name, fn in (("ifunction", lambda: prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)),
+                         ("ijacobian", lambda: prob.ijacobian(dq, v, shift, mem=CEED_MEM_DEVICE))):
+            for _ in range(2):
+                fn()
+            prob.synchronize()
+            prob.reset_times()
+            prob.region_begin()
+            for _ in range(steps):
+                fn()
+            prob.region_end(name)
+            r[name] = prob.kernel_times()[name][1] / steps * 1e-3
+        if label == "fp64":
+            out["ifunction_gdofs"] = dof / r["ifunction"] / 1e9
+            out["ifunction_roofline_frac"] = b_if / r["ifunction"] / 1e9 / HBM_PEAK_GBS
+            out["ifunction_apply_ms"] = r["ifunction"] * 1e3
+            out["ijacobian_gdofs"] = dof / r["ijacobian"] / 1e9
+            out["ijacobian_roofline_frac"] = b_ij / r["ijacobian"] / 1e9 / HBM_PEAK_GBS
+            prob.close()
+            continue
+        out["ijacobian_fp32_gdofs"] = dof / r["ijacobian"] / 1e9
+        out["ijacobian_fp32_roofline_frac"] = b32 / r["ijacobian"] / 1e9 / HBM_PEAK_GBS
+        if ts_steps > 0:
+            print("# C5 BDF2 with the FP32 Jacobian", file=sys.stderr, flush=True)
+            ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_BDF, bdf_order=2, dt=dt, ksp_pc=TS.PC_PBJACOBI))
+            ts.set_solution(Y.ravel(), 0.0)
+            ke0 = float(ts.monitor_ke()[0])
+            ts.step(warm_steps)
+            s0 = ts.stats()
+            t1 = time.perf_counter()
+            ts.step(ts_steps)
+            el = time.perf_counter() - t1
+            s1 = ts.stats()
+            ke1 = float(ts.monitor_ke()[0])
+            qf = ts.solution()
+            out["bdf2_fp32_jacobian"] = {
+                "s_per_step": el / ts_steps, "newton_its_per_step": (s1["newton_its"] - s0["newton_its"]) / ts_steps,
+                "krylov_its_per_step": (s1["krylov_its"] - s0["krylov_its"]) / ts_steps,
+                "newton_converged": bool(s1["last_fnorm"] <= 1e-8 * s1["last_fnorm0"] or s1["last_fnorm"] < 1e-8),
+                "last_fnorm": s1["last_fnorm"], "last_fnorm0": s1["last_fnorm0"], "finite": bool(np.isfinite(qf).all()),
+                "kinetic_energy_ratio": ke1 / ke0, "steps": warm_steps + ts_steps}
+            ts.close()
+        prob.close()
+    return out
+
+
 def les_study():
     """The learned closure a posteriori: decaying isotropic turbulence (E(k) ~ k^4 exp(-2 (k/4)^2), u_rms 1, Mach 0.1),
     plain fine run 96^3 p=2 (SSP-RK3) differentially filtered to the coarse element size, network trained on the GPU
@@ -284,6 +366,7 @@ def main():
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
     ap.add_argument("--no-les", action="store_true", help="skip the a-posteriori LES study of the learned closure")
     ap.add_argument("--no-faces", action="store_true", help="skip the channel applies with boundary faces")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 96^3 p=4 scale-configuration applies and BDF2 steps")
     ap.add_argument("--calibrate-traffic", action="store_true",
                     help="only run kernels of known HBM traffic (3 reductions reading 1 GiB, 3 fills writing 1 GiB) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc")
@@ -444,6 +527,8 @@ def main():
     if not args.no_faces:
         print("# channel applies with faces", file=sys.stderr, flush=True)
         line["faces_c3"] = faces_c3()
+    if not args.no_c5:
+        line["c5_scale"] = c5_scale()
     if not args.no_les:
         line["les"] = les_study()
     if not args.no_cpu_baseline:

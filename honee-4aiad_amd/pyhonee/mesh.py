@@ -106,12 +106,14 @@ class BoxMesh:
     def P(self) -> int:
         return self.degree + 1
 
-    def element_node_coords(self, tab: Tabulation | None = None) -> np.ndarray:
-        """Physical coordinates of every element's nodes, [nelem][P^3][3] (trilinear map at GLL nodes)."""
+    def element_node_coords(self, tab: Tabulation | None = None, fast: bool = False) -> np.ndarray:
+        """Physical coordinates of every element's nodes, [nelem][P^3][3] (trilinear map at GLL nodes).  fast: the
+        contraction in pairwise steps (for large meshes; the sums run in another order, so values differ in the last
+        bits)."""
         nodes = gll_nodes(self.P)
         Bn, _ = lagrange_tabulate(np.array([-1.0, 1.0]), nodes)  # [P][2]
         xe = self.xcoord.reshape(self.nelem, 2, 2, 2, 3)  # [e][z][y][x][c]
-        out = np.einsum("ka,jb,ic,eabcd->ekjid", Bn, Bn, Bn, xe)
+        out = np.einsum("ka,jb,ic,eabcd->ekjid", Bn, Bn, Bn, xe, optimize=fast)
         return out.reshape(self.nelem, self.P**3, 3)
 
     def multiplicity(self) -> np.ndarray:

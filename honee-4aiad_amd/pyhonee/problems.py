@@ -84,9 +84,16 @@ def taylor_green_prim(xyz, ref_Y=TGV_REFERENCE_Y, gas=TGV_GAS, background=(0.0, 
     return Y
 
 
-def nodal_field(mesh: BoxMesh, per_node_fn) -> np.ndarray:
+def nodal_field(mesh: BoxMesh, per_node_fn, average: bool = True) -> np.ndarray:
     """Evaluate a pointwise function at every element's nodes and average over node multiplicity, as HONEE's
-    ICs_FixMultiplicity does (src/misc.c:16-48).  Returns an L-vector [nnodes][5]."""
+    ICs_FixMultiplicity does (src/misc.c:16-48).  Returns an L-vector [nnodes][5].  average=False keeps one element's
+    value per node instead (the same for a continuous, periodic function up to rounding; for meshes too large for the
+    averaging's unbuffered accumulation)."""
+    if not average:  # one coordinate per node (a periodic image's on shared periodic nodes), evaluated once
+        xyz = mesh.element_node_coords(fast=True)
+        xn = np.empty((mesh.nnodes, 3))
+        xn[mesh.offsets.ravel()] = xyz.reshape(-1, 3)
+        return per_node_fn(xn).reshape(mesh.nnodes, 5)
     xyz = mesh.element_node_coords()
     vals = per_node_fn(xyz.reshape(-1, 3)).reshape(mesh.nelem, -1, 5)
     q = np.zeros((mesh.nnodes, 5))
