@@ -2,7 +2,9 @@
 
 Layout, all big-endian: an optional HONEE header ``int32 token`` (0xceedf32 -> int32 step, 0xceedf64 -> int64 step,
 0xceedf00 legacy) followed by ``int step`` and ``float64 time``; then PETSc's ``VecView`` binary record
-``int32 VEC_FILE_CLASSID (1211214)``, ``int32 n`` and ``n`` float64 values.  A file may also be a bare Vec record.
+``PetscInt VEC_FILE_CLASSID (1211214)``, ``PetscInt n`` and ``n`` float64 values, where PetscInt is int32 or, from a
+PETSc built with 64-bit indices, int64 (VecLoad_Binary tells them apart the same way: an int32 0 followed by the
+class id).  A file may also be a bare Vec record.
 """
 from __future__ import annotations
 
@@ -22,23 +24,33 @@ def read_vec(path: str) -> tuple[np.ndarray, int, float]:
         step = int(np.frombuffer(buf[4:4 + isz], ">i8" if isz == 8 else ">i4")[0])
         time = float(np.frombuffer(buf[4 + isz:12 + isz], ">f8")[0])
         off = 12 + isz
-    cid, n = (int(v) for v in np.frombuffer(buf[off:off + 8], ">i4"))
+    w0, w1 = (int(v) for v in np.frombuffer(buf[off:off + 8], ">i4"))
+    if w0 == 0 and w1 == VEC_FILE_CLASSID:  # 64-bit PetscInt header
+        cid, n = (int(v) for v in np.frombuffer(buf[off:off + 16], ">i8"))
+        off += 16
+    else:
+        cid, n = w0, w1
+        off += 8
     if cid != VEC_FILE_CLASSID:
         raise ValueError(f"{path}: not a PETSc Vec record (class id {cid})")
-    vals = np.frombuffer(buf[off + 8:off + 8 + 8 * n], ">f8").astype(np.float64)
+    vals = np.frombuffer(buf[off:off + 8 * n], ">f8").astype(np.float64)
     if vals.size != n:
         raise ValueError(f"{path}: truncated Vec ({vals.size} of {n} values)")
     return vals, step, time
 
 
 def write_vec(path: str, values: np.ndarray, step: int = 0, time: float = 0.0, index_bits: int = 32) -> None:
-    """Write a HONEE-formatted file (HoneeWriteBinaryVec)."""
+    """Write a HONEE-formatted file (HoneeWriteBinaryVec) as a PETSc with index_bits-wide PetscInt writes it."""
     v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    it = ">i8" if index_bits == 64 else ">i4"
     with open(path, "wb") as f:
-        if index_bits == 64:
-            f.write(np.array([TOKEN_64], ">i4").tobytes() + np.array([step], ">i8").tobytes())
-        else:
-            f.write(np.array([TOKEN_32, step], ">i4").tobytes())
+        f.write(np.array([TOKEN_64 if index_bits == 64 else TOKEN_32], ">i4").tobytes() + np.array([step], it).tobytes())
         f.write(np.array([time], ">f8").tobytes())
-        f.write(np.array([VEC_FILE_CLASSID, v.size], ">i4").tobytes())
+        f.write(np.array([VEC_FILE_CLASSID, v.size], it).tobytes())
         f.write(v.astype(">f8").tobytes())
+
+
+def index_bits(path: str) -> int:
+    """PetscInt width (32 or 64) of a HONEE-formatted file's header."""
+    tok = int(np.frombuffer(open(path, "rb").read(4), ">i4")[0])
+    return 64 if tok == TOKEN_64 else 32

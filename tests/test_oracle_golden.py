@@ -108,12 +108,13 @@ def test_product_ic_matches_oracle_ic():
         np.testing.assert_allclose(qp, qo, rtol=1e-13, atol=1e-13)
 
 
-@pytest.mark.parametrize("name", ["taylor-green.bin", "taylor-green-IC.bin"])
+@pytest.mark.parametrize("name", ["taylor-green.bin", "taylor-green-IC.bin", "gaussianwave-explicit.bin"])
 def test_writer_round_trips_reference_bin(name, tmp_path):
     """HoneeWriteBinaryVec's layout (src/honee-file.c:196-217): writing back what the reader returns reproduces the
-    reference's file byte for byte (HONEE token, int32 step, float64 time, PETSc Vec record)."""
+    reference's file byte for byte (HONEE token, PetscInt step, float64 time, PETSc Vec record; the gaussian-wave file
+    comes from a PETSc with 64-bit PetscInt, the Taylor-Green files from one with 32-bit)."""
     path = os.path.join(GOLDEN, name)
     vals, step, t = honee_file.read_vec(path)
     out = tmp_path / name
-    honee_file.write_vec(str(out), vals, step=step, time=t)
+    honee_file.write_vec(str(out), vals, step=step, time=t, index_bits=honee_file.index_bits(path))
     assert out.read_bytes() == open(path, "rb").read()
