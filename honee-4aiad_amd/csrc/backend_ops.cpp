@@ -120,7 +120,7 @@ static int bind_solution_basis(CeedOperator op, OpPlan *p, CeedOperatorField_pri
   return 0;
 }
 
-// Face solution field: 5-component field on the boundary faces with a 2D tensor basis (P_1d = Q_1d)
+// Face solution field: 5-component field on the boundary faces with a 2D tensor basis (Q_1d = P_1d + q_extra)
 static int bind_face_basis(CeedOperator op, OpPlan *p, CeedOperatorField_private *f) {
   BIND_CHECK(is_offsets(f->rstr) && is_tensor2(f->basis), "face field '%s' needs an offset restriction and a 2D tensor basis", f->name.c_str());
   BIND_CHECK(f->rstr->num_comp == 5 && f->basis->num_comp == 5, "face field '%s' must have 5 components", f->name.c_str());
@@ -679,7 +679,7 @@ static int plan_bind_fields(CeedOperator op) {
       p->nelem = fx->rstr->num_elem;
       p->r_qd  = fq->rstr;
       p->qd    = strided_layout(fq->rstr);
-      if (!face_pq_supported(p->Q, p->Q)) return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "SetupBoundary: Q_1d=%d not supported", p->Q));
+      if (p->Q < 2 || p->Q > 6) return fail(MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, "SetupBoundary: Q_1d=%d not supported", p->Q));
       if ((ierr = basis_device_tables(fx->basis))) return fail(ierr);
       p->kernel_name = "setup_boundary";
       return 0;

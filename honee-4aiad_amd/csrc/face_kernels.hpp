@@ -376,8 +376,15 @@ hipError_t launch_face_op(const VolumeArgs &a, const Ctx &ctx, hipStream_t strea
 
 }  // namespace
 
+// the volume kernels' (P_1d, Q_1d) pairs: Q_1d = P_1d, and Q_1d = P_1d + q_extra (1, 2) at degrees 1-3
 #define HONEE_FACE_PQ(CALL)                   \
   if (P == 2 && Q == 2) return CALL(2, 2);   \
   if (P == 3 && Q == 3) return CALL(3, 3);   \
   if (P == 4 && Q == 4) return CALL(4, 4);   \
-  if (P == 5 && Q == 5) return CALL(5, 5);
+  if (P == 5 && Q == 5) return CALL(5, 5);   \
+  if (P == 2 && Q == 3) return CALL(2, 3);   \
+  if (P == 2 && Q == 4) return CALL(2, 4);   \
+  if (P == 3 && Q == 4) return CALL(3, 4);   \
+  if (P == 3 && Q == 5) return CALL(3, 5);   \
+  if (P == 4 && Q == 5) return CALL(4, 5);   \
+  if (P == 4 && Q == 6) return CALL(4, 6);

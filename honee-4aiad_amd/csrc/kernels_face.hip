@@ -90,7 +90,9 @@ hipError_t face_diag_var(bool pblock, bool freestream, bool hllc, int var, bool 
 }  // namespace
 
 
-bool face_pq_supported(int P, int Q) { return P == Q && P >= 2 && P <= 5; }
+bool face_pq_supported(int P, int Q) {
+  return (P == Q && P >= 2 && P <= 5) || (P >= 2 && P <= 4 && Q >= P + 1 && Q <= P + 2);
+}
 
 hipError_t launch_setup_boundary(int Q, const SetupArgs &a, hipStream_t stream) {
   const long long n     = (long long)a.nelem * Q * Q;
@@ -102,6 +104,7 @@ hipError_t launch_setup_boundary(int Q, const SetupArgs &a, hipStream_t stream) 
     case 3: hipLaunchKernelGGL((k_setup_boundary<3>), dim3(grid), dim3(block), 0, stream, a); break;
     case 4: hipLaunchKernelGGL((k_setup_boundary<4>), dim3(grid), dim3(block), 0, stream, a); break;
     case 5: hipLaunchKernelGGL((k_setup_boundary<5>), dim3(grid), dim3(block), 0, stream, a); break;
+    case 6: hipLaunchKernelGGL((k_setup_boundary<6>), dim3(grid), dim3(block), 0, stream, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -611,13 +611,22 @@ hipError_t launch_ijacobian_diag_var(int var, bool fp32, const VolumeArgs &a, co
 
 }  // namespace
 
+#define HONEE_PQ_LIST(X) X(2, 2) X(3, 3) X(4, 4) X(5, 5) X(2, 3) X(2, 4) X(3, 4) X(3, 5) X(4, 5) X(4, 6)
 #define HONEE_PQ_DISPATCH(P_, Q_, CALL) \
   if (P == P_ && Q == Q_) return CALL;
+// P_1d = degree + 1 for degrees 1-4 with Q_1d = P_1d, and Q_1d = P_1d + q_extra for q_extra 1, 2 at degrees 1-3
+// (HONEE's -q_extra, src/cloptions.c:152-153)
 #define HONEE_PQ_ALL(CALL_T)          \
   HONEE_PQ_DISPATCH(2, 2, CALL_T(2, 2)) \
   HONEE_PQ_DISPATCH(3, 3, CALL_T(3, 3)) \
   HONEE_PQ_DISPATCH(4, 4, CALL_T(4, 4)) \
-  HONEE_PQ_DISPATCH(5, 5, CALL_T(5, 5))
+  HONEE_PQ_DISPATCH(5, 5, CALL_T(5, 5)) \
+  HONEE_PQ_DISPATCH(2, 3, CALL_T(2, 3)) \
+  HONEE_PQ_DISPATCH(2, 4, CALL_T(2, 4)) \
+  HONEE_PQ_DISPATCH(3, 4, CALL_T(3, 4)) \
+  HONEE_PQ_DISPATCH(3, 5, CALL_T(3, 5)) \
+  HONEE_PQ_DISPATCH(4, 5, CALL_T(4, 5)) \
+  HONEE_PQ_DISPATCH(4, 6, CALL_T(4, 6))
 
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream) {
   const long long n     = (long long)a.nelem * Q * Q * Q;
@@ -629,13 +638,16 @@ hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t str
     case 3: hipLaunchKernelGGL((k_setup_qdata<2, 3>), dim3(grid), dim3(block), 0, stream, a); break;
     case 4: hipLaunchKernelGGL((k_setup_qdata<2, 4>), dim3(grid), dim3(block), 0, stream, a); break;
     case 5: hipLaunchKernelGGL((k_setup_qdata<2, 5>), dim3(grid), dim3(block), 0, stream, a); break;
+    case 6: hipLaunchKernelGGL((k_setup_qdata<2, 6>), dim3(grid), dim3(block), 0, stream, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 bool volume_pq_supported(int P, int Q) {
-  return (P == 2 && Q == 2) || (P == 3 && Q == 3) || (P == 4 && Q == 4) || (P == 5 && Q == 5);
+#define SUP(P_, Q_) (P == P_ && Q == Q_) ||
+  return HONEE_PQ_LIST(SUP) false;
+#undef SUP
 }
 
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {

@@ -270,3 +270,19 @@ def test_oracle_point_block_diagonal_brute_force():
             ref[node, :, ci] = O.ijacobian(om, oc, qd, jac, e)[node * 5: node * 5 + 5]
     assert np.abs(ref - pb).max() <= 1e-13 * np.abs(ref).max()
     np.testing.assert_allclose(np.einsum("nkk->nk", pb).ravel(), O.ijacobian_diagonal(om, oc, qd, jac), rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_q_extra_quadrature_integrates_the_volume(degree):
+    """With Q_1d = P_1d + q_extra the quadrature still integrates det J exactly: the trilinear map's det J is of degree
+    at most 2 in each direction, inside every Gauss rule here, so a perturbed mesh's volume is the same for q_extra
+    0, 1 and 2, and an affine box's is the box volume."""
+    mesh = M.box_mesh((2, 2, 2), degree, perturb=0.15, seed=3)
+    vols = []
+    for q_extra in (0, 1, 2):
+        om = O.Mesh(mesh, M.tabulate(degree, q_extra))
+        vols.append(O.volume(om, om.setup_qdata()))
+    assert max(vols) - min(vols) < 1e-14 * vols[0]
+    box = M.box_mesh((2, 2, 2), degree)
+    om = O.Mesh(box, M.tabulate(degree, 2))
+    assert abs(O.volume(om, om.setup_qdata()) - (2 * np.pi) ** 3) < 1e-13 * (2 * np.pi) ** 3  # 1728 positive terms summed
