@@ -32,6 +32,40 @@ HBM_PEAK_GBS = 8000.0
 # line says so); other workloads report traffic null.
 PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 691.1e6}
 METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
+FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS = 78.6, 157.3
+# Floating-point operations per quadrature point of the QFunction bodies at the default workload (primitive
+# variables, SUPG, no damping layer, no gravity), counted exactly by running the oracle's restatement over an
+# operation-counting scalar; tests/test_oracle_opcount.py checks these figures against that count.
+QF_OPS_PER_POINT = {
+    "ifunction": {"add": 432, "mul": 472, "div": 12, "sqrt": 3},
+    "ijacobian": {"add": 405, "mul": 463, "div": 9, "sqrt": 3},
+}
+# Executed floating-point operations of the fused kernels at the default workload, per dispatch, from rocprofv3 --pmc
+# runs of SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 / _F32 and SQ_INSTS_VALU_MFMA_MOPS_F64 (64 per VALU instruction,
+# 2 x 64 per FMA, 512 per MFMA op unit); the counts and commands are in profiles/flops_newtonian_tgv32_p3.md.  A stored
+# figure, not measured by the run that reports it.
+PMC_EXECUTED_FLOPS = {((32, 32, 32), 3, "ifunction"): 2883584000.0}
+
+
+def basis_flops_per_element(P, Q, kind):
+    """Flops of the tensor-product basis passes on one element with P = Q (collocated quadrature, the workloads timed
+    here), five components.  A contraction of length n costs n multiplications and n - 1 additions per output; the
+    interpolation and each gradient direction are three passes of n^3 outputs; the transposes are the same passes
+    plus three additions per point to sum the value term and the three gradient terms.  IFunction: interpolation and
+    gradient of q, interpolation of q-dot, gradient and interpolation transposes.  IJacobian: the same without
+    q-dot.  Returns None for P != Q."""
+    if P != Q:
+        return None
+    n3, c = Q**3, 2 * Q - 1
+    pass3 = 3 * n3 * c
+    per_comp = {"ifunction": 5 * pass3 + 3 * n3, "ijacobian": 4 * pass3 + 3 * n3}[kind]
+    return 5 * per_comp
+
+
+def flops_per_apply(kind, nelem, P, Q):
+    qf = sum(QF_OPS_PER_POINT[kind].values())
+    b = basis_flops_per_element(P, Q, kind)
+    return None if b is None else nelem * (Q**3 * qf + b)
 
 
 def algorithmic_bytes(nelem, P, Q, nnodes, njac=14, jac_bytes=8, vec_bytes=8):
@@ -46,8 +80,20 @@ def algorithmic_bytes(nelem, P, Q, nnodes, njac=14, jac_bytes=8, vec_bytes=8):
     return ifunc, ijac
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(mesh_shape_full, degree, seconds=10.0):
-    """Time the oracle's IFunction + IJacobian on a 1/64 sample (8^3 elements) with all host threads."""
+    """Time the oracle's IFunction + IJacobian on a 1/64 sample (8^3 elements), with all host threads and then with
+    one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     from pyhonee import mesh as M
@@ -64,19 +110,28 @@ def cpu_baseline(mesh_shape_full, degree, seconds=10.0):
     qdot = np.zeros_like(q)
     dq = np.random.default_rng(0).uniform(-1e-3, 1e-3, q.shape)
     c.ijacobian_time_shift = 1.0 / 2e-3
-    n, t0 = 0, time.perf_counter()
-    while True:
-        _, jac = O.ifunction(om, c, qd, q, qdot)
-        O.ijacobian(om, c, qd, jac, dq)
-        n += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = (time.perf_counter() - t0) / n
+
+    def timed(threads):
+        O.set_num_threads(threads)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            _, jac = O.ifunction(om, c, qd, q, qdot)
+            O.ijacobian(om, c, qd, jac, dq)
+            n += 1
+            if time.perf_counter() - t0 > seconds:
+                break
+        return n, (time.perf_counter() - t0) / n
+
+    threads = O.max_threads()
+    n, dt = timed(threads)
+    n1, dt1 = timed(1)
+    O.set_num_threads(threads)
     dof = 5 * mesh.nnodes
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": dof / dt / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "value_1thread": dof / dt1 / 1e9, "cpu_model": _cpu_model(), "host_cpus": len(os.sched_getaffinity(0)),
             "sample": f"TGV {shape[0]}x{shape[1]}x{shape[2]} p={degree} (1/{int(np.prod(mesh_shape_full) // np.prod(shape))} of the "
-                      f"workload), {n} residual+Jacobian pairs in {n * dt:.1f} s, oracle (FP64 C restatement of /cpu/self)"}
+                      f"workload), {n} residual+Jacobian pairs in {n * dt:.1f} s on {threads} threads and {n1} in {n1 * dt1:.1f} s on "
+                      f"1 thread, oracle (FP64 C restatement of /cpu/self, OpenMP over elements, gcc -O2)"}
 
 
 def time_integration_c2(mesh, tab, Y, steps, warm_steps=1):
@@ -301,6 +356,8 @@ def c5_scale(steps=10, ts_steps=2, warm_steps=1):
             out["ifunction_gdofs"] = dof / r["ifunction"] / 1e9
             out["ifunction_roofline_frac"] = b_if / r["ifunction"] / 1e9 / HBM_PEAK_GBS
             out["ifunction_apply_ms"] = r["ifunction"] * 1e3
+            out["ifunction_tflops"] = flops_per_apply("ifunction", mesh.nelem, tab.P, tab.Q) / r["ifunction"] / 1e12
+            out["ijacobian_tflops"] = flops_per_apply("ijacobian", mesh.nelem, tab.P, tab.Q) / r["ijacobian"] / 1e12
             out["ijacobian_gdofs"] = dof / r["ijacobian"] / 1e9
             out["ijacobian_roofline_frac"] = b_ij / r["ijacobian"] / 1e9 / HBM_PEAK_GBS
             prob.close()
@@ -435,15 +492,21 @@ def main():
         # steady-state apply durations: `steps` back-to-back applies bracketed by two events on the backend stream
         # (events between launches slow the kernels that follow them, so the per-kernel figures above only split an
         # apply into its fused kernel and node gather)
-        apply_s = {}
+        # (three repetitions, the fastest kept: a host stall inside one repetition leaves the GPU idle between two
+        # events, which is not the apply's duration)
+        apply_s, apply_reps = {}, {}
         for name, fn in (("ifunction_apply", lambda: prob.ifunction(q, qdot, g, mem=CEED_MEM_DEVICE)),
                          ("ijacobian_apply", lambda: prob.ijacobian(dq, v, shift, mem=CEED_MEM_DEVICE))):
-            prob.reset_times()
-            prob.region_begin()
-            for _ in range(args.steps):
-                fn()
-            prob.region_end(name)
-            apply_s[name] = prob.kernel_times()[name][1] / args.steps * 1e-3
+            reps = []
+            for _ in range(3):
+                prob.reset_times()
+                prob.region_begin()
+                for _ in range(args.steps):
+                    fn()
+                prob.region_end(name)
+                reps.append(prob.kernel_times()[name][1] / args.steps * 1e-3)
+            apply_s[name], apply_reps[name] = min(reps), reps
+        results[label]["apply_reps_us"] = {k: [round(x * 1e6, 2) for x in v_] for k, v_ in apply_reps.items()}
         # Jacobi and point-block diagonal assembly of the IJacobian (the preconditioners of the implicit solve)
         diag = torch.zeros_like(q)
         pb = torch.zeros(25 * mesh.nnodes, dtype=torch.float64, device=dev)
@@ -502,14 +565,32 @@ def main():
                                        "(profiles/ifunction_newtonian_tgv32_p3_traffic.md), not measured by this run",
                      "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
                      "avg_launch_us": t_if_k * 1e6, "avg_node_gather_us": t_gather * 1e6, "scatter": args.scatter,
+                     "apply_repetitions_us": r64["apply_reps_us"]["ifunction_apply"],
                      "timing": "avg_apply_us: back-to-back applies between two HIP events on the backend stream (fused kernel + node "
-                               "gather); avg_launch_us / avg_node_gather_us: per-kernel events"},
+                               "gather), the fastest of three repetitions; avg_launch_us / avg_node_gather_us: per-kernel events"},
     }
+    f_if = flops_per_apply("ifunction", mesh.nelem, tab.P, tab.Q)
+    if f_if is not None:
+        ex = PMC_EXECUTED_FLOPS.get((shape, degree, "ifunction"))
+        line["roofline"]["flops"] = {
+            "dtype": "f64", "per_launch": f_if, "per_point": f_if / (mesh.nelem * tab.Q**3), "achieved": f_if / t_if / 1e12,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f_if / t_if / 1e12 / FP64_PEAK_TFLOPS,
+            "source": "QFunction: exact count of the oracle's restatement (operation-counting scalar); basis: sum-factorized passes",
+            "executed_per_launch": ex,
+            "executed_frac": None if ex is None else ex / t_if_k / 1e12 / FP64_PEAK_TFLOPS,
+            "executed_source": "stored: rocprofv3 --pmc SQ_INSTS_VALU_*_F64 counts of the fused kernel "
+                               "(profiles/flops_newtonian_tgv32_p3.md), over its per-kernel launch time",
+        }
+    f_ij = flops_per_apply("ijacobian", mesh.nelem, tab.P, tab.Q)
+    if f_ij is not None:
+        line["ijacobian_tflops"] = f_ij / t_ij / 1e12
     if "fp32" in results:
         t32 = results["fp32"]["apply_s"]["ijacobian_apply"]
         _, b32 = algorithmic_bytes(mesh.nelem, tab.P, tab.Q, mesh.nnodes, jac_bytes=4)
         line["ijacobian_fp32_gdofs"] = dof / t32 / 1e9
         line["ijacobian_fp32_roofline_frac"] = b32 / t32 / 1e9 / HBM_PEAK_GBS
+        if f_ij is not None:
+            line["ijacobian_fp32_tflops"] = f_ij / t32 / 1e12
     line["ijacobian_roofline_frac"] = b_ij / t_ij / 1e9 / HBM_PEAK_GBS
     # diagonal assembly reads what the IJacobian apply reads except the input vector
     line["assembly"] = {}

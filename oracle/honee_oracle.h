@@ -166,10 +166,14 @@ void oracle_riemann_flux_fwd(const OGas *g, int hllc, const double *qL, const do
 void oracle_hll_speeds(const OGas *g, const double *qL, const double *qR, int var, const double *n, double *sL, double *sR);
 void oracle_hll_speeds_fwd(const OGas *g, const double *qL, const double *dqL, const double *qR, const double *dqR, int var, const double *n,
                            double *sL, double *dsL, double *sR, double *dsR);
+void oracle_set_num_threads(int n);
+int  oracle_max_threads(void);
 void oracle_qf_ifunction(const ONewt *c, const double *q, const double *gref, const double *qdot, double wdetJ, const double *dXdx,
                          const double *x, double *v, double *gv, double *jd);
 void oracle_qf_ijacobian(const ONewt *c, const double *dq, const double *gref, double wdetJ, const double *dXdx, const double *jd, double *v,
                          double *gv);
+void oracle_qf_rhs(const ONewt *c, const double *U, const double *gref, double wdetJ, const double *dXdx, const double *x, double *v,
+                   double *gv);
 void oracle_tau_diag(const OGas *g, const OTauCoef *tc, const double *q, int var, const double *dXdx, double dt, double *Tau);
 
 #ifdef __cplusplus

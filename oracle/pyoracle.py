@@ -81,6 +81,14 @@ def lib():
     return _lib
 
 
+def set_num_threads(n: int) -> None:
+    lib().oracle_set_num_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
+
+
 def _d(a):
     return a.ctypes.data_as(_dp)
 
@@ -258,6 +266,65 @@ def qf_ijacobian(c: ONewt, dq, gref, wdetJ, dXdx, jd):
     a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
     lib().oracle_qf_ijacobian(ct.byref(c), _d(a(dq)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(jd)), _d(v), _d(gv))
     return v, gv.reshape(3, 5)
+
+
+def qf_rhs(c: ONewt, U, gref, wdetJ, dXdx, x=(0.0, 0.0, 0.0)):
+    v, gv = np.zeros(5), np.zeros(15)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    lib().oracle_qf_rhs(ct.byref(c), _d(a(U)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(x)), _d(v), _d(gv))
+    return v, gv.reshape(3, 5)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Operation counts of the QFunction bodies (liboracle_opcount.so, opcount.cpp): the same code over a counting scalar
+# ---------------------------------------------------------------------------------------------------------------------
+OPCOUNT_PATH = os.path.join(HERE, "liboracle_opcount.so")
+OP_KINDS = ("add", "mul", "div", "sqrt", "exp", "log")
+_oplib = None
+
+
+def oplib():
+    global _oplib
+    if _oplib is None:
+        if not os.path.exists(OPCOUNT_PATH):
+            build()
+        _oplib = ct.CDLL(OPCOUNT_PATH)
+    return _oplib
+
+
+def _counts(n):
+    return {k: int(n[i]) for i, k in enumerate(OP_KINDS)}
+
+
+def opcount_prim_from_cons(gas: OGas, U):
+    Y, n = np.zeros(5), np.zeros(6, dtype=np.int64)
+    oplib().oracle_opcount_prim_from_cons(ct.byref(gas), _d(np.ascontiguousarray(U, float)), _d(Y), n.ctypes.data_as(ct.c_void_p))
+    return Y, _counts(n)
+
+
+def opcount_ifunction(c: ONewt, q, gref, qdot, wdetJ, dXdx, x=(0.0, 0.0, 0.0)):
+    """IFunction body at one point over the counting scalar: (v, gv, jd, counts)."""
+    v, gv, jd, n = np.zeros(5), np.zeros(15), np.zeros(15), np.zeros(6, dtype=np.int64)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    oplib().oracle_opcount_ifunction(ct.byref(c), _d(a(q)), _d(a(gref)), _d(a(qdot)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(x)), _d(v), _d(gv),
+                                     _d(jd), n.ctypes.data_as(ct.c_void_p))
+    return v, gv.reshape(3, 5), jd, _counts(n)
+
+
+def opcount_ijacobian(c: ONewt, dq, gref, wdetJ, dXdx, jd):
+    v, gv, n = np.zeros(5), np.zeros(15), np.zeros(6, dtype=np.int64)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    oplib().oracle_opcount_ijacobian(ct.byref(c), _d(a(dq)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(jd)), _d(v), _d(gv),
+                                     n.ctypes.data_as(ct.c_void_p))
+    return v, gv.reshape(3, 5), _counts(n)
+
+
+def opcount_rhs(c: ONewt, U, gref, wdetJ, dXdx, x=(0.0, 0.0, 0.0)):
+    v, gv, n = np.zeros(5), np.zeros(15), np.zeros(6, dtype=np.int64)
+    a = lambda z: np.ascontiguousarray(z, dtype=np.float64)  # noqa: E731
+    oplib().oracle_opcount_rhs(ct.byref(c), _d(a(U)), _d(a(gref)), ct.c_double(wdetJ), _d(a(dXdx)), _d(a(x)), _d(v), _d(gv),
+                               n.ctypes.data_as(ct.c_void_p))
+    return v, gv.reshape(3, 5), _counts(n)
 
 
 # ---------------------------------------------------------------------------------------------------------------------
