@@ -1266,6 +1266,23 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
   }
 }
 
+// The point blocks of the 4 x 4 x 4 element go through the element-ordered E-vector (25 doubles per element node of
+// workspace) and a per-node gather that adds them to the output in a fixed order: deterministic, and measured 2.4 ->
+// 1.0 ms at 32^3 p=3 against 25 FP64 atomics per element node.  The Jacobi diagonal (5 atomics per element node) and
+// the point blocks of other elements add atomically (the diagonal measured 0.62 ms atomic, 0.68 ms two-pass).
+static int diag_evec_begin(Ceed ceed, OpPlan *p, VolumeArgs &a, bool point_block) {
+  a.ev = nullptr;
+  if (!point_block || !(p->P == 4 && p->Q == 4)) return 0;
+  MI_CALL(rstr_transpose_map(p->r_q));
+  return ceed_workspace(ceed, (size_t)p->nelem * p->r_q->elem_size * 25, &a.ev);
+}
+static int diag_evec_finish(Ceed ceed, OpPlan *p, const VolumeArgs &a) {
+  if (!a.ev) return 0;
+  const CeedElemRestriction r = p->r_q;
+  MI_HIP(ceed, launch_node_gather_blocks(r->t_nrows, r->d_t_rowptr, r->d_t_index, r->d_t_offset, a.ev, a.v, ceed->stream));
+  return 0;
+}
+
 // Point-Jacobi diagonal (MatGetDiagonal_Ceed -> CeedOperatorLinearAssembleDiagonal, src/mat-ceed.c:1603-1630) or, with
 // point_block, the [node][c_out][c_in] point-block diagonal (PCVPBJACOBI's MatInvertVariableBlockDiagonal input,
 // CeedOperatorLinearAssemblePointBlockDiagonal) of an interleaved (comp_stride 1) 5-component operator.
@@ -1331,8 +1348,10 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "mass_diagonal", &ev));
+    MI_CALL(diag_evec_begin(ceed, p, a, point_block));
     if (point_block) MI_HIP(ceed, launch_mass_point_block_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
     else MI_HIP(ceed, launch_mass_diagonal(p->P, p->Q, p->kind == QfKind::MassNewtonian, a, ctx, ceed->stream));
+    MI_CALL(diag_evec_finish(ceed, p, a));
     MI_CALL(timing_end(ceed, "mass_diagonal", ev));
     return 0;
   }
@@ -1395,8 +1414,10 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
   else MI_CALL(vec_device_read(p->v_jac, &a.jacr));
   hipEvent_t ev;
   MI_CALL(timing_begin(ceed, "ijacobian_diagonal", &ev));
+  MI_CALL(diag_evec_begin(ceed, p, a, point_block));
   if (point_block) MI_HIP(ceed, launch_ijacobian_point_block_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
   else MI_HIP(ceed, launch_ijacobian_diagonal(p->P, p->Q, p->var, ceed->jac_fp32, a, ctx, ceed->stream));
+  MI_CALL(diag_evec_finish(ceed, p, a));
   MI_CALL(timing_end(ceed, "ijacobian_diagonal", ev));
   return 0;
 }

@@ -409,6 +409,14 @@ struct ContractPairs {
     zacc(W[1], BB, g[3]);                                              // (GG, BB)
     zacc(W[2], BB, g[4]);                                              // (BB, GG)
     zacc(W[7], BB, g[5]);                                              // (GB, GB)
+    yx_groups<NC>(g, BB, GB, GG, qx, qy, r);
+  }
+
+  // The y and x passes of the six z-contracted (Fx, Fy) groups g[m][k] of the 4 x 4 x 4 element (lane layout kept:
+  // qx, qy, node z), across lanes: r[k] += sum of the groups' y then x contractions
+  template <int K>
+  __device__ static void yx_groups(const double (&g)[6][K], const double *BB, const double *GB, const double *GG, int qx, int qy,
+                                   double (&r)[K]) {
     const RowRot rr(qy);
     double       yBB[4], yGB[4], yGG[4], xBB[4], xGB[4], xGG[4];
 #pragma unroll
@@ -417,7 +425,7 @@ struct ContractPairs {
       xBB[i] = BB[i * 4 + qx], xGB[i] = GB[i * 4 + qx], xGG[i] = GG[i * 4 + qx];
     }
 #pragma unroll
-    for (int k = 0; k < NC; k++) {
+    for (int k = 0; k < K; k++) {
       const double h0 = row_contract(yBB, g[0][k]) + row_contract(yGB, g[2][k]) + row_contract(yGG, g[4][k]);  // Fx = BB
       const double h1 = row_contract(yBB, g[1][k]) + row_contract(yGB, g[5][k]);                                // Fx = GB
       const double h2 = row_contract(yBB, g[3][k]);                                                             // Fx = GG

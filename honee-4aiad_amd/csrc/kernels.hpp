@@ -36,7 +36,7 @@ struct VolumeArgs {
   const double *u;        // active input L-vector (q or dq)
   const double *u2;       // second L-vector input (q_dot), if any
   double       *v;        // output L-vector (summed into with atomics when ev is null)
-  double       *ev;       // two-pass scatter: node-sorted element-node contributions [nelem * P^3][5] (replaces v)
+  double       *ev;       // two-pass scatter: element-node contributions [nelem * P^3][5] ([..][25] for point blocks; replaces v)
   const int    *perm;     // two-pass scatter: slot of element node e * P^3 + j in ev (null: slot e * P^3 + j)
   const double *qdata;
   QDataLayout   qd;
@@ -141,6 +141,9 @@ hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int ele
 // element-ordered E-vector or slot(j) = j for a node-sorted one (t_index null).
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
                               const double *ev, double *v, bool overwrite, hipStream_t s);
+// The same for 5 x 5 point blocks: v[25 (t_offset[k] / 5) + i] += sum over the node's slots of ev[t_index[j] * 25 + i]
+hipError_t launch_node_gather_blocks(int nrows, const int *t_rowptr, const int *t_index, const int *t_offset, const double *ev, double *v,
+                                     hipStream_t s);
 hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u, hipStream_t s);
 
 // SGS closure and projections (kernels_sgs.hip)

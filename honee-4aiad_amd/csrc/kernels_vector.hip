@@ -177,6 +177,28 @@ __global__ void __launch_bounds__(256) k_node_gather_rows(int nrows, int cs, con
   }
 }
 
+// The point-block form of the element-ordered gather: the E-vector holds a 5 x 5 block [c_out][c_in] per element node
+// (25 consecutive values), the output a block per node at 25 * (offset / 5) (the L-vector offsets of an interleaved
+// 5-component field are 5 * node).  One thread per (node, c_out) row of the block sums its 5 values over the node's
+// contributions in ascending slot order and adds them to the output.
+__global__ void __launch_bounds__(256) k_node_gather_blocks(int nrows, const int *__restrict__ rowptr, const int *__restrict__ index,
+                                                            const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v) {
+  const long long n = (long long)nrows * 5;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(i / 5), co = (int)(i % 5);
+    const int b = rowptr[row], m = rowptr[row + 1] - b;
+    double    acc[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < m; k++) {
+      const double *src = ev + (long long)index[b + k] * 25 + co * 5;
+#pragma unroll
+      for (int c = 0; c < 5; c++) acc[c] += src[c];
+    }
+    double *dst = v + (long long)toff[row] * 5 + co * 5;
+#pragma unroll
+    for (int c = 0; c < 5; c++) dst[c] += acc[c];
+  }
+}
+
 // Per-block partial sums of x . y over a fixed grid (deterministic for a given n and npartial)
 __global__ void k_dot(const double *x, const double *y, long long n, double *partial) {
   __shared__ double red[kBlock];
@@ -210,6 +232,14 @@ hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *
   else
     hipLaunchKernelGGL((k_node_gather<false, NPT>), dim3(grid), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev,
                        v, overwrite);
+  return hipGetLastError();
+}
+
+hipError_t launch_node_gather_blocks(int nrows, const int *t_rowptr, const int *t_index, const int *t_offset, const double *ev, double *v,
+                                     hipStream_t s) {
+  const long long n = (long long)nrows * 5;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_node_gather_blocks, dim3(grid_for(n)), dim3(kBlock), 0, s, nrows, t_rowptr, t_index, t_offset, ev, v);
   return hipGetLastError();
 }
 

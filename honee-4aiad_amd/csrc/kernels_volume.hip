@@ -5,90 +5,9 @@
 // operator's quadrature-point body (newtonian_ops.hpp) in registers, applies the basis transpose and scatter-adds into
 // the output L-vector.  Quadrature data (qdata, jac_data) is streamed in the backend's strided layout.  Each element
 // owns Q^3 threads (one per quadrature point) and a block holds EPB elements; blocks loop over element batches.
-//
-// k_diagonal<P, Q, Op> assembles the diagonal of a linear operator (CeedOperatorLinearAssembleDiagonal): at each
-// quadrature point the 4x4 block coupling {value, d/dX0, d/dX1, d/dX2} of a component to itself is obtained by applying
-// the linear quadrature-point action to unit inputs, then contracted against products of the 1D tables
-// (B.B, G.B, G.G) with sum factorization.
-#include <hip/hip_runtime.h>
-
-#include <type_traits>
-
-#include "basis3d.hpp"
-#include "kernels.hpp"
-#include "newtonian_ops.hpp"
+#include "volume_common.hpp"
 
 namespace {
-
-using basis3::Dims;
-using basis3::Elem;
-
-template <int Q>
-constexpr int elems_per_block() {
-  return (Q * Q * Q >= 64) ? 1 : 64 / (Q * Q * Q);
-}
-
-// Element-batch schedule of the persistent blocks.  The workgroups of a dispatch are distributed round-robin over
-// the 8 XCDs of an MI355X, each with its own L2 (block b runs on XCD b % 8).  Each XCD gets one contiguous range of
-// element batches, which its blocks sweep together, so the nodes that neighbouring elements share are fetched into
-// one L2 once instead of into several.
-constexpr int kNumXcd = 8;
-
-struct BatchRange {
-  long long first, end, stride;
-};
-
-__device__ __forceinline__ BatchRange batch_range(long long nbatch) {
-  const int G = gridDim.x, b = blockIdx.x;
-  if (G % kNumXcd != 0) return {b, nbatch, G};
-  const long long per = (nbatch + kNumXcd - 1) / kNumXcd;
-  const int       x = b % kNumXcd, j = b / kNumXcd;
-  const long long lo = x * per, hi = (x + 1) * per < nbatch ? (x + 1) * per : nbatch;
-  return {lo + j, hi, G / kNumXcd};
-}
-
-// The basis stage of the fused kernel: for the 4 x 4 x 4 element the x and z passes run on the FP64 matrix cores and the
-// y passes across lanes (basis3::StageMF), except for ops with Op::kZCrossLane (the FP32 Jacobian, measured faster with
-// all passes across lanes); otherwise sum factorization through LDS (basis3::Stage, ZX: z passes across lanes)
-template <int P, int Q, bool ZX>
-struct StageFor {
-  using type = basis3::Stage<P, Q, ZX>;
-};
-template <>
-struct StageFor<4, 4, false> {
-  using type = basis3::StageMF;
-};
-
-// Op::kZCrossLane selects the basis stage whose z passes also run across lanes (basis3::Stage's ZX); default off
-template <class Op, class = void>
-struct ZCross {
-  static constexpr bool value = false;
-};
-template <class Op>
-struct ZCross<Op, std::void_t<decltype(Op::kZCrossLane)>> {
-  static constexpr bool value = Op::kZCrossLane;
-};
-
-template <int P, int Q>
-__device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc) {
-  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) sB[i] = bt.B[i];
-  for (int i = threadIdx.x; i < Q * Q; i += blockDim.x) sDc[i] = bt.Dc[i];
-}
-// with the derivative table G [Q][P] (formed as Dc B when the basis does not provide it)
-template <int P, int Q>
-__device__ __forceinline__ void load_tables(const BasisTables &bt, double *sB, double *sDc, double *sG) {
-  load_tables<P, Q>(bt, sB, sDc);
-  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
-    if (bt.G) {
-      sG[i] = bt.G[i];
-    } else {
-      const int q = i / P, n = i % P;
-      double    g = 0;
-      for (int m = 0; m < Q; m++) g += bt.Dc[q * Q + m] * bt.B[m * P + n];
-      sG[i] = g;
-    }
-  }
-}
 
 // ------------------------------------------------------------------------------------------------------------------
 // Setup: qdata = {w detJ, dXdx} from the coordinate field (qfunctions/setupgeo.h:42-56).  The trilinear coordinate
@@ -319,209 +238,6 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   }
 }
 
-// ------------------------------------------------------------------------------------------------------------------
-// Diagonal of a linear operator whose quadrature-point action is Op::action (IJacobian)
-// ------------------------------------------------------------------------------------------------------------------
-template <int P, int Q, class Op, typename T>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
-  using Dm          = Dims<P, Q>;
-  constexpr int EPB = elems_per_block<Q>();
-  constexpr int Q3  = Dm::Q3;
-  __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
-  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
-  __shared__ double sWd[EPB][10][1][Dm::MAX3];  // slot-pair weights of one component
-  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
-    sBB[i] = a.bt.BB[i];
-    sGB[i] = a.bt.GB[i];
-    sGG[i] = a.bt.GG[i];
-  }
-  __shared__ double sB[Q * P], sDc[Q * Q];
-  load_tables<P, Q>(a.bt, sB, sDc);
-  const int       le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>      el(sB, sDc, s0[le], s1[le], s2[le], t);
-  double(*Wd)[1][Dm::MAX3] = sWd[le];
-  const long long nbatch = ((long long)a.nelem + EPB - 1) / EPB;
-
-  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
-    const long long e_  = bt * EPB + le;
-    const bool      ok  = e_ < a.nelem;
-    const int       e   = ok ? (int)e_ : 0;
-    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
-    double          u2[5];
-    if constexpr (Op::kIn2) {
-#pragma unroll
-      for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
-    }
-    typename Op::Point p;
-    double             qd[10];
-    nsop::fetch_qdata<Q>(a, e, t, ok, qd);
-    Op::template load_point<Q>(a, ctx, e, t, ok, qd, u2, p);
-    // One action body in the code: the unit input (s, c) and the rows of M are selected by comparisons with the
-    // runtime loop indices (no runtime array index), so only one action's temporaries are live at a time.
-#pragma unroll 1
-    for (int c = 0; c < 5; c++) {
-      double M[4][4] = {{0}};
-#pragma unroll 1
-      for (int s = 0; s < Op::kInSlots; s++) {
-        T dq[5], gref[3][5];
-#pragma unroll
-        for (int cc = 0; cc < 5; cc++) {
-          dq[cc] = (s == 0 && cc == c) ? T(1) : T(0);
-#pragma unroll
-          for (int d = 0; d < 3; d++) gref[d][cc] = (s == 1 + d && cc == c) ? T(1) : T(0);
-        }
-        double v[5], gw[3][5];
-        Op::action(ctx, p, dq, gref, v, gw);
-        double vc = 0, gc[3] = {0, 0, 0};
-#pragma unroll
-        for (int cc = 0; cc < 5; cc++) {
-          vc = cc == c ? v[cc] : vc;
-#pragma unroll
-          for (int d = 0; d < 3; d++) gc[d] = cc == c ? gw[d][cc] : gc[d];
-        }
-#pragma unroll
-        for (int ss = 0; ss < 4; ss++) {
-          if (ss == s) {
-            M[ss][0] = ok ? vc : 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) M[ss][1 + d] = ok ? gc[d] : 0.0;
-          }
-        }
-      }
-      // sum over input slot a and test slot b of M[a][b] phi_a phi_b: the weights of the unordered slot pairs, each with
-      // its 1D factor triple, contracted together (basis3::ContractPairs); the contraction's coefficient addresses
-      // derive from an opaque copy of t, so that its table loads are not hoisted out of the component loop
-      int tc = t;
-      asm volatile("" : "+v"(tc));
-      __syncthreads();  // the previous component's contraction has read Wd
-      Wd[0][0][t] = M[0][0], Wd[1][0][t] = M[1][1], Wd[2][0][t] = M[2][2], Wd[3][0][t] = M[3][3];
-      Wd[4][0][t] = M[0][1] + M[1][0], Wd[5][0][t] = M[0][2] + M[2][0], Wd[6][0][t] = M[0][3] + M[3][0];
-      Wd[7][0][t] = M[1][2] + M[2][1], Wd[8][0][t] = M[1][3] + M[3][1], Wd[9][0][t] = M[2][3] + M[3][2];
-      double r[1] = {0};
-      basis3::ContractPairs<P, Q, 1>{tc}.add(Wd, sBB, sGB, sGG, r);
-      if (ok && t < Dm::P3) atomicAdd(&a.v[off + c * a.comp_stride], r[0]);
-    }
-  }
-}
-
-// ------------------------------------------------------------------------------------------------------------------
-// Point-block diagonal of a linear operator whose quadrature-point action is Op::action: for each input component
-// c_in and input slot s (value, three reference derivatives), one action gives the test-slot weights of all five
-// output components; the weights of each unordered slot pair are summed (the diagonal's 1D products are symmetric in
-// the pair) and contracted five components at a time, grouped by shared 1D factors (basis3::ContractPairs).  Output
-// layout [node][c_out][c_in] at 25 * (offset / 5).
-// ------------------------------------------------------------------------------------------------------------------
-// The weight slot of the unordered input/test slot pair (s, b), as in k_diagonal: 0..3 the diagonal pairs, then
-// (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) -> 4..9
-__device__ __forceinline__ int pair_slot(int s, int b) {
-  const int i = s < b ? s : b, j = s < b ? b : s;
-  return i == j ? i : (i == 0 ? 3 + j : (i == 1 ? 5 + j : 9));
-}
-
-template <int P, int Q, class Op, typename T>
-__global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_pblock_diagonal(VolumeArgs a, const CeedMiNewtonianContext ctx) {
-  using Dm          = Dims<P, Q>;
-  constexpr int EPB = elems_per_block<Q>();
-  constexpr int Q3  = Dm::Q3;
-  constexpr int NS  = Op::kInSlots;
-  constexpr int NP  = 10;  // slot pairs (with one input slot only pairs 0, 4, 5, 6 carry weights)
-  __shared__ double sBB[Q * P], sGB[Q * P], sGG[Q * P];
-  __shared__ double s0[EPB][Dm::MAX3], s1[EPB][Dm::MAX3], s2[EPB][Dm::MAX3];
-  // pair weights W[pair][c_out][q] (each thread owns its q column, so the accumulation needs no barrier) and the
-  // contraction's intermediate arrays
-  __shared__ double sW[EPB][NP][5][Dm::MAX3];
-  __shared__ double c1[EPB][5][Dm::MAX3], c2[EPB][5][Dm::MAX3];
-  for (int i = threadIdx.x; i < Q * P; i += blockDim.x) {
-    sBB[i] = a.bt.BB[i];
-    sGB[i] = a.bt.GB[i];
-    sGG[i] = a.bt.GG[i];
-  }
-  __shared__ double sB[Q * P], sDc[Q * Q];
-  load_tables<P, Q>(a.bt, sB, sDc);
-  const int               le = threadIdx.x / Q3, t = threadIdx.x % Q3;
-  Elem<P, Q>              el(sB, sDc, s0[le], s1[le], s2[le], t);
-  double(*W)[5][Dm::MAX3] = sW[le];
-  const long long         nbatch = ((long long)a.nelem + EPB - 1) / EPB;
-
-  for (long long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
-    const long long e_  = bt * EPB + le;
-    const bool      ok  = e_ < a.nelem;
-    const int       e   = ok ? (int)e_ : 0;
-    const int       off = (t < Dm::P3) ? a.offsets[(long long)e * Dm::P3 + t] : 0;
-    double          u2[5];
-    if constexpr (Op::kIn2) {
-#pragma unroll
-      for (int c = 0; c < 5; c++) u2[c] = el.interp((t < Dm::P3) ? a.u2[off + c * a.comp_stride] : 0.0);
-    }
-#pragma unroll 1
-    for (int ci = 0; ci < 5; ci++) {
-      __syncthreads();  // the previous component's contractions have read W
-      for (int k = 0; k < NP; k++)
-        if (NS > 1 || k == 0 || (k >= 4 && k <= 6))
-#pragma unroll
-          for (int co = 0; co < 5; co++) W[k][co][t] = 0;
-      // The point data is reloaded for each input component (an opaque element index keeps the compiler from hoisting
-      // it out of the loop), so that it is not live across the contractions below.
-      int eo = e;
-      asm volatile("" : "+v"(eo));
-      typename Op::Point p;
-      double             qd[10];
-      nsop::fetch_qdata<Q>(a, eo, t, ok, qd);
-      Op::template load_point<Q>(a, ctx, eo, t, ok, qd, u2, p);
-      // the slot loop unrolled (the four actions overlap; measured faster than one action body here, unlike k_diagonal)
-#pragma unroll
-      for (int s = 0; s < NS; s++) {
-        T dq[5], gref[3][5];
-#pragma unroll
-        for (int c = 0; c < 5; c++) {  // unit input (s, ci) without a runtime array index
-          dq[c] = (s == 0 && c == ci) ? T(1) : T(0);
-#pragma unroll
-          for (int d = 0; d < 3; d++) gref[d][c] = (s == 1 + d && c == ci) ? T(1) : T(0);
-        }
-        double v[5], gw[3][5];
-        Op::action(ctx, p, dq, gref, v, gw);
-        if (!ok) continue;
-#pragma unroll
-        for (int co = 0; co < 5; co++) {
-          W[pair_slot(s, 0)][co][t] += v[co];
-#pragma unroll
-          for (int d = 0; d < 3; d++) W[pair_slot(s, 1 + d)][co][t] += gw[d][co];
-        }
-      }
-      // the contraction's coefficient addresses derive from an opaque copy of t, so that the compiler does not hoist
-      // the ten contractions' table loads out of the component loop (they would stay live across the actions)
-      int tc = t;
-      asm volatile("" : "+v"(tc));
-      double r[5] = {0, 0, 0, 0, 0};
-      if constexpr (NS > 1) {
-        basis3::ContractPairs<P, Q, 5>{tc}.add(W, sBB, sGB, sGG, r);
-      } else {
-        basis3::Contract5<P, Q> pb{nullptr, c1[le], c2[le], tc};
-        pb.add_staged(W[0], sBB, sBB, sBB, r);
-        pb.add_staged(W[4], sGB, sBB, sBB, r);
-        pb.add_staged(W[5], sBB, sGB, sBB, r);
-        pb.add_staged(W[6], sBB, sBB, sGB, r);
-      }
-      if (ok && t < Dm::P3)
-#pragma unroll
-        for (int co = 0; co < 5; co++) atomicAdd(&a.v[(long long)off * 5 + co * 5 + ci], r[co]);
-    }
-  }
-}
-
-template <typename K>
-int grid_for(K kernel, int block, long long nbatch) {
-  int dev = 0, ncu = 0, per_cu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
-  // one persistent block per resident slot: each XCD's blocks then sweep a contiguous window of its batch range
-  long long g = (long long)ncu * (per_cu > 0 ? per_cu : 1);
-  if (g > nbatch) g = nbatch >= kNumXcd ? nbatch / kNumXcd * kNumXcd : nbatch;
-  if (g < 1) g = 1;
-  return (int)g;
-}
-
 template <int P, int Q, class Op>
 hipError_t launch_op(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
   constexpr int EPB   = elems_per_block<Q>();
@@ -530,42 +246,6 @@ hipError_t launch_op(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hip
   const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
   return hipGetLastError();
-}
-
-template <int P, int Q, class Op, typename T>
-hipError_t launch_diag(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
-  constexpr int EPB   = elems_per_block<Q>();
-  const int     block = EPB * Q * Q * Q;
-  auto          k     = k_diagonal<P, Q, Op, T>;
-  const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
-  return hipGetLastError();
-}
-
-template <int P, int Q, class Op, typename T>
-hipError_t launch_pblock(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
-  constexpr int EPB   = elems_per_block<Q>();
-  const int     block = EPB * Q * Q * Q;
-  auto          k     = k_pblock_diagonal<P, Q, Op, T>;
-  const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
-  return hipGetLastError();
-}
-
-template <int P, int Q>
-hipError_t launch_ijacobian_pblock_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-  if (fp32) {
-    switch (var) {
-      case 0: return launch_pblock<P, Q, nsop::IJacobian<0, float>, float>(a, ctx, s);
-      case 1: return launch_pblock<P, Q, nsop::IJacobian<1, float>, float>(a, ctx, s);
-      default: return launch_pblock<P, Q, nsop::IJacobian<2, float>, float>(a, ctx, s);
-    }
-  }
-  switch (var) {
-    case 0: return launch_pblock<P, Q, nsop::IJacobian<0, double>, double>(a, ctx, s);
-    case 1: return launch_pblock<P, Q, nsop::IJacobian<1, double>, double>(a, ctx, s);
-    default: return launch_pblock<P, Q, nsop::IJacobian<2, double>, double>(a, ctx, s);
-  }
 }
 
 template <int P, int Q>
@@ -593,40 +273,7 @@ hipError_t launch_ijacobian_var(int var, bool fp32, const VolumeArgs &a, const C
   }
 }
 
-template <int P, int Q>
-hipError_t launch_ijacobian_diag_var(int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-  if (fp32) {
-    switch (var) {
-      case 0: return launch_diag<P, Q, nsop::IJacobian<0, float>, float>(a, ctx, s);
-      case 1: return launch_diag<P, Q, nsop::IJacobian<1, float>, float>(a, ctx, s);
-      default: return launch_diag<P, Q, nsop::IJacobian<2, float>, float>(a, ctx, s);
-    }
-  }
-  switch (var) {
-    case 0: return launch_diag<P, Q, nsop::IJacobian<0, double>, double>(a, ctx, s);
-    case 1: return launch_diag<P, Q, nsop::IJacobian<1, double>, double>(a, ctx, s);
-    default: return launch_diag<P, Q, nsop::IJacobian<2, double>, double>(a, ctx, s);
-  }
-}
-
 }  // namespace
-
-#define HONEE_PQ_LIST(X) X(2, 2) X(3, 3) X(4, 4) X(5, 5) X(2, 3) X(2, 4) X(3, 4) X(3, 5) X(4, 5) X(4, 6)
-#define HONEE_PQ_DISPATCH(P_, Q_, CALL) \
-  if (P == P_ && Q == Q_) return CALL;
-// P_1d = degree + 1 for degrees 1-4 with Q_1d = P_1d, and Q_1d = P_1d + q_extra for q_extra 1, 2 at degrees 1-3
-// (HONEE's -q_extra, src/cloptions.c:152-153)
-#define HONEE_PQ_ALL(CALL_T)          \
-  HONEE_PQ_DISPATCH(2, 2, CALL_T(2, 2)) \
-  HONEE_PQ_DISPATCH(3, 3, CALL_T(3, 3)) \
-  HONEE_PQ_DISPATCH(4, 4, CALL_T(4, 4)) \
-  HONEE_PQ_DISPATCH(5, 5, CALL_T(5, 5)) \
-  HONEE_PQ_DISPATCH(2, 3, CALL_T(2, 3)) \
-  HONEE_PQ_DISPATCH(2, 4, CALL_T(2, 4)) \
-  HONEE_PQ_DISPATCH(3, 4, CALL_T(3, 4)) \
-  HONEE_PQ_DISPATCH(3, 5, CALL_T(3, 5)) \
-  HONEE_PQ_DISPATCH(4, 5, CALL_T(4, 5)) \
-  HONEE_PQ_DISPATCH(4, 6, CALL_T(4, 6))
 
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream) {
   const long long n     = (long long)a.nelem * Q * Q * Q;
@@ -664,13 +311,6 @@ hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_ijacobian_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-#define CALL_ID(P_, Q_) (launch_ijacobian_diag_var<P_, Q_>(var, fp32, a, ctx, s))
-  HONEE_PQ_ALL(CALL_ID)
-#undef CALL_ID
-  return hipErrorInvalidValue;
-}
-
 hipError_t launch_rhs(int P, int Q, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
 #define CALL_RHS(P_, Q_) (launch_op<P_, Q_, nsop::RHSFunction>(a, ctx, s))
   HONEE_PQ_ALL(CALL_RHS)
@@ -682,30 +322,6 @@ hipError_t launch_mass(int P, int Q, bool stabilized, const VolumeArgs &a, const
 #define CALL_MS(P_, Q_) (stabilized ? launch_op<P_, Q_, nsop::MassNewtonian>(a, ctx, s) : launch_op<P_, Q_, nsop::Mass5>(a, ctx, s))
   HONEE_PQ_ALL(CALL_MS)
 #undef CALL_MS
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_mass_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-#define CALL_MD(P_, Q_) \
-  (stabilized ? launch_diag<P_, Q_, nsop::MassNewtonian, double>(a, ctx, s) : launch_diag<P_, Q_, nsop::Mass5, double>(a, ctx, s))
-  HONEE_PQ_ALL(CALL_MD)
-#undef CALL_MD
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_ijacobian_point_block_diagonal(int P, int Q, int var, bool fp32, const VolumeArgs &a, const CeedMiNewtonianContext &ctx,
-                                                 hipStream_t s) {
-#define CALL_PB(P_, Q_) (launch_ijacobian_pblock_var<P_, Q_>(var, fp32, a, ctx, s))
-  HONEE_PQ_ALL(CALL_PB)
-#undef CALL_PB
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_mass_point_block_diagonal(int P, int Q, bool stabilized, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-#define CALL_MPB(P_, Q_) \
-  (stabilized ? launch_pblock<P_, Q_, nsop::MassNewtonian, double>(a, ctx, s) : launch_pblock<P_, Q_, nsop::Mass5, double>(a, ctx, s))
-  HONEE_PQ_ALL(CALL_MPB)
-#undef CALL_MPB
   return hipErrorInvalidValue;
 }
 
