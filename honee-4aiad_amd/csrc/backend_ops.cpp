@@ -852,10 +852,12 @@ static int sgs_model_upload(CeedOperator op, OpPlan *p, CeedMiNewtonianContext *
   if (c->size < offsetof(CeedMiSgsDDContext, data))
     return MI_ERR(ceed, CEED_ERROR_INCOMPATIBLE, "%s: context is smaller than struct SgsDDContext_", op->qf->name.c_str());
   const auto *sc = (const CeedMiSgsDDContext *)c->data;
-  if (sc->num_inputs != 6 || sc->num_outputs != 6 || sc->num_neurons != 20 || sc->num_layers != 1)
-    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "%s: only the 6-20-6 single-hidden-layer model has a fused kernel", op->qf->name.c_str());
-  const size_t ends[5] = {sc->offsets.bias1 + 20, sc->offsets.bias2 + 6, sc->offsets.weight1 + 120, sc->offsets.weight2 + 120,
-                          sc->offsets.out_scaling + 12};
+  if (sc->num_inputs != 6 || sc->num_outputs != 6 || sc->num_neurons != 20 || (sc->num_layers != 1 && sc->num_layers != 2))
+    return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "%s: only the 6-20-6 and 6-20-20-6 models have a fused kernel", op->qf->name.c_str());
+  // a second hidden layer (num_layers 2) follows out_scaling: its biases [20], then its weights [20][20]
+  const size_t hidden2 = sc->offsets.out_scaling + 12;
+  const size_t ends[6] = {sc->offsets.bias1 + 20, sc->offsets.bias2 + 6, sc->offsets.weight1 + 120, sc->offsets.weight2 + 120,
+                          sc->offsets.out_scaling + 12, sc->num_layers == 2 ? hidden2 + 420 : 0};
   for (size_t e : ends)
     if (offsetof(CeedMiSgsDDContext, data) + e * sizeof(double) > c->size)
       return MI_ERR(ceed, CEED_ERROR_DIMENSION, "%s: model parameters extend past the context", op->qf->name.c_str());
@@ -871,6 +873,9 @@ static int sgs_model_upload(CeedOperator op, OpPlan *p, CeedMiNewtonianContext *
   put(sc->offsets.weight2, 120);
   put(sc->offsets.out_scaling, 12);
   packed.push_back(sc->alpha);
+  packed.push_back((double)sc->num_layers);
+  if (sc->num_layers == 2) put(hidden2, 420);
+  else packed.insert(packed.end(), 420, 0.0);
   if (packed.size() != sgs_model_doubles()) return MI_ERR(ceed, CEED_ERROR_BACKEND, "sgs model packing mismatch");
   if (p->d_model && packed == p->model_key) return 0;
   if (!p->d_model && g_no_alloc) return MI_ERR(ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the SGS model is not on the device yet");

@@ -689,14 +689,16 @@ static int FreeSgsContext(void *data) {
 // SgsDDContextFill (problems/sgs_dd_model.c:602-650): parameters at the reference's offsets, with w1.dat
 // [input][neuron] and w2.dat [neuron][output] transposed to [neuron][input] and [output][neuron]
 static int SgsDDContextCreate(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
-  const int    ni = 6, no = 6, nn = 20;
+  if ((prm->wh == NULL) != (prm->bh == NULL)) return CEED_ERROR_MINOR;  // a second hidden layer needs both arrays
+  const int    ni = 6, no = 6, nn = 20, layers = prm->wh ? 2 : 1;
   const size_t bias1 = 0, bias2 = bias1 + nn, weight1 = bias2 + nn, weight2 = weight1 + (size_t)nn * ni, out_scaling = weight2 + (size_t)ni * nn;
-  const size_t total_scalars = out_scaling + 2 * (size_t)no;
+  const size_t hidden2       = out_scaling + 2 * (size_t)no;  // second hidden layer (extension): biases, then weights
+  const size_t total_scalars = hidden2 + (layers == 2 ? (size_t)nn + (size_t)nn * nn : 0);
   const size_t total_bytes   = sizeof(CeedMiSgsDDContext) + total_scalars * sizeof(double);
   CeedMiSgsDDContext *c      = calloc(1, total_bytes);
   c->num_inputs              = ni;
   c->num_outputs             = no;
-  c->num_layers              = 1;
+  c->num_layers              = layers;
   c->num_neurons             = nn;
   c->alpha                   = prm->alpha;
   c->newt_ctx                = *p->ctx;
@@ -713,6 +715,11 @@ static int SgsDDContextCreate(HoneeMiProblem p, const HoneeMiSgsDDParams *prm) {
     for (int j = 0; j < nn; j++) c->data[weight1 + (size_t)j * ni + i] = prm->w1[(size_t)i * nn + j];
   for (int i = 0; i < nn; i++)  // TransposeMatrix(w2, N = num_neurons, M = num_outputs)
     for (int j = 0; j < no; j++) c->data[weight2 + (size_t)j * nn + i] = prm->w2[(size_t)i * no + j];
+  if (layers == 2) {
+    memcpy(&c->data[hidden2], prm->bh, sizeof(double) * nn);
+    for (int i = 0; i < nn; i++)  // [first-layer neuron][second-layer neuron] -> [second][first]
+      for (int j = 0; j < nn; j++) c->data[hidden2 + nn + (size_t)j * nn + i] = prm->wh[(size_t)i * nn + j];
+  }
   CHK(p, CeedQFunctionContextCreate(p->ceed, &p->sgsdd_qfctx));
   CHK(p, CeedQFunctionContextSetData(p->sgsdd_qfctx, CEED_MEM_HOST, CEED_USE_POINTER, total_bytes, c));
   CHK(p, CeedQFunctionContextSetDataDestroy(p->sgsdd_qfctx, CEED_MEM_HOST, FreeSgsContext));

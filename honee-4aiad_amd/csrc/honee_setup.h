@@ -67,9 +67,12 @@ int          HoneeMiGetJacData32(HoneeMiProblem p, float *out);
 // them (b1.dat [20], b2.dat [6], w1.dat [6 inputs][20 neurons], w2.dat [20 neurons][6 outputs], OutScaling.dat [6][2])
 // and -sgs_model_dd_leakyrelu_alpha.  Setup projects the grid anisotropy tensor (CG) and adds the SGS residual to
 // every HoneeMiApplyIFunction (implicit problems).
+// An optional second hidden layer (an extension; null for the reference's model): bh [20] and wh [20 first-layer
+// neurons][20 second-layer neurons], the .dat orientation; w2 then holds [second-layer neuron][output].
 typedef struct {
   double        alpha;
   const double *b1, *b2, *w1, *w2, *out_scaling;
+  const double *bh, *wh;
 } HoneeMiSgsDDParams;
 int HoneeMiProblemSetupSgsDD(HoneeMiProblem p, const HoneeMiSgsDDParams *params);
 int HoneeMiSgsGridAnisotropy(HoneeMiProblem p, double *out, int *cg_iterations);  // [nnodes][7], host

@@ -2,7 +2,7 @@
 //   * max-pivot Jacobi eigen solver for symmetric 3x3 matrices, eigenvalues decreasing, eigenvectors as rows
 //     (qfunctions/utils_eigensolver_jacobi.h:29-335),
 //   * element anisotropy tensor from dX/dx (qfunctions/grid_anisotropy_tensor.h:14-31),
-//   * model inputs, 6-20-6 leaky-ReLU MLP, de-normalization and rotation back to the physical frame
+//   * model inputs, 6-20-6 (or 6-20-20-6) leaky-ReLU MLP, de-normalization and rotation back to the physical frame
 //     (qfunctions/sgs_dd_utils.h:17-127, qfunctions/sgs_dd_model.h:33-63),
 //   * the SGS flux of IFunction_NodalSgs (qfunctions/sgs_dd_model.h:199-213).
 // Kelvin-Mandel packing as in qfunctions/utils.h:239-258.  Every 3x3 index is a compile-time constant (the Jacobi
@@ -15,9 +15,12 @@ namespace sgs {
 
 constexpr int kInputs = 6, kOutputs = 6, kNeurons = 20;
 
-// Model parameters in device memory: b1[20], b2[6], W1[20][6], W2[6][20], bounds[6][2], alpha
+// Model parameters in device memory: b1[20], b2[6], W1[20][6], W2[6][20], bounds[6][2], alpha, then the number of
+// hidden layers (1, the reference's model, or 2) and the second hidden layer bh[20], Wh[20][20] (used when 2): with two
+// hidden layers W2 maps the second hidden layer to the outputs.
 struct Model {
   double b1[kNeurons], b2[kOutputs], W1[kNeurons][kInputs], W2[kOutputs][kNeurons], bounds[kOutputs][2], alpha;
+  double layers, bh[kNeurons], Wh[kNeurons][kNeurons];
 };
 
 __device__ __forceinline__ void km_pack(const double A[3][3], double v[6]) {
@@ -211,6 +214,19 @@ __device__ __forceinline__ void sgs_finish(const Model &m, const double in[6], d
     for (int j = 0; j < kInputs; j++) acc += m.W1[i][j] * in[j];
     V[i] = m.b1[i] + acc;
     V[i] *= V[i] < 0 ? m.alpha : 1.;
+  }
+  if (m.layers > 1.5) {  // second hidden layer (an extension of the reference's single hidden layer)
+    double V2[kNeurons];
+#pragma unroll
+    for (int i = 0; i < kNeurons; i++) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < kNeurons; j++) acc += m.Wh[i][j] * V[j];
+      V2[i] = m.bh[i] + acc;
+      V2[i] *= V2[i] < 0 ? m.alpha : 1.;
+    }
+#pragma unroll
+    for (int i = 0; i < kNeurons; i++) V[i] = V2[i];
   }
   double out[kOutputs];
 #pragma unroll

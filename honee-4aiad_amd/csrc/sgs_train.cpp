@@ -91,7 +91,7 @@ static int stage(HoneeMiMlp p, int n, const float *X, const float *T, int mem_ty
 // Gradient (and optionally an Adam step) on samples idx[first .. first+count) of dX/dT
 static int batch(HoneeMiMlp p, const float *dX, const float *dT, const int *idx, int first, int count, const HoneeMiMlpTrainOptions *o, bool adam,
                  float *loss_dst) {
-  const int nb = (count + 255) / 256;
+  const int tpb = mlp_grad_block(p->d), nb = (count + tpb - 1) / tpb;
   if (ensure_partials(p, nb)) return 1;
   int nblocks = 0;
   HIPCHK(launch_mlp_grad(p->d, p->theta, dX, dT, idx, first, count, 1.0f / ((float)count * p->d.no), p->partial, p->loss_partial, &nblocks, p->s));
@@ -110,8 +110,12 @@ extern "C" {
 const char *HoneeMiMlpLastError(void) { return g_err.c_str(); }
 
 int HoneeMiMlpCreate(int ni, int nh, int no, float alpha, unsigned long long seed, HoneeMiMlp *mlp) {
-  const MlpDims d{ni, nh, no, alpha};
-  if (!mlp_dims_supported(d)) return fail("HoneeMiMlpCreate: only the 6-20-6 network has training kernels");
+  return HoneeMiMlpCreate2(ni, nh, 0, no, alpha, seed, mlp);
+}
+
+int HoneeMiMlpCreate2(int ni, int nh, int nh2, int no, float alpha, unsigned long long seed, HoneeMiMlp *mlp) {
+  const MlpDims d{ni, nh, no, alpha, nh2};
+  if (!mlp_dims_supported(d)) return fail("HoneeMiMlpCreate: only the 6-20-6 and 6-20-20-6 networks have training kernels");
   auto p = new HoneeMiMlp_();
   p->d   = d;
   p->np  = mlp_nparams(d);
@@ -124,9 +128,16 @@ int HoneeMiMlpCreate(int ni, int nh, int no, float alpha, unsigned long long see
   HIPCHK(hipMalloc((void **)&p->loss, sizeof(float)));
   std::vector<float> theta(p->np, 0.f);
   SplitMix           rng{seed};
-  const double       r1 = sqrt(6.0 / (ni + nh)), r2 = sqrt(6.0 / (nh + no));
+  // Glorot-uniform weights layer by layer, biases zero
+  const int    nl = nh2 > 0 ? nh2 : no;
+  const double r1 = sqrt(6.0 / (ni + nh)), r2 = sqrt(6.0 / (nh + nl));
   for (int i = 0; i < nh * ni; i++) theta[i] = (float)((2 * rng.uniform() - 1) * r1);
-  for (int i = 0; i < no * nh; i++) theta[nh * ni + nh + i] = (float)((2 * rng.uniform() - 1) * r2);
+  for (int i = 0; i < nl * nh; i++) theta[nh * ni + nh + i] = (float)((2 * rng.uniform() - 1) * r2);
+  if (nh2 > 0) {
+    const double r3 = sqrt(6.0 / (nh2 + no));
+    const int    o3 = nh * ni + nh + nh2 * nh + nh2;
+    for (int i = 0; i < no * nh2; i++) theta[o3 + i] = (float)((2 * rng.uniform() - 1) * r3);
+  }
   return HoneeMiMlpSetParams(p, theta.data());
 }
 

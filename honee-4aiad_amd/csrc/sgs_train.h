@@ -1,6 +1,6 @@
-// Training of the data-driven SGS model on the GPU (FP32): the 6-20-6 leaky-ReLU MLP that the fused FP64 closure
-// evaluates (problems/sgs_dd_model.c, qfunctions/sgs_dd_model.h:37-53), trained by mean-squared error with Adam on
-// model-space samples built from the filtered training fields (src/smartsim/sgs_dd_training.c).  The reference
+// Training of the data-driven SGS model on the GPU (FP32): the 6-20-6 (or 6-20-20-6) leaky-ReLU MLP that the fused
+// FP64 closure evaluates (problems/sgs_dd_model.c, qfunctions/sgs_dd_model.h:37-53), trained by mean-squared error with
+// Adam on model-space samples built from the filtered training fields (src/smartsim/sgs_dd_training.c).  The reference
 // trains its network outside the solver; these entry points have no counterpart in it.
 #pragma once
 
@@ -21,6 +21,9 @@ typedef struct {
 // Layer sizes must be 6, 20, 6.  Weights Glorot-uniform from seed, biases zero.  Parameter vector layout (FP32):
 // W1 [nh][ni], b1 [nh], W2 [no][nh], b2 [no].
 int HoneeMiMlpCreate(int ni, int nh, int no, float alpha, unsigned long long seed, HoneeMiMlp *mlp);
+// With a second hidden layer of nh2 neurons (0: as HoneeMiMlpCreate); sizes 6, 20, 20, 6.  Layout: W1 [nh][ni], b1 [nh],
+// W2 [nh2][nh], b2 [nh2], W3 [no][nh2], b3 [no].
+int HoneeMiMlpCreate2(int ni, int nh, int nh2, int no, float alpha, unsigned long long seed, HoneeMiMlp *mlp);
 int HoneeMiMlpNumParams(HoneeMiMlp mlp);
 int HoneeMiMlpSetParams(HoneeMiMlp mlp, const float *theta);  // host arrays; resets the Adam state
 int HoneeMiMlpGetParams(HoneeMiMlp mlp, float *theta);

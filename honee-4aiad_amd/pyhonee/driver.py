@@ -155,7 +155,7 @@ def load_libs():
 
 
 class HoneeMiSgsDDParams(ct.Structure):
-    _fields_ = [("alpha", ct.c_double), ("b1", _dp), ("b2", _dp), ("w1", _dp), ("w2", _dp), ("out_scaling", _dp)]
+    _fields_ = [("alpha", ct.c_double), ("b1", _dp), ("b2", _dp), ("w1", _dp), ("w2", _dp), ("out_scaling", _dp), ("bh", _dp), ("wh", _dp)]
 
 
 class HoneeMiDiffFilterOptions(ct.Structure):
@@ -256,7 +256,12 @@ class NewtonianProblem:
                     w1=np.ascontiguousarray(np.asarray(model["W1"]).T, dtype=np.float64),  # w1.dat order [input][neuron]
                     w2=np.ascontiguousarray(np.asarray(model["W2"]).T, dtype=np.float64),  # w2.dat order [neuron][output]
                     out_scaling=np.ascontiguousarray(model["bounds"], dtype=np.float64))
-        prm = HoneeMiSgsDDParams(float(model["alpha"]), *(arrs[k].ctypes.data_as(_dp) for k in ("b1", "b2", "w1", "w2", "out_scaling")))
+        if "Wh" in model:  # second hidden layer: [first-layer neuron][second-layer neuron]
+            arrs["bh"] = np.ascontiguousarray(model["bh"], dtype=np.float64)
+            arrs["wh"] = np.ascontiguousarray(np.asarray(model["Wh"]).T, dtype=np.float64)
+        ptr = lambda k: arrs[k].ctypes.data_as(_dp) if k in arrs else None  # noqa: E731
+        prm = HoneeMiSgsDDParams(float(model["alpha"]), *(ptr(k) for k in ("b1", "b2", "w1", "w2", "out_scaling", "bh", "wh")))
+        self._sgs_arrays = arrs  # alive until the setup has copied them
         self._check(self.lib.HoneeMiProblemSetupSgsDD(self.handle, ct.byref(prm)), "SetupSgsDD")
 
     def sgs_grid_anisotropy(self):

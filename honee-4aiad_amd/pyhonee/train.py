@@ -57,12 +57,20 @@ def options(lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, batch=4096, epochs=1, see
 
 
 class Mlp:
-    """The FP32 6-20-6 leaky-ReLU network on the GPU."""
+    """The FP32 leaky-ReLU network on the GPU: dims (6, 20, 6), the reference closure's shape, or (6, 20, 20, 6) with a
+    second hidden layer."""
 
     def __init__(self, alpha=0.01, seed=0, dims=(6, 20, 6)):
-        self.alpha, self.dims = float(alpha), dims
+        self.alpha, self.dims = float(alpha), tuple(dims)
         self.h = ct.c_void_p()
-        _chk(lib().HoneeMiMlpCreate(*dims, ct.c_float(self.alpha), ct.c_ulonglong(seed), ct.byref(self.h)), "HoneeMiMlpCreate")
+        if len(self.dims) == 3:
+            ni, nh, no = self.dims
+            nh2 = 0
+        elif len(self.dims) == 4:
+            ni, nh, nh2, no = self.dims
+        else:
+            raise ValueError(f"dims {dims}: one or two hidden layers")
+        _chk(lib().HoneeMiMlpCreate2(ni, nh, nh2, no, ct.c_float(self.alpha), ct.c_ulonglong(seed), ct.byref(self.h)), "HoneeMiMlpCreate2")
         self.nparams = lib().HoneeMiMlpNumParams(self.h)
 
     @property
@@ -96,20 +104,24 @@ class Mlp:
 
     def predict(self, X) -> np.ndarray:
         X = _f32(X)
-        Y = np.zeros((X.shape[0], self.dims[2]), dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.dims[-1]), dtype=np.float32)
         _chk(lib().HoneeMiMlpPredict(self.h, X.shape[0], X.ctypes.data, 0, Y.ctypes.data), "Predict")
         return Y
 
     def closure_model(self, bounds) -> dict:
         """Parameters for the fused closure (pyhonee.sgs.load_dd_model layout) with the output bounds the training
         targets were normalized with."""
-        ni, nh, no = self.dims
         th = self.params.astype(np.float64)
-        W1 = th[:nh * ni].reshape(nh, ni)
-        b1 = th[nh * ni:nh * ni + nh]
-        W2 = th[nh * ni + nh:nh * ni + nh + no * nh].reshape(no, nh)
-        b2 = th[nh * ni + nh + no * nh:]
-        return {"b1": b1, "b2": b2, "W1": W1, "W2": W2, "bounds": np.asarray(bounds, dtype=np.float64).reshape(no, 2), "alpha": self.alpha}
+        layers, o = [], 0
+        for n_in, n_out in zip(self.dims[:-1], self.dims[1:]):
+            layers.append((th[o:o + n_out * n_in].reshape(n_out, n_in), th[o + n_out * n_in:o + n_out * n_in + n_out]))
+            o += n_out * n_in + n_out
+        no = self.dims[-1]
+        out = {"W1": layers[0][0], "b1": layers[0][1], "W2": layers[-1][0], "b2": layers[-1][1],
+               "bounds": np.asarray(bounds, dtype=np.float64).reshape(no, 2), "alpha": self.alpha}
+        if len(layers) == 3:  # second hidden layer
+            out["Wh"], out["bh"] = layers[1]
+        return out
 
     def close(self):
         if self.h:

@@ -70,6 +70,9 @@ typedef struct {
 // struct SgsDDContext_ (qfunctions/sgs_dd_model.h:15-31): the data-driven SGS model, its parameters in data[] at the
 // given offsets (bias1[num_neurons], bias2[num_outputs], weight1[num_neurons][num_inputs],
 // weight2[num_outputs][num_neurons], out_scaling[num_outputs][2]); total_bytes is the size of the whole allocation.
+// Extension (the reference evaluates one hidden layer): with num_layers = 2, a second hidden layer of num_neurons
+// follows out_scaling in data[]: its biases [num_neurons], then its weights [num_neurons (out)][num_neurons (in)];
+// weight2 then maps the second hidden layer to the outputs.
 typedef struct {
   CeedInt                num_inputs, num_outputs;
   CeedInt                num_layers;

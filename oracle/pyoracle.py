@@ -545,17 +545,19 @@ def bdf(m: Mesh, c: ONewt, qdata, q, dt, nsteps, order=2, t0=0.0, snes_rtol=1e-8
 # ---- data-driven SGS closure (honee_oracle.c, sgs_pointwise.h) ----
 class OSgsModel(ct.Structure):
     _fields_ = [("num_inputs", ct.c_int), ("num_outputs", ct.c_int), ("num_neurons", ct.c_int), ("alpha", ct.c_double),
-                ("b1", _dp), ("b2", _dp), ("W1", _dp), ("W2", _dp), ("bounds", _dp)]
+                ("b1", _dp), ("b2", _dp), ("W1", _dp), ("W2", _dp), ("bounds", _dp), ("bh", _dp), ("Wh", _dp)]
 
 
 class SgsModel:
-    """Keeps the parameter arrays alive for the C struct."""
+    """Keeps the parameter arrays alive for the C struct.  A model dict with "Wh" [neuron][neuron] and "bh" has a
+    second hidden layer."""
 
     def __init__(self, params: dict):
-        self.arrays = {k: np.ascontiguousarray(params[k], dtype=np.float64) for k in ("b1", "b2", "W1", "W2", "bounds")}
+        keys = ("b1", "b2", "W1", "W2", "bounds") + (("bh", "Wh") if "Wh" in params else ())
+        self.arrays = {k: np.ascontiguousarray(params[k], dtype=np.float64) for k in keys}
         a = self.arrays
         self.s = OSgsModel(a["W1"].shape[1], a["W2"].shape[0], a["W1"].shape[0], float(params["alpha"]), _d(a["b1"]), _d(a["b2"]),
-                           _d(a["W1"]), _d(a["W2"]), _d(a["bounds"]))
+                           _d(a["W1"]), _d(a["W2"]), _d(a["bounds"]), _d(a["bh"]) if "bh" in a else None, _d(a["Wh"]) if "Wh" in a else None)
 
 
 def diagonalize3(A, max_sweeps=5):
@@ -686,45 +688,55 @@ MLP_DIMS = (6, 20, 6)
 
 
 def mlp_unpack(theta, dims=MLP_DIMS):
-    """Parameter vector layout W1 [nh][ni], b1 [nh], W2 [no][nh], b2 [no]."""
-    ni, nh, no = dims
+    """Layers (W, b) of the parameter vector: dims (ni, nh, no) gives W1 [nh][ni], b1 [nh], W2 [no][nh], b2 [no];
+    dims (ni, nh, nh2, no) gives W1 [nh][ni], b1 [nh], W2 [nh2][nh], b2 [nh2], W3 [no][nh2], b3 [no]."""
     theta = np.asarray(theta, dtype=np.float64)
-    o = 0
-    W1 = theta[o:o + nh * ni].reshape(nh, ni)
-    o += nh * ni
-    b1 = theta[o:o + nh]
-    o += nh
-    W2 = theta[o:o + no * nh].reshape(no, nh)
-    o += no * nh
-    b2 = theta[o:o + no]
-    return W1, b1, W2, b2
+    layers, o = [], 0
+    for n_in, n_out in zip(dims[:-1], dims[1:]):
+        W = theta[o:o + n_out * n_in].reshape(n_out, n_in)
+        o += n_out * n_in
+        b = theta[o:o + n_out]
+        o += n_out
+        layers.append((W, b))
+    if o != theta.size:
+        raise ValueError(f"parameter vector of {theta.size} entries does not match dims {dims}")
+    return layers
 
 
 def mlp_forward(theta, X, alpha, dims=MLP_DIMS):
-    W1, b1, W2, b2 = mlp_unpack(theta, dims)
-    h = np.asarray(X, dtype=np.float64) @ W1.T + b1
-    a = np.where(h < 0, alpha * h, h)
-    return a @ W2.T + b2
+    a = np.asarray(X, dtype=np.float64)
+    layers = mlp_unpack(theta, dims)
+    for W, b in layers[:-1]:
+        h = a @ W.T + b
+        a = np.where(h < 0, alpha * h, h)
+    W, b = layers[-1]
+    return a @ W.T + b
 
 
 def mlp_loss_grad(theta, X, T, alpha, dims=MLP_DIMS):
-    """Mean squared error over samples and outputs and its gradient by back-propagation."""
-    W1, b1, W2, b2 = mlp_unpack(theta, dims)
+    """Mean squared error over samples and outputs and its gradient by back-propagation (any number of hidden layers)."""
+    layers = mlp_unpack(theta, dims)
     X = np.asarray(X, dtype=np.float64)
     T = np.asarray(T, dtype=np.float64)
     n, no = T.shape
-    h = X @ W1.T + b1
-    a = np.where(h < 0, alpha * h, h)
-    y = a @ W2.T + b2
+    acts, pre = [X], []
+    for W, b in layers[:-1]:
+        h = acts[-1] @ W.T + b
+        pre.append(h)
+        acts.append(np.where(h < 0, alpha * h, h))
+    W, b = layers[-1]
+    y = acts[-1] @ W.T + b
     e = y - T
     loss = float(np.sum(e * e) / (n * no))
-    dy = 2 * e / (n * no)
-    gW2 = dy.T @ a
-    gb2 = dy.sum(0)
-    dh = (dy @ W2) * np.where(h < 0, alpha, 1.0)
-    gW1 = dh.T @ X
-    gb1 = dh.sum(0)
-    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2])
+    d = 2 * e / (n * no)
+    grads = []
+    for li in range(len(layers) - 1, -1, -1):
+        W, _ = layers[li]
+        grads.append((d.T @ acts[li], d.sum(0)))
+        if li > 0:
+            d = (d @ W) * np.where(pre[li - 1] < 0, alpha, 1.0)
+    grads.reverse()
+    return loss, np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
 
 
 def adam_step(theta, grad, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):

@@ -6,7 +6,7 @@
 //   * the element anisotropy tensor A_ij = Delta_ij / ||Delta||, Delta = g^(-1/2) (qfunctions/grid_anisotropy_tensor.h:14-31),
 //   * the model inputs (eigenframe strain rates, vorticity, viscous scale) and outputs (de-normalization, rotation back to
 //     the physical frame) (qfunctions/sgs_dd_utils.h:17-127),
-//   * the 6-20-6 MLP with leaky ReLU (qfunctions/sgs_dd_model.h:33-53),
+//   * the 6-20-6 MLP with leaky ReLU (qfunctions/sgs_dd_model.h:33-53), optionally with a second hidden layer,
 //   * the SGS flux added to the residual (qfunctions/sgs_dd_model.h:199-213).
 // Kelvin-Mandel packing as in qfunctions/utils.h:239-258.
 #pragma once
@@ -24,6 +24,10 @@ typedef struct {
   const double *W1;       // [num_neurons][num_inputs]
   const double *W2;       // [num_outputs][num_neurons]
   const double *bounds;   // [num_outputs][2] new (min, max) of the de-normalization
+  // second hidden layer (an extension of the reference's model; Wh null for one hidden layer): W2 then maps it to the
+  // outputs
+  const double *bh;  // [num_neurons]
+  const double *Wh;  // [num_neurons][num_neurons]
 } OSgsModel;
 
 static inline void o_km_pack(const double A[3][3], double v[6]) {
@@ -191,6 +195,16 @@ static inline void o_dd_inference(const OSgsModel *m, const double *in, double *
     for (int j = 0; j < m->num_inputs; j++) acc += m->W1[i * m->num_inputs + j] * in[j];
     V[i] = m->b1[i] + acc;
     V[i] *= V[i] < 0 ? m->alpha : 1.;
+  }
+  if (m->Wh) {  // second hidden layer, same form as the first
+    double V2[64];
+    for (int i = 0; i < m->num_neurons; i++) {
+      double acc = 0;
+      for (int j = 0; j < m->num_neurons; j++) acc += m->Wh[i * m->num_neurons + j] * V[j];
+      V2[i] = m->bh[i] + acc;
+      V2[i] *= V2[i] < 0 ? m->alpha : 1.;
+    }
+    for (int i = 0; i < m->num_neurons; i++) V[i] = V2[i];
   }
   for (int i = 0; i < m->num_outputs; i++) {
     double acc = 0;

@@ -105,3 +105,47 @@ def test_model_samples_match_oracle():
     X_ref, S_ref = O.sgs_model_samples(gas, train, filtered, G, A)
     assert np.abs(X - X_ref).max() < 1e-6 * np.abs(X_ref).max()
     assert np.linalg.norm(S - S_ref) < 1e-11 * np.linalg.norm(S_ref)
+
+
+# ---- the 6-20-20-6 network (a second hidden layer) ----
+DIMS2 = (6, 20, 20, 6)
+
+
+@pytest.mark.parametrize("n", [1000, 3001])
+def test_two_hidden_layers_loss_grad_matches_oracle(n):
+    _say(f"mlp 6-20-20-6 loss/grad n={n}")
+    rng = np.random.default_rng(10 + n)
+    mlp = TR.Mlp(alpha=ALPHA, seed=13, dims=DIMS2)
+    assert mlp.nparams == 20 * 6 + 20 + 20 * 20 + 20 + 6 * 20 + 6
+    X, T = rng.normal(size=(n, 6)).astype(np.float32), rng.normal(size=(n, 6)).astype(np.float32)
+    loss, g = mlp.loss_grad(X, T)
+    loss_ref, g_ref = O.mlp_loss_grad(_f64(mlp.params), _f64(X), _f64(T), float(np.float32(ALPHA)), DIMS2)
+    assert abs(loss - loss_ref) < 1e-5 * loss_ref
+    assert np.linalg.norm(g - g_ref) < 5e-5 * np.linalg.norm(g_ref), np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    Y = mlp.predict(X[:500])
+    Y_ref = O.mlp_forward(_f64(mlp.params), _f64(X[:500]), float(np.float32(ALPHA)), DIMS2)
+    assert np.abs(Y - Y_ref).max() < 1e-5 * max(1.0, np.abs(Y_ref).max())
+    mlp.close()
+
+
+def test_two_hidden_layers_adam_and_training():
+    _say("mlp 6-20-20-6 adam and training")
+    rng = np.random.default_rng(16)
+    mlp = TR.Mlp(alpha=ALPHA, seed=17, dims=DIMS2)
+    X, T = rng.normal(size=(2048, 6)).astype(np.float32), rng.normal(size=(2048, 6)).astype(np.float32)
+    th = _f64(mlp.params)
+    m, v = np.zeros_like(th), np.zeros_like(th)
+    for t in range(1, 4):
+        mlp.adam_step(X, T, TR.options(lr=1e-3))
+        _, g = O.mlp_loss_grad(th, _f64(X), _f64(T), float(np.float32(ALPHA)), DIMS2)
+        th, m, v = O.adam_step(th, g, m, v, t, 1e-3)
+    assert np.abs(mlp.params - th).max() < 2e-6, np.abs(mlp.params - th).max()
+    mlp.close()
+    teacher = TR.Mlp(alpha=ALPHA, seed=101, dims=DIMS2)
+    Xt = rng.uniform(-1, 1, size=(20000, 6)).astype(np.float32)
+    Tt = teacher.predict(Xt)
+    student = TR.Mlp(alpha=ALPHA, seed=2, dims=DIMS2)
+    hist = student.train(Xt, Tt, TR.options(lr=3e-3, batch=1024, epochs=40, seed=5))
+    assert np.all(np.isfinite(hist)) and hist[-1] < 0.05 * hist[0], (hist[0], hist[-1])
+    student.close()
+    teacher.close()

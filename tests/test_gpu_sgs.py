@@ -88,6 +88,34 @@ def test_sgs_nodal_stress(model, state_var, periodic):
     prob.close()
 
 
+def _two_layer_model(seed=31):
+    """A 6-20-20-6 closure: the shipped model's first and output layers with a random second hidden layer."""
+    p = dict(S.load_dd_model(GOLDEN, alpha=ALPHA))
+    rng = np.random.default_rng(seed)
+    p["Wh"] = rng.normal(size=(20, 20)) * 0.3
+    p["bh"] = rng.normal(size=20) * 0.1
+    return p, O.SgsModel(p)
+
+
+@pytest.mark.parametrize("state_var", [1, 2])
+def test_sgs_nodal_stress_two_hidden_layers(state_var):
+    """The fused nodal closure with a second hidden layer (the context's num_layers = 2 extension) against the oracle."""
+    _say(f"sgs nodal stress 6-20-20-6 var={state_var}")
+    m2 = _two_layer_model()
+    case, om, oc, qd, prob = _setup(state_var, 450 + state_var, m2, periodic=(True, False, True))
+    q = case["q"]
+    G = O.velocity_gradient_projection(om, oc, qd, q)
+    A, _ = prob.sgs_grid_anisotropy()
+    km = np.zeros(6 * om.nnodes)
+    prob.sgs_nodal_stress(q, G, km)
+    km_ref = O.sgs_nodal(oc.gas, state_var, m2[1], q, G, A.ravel())
+    km_one = O.sgs_nodal(oc.gas, state_var, O.SgsModel(S.load_dd_model(GOLDEN, alpha=ALPHA)), q, G, A.ravel())
+    assert np.linalg.norm(km_ref - km_one) > 0.1 * np.linalg.norm(km_one)  # the second layer changes the stress
+    e2, emax = rel_err(km, km_ref)
+    assert e2 < 1e-11 and emax < 1e-10, (e2, emax)
+    prob.close()
+
+
 @pytest.mark.parametrize("state_var", [0, 1, 2])
 def test_sgs_residual(model, state_var):
     _say(f"sgs residual var={state_var}")
