@@ -398,13 +398,16 @@ def les_study():
 
     out = LES.study(log=lambda m: print(m, file=sys.stderr, flush=True))
     res = {"config": out["config"], "fine_gpu_s": out["fine_gpu_s"], "fine_steps": out["fine_steps"],
-           "training_samples": out["training_samples"], "training_s": out["training_s"], "training_loss": out["training_loss"],
-           "filtered_ke_t_end": out["filtered_ke"][-1], "spectra_t_end": out["spectra_t_end"]}
+           "training_flow_gpu_s": out["training_flow_gpu_s"], "training_samples": out["training_samples"], "training_s": out["training_s"],
+           "training_loss": out["training_loss"], "filtered_ke_t_end": out["filtered_ke"][-1], "spectra_t_end": out["spectra_t_end"]}
     for name in ("no_model", "closure"):
         r = out[name]
         res[name] = {"gpu_s": r["gpu_s"], "spectrum_error_t_end": r["spectrum_error"][-1], "spectrum_error": r["spectrum_error"],
                      "ke_error_t_end": r["ke_error"][-1], "speedup_vs_fine": r["speedup_vs_fine"],
                      "les_error_x_gpu_s": r["les_error_x_gpu_s"], "newton_its": r["newton_its"], "krylov_its": r["krylov_its"]}
+        for k in ("speedup_vs_fine_incl_training", "speedup_vs_fine_incl_training_and_data"):
+            if k in r:
+                res[name][k] = r[k]
     return res
 
 

@@ -312,7 +312,9 @@ hipError_t launch_ijacobian(int P, int Q, int var, bool fp32, const VolumeArgs &
 }
 
 hipError_t launch_rhs(int P, int Q, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t s) {
-#define CALL_RHS(P_, Q_) (launch_op<P_, Q_, nsop::RHSFunction>(a, ctx, s))
+#define CALL_RHS(P_, Q_)                                                                                     \
+  (ctx.stabilization == ns::STAB_NONE ? launch_op<P_, Q_, nsop::RHSFunctionT<false>>(a, ctx, s) \
+                                      : launch_op<P_, Q_, nsop::RHSFunctionT<true>>(a, ctx, s))
   HONEE_PQ_ALL(CALL_RHS)
 #undef CALL_RHS
   return hipErrorInvalidValue;
