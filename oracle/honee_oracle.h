@@ -106,6 +106,17 @@ typedef struct {
 // rows (step, time, 5 monitor sums / volume) every record_every steps
 int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
               const double *cs, const ORKMass *mo, int record_every, double *ke_rows, int *nrows, long long *mass_its);
+// The RHS composite's extra parts: freestream faces (explicit sign) and essential boundary conditions as the box-mesh
+// mirror restates HONEE's DMPlex constraints (src/misc.c:51-95): operator inputs q .* mask + qbc, RHS rows masked, the
+// mass matrix the identity on the constrained entries.  Any member may be null.
+typedef struct {
+  const OFaces      *faces;
+  const OFreestream *freestream;
+  const double      *qdata_sur;
+  const double      *mask, *qbc;  // [5 nnodes]: 0 / the boundary value at constrained entries, 1 / 0 elsewhere
+} ORKExtra;
+int oracle_rk_ex(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
+                 const double *cs, const ORKMass *mo, const ORKExtra *ex, int record_every, double *ke_rows, int *nrows, long long *mass_its);
 // BDF(order) with Newton-GMRES and point-Jacobi; returns nonzero when a nonlinear solve fails
 int oracle_bdf(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, int nsteps, const OBDFOptions *o, OBDFStats *st);
 

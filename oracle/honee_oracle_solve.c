@@ -200,11 +200,34 @@ static void record_row(const OMesh *m, const ONewt *c, const double *qdata, cons
   for (int k = 0; k < 5; k++) row[2 + k] = sums[k] / vol;
 }
 
+// The mass action with essential boundary conditions: y = M(q) (x .* mask), then the identity on the constrained
+// entries (the mirror's restatement of HONEE's DMPlex-constrained mass matrix)
+typedef struct {
+  MassCtx       mc;
+  const double *mask;
+  double       *xm;
+  int           n;
+} MassEssCtx;
+
+static void mass_apply_ess(void *ctx, const double *x, double *y) {
+  MassEssCtx *me = ctx;
+  for (int l = 0; l < me->n; l++) me->xm[l] = x[l] * me->mask[l];
+  mass_apply(&me->mc, me->xm, y);
+  for (int l = 0; l < me->n; l++) y[l] = y[l] * me->mask[l] + (1.0 - me->mask[l]) * x[l];
+}
+
 int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
               const double *cs, const ORKMass *mo, int record_every, double *ke_rows, int *nrows, long long *mass_its) {
-  const int    n   = 5 * m->nnodes;
-  const double vol = oracle_volume(m, qdata);
-  double     **K = malloc(sizeof(double *) * s), *Y = malloc(sizeof(double) * n), *R = malloc(sizeof(double) * n), *dinv = NULL;
+  return oracle_rk_ex(m, c, qdata, q, t0, dt, nsteps, s, A, b, cs, mo, NULL, record_every, ke_rows, nrows, mass_its);
+}
+
+int oracle_rk_ex(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, double dt, int nsteps, int s, const double *A, const double *b,
+                 const double *cs, const ORKMass *mo, const ORKExtra *ex, int record_every, double *ke_rows, int *nrows, long long *mass_its) {
+  const int     n    = 5 * m->nnodes;
+  const double  vol  = oracle_volume(m, qdata);
+  const double *mask = ex ? ex->mask : NULL;
+  double      **K = malloc(sizeof(double *) * s), *Y = malloc(sizeof(double) * n), *R = malloc(sizeof(double) * n), *dinv = NULL;
+  double       *Yin = malloc(sizeof(double) * n), *xm = malloc(sizeof(double) * n);
   for (int i = 0; i < s; i++) K[i] = malloc(sizeof(double) * n);
   double    t     = t0;
   int       rows  = 0;
@@ -220,24 +243,38 @@ int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t
           for (int l = 0; l < n; l++) Y[l] += wj * K[j][l];
       }
       c->time = t + dt * cs[i];
-      oracle_rhs(m, c, qdata, Y, R);
-      MassCtx mc = {m, c, qdata, Y};
+      // the operators' input: boundary values inserted at the constrained entries (UpdateBoundaryValues)
+      if (mask)
+        for (int l = 0; l < n; l++) Yin[l] = Y[l] * mask[l] + ex->qbc[l];
+      else memcpy(Yin, Y, sizeof(double) * n);
+      oracle_rhs(m, c, qdata, Yin, R);
+      if (ex && ex->faces && ex->faces->nfaces > 0) oracle_freestream(m, ex->faces, ex->freestream, ex->qdata_sur, Yin, R, NULL);
+      if (mask)
+        for (int l = 0; l < n; l++) R[l] *= mask[l];
+      MassCtx    mc = {m, c, qdata, Yin};
+      MassEssCtx me = {mc, mask, xm, n};
       if (!dinv) {  // PCSetUp at the first solve
         dinv = malloc(sizeof(double) * n);
         if (mo->jacobi_rowsum) {
           double *ones = malloc(sizeof(double) * n);
           for (int l = 0; l < n; l++) ones[l] = 1.0;
-          mass_apply(&mc, ones, dinv);
+          if (mask) mass_apply_ess(&me, ones, dinv);
+          else mass_apply(&mc, ones, dinv);
           free(ones);
         } else {
-          oracle_mass_newtonian_diagonal(m, c, qdata, Y, dinv);
+          oracle_mass_newtonian_diagonal(m, c, qdata, Yin, dinv);
+          if (mask)
+            for (int l = 0; l < n; l++) dinv[l] = dinv[l] * mask[l] + (1.0 - mask[l]);
         }
         jacobi_invert(n, dinv);
       }
       const Prec pc = {dinv, 1};
-      if (mo->gmres) total += gmres(n, mass_apply, &mc, &pc, R, K[i], mo->rtol, 1e-50, mo->restart, mo->max_it);
-      else
+      if (mo->gmres) {
+        if (mask) total += gmres(n, mass_apply_ess, &me, &pc, R, K[i], mo->rtol, 1e-50, mo->restart, mo->max_it);
+        else total += gmres(n, mass_apply, &mc, &pc, R, K[i], mo->rtol, 1e-50, mo->restart, mo->max_it);
+      } else {
         for (int l = 0; l < n; l++) K[i][l] = dinv[l] * R[l];
+      }
     }
     for (int i = 0; i < s; i++) {
       const double ws = dt * b[i];
@@ -253,6 +290,8 @@ int oracle_rk(const OMesh *m, ONewt *c, const double *qdata, double *q, double t
   free(K);
   free(Y);
   free(R);
+  free(Yin);
+  free(xm);
   free(dinv);
   return 0;
 }

@@ -518,16 +518,37 @@ def tableau_ref(name: str, stages: int = 4):
     return A, b, A.sum(axis=1)
 
 
+class ORKExtra(ct.Structure):
+    _fields_ = [("faces", ct.c_void_p), ("freestream", ct.c_void_p), ("qdata_sur", _dp), ("mask", _dp), ("qbc", _dp)]
+
+
 def rk(m: Mesh, c: ONewt, qdata, q, dt, nsteps, A, b, cc, t0=0.0, gmres=False, rowsum=True, rtol=1e-5, restart=30, max_it=10000,
-       record_every=0):
+       record_every=0, faces=None, freestream=None, qdata_sur=None, essential=None):
+    """Explicit RK with the mass KSP.  Optional: freestream faces (Faces, OFreestream with the explicit sign, their
+    surface qdata) added to the RHS, and essential boundary conditions essential = (dofs, values)."""
     q = np.array(q, dtype=np.float64)
     A, b, cc = (np.ascontiguousarray(x, dtype=np.float64) for x in (A, b, cc))
     mo = ORKMass(int(gmres), int(rowsum), rtol, restart, max_it)
     rows = np.zeros((nsteps + 2, 7)) if record_every else None
     nrows = ct.c_int(0)
     its = ct.c_longlong(0)
-    lib().oracle_rk(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(t0), ct.c_double(dt), nsteps, len(b), _d(A), _d(b), _d(cc),
-                    ct.byref(mo), max(record_every, 1), _d(rows) if rows is not None else None, ct.byref(nrows), ct.byref(its))
+    ex, keep = None, []
+    if faces is not None or essential is not None:
+        mask = qbc = None
+        if essential is not None:
+            dofs, values = essential
+            mask, qbc = np.ones(q.size), np.zeros(q.size)
+            mask[np.asarray(dofs)] = 0.0
+            qbc[np.asarray(dofs)] = values
+            keep += [mask, qbc]
+        qds = np.ascontiguousarray(qdata_sur, dtype=np.float64) if faces is not None else None
+        keep.append(qds)
+        ex = ORKExtra(ct.cast(ct.byref(faces.s), ct.c_void_p) if faces is not None else None,
+                      ct.cast(ct.byref(freestream), ct.c_void_p) if faces is not None else None,
+                      _d(qds) if qds is not None else None, _d(mask) if mask is not None else None, _d(qbc) if qbc is not None else None)
+    lib().oracle_rk_ex(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(t0), ct.c_double(dt), nsteps, len(b), _d(A), _d(b), _d(cc),
+                       ct.byref(mo), ct.byref(ex) if ex is not None else None, max(record_every, 1), _d(rows) if rows is not None else None,
+                       ct.byref(nrows), ct.byref(its))
     return q, (rows[: nrows.value] if rows is not None else None), its.value
 
 

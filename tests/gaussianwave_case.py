@@ -1,0 +1,96 @@
+"""The reference run behind tests/output/gaussianwave-explicit.bin (examples/navierstokes.c:233, "Gaussian Wave,
+explicit, supg, IDL"), shared by the HIP-path test and the oracle test: examples/gaussianwave.yaml on a 2 x 2 x 1 box
+[0,1] x [0,1] x [0,0.025], degree 3, 5 explicit steps of dt 2e-3 with TSRK5F (src/setupts.c:354-355; adapt none),
+conservative variables, SUPG with the default Tau coefficients of NS_NEWTONIAN_IG at degree 3
+(problems/newtonian.c:588-610), mu 0, the internal damping layer (decay time 2e-3, start 0, length 0.25, pressure 70),
+freestream faces with the default HLLC flux on the four lateral sides (face sets 3-6) against the reference state
+(pressure 71.75, velocity (2, 2, 0), temperature 0.25), essential z-momentum on the two z sides (face sets 1-2), and
+the consistent mass solved by GMRES with diagonal Jacobi (KSP defaults: rtol 1e-5, restart 30).
+
+The initial condition restates IC_GaussianWave (qfunctions/gaussianwave.h:20-53) here, in the tests only.
+
+The fixture is a PETSc global vector: DMPlex point order, each node's components together, and the constrained
+z-momentum of the 98 nodes on the z sides left out (882 = 196 x 5 - 98).  Since the constrained component sits between
+the y-momentum (about 2) and the energy (about 180) and the free z-momentum is of order 1e-9, the records split
+unambiguously into nodes of 5 and of 4 values; they are matched to this build's nodes by a minimum-cost assignment
+(order-invariant), constrained records to the z-side nodes only.  The pass criterion is HONEE's RegressionTest metric
+(src/misc.c:98-126) max|Q - Qref| / max|Qref| against the test's tolerance 1e-8."""
+import os
+
+import numpy as np
+from scipy.optimize import linear_sum_assignment
+
+from pyhonee import honee_file
+from pyhonee import mesh as M
+from pyhonee import problems as PR
+
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "gaussianwave-explicit.bin")
+GAS = {"cv": 717.0, "cp": 1004.0, "lambda": -2.0 / 3.0, "mu": 0.0, "k": 0.02638}  # NS_NEWTONIAN_IG defaults, mu from the YAML
+REF_Y = np.array([71.75, 2.0, 2.0, 0.0, 0.25])  # reference pressure, velocity, temperature (examples/gaussianwave.yaml)
+EPICENTER, AMPLITUDE, WIDTH = (0.33, 0.75), 2.0, 0.05
+DT, STEPS = 2e-3, 5
+TOLERANCE = 1e-8  # -compare_final_state_atol (examples/navierstokes.c:233)
+
+
+def gaussian_wave_ic(xyz):
+    """IC_GaussianWave in conservative variables: density and total energy scaled by the Gaussian perturbation, the
+    momentum following the density at the reference velocity, and the kinetic energy of the unperturbed reference state
+    (as the reference writes it)."""
+    R = GAS["cp"] - GAS["cv"]
+    gamma = GAS["cp"] / GAS["cv"]
+    rho_inf = REF_Y[0] / (R * REF_Y[4])
+    u = REF_Y[1:4]
+    e_kinetic = 0.5 * rho_inf * float(u @ u)
+    x0, y0 = xyz[:, 0] - EPICENTER[0], xyz[:, 1] - EPICENTER[1]
+    pert = 1 + AMPLITUDE * np.exp(-(x0 * x0 + y0 * y0) / (2 * WIDTH * WIDTH))
+    U = np.zeros((xyz.shape[0], 5))
+    U[:, 0] = rho_inf * pert
+    U[:, 1:4] = u[None, :] * U[:, 0:1]
+    U[:, 4] = REF_Y[0] / (gamma - 1) * pert + e_kinetic
+    return U
+
+
+def setup():
+    """Mesh, basis tables, context dict, lateral faces (element, face id), essential dofs and the initial state."""
+    mesh = M.box_mesh((2, 2, 1), 3, lengths=(1.0, 1.0, 0.025), periodic=(False, False, False))
+    tab = M.tabulate(3, 0)
+    d = {"gas": dict(GAS), "tau": PR.default_tau(3), "state_var": 0, "stabilization": 2, "is_implicit": False, "dt": DT,
+         "idl_enable": True, "idl_amplitude": 1.0 / 2e-3, "idl_start": 0.0, "idl_length": 0.25, "idl_pressure": 70.0}
+    fe, fi = M.box_boundary(mesh, [0, 1, 2, 3])
+    ess = M.essential_dofs(mesh, [4, 5], [3])
+    q0 = PR.nodal_field(mesh, gaussian_wave_ic).ravel()
+    return mesh, tab, d, (fe, fi), ess, q0
+
+
+def _records(vals):
+    """Split the global vector into node records of 5 values, or of 4 with the z-momentum left out."""
+    recs, constrained, i = [], [], 0
+    while i < len(vals):
+        if abs(vals[i + 3]) < 1.0:
+            recs.append(vals[i:i + 5])
+            constrained.append(False)
+            i += 5
+        else:
+            recs.append(np.array([vals[i], vals[i + 1], vals[i + 2], 0.0, vals[i + 3]]))
+            constrained.append(True)
+            i += 4
+    return np.array(recs), np.array(constrained)
+
+
+def compare(mesh, q, q0, t):
+    """RegressionTest metric of the state q against the fixture (order-invariant match), and the initial state's."""
+    vals, step, t_ref = honee_file.read_vec(FIXTURE)
+    assert step == STEPS and abs(t_ref - STEPS * DT) < 1e-15 and abs(t - t_ref) < 1e-15
+    ref, constrained = _records(vals)
+    zside = np.zeros(mesh.nnodes, dtype=bool)
+    zside[M.boundary_nodes(mesh, [4, 5])] = True
+    mine = np.asarray(q).reshape(-1, 5)
+    assert ref.shape[0] == mesh.nnodes and constrained.sum() == zside.sum() == 98
+    assert np.all(mine[zside, 3] == 0.0)
+    scale = np.abs(vals).max()  # max|Qref|
+    cost = np.abs(ref[:, None, :] - mine[None, :, :]).max(-1) / scale
+    cost[constrained[:, None] != zside[None, :]] = 1e3
+    r, c = linear_sum_assignment(cost)
+    err = float(np.abs(ref[r] - mine[c]).max() / scale)
+    err0 = float(np.abs(ref[r] - np.asarray(q0).reshape(-1, 5)[c]).max() / scale)
+    return err, err0
