@@ -1271,13 +1271,13 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
   }
 }
 
-// The point blocks of the 4 x 4 x 4 element go through the element-ordered E-vector (25 doubles per element node of
-// workspace) and a per-node gather that adds them to the output in a fixed order: deterministic, and measured 2.4 ->
-// 1.0 ms at 32^3 p=3 against 25 FP64 atomics per element node.  The Jacobi diagonal (5 atomics per element node) and
-// the point blocks of other elements add atomically (the diagonal measured 0.62 ms atomic, 0.68 ms two-pass).
+// The point blocks go through the element-ordered E-vector (25 doubles per element node of workspace) and a per-node
+// gather that adds them to the output in a fixed order: deterministic, and measured 2.4 -> 1.0 ms at 32^3 p=3 against
+// 25 FP64 atomics per element node.  The Jacobi diagonal of the volume operators (5 atomics per element node) adds
+// atomically (measured 0.62 ms atomic, 0.68 ms two-pass).
 static int diag_evec_begin(Ceed ceed, OpPlan *p, VolumeArgs &a, bool point_block) {
   a.ev = nullptr;
-  if (!point_block || !(p->P == 4 && p->Q == 4)) return 0;
+  if (!point_block) return 0;
   MI_CALL(rstr_transpose_map(p->r_q));
   return ceed_workspace(ceed, (size_t)p->nelem * p->r_q->elem_size * 25, &a.ev);
 }
@@ -1285,6 +1285,20 @@ static int diag_evec_finish(Ceed ceed, OpPlan *p, const VolumeArgs &a) {
   if (!a.ev) return 0;
   const CeedElemRestriction r = p->r_q;
   MI_HIP(ceed, launch_node_gather_blocks(r->t_nrows, r->d_t_rowptr, r->d_t_index, r->d_t_offset, a.ev, a.v, ceed->stream));
+  return 0;
+}
+
+// The diagonals of the projection and filter operators (Mass_N, the filter LHS) through the node-sorted E-vector and
+// the fixed-order node gather, as their applies do: the Jacobi preconditioners of those CG solves are then bitwise
+// reproducible, and so are the solves.
+static int field_diag_evec_begin(Ceed ceed, OpPlan *p, CeedElemRestriction r, FieldArgs &a) {
+  MI_CALL(rstr_transpose_map(r));
+  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * r->elem_size * p->ncomp_out, &a.ev));
+  a.perm = r->d_t_perm;
+  return 0;
+}
+static int field_diag_evec_finish(Ceed ceed, OpPlan *p, CeedElemRestriction r, const FieldArgs &a) {
+  MI_HIP(ceed, launch_node_gather(r->t_nrows, p->ncomp_out, r->comp_stride, r->d_t_rowptr, nullptr, r->d_t_offset, a.ev, a.v, false, ceed->stream));
   return 0;
 }
 
@@ -1307,9 +1321,11 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     a.qd         = p->qd;
     const int PQ = p->P * p->Q;
     a.bt         = BasisTables{p->b_out->d_interp_1d, p->b_out->d_collo, p->b_out->d_prod, p->b_out->d_prod + PQ, p->b_out->d_prod + 2 * PQ};
+    MI_CALL(field_diag_evec_begin(ceed, p, p->r_out, a));
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "mass_n_diagonal", &ev));
     MI_HIP(ceed, launch_mass_n_diagonal(p->P, p->Q, p->ncomp_out, a, ceed->stream));
+    MI_CALL(field_diag_evec_finish(ceed, p, p->r_out, a));
     MI_CALL(timing_end(ceed, "mass_n_diagonal", ev));
     return 0;
   }
@@ -1328,9 +1344,11 @@ int op_plan_diagonal(CeedOperator op, CeedVector diag, bool point_block) {
     a.qd         = p->qd;
     const int PQ = p->P * p->Q;
     a.bt         = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ};
+    MI_CALL(field_diag_evec_begin(ceed, p, p->r_q, a));
     hipEvent_t ev;
     MI_CALL(timing_begin(ceed, "diff_filter_lhs_diagonal", &ev));
     MI_HIP(ceed, launch_diff_filter_lhs_diagonal(p->P, p->Q, p->ncomp_out, a, fctx, ceed->stream));
+    MI_CALL(field_diag_evec_finish(ceed, p, p->r_q, a));
     MI_CALL(timing_end(ceed, "diff_filter_lhs_diagonal", ev));
     return 0;
   }

@@ -205,9 +205,16 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_pblock_
         pb.add_staged(W[5], sBB, sGB, sBB, r);
         pb.add_staged(W[6], sBB, sBB, sGB, r);
       }
-      if (ok && t < Dm::P3)
+      if (ok && t < Dm::P3) {
+        if (a.ev) {  // two-pass: the element node's 5 x 5 block in the element-ordered E-vector
+          double *dst = a.ev + ((long long)e * Dm::P3 + t) * 25 + ci;
 #pragma unroll
-        for (int co = 0; co < 5; co++) atomicAdd(&a.v[(long long)off * 5 + co * 5 + ci], r[co]);
+          for (int co = 0; co < 5; co++) dst[co * 5] = r[co];
+        } else {
+#pragma unroll
+          for (int co = 0; co < 5; co++) atomicAdd(&a.v[(long long)off * 5 + co * 5 + ci], r[co]);
+        }
+      }
     }
   }
 }

@@ -316,6 +316,18 @@ __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_field(
   }
 }
 
+// A diagonal entry r of element node enode in each of the ncomp components: into the node-sorted E-vector (two-pass,
+// summed per node in a fixed order by launch_node_gather) or added atomically
+__device__ __forceinline__ void diag_out(const FieldArgs &a, long long enode, int ncomp, double r) {
+  if (a.ev) {
+    double *dst = a.ev + (long long)a.perm[enode] * ncomp;
+    for (int c = 0; c < ncomp; c++) dst[c] = r;
+  } else {
+    const int off = a.out_offsets[enode];
+    for (int c = 0; c < ncomp; c++) atomicAdd(&a.v[off + c * a.out_comp_stride], r);
+  }
+}
+
 // Diagonal of Mass_N: node n of every element receives sum_q wdetJ(q) B3(q, n)^2 in each of its NC components
 template <int P, int Q>
 __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_mass_n_diagonal(FieldArgs a, int ncomp) {
@@ -334,10 +346,7 @@ __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_mass_n
     const double    w  = ok ? a.qdata[(long long)e * a.qd.se + t] : 0.0;
     basis3::Elem<P, Q> el(nullptr, nullptr, sS[le][0], sS[le][1], sS[le][2], t);
     const double       r = el.contract_t(w, sBB, sBB, sBB);
-    if (ok && t < Dm::P3) {
-      const int off = a.out_offsets[(long long)e * Dm::P3 + t];
-      for (int c = 0; c < ncomp; c++) atomicAdd(&a.v[off + c * a.out_comp_stride], r);
-    }
+    if (ok && t < Dm::P3) diag_out(a, (long long)e * Dm::P3 + t, ncomp, r);
     __syncthreads();
   }
 }
@@ -445,10 +454,7 @@ __global__ void __launch_bounds__(sgs_elems_per_block<Q>() * Q * Q * Q) k_filter
     r += el.contract_t(K[0][1] + K[1][0], sGB, sGB, sBB);
     r += el.contract_t(K[0][2] + K[2][0], sGB, sBB, sGB);
     r += el.contract_t(K[1][2] + K[2][1], sBB, sGB, sGB);
-    if (ok && t < Dm::P3) {
-      const int off = a.out_offsets[(long long)e * Dm::P3 + t];
-      for (int c = 0; c < ncomp; c++) atomicAdd(&a.v[off + c * a.out_comp_stride], r);
-    }
+    if (ok && t < Dm::P3) diag_out(a, (long long)e * Dm::P3 + t, ncomp, r);
     __syncthreads();
   }
 }

@@ -36,3 +36,17 @@ def test_les_study_small():
     # the closure's gain a posteriori: at the final time its spectrum is closer to the filtered fine run's
     # (measured on MI355X: 0.12 against 0.30 without a model)
     assert out["closure"]["spectrum_error"][-1] < out["no_model"]["spectrum_error"][-1]
+
+
+def test_les_study_is_reproducible():
+    """Two runs of the whole pipeline give bitwise identical results: every reduction on the path (operator scatters,
+    diagonal and point-block assembly, projections, filter solves, dot products, training gradients) sums in a fixed
+    order."""
+    _say("les study twice: fine 12^3 -> coarse 6^3")
+    kw = dict(n_fine=12, n_coarse=6, t_end=0.25, n_out=1, coarse_dt=0.05, train_offsets=4, epochs=5, batch=512, log=lambda m: None)
+    a, b = LES.study(**kw), LES.study(**kw)
+    assert a["training_loss"] == b["training_loss"]
+    assert a["filtered_ke"] == b["filtered_ke"]
+    for name in ("no_model", "closure"):
+        assert a[name]["newton_its"] == b[name]["newton_its"] and a[name]["krylov_its"] == b[name]["krylov_its"]
+        assert a[name]["spectrum_error"] == b[name]["spectrum_error"]
