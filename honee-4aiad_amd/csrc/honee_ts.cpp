@@ -435,7 +435,19 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
   if (!std::isfinite(fnorm)) return fail(ts, -3, "non-finite residual at t = %g", tnew);
   const double ttol = std::fmax(o.snes_atol, o.snes_rtol * fnorm0);
   bool         converged = fnorm <= o.snes_atol;
+  double       xnorm_mf  = 0;  // ||x|| of the iterate the finite-difference products are taken at
   auto         apply     = [&](const double *xin, double *yout) -> int {
+    if (o.snes_mf_operator) {  // MatMFFD "wp": J v = (F(x + h v) - F(x)) / h, h = sqrt(eps) sqrt(1 + ||x||) / ||v||
+      double vnorm;
+      TS_CALL(norm2(ts, xin, &vnorm));
+      if (vnorm == 0) return lincomb(ts, yout, 0.0, {0.0}, {xin});
+      const double h = 1.4901161193847656e-08 * std::sqrt(1.0 + xnorm_mf) / vnorm;
+      TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, h}, {ts->x, xin}));  // xtry and Ftry are free until the line search
+      TS_CALL(ifunction_at(ts, tnew, shift, ts->xtry, ts->Ftry));
+      TS_CALL(lincomb(ts, yout, 0.0, {1.0 / h, -1.0 / h}, {ts->Ftry, ts->F}));
+      ts->st.jacobian_applies++;
+      return 0;
+    }
     ts->st.jacobian_applies++;
     TS_OP(ts, HoneeMiApplyIJacobian(ts->p, kMemDevice, shift, xin, yout));
     return 0;
@@ -449,6 +461,7 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
       TS_CALL(invert_diagonal(ts, ts->dinv, block, "Jacobian preconditioner"));
     }
     int kits = 0;
+    if (o.snes_mf_operator) TS_CALL(norm2(ts, ts->x, &xnorm_mf));
     TS_CALL(gmres(ts, apply, Precond{ts, ts->dinv, block}, ts->F, ts->dx, o.ksp_rtol, o.ksp_atol, o.ksp_restart, o.ksp_max_it, &kits));
     ts->st.krylov_its += kits;
     ts->st.last_krylov_its += kits;
@@ -544,6 +557,7 @@ void HoneeMiTSOptionsDefault(HoneeMiTSOptions *o) {
   o->ksp_pc             = 0;
   o->pc_lag             = 1;
   o->mass_pc            = 0;
+  o->snes_mf_operator   = 0;
 }
 
 int HoneeMiTSGetTableau(int rk_type, int ssp_stages, int max_stages, double *A, double *b, double *c, int *num_stages) {

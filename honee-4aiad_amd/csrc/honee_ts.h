@@ -43,6 +43,10 @@ typedef struct {
   // rebuild the Jacobian preconditioner every pc_lag-th Newton iteration of a solve (the first always; PETSc's
   // -snes_lag_preconditioner; 1 rebuilds every iteration)
   int    pc_lag;
+  // PETSc's -snes_mf_operator: the Krylov operator is the finite-difference product of the whole IFunction (MatMFFD,
+  // Walker-Pernice step), the preconditioner still comes from the IJacobian.  Terms the IJacobian leaves out (the
+  // data-driven SGS residual) then enter the Newton direction.
+  int    snes_mf_operator;
 } HoneeMiTSOptions;
 
 typedef struct {

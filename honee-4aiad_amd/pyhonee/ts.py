@@ -24,7 +24,7 @@ class TSOptions(ct.Structure):
         ("mass_max_it", ct.c_int),
         ("snes_rtol", ct.c_double), ("snes_atol", ct.c_double), ("snes_stol", ct.c_double), ("snes_max_it", ct.c_int),
         ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_restart", ct.c_int), ("ksp_max_it", ct.c_int),
-        ("ksp_pc", ct.c_int), ("mass_pc", ct.c_int), ("pc_lag", ct.c_int),
+        ("ksp_pc", ct.c_int), ("mass_pc", ct.c_int), ("pc_lag", ct.c_int), ("snes_mf_operator", ct.c_int),
     ]
 
 

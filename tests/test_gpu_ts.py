@@ -157,6 +157,33 @@ def test_bdf2_fp32_jacobian_against_fp64_oracle():
     prob.close()
 
 
+def test_bdf2_matrix_free_operator_against_oracle():
+    """-snes_mf_operator: GMRES applies the Jacobian as a Walker-Pernice finite difference of the residual while the
+    preconditioner still comes from the assembled point blocks.  Newton converges to the same BDF2 solution as the exact
+    Jacobian (the oracle's) within its tolerance; the differenced products cost one residual each."""
+    _say("ts bdf2 matrix-free operator")
+    dt = 2e-2
+    mesh, tab, d, q0, om, oc, qd = _tgv_case((2, 2, 3), 2, 1, 2, True, dt, perturb=0.08)
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    kw = dict(snes_rtol=1e-10, ksp_rtol=1e-9, ksp_restart=30)
+    opts = TS.default_options(type=TS.TS_BDF, bdf_order=2, dt=dt, ksp_pc=TS.PC_PBJACOBI, **kw)
+    opts.snes_mf_operator = 1
+    ts = TS.TimeStepper(prob, opts)
+    ts.set_solution(q0, 0.0)
+    ts.step(3)
+    q_ref, st_ref = O.bdf(om, oc, qd, q0, dt, 3, order=2, pbjacobi=True, **kw)
+    st = ts.stats()
+    e = _rel(ts.solution(), q_ref)
+    print(f"bdf2 matrix-free operator: rel err {e:.3e}; newton its {st['newton_its']} (oracle {st_ref['newton_its']}); "
+          f"krylov its {st['krylov_its']} (oracle {st_ref['krylov_its']}); final ||F|| {st['last_fnorm']:.3e} from "
+          f"{st['last_fnorm0']:.3e}", flush=True)
+    assert st["steps"] == 3
+    assert st["last_fnorm"] <= 1e-10 * st["last_fnorm0"]
+    assert e < 1e-8
+    ts.close()
+    prob.close()
+
+
 def test_mismatched_integrator_type_is_rejected():
     _say("ts type mismatch")
     mesh, tab, d, q0, om, oc, qd = _tgv_case((2, 2, 2), 1, 0, 0, True, 1e-2)
