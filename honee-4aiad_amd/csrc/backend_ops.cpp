@@ -96,6 +96,18 @@ static bool is_tensor2(CeedBasis b) { return b && b != CEED_BASIS_NONE && b->ten
     if (!(cond)) return MI_ERR(op->ceed, CEED_ERROR_UNSUPPORTED, __VA_ARGS__);      \
   } while (0)
 
+// Whether every two element nodes that share an offset of `out` also read the same offset of each restriction in `ins`
+// (all with out's element nodes): nodal kernels then evaluate each output node once.
+static bool shared_outputs_read_same_inputs(CeedElemRestriction out, std::initializer_list<CeedElemRestriction> ins) {
+  std::vector<int> rowptr, index, toff;
+  group_by_offset(out->h_offsets, rowptr, index, toff);
+  for (size_t r = 0; r + 1 < rowptr.size(); r++)
+    for (int j = rowptr[r] + 1; j < rowptr[r + 1]; j++)
+      for (CeedElemRestriction in : ins)
+        if (in->h_offsets[index[j - 1]] != in->h_offsets[index[j]]) return false;
+  return true;
+}
+
 static int check_field(CeedOperator op, size_t idx, bool input, const char *name, CeedEvalMode emode, CeedInt size) {
   const auto &fields = input ? op->qf->inputs : op->qf->outputs;
   BIND_CHECK(idx < fields.size(), "QFunction %s: missing %s field %zu (%s)", op->qf->name.c_str(), input ? "input" : "output", idx, name);
@@ -316,19 +328,9 @@ static int bind_sgs_op(CeedOperator op, OpPlan *p) {
       const CeedElemRestriction ins[4] = {p->r_q, p->r_aux, p->r_aux2, p->r_im};
       for (CeedElemRestriction r : ins)
         BIND_CHECK(r->num_elem == p->nelem && r->elem_size == esz, "%s: nodal fields must share the element nodes of km_sgs", qf->name.c_str());
-      {  // one closure evaluation per node: the element nodes of a km_sgs node must read the same input nodes
-        const auto                  &oo = p->r_out->h_offsets;
-        std::vector<std::pair<CeedInt, int>> first(oo.size());
-        for (size_t i = 0; i < oo.size(); i++) first[i] = {oo[i], (int)i};
-        std::sort(first.begin(), first.end());
-        for (size_t j = 1; j < first.size(); j++) {
-          if (first[j].first != first[j - 1].first) continue;
-          const int i0 = first[j - 1].second, i1 = first[j].second;
-          for (CeedElemRestriction r : {p->r_q, p->r_aux, p->r_aux2})
-            BIND_CHECK(r->h_offsets[i0] == r->h_offsets[i1], "%s: element nodes sharing a km_sgs node read different input nodes",
-                       qf->name.c_str());
-        }
-      }
+      // one closure evaluation per node: the element nodes of a km_sgs node must read the same input nodes
+      BIND_CHECK(shared_outputs_read_same_inputs(p->r_out, {p->r_q, p->r_aux, p->r_aux2}),
+                 "%s: element nodes sharing a km_sgs node read different input nodes", qf->name.c_str());
       p->kernel_name = "sgs_dd_nodal";
       return 0;
     }
@@ -464,18 +466,8 @@ static int bind_sgs_op(CeedOperator op, OpPlan *p) {
       const CeedElemRestriction ins[5] = {p->r_q, p->r_aux, p->r_in2, p->r_aux2, p->r_im};
       for (CeedElemRestriction r : ins)
         BIND_CHECK(r->num_elem == p->nelem && r->elem_size == esz, "%s: nodal fields must share the element nodes of the output", qf->name.c_str());
-      {
-        const auto                          &oo = p->r_out->h_offsets;
-        std::vector<std::pair<CeedInt, int>> first(oo.size());
-        for (size_t i = 0; i < oo.size(); i++) first[i] = {oo[i], (int)i};
-        std::sort(first.begin(), first.end());
-        for (size_t j = 1; j < first.size(); j++) {
-          if (first[j].first != first[j - 1].first) continue;
-          const int i0 = first[j - 1].second, i1 = first[j].second;
-          for (CeedElemRestriction r : {p->r_q, p->r_aux, p->r_in2, p->r_aux2})
-            BIND_CHECK(r->h_offsets[i0] == r->h_offsets[i1], "%s: element nodes sharing an output node read different input nodes", qf->name.c_str());
-        }
-      }
+      BIND_CHECK(shared_outputs_read_same_inputs(p->r_out, {p->r_q, p->r_aux, p->r_in2, p->r_aux2}),
+                 "%s: element nodes sharing an output node read different input nodes", qf->name.c_str());
       p->kernel_name = "sgs_training_nodal";
       return 0;
     }

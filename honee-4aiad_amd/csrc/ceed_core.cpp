@@ -788,6 +788,27 @@ int rstr_device_offsets(CeedElemRestriction r, const int **d) {
   return 0;
 }
 
+void group_by_offset(const std::vector<CeedInt> &offsets, std::vector<int> &rowptr, std::vector<int> &index, std::vector<int> &toff) {
+  // counting sort on the offset value: a stable pass over the slots keeps each group in slot order
+  const size_t n    = offsets.size();
+  CeedInt      omax = -1;
+  for (CeedInt o : offsets) omax = std::max(omax, o);
+  std::vector<int> start((size_t)omax + 2, 0);
+  for (CeedInt o : offsets) start[(size_t)o + 1]++;
+  rowptr.clear();
+  toff.clear();
+  for (CeedInt v = 0; v <= omax; v++) {
+    if (start[(size_t)v + 1]) {
+      rowptr.push_back(start[v]);
+      toff.push_back(v);
+    }
+    start[(size_t)v + 1] += start[v];
+  }
+  rowptr.push_back((int)n);
+  index.resize(n);
+  for (size_t i = 0; i < n; i++) index[start[offsets[i]]++] = (int)i;
+}
+
 // Transpose of the offsets for the two-pass scatter: L-vector offsets in ascending order (rows), each with the
 // ascending list of element-node indices e * elem_size + j that gather from it; t_perm maps an element node to its
 // slot in that node-sorted order.  Built once per restriction on the host.
@@ -795,21 +816,10 @@ int rstr_transpose_map(CeedElemRestriction r) {
   if (r->d_t_rowptr) return 0;
   if (r->strided) return MI_ERR(r->ceed, CEED_ERROR_BACKEND, "strided restriction has no transpose map");
   if (g_no_alloc) return MI_ERR(r->ceed, CEED_ERROR_UNSUPPORTED, "allocation-free apply: the scatter's transpose map is not built yet");
-  const size_t                         n = r->h_offsets.size();
-  std::vector<std::pair<CeedInt, int>> pairs(n);
-  for (size_t i = 0; i < n; i++) pairs[i] = {r->h_offsets[i], (int)i};
-  std::sort(pairs.begin(), pairs.end());
-  std::vector<int> rowptr, perm(n), index(n), toff;
-  rowptr.reserve(n / 2 + 2);
-  for (size_t i = 0; i < n; i++) {
-    if (i == 0 || pairs[i].first != pairs[i - 1].first) {
-      rowptr.push_back((int)i);
-      toff.push_back(pairs[i].first);
-    }
-    perm[pairs[i].second] = (int)i;
-    index[i]              = pairs[i].second;
-  }
-  rowptr.push_back((int)n);
+  const size_t     n = r->h_offsets.size();
+  std::vector<int> rowptr, perm(n), index, toff;
+  group_by_offset(r->h_offsets, rowptr, index, toff);
+  for (size_t j = 0; j < n; j++) perm[index[j]] = (int)j;
   r->t_nrows      = (int)toff.size();
   r->t_covers_all = (CeedSize)r->t_nrows * r->num_comp == r->l_size;
   const size_t nr = std::max<size_t>(toff.size(), 1);

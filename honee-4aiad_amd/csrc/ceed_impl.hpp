@@ -209,6 +209,9 @@ int  ceed_set_error(Ceed ceed, const char *file, int line, const char *func, int
 
 int vec_device_read(CeedVector v, const double **d);
 int rstr_transpose_map(CeedElemRestriction r);
+// Element-node slots grouped by offset, in increasing offset order, each group listing its slots in increasing order:
+// rows r = 0..toff.size()-1 hold slots index[rowptr[r] .. rowptr[r+1]) of offset toff[r].  O(n + max offset).
+void group_by_offset(const std::vector<CeedInt> &offsets, std::vector<int> &rowptr, std::vector<int> &index, std::vector<int> &toff);
 int ceed_workspace(Ceed ceed, size_t len, double **d);
 int vec_device_write(CeedVector v, double **d, bool keep_contents);
 int vec_device_shadow32(CeedVector v, float **d, bool create);
