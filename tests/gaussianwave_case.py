@@ -94,3 +94,54 @@ def compare(mesh, q, q0, t):
     err = float(np.abs(ref[r] - mine[c]).max() / scale)
     err0 = float(np.abs(ref[r] - np.asarray(q0).reshape(-1, 5)[c]).max() / scale)
     return err, err0
+
+
+# ---- the implicit runs behind tests/output/gaussianwave-IDL.bin and gaussianwave-IDL-entropy.bin ----
+# examples/navierstokes.c:245 ("Gaussian Wave, with IDL": -ts_alpha_radius 0.5, primitive variables from the YAML) and
+# :230 ("Gaussian Wave, IDL and Entropy variables": -state_var entropy, the default radius): examples/gaussianwave.yaml
+# on a 5 x 5 x 1 box at the default degree 1, implicit SUPG, TSALPHA with dt 2e-3 for 5 steps, Newton rtol 1e-4 with
+# the Jacobian lagged 20 iterations across solves, GMRES rtol 1e-2 with the default ILU(0), and the same damping layer,
+# faces and essential z-velocity as the explicit run.  Every node lies on a z side, so the global vector holds 4 values
+# per node in DMPlex vertex order (x fastest, then y, then z: the box mesh's vertex numbering).
+IMPLICIT = {1: ("gaussianwave-IDL.bin", 0.5), 2: ("gaussianwave-IDL-entropy.bin", 1.0)}
+IMPLICIT_TOLERANCE = 2e-11  # -compare_final_state_atol of both tests
+IMPLICIT_SOLVER = dict(rtol=1e-4, ksp_rtol=1e-2, lag=20)
+
+
+def prim_from_cons(U):
+    R = GAS["cp"] - GAS["cv"]
+    rho = U[:, 0]
+    u = U[:, 1:4] / rho[:, None]
+    p = (GAS["cp"] / GAS["cv"] - 1.0) * (U[:, 4] - 0.5 * rho * np.sum(u * u, axis=1))
+    return np.column_stack([p, u, p / (rho * R)])
+
+
+def setup_implicit(state_var):
+    """Mesh, tables, context dict, lateral faces, essential dofs, the initial state in state_var, the global order of
+    the free entries and the global matrix pattern (element-coupled nodes, all free component pairs)."""
+    mesh = M.box_mesh((5, 5, 1), 1, lengths=(1.0, 1.0, 0.025), periodic=(False, False, False))
+    tab = M.tabulate(1, 0)
+    d = {"gas": dict(GAS), "tau": PR.default_tau(1), "state_var": state_var, "stabilization": 2, "is_implicit": True, "dt": DT,
+         "idl_enable": True, "idl_amplitude": 1.0 / 2e-3, "idl_start": 0.0, "idl_length": 0.25, "idl_pressure": 70.0}
+    fe, fi = M.box_boundary(mesh, [0, 1, 2, 3])
+    ess = M.essential_dofs(mesh, [4, 5], [3])
+    U0 = PR.nodal_field(mesh, gaussian_wave_ic)
+    q0 = PR.q_from_prim(GAS, prim_from_cons(U0), state_var).ravel()
+    q0[ess] = 0.0  # the IC's z-velocity component, zero in every variable set
+    free = np.setdiff1d(np.arange(5 * mesh.nnodes), ess)
+    node_adj = np.zeros((mesh.nnodes, mesh.nnodes), dtype=bool)
+    for e in np.asarray(mesh.offsets).reshape(mesh.nelem, -1):
+        node_adj[np.ix_(e, e)] = True
+    pattern = node_adj[np.ix_(free // 5, free // 5)]
+    return mesh, tab, d, (fe, fi), ess, q0, free, pattern
+
+
+def read_implicit(state_var):
+    vals, step, t = honee_file.read_vec(os.path.join(os.path.dirname(__file__), "golden", IMPLICIT[state_var][0]))
+    assert step == STEPS and abs(t - STEPS * DT) < 1e-15
+    return vals
+
+
+def regression_metric(x, ref):
+    """RegressionTest (src/misc.c:98-126): max|Q - Qref| / max|Qref| over the global vector, in order."""
+    return float(np.abs(np.asarray(x) - ref).max() / np.abs(ref).max())
