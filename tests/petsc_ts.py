@@ -43,14 +43,22 @@ class Operators:
     solution-time step dt in its context; it keeps the Jacobian data of that evaluation.
     jacobian(shift, dX_loc) -> dF_loc: the IJacobian composite at that data and shift.
     free: local indices of the global vector's entries, in global order; fixed: local values of the constrained
-    entries (zero time derivative); pattern: the assembled matrix's nonzero structure (COO preallocation keeps every
-    element-coupled entry, zero or not)."""
+    entries (zero time derivative).
 
-    def __init__(self, residual, jacobian, free, fixed, pattern):
+    The Jacobian is either assembled into an AIJ matrix with ILU(0) (pattern: the matrix's nonzero structure, since COO
+    preallocation keeps every element-coupled entry, zero or not), or, with point_blocks(shift) -> [nodes][5][5]
+    (row: residual component, column: state component), HONEE's -amat_type shell with -pc_type vpbjacobi: the MATCEED
+    itself is operator and preconditioner matrix (CreateSolveOperatorsFromMatCeed, src/petsc_ops.c:320-370).  Its
+    action then uses the Jacobian data of the latest IFunction evaluation with the shift of the latest Jacobian
+    evaluation (MatCeedSetContextReal in FormIJacobian_NS), and VPBJACOBI inverts each node's block of free
+    components at that evaluation."""
+
+    def __init__(self, residual, jacobian, free, fixed, pattern=None, point_blocks=None):
         self.residual, self.jacobian = residual, jacobian
         self.free = np.asarray(free)
         self.fixed = np.asarray(fixed, dtype=float)
-        self.pattern = np.asarray(pattern, dtype=bool)
+        self.pattern = None if pattern is None else np.asarray(pattern, dtype=bool)
+        self.point_blocks = point_blocks
 
     def to_local(self, x, fill):
         out = fill.copy()
@@ -66,6 +74,26 @@ class Operators:
             J[:, j] = self.jacobian(shift, e)[self.free]
             e[k] = 0.0
         return J
+
+    def setup(self, shift):
+        """(operator, preconditioner) of one Jacobian evaluation at the given shift, as callables on global vectors."""
+        if self.point_blocks is None:
+            J = self.assemble(shift)
+            LU = ilu0(J, self.pattern)
+            return (lambda y: J @ y), (lambda r: ilu_solve(LU, r))
+        blocks = self.point_blocks(shift)
+        node, comp = self.free // 5, self.free % 5
+        groups = [np.nonzero(node == n)[0] for n in np.unique(node)]
+        inv = [np.linalg.inv(blocks[node[g[0]]][np.ix_(comp[g], comp[g])]) for g in groups]
+        zero = np.zeros(self.fixed.size)
+
+        def pc(r):
+            z = np.empty_like(r)
+            for g, Bi in zip(groups, inv):
+                z[g] = Bi @ r[g]
+            return z
+
+        return (lambda y: self.jacobian(shift, self.to_local(y, zero))[self.free]), pc
 
 
 def ilu0(A, pattern):
@@ -93,14 +121,15 @@ def ilu_solve(LU, b):
     return y
 
 
-def gmres(A, LU, b, rtol, atol=1e-50, restart=30, max_it=10000):
-    """KSPGMRES with left ILU preconditioning and zero initial guess; returns (x, iterations)."""
+def gmres(A, PC, b, rtol, atol=1e-50, restart=30, max_it=10000):
+    """KSPGMRES with left preconditioning (operator A and preconditioner PC as callables) and zero initial guess;
+    returns (x, iterations)."""
     n = b.size
     x = np.zeros(n)
     its = 0
     ttol = None
     while True:
-        r = ilu_solve(LU, b - A @ x) if its else ilu_solve(LU, b)
+        r = PC(b - A(x)) if its else PC(b)
         res = np.linalg.norm(r)
         if ttol is None:
             ttol = max(rtol * res, atol)
@@ -114,7 +143,7 @@ def gmres(A, LU, b, rtol, atol=1e-50, restart=30, max_it=10000):
         k = 0
         done = False
         while k < restart and its < max_it:
-            w = ilu_solve(LU, A @ V[k])
+            w = PC(A(V[k]))
             h = np.array([v @ w for v in V])  # classical Gram-Schmidt: all products first
             w = w - sum(hj * v for hj, v in zip(h, V))
             tt = np.linalg.norm(w)
@@ -150,14 +179,14 @@ class Newton:
         self.rtol, self.atol, self.stol, self.max_it = rtol, atol, stol, max_it
         self.ksp_rtol, self.ksp_restart, self.lag, self.alpha = ksp_rtol, ksp_restart, lag, alpha
         self.jac_iter = 0
-        self.J = self.LU = None
+        self.A = self.PC = None
         self.newton_its = self.linear_its = self.assemblies = self.backtracks = 0
 
     def _linesearch(self, func, x, F, fnorm, y):
         """SNESLineSearchApply_BT (cubic): returns (x_new, F_new, fnorm_new, ynorm, xnorm)."""
         ynorm = np.linalg.norm(y)
         f = fnorm * fnorm
-        initslope = F @ (self.J @ y)
+        initslope = F @ self.A(y)
         if initslope > 0.0:
             initslope = -initslope
         if initslope == 0.0:
@@ -195,8 +224,9 @@ class Newton:
                 g = gnorm * gnorm
         return W, G, gnorm, lam * ynorm, np.linalg.norm(W)
 
-    def solve(self, func, assemble, pattern, x):
-        """Solves func(x) = 0 from the initial guess x; assemble() returns the Jacobian at the last func evaluation."""
+    def solve(self, func, setup, x):
+        """Solves func(x) = 0 from the initial guess x; setup() returns the (operator, preconditioner) of a Jacobian
+        evaluation at the last func evaluation."""
         F = func(x)
         fnorm = np.linalg.norm(F)
         ttol = self.rtol * fnorm
@@ -205,11 +235,10 @@ class Newton:
             raise RuntimeError("non-finite residual")
         if fnorm >= self.atol:
             for i in range(self.max_it):
-                if self.J is None or self.lag <= 1 or (i + self.jac_iter) % self.lag == 0:
-                    self.J = assemble()
-                    self.LU = ilu0(self.J, pattern)
+                if self.A is None or self.lag <= 1 or (i + self.jac_iter) % self.lag == 0:
+                    self.A, self.PC = setup()
                     self.assemblies += 1
-                y, lits = gmres(self.J, self.LU, F, self.ksp_rtol, restart=self.ksp_restart)
+                y, lits = gmres(self.A, self.PC, F, self.ksp_rtol, restart=self.ksp_restart)
                 self.linear_its += lits
                 x, F, fnorm, ynorm, xnorm = self._linesearch(func, x, F, fnorm, y)
                 it = i + 1
@@ -238,7 +267,7 @@ def ts_alpha(ops: Operators, x0_loc, dt, nsteps, radius, newton: Newton, t0=0.0)
             Va = V0 + am * (V1 - V0)
             return ops.residual(ta, h, ops.to_local(Xa, ops.fixed), ops.to_local(Va, zero))[ops.free] / af
 
-        X1 = newton.solve(func, lambda: ops.assemble(shift), ops.pattern, X0.copy())
+        X1 = newton.solve(func, lambda: ops.setup(shift), X0.copy())
         return X1, (X1 - X0) / (gam * h) + (1.0 - 1.0 / gam) * V0
 
     t0_ = [t0]
@@ -268,5 +297,5 @@ def ts_beuler(ops: Operators, x0_loc, dt, nsteps, newton: Newton, t0=0.0):
         def func(X1, X0=X0, t1=t1):
             return ops.residual(t1, dt, ops.to_local(X1, ops.fixed), ops.to_local((X1 - X0) / dt, zero))[ops.free]
 
-        X0 = newton.solve(func, lambda: ops.assemble(1.0 / dt), ops.pattern, X0.copy())
+        X0 = newton.solve(func, lambda: ops.setup(1.0 / dt), X0.copy())
     return X0

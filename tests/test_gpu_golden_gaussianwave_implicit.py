@@ -49,3 +49,41 @@ def test_gaussianwave_idl_reproduces_reference_fixture(state_var):
     _say(f"{GW.IMPLICIT[state_var][0]} regression metric {err:.3e}; Newton {newton.newton_its}, GMRES {newton.linear_its}, "
          f"Jacobian assemblies {newton.assemblies}")
     assert err < GW.IMPLICIT_TOLERANCE
+
+
+def test_gaussianwave_shell_reproduces_reference_fixture():
+    """tests/output/gaussianwave-shell.bin (p=3, primitive, matrix-free Jacobian with point-block Jacobi): the HIP
+    IJacobian apply is the operator and the HIP point-block diagonal assembly of the composite (volume + freestream
+    faces) the preconditioner, both at the lagged Jacobian evaluations."""
+    _say("gaussian wave shell p=3 primitive: reference fixture")
+    mesh, tab, d, (fe, fi), ess, q0, free = GW.setup_shell()
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    prob.add_boundary(prob.FREESTREAM_HLLC, fe, fi, GW.REF_Y)
+    prob.set_essential(ess, q0[ess])
+    g = np.zeros_like(q0)
+    v = np.zeros_like(q0)
+    pb = np.zeros(25 * mesh.nnodes)
+
+    def residual(t, dt, X, V):
+        prob.ifunction(np.ascontiguousarray(X), np.ascontiguousarray(V), g, time=t, dt=dt)
+        return g.copy()
+
+    def jacobian(shift, dq):
+        prob.ijacobian(np.ascontiguousarray(dq), v, shift)
+        return v.copy()
+
+    def point_blocks(shift):
+        prob.ijacobian_point_block_diagonal(pb, shift)
+        return pb.reshape(-1, 5, 5).copy()
+
+    fixed = np.zeros_like(q0)
+    fixed[ess] = q0[ess]
+    ops = PA.Operators(residual, jacobian, free, fixed, point_blocks=point_blocks)
+    newton = PA.Newton(**GW.IMPLICIT_SOLVER)
+    x = PA.ts_alpha(ops, q0, GW.DT, GW.STEPS, GW.SHELL_RADIUS, newton)
+    prob.close()
+    err, err0 = GW.compare_shell(mesh, ops.to_local(x, fixed), q0)
+    _say(f"gaussianwave-shell.bin regression metric {err:.3e} (initial state {err0:.1e}); Newton {newton.newton_its}, "
+         f"GMRES {newton.linear_its}")
+    assert err0 > 1e-3
+    assert err < GW.TOLERANCE
