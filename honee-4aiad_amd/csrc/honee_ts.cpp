@@ -127,6 +127,16 @@ struct HoneeMiTS_ {
   int                   nhist = 0, korder = 1;
   double               *x = nullptr, *xdot = nullptr, *hsum = nullptr, *F = nullptr, *Ftry = nullptr, *dx = nullptr, *xtry = nullptr,
          *dinv = nullptr;
+  // Newton stage residual R(x) = scale F(t, a x + xc, b x + hsum) with the IFunction's "timestep size" ctx_dt
+  struct Stage {
+    double a = 1, b = 0, scale = 1, ctx_dt = 0;
+    bool   xc = false;
+  } stage;
+  double   *v0 = nullptr, *xa = nullptr, *xc = nullptr;  // generalized-alpha: time derivative, stage state, its offset
+  bool      alpha_started = false;
+  long long jac_iter      = 0;  // Newton iterations of earlier solves counted by a persistent Jacobian lag
+  double    jac_shift     = 0;  // shift of the last Jacobian evaluation
+  bool      jac_ready     = false;
   // Krylov
   int     restart = 0;
   double *V = nullptr, *w = nullptr, *partial = nullptr, *d_red = nullptr, *h_red = nullptr;
@@ -393,40 +403,36 @@ void lagrange_values(const double *X, int m, double t, double *L) {
   }
 }
 
-int ifunction_at(HoneeMiTS ts, double tnew, double shift, const double *x, double *F) {
-  TS_CALL(lincomb(ts, ts->xdot, 0.0, {shift, 1.0}, {x, ts->hsum}));
-  TS_OP(ts, HoneeMiApplyIFunction(ts->p, kMemDevice, tnew, ts->o.dt, x, ts->xdot, F));
+// Stage residual R(x) = scale F(tnew, a x + xc, b x + hsum)
+int ifunction_at(HoneeMiTS ts, double tnew, const double *x, double *F) {
+  const HoneeMiTS_::Stage &sg = ts->stage;
+  const double            *xs = x;
+  if (sg.xc) {
+    TS_CALL(lincomb(ts, ts->xa, 0.0, {sg.a, 1.0}, {x, ts->xc}));
+    xs = ts->xa;
+  }
+  TS_CALL(lincomb(ts, ts->xdot, 0.0, {sg.b, 1.0}, {x, ts->hsum}));
+  TS_OP(ts, HoneeMiApplyIFunction(ts->p, kMemDevice, tnew, sg.ctx_dt, xs, ts->xdot, F));
+  if (sg.scale != 1.0) TS_CALL(lincomb(ts, F, sg.scale, {}, {}));
   ts->st.ifunction_evals++;
   return 0;
 }
 
-// Solve G(tnew, x, alpha_0 x + sum_i alpha_i hist_{i-1}) = 0 for x with the history's k most recent solutions
-int bdf_solve(HoneeMiTS ts, double tnew, int k) {
-  double T[4], alpha[4];
-  T[0] = tnew;
-  for (int i = 1; i <= k; i++) T[i] = ts->hist_t[i - 1];
-  bdf_coefficients(T, k, alpha);
-  const double shift = alpha[0];
-  {
-    std::vector<double>         c;
-    std::vector<const double *> xs;
-    for (int i = 1; i <= k; i++) {
-      c.push_back(alpha[i]);
-      xs.push_back(ts->hist[i - 1]);
-    }
-    TS_CALL(lincomb(ts, ts->hsum, 0.0, c, xs));
-  }
-  {  // predictor
-    const int m = std::min(k + 1, ts->nhist);
-    double    L[4];
-    lagrange_values(ts->hist_t.data(), m, tnew, L);
-    std::vector<double>         c(L, L + m);
-    std::vector<const double *> xs(ts->hist.begin(), ts->hist.begin() + m);
-    TS_CALL(lincomb(ts, ts->x, 0.0, c, xs));
-  }
+void set_stage(HoneeMiTS ts, double a, double b, double scale, double ctx_dt, bool xc) {
+  ts->stage.a      = a;
+  ts->stage.b      = b;
+  ts->stage.scale  = scale;
+  ts->stage.ctx_dt = ctx_dt;
+  ts->stage.xc     = xc;
+}
+
+// Newton (SNESNEWTONLS) on the stage residual R(x) = scale F(tnew, a x + xc, b x + hsum) from the initial guess in
+// ts->x.  scale a = 1 in every scheme here, so dR/dx is the IJacobian at shift b / a.
+int newton_solve(HoneeMiTS ts, double tnew) {
+  const double shift = ts->stage.b / ts->stage.a;
   const HoneeMiTSOptions &o = ts->o;
   double                  fnorm, fnorm0;
-  TS_CALL(ifunction_at(ts, tnew, shift, ts->x, ts->F));
+  TS_CALL(ifunction_at(ts, tnew, ts->x, ts->F));
   TS_CALL(norm2(ts, ts->F, &fnorm));
   fnorm0                = fnorm;
   ts->st.last_fnorm0    = fnorm0;
@@ -443,22 +449,27 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
       if (vnorm == 0) return lincomb(ts, yout, 0.0, {0.0}, {xin});
       const double h = 1.4901161193847656e-08 * std::sqrt(1.0 + xnorm_mf) / vnorm;
       TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, h}, {ts->x, xin}));  // xtry and Ftry are free until the line search
-      TS_CALL(ifunction_at(ts, tnew, shift, ts->xtry, ts->Ftry));
+      TS_CALL(ifunction_at(ts, tnew, ts->xtry, ts->Ftry));
       TS_CALL(lincomb(ts, yout, 0.0, {1.0 / h, -1.0 / h}, {ts->Ftry, ts->F}));
       ts->st.jacobian_applies++;
       return 0;
     }
     ts->st.jacobian_applies++;
-    TS_OP(ts, HoneeMiApplyIJacobian(ts->p, kMemDevice, shift, xin, yout));
+    TS_OP(ts, HoneeMiApplyIJacobian(ts->p, kMemDevice, ts->jac_shift, xin, yout));
     return 0;
   };
+  const int lag = std::max(1, o.snes_lag_jacobian);
   for (int it = 1; it <= o.snes_max_it && !converged; it++) {
     const bool block = o.ksp_pc == 1;
-    if ((it - 1) % std::max(1, o.pc_lag) == 0) {
-      if (block) TS_OP(ts, HoneeMiIJacobianPointBlockDiagonal(ts->p, kMemDevice, shift, ts->dinv));
-      else TS_OP(ts, HoneeMiIJacobianDiagonal(ts->p, kMemDevice, shift, ts->dinv));
-      ts->st.diagonal_assemblies++;
-      TS_CALL(invert_diagonal(ts, ts->dinv, block, "Jacobian preconditioner"));
+    if (!ts->jac_ready || (it - 1 + ts->jac_iter) % lag == 0) {  // Jacobian evaluation
+      ts->jac_shift = shift;
+      ts->jac_ready = true;
+      if ((it - 1) % std::max(1, o.pc_lag) == 0) {
+        if (block) TS_OP(ts, HoneeMiIJacobianPointBlockDiagonal(ts->p, kMemDevice, shift, ts->dinv));
+        else TS_OP(ts, HoneeMiIJacobianDiagonal(ts->p, kMemDevice, shift, ts->dinv));
+        ts->st.diagonal_assemblies++;
+        TS_CALL(invert_diagonal(ts, ts->dinv, block, "Jacobian preconditioner"));
+      }
     }
     int kits = 0;
     if (o.snes_mf_operator) TS_CALL(norm2(ts, ts->x, &xnorm_mf));
@@ -472,7 +483,7 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
     double       lambda = 1.0, gnorm = 0;
     for (int bt = 0;; bt++) {
       TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, -lambda}, {ts->x, ts->dx}));
-      TS_CALL(ifunction_at(ts, tnew, shift, ts->xtry, ts->Ftry));
+      TS_CALL(ifunction_at(ts, tnew, ts->xtry, ts->Ftry));
       TS_CALL(norm2(ts, ts->Ftry, &gnorm));
       if (std::isfinite(gnorm) && 0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope) break;
       if (bt >= 40) return fail(ts, -4, "line search failed at t = %g (||F|| = %g)", tnew, fnorm);
@@ -491,7 +502,36 @@ int bdf_solve(HoneeMiTS ts, double tnew, int k) {
   }
   ts->st.last_fnorm = fnorm;
   if (!converged) return fail(ts, -4, "Newton did not converge at t = %g: ||F|| = %g from %g in %d iterations", tnew, fnorm, fnorm0, o.snes_max_it);
+  ts->jac_iter = o.snes_lag_jacobian_persists ? ts->jac_iter + ts->st.last_newton_its : 0;
   return 0;
+}
+
+
+// Solve G(tnew, x, alpha_0 x + sum_i alpha_i hist_{i-1}) = 0 for x with the history's k most recent solutions
+int bdf_solve(HoneeMiTS ts, double tnew, int k) {
+  double T[4], alpha[4];
+  T[0] = tnew;
+  for (int i = 1; i <= k; i++) T[i] = ts->hist_t[i - 1];
+  bdf_coefficients(T, k, alpha);
+  {
+    std::vector<double>         c;
+    std::vector<const double *> xs;
+    for (int i = 1; i <= k; i++) {
+      c.push_back(alpha[i]);
+      xs.push_back(ts->hist[i - 1]);
+    }
+    TS_CALL(lincomb(ts, ts->hsum, 0.0, c, xs));
+  }
+  {  // predictor
+    const int m = std::min(k + 1, ts->nhist);
+    double    L[4];
+    lagrange_values(ts->hist_t.data(), m, tnew, L);
+    std::vector<double>         c(L, L + m);
+    std::vector<const double *> xs(ts->hist.begin(), ts->hist.begin() + m);
+    TS_CALL(lincomb(ts, ts->x, 0.0, c, xs));
+  }
+  set_stage(ts, 1.0, alpha[0], 1.0, ts->o.dt, false);
+  return newton_solve(ts, tnew);
 }
 
 // Push x (time tx) to the front of the history
@@ -530,6 +570,50 @@ int bdf_step(HoneeMiTS ts) {
   return 0;
 }
 
+// ---- backward Euler (TSBEULER: TSTHETA with theta 1) ----
+// F(tprev + h, x, (x - xprev) / h) = 0 from the initial guess xprev, Jacobian shift 1 / h
+int beuler_solve(HoneeMiTS ts, const double *xprev, double tprev, double h) {
+  set_stage(ts, 1.0, 1.0 / h, 1.0, h, false);
+  TS_CALL(lincomb(ts, ts->hsum, 0.0, {-1.0 / h}, {xprev}));
+  TS_CALL(copy(ts, ts->x, xprev));
+  return newton_solve(ts, tprev + h);
+}
+
+int beuler_step(HoneeMiTS ts) {
+  TS_CALL(beuler_solve(ts, ts->q, ts->t, ts->o.dt));
+  TS_CALL(copy(ts, ts->q, ts->x));
+  ts->t += ts->o.dt;
+  return 0;
+}
+
+// ---- generalized-alpha (TSALPHA, Jansen, Whiting and Hulbert 2000) ----
+// With V1 = (X1 - X0) / (gamma dt) + (1 - 1/gamma) V0, Xa = X0 + alpha_f (X1 - X0) and Va = V0 + alpha_m (V1 - V0),
+// the step solves F(t + alpha_f dt, Xa, Va) / alpha_f = 0 for X1 from X0, with Jacobian shift
+// alpha_m / (alpha_f gamma dt); V0 is the previous step's V1.  The first step after a restart computes V0 from two
+// backward-Euler half steps, V0 = (-3 X0 + 4 X(t + dt/2) - X(t + dt)) / dt (TSAlpha_Restart).
+int alpha_step(HoneeMiTS ts) {
+  const double dt = ts->o.dt, rho = ts->o.alpha_radius;
+  const double am = 0.5 * (3.0 - rho) / (1.0 + rho), af = 1.0 / (1.0 + rho), gam = 0.5 + am - af;
+  if (!ts->alpha_started) {
+    const double h = 0.5 * dt;
+    TS_CALL(beuler_solve(ts, ts->q, ts->t, h));
+    TS_CALL(copy(ts, ts->xc, ts->x));
+    TS_CALL(beuler_solve(ts, ts->xc, ts->t + h, h));
+    TS_CALL(lincomb(ts, ts->v0, 0.0, {-3.0 / dt, 4.0 / dt, -1.0 / dt}, {ts->q, ts->xc, ts->x}));
+    ts->alpha_started = true;
+  }
+  const double b = am / (gam * dt);
+  set_stage(ts, af, b, 1.0 / af, dt, true);
+  TS_CALL(lincomb(ts, ts->xc, 0.0, {1.0 - af}, {ts->q}));
+  TS_CALL(lincomb(ts, ts->hsum, 0.0, {1.0 - am / gam, -b}, {ts->v0, ts->q}));
+  TS_CALL(copy(ts, ts->x, ts->q));
+  TS_CALL(newton_solve(ts, ts->t + af * dt));
+  TS_CALL(lincomb(ts, ts->v0, 1.0 - 1.0 / gam, {1.0 / (gam * dt), -1.0 / (gam * dt)}, {ts->x, ts->q}));
+  TS_CALL(copy(ts, ts->q, ts->x));
+  ts->t += dt;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -558,6 +642,9 @@ void HoneeMiTSOptionsDefault(HoneeMiTSOptions *o) {
   o->pc_lag             = 1;
   o->mass_pc            = 0;
   o->snes_mf_operator   = 0;
+  o->alpha_radius       = 1.0;
+  o->snes_lag_jacobian  = 1;
+  o->snes_lag_jacobian_persists = 0;
 }
 
 int HoneeMiTSGetTableau(int rk_type, int ssp_stages, int max_stages, double *A, double *b, double *c, int *num_stages) {
@@ -591,7 +678,7 @@ int HoneeMiTSCreate(HoneeMiProblem p, const HoneeMiTSOptions *o, HoneeMiTS *out)
   };
   const bool explicit_ts = o->type == HONEE_MI_TS_RK;
   if (explicit_ts != (bool)HoneeMiProblemIsExplicit(p))
-    return bail(fail(ts, -5, "time integrator type does not match the problem (explicit RK needs is_implicit false, BDF needs true)"));
+    return bail(fail(ts, -5, "time integrator type does not match the problem (explicit RK needs is_implicit false, BDF, alpha and backward Euler need true)"));
   if (!(o->dt > 0)) return bail(fail(ts, -5, "dt must be positive"));
   if (o->ksp_pc < 0 || o->ksp_pc > 1 || o->mass_pc < 0 || o->mass_pc > 1) return bail(fail(ts, -5, "unknown preconditioner type"));
   void *stream = nullptr;
@@ -628,11 +715,21 @@ int HoneeMiTSCreate(HoneeMiProblem p, const HoneeMiTSOptions *o, HoneeMiTS *out)
         (ierr = dev_alloc(ts, &ts->mass_dinv, o->mass_pc == 1 ? 5 * ts->n : ts->n)))
       return bail(ierr);
   } else {
-    if (o->bdf_order < 1 || o->bdf_order > 2) return bail(fail(ts, -5, "BDF order must be 1 or 2"));
-    ts->hist.assign(o->bdf_order + 1, nullptr);
-    ts->hist_t.assign(o->bdf_order + 1, 0.0);
-    for (auto &hv : ts->hist)
-      if ((ierr = dev_alloc(ts, &hv, ts->n))) return bail(ierr);
+    if (o->type == HONEE_MI_TS_BDF) {
+      if (o->bdf_order < 1 || o->bdf_order > 2) return bail(fail(ts, -5, "BDF order must be 1 or 2"));
+      ts->hist.assign(o->bdf_order + 1, nullptr);
+      ts->hist_t.assign(o->bdf_order + 1, 0.0);
+      for (auto &hv : ts->hist)
+        if ((ierr = dev_alloc(ts, &hv, ts->n))) return bail(ierr);
+    } else if (o->type == HONEE_MI_TS_ALPHA) {
+      if (!(o->alpha_radius >= 0.0 && o->alpha_radius <= 1.0)) return bail(fail(ts, -5, "generalized-alpha radius must be in [0, 1]"));
+      double **abufs[] = {&ts->v0, &ts->xa, &ts->xc};
+      for (double **b : abufs)
+        if ((ierr = dev_alloc(ts, b, ts->n))) return bail(ierr);
+    } else if (o->type != HONEE_MI_TS_BEULER) {
+      return bail(fail(ts, -5, "unknown time integrator type %d", o->type));
+    }
+    if (o->snes_lag_jacobian < 1) return bail(fail(ts, -5, "snes_lag_jacobian must be at least 1"));
     double **bufs[] = {&ts->x, &ts->xdot, &ts->hsum, &ts->F, &ts->Ftry, &ts->dx, &ts->xtry};
     for (double **b : bufs)
       if ((ierr = dev_alloc(ts, b, ts->n))) return bail(ierr);
@@ -648,6 +745,9 @@ int HoneeMiTSSetSolution(HoneeMiTS ts, int mem_type, const double *q, double tim
   ts->t             = time;
   ts->nhist         = 0;
   ts->korder        = 1;
+  ts->alpha_started = false;
+  ts->jac_iter      = 0;
+  ts->jac_ready     = false;
   ts->mass_pc_ready = false;
   ts->st            = HoneeMiTSStats{};
   ts->st.time       = time;
@@ -664,7 +764,12 @@ const double *HoneeMiTSGetSolutionDevice(HoneeMiTS ts) { return ts->q; }
 
 int HoneeMiTSStep(HoneeMiTS ts, int nsteps) {
   for (int i = 0; i < nsteps; i++) {
-    TS_CALL(ts->o.type == HONEE_MI_TS_RK ? rk_step(ts) : bdf_step(ts));
+    switch (ts->o.type) {
+      case HONEE_MI_TS_RK: TS_CALL(rk_step(ts)); break;
+      case HONEE_MI_TS_ALPHA: TS_CALL(alpha_step(ts)); break;
+      case HONEE_MI_TS_BEULER: TS_CALL(beuler_step(ts)); break;
+      default: TS_CALL(bdf_step(ts));
+    }
     ts->st.steps++;
     ts->st.time = ts->t;
     TS_CALL(check_finite(ts, ts->q, "solution"));

@@ -3,9 +3,10 @@
 // :121-156 and MatGetDiagonal_Ceed, src/mat-ceed.c:1603-1630), restated over the device operators:
 //   * explicit Runge-Kutta (TSRK; TSSSP third order) with the "mass_" KSP: preonly + point-Jacobi (row sum or diagonal)
 //     or GMRES + point-Jacobi (src/setuplibceed.c:56-95),
-//   * implicit BDF of order 1 or 2 (TSBDF) with Newton (SNESNEWTONLS, backtracking) and GMRES(restart) left-
-//     preconditioned by point-Jacobi from the IJacobian diagonal; the Jacobian action runs in FP32 when the problem's
-//     resource has jacobian_precision=fp32.
+//   * implicit BDF of order 1 or 2 (TSBDF), the generalized-alpha method (TSALPHA) and backward Euler (TSBEULER) with
+//     Newton (SNESNEWTONLS, backtracking) and GMRES(restart) left-preconditioned by point-Jacobi from the IJacobian
+//     diagonal or point blocks; the Jacobian action runs in FP32 when the problem's resource has
+//     jacobian_precision=fp32.
 // The solution, stage and Krylov vectors live in device memory; every operator apply, vector update and reduction is
 // a HIP launch on the backend's stream, and the host reads back only the scalars the algorithms branch on.
 #pragma once
@@ -16,7 +17,9 @@
 extern "C" {
 #endif
 
-typedef enum { HONEE_MI_TS_RK = 0, HONEE_MI_TS_BDF = 1 } HoneeMiTSType;
+// TSALPHA (-ts_type alpha, examples/gaussianwave.yaml) and TSBEULER (-ts_type beuler, examples/channel.yaml) are the
+// implicit integrators of HONEE's examples beside the default TSBDF (src/setupts.c:343)
+typedef enum { HONEE_MI_TS_RK = 0, HONEE_MI_TS_BDF = 1, HONEE_MI_TS_ALPHA = 2, HONEE_MI_TS_BEULER = 3 } HoneeMiTSType;
 // Explicit schemes: classic 4-stage RK ("4"), Fehlberg 5(4) propagating the 5th-order solution ("5f", HONEE's explicit
 // default), optimal third-order SSP with n^2 stages (TSSSP "rks3", stages given by ssp_stages), Shu-Osher SSP(3,3),
 // forward Euler
@@ -47,6 +50,15 @@ typedef struct {
   // Walker-Pernice step), the preconditioner still comes from the IJacobian.  Terms the IJacobian leaves out (the
   // data-driven SGS residual) then enter the Newton direction.
   int    snes_mf_operator;
+  // TSALPHA's -ts_alpha_radius: spectral radius rho at infinity, alpha_m = (3 - rho) / (2 (1 + rho)),
+  // alpha_f = 1 / (1 + rho), gamma = 1/2 + alpha_m - alpha_f (default 1)
+  double alpha_radius;
+  // PETSc's -snes_lag_jacobian L and -snes_lag_jacobian_persists: the Jacobian is evaluated (the preconditioner rebuilt
+  // and the operator's shift updated) only when (Newton iteration + iterations of earlier solves, if persistent) is a
+  // multiple of L.  As with HONEE's MATCEED Jacobian, the operator between evaluations acts with the latest IFunction's
+  // Jacobian data and the shift of the last evaluation (FormIJacobian_NS, src/setupts.c:129-157).  Default 1, 0.
+  int    snes_lag_jacobian;
+  int    snes_lag_jacobian_persists;
 } HoneeMiTSOptions;
 
 typedef struct {

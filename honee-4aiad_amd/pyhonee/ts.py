@@ -10,7 +10,7 @@ import numpy as np
 from . import LIB_DIR
 from .driver import CEED_MEM_DEVICE, CEED_MEM_HOST, NewtonianProblem, _ptr, load_libs
 
-TS_RK, TS_BDF = 0, 1
+TS_RK, TS_BDF, TS_ALPHA, TS_BEULER = 0, 1, 2, 3
 PC_JACOBI, PC_PBJACOBI = 0, 1
 RK_4, RK_5F, RK_SSP_S3, RK_SSP33, RK_EULER = 0, 1, 2, 3, 4
 
@@ -25,6 +25,7 @@ class TSOptions(ct.Structure):
         ("snes_rtol", ct.c_double), ("snes_atol", ct.c_double), ("snes_stol", ct.c_double), ("snes_max_it", ct.c_int),
         ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_restart", ct.c_int), ("ksp_max_it", ct.c_int),
         ("ksp_pc", ct.c_int), ("mass_pc", ct.c_int), ("pc_lag", ct.c_int), ("snes_mf_operator", ct.c_int),
+        ("alpha_radius", ct.c_double), ("snes_lag_jacobian", ct.c_int), ("snes_lag_jacobian_persists", ct.c_int),
     ]
 
 

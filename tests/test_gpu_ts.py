@@ -184,6 +184,55 @@ def test_bdf2_matrix_free_operator_against_oracle():
     prob.close()
 
 
+def test_alpha_integrator_reproduces_gaussianwave_shell_fixture():
+    """The GPU integrator's TSALPHA (radius 0.5, with its backward-Euler restart) with Newton rtol 1e-4, GMRES rtol 1e-2
+    and point-block Jacobi from the persistent lagged Jacobian (-snes_lag_jacobian 20 -snes_lag_jacobian_persists, the
+    operator keeping the shift of the last evaluation as HONEE's MATCEED does) reproduces the reference's p=3 fixture
+    tests/output/gaussianwave-shell.bin (examples/navierstokes.c:240) within its 1e-8."""
+    import gaussianwave_case as GW
+    _say("ts alpha: gaussianwave-shell fixture")
+    mesh, tab, d, (fe, fi), ess, q0, free = GW.setup_shell()
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    prob.add_boundary(prob.FREESTREAM_HLLC, fe, fi, GW.REF_Y)
+    prob.set_essential(ess, q0[ess])
+    ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_ALPHA, alpha_radius=GW.SHELL_RADIUS, dt=GW.DT, snes_rtol=1e-4, ksp_rtol=1e-2,
+                                                  ksp_pc=TS.PC_PBJACOBI, snes_lag_jacobian=20, snes_lag_jacobian_persists=1))
+    ts.set_solution(q0, 0.0)
+    ts.step(GW.STEPS)
+    q, st = ts.solution(), ts.stats()
+    ts.close()
+    prob.close()
+    err, err0 = GW.compare_shell(mesh, q, q0)
+    print(f"gaussianwave-shell.bin through the GPU integrator: metric {err:.3e} (initial {err0:.1e}); Newton {st['newton_its']}, "
+          f"GMRES {st['krylov_its']}, preconditioner builds {st['diagonal_assemblies']}", flush=True)
+    assert abs(st["time"] - GW.STEPS * GW.DT) < 1e-15
+    assert st["diagonal_assemblies"] == 2  # Newton iterations 0 and 20 across the restart's and the steps' solves
+    assert err0 > 1e-3
+    assert err < GW.TOLERANCE
+
+
+def test_beuler_integrator_reproduces_channel_prim_fixture():
+    """The GPU integrator's TSBEULER at PETSc's default tolerances with point-block Jacobi reproduces the reference's
+    tests/output/channel-prim.bin (examples/navierstokes.c:253; p=1 with q_extra 2, body force, essential walls)
+    within its 2e-11.  The reference preconditions with ILU(0); the conservative fixture channel.bin is reproduced with
+    it (tests/test_gpu_golden_channel.py), and with point-block Jacobi misses it at 1.8e-10."""
+    import channel_case as CH
+    _say("ts beuler: channel-prim fixture")
+    mesh, tab, d, ess, q0, free, _ = CH.setup(1)
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    prob.set_essential(ess, q0[ess])
+    ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_BEULER, dt=CH.DT, ksp_pc=TS.PC_PBJACOBI))
+    ts.set_solution(q0, 0.0)
+    ts.step(CH.STEPS)
+    q, st = ts.solution(), ts.stats()
+    ts.close()
+    prob.close()
+    ref = CH.read(1)
+    err = float(np.abs(q[free] - ref).max() / np.abs(ref).max())
+    print(f"channel-prim.bin through the GPU integrator: metric {err:.3e}; Newton {st['newton_its']}, GMRES {st['krylov_its']}", flush=True)
+    assert err < CH.TOLERANCE
+
+
 def test_mismatched_integrator_type_is_rejected():
     _say("ts type mismatch")
     mesh, tab, d, q0, om, oc, qd = _tgv_case((2, 2, 2), 1, 0, 0, True, 1e-2)
