@@ -30,7 +30,8 @@ HBM_PEAK_GBS = 8000.0
 # the known-traffic kernels of --calibrate-traffic); the counts and commands are in
 # profiles/ifunction_newtonian_tgv32_p3_traffic.md.  A stored figure, not measured by the run that reports it (the
 # line says so); other workloads report traffic null.
-PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass"): 691.1e6}
+# With --qdata recompute the same two runs give 520.6 MB (fused kernel 381.5 MB, node gather 139.1 MB).
+PMC_TRAFFIC_BYTES = {((32, 32, 32), 3, "twopass", "stored"): 691.1e6, ((32, 32, 32), 3, "twopass", "recompute"): 520.6e6}
 METRIC = "GDoF/s Newtonian residual+Jacobian apply (p=3); LES error per GPU-second"
 FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS = 78.6, 157.3
 # Floating-point operations per quadrature point of the QFunction bodies at the default workload (primitive
@@ -422,6 +423,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--scatter", choices=("atomic", "twopass"), default="twopass",
                     help="element scatter-add: FP64 atomics, or the deterministic two-pass node gather")
+    ap.add_argument("--qdata", choices=("recompute", "stored"), default="stored",
+                    help="volume kernels read the quadrature data (default) or recompute it from the element vertices")
     ap.add_argument("--ts-steps", type=int, default=5, help="timed BDF2 steps of the implicit workload (0 skips time integration)")
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit 8^3 p=2 SSP-RK3 run")
     ap.add_argument("--no-les", action="store_true", help="skip the a-posteriori LES study of the learned closure")
@@ -470,7 +473,7 @@ def main():
     shift = 1.0 / 2e-3
 
     results = {}
-    base = f"/gpu/hip/mi355x:scatter={args.scatter}"
+    base = f"/gpu/hip/mi355x:scatter={args.scatter}:qdata={args.qdata}"
     for label, resource in [("fp64", base)] + ([] if args.no_fp32 else [("fp32", base + ":jacobian_precision=fp32")]):
         print(f"# setting up {label} operators", file=sys.stderr, flush=True)
         prob = NewtonianProblem(mesh, tab, ctx, 1, resource)
@@ -563,11 +566,11 @@ def main():
         "ifunction_gdofs": dof / t_if / 1e9,
         "ijacobian_gdofs": dof / t_ij / 1e9,
         "roofline": {"bound": "hbm", "kernel": "ifunction_newtonian", "achieved": b_if / t_if / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((shape, degree, args.scatter)),
+                     "frac": b_if / t_if / 1e9 / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((shape, degree, args.scatter, args.qdata)),
                      "traffic_source": "stored: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
                                        "(profiles/ifunction_newtonian_tgv32_p3_traffic.md), not measured by this run",
                      "algorithmic_bytes": b_if, "avg_apply_us": t_if * 1e6,
-                     "avg_launch_us": t_if_k * 1e6, "avg_node_gather_us": t_gather * 1e6, "scatter": args.scatter,
+                     "avg_launch_us": t_if_k * 1e6, "avg_node_gather_us": t_gather * 1e6, "scatter": args.scatter, "qdata": args.qdata,
                      "apply_repetitions_us": r64["apply_reps_us"]["ifunction_apply"],
                      "timing": "avg_apply_us: back-to-back applies between two HIP events on the backend stream (fused kernel + node "
                                "gather), the fastest of three repetitions; avg_launch_us / avg_node_gather_us: per-kernel events"},

@@ -784,6 +784,19 @@ static int face_contexts(CeedOperator op, int ctx_type, FaceContexts *fc) {
   return 0;
 }
 
+// Recomputed quadrature data (qdata=recompute): the kernel rebuilds {w detJ, dXdx} from the element vertex coordinates
+// when the operator's qdata vector is still the Setup operator's output for these elements, in this strided layout,
+// with no write since (GeoRecord).  P = Q only; otherwise the kernel reads the qdata vector.
+static void geo_args(Ceed ceed, const OpPlan *p, VolumeArgs &a) {
+  a.xe   = nullptr;
+  a.gtab = nullptr;
+  if (!ceed->qdata_recompute || !p->v_qdata || p->P != p->Q) return;
+  const GeoRecord *g = p->v_qdata->geo.get();
+  if (!g || g->nelem != p->nelem || g->Q != p->Q || g->qd.sn != p->qd.sn || g->qd.sc != p->qd.sc || g->qd.se != p->qd.se) return;
+  a.xe   = g->d_xe;
+  a.gtab = g->d_tab;
+}
+
 // Output binding of a scattering volume kernel.  Scatter mode 0 adds into the L-vector with FP64 atomics; mode 1
 // (deterministic two-pass) writes the element-node contributions to the Ceed's E-vector workspace, and
 // scatter_finish sums them per node in a fixed order with launch_node_gather.
@@ -1087,6 +1100,20 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));
       MI_HIP(ceed, launch_setup_qdata(2, p->Q, a, s));
       MI_CALL(timing_end(ceed, p->kernel_name.c_str(), ev));
+      if (ceed->qdata_recompute && !g_no_alloc) {  // the geometry the values came from, for the kernels that recompute them
+        auto g   = std::make_shared<GeoRecord>();
+        g->nelem = p->nelem;
+        g->Q     = p->Q;
+        g->qd    = p->qd;
+        const int Q = p->Q;
+        MI_HIP(ceed, hipMalloc((void **)&g->d_xe, sizeof(double) * 24 * std::max<size_t>((size_t)p->nelem, 1)));
+        MI_HIP(ceed, hipMalloc((void **)&g->d_tab, sizeof(double) * 5 * Q));
+        MI_HIP(ceed, hipMemcpyAsync(g->d_tab, a.Bx, sizeof(double) * 2 * Q, hipMemcpyDeviceToDevice, s));
+        MI_HIP(ceed, hipMemcpyAsync(g->d_tab + 2 * Q, a.Dx, sizeof(double) * 2 * Q, hipMemcpyDeviceToDevice, s));
+        MI_HIP(ceed, hipMemcpyAsync(g->d_tab + 4 * Q, a.w1d, sizeof(double) * Q, hipMemcpyDeviceToDevice, s));
+        if (p->nelem) MI_HIP(ceed, launch_gather_vertex_coords(p->nelem, a.xoffsets, a.x, a.xcomp_stride, g->d_xe, s));
+        out->geo = g;
+      }
       return 0;
     }
     case QfKind::IFunction:
@@ -1105,6 +1132,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
       a.qd = p->qd;
       a.jd = p->jd;
+      geo_args(ceed, p, a);
       const int PQ = p->P * p->Q;
       a.bt = BasisTables{p->b_q->d_interp_1d, p->b_q->d_collo, p->b_q->d_prod, p->b_q->d_prod + PQ, p->b_q->d_prod + 2 * PQ, p->b_q->d_grad_1d};
       if (p->kind == QfKind::IFunction) {
@@ -1163,6 +1191,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
         MI_CALL(rstr_device_offsets(p->r_x, &a.xoffsets));
         MI_CALL(vec_device_read(p->v_x, &a.x));
         a.Bx = p->b_x->d_interp_1d;
+        geo_args(ceed, p, a);
       }
       if (p->kind == QfKind::MassNewtonian) MI_CALL(vec_device_read(p->v_qdot, &a.u2));
       MI_CALL(timing_begin(ceed, p->kernel_name.c_str(), &ev));

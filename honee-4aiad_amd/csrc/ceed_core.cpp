@@ -157,7 +157,7 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
     return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: resource '%s' is not provided by the MI355X library (use /gpu/hip/mi355x)", resource);
   // options: ":key=value" pairs, validated before the device is touched
   int    device = -1, scatter = 1;
-  bool   jac_fp32 = false;
+  bool   jac_fp32 = false, qdata_recompute = false;
   size_t pos      = res.find(':');
   while (pos != std::string::npos) {
     size_t      next = res.find_first_of(":,", pos + 1);
@@ -174,6 +174,10 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
       if (v == "atomic") scatter = 0;
       else if (v == "twopass") scatter = 1;
       else return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: scatter must be twopass or atomic, got '%s'", v.c_str());
+    } else if (k == "qdata") {
+      if (v == "recompute") qdata_recompute = true;
+      else if (v == "stored") qdata_recompute = false;
+      else return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: qdata must be recompute or stored, got '%s'", v.c_str());
     } else if (!k.empty()) {
       return init_error(CEED_ERROR_UNSUPPORTED, "CeedInit: unknown resource option '%s'", k.c_str());
     }
@@ -190,6 +194,7 @@ extern "C" int CeedInit(const char *resource, Ceed *ceed) {
   c->device   = device;
   c->jac_fp32 = jac_fp32;
   c->scatter  = scatter;
+  c->qdata_recompute = qdata_recompute;
   fixed_scratch_setup(c);
   init_err_msg.clear();
   *ceed = c;
@@ -441,7 +446,13 @@ int vec_device_read(CeedVector v, const double **d) {
   return 0;
 }
 
+GeoRecord::~GeoRecord() {
+  if (d_xe) (void)hipFree(d_xe);
+  if (d_tab) (void)hipFree(d_tab);
+}
+
 int vec_device_write(CeedVector v, double **d, bool keep_contents) {
+  v->geo.reset();
   if (keep_contents) {
     if (vec_has_data(v)) MI_CALL(vec_sync_device(v));
     else MI_CALL(vec_alloc_device(v));
@@ -475,6 +486,7 @@ extern "C" int CeedMiVectorGetFp32ShadowRead(CeedVector vec, const float **d_arr
 
 extern "C" int CeedVectorSetArray(CeedVector v, CeedMemType mem_type, CeedCopyMode copy_mode, CeedScalar *array) {
   if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "cannot set array while it is accessed");
+  v->geo.reset();
   if (mem_type == CEED_MEM_HOST) {
     switch (copy_mode) {
       case CEED_COPY_VALUES:
@@ -523,6 +535,7 @@ extern "C" int CeedVectorSyncArray(CeedVector v, CeedMemType mem_type) {
 
 extern "C" int CeedVectorTakeArray(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
   if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "cannot take array while it is accessed");
+  v->geo.reset();
   if (mem_type == CEED_MEM_HOST) {
     if (vec_has_data(v)) MI_CALL(vec_sync_host(v));
     *array     = v->h_array;
@@ -541,6 +554,7 @@ extern "C" int CeedVectorTakeArray(CeedVector v, CeedMemType mem_type, CeedScala
 
 extern "C" int CeedVectorGetArray(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
   if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "vector already accessed");
+  v->geo.reset();
   if (mem_type == CEED_MEM_HOST) {
     if (vec_has_data(v)) MI_CALL(vec_sync_host(v));
     else MI_CALL(vec_alloc_host(v));
@@ -556,6 +570,7 @@ extern "C" int CeedVectorGetArray(CeedVector v, CeedMemType mem_type, CeedScalar
 
 extern "C" int CeedVectorGetArrayWrite(CeedVector v, CeedMemType mem_type, CeedScalar **array) {
   if (v->num_readers || v->writing) return MI_ERR(v->ceed, CEED_ERROR_ACCESS, "vector already accessed");
+  v->geo.reset();
   if (mem_type == CEED_MEM_HOST) {
     MI_CALL(vec_alloc_host(v));
     v->h_valid = true;
@@ -720,6 +735,7 @@ extern "C" int CeedVectorDestroy(CeedVector *vec) {
   if (v->h_owned) free(v->h_array);
   if (v->d_owned) (void)hipFree(v->d_array);
   if (v->d_shadow32) (void)hipFree(v->d_shadow32);
+  v->geo.reset();
   Ceed c = v->ceed;
   delete v;
   CeedDestroy(&c);

@@ -23,6 +23,7 @@ struct Ceed_private {
   hipStream_t      stream   = nullptr;  // null stream unless CeedSetStream
   bool             jac_fp32 = false;
   int              scatter  = 1;  // ScatterMode: 0 atomic, 1 two-pass (deterministic, the default)
+  bool             qdata_recompute = false;  // volume kernels recompute qdata from a GeoRecord (qdata=recompute)
   double          *d_work   = nullptr;  // E-vector workspace of the two-pass scatter
   size_t           work_len = 0;
   CeedErrorHandler handler  = nullptr;
@@ -42,6 +43,18 @@ struct Ceed_private {
 // True on this thread while CeedMiOperatorApplyNoAlloc runs: every helper that would allocate returns an error instead
 extern thread_local bool g_no_alloc;
 
+// Geometry of a qdata vector written by the Setup operator (setupgeo.h:42-56): the element vertex coordinates and the
+// 1D tables of the trilinear coordinate basis it was computed from.  Volume kernels that read this vector as qdata
+// recompute {w detJ, dXdx} from it instead of streaming 10 values per quadrature point (identical values: the setup
+// kernel evaluates the same device function).  Any write to the vector drops the record.
+struct GeoRecord {
+  double *d_xe  = nullptr;  // [nelem][3][8] vertex coordinates, lexicographic vertices
+  double *d_tab = nullptr;  // Bx [Q][2], Dx [Q][2], w [Q]
+  int     nelem = 0, Q = 0;
+  QDataLayout qd{};         // strided layout the values were written in
+  ~GeoRecord();
+};
+
 struct CeedVector_private {
   Ceed     ceed;
   int      refcount = 1;
@@ -54,6 +67,7 @@ struct CeedVector_private {
   bool     writing     = false;
   float   *d_shadow32  = nullptr;  // FP32 mirror (backend-internal; jac_data in FP32 Jacobian mode)
   CeedSize shadow_len  = 0;
+  std::shared_ptr<GeoRecord> geo;  // set by the Setup operator on its qdata output
 };
 
 struct CeedElemRestriction_private {

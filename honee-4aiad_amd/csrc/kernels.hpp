@@ -48,6 +48,10 @@ struct VolumeArgs {
   const int    *xoffsets;  // coordinate restriction [nelem][8]
   const double *x;
   const double *Bx;        // [Q][2]
+  // qdata recomputed from the coordinates (null: read qdata): element vertex coordinates [nelem][3][8] and the
+  // coordinate basis tables {Bx [Q][2], Dx [Q][2], w [Q]}
+  const double *xe;
+  const double *gtab;
 };
 
 // Volume operator whose fields have their own restrictions (SGS closure inputs and residual, Mass_N): an active input
@@ -95,6 +99,7 @@ struct SgsNodalArgs {
   int           p_comp_stride;
 };
 
+hipError_t launch_gather_vertex_coords(int nelem, const int *xoffsets, const double *x, int xcs, double *xe, hipStream_t s);
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream);
 bool       volume_pq_supported(int P, int Q);
 hipError_t launch_ifunction(int P, int Q, int var, const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream);

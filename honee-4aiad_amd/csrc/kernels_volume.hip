@@ -15,52 +15,55 @@ namespace {
 // ------------------------------------------------------------------------------------------------------------------
 template <int PX, int Q>
 __global__ void __launch_bounds__(256) k_setup_qdata(SetupArgs a) {
+  static_assert(PX == 2, "trilinear coordinate field");
   constexpr int   Q3  = Q * Q * Q;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)a.nelem * Q3) return;
   const int e = (int)(gid / Q3), i = (int)(gid % Q3);
-  const int qx = i % Q, qy = (i / Q) % Q, qz = i / (Q * Q);
-  double    J[3][3] = {{0}};
+  double    xv[24], qd[10];
 #pragma unroll
-  for (int c = 0; c < PX; c++)
+  for (int n = 0; n < 8; n++) {
+    const int off = a.xoffsets[(long long)e * 8 + n];
 #pragma unroll
-    for (int b = 0; b < PX; b++)
+    for (int comp = 0; comp < 3; comp++) xv[comp * 8 + n] = a.x[off + comp * a.xcomp_stride];
+  }
+  nsop::geo_point<Q>(xv, a.Bx, a.Dx, a.w1d, i, qd);
 #pragma unroll
-      for (int aa = 0; aa < PX; aa++) {
-        const int    n   = aa + PX * (b + PX * c);
-        const int    off = a.xoffsets[(long long)e * PX * PX * PX + n];
-        const double bx = a.Bx[qx * PX + aa], by = a.Bx[qy * PX + b], bz = a.Bx[qz * PX + c];
-        const double dx = a.Dx[qx * PX + aa], dy = a.Dx[qy * PX + b], dz = a.Dx[qz * PX + c];
-#pragma unroll
-        for (int comp = 0; comp < 3; comp++) {
-          const double xv = a.x[off + comp * a.xcomp_stride];
-          J[comp][0] += dx * by * bz * xv;
-          J[comp][1] += bx * dy * bz * xv;
-          J[comp][2] += bx * by * dz * xv;
-        }
-      }
-  // MatInv3 (qfunctions/utils.h:201-216)
-  double Ai[3][3];
-  Ai[0][0]       = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-  Ai[0][1]       = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-  Ai[0][2]       = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-  Ai[1][0]       = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-  Ai[1][1]       = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-  Ai[1][2]       = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-  Ai[2][0]       = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-  Ai[2][1]       = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-  Ai[2][2]       = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-  const double d = J[0][0] * Ai[0][0] + J[1][0] * Ai[0][1] + J[2][0] * Ai[0][2];
-  const double s = 1 / d;
-  const double w = a.w1d[qx] * a.w1d[qy] * a.w1d[qz];
-  a.qdata[nsop::qidx<Q>(a.qd, e, 0, i)] = w * d;
-#pragma unroll
-  for (int k = 0; k < 9; k++) a.qdata[nsop::qidx<Q>(a.qd, e, 1 + k, i)] = (&Ai[0][0])[k] * s;
+  for (int k = 0; k < 10; k++) a.qdata[nsop::qidx<Q>(a.qd, e, k, i)] = qd[k];
+}
+
+// Element vertex coordinates [nelem][3][8] of the trilinear coordinate field (the GeoRecord of a qdata vector)
+__global__ void __launch_bounds__(256) k_gather_vertex_coords(int nelem, const int *xoffsets, const double *x, int xcs, double *xe) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nelem * 24) return;
+  const long long e = i / 24;
+  const int       r = (int)(i % 24), comp = r / 8, n = r % 8;
+  xe[i] = x[xoffsets[e * 8 + n] + comp * xcs];
 }
 
 // ------------------------------------------------------------------------------------------------------------------
 // Generic fused volume kernel
 // ------------------------------------------------------------------------------------------------------------------
+// Asynchronous copy of the vertex coordinates of element batch bt ([elem][3][8], 192 bytes each) into sXE through the
+// same LDS-DMA path, for the kernels that recompute qdata
+template <int EPB>
+__device__ __forceinline__ void stage_coords_async(const VolumeArgs &a, long long bt, double (*sXE)[24]) {
+  const long long e0    = bt * EPB;
+  const long long ne    = (long long)a.nelem - e0 < EPB ? (long long)a.nelem - e0 : EPB;
+  const int       nb    = (int)(ne * 24 * 8);
+  const char     *src   = (const char *)(a.xe + e0 * 24);
+  char           *dst   = (char *)&sXE[0][0];
+  const int       tid   = threadIdx.x;
+  const int       nt    = blockDim.x;
+  const int       wave0 = tid - (tid & 63);
+  for (int k = 0; k * nt * 16 < nb; k++) {
+    const int chunk = k * nt + tid;
+    if (chunk * 16 < nb)
+      __builtin_amdgcn_global_load_lds((const void *)(src + chunk * 16), (__attribute__((address_space(3))) void *)(dst + (k * nt + wave0) * 16),
+                                       16, 0, 0);
+  }
+}
+
 // The interpolated inputs of every quadrature point and the quadrature-point outputs are staged in LDS (X), one slot
 // per (field component, point).  The basis stages treat the five components of a field together (basis3::Stage), so
 // a 1D coefficient row is read once per pass and five independent contractions are in flight; the quadrature-point
@@ -105,7 +108,10 @@ __device__ __forceinline__ void stage_qdata_async(const VolumeArgs &a, long long
 // Software pipeline over a block's element batches: the next batch's quadrature data is copied into sQD by LDS DMA
 // while the current batch's quadrature-point body runs, and its node values are loaded into registers while the
 // current batch's basis transpose runs, so neither waits on HBM latency at the start of the next batch.
-template <int P, int Q, class Op>
+//
+// With GEO the quadrature data is recomputed from the element's vertex coordinates (192 bytes per element, staged by
+// the same DMA path) instead of read (80 bytes per quadrature point).
+template <int P, int Q, class Op, bool GEO>
 __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(VolumeArgs a, const CeedMiNewtonianContext ctx) {
   using Dm           = Dims<P, Q>;
   constexpr int EPB  = elems_per_block<Q>();
@@ -121,8 +127,12 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   __shared__ double                 sB[Q * P], sDc[Q * Q], sG[Q * P];
   __shared__ double                 sX[EPB][NST][Q3];
   // quadrature data staged through LDS by DMA (Op::kStageQData) or read from global memory at the point body
-  constexpr bool                    SQ = Op::kStageQData;
+  constexpr bool                    SQ = Op::kStageQData && !GEO;
   __shared__ __align__(16) double   sQD[SQ ? EPB : 1][10][SQ ? Q3 : 1];
+  __shared__ __align__(16) double   sXE[GEO ? EPB : 1][24];
+  __shared__ double                 sGT[GEO ? 5 * Q : 1];
+  if constexpr (GEO)
+    for (int i = threadIdx.x; i < 5 * Q; i += blockDim.x) sGT[i] = a.gtab[i];
   static_assert(Dm::P3 <= Q3, "P <= Q is required");
   using Stage = typename StageFor<P, Q, ZCross<Op>::value>::type;
   load_tables<P, Q>(a.bt, sB, sDc, sG);
@@ -137,6 +147,7 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   double un[5] = {0, 0, 0, 0, 0}, u2n[5] = {0, 0, 0, 0, 0};
   if (br.first < br.end) {
     if constexpr (SQ) stage_qdata_async<EPB, Q3>(a, br.first, sQD);
+    if constexpr (GEO) stage_coords_async<EPB>(a, br.first, sXE);
     const long long e_ = br.first * EPB + le;
     if (nd && e_ < a.nelem) {
       off = a.offsets[e_ * Dm::P3 + t];
@@ -177,6 +188,12 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
         for (int k = 0; k < 10; k++) qd[k] = sQD[le][k][t];
         __syncthreads();  // all of sQD has been read: the next batch's copy may overwrite it
         if (more) stage_qdata_async<EPB, Q3>(a, btn, sQD);
+      } else if constexpr (GEO) {
+        __builtin_amdgcn_s_waitcnt(0);  // this batch's vertex coordinates have landed in sXE
+        __syncthreads();
+        nsop::geo_point<Q>(sXE[le], sGT, sGT + 2 * Q, sGT + 4 * Q, t, qd);
+        __syncthreads();  // all of sXE has been read: the next batch's copy may overwrite it
+        if (more) stage_coords_async<EPB>(a, btn, sXE);
       } else {
         nsop::fetch_qdata<Q>(a, e, t, ok, qd);
       }
@@ -238,11 +255,27 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
   }
 }
 
+// Operators and orders whose kernels recompute qdata when the backend passes the element vertices (P = Q); the
+// others read it.  Measured against reading it (back-to-back applies, bench workloads):
+//   * IFunction, Q = 4 (32^3 p=3): 175.9 -> 153.2 us per apply; Q = 5 (96^3 p=4): 19.0 -> 19.7 GDoF/s;
+//   * FP64 IJacobian, Q = 4: 27.0 -> 29.1 GDoF/s; Q = 5: 22.9 -> 22.1 GDoF/s (not adopted);
+//   * FP32 IJacobian, Q = 4: 28.7 -> 26.4 GDoF/s (not adopted): 136 -> 196 VGPRs, three waves per SIMD to two, and
+//     forced to three waves it spills 31 registers;
+//   * explicit RHS, Q = 3 (96^3 p=2 fine run of the LES study): 5.92 -> 6.43 GPU s (not adopted).
+template <class Op, int Q>
+struct RecomputeAt : std::false_type {};
+template <int VAR, int Q>
+struct RecomputeAt<nsop::IFunction<VAR>, Q> : std::integral_constant<bool, Q == 4 || Q == 5> {};
+template <int VAR>
+struct RecomputeAt<nsop::IJacobian<VAR, double>, 4> : std::true_type {};
+
 template <int P, int Q, class Op>
 hipError_t launch_op(const VolumeArgs &a, const CeedMiNewtonianContext &ctx, hipStream_t stream) {
   constexpr int EPB   = elems_per_block<Q>();
   const int     block = EPB * Q * Q * Q;
-  auto          k     = k_volume<P, Q, Op>;
+  auto          k     = k_volume<P, Q, Op, false>;
+  if constexpr (P == Q && RecomputeAt<Op, Q>::value)
+    if (a.xe) k = k_volume<P, Q, Op, true>;
   const int     grid  = grid_for(k, block, ((long long)a.nelem + EPB - 1) / EPB);
   hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, a, ctx);
   return hipGetLastError();
@@ -274,6 +307,12 @@ hipError_t launch_ijacobian_var(int var, bool fp32, const VolumeArgs &a, const C
 }
 
 }  // namespace
+
+hipError_t launch_gather_vertex_coords(int nelem, const int *xoffsets, const double *x, int xcs, double *xe, hipStream_t s) {
+  const long long n = (long long)nelem * 24;
+  hipLaunchKernelGGL(k_gather_vertex_coords, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nelem, xoffsets, x, xcs, xe);
+  return hipGetLastError();
+}
 
 hipError_t launch_setup_qdata(int Px, int Q, const SetupArgs &a, hipStream_t stream) {
   const long long n     = (long long)a.nelem * Q * Q * Q;

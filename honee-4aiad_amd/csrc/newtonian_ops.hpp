@@ -72,6 +72,49 @@ __device__ __forceinline__ void finish_grad_out(T wdetJ, const T acc[3][5], doub
     for (int c = 0; c < 5; c++) gw[d][c] = (double)(wdetJ * acc[d][c]);
 }
 
+// {w detJ, dXdx} of quadrature point t from an element's vertex coordinates xv [3][8] (lexicographic vertices) and the
+// 1D tables of the trilinear coordinate basis (Setup, qfunctions/setupgeo.h:42-56, with MatInv3, qfunctions/utils.h:201-
+// 216).  The setup kernel and the volume kernels that recompute qdata both evaluate this function, so the recomputed
+// values are the stored ones.
+template <int Q>
+__device__ __forceinline__ void geo_point(const double *xv, const double *Bx, const double *Dx, const double *w1d, int t, double qd[10]) {
+  const int qx = t % Q, qy = (t / Q) % Q, qz = t / (Q * Q);
+  double    J[3][3] = {{0}};
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int aa = 0; aa < 2; aa++) {
+        const int    n  = aa + 2 * (b + 2 * c);
+        const double bx = Bx[qx * 2 + aa], by = Bx[qy * 2 + b], bz = Bx[qz * 2 + c];
+        const double dx = Dx[qx * 2 + aa], dy = Dx[qy * 2 + b], dz = Dx[qz * 2 + c];
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+          const double x = xv[comp * 8 + n];
+          J[comp][0] += dx * by * bz * x;
+          J[comp][1] += bx * dy * bz * x;
+          J[comp][2] += bx * by * dz * x;
+        }
+      }
+  double Ai[3][3];
+  Ai[0][0]       = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  Ai[0][1]       = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  Ai[0][2]       = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  Ai[1][0]       = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  Ai[1][1]       = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  Ai[1][2]       = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  Ai[2][0]       = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  Ai[2][1]       = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  Ai[2][2]       = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double d = J[0][0] * Ai[0][0] + J[1][0] * Ai[0][1] + J[2][0] * Ai[0][2];
+  const double s = 1 / d;
+  const double w = w1d[qx] * w1d[qy] * w1d[qz];
+  qd[0]          = w * d;
+#pragma unroll
+  for (int k = 0; k < 9; k++) qd[1 + k] = (&Ai[0][0])[k] * s;
+}
+
 // x coordinate (first component) of the quadrature point from the trilinear coordinate field
 template <int Q>
 __device__ __forceinline__ double coord_x(const VolumeArgs &a, int e, int t) {

@@ -7,6 +7,10 @@
 //   :scatter=twopass|atomic     scatter-add strategy of volume operators: twopass (the default) writes element-node
 //                               contributions and sums them per node in a fixed order (deterministic); atomic adds
 //                               them with FP64 atomics
+//   :qdata=stored|recompute     stored (the default): volume kernels read the Setup operator's quadrature data;
+//                               recompute: the IFunction (P = Q = 4, 5) and FP64 IJacobian (P = Q = 4) kernels rebuild
+//                               it from the element vertex coordinates while the qdata vector holds the Setup
+//                               operator's unmodified output (bitwise-identical results)
 // A CeedInit failure stores its message without printing; CeedGetErrorMessage(NULL, &msg) returns it.
 // The resource name sits under /gpu/hip so HONEE selects device vectors (examples/navierstokes.c:95-103).
 //
