@@ -806,23 +806,36 @@ static void geo_args(Ceed ceed, const OpPlan *p, VolumeArgs &a) {
 // 136 us + gather 50 us, per-kernel events), node-sorted 197 us (166 + 30).
 constexpr bool kSortedEvec = false;
 
-static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a) {
+// fp32: the FP32 Jacobian apply writes its contributions in FP32 (element order), halving the E-vector's round trip;
+// the node gather sums them in FP64.  Measured at 32^3 p=3: see the FP32 IJacobian figures of the bench.
+static int scatter_prepare(OpPlan *p, Ceed ceed, VolumeArgs &a, bool fp32 = false) {
   a.ev   = nullptr;
+  a.ev32 = nullptr;
   a.perm = nullptr;
   if (ceed->scatter != 1) return 0;
   MI_CALL(rstr_transpose_map(p->r_q));
-  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->r_q->elem_size * 5, &a.ev));
+  double *ws;
+  MI_CALL(ceed_workspace(ceed, (size_t)p->nelem * p->r_q->elem_size * 5, &ws));
+  if (fp32 && !kSortedEvec) {
+    a.ev32 = (float *)ws;
+    return 0;
+  }
+  a.ev   = ws;
   a.perm = kSortedEvec ? p->r_q->d_t_perm : nullptr;
   return 0;
 }
 
 static int scatter_finish(OpPlan *p, Ceed ceed, const VolumeArgs &a, bool overwrite) {
-  if (!a.ev) return 0;
+  if (!a.ev && !a.ev32) return 0;
   CeedElemRestriction r = p->r_q;
   hipEvent_t          ev;
   MI_CALL(timing_begin(ceed, "scatter_node_gather", &ev));
-  MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, a.perm ? nullptr : r->d_t_index, r->d_t_offset, a.ev, a.v,
-                                  overwrite, ceed->stream));
+  if (a.ev32)
+    MI_HIP(ceed, launch_node_gather_rows_f32(r->t_nrows, r->comp_stride, r->d_t_rowptr, r->d_t_index, r->d_t_offset, a.ev32, a.v, overwrite,
+                                             ceed->stream));
+  else
+    MI_HIP(ceed, launch_node_gather(r->t_nrows, 5, r->comp_stride, r->d_t_rowptr, a.perm ? nullptr : r->d_t_index, r->d_t_offset, a.ev, a.v,
+                                    overwrite, ceed->stream));
   MI_CALL(timing_end(ceed, "scatter_node_gather", ev));
   return 0;
 }
@@ -1128,7 +1141,7 @@ int op_plan_apply(CeedOperator op, CeedVector in, CeedVector out, bool overwrite
       a.comp_stride = p->r_q->comp_stride;
       MI_CALL(vec_device_read(in, &a.u));
       MI_CALL(vec_device_write(out, &a.v, true));
-      MI_CALL(scatter_prepare(p, ceed, a));
+      MI_CALL(scatter_prepare(p, ceed, a, p->kind == QfKind::IJacobian && ceed->jac_fp32));
       MI_CALL(vec_device_read(p->v_qdata, &a.qdata));
       a.qd = p->qd;
       a.jd = p->jd;

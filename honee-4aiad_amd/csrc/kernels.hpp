@@ -37,6 +37,7 @@ struct VolumeArgs {
   const double *u2;       // second L-vector input (q_dot), if any
   double       *v;        // output L-vector (summed into with atomics when ev is null)
   double       *ev;       // two-pass scatter: element-node contributions [nelem * P^3][5] ([..][25] for point blocks; replaces v)
+  float        *ev32;     // the same contributions in FP32 (element-ordered; the FP32 Jacobian apply), when ev is null
   const int    *perm;     // two-pass scatter: slot of element node e * P^3 + j in ev (null: slot e * P^3 + j)
   const double *qdata;
   QDataLayout   qd;
@@ -147,6 +148,9 @@ hipError_t launch_restriction_notranspose(const int *offsets, int nelem, int ele
 hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset,
                               const double *ev, double *v, bool overwrite, hipStream_t s);
 // The same for 5 x 5 point blocks: v[25 (t_offset[k] / 5) + i] += sum over the node's slots of ev[t_index[j] * 25 + i]
+// Two-pass scatter with an FP32 element-ordered E-vector [elem node][5]: sums per node in FP64, in slot order
+hipError_t launch_node_gather_rows_f32(int nrows, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset, const float *ev,
+                                      double *v, bool overwrite, hipStream_t s);
 hipError_t launch_node_gather_blocks(int nrows, const int *t_rowptr, const int *t_index, const int *t_offset, const double *ev, double *v,
                                      hipStream_t s);
 hipError_t launch_restriction_transpose(const int *offsets, int nelem, int elemsize, int ncomp, int comp_stride, const double *e, double *u, hipStream_t s);

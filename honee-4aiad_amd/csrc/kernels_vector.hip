@@ -135,9 +135,10 @@ __global__ void __launch_bounds__(256) k_node_gather(int nrows, int ncomp, int c
 // The rows are split into 8 contiguous ranges, one per XCD (block b runs on XCD b % 8; the grid is a multiple of 8):
 // the nodes of one range belong to a contiguous range of elements, whose E-vector lines are then fetched into one XCD's
 // L2 instead of into several.
-template <int NC>
+// E is the E-vector's type: double, or float for the FP32 Jacobian's contributions (summed in FP64 here).
+template <int NC, typename E>
 __global__ void __launch_bounds__(256) k_node_gather_rows(int nrows, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
-                                                          const int *__restrict__ toff, const double *__restrict__ ev, double *__restrict__ v,
+                                                          const int *__restrict__ toff, const E *__restrict__ ev, double *__restrict__ v,
                                                           bool overwrite) {
   const long long per = ((long long)nrows + 7) / 8;
   const long long x = blockIdx.x % 8, j = blockIdx.x / 8, nb = gridDim.x / 8;
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(256) k_node_gather_rows(int nrows, int cs, con
 #pragma unroll
       for (int k = 0; k < 8; k++)
 #pragma unroll
-        for (int c = 0; c < NC; c++) x[k][c] = k < m ? ev[(long long)slot[k] * NC + c] : 0.0;
+        for (int c = 0; c < NC; c++) x[k][c] = k < m ? (double)ev[(long long)slot[k] * NC + c] : 0.0;
 #pragma unroll
       for (int k = 0; k < 8; k++)
         if (k < m)
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(256) k_node_gather_rows(int nrows, int cs, con
       for (int k = 0; k < m; k++) {
         const long long sl = index[b + k];
 #pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] += ev[sl * NC + c];
+        for (int c = 0; c < NC; c++) acc[c] += (double)ev[sl * NC + c];
       }
     }
     const long long o = toff[row];
@@ -222,7 +223,8 @@ hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *
   if (n <= 0) return hipSuccess;
   if (t_index && ncomp == 5) {  // element-ordered E-vector: one thread per node, 8 row ranges (one per XCD)
     const int grid = (grid_for(nrows) + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_node_gather_rows<5>), dim3(grid), dim3(kBlock), 0, s, nrows, comp_stride, t_rowptr, t_index, t_offset, ev, v, overwrite);
+    hipLaunchKernelGGL((k_node_gather_rows<5, double>), dim3(grid), dim3(kBlock), 0, s, nrows, comp_stride, t_rowptr, t_index, t_offset, ev, v,
+                       overwrite);
     return hipGetLastError();
   }
   const int grid = grid_for((n + NPT - 1) / NPT);
@@ -232,6 +234,14 @@ hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *
   else
     hipLaunchKernelGGL((k_node_gather<false, NPT>), dim3(grid), dim3(kBlock), 0, s, nrows, ncomp, comp_stride, t_rowptr, t_index, t_offset, ev,
                        v, overwrite);
+  return hipGetLastError();
+}
+
+hipError_t launch_node_gather_rows_f32(int nrows, int comp_stride, const int *t_rowptr, const int *t_index, const int *t_offset, const float *ev,
+                                      double *v, bool overwrite, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  const int grid = (grid_for(nrows) + 7) / 8 * 8;
+  hipLaunchKernelGGL((k_node_gather_rows<5, float>), dim3(grid), dim3(kBlock), 0, s, nrows, comp_stride, t_rowptr, t_index, t_offset, ev, v, overwrite);
   return hipGetLastError();
 }
 

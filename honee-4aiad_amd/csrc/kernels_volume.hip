@@ -240,7 +240,11 @@ __global__ void __launch_bounds__(elems_per_block<Q>() * Q * Q * Q, 2) k_volume(
       double              r[5];
       sto.template transpose<5, Op::kOutV, Op::kOutG>(0, 5, r);
       if (ok && nd) {
-        if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
+        if (a.ev32) {  // two-pass scatter in FP32: element order
+          float *dst = a.ev32 + enode * 5;
+#pragma unroll
+          for (int c = 0; c < 5; c++) dst[c] = (float)r[c];
+        } else if (a.ev) {  // two-pass scatter: element-node contributions, summed per node by a second kernel
           double *dst = a.ev + (a.perm ? (long long)slot : enode) * 5;
 #pragma unroll
           for (int c = 0; c < 5; c++) dst[c] = r[c];
