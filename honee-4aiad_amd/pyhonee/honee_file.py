@@ -54,3 +54,32 @@ def index_bits(path: str) -> int:
     """PetscInt width (32 or 64) of a HONEE-formatted file's header."""
     tok = int(np.frombuffer(open(path, "rb").read(4), ">i4")[0])
     return 64 if tok == TOKEN_64 else 32
+
+
+def global_indices(mesh, essential_dofs=()) -> np.ndarray:
+    """L-vector entries of a PETSc global vector, in its order, for a degree-1 box mesh of this package.
+
+    HONEE's state files hold the DMPlex global vector (src/honee-file.c:196-217): the section's points in chart order
+    with their free components, constrained (essential) components left out.  At degree 1 the dofs sit on the
+    vertices only, and the box mesh numbers its vertices x fastest, then y, then z, as this package's box_mesh numbers
+    its nodes, so the global vector is the L-vector without the essential entries.  Higher degrees add edge, face and
+    cell dofs, whose DMPlex order is not reproduced here: they raise."""
+    if getattr(mesh, "degree", None) != 1:
+        raise NotImplementedError("the DMPlex global order is reproduced for degree-1 box meshes only")
+    return np.setdiff1d(np.arange(5 * mesh.nnodes), np.asarray(essential_dofs, dtype=np.int64))
+
+
+def to_global(q_local, mesh, essential_dofs=()) -> np.ndarray:
+    """The global vector of the L-vector q_local (5 components per node), for write_vec."""
+    return np.asarray(q_local, dtype=np.float64).ravel()[global_indices(mesh, essential_dofs)]
+
+
+def to_local(values, mesh, essential_dofs=(), essential_values=0.0) -> np.ndarray:
+    """The L-vector of a global vector read by read_vec, with the essential entries set to essential_values."""
+    idx = global_indices(mesh, essential_dofs)
+    if np.size(values) != idx.size:
+        raise ValueError(f"global vector of {np.size(values)} values, the mesh and constraints give {idx.size}")
+    q = np.zeros(5 * mesh.nnodes)
+    q[np.asarray(essential_dofs, dtype=np.int64)] = essential_values
+    q[idx] = values
+    return q

@@ -41,7 +41,7 @@ def setup(state_var, q_extra=2):
     ess = np.union1d(M.essential_dofs(mesh, [4, 5], [3]), M.essential_dofs(mesh, [2, 3], [1, 2, 3]))
     Y0 = PR.nodal_field(mesh, lambda xyz: PR.channel_prim(xyz, GAS, umax=UMAX, theta0=THETA0, P0=P0, center=H, H=H))
     q0 = PR.q_from_prim(GAS, Y0, state_var).ravel()
-    free = np.setdiff1d(np.arange(5 * mesh.nnodes), ess)
+    free = honee_file.global_indices(mesh, ess)  # DMPlex global order of the free entries
     node_adj = np.zeros((mesh.nnodes, mesh.nnodes), dtype=bool)
     for e in np.asarray(mesh.offsets).reshape(mesh.nelem, -1):
         node_adj[np.ix_(e, e)] = True

@@ -129,7 +129,7 @@ def setup_implicit(state_var):
     U0 = PR.nodal_field(mesh, gaussian_wave_ic)
     q0 = PR.q_from_prim(GAS, prim_from_cons(U0), state_var).ravel()
     q0[ess] = 0.0  # the IC's z-velocity component, zero in every variable set
-    free = np.setdiff1d(np.arange(5 * mesh.nnodes), ess)
+    free = honee_file.global_indices(mesh, ess)  # DMPlex global order of the free entries
     node_adj = np.zeros((mesh.nnodes, mesh.nnodes), dtype=bool)
     for e in np.asarray(mesh.offsets).reshape(mesh.nelem, -1):
         node_adj[np.ix_(e, e)] = True
