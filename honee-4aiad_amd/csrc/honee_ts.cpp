@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -139,7 +140,11 @@ struct HoneeMiTS_ {
   bool      jac_ready     = false;
   // Krylov
   int     restart = 0;
-  double *V = nullptr, *w = nullptr, *partial = nullptr, *d_red = nullptr, *h_red = nullptr;
+  double *V = nullptr, *w = nullptr, *partial = nullptr, *d_red = nullptr;
+  // Reduction results read by the host: mapped, coherent host memory (kMaxDots values, then a sequence number) that a
+  // kernel stores into; h_* are the host addresses, g_* the device ones
+  double             *h_red = nullptr, *g_red = nullptr;
+  unsigned long long *h_seq = nullptr, *g_seq = nullptr, seq = 0;
   std::vector<void *> allocs;
   std::string         err;
 };
@@ -180,10 +185,20 @@ int dev_alloc(HoneeMiTS ts, double **ptr, long long len) {
   return 0;
 }
 
-// Reads reduction results d_red[0..count) into h_red (waits for the stream)
+// Reads reduction results d_red[0..count) into h_red: a kernel stores them into mapped host memory followed by a new
+// sequence number, which the host polls.  After 2 ms without it the host waits on the stream instead (which also
+// reports a failed launch), so a long queue ahead of the read does not keep a core spinning.
 int read_scalars(HoneeMiTS ts, int count) {
-  TS_HIP(ts, hipMemcpyAsync(ts->h_red, ts->d_red, sizeof(double) * count, hipMemcpyDeviceToHost, ts->s));
-  TS_HIP(ts, hipStreamSynchronize(ts->s));
+  const unsigned long long seq = ++ts->seq;
+  TS_HIP(ts, launch_publish(ts->d_red, count, ts->g_red, ts->g_seq, seq, ts->s));
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != seq) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      TS_HIP(ts, hipStreamSynchronize(ts->s));
+      if (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != seq) return fail(ts, -2, "reduction result not published");
+      break;
+    }
+  }
   return 0;
 }
 
@@ -476,15 +491,20 @@ int newton_solve(HoneeMiTS ts, double tnew) {
     TS_CALL(gmres(ts, apply, Precond{ts, ts->dinv, block}, ts->F, ts->dx, o.ksp_rtol, o.ksp_atol, o.ksp_restart, o.ksp_max_it, &kits));
     ts->st.krylov_its += kits;
     ts->st.last_krylov_its += kits;
-    double ynorm;
-    TS_CALL(norm2(ts, ts->dx, &ynorm));
-    // backtracking on f = ||F||^2 / 2 along -dx with slope -||F||^2
+    // backtracking on f = ||F||^2 / 2 along -dx with slope -||F||^2.  Each trial reads ||F(xtry)||, ||xtry|| (the
+    // step-length test's ||x|| if the trial is accepted) and, at the first trial, ||dx|| in one host read.
     const double initslope = -fnorm * fnorm, alpha_ls = 1e-4;
-    double       lambda = 1.0, gnorm = 0;
+    double       lambda = 1.0, gnorm = 0, xnorm = 0, ynorm = 0;
     for (int bt = 0;; bt++) {
       TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, -lambda}, {ts->x, ts->dx}));
       TS_CALL(ifunction_at(ts, tnew, ts->xtry, ts->Ftry));
-      TS_CALL(norm2(ts, ts->Ftry, &gnorm));
+      TS_HIP(ts, launch_dot(ts->Ftry, ts->Ftry, ts->n, 0, ts->partial, ts->d_red, ts->s));
+      TS_HIP(ts, launch_dot(ts->xtry, ts->xtry, ts->n, 1, ts->partial, ts->d_red, ts->s));
+      if (bt == 0) TS_HIP(ts, launch_dot(ts->dx, ts->dx, ts->n, 2, ts->partial, ts->d_red, ts->s));
+      TS_CALL(read_scalars(ts, bt == 0 ? 3 : 2));
+      gnorm = std::sqrt(ts->h_red[0]);
+      xnorm = std::sqrt(ts->h_red[1]);
+      if (bt == 0) ynorm = std::sqrt(ts->h_red[2]);
       if (std::isfinite(gnorm) && 0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope) break;
       if (bt >= 40) return fail(ts, -4, "line search failed at t = %g (||F|| = %g)", tnew, fnorm);
       ts->st.line_search_backtracks++;
@@ -496,8 +516,6 @@ int newton_solve(HoneeMiTS ts, double tnew) {
     fnorm = gnorm;
     ts->st.newton_its++;
     ts->st.last_newton_its = it;
-    double xnorm;
-    TS_CALL(norm2(ts, ts->x, &xnorm));
     if (fnorm <= ttol || lambda * ynorm <= o.snes_stol * xnorm) converged = true;
   }
   ts->st.last_fnorm = fnorm;
@@ -694,8 +712,12 @@ int HoneeMiTSCreate(HoneeMiProblem p, const HoneeMiTSOptions *o, HoneeMiTS *out)
   if ((ierr = dev_alloc(ts, &ts->q, ts->n))) return bail(ierr);
   if ((ierr = dev_alloc(ts, &ts->partial, (long long)kMaxReductionBlocks * kMaxDots))) return bail(ierr);
   if ((ierr = dev_alloc(ts, &ts->d_red, kMaxDots))) return bail(ierr);
-  if (hipHostMalloc((void **)&ts->h_red, sizeof(double) * kMaxDots, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&ts->h_red, sizeof(double) * (kMaxDots + 1), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     return bail(fail(ts, -2, "hipHostMalloc failed"));
+  ts->h_seq  = reinterpret_cast<unsigned long long *>(ts->h_red + kMaxDots);
+  *ts->h_seq = 0;
+  if (hipHostGetDevicePointer((void **)&ts->g_red, ts->h_red, 0) != hipSuccess) return bail(fail(ts, -2, "hipHostGetDevicePointer failed"));
+  ts->g_seq = reinterpret_cast<unsigned long long *>(ts->g_red + kMaxDots);
   const int restart = explicit_ts ? (o->mass_gmres ? o->mass_restart : 0) : o->ksp_restart;
   if (restart < 0 || restart + 1 > kMaxDots) return bail(fail(ts, -5, "GMRES restart must be at most %d", kMaxDots - 1));
   ts->restart = restart;

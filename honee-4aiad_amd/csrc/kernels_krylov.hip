@@ -200,6 +200,16 @@ __global__ void __launch_bounds__(kThreads) k_pb_invert(double *__restrict__ pb,
   if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
 }
 
+// h[i] = d[i] for i < count, in host memory that is mapped and coherent, then *seq_flag = seq: the host reads h once it
+// sees seq, without a copy or a stream synchronization
+__global__ void __launch_bounds__(kThreads) k_publish(const double *__restrict__ d, int count, double *h, unsigned long long *seq_flag,
+                                                      unsigned long long seq) {
+  if ((int)threadIdx.x < count) h[threadIdx.x] = d[threadIdx.x];
+  __threadfence_system();  // each thread's value reaches host memory before the flag
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(seq_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // z_n = Binv_n (x_n - y_n) per node n (y may be null)
 __global__ void __launch_bounds__(kThreads) k_pb_apply(double *__restrict__ z, const double *__restrict__ pbinv, const double *__restrict__ x,
                                                        const double *__restrict__ y, long long nn) {
@@ -300,6 +310,12 @@ hipError_t launch_pb_invert(double *pb, long long nnodes, int slot, double *part
   const int nb = krylov_reduction_blocks(nnodes);
   hipLaunchKernelGGL(k_pb_invert, dim3(nb), dim3(kThreads), 0, s, pb, nnodes, slot, partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_publish(const double *d, int count, double *h, unsigned long long *seq_flag, unsigned long long seq, hipStream_t s) {
+  if (count < 0 || count > kThreads) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kThreads), 0, s, d, count, h, seq_flag, seq);
   return hipGetLastError();
 }
 

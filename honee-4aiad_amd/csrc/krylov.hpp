@@ -40,5 +40,8 @@ hipError_t launch_set(double *x, double v, long long n, hipStream_t s);
 hipError_t launch_count_nonfinite(const double *x, long long n, int slot, double *partial, double *out, hipStream_t s);
 // In-place inverse of each node's 5x5 block [c_out][c_in]; out[slot] = number of singular blocks
 hipError_t launch_pb_invert(double *pb, long long nnodes, int slot, double *partial, double *out, hipStream_t s);
+// h[i] = d[i] for i < count (count <= 256; h and seq_flag: device addresses of mapped, coherent host memory), then
+// *seq_flag = seq with system-scope release ordering
+hipError_t launch_publish(const double *d, int count, double *h, unsigned long long *seq_flag, unsigned long long seq, hipStream_t s);
 // z_n = Binv_n (x_n - y_n) for each node n (y may be null; 5 interleaved components per node)
 hipError_t launch_pb_apply(double *z, const double *pbinv, const double *x, const double *y, long long nnodes, hipStream_t s);
