@@ -185,17 +185,25 @@ int dev_alloc(HoneeMiTS ts, double **ptr, long long len) {
   return 0;
 }
 
-// Reads reduction results d_red[0..count) into h_red: a kernel stores them into mapped host memory followed by a new
-// sequence number, which the host polls.  After 2 ms without it the host waits on the stream instead (which also
-// reports a failed launch), so a long queue ahead of the read does not keep a core spinning.
-int read_scalars(HoneeMiTS ts, int count) {
-  const unsigned long long seq = ++ts->seq;
-  TS_HIP(ts, launch_publish(ts->d_red, count, ts->g_red, ts->g_seq, seq, ts->s));
+// Host reads of reduction results d_red[0..count): the launch's final reduction kernel stores them into h_red (mapped
+// host memory) followed by a new sequence number (Publish), which wait_published polls.  After 2 ms without it the host
+// waits on the stream instead (which also reports a failed launch), so a long queue ahead of the read does not keep a
+// core spinning.
+Publish next_publish(HoneeMiTS ts, int count) {
+  Publish p;
+  p.h     = ts->g_red;
+  p.flag  = ts->g_seq;
+  p.seq   = ++ts->seq;
+  p.count = count;
+  return p;
+}
+
+int wait_published(HoneeMiTS ts, const Publish &p) {
   const auto t0 = std::chrono::steady_clock::now();
-  while (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != seq) {
+  while (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != p.seq) {
     if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
       TS_HIP(ts, hipStreamSynchronize(ts->s));
-      if (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != seq) return fail(ts, -2, "reduction result not published");
+      if (__atomic_load_n(ts->h_seq, __ATOMIC_ACQUIRE) != p.seq) return fail(ts, -2, "reduction result not published");
       break;
     }
   }
@@ -203,8 +211,9 @@ int read_scalars(HoneeMiTS ts, int count) {
 }
 
 int norm2(HoneeMiTS ts, const double *x, double *out) {
-  TS_HIP(ts, launch_dot(x, x, ts->n, 0, ts->partial, ts->d_red, ts->s));
-  TS_CALL(read_scalars(ts, 1));
+  const Publish pub = next_publish(ts, 1);
+  TS_HIP(ts, launch_dot(x, x, ts->n, 0, ts->partial, ts->d_red, ts->s, pub));
+  TS_CALL(wait_published(ts, pub));
   *out = std::sqrt(ts->h_red[0]);
   return 0;
 }
@@ -249,30 +258,29 @@ int gmres(HoneeMiTS ts, Apply apply, Prec prec, const double *b, double *x, doub
       TS_CALL(apply(x, ts->w));
       TS_CALL(prec(V, b, ts->w));
     }
-    double res;
-    TS_CALL(norm2(ts, V, &res));
+    // ||V_0||, with V_0 normalized on the device while the host reads the norm
+    const Publish pub = next_publish(ts, 1);
+    TS_HIP(ts, launch_dot(V, V, n, 0, ts->partial, ts->d_red, ts->s, pub));
+    TS_HIP(ts, launch_normalize(V, ts->d_red, n, ts->s));
+    TS_CALL(wait_published(ts, pub));
+    double res = std::sqrt(ts->h_red[0]);
     if (its == 0) ttol = std::fmax(rtol * res, atol);
     if (res <= ttol) break;
-    {
-      LinComb none{};
-      TS_HIP(ts, launch_lincomb(V, 1.0 / res, none, n, ts->s));
-    }
     grs[0] = res;
     int it = 0;
     while (it < K && its < maxit) {
       double *v_new = V + (size_t)(it + 1) * n;
       TS_CALL(apply(V + (size_t)it * n, ts->w));
       TS_CALL(prec(v_new, ts->w, nullptr));
-      // classical Gram-Schmidt: all dots first, then one projection pass that also forms ||v_new||^2
+      // classical Gram-Schmidt: all dots first, then one projection pass that also forms ||v_new||^2; v_new is
+      // normalized on the device while the host reads (h_0..h_it, ||v_new||^2)
       TS_HIP(ts, launch_mdot(v_new, V, n, it + 1, n, ts->partial, ts->d_red, ts->s));
-      TS_HIP(ts, launch_project_out(v_new, V, n, it + 1, ts->d_red, n, it + 1, ts->partial, ts->d_red, ts->s));
-      TS_CALL(read_scalars(ts, it + 2));
+      const Publish pub = next_publish(ts, it + 2);
+      TS_HIP(ts, launch_project_out(v_new, V, n, it + 1, ts->d_red, n, it + 1, ts->partial, ts->d_red, ts->s, pub));
+      TS_HIP(ts, launch_normalize(v_new, ts->d_red + it + 1, n, ts->s));
+      TS_CALL(wait_published(ts, pub));
       for (int j = 0; j <= it; j++) HH(j, it) = ts->h_red[j];
       HH(it + 1, it) = std::sqrt(ts->h_red[it + 1]);
-      if (HH(it + 1, it) != 0) {
-        LinComb none{};
-        TS_HIP(ts, launch_lincomb(v_new, 1.0 / HH(it + 1, it), none, n, ts->s));
-      }
       for (int j = 0; j < it; j++) {
         const double tt = HH(j, it);
         HH(j, it)       = cc[j] * tt + ss[j] * HH(j + 1, it);
@@ -326,15 +334,17 @@ int invert_diagonal(HoneeMiTS ts, double *d, bool block, const char *what) {
     TS_HIP(ts, launch_jacobi_invert(d, ts->n, ts->s));
     return 0;
   }
-  TS_HIP(ts, launch_pb_invert(d, ts->n / 5, 0, ts->partial, ts->d_red, ts->s));
-  TS_CALL(read_scalars(ts, 1));
+  const Publish pub = next_publish(ts, 1);
+  TS_HIP(ts, launch_pb_invert(d, ts->n / 5, 0, ts->partial, ts->d_red, ts->s, pub));
+  TS_CALL(wait_published(ts, pub));
   if (ts->h_red[0] != 0) return fail(ts, -6, "%s: %.0f singular point blocks", what, ts->h_red[0]);
   return 0;
 }
 
 int check_finite(HoneeMiTS ts, const double *x, const char *what) {
-  TS_HIP(ts, launch_count_nonfinite(x, ts->n, 0, ts->partial, ts->d_red, ts->s));
-  TS_CALL(read_scalars(ts, 1));
+  const Publish pub = next_publish(ts, 1);
+  TS_HIP(ts, launch_count_nonfinite(x, ts->n, 0, ts->partial, ts->d_red, ts->s, pub));
+  TS_CALL(wait_published(ts, pub));
   if (ts->h_red[0] != 0) return fail(ts, -3, "%s has %.0f non-finite entries at step %d (t = %g)", what, ts->h_red[0], ts->st.steps, ts->t);
   return 0;
 }
@@ -498,10 +508,11 @@ int newton_solve(HoneeMiTS ts, double tnew) {
     for (int bt = 0;; bt++) {
       TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, -lambda}, {ts->x, ts->dx}));
       TS_CALL(ifunction_at(ts, tnew, ts->xtry, ts->Ftry));
+      const Publish pub = next_publish(ts, bt == 0 ? 3 : 2);
       TS_HIP(ts, launch_dot(ts->Ftry, ts->Ftry, ts->n, 0, ts->partial, ts->d_red, ts->s));
-      TS_HIP(ts, launch_dot(ts->xtry, ts->xtry, ts->n, 1, ts->partial, ts->d_red, ts->s));
       if (bt == 0) TS_HIP(ts, launch_dot(ts->dx, ts->dx, ts->n, 2, ts->partial, ts->d_red, ts->s));
-      TS_CALL(read_scalars(ts, bt == 0 ? 3 : 2));
+      TS_HIP(ts, launch_dot(ts->xtry, ts->xtry, ts->n, 1, ts->partial, ts->d_red, ts->s, pub));
+      TS_CALL(wait_published(ts, pub));
       gnorm = std::sqrt(ts->h_red[0]);
       xnorm = std::sqrt(ts->h_red[1]);
       if (bt == 0) ynorm = std::sqrt(ts->h_red[2]);

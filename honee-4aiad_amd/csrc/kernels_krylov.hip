@@ -78,14 +78,32 @@ __global__ void __launch_bounds__(kThreads) k_project_out(double *__restrict__ w
   if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
 }
 
-// out[slot] = sum over blocks b (in order) of partial[b][slot], for slots [s0, s0 + gridDim.x)
-__global__ void __launch_bounds__(kThreads) k_reduce_partials(const double *__restrict__ partial, int nblocks, int s0, double *__restrict__ out) {
+// out[slot] = sum over blocks b (in order) of partial[b][slot], for slots [s0, s0 + gridDim.x).  A single-block launch
+// with pub.h set then stores out[0 .. pub.count) into mapped host memory and pub.seq into *pub.flag (Publish).
+__global__ void __launch_bounds__(kThreads) k_reduce_partials(const double *__restrict__ partial, int nblocks, int s0, double *__restrict__ out,
+                                                              Publish pub) {
   __shared__ double s[kThreads];
   const int         slot = s0 + blockIdx.x;
   double            acc  = 0;
   for (int b = threadIdx.x; b < nblocks; b += kThreads) acc += partial[(long long)b * kMaxDots + slot];
   const double r = block_sum(acc, s);
   if (threadIdx.x == 0) out[slot] = r;
+  if (pub.h) {
+    const int i = threadIdx.x;
+    if (i < pub.count) pub.h[i] = i == slot ? r : out[i];  // the other slots come from earlier launches
+    __threadfence_system();  // each thread's value reaches host memory before the sequence number
+    __syncthreads();
+    if (i == 0) __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// y = y / sqrt(*hsq) unless *hsq is zero (a Krylov vector normalized by its squared norm in device memory)
+__global__ void __launch_bounds__(kThreads) k_normalize(double *__restrict__ y, const double *__restrict__ hsq, long long n) {
+  const double h = *hsq;
+  if (h == 0) return;
+  const double    a      = 1.0 / sqrt(h);
+  const long long stride = (long long)gridDim.x * kThreads;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) y[i] = a * y[i];
 }
 
 // y[i] = a y[i] + sum_{j < m} c_j X_j[i]
@@ -200,16 +218,6 @@ __global__ void __launch_bounds__(kThreads) k_pb_invert(double *__restrict__ pb,
   if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
 }
 
-// h[i] = d[i] for i < count, in host memory that is mapped and coherent, then *seq_flag = seq: the host reads h once it
-// sees seq, without a copy or a stream synchronization
-__global__ void __launch_bounds__(kThreads) k_publish(const double *__restrict__ d, int count, double *h, unsigned long long *seq_flag,
-                                                      unsigned long long seq) {
-  if ((int)threadIdx.x < count) h[threadIdx.x] = d[threadIdx.x];
-  __threadfence_system();  // each thread's value reaches host memory before the flag
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(seq_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 // z_n = Binv_n (x_n - y_n) per node n (y may be null)
 __global__ void __launch_bounds__(kThreads) k_pb_apply(double *__restrict__ z, const double *__restrict__ pbinv, const double *__restrict__ x,
                                                        const double *__restrict__ y, long long nn) {
@@ -249,24 +257,29 @@ hipError_t launch_mdot(const double *w, const double *V, long long ldv, int k, l
     const int nj = k - j0 < 8 ? k - j0 : 8;
     hipLaunchKernelGGL(k_mdot8, dim3(nb), dim3(kThreads), 0, s, w, V, ldv, j0, nj, n, partial);
   }
-  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kThreads), 0, s, partial, nb, 0, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kThreads), 0, s, partial, nb, 0, out, Publish{});
   return hipGetLastError();
 }
 
-hipError_t launch_dot(const double *x, const double *y, long long n, int slot, double *partial, double *out, hipStream_t s) {
-  if (slot < 0 || slot >= kMaxDots) return hipErrorInvalidValue;
+hipError_t launch_dot(const double *x, const double *y, long long n, int slot, double *partial, double *out, hipStream_t s, const Publish &pub) {
+  if (slot < 0 || slot >= kMaxDots || pub.count > kThreads) return hipErrorInvalidValue;
   const int nb = krylov_reduction_blocks(n);
   hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kThreads), 0, s, x, y, n, slot, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out, pub);
   return hipGetLastError();
 }
 
 hipError_t launch_project_out(double *w, const double *V, long long ldv, int k, const double *h, long long n, int slot, double *partial, double *out,
-                              hipStream_t s) {
-  if (k < 0 || k > kMaxDots || slot < 0 || slot >= kMaxDots) return hipErrorInvalidValue;
+                              hipStream_t s, const Publish &pub) {
+  if (k < 0 || k > kMaxDots || slot < 0 || slot >= kMaxDots || pub.count > kThreads) return hipErrorInvalidValue;
   const int nb = krylov_reduction_blocks(n);
   hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(kThreads), 0, s, w, V, ldv, k, h, n, slot, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out, pub);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(double *y, const double *hsq, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize, dim3(elementwise_grid(n)), dim3(kThreads), 0, s, y, hsq, n);
   return hipGetLastError();
 }
 
@@ -297,25 +310,19 @@ hipError_t launch_set(double *x, double v, long long n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_count_nonfinite(const double *x, long long n, int slot, double *partial, double *out, hipStream_t s) {
-  if (slot < 0 || slot >= kMaxDots) return hipErrorInvalidValue;
+hipError_t launch_count_nonfinite(const double *x, long long n, int slot, double *partial, double *out, hipStream_t s, const Publish &pub) {
+  if (slot < 0 || slot >= kMaxDots || pub.count > kThreads) return hipErrorInvalidValue;
   const int nb = krylov_reduction_blocks(n);
   hipLaunchKernelGGL(k_count_nonfinite, dim3(nb), dim3(kThreads), 0, s, x, n, slot, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out, pub);
   return hipGetLastError();
 }
 
-hipError_t launch_pb_invert(double *pb, long long nnodes, int slot, double *partial, double *out, hipStream_t s) {
-  if (slot < 0 || slot >= kMaxDots) return hipErrorInvalidValue;
+hipError_t launch_pb_invert(double *pb, long long nnodes, int slot, double *partial, double *out, hipStream_t s, const Publish &pub) {
+  if (slot < 0 || slot >= kMaxDots || pub.count > kThreads) return hipErrorInvalidValue;
   const int nb = krylov_reduction_blocks(nnodes);
   hipLaunchKernelGGL(k_pb_invert, dim3(nb), dim3(kThreads), 0, s, pb, nnodes, slot, partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_publish(const double *d, int count, double *h, unsigned long long *seq_flag, unsigned long long seq, hipStream_t s) {
-  if (count < 0 || count > kThreads) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kThreads), 0, s, d, count, h, seq_flag, seq);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s, partial, nb, slot, out, pub);
   return hipGetLastError();
 }
 
