@@ -501,14 +501,26 @@ int newton_solve(HoneeMiTS ts, double tnew) {
     TS_CALL(gmres(ts, apply, Precond{ts, ts->dinv, block}, ts->F, ts->dx, o.ksp_rtol, o.ksp_atol, o.ksp_restart, o.ksp_max_it, &kits));
     ts->st.krylov_its += kits;
     ts->st.last_krylov_its += kits;
-    // backtracking on f = ||F||^2 / 2 along -dx with slope -||F||^2.  Each trial reads ||F(xtry)||, ||xtry|| (the
-    // step-length test's ||x|| if the trial is accepted) and, at the first trial, ||dx|| in one host read.
-    const double initslope = -fnorm * fnorm, alpha_ls = 1e-4;
-    double       lambda = 1.0, gnorm = 0, xnorm = 0, ynorm = 0;
+    // SNESLINESEARCHBT (cubic order, alpha 1e-4, minlambda 1e-12, 40 steps) along -dx, as SNESLineSearchApply_BT:
+    // initial slope F . (J dx) (made negative, -1 if zero); the full step is taken when ||G||^2/2 <= ||F||^2/2 +
+    // lambda alpha slope, else a quadratic fit and then cubic fits through the last two trials, each clamped to
+    // [0.1, 0.5] of the previous lambda and accepted under the strict inequality.  A trial with a non-finite residual
+    // cuts lambda by 10 without a fit.  Each trial reads ||F(xtry)|| and ||xtry|| (the step-length test's ||x|| if the
+    // trial is accepted), the first one also ||dx|| and F . (J dx), in one host read.
+    // SNESLINESEARCHBASIC takes the full step (one trial, accepted when its residual is finite); type 2 takes the slope
+    // as -||F||^2 and fits quadratics only
+    const bool basic = o.snes_linesearch_type == 1, norm_slope = o.snes_linesearch_type == 2;
+    if (!basic && !norm_slope) {
+      TS_CALL(apply(ts->dx, ts->w));  // J dx; w is free between Krylov solves
+      TS_HIP(ts, launch_dot(ts->F, ts->w, ts->n, 3, ts->partial, ts->d_red, ts->s));
+    }
+    const double alpha_ls = 1e-4;
+    double       initslope = 0, lambda = 1.0, gnorm = 0, xnorm = 0, ynorm = 0, lamprev = 0, gnormprev = 0;
+    int          nfit = 0;
     for (int bt = 0;; bt++) {
       TS_CALL(lincomb(ts, ts->xtry, 0.0, {1.0, -lambda}, {ts->x, ts->dx}));
       TS_CALL(ifunction_at(ts, tnew, ts->xtry, ts->Ftry));
-      const Publish pub = next_publish(ts, bt == 0 ? 3 : 2);
+      const Publish pub = next_publish(ts, bt == 0 ? (basic || norm_slope ? 3 : 4) : 2);
       TS_HIP(ts, launch_dot(ts->Ftry, ts->Ftry, ts->n, 0, ts->partial, ts->d_red, ts->s));
       if (bt == 0) TS_HIP(ts, launch_dot(ts->dx, ts->dx, ts->n, 2, ts->partial, ts->d_red, ts->s));
       TS_HIP(ts, launch_dot(ts->xtry, ts->xtry, ts->n, 1, ts->partial, ts->d_red, ts->s, pub));
@@ -516,11 +528,39 @@ int newton_solve(HoneeMiTS ts, double tnew) {
       gnorm = std::sqrt(ts->h_red[0]);
       xnorm = std::sqrt(ts->h_red[1]);
       if (bt == 0) ynorm = std::sqrt(ts->h_red[2]);
-      if (std::isfinite(gnorm) && 0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope) break;
-      if (bt >= 40) return fail(ts, -4, "line search failed at t = %g (||F|| = %g)", tnew, fnorm);
+      if (basic) {
+        if (!std::isfinite(gnorm)) return fail(ts, -4, "non-finite residual after a full Newton step at t = %g", tnew);
+        break;
+      }
+      if (bt == 0) {
+        initslope = norm_slope ? -fnorm * fnorm : ts->h_red[3];
+        if (initslope > 0.0) initslope = -initslope;
+        if (initslope == 0.0) initslope = -1.0;
+      }
+      const double lhs = 0.5 * gnorm * gnorm, rhs = 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope;
+      if (std::isfinite(gnorm) && (bt == 0 || norm_slope ? lhs <= rhs : lhs < rhs)) break;
+      if (bt >= 40 || (!norm_slope && lambda <= 1e-12)) return fail(ts, -4, "line search failed at t = %g (||F|| = %g)", tnew, fnorm);
       ts->st.line_search_backtracks++;
-      double lq = std::isfinite(gnorm) ? -initslope * lambda * lambda / (2.0 * (0.5 * gnorm * gnorm - 0.5 * fnorm * fnorm - lambda * initslope)) : 0;
-      lambda    = std::fmin(std::fmax(lq, 0.1 * lambda), 0.5 * lambda);
+      if (!std::isfinite(gnorm)) {
+        lambda *= 0.1;
+        continue;
+      }
+      double lt;
+      if (nfit == 0 || norm_slope) {  // quadratic fit through f(0), f'(0) and f(lambda)
+        lt = -initslope * lambda * lambda / (gnorm * gnorm - fnorm * fnorm - 2.0 * lambda * initslope);
+      } else {  // cubic fit through f(0), f'(0), f(lambda) and f(lamprev)
+        const double t1 = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lambda * initslope;
+        const double t2 = 0.5 * (gnormprev * gnormprev - fnorm * fnorm) - lamprev * initslope;
+        const double a  = (t1 / (lambda * lambda) - t2 / (lamprev * lamprev)) / (lambda - lamprev);
+        const double b  = (-lamprev * t1 / (lambda * lambda) + lambda * t2 / (lamprev * lamprev)) / (lambda - lamprev);
+        const double d  = std::fmax(b * b - 3.0 * a * initslope, 0.0);
+        lt              = a == 0.0 ? -initslope / (2.0 * b) : (-b + std::sqrt(d)) / (3.0 * a);
+      }
+      nfit++;
+      lamprev   = lambda;
+      gnormprev = gnorm;
+      if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+      lambda = lt <= 0.1 * lambda ? 0.1 * lambda : lt;
     }
     std::swap(ts->x, ts->xtry);
     std::swap(ts->F, ts->Ftry);
@@ -674,6 +714,7 @@ void HoneeMiTSOptionsDefault(HoneeMiTSOptions *o) {
   o->alpha_radius       = 1.0;
   o->snes_lag_jacobian  = 1;
   o->snes_lag_jacobian_persists = 0;
+  o->snes_linesearch_type       = 0;
 }
 
 int HoneeMiTSGetTableau(int rk_type, int ssp_stages, int max_stages, double *A, double *b, double *c, int *num_stages) {

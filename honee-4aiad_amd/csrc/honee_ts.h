@@ -59,6 +59,11 @@ typedef struct {
   // Jacobian data and the shift of the last evaluation (FormIJacobian_NS, src/setupts.c:129-157).  Default 1, 0.
   int    snes_lag_jacobian;
   int    snes_lag_jacobian_persists;
+  // Line search of the Newton solves: 0 PETSc's SNESLINESEARCHBT (cubic order; the default of SNESNEWTONLS and so of
+  // HONEE), 1 SNESLINESEARCHBASIC (the full step), 2 backtracking by quadratic fits with the slope -||F||^2 (the slope
+  // along an exact Newton direction; no Jacobian product, no minimum lambda, at most 40 cuts; not a PETSc type).
+  // Default 0.
+  int    snes_linesearch_type;
 } HoneeMiTSOptions;
 
 typedef struct {

@@ -206,18 +206,22 @@ def sample_offsets(stride: int, count: int, seed: int):
 
 
 def coarse_run(coarse: CoarseGrid, gas, Yc0, t_out, dt, model=None, snes_rtol=1e-3, ksp_rtol=0.1, pc_lag=50,
-               resource="/gpu/hip/mi355x:jacobian_precision=fp32", mf_operator=False, log=print):
+               resource="/gpu/hip/mi355x:jacobian_precision=fp32", mf_operator=False, linesearch="norm_slope", log=print):
     """BDF2 (Newton-GMRES(30) with point-block Jacobi and the FP32 Jacobian) on the coarse mesh from Yc0, with the
     closure in the IFunction when a model is given.  IJacobian_Newtonian leaves the closure out, so Newton converges
     only linearly with it.  The tolerances are LES-sized (relative to each step's initial residual) and the same for
-    both runs; pc_lag is PETSc's -snes_lag_preconditioner.  Returns per-output spectra, GPU seconds and solver
-    counts."""
+    both runs; pc_lag is PETSc's -snes_lag_preconditioner.  linesearch: "bt" (PETSc's default), "basic" or
+    "norm_slope" (quadratic backtracking with the slope -||F||^2, the default here).  With the closure left out of the
+    Jacobian, the Newton direction need not decrease ||F||: PETSc's bt, whose slope F . (J dx) sees only the modelled
+    part, fails its line search at t = 0.9 of the 24^3 study, and basic stalls on the 8^3 test case.  Returns per-output
+    spectra, GPU seconds and solver counts."""
     d, ctx = _context(gas, 1, 2, True, dt)
     prob = NewtonianProblem(coarse.mesh, coarse.tab, ctx, 1, resource)
     if model is not None:
         prob.setup_sgs_dd(model)
     ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_BDF, bdf_order=2, dt=dt, ksp_pc=TS.PC_PBJACOBI, snes_rtol=snes_rtol,
-                                                 ksp_rtol=ksp_rtol, pc_lag=pc_lag, snes_mf_operator=int(mf_operator)))
+                                                 ksp_rtol=ksp_rtol, pc_lag=pc_lag, snes_mf_operator=int(mf_operator),
+                                                 snes_linesearch_type={"bt": 0, "basic": 1, "norm_slope": 2}[linesearch]))
     ts.set_solution(np.ascontiguousarray(Yc0.ravel()), 0.0)
     spectra = [energy_spectrum(_as_grid(Yc0[:, 1:4], coarse.n, 3))]
     gpu_s, nsteps, t_prev = 0.0, 0, 0.0
@@ -245,7 +249,7 @@ def coarse_run(coarse: CoarseGrid, gas, Yc0, t_out, dt, model=None, snes_rtol=1e
 
 def study(n_fine=96, n_coarse=24, mu=5e-3, t_end=1.0, n_out=4, seed_eval=2024, seed_train=2025, train_offsets=8,
           coarse_dt=0.025, snes_rtol=1e-3, ksp_rtol=0.1, pc_lag=50, epochs=30, batch=4096, lr=3e-3, kmax_frac=0.5,
-          dims=(6, 20, 20, 6), closure_mf_operator=False, log=print) -> dict:
+          dims=(6, 20, 20, 6), closure_mf_operator=False, linesearch="norm_slope", log=print) -> dict:
     """Run the whole study; returns a JSON-ready summary.  dims: the closure network (6-20-20-6 by default; (6, 20, 6)
     is the reference model's shape)."""
     if n_fine % n_coarse:
@@ -301,14 +305,16 @@ def study(n_fine=96, n_coarse=24, mu=5e-3, t_end=1.0, n_out=4, seed_eval=2024, s
     fine_gpu_s, fine_steps = fine.run(_initial_prim(nf, PR.random_fourier_modes(seed_eval)), t_out, judge)
     fine.close()
 
-    runs = {name: coarse_run(coarse, gas, Yc0["Y"], t_out, coarse_dt, m, snes_rtol, ksp_rtol, pc_lag, mf_operator=mf, log=log)
+    runs = {name: coarse_run(coarse, gas, Yc0["Y"], t_out, coarse_dt, m, snes_rtol, ksp_rtol, pc_lag, mf_operator=mf, linesearch=linesearch,
+                             log=log)
             for name, m, mf in (("no_model", None, False), ("closure", model, closure_mf_operator))}
     coarse.close()
 
     kmax = max(1, int(kmax_frac * nc / 2))
     out = {"config": (f"decaying isotropic turbulence, E(k) ~ k^4 exp(-2 (k/4)^2), u_rms 1, Mach 0.1, mu {mu:g}; fine {n_fine}^3 p=2 "
                       f"conservative SSP-RK3 (CFL {fine.cfl}), filter width = coarse element; coarse {n_coarse}^3 p=2 primitive SUPG "
-                      f"BDF2 dt {coarse_dt:g} FP32 Jacobian, Newton rtol {snes_rtol:g}, GMRES rtol {ksp_rtol:g}, preconditioner lag {pc_lag}; "
+                      f"BDF2 dt {coarse_dt:g} FP32 Jacobian, Newton rtol {snes_rtol:g} ({linesearch} line search), GMRES rtol {ksp_rtol:g}, "
+                      f"preconditioner lag {pc_lag}; "
                       f"closure network {'-'.join(str(n) for n in dims)}; outputs at t = {', '.join(f'{t:g}' for t in t_out)}; spectra over "
                       f"1 <= k <= {kmax}"),
            "fine_gpu_s": fine_gpu_s, "fine_steps": fine_steps, "training_flow_gpu_s": train_gpu_s,

@@ -26,6 +26,7 @@ class TSOptions(ct.Structure):
         ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_restart", ct.c_int), ("ksp_max_it", ct.c_int),
         ("ksp_pc", ct.c_int), ("mass_pc", ct.c_int), ("pc_lag", ct.c_int), ("snes_mf_operator", ct.c_int),
         ("alpha_radius", ct.c_double), ("snes_lag_jacobian", ct.c_int), ("snes_lag_jacobian_persists", ct.c_int),
+        ("snes_linesearch_type", ct.c_int),
     ]
 
 

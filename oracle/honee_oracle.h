@@ -92,6 +92,7 @@ typedef struct {
   double ksp_rtol, ksp_atol;
   int    ksp_restart, ksp_max_it;
   int    pc_pbjacobi;  // 0 point-Jacobi, 1 point-block Jacobi
+  int    linesearch;   // 0 SNESLINESEARCHBT (cubic), 1 SNESLINESEARCHBASIC (full step), 2 quadratic fits with slope -||F||^2
 } OBDFOptions;
 
 typedef struct {
@@ -119,6 +120,9 @@ int oracle_rk_ex(const OMesh *m, ONewt *c, const double *qdata, double *q, doubl
                  const double *cs, const ORKMass *mo, const ORKExtra *ex, int record_every, double *ke_rows, int *nrows, long long *mass_its);
 // BDF(order) with Newton-GMRES and point-Jacobi; returns nonzero when a nonlinear solve fails
 int oracle_bdf(const OMesh *m, ONewt *c, const double *qdata, double *q, double t0, int nsteps, const OBDFOptions *o, OBDFStats *st);
+// The backtracking line search's next lambda after a rejected trial with a finite residual (SNESLineSearchApply_BT,
+// cubic order): quadratic fit when nfit == 0, else cubic fit through the last two trials, clamped to [0.1, 0.5] lambda
+double oracle_bt_next_lambda(double fnorm, double initslope, double lambda, double gnorm, int nfit, double lamprev, double gnormprev);
 
 // Explicit time stepping of the RHS with the SUPG-stabilized consistent mass (problems/newtonian.c:212-265) solved by
 // left-Jacobi-preconditioned GMRES; used to reproduce the reference's Taylor-Green kinetic-energy fixture.

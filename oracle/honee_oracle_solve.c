@@ -17,8 +17,12 @@
 //     (Lagrange through up to k + 1 points).  The "timestep size" context field stays the nominal h.
 //   * PCJACOBI / PCVPBJACOBI: the point diagonal (zero entries replaced by 1) or the 5x5 node blocks inverted by
 //     Gauss-Jordan elimination with partial pivoting (a singular block is an error).
-//   * SNESNEWTONLS with backtracking on f = ||F||^2/2 along the Newton direction (slope -||F||^2, sufficient-decrease
-//     constant 1e-4, step cut by the quadratic model clamped to [0.1, 0.5] of the previous step); the IJacobian
+//   * SNESNEWTONLS with SNESLINESEARCHBT (cubic order, alpha 1e-4, minlambda 1e-12, 40 steps), restated from
+//     SNESLineSearchApply_BT: initial slope F . (J dx) (made negative, -1 if zero); the full step is taken when
+//     ||G||^2/2 <= ||F||^2/2 + lambda alpha slope, else a quadratic fit and then cubic fits through the last two
+//     trials, each clamped to [0.1, 0.5] of the previous lambda and accepted under the strict inequality (a trial
+//     with a non-finite residual cuts lambda by 10 without a fit), SNESLINESEARCHBASIC (the full step), or quadratic
+//     fits only with the slope -||F||^2 of an exact Newton direction (not a PETSc type); the IJacobian
 //     (src/setupts.c:121-156) uses the Jacobian data of the last IFunction evaluation and the point-Jacobi diagonal is
 //     reassembled at every Newton iteration (MatGetDiagonal_Ceed, src/mat-ceed.c:1603-1630).  Convergence:
 //     ||F|| <= atol, ||F|| <= max(atol, rtol ||F_0||) or ||step|| <= stol ||x||.
@@ -359,6 +363,26 @@ static void residual(BDF *b, double tnew, double shift, const double *x, double 
   b->st->ifunction_evals++;
 }
 
+// SNESLineSearchApply_BT's step after a rejected trial with a finite residual: with f(l) = ||F(x - l dx)||^2 / 2,
+// the minimizer of the quadratic through f(0), f'(0) = initslope and f(lambda) when no fit has been made yet (nfit
+// == 0), else of the cubic through f(0), f'(0), f(lambda) and f(lamprev); clamped to [0.1, 0.5] lambda
+double oracle_bt_next_lambda(double fnorm, double initslope, double lambda, double gnorm, int nfit, double lamprev, double gnormprev) {
+  double lt;
+  if (nfit == 0) {
+    lt = -initslope * lambda * lambda / (gnorm * gnorm - fnorm * fnorm - 2.0 * lambda * initslope);
+  } else {
+    const double t1 = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lambda * initslope;
+    const double t2 = 0.5 * (gnormprev * gnormprev - fnorm * fnorm) - lamprev * initslope;
+    const double a  = (t1 / (lambda * lambda) - t2 / (lamprev * lamprev)) / (lambda - lamprev);
+    const double b  = (-lamprev * t1 / (lambda * lambda) + lambda * t2 / (lamprev * lamprev)) / (lambda - lamprev);
+    double       d  = b * b - 3.0 * a * initslope;
+    if (d < 0.0) d = 0.0;
+    lt = a == 0.0 ? -initslope / (2.0 * b) : (-b + sqrt(d)) / (3.0 * a);
+  }
+  if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+  return lt <= 0.1 * lambda ? 0.1 * lambda : lt;
+}
+
 static int bdf_solve(BDF *b, double tnew, int k) {
   const int n = b->n;
   double    T[4], alpha[4];
@@ -403,17 +427,38 @@ static int bdf_solve(BDF *b, double tnew, int k) {
     b->st->krylov_its += kits;
     b->st->last_krylov_its += kits;
     const double ynorm = sqrt(vdot(n, b->dx, b->dx));
-    const double initslope = -fnorm * fnorm, alpha_ls = 1e-4;
-    double       lambda = 1.0, gnorm = 0;
+    const int norm_slope = o->linesearch == 2;
+    double    initslope  = -fnorm * fnorm;
+    if (o->linesearch == 0) {
+      jac_apply(&jc, b->dx, b->Ftry);  // J dx (Ftry is free until the first trial)
+      initslope = vdot(n, b->F, b->Ftry);
+    }
+    if (initslope > 0.0) initslope = -initslope;
+    if (initslope == 0.0) initslope = -1.0;
+    const double alpha_ls = 1e-4;
+    double       lambda = 1.0, gnorm = 0, lamprev = 0, gnormprev = 0;
+    int          nfit = 0;
     for (int bt = 0;; bt++) {
       for (int l = 0; l < n; l++) b->xtry[l] = b->x[l] - lambda * b->dx[l];
       residual(b, tnew, shift, b->xtry, b->Ftry, b->jac_try);
       gnorm = sqrt(vdot(n, b->Ftry, b->Ftry));
-      if (isfinite(gnorm) && 0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope) break;
-      if (bt >= 40) return -4;
+      if (o->linesearch == 1) {  // SNESLINESEARCHBASIC: the full step, accepted when its residual is finite
+        if (!isfinite(gnorm)) return -4;
+        break;
+      }
+      const double lhs = 0.5 * gnorm * gnorm, rhs = 0.5 * fnorm * fnorm + lambda * alpha_ls * initslope;
+      if (isfinite(gnorm) && (bt == 0 || norm_slope ? lhs <= rhs : lhs < rhs)) break;
+      if (bt >= 40 || (!norm_slope && lambda <= 1e-12)) return -4;
       b->st->line_search_backtracks++;
-      double lq = isfinite(gnorm) ? -initslope * lambda * lambda / (2.0 * (0.5 * gnorm * gnorm - 0.5 * fnorm * fnorm - lambda * initslope)) : 0;
-      lambda    = fmin(fmax(lq, 0.1 * lambda), 0.5 * lambda);
+      if (!isfinite(gnorm)) {
+        lambda *= 0.1;
+        continue;
+      }
+      const double next = oracle_bt_next_lambda(fnorm, initslope, lambda, gnorm, norm_slope ? 0 : nfit, lamprev, gnormprev);
+      nfit++;
+      lamprev   = lambda;
+      gnormprev = gnorm;
+      lambda    = next;
     }
     double *sw;
     sw = b->x, b->x = b->xtry, b->xtry = sw;

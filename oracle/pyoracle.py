@@ -462,7 +462,7 @@ class ORKMass(ct.Structure):
 class OBDFOptions(ct.Structure):
     _fields_ = [("order", ct.c_int), ("dt", ct.c_double), ("snes_rtol", ct.c_double), ("snes_atol", ct.c_double), ("snes_stol", ct.c_double),
                 ("snes_max_it", ct.c_int), ("ksp_rtol", ct.c_double), ("ksp_atol", ct.c_double), ("ksp_restart", ct.c_int),
-                ("ksp_max_it", ct.c_int), ("pc_pbjacobi", ct.c_int)]
+                ("ksp_max_it", ct.c_int), ("pc_pbjacobi", ct.c_int), ("linesearch", ct.c_int)]
 
 
 class OBDFStats(ct.Structure):
@@ -553,14 +553,23 @@ def rk(m: Mesh, c: ONewt, qdata, q, dt, nsteps, A, b, cc, t0=0.0, gmres=False, r
 
 
 def bdf(m: Mesh, c: ONewt, qdata, q, dt, nsteps, order=2, t0=0.0, snes_rtol=1e-8, snes_atol=1e-50, snes_stol=1e-8, snes_max_it=50,
-        ksp_rtol=1e-5, ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, pbjacobi=False):
+        ksp_rtol=1e-5, ksp_atol=1e-50, ksp_restart=30, ksp_max_it=10000, pbjacobi=False, linesearch="bt"):
     q = np.array(q, dtype=np.float64)
-    o = OBDFOptions(order, dt, snes_rtol, snes_atol, snes_stol, snes_max_it, ksp_rtol, ksp_atol, ksp_restart, ksp_max_it, int(pbjacobi))
+    o = OBDFOptions(order, dt, snes_rtol, snes_atol, snes_stol, snes_max_it, ksp_rtol, ksp_atol, ksp_restart, ksp_max_it, int(pbjacobi),
+                    {"bt": 0, "basic": 1, "norm_slope": 2}[linesearch])
     st = OBDFStats()
     ierr = lib().oracle_bdf(ct.byref(m.s), ct.byref(c), _d(qdata), _d(q), ct.c_double(t0), nsteps, ct.byref(o), ct.byref(st))
     if ierr:
         raise RuntimeError(f"oracle BDF failed ({ierr})")
     return q, {k: getattr(st, k) for k, _ in OBDFStats._fields_}
+
+
+def bt_next_lambda(fnorm, initslope, lam, gnorm, nfit, lamprev=0.0, gnormprev=0.0):
+    """Next lambda of the backtracking line search after a rejected trial (oracle_bt_next_lambda)."""
+    f = lib().oracle_bt_next_lambda
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_int, ct.c_double, ct.c_double]
+    return f(fnorm, initslope, lam, gnorm, nfit, lamprev, gnormprev)
 
 
 # ---- data-driven SGS closure (honee_oracle.c, sgs_pointwise.h) ----

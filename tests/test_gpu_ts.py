@@ -113,6 +113,39 @@ def test_explicit_schemes_match_oracle(rk_type, stages, gmres, rowsum, stab):
     prob.close()
 
 
+@pytest.mark.parametrize("linesearch", ["bt", "basic", "norm_slope"])
+def test_backtracking_line_search_matches_oracle(linesearch):
+    """Inexact Newton (GMRES capped at 5 iterations, rtol 0.7) on a TGV case with a noisy velocity, where the full
+    Newton step often fails the sufficient-decrease test: the GPU integrator's line search (cubic backtracking with
+    the slope F . (J dx), as PETSc's SNESLINESEARCHBT) backtracks like the oracle's and ends at its state.  With
+    SNESLINESEARCHBASIC both take full steps; with quadratic fits and the slope -||F||^2 both backtrack alike."""
+    _say(f"ts bdf2 inexact Newton, {linesearch} line search")
+    basic = linesearch == "basic"
+    ls_type = {"bt": 0, "basic": 1, "norm_slope": 2}[linesearch]
+    dt = 0.2
+    mesh, tab, d, q0, om, oc, qd = _tgv_case((2, 2, 2), 2, 1, 2, True, dt, perturb=0.08, noise=0.3)
+    prob = NewtonianProblem(mesh, tab, context_from_dict(d), 1)
+    kw = dict(snes_rtol=1e-10, ksp_rtol=0.7, ksp_restart=5, ksp_max_it=5)
+    ts = TS.TimeStepper(prob, TS.default_options(type=TS.TS_BDF, bdf_order=2, dt=dt, ksp_pc=TS.PC_PBJACOBI, snes_linesearch_type=ls_type,
+                                                  **kw))
+    ts.set_solution(q0, 0.0)
+    ts.step(3)
+    q_ref, st_ref = O.bdf(om, oc, qd, q0, dt, 3, order=2, pbjacobi=True, linesearch=linesearch, **kw)
+    st = ts.stats()
+    e = _rel(ts.solution(), q_ref)
+    print(f"{linesearch} bdf2: rel err {e:.3e}; newton gpu {st['newton_its']} oracle {st_ref['newton_its']}; backtracks gpu "
+          f"{st['line_search_backtracks']} oracle {st_ref['line_search_backtracks']}", flush=True)
+    if basic:
+        assert st["line_search_backtracks"] == 0 and st_ref["line_search_backtracks"] == 0
+    else:
+        assert st_ref["line_search_backtracks"] > 0 and st["line_search_backtracks"] > 0
+        assert abs(st["line_search_backtracks"] - st_ref["line_search_backtracks"]) <= 2
+    assert abs(st["newton_its"] - st_ref["newton_its"]) <= 3
+    assert e < 1e-6  # both end within the step-length test's 1e-8 ||x|| of the same solution
+    ts.close()
+    prob.close()
+
+
 @pytest.mark.parametrize("order,state_var,degree,pc", [(2, 1, 2, TS.PC_PBJACOBI), (1, 0, 2, TS.PC_JACOBI), (2, 2, 1, TS.PC_PBJACOBI),
                                                        (2, 0, 3, TS.PC_JACOBI)])
 def test_bdf_newton_gmres_matches_oracle(order, state_var, degree, pc):
