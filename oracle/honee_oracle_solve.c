@@ -28,7 +28,8 @@
 //     ||F|| <= atol, ||F|| <= max(atol, rtol ||F_0||) or ||step|| <= stol ||x||.
 // Parity unpinned against PETSc itself (no PETSc here): these restatements are pinned by the reference's Taylor-Green
 // kinetic-energy fixture (RK4 + GMRES mass), by the order conditions and convergence rates of the schemes, and by
-// closed-form linear test problems in the tests.
+// closed-form linear test problems in the tests; the line search's step (oracle_bt_next_lambda) by the exact minimizers
+// of quadratic and cubic merit functions (tests/test_oracle_linesearch.py).
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
