@@ -1,8 +1,9 @@
 // Vector kernels of the time integrator and its Krylov solvers (GMRES with point-Jacobi, Newton, explicit RK).
 //
 // Reductions are deterministic: a grid of fixed size (a function of the length only) reduces contiguous grid-stride
-// slices per block in a fixed tree order into partials[block][slot], and a second kernel sums each slot over the
-// blocks in a fixed order.  The same inputs give bitwise identical results from run to run.
+// slices per block in a fixed tree order into partials[slot][block] (slot-major, so that the second pass reads each
+// slot's partials contiguously), and a second kernel sums each slot over the blocks in a fixed order.  The same inputs
+// give bitwise identical results from run to run.
 //
 // Multi-dot: the dots of w with up to kMaxDots basis vectors are taken in groups of 8 (w is read once per group).
 // Projection update: w -= sum_j h_j V_j with the h_j read from device memory (the reduced dots), fused with the
@@ -28,7 +29,7 @@ __device__ __forceinline__ double block_sum(double v, double *s) {
   return r;
 }
 
-// partial[block][j0 + j] = sum over this block's slice of w[i] V[(j0 + j) ldv + i], j < nj <= 8
+// partial[j0 + j][block] = sum over this block's slice of w[i] V[(j0 + j) ldv + i], j < nj <= 8
 __global__ void __launch_bounds__(kThreads) k_mdot8(const double *__restrict__ w, const double *__restrict__ V, long long ldv, int j0, int nj,
                                                     long long n, double *__restrict__ partial) {
   __shared__ double s[kThreads];
@@ -44,11 +45,11 @@ __global__ void __launch_bounds__(kThreads) k_mdot8(const double *__restrict__ w
   for (int j = 0; j < 8; j++) {
     if (j >= nj) break;
     const double r = block_sum(acc[j], s);
-    if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + j0 + j] = r;
+    if (threadIdx.x == 0) partial[(long long)(j0 + j) * kMaxReductionBlocks + blockIdx.x] = r;
   }
 }
 
-// partial[block][slot] = sum over this block's slice of x[i] y[i]
+// partial[slot][block] = sum over this block's slice of x[i] y[i]
 __global__ void __launch_bounds__(kThreads) k_dot(const double *__restrict__ x, const double *__restrict__ y, long long n, int slot,
                                                   double *__restrict__ partial) {
   __shared__ double s[kThreads];
@@ -56,10 +57,10 @@ __global__ void __launch_bounds__(kThreads) k_dot(const double *__restrict__ x, 
   const long long   stride = (long long)gridDim.x * kThreads;
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) acc += x[i] * y[i];
   const double r = block_sum(acc, s);
-  if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
+  if (threadIdx.x == 0) partial[(long long)slot * kMaxReductionBlocks + blockIdx.x] = r;
 }
 
-// w[i] -= sum_{j < k} h[j] V[j ldv + i] (in order j = 0, 1, ...); partial[block][slot] = sum of the new w[i]^2
+// w[i] -= sum_{j < k} h[j] V[j ldv + i] (in order j = 0, 1, ...); partial[slot][block] = sum of the new w[i]^2
 __global__ void __launch_bounds__(kThreads) k_project_out(double *__restrict__ w, const double *__restrict__ V, long long ldv, int k,
                                                           const double *__restrict__ h, long long n, int slot, double *__restrict__ partial) {
   __shared__ double s[kThreads];
@@ -75,17 +76,17 @@ __global__ void __launch_bounds__(kThreads) k_project_out(double *__restrict__ w
     ss += acc * acc;
   }
   const double r = block_sum(ss, s);
-  if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
+  if (threadIdx.x == 0) partial[(long long)slot * kMaxReductionBlocks + blockIdx.x] = r;
 }
 
-// out[slot] = sum over blocks b (in order) of partial[b][slot], for slots [s0, s0 + gridDim.x).  A single-block launch
+// out[slot] = sum over blocks b (in order) of partial[slot][b], for slots [s0, s0 + gridDim.x).  A single-block launch
 // with pub.h set then stores out[0 .. pub.count) into mapped host memory and pub.seq into *pub.flag (Publish).
 __global__ void __launch_bounds__(kThreads) k_reduce_partials(const double *__restrict__ partial, int nblocks, int s0, double *__restrict__ out,
                                                               Publish pub) {
   __shared__ double s[kThreads];
   const int         slot = s0 + blockIdx.x;
   double            acc  = 0;
-  for (int b = threadIdx.x; b < nblocks; b += kThreads) acc += partial[(long long)b * kMaxDots + slot];
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) acc += partial[(long long)slot * kMaxReductionBlocks + b];
   const double r = block_sum(acc, s);
   if (threadIdx.x == 0) out[slot] = r;
   if (pub.h) {
@@ -144,19 +145,19 @@ __global__ void __launch_bounds__(kThreads) k_set(double *__restrict__ x, double
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
-// partial[block][slot] = number of non-finite entries in the block's slice (as a double)
+// partial[slot][block] = number of non-finite entries in the block's slice (as a double)
 __global__ void __launch_bounds__(kThreads) k_count_nonfinite(const double *__restrict__ x, long long n, int slot, double *__restrict__ partial) {
   __shared__ double s[kThreads];
   double            cnt    = 0;
   const long long   stride = (long long)gridDim.x * kThreads;
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) cnt += isfinite(x[i]) ? 0.0 : 1.0;
   const double r = block_sum(cnt, s);
-  if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
+  if (threadIdx.x == 0) partial[(long long)slot * kMaxReductionBlocks + blockIdx.x] = r;
 }
 
 // In-place inverse of each node's 5x5 block (row-major [c_out][c_in]) by Gauss-Jordan elimination with partial
 // pivoting on [A | I]; every index is a compile-time constant so the blocks stay in registers.
-// partial[block][slot] counts the singular blocks of the block's slice.
+// partial[slot][block] counts the singular blocks of the block's slice.
 __global__ void __launch_bounds__(kThreads) k_pb_invert(double *__restrict__ pb, long long nn, int slot, double *__restrict__ partial) {
   __shared__ double s[kThreads];
   double            bad    = 0;
@@ -215,7 +216,7 @@ __global__ void __launch_bounds__(kThreads) k_pb_invert(double *__restrict__ pb,
       for (int j = 0; j < 5; j++) pb[node * 25 + i * 5 + j] = X[i][j];
   }
   const double r = block_sum(bad, s);
-  if (threadIdx.x == 0) partial[(long long)blockIdx.x * kMaxDots + slot] = r;
+  if (threadIdx.x == 0) partial[(long long)slot * kMaxReductionBlocks + blockIdx.x] = r;
 }
 
 // z_n = Binv_n (x_n - y_n) per node n (y may be null)

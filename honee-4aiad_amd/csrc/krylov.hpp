@@ -5,7 +5,7 @@
 #include <hip/hip_runtime.h>
 
 constexpr int kMaxDots            = 32;    // GMRES restart + 1 must not exceed this
-constexpr int kMaxReductionBlocks = 1024;  // partial buffers hold kMaxReductionBlocks x kMaxDots doubles
+constexpr int kMaxReductionBlocks = 1024;  // partial buffers hold kMaxDots x kMaxReductionBlocks doubles, slot-major
 constexpr int kMaxLinComb         = 8;
 
 struct LinComb {  // sum_j c[j] x[j], passed by value
