@@ -141,3 +141,15 @@ def test_oracle_gmres_mass_solve_is_accurate():
     q1_exact = q0 + dt * np.linalg.solve(Mdense, r)
     assert its > 0
     assert np.linalg.norm(q1 - q1_exact) < 1e-10 * np.linalg.norm(q1_exact - q0)
+
+
+def test_options_layout_and_defaults():
+    """The ctypes mirror of HoneeMiTSOptions agrees with the C struct: HoneeMiTSOptionsDefault's values arrive in the
+    right fields, including the last ones (a missing or misplaced field would shift them)."""
+    o = TS.default_options()
+    assert o.type == TS.TS_BDF and o.bdf_order == 2 and o.dt == 1e-2
+    assert o.snes_rtol == 1e-8 and o.snes_stol == 1e-8 and o.snes_max_it == 50
+    assert o.ksp_rtol == 1e-5 and o.ksp_restart == 30 and o.ksp_max_it == 10000
+    assert o.pc_lag == 1 and o.snes_mf_operator == 0
+    assert o.alpha_radius == 1.0 and o.snes_lag_jacobian == 1 and o.snes_lag_jacobian_persists == 0
+    assert o.snes_linesearch_type == 0  # PETSc's SNESLINESEARCHBT, HONEE's default
