@@ -7,7 +7,8 @@
 //
 // Multi-dot: the dots of w with up to kMaxDots basis vectors are taken in groups of 8 (w is read once per group).
 // Projection update: w -= sum_j h_j V_j with the h_j read from device memory (the reduced dots), fused with the
-// partial sums of ||w||^2, so that one Arnoldi step needs a single device-to-host copy of (h_0..h_k, ||w||^2).
+// partial sums of ||w||^2, so that one Arnoldi step needs a single host read of (h_0..h_k, ||w||^2), which the final
+// reduction kernel stores into mapped host memory (Publish).
 #include <hip/hip_runtime.h>
 
 #include "krylov.hpp"
