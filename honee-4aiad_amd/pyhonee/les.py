@@ -14,7 +14,7 @@ unit RMS per component) at the reference pressure and temperature (Mach 0.1 at U
   offsets), with the velocity gradient and the grid anisotropy of the coarse mesh, as the closure sees them in the
   coarse run; targets are the filtered SGS stress tau_ij = <u_i u_j> - <u_i><u_j> (qfunctions/sgs_dd_training.h:23-60).
   The FP32 network is trained on the GPU.
-* Judgement: shell-summed kinetic-energy spectra of the coarse runs against the filtered fine run sampled on the same
+* Assessment: shell-summed kinetic-energy spectra of the coarse runs against the filtered fine run sampled on the same
   coarse grid, at every output time.
 
 Host-side orchestration and measurement only: every operator, filter, projection, training step and time step runs
@@ -294,7 +294,7 @@ def study(n_fine=96, n_coarse=24, mu=5e-3, t_end=1.0, n_out=4, seed_eval=2024, s
     # --- evaluation flow: the truth (filtered fine spectra on the coarse grid) and the coarse initial state
     truth, Yc0 = [], {}
 
-    def judge(i, t, Y):
+    def assess(i, t, Y):
         _, f, its = fine.filtered(Y)
         fc = _sample(_as_grid(f, nf, 11), stride, (0, 0, 0))
         if i == 0:
@@ -302,7 +302,7 @@ def study(n_fine=96, n_coarse=24, mu=5e-3, t_end=1.0, n_out=4, seed_eval=2024, s
         truth.append(energy_spectrum(_as_grid(fc[:, 1:4], nc, 3)))
 
     log(f"# evaluation flow (seed {seed_eval})")
-    fine_gpu_s, fine_steps = fine.run(_initial_prim(nf, PR.random_fourier_modes(seed_eval)), t_out, judge)
+    fine_gpu_s, fine_steps = fine.run(_initial_prim(nf, PR.random_fourier_modes(seed_eval)), t_out, assess)
     fine.close()
 
     runs = {name: coarse_run(coarse, gas, Yc0["Y"], t_out, coarse_dt, m, snes_rtol, ksp_rtol, pc_lag, mf_operator=mf, linesearch=linesearch,
