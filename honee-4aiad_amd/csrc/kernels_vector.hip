@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_node_gather(int nrows, int ncomp, int c
 // L2 instead of into several.
 // E is the E-vector's type: double, or float for the FP32 Jacobian's contributions (summed in FP64 here).
 template <int NC, typename E>
-__global__ void __launch_bounds__(256) k_node_gather_rows(int nrows, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
+__global__ void __launch_bounds__(512) k_node_gather_rows(int nrows, int cs, const int *__restrict__ rowptr, const int *__restrict__ index,
                                                           const int *__restrict__ toff, const E *__restrict__ ev, double *__restrict__ v,
                                                           bool overwrite) {
   const long long per = ((long long)nrows + 7) / 8;
@@ -222,8 +222,10 @@ hipError_t launch_node_gather(int nrows, int ncomp, int comp_stride, const int *
   const long long n   = (long long)nrows * ncomp;
   if (n <= 0) return hipSuccess;
   if (t_index && ncomp == 5) {  // element-ordered E-vector: one thread per node, 8 row ranges (one per XCD)
-    const int grid = (grid_for(nrows) + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_node_gather_rows<5, double>), dim3(grid), dim3(kBlock), 0, s, nrows, comp_stride, t_rowptr, t_index, t_offset, ev, v,
+    // 512-thread blocks: 49.4 us against 50.6 us (256) and 50.3 us (128) at 884,736 rows, two A/B pairs
+    constexpr int kRowsBlock = 512;
+    const int     grid       = (int)(((nrows + kRowsBlock - 1) / kRowsBlock + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_node_gather_rows<5, double>), dim3(grid), dim3(kRowsBlock), 0, s, nrows, comp_stride, t_rowptr, t_index, t_offset, ev, v,
                        overwrite);
     return hipGetLastError();
   }
